@@ -1,0 +1,193 @@
+"""`FLIP.apic` -- the reference's public simulation API (src/apic.rs) over libapic_hip.
+
+Names, argument meaning and defaults follow the Rust module:
+  Apic::new(width, height, cell_size, world_properties)      src/apic.rs:779
+  Apic::update(&DebugOptions, frame_deltatime)                src/apic.rs:2429
+  Apic::add_radial_force(x, y, radius, strength)              src/apic.rs:2352
+  Apic::get_max_particle_velocity()                           src/apic.rs:2412
+  Apic::is_fluid(i, j)                                        src/apic.rs:1015
+  the pub stage methods                                       src/apic.rs:1043-2266
+  WorldProperties {gravity, border_damping, cfl, solve_error} src/apic.rs:707-713
+  DebugOptions {disable_particle_sort}                        src/apic.rs:397-400
+The reference's pub fields are device-resident here; `read`/`write` copy a
+field between HBM and a numpy array, and the particle views keep the
+reference's flat chunk*8+lane order.
+"""
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+@dataclass
+class WorldProperties:
+    gravity: float = 0.0
+    border_damping: float = 0.0
+    cfl: float = 0.0
+    solve_error: float = 0.0
+
+    def _c(self):
+        return _lib.WorldPropertiesC(self.gravity, self.border_damping, self.cfl, self.solve_error)
+
+
+@dataclass
+class DebugOptions:
+    disable_particle_sort: bool = False
+
+
+class Apic:
+    """One simulation resident on the GPU."""
+
+    def __init__(self, handle):
+        self._h = handle
+
+    # ---- construction ------------------------------------------------------
+    @classmethod
+    def new(cls, width, height, cell_size, world_properties):
+        h = ctypes.c_void_p()
+        check(lib().apic_create(width, height, cell_size, ctypes.byref(world_properties._c()), ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def with_particles(cls, width, height, cell_size, world_properties, xy):
+        """Apic::new with the particle set replaced by explicit positions (extension)."""
+        xy = np.ascontiguousarray(np.asarray(xy, dtype=np.float32).reshape(-1))
+        h = ctypes.c_void_p()
+        ptr = xy.ctypes.data if xy.size else None
+        check(lib().apic_create_with_particles(width, height, cell_size, ctypes.byref(world_properties._c()),
+                                               ptr, xy.size // 2, ctypes.byref(h)))
+        return cls(h)
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().apic_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- the step ----------------------------------------------------------
+    def update(self, debug_options, frame_deltatime):
+        opts = _lib.DebugOptionsC(1 if debug_options.disable_particle_sort else 0)
+        check(lib().apic_update(self._h, ctypes.byref(opts), frame_deltatime))
+
+    def add_radial_force(self, world_x, world_y, radius, strength):
+        check(lib().apic_add_radial_force(self._h, world_x, world_y, radius, strength))
+
+    def get_max_particle_velocity(self):
+        v = ctypes.c_float()
+        check(lib().apic_get_max_particle_velocity(self._h, ctypes.byref(v)))
+        return v.value
+
+    def is_fluid(self, i, j):
+        v = ctypes.c_uint64()
+        check(lib().apic_is_fluid(self._h, i, j, ctypes.byref(v)))
+        return v.value
+
+    def set_solver_config(self, mode=0, rel_tol=1e-6, max_iterations=500, fp64_reductions=True,
+                          recompute_every=25):
+        cfg = _lib.SolverConfigC(mode, rel_tol, max_iterations, int(fp64_reductions), recompute_every)
+        check(lib().apic_set_solver_config(self._h, ctypes.byref(cfg)))
+
+    def stats(self):
+        s = _lib.StepStatsC()
+        check(lib().apic_get_stats(self._h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in _lib.StepStatsC._fields_}
+
+    # ---- pub fields --------------------------------------------------------
+    def dims(self):
+        v = [ctypes.c_uint32() for _ in range(5)]
+        check(lib().apic_get_dims(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("width", "height", "num_particles", "num_chunks", "num_levels"), [x.value for x in v]))
+
+    @property
+    def width(self):
+        return self.dims()["width"]
+
+    @property
+    def height(self):
+        return self.dims()["height"]
+
+    @property
+    def num_particles(self):
+        return self.dims()["num_particles"]
+
+    @property
+    def num_chunks(self):
+        return self.dims()["num_chunks"]
+
+    @property
+    def timestamp(self):
+        ts = ctypes.c_uint32()
+        pd = ctypes.c_float()
+        check(lib().apic_get_timestamp(self._h, ctypes.byref(ts), ctypes.byref(pd)))
+        return ts.value
+
+    @property
+    def prev_deltatime(self):
+        ts = ctypes.c_uint32()
+        pd = ctypes.c_float()
+        check(lib().apic_get_timestamp(self._h, ctypes.byref(ts), ctypes.byref(pd)))
+        return pd.value
+
+    def set_timestamp(self, timestamp, prev_deltatime):
+        check(lib().apic_set_timestamp(self._h, timestamp, prev_deltatime))
+
+    def field_length(self, name, level=0):
+        n = ctypes.c_size_t()
+        check(lib().apic_field_length(self._h, _lib.FIELD_ID[name], level, ctypes.byref(n)))
+        return n.value
+
+    @staticmethod
+    def _dtype(name):
+        return np.uint32 if name in _lib.U32_FIELDS else (np.uint8 if name in _lib.U8_FIELDS else np.float32)
+
+    def read(self, name, level=0):
+        n = self.field_length(name, level)
+        out = np.zeros(n, dtype=self._dtype(name))
+        check(lib().apic_read_field(self._h, _lib.FIELD_ID[name], level, out.ctypes.data if n else None, n))
+        return out
+
+    def write(self, name, values, level=0):
+        n = self.field_length(name, level)
+        arr = np.ascontiguousarray(np.asarray(values).astype(self._dtype(name)).reshape(-1))
+        if arr.size != n:
+            raise ValueError(f"{name}: expected {n} elements, got {arr.size}")
+        check(lib().apic_write_field(self._h, _lib.FIELD_ID[name], level, arr.ctypes.data if n else None, n))
+
+    # ---- pub stage methods ---------------------------------------------------
+    def update_spatial_lookup(self):
+        check(lib().apic_update_spatial_lookup(self._h))
+
+    def transfer_particles_to_grid(self, deltatime=0.0, timestamp=0):
+        check(lib().apic_transfer_particles_to_grid(self._h, deltatime, timestamp))
+
+    def apply_external_forces(self, deltatime):
+        check(lib().apic_apply_external_forces(self._h, deltatime))
+
+    def build_pressure_system(self):
+        check(lib().apic_build_pressure_system(self._h))
+
+    def update_multilevel_grid(self):
+        check(lib().apic_update_multilevel_grid(self._h))
+
+    def apply_preconditioner(self):
+        check(lib().apic_apply_preconditioner(self._h))
+
+    def solve_pcg(self):
+        check(lib().apic_solve_pcg(self._h))
+
+    def apply_pressure_gradient(self):
+        check(lib().apic_apply_pressure_gradient(self._h))
+
+    def extrapolate_velocity(self):
+        check(lib().apic_extrapolate_velocity(self._h))
+
+    def transfer_grid_to_particles_and_advect(self, deltatime):
+        check(lib().apic_transfer_grid_to_particles_and_advect(self._h, deltatime))

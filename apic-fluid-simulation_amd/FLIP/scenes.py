@@ -1,0 +1,79 @@
+"""Seeded scene construction for the dam-break workloads (host-side input data).
+
+Particles of a fluid cell are placed at (cell + U[0,1)^2) * dx with U drawn
+from SplitMix64 (Steele, Lea & Flood, 2014) as a 24-bit mantissa,
+(r >> 40) * 2^-24; cells are enumerated y-outer, x-inner as the reference's
+constructor does (src/apic.rs:832-845).  A fluid cell is an interior cell
+(not a wall) whose centre lies in the block.
+"""
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed, n, offset=0):
+    """The first n outputs of SplitMix64 seeded with `seed` (after `offset` draws)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform24(seed, n):
+    r = splitmix64(seed, n)
+    return ((r >> np.uint64(40)).astype(np.float64) * (1.0 / (1 << 24))).astype(np.float32)
+
+
+def block_cells(width, height, x0, x1, y0, y1):
+    """Interior cells (x, y) whose centres (in cell units) lie in [x0,x1) x [y0,y1), y-outer order."""
+    ys, xs = np.meshgrid(np.arange(1, height - 1), np.arange(1, width - 1), indexing="ij")
+    cx, cy = xs + 0.5, ys + 0.5
+    m = (cx >= x0) & (cx < x1) & (cy >= y0) & (cy < y1)
+    return xs[m].astype(np.int64), ys[m].astype(np.int64)
+
+
+def seed_cells(xs, ys, ppc, dx, seed):
+    """ppc jittered particles per listed cell; returns float32 (N, 2) positions."""
+    n = xs.size * ppc
+    u = uniform24(seed, 2 * n).reshape(n, 2)
+    cx = np.repeat(xs, ppc).astype(np.float32)
+    cy = np.repeat(ys, ppc).astype(np.float32)
+    dx = np.float32(dx)
+    px = (cx + u[:, 0]) * dx
+    py = (cy + u[:, 1]) * dx
+    return np.stack([px, py], axis=1).astype(np.float32)
+
+
+def dam_break(width, height, dx, fill_x=0.4, fill_y=0.6, ppc=4, seed=1):
+    """Dam break: the block fills the left fill_x of the width and bottom fill_y of the height."""
+    xs, ys = block_cells(width, height, 0.0, fill_x * width, 0.0, fill_y * height)
+    return seed_cells(xs, ys, ppc, dx, seed)
+
+
+def falling_block(width, height, dx, ppc=8, seed=3, pool=0.5, block=512):
+    """Mixed-precision large case: fluid in the bottom `pool` of the height plus a
+    block x block square seeded near the top centre."""
+    xs0, ys0 = block_cells(width, height, 0.0, width, 0.0, pool * height)
+    bx0 = (width - block) / 2.0
+    by1 = height - 1 - 16
+    xs1, ys1 = block_cells(width, height, bx0, bx0 + block, by1 - block, by1)
+    xs = np.concatenate([xs0, xs1])
+    ys = np.concatenate([ys0, ys1])
+    order = np.lexsort((xs, ys))
+    return seed_cells(xs[order], ys[order], ppc, dx, seed)
+
+
+CONFIGS = {
+    # (width, height, dx, dt, steps, ppc, seed, fill_x, fill_y)
+    "cfg1_dam_break_128": dict(width=128, height=128, dx=1.0 / 128, dt=0.002, steps=100, ppc=4, seed=1),
+    "cfg2_dam_break_1024": dict(width=1024, height=1024, dx=1.0 / 1024, dt=2.5e-4, steps=200, ppc=8, seed=2),
+}
+
+
+def config_scene(name):
+    c = CONFIGS[name]
+    return dam_break(c["width"], c["height"], c["dx"], 0.4, 0.6, c["ppc"], c["seed"])

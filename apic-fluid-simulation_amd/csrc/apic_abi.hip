@@ -1,0 +1,602 @@
+// apic_abi.hip -- the extern "C" boundary (include/apic.h) and the substep
+// driver of Apic::update (src/apic.rs:2429-2474) on one MI355X.
+//
+// The host side sequences kernels on one HIP stream and keeps every array of
+// the simulation resident in HBM between calls.  It synchronises with the
+// device only where the reference's control flow needs a reduced scalar:
+// max |v| for the substep dt (once per substep), and the PCG exit flag,
+// read every few iterations while the device-side state machine predicates
+// the queued iterations.
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "apic_internal.hpp"
+#include "device_math.hpp"
+
+using namespace apic;
+
+struct apic_handle {
+    Sim sim;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+apic_status fail(apic_status st, const std::string& msg) {
+    g_last_error = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            return fail(e_ == hipErrorOutOfMemory ? APIC_ERR_OUT_OF_MEMORY : APIC_ERR_HIP,    \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                  \
+    } while (0)
+
+#define CHECK_LAUNCH()                                                                       \
+    do {                                                                                     \
+        hipError_t e_ = hipGetLastError();                                                   \
+        if (e_ != hipSuccess) return fail(APIC_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+template <class T>
+apic_status dalloc(Sim& s, T*& p, size_t count) {
+    size_t bytes = std::max<size_t>(count * sizeof(T), 16);
+    bytes = (bytes + 255) / 256 * 256;
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+        return fail(e == hipErrorOutOfMemory ? APIC_ERR_OUT_OF_MEMORY : APIC_ERR_HIP,
+                    std::string("hipMalloc: ") + hipGetErrorString(e));
+    s.allocations.push_back(q);
+    e = hipMemset(q, 0, bytes);
+    if (e != hipSuccess) return fail(APIC_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    p = static_cast<T*>(q);
+    return APIC_OK;
+}
+
+#define ALLOC(ptr, n)                                     \
+    do {                                                  \
+        apic_status st_ = dalloc(s, ptr, (size_t)(n));    \
+        if (st_ != APIC_OK) return st_;                   \
+    } while (0)
+
+void release(Sim& s) {
+    if (s.stream) (void)hipStreamSynchronize(s.stream);
+    for (void* p : s.allocations) (void)hipFree(p);
+    s.allocations.clear();
+    if (s.h_pcg) (void)hipHostFree(s.h_pcg);
+    if (s.h_maxv) (void)hipHostFree(s.h_maxv);
+    if (s.h_fstats) (void)hipHostFree(s.h_fstats);
+    s.h_pcg = nullptr; s.h_maxv = nullptr; s.h_fstats = nullptr;
+    if (s.stream) (void)hipStreamDestroy(s.stream);
+    s.stream = nullptr;
+}
+
+// Apic::new (src/apic.rs:779-916) with optional explicit particles.
+apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_properties& wp, bool explicit_set,
+                  const float* xy, uint32_t n) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(APIC_ERR_NO_DEVICE, "no HIP device found");
+    if (W < 3 || H < 3) return fail(APIC_ERR_INVALID_ARGUMENT, "grid sides must be at least 3");
+    if (!(cell > 0.0f)) return fail(APIC_ERR_INVALID_ARGUMENT, "cell_size must be positive");
+    if ((uint64_t)W * H > (1ull << 31)) return fail(APIC_ERR_INVALID_ARGUMENT, "grid too large");
+    HIP_TRY(hipGetDevice(&s.device));
+    s.W = W; s.H = H; s.cells = W * H; s.cell = cell; s.props = wp;
+    s.solver = apic_solver_config{0, 1e-6f, 500, 1, 25};
+    s.timestamp = 0;
+    s.prev_dt = 1.0f / 60.0f;
+
+    // scene: walls one cell thick and the 3x3 stratified block (:831-849)
+    std::vector<uint8_t> obstacle(s.cells, 0);
+    std::vector<float> pos;
+    for (uint32_t y = 0; y < H; ++y)
+        for (uint32_t x = 0; x < W; ++x) {
+            if (x == 0 || x == W - 1 || y == 0 || y == H - 1) {
+                obstacle[(size_t)y * W + x] = 1;
+            } else if (!explicit_set && x > 4 && x < W - 4 && y > 4 && y < H / 2) {
+                for (int py = 0; py < 3; ++py)
+                    for (int px = 0; px < 3; ++px) {
+                        pos.push_back(((float)x + ((float)px + 0.5f) / 3.0f) * cell);
+                        pos.push_back(((float)y + ((float)py + 0.5f) / 3.0f) * cell);
+                    }
+            }
+        }
+    if (explicit_set && n) pos.assign(xy, xy + 2 * (size_t)n);
+    s.num_particles = (uint32_t)(pos.size() / 2);
+    s.num_chunks = (s.num_particles + 7) / 8;
+    s.nalloc = (size_t)s.num_chunks * 8;
+
+    // multigrid levels while both sides are even and > 4 (:900-911); the
+    // reference builds none when the scene has no particle (:852-855)
+    std::vector<std::pair<uint32_t, uint32_t>> dims{{W, H}};
+    if (s.num_particles > 0) {
+        uint32_t cw = W, ch = H;
+        while (cw % 2 == 0 && ch % 2 == 0 && cw > 4 && ch > 4) {
+            cw /= 2; ch /= 2;
+            dims.push_back({cw, ch});
+        }
+        if (dims.size() == 1)
+            return fail(APIC_ERR_UNSUPPORTED_SIZE,
+                        "no multigrid level: both grid sides must be even and > 4 (the reference panics here)");
+    }
+
+    HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    const size_t np = std::max<uint32_t>(s.num_particles, 1);
+    const size_t na = std::max<size_t>(s.nalloc, 8);
+    ALLOC(s.pbuf[0], NUM_PFIELDS * na);
+    ALLOC(s.pbuf[1], NUM_PFIELDS * na);
+    ALLOC(s.keys[0], np); ALLOC(s.keys[1], np);
+    ALLOC(s.vals[0], np); ALLOC(s.vals[1], np);
+    ALLOC(s.radix_scratch, radix_scratch_words(np));
+    const size_t nu = s.nu(), nv = s.nv(), nc = s.cells;
+    ALLOC(s.mac_u, nu); ALLOC(s.mac_v, nv); ALLOC(s.old_u, nu); ALLOC(s.old_v, nv);
+    ALLOC(s.valid_u, nu); ALLOC(s.valid_v, nv); ALLOC(s.nvalid_u, nu); ALLOC(s.nvalid_v, nv);
+    ALLOC(s.u_acc, nu); ALLOC(s.v_acc, nv);
+    ALLOC(s.density, nc); ALLOC(s.smooth_a, nc); ALLOC(s.smooth_b, nc); ALLOC(s.pressure, nc);
+    ALLOC(s.obstacle, nc); ALLOC(s.open_u, nu); ALLOC(s.open_v, nv);
+    ALLOC(s.ext_u, nu); ALLOC(s.ext_v, nv); ALLOC(s.post_u, nu); ALLOC(s.post_v, nv);
+    s.levels.clear();
+    for (auto& d : dims) {
+        LevelView L{};
+        L.w = d.first; L.h = d.second; L.cells = d.first * d.second;
+        size_t pa = s.padded(L.w, L.h);
+        ALLOC(L.diag, pa); ALLOC(L.inv_diag, pa); ALLOC(L.px, pa); ALLOC(L.py, pa);
+        ALLOC(L.r, pa); ALLOC(L.e, pa);
+        ALLOC(L.fluid, ((size_t)L.cells + 3) / 4 * 4);
+        s.levels.push_back(L);
+    }
+    if (s.levels.size() > 16) return fail(APIC_ERR_UNSUPPORTED_SIZE, "more than 15 multigrid levels");
+    // the one-workgroup bottom solver takes every level with <= 4096 cells
+    s.bottom_level = s.levels.size() > 1 ? s.levels.size() - 1 : 0;
+    for (size_t l = 1; l < s.levels.size(); ++l)
+        if (s.levels[l].cells <= 4096) { s.bottom_level = l; break; }
+    const LevelView& B = s.levels[0];
+    size_t pb = s.padded(B.w, B.h);
+    ALLOC(s.search[0], pb); ALLOC(s.search[1], pb); ALLOC(s.As, pb); ALLOC(s.rhs, pb);
+    const LevelView& C = s.levels.back();
+    size_t pcs = s.padded(C.w, C.h);
+    ALLOC(s.cg_s, pcs); ALLOC(s.cg_ap, pcs); ALLOC(s.cg_r, pcs);
+    ALLOC(s.pcg, 1);
+    s.partials_cap = (nc + 255) / 256 + 64;
+    double* partials = nullptr;
+    ALLOC(partials, s.partials_cap);
+    s.partials = partials;
+    ALLOC(s.fstats, 1);
+    ALLOC(s.maxv_bits, 1);
+    HIP_TRY(hipHostMalloc((void**)&s.h_pcg, sizeof(PcgState), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&s.h_maxv, sizeof(uint32_t), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&s.h_fstats, sizeof(FluidStats), hipHostMallocDefault));
+    std::memset(s.h_pcg, 0, sizeof(PcgState));
+    *s.h_maxv = 0;
+    std::memset(s.h_fstats, 0, sizeof(FluidStats));
+
+    // upload particles (tail lanes of the last chunk repeat the last particle, :887-898)
+    if (s.nalloc) {
+        std::vector<float> px(s.nalloc), py(s.nalloc);
+        for (size_t i = 0; i < s.nalloc; ++i) {
+            size_t k = std::min<size_t>(i, s.num_particles - 1);
+            px[i] = pos[2 * k];
+            py[i] = pos[2 * k + 1];
+        }
+        HIP_TRY(hipMemcpy(s.pfield(P_X, 0), px.data(), s.nalloc * 4, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s.pfield(P_Y, 0), py.data(), s.nalloc * 4, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(s.obstacle, obstacle.data(), nc, hipMemcpyHostToDevice));
+    // update_world (:913) runs only when the scene has particles
+    if (s.num_particles > 0) launch_open_faces(s);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    s.cur = 0;
+    s.keys_valid = false;
+    s.maxv_valid = true;  // velocities start at zero
+    *s.h_maxv = 0;
+    return APIC_OK;
+}
+
+float host_maxv(Sim& s) {
+    float f;
+    std::memcpy(&f, s.h_maxv, 4);
+    return f;
+}
+
+apic_status refresh_maxv(Sim& s) {
+    if (!s.maxv_valid) {
+        launch_max_velocity(s);
+        CHECK_LAUNCH();
+        HIP_TRY(hipMemcpyAsync(s.h_maxv, s.maxv_bits, 4, hipMemcpyDeviceToHost, s.stream));
+        s.maxv_valid = true;
+    }
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    return APIC_OK;
+}
+
+apic_status do_sort(Sim& s) {
+    if (s.num_particles == 0) return APIC_OK;
+    if (!s.keys_valid) launch_keygen(s);
+    launch_radix_sort(s);
+    launch_permute(s);
+    s.keys_valid = false;
+    CHECK_LAUNCH();
+    return APIC_OK;
+}
+
+// solve_pcg (src/apic.rs:1865-1956) with the reference's rules.
+apic_status do_pcg(Sim& s, bool zero_pressure) {
+    launch_pcg_reset(s);
+    if (zero_pressure) launch_zero_pressure(s);
+    launch_vcycle(s, 1);
+    CHECK_LAUNCH();
+    int k = 0;
+    int next_check = std::max(1, s.pcg_batch);
+    bool done = false;
+    while (k < 100 && !done) {
+        launch_pcg_iteration_head(s, k);
+        launch_vcycle(s, 2);
+        CHECK_LAUNCH();
+        ++k;
+        if (k == next_check || k == 100) {
+            HIP_TRY(hipMemcpyAsync(s.h_pcg, s.pcg, sizeof(PcgState), hipMemcpyDeviceToHost, s.stream));
+            HIP_TRY(hipStreamSynchronize(s.stream));
+            done = s.h_pcg->done != 0;
+            next_check = k + 2;
+        }
+    }
+    if (!done) {
+        HIP_TRY(hipMemcpyAsync(s.h_pcg, s.pcg, sizeof(PcgState), hipMemcpyDeviceToHost, s.stream));
+        HIP_TRY(hipStreamSynchronize(s.stream));
+    }
+    s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
+    s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
+    s.stats.pcg_max_residual = s.h_pcg->maxerr;
+    s.stats.pcg_rel_residual = -1.0;
+    // predict the next solve's length so that the first check lands on it
+    s.pcg_batch = std::max(1, s.h_pcg->iters);
+    return APIC_OK;
+}
+
+apic_status substep(Sim& s, bool sort, float dt) {
+    apic_status st;
+    if (sort && (st = do_sort(s)) != APIC_OK) return st;
+    launch_p2g(s);
+    launch_normalize_forces(s, true, true, dt);
+    launch_build_pressure(s, true);
+    launch_mg_build(s);
+    CHECK_LAUNCH();
+    if ((st = do_pcg(s, false)) != APIC_OK) return st;
+    launch_gradient(s, true, dt);
+    launch_extrapolate(s);
+    launch_g2p(s, dt, true);
+    CHECK_LAUNCH();
+    HIP_TRY(hipMemcpyAsync(s.h_maxv, s.maxv_bits, 4, hipMemcpyDeviceToHost, s.stream));
+    s.keys_valid = true;
+    s.maxv_valid = true;
+    return APIC_OK;
+}
+
+struct FieldDesc {
+    void* ptr;
+    size_t count;
+    size_t elem;
+};
+
+bool field_desc(Sim& s, int f, uint32_t level, FieldDesc& d) {
+    d = FieldDesc{nullptr, 0, 4};
+    if (f >= APIC_POS_X && f <= APIC_TIME_RESIDUAL) {
+        d.ptr = s.pfield(f - APIC_POS_X);
+        d.count = s.nalloc;
+        return true;
+    }
+    const LevelView* L = level < s.levels.size() ? &s.levels[level] : nullptr;
+    switch (f) {
+        case APIC_SORT_KEY: d.ptr = s.keys[0]; d.count = s.num_particles; return true;
+        case APIC_SORT_VAL: d.ptr = s.vals[0]; d.count = s.num_particles; return true;
+        case APIC_MAC_U: d.ptr = s.mac_u; d.count = s.nu(); return true;
+        case APIC_MAC_V: d.ptr = s.mac_v; d.count = s.nv(); return true;
+        case APIC_VALID_U: d.ptr = s.valid_u; d.count = s.nu(); d.elem = 1; return true;
+        case APIC_VALID_V: d.ptr = s.valid_v; d.count = s.nv(); d.elem = 1; return true;
+        case APIC_DENSITY: d.ptr = s.density; d.count = s.cells; return true;
+        case APIC_SMOOTHED_DENSITY: d.ptr = s.smooth_a; d.count = s.cells; return true;
+        case APIC_PRESSURE: d.ptr = s.pressure; d.count = s.cells; return true;
+        case APIC_OBSTACLE: d.ptr = s.obstacle; d.count = s.cells; d.elem = 1; return true;
+        case APIC_OPEN_U: d.ptr = s.open_u; d.count = s.nu(); d.elem = 1; return true;
+        case APIC_OPEN_V: d.ptr = s.open_v; d.count = s.nv(); d.elem = 1; return true;
+        case APIC_EXT_FORCE_U: d.ptr = s.ext_u; d.count = s.nu(); return true;
+        case APIC_EXT_FORCE_V: d.ptr = s.ext_v; d.count = s.nv(); return true;
+        case APIC_POST_FORCE_U: d.ptr = s.post_u; d.count = s.nu(); return true;
+        case APIC_POST_FORCE_V: d.ptr = s.post_v; d.count = s.nv(); return true;
+        case APIC_SEARCH: d.ptr = s.search[s.h_pcg ? (s.h_pcg->iters & 1) : 0]; d.count = s.padded(s.W, s.H); return true;
+        case APIC_MATRIX_TIMES_SEARCH: d.ptr = s.As; d.count = s.padded(s.W, s.H); return true;
+        default: break;
+    }
+    if (!L) return false;
+    const size_t pa = s.padded(L->w, L->h);
+    switch (f) {
+        case APIC_FLUID: d.ptr = L->fluid; d.count = L->cells; d.elem = 1; return true;
+        case APIC_DIAG: d.ptr = L->diag; d.count = pa; return true;
+        case APIC_INV_DIAG: d.ptr = L->inv_diag; d.count = pa; return true;
+        case APIC_PLUS_X: d.ptr = L->px; d.count = pa; return true;
+        case APIC_PLUS_Y: d.ptr = L->py; d.count = pa; return true;
+        case APIC_RESIDUAL: d.ptr = L->r; d.count = pa; return true;
+        case APIC_ERROR: d.ptr = L->e; d.count = pa; return true;
+        default: return false;
+    }
+}
+
+apic_status create(uint32_t width, uint32_t height, float cell_size, const apic_world_properties* world_properties,
+                   bool explicit_set, const float* xy, uint32_t n, apic_handle** out) {
+    if (!out || !world_properties) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (xy == nullptr && n != 0) return fail(APIC_ERR_INVALID_ARGUMENT, "xy is null but n > 0");
+    apic_handle* h = new (std::nothrow) apic_handle();
+    if (!h) return fail(APIC_ERR_OUT_OF_MEMORY, "host allocation failed");
+    apic_status st = build(h->sim, width, height, cell_size, *world_properties, explicit_set, xy, n);
+    if (st != APIC_OK) {
+        release(h->sim);
+        delete h;
+        return st;
+    }
+    *out = h;
+    return APIC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* apic_last_error(void) { return g_last_error.c_str(); }
+
+apic_status apic_create_with_particles(uint32_t width, uint32_t height, float cell_size,
+                                       const apic_world_properties* world_properties, const float* xy, uint32_t n,
+                                       apic_handle** out) {
+    return create(width, height, cell_size, world_properties, true, xy, n, out);
+}
+
+apic_status apic_create(uint32_t width, uint32_t height, float cell_size,
+                        const apic_world_properties* world_properties, apic_handle** out) {
+    return create(width, height, cell_size, world_properties, false, nullptr, 0, out);
+}
+
+void apic_destroy(apic_handle* h) {
+    if (!h) return;
+    release(h->sim);
+    delete h;
+}
+
+apic_status apic_update(apic_handle* h, const apic_debug_options* opts, float frame_dt) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    Sim& s = h->sim;
+    const bool disable_sort = opts && opts->disable_particle_sort;
+    float t = 0.0f;
+    uint32_t substeps = 0, total_it = 0;
+    float last_maxv = 0.0f;
+    apic_status st;
+    while (t < frame_dt) {
+        if ((st = refresh_maxv(s)) != APIC_OK) return st;
+        float maxv = host_maxv(s);
+        last_maxv = maxv;
+        float safe = maxv > 1e-5f ? (s.props.cfl * s.cell) / maxv : frame_dt;
+        float rem = frame_dt - t;
+        float dt = safe < rem ? safe : rem;  // f32::min
+        if ((st = substep(s, !disable_sort && t == 0.0f, dt)) != APIC_OK) return st;
+        t += dt;
+        s.prev_dt = dt;
+        s.timestamp += 1;
+        substeps += 1;
+        total_it += s.stats.pcg_iterations;
+    }
+    if (s.forces_active) {
+        launch_fill_f32(s.stream, s.ext_u, 0.0f, s.nu());
+        launch_fill_f32(s.stream, s.ext_v, 0.0f, s.nv());
+        launch_fill_f32(s.stream, s.post_u, 0.0f, s.nu());
+        launch_fill_f32(s.stream, s.post_v, 0.0f, s.nv());
+        CHECK_LAUNCH();
+        s.forces_active = false;
+    }
+    s.stats.substeps = substeps;
+    s.stats.pcg_iterations_total = total_it;
+    s.stats.fluid_cells = s.h_fstats->count;
+    s.stats.timestamp = s.timestamp;
+    s.stats.prev_deltatime = s.prev_dt;
+    s.stats.max_particle_velocity = last_maxv;
+    return APIC_OK;
+}
+
+apic_status apic_add_radial_force(apic_handle* h, float wx, float wy, float radius, float strength) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    launch_radial_force(h->sim, wx, wy, radius, strength);
+    CHECK_LAUNCH();
+    return APIC_OK;
+}
+
+apic_status apic_get_max_particle_velocity(apic_handle* h, float* out) {
+    if (!h || !out) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    apic_status st = refresh_maxv(h->sim);
+    if (st != APIC_OK) return st;
+    *out = host_maxv(h->sim);
+    return APIC_OK;
+}
+
+apic_status apic_is_fluid(apic_handle* h, int32_t i, int32_t j, uint64_t* out) {
+    if (!h || !out) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    Sim& s = h->sim;
+    if (i < 0 || i >= (int32_t)s.W || j < 0 || j >= (int32_t)s.H) { *out = 0; return APIC_OK; }
+    uint8_t v = 0;
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    HIP_TRY(hipMemcpy(&v, s.levels[0].fluid + (size_t)j * s.W + i, 1, hipMemcpyDeviceToHost));
+    *out = v ? 1 : 0;
+    return APIC_OK;
+}
+
+apic_status apic_set_solver_config(apic_handle* h, const apic_solver_config* cfg) {
+    if (!h || !cfg) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    if (cfg->mode != 0) return fail(APIC_ERR_INVALID_ARGUMENT, "only the reference solver rules (mode 0) are built");
+    h->sim.solver = *cfg;
+    return APIC_OK;
+}
+
+apic_status apic_get_stats(apic_handle* h, apic_step_stats* out) {
+    if (!h || !out) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    *out = h->sim.stats;
+    return APIC_OK;
+}
+
+apic_status apic_get_dims(apic_handle* h, uint32_t* width, uint32_t* height, uint32_t* num_particles,
+                          uint32_t* num_chunks, uint32_t* num_levels) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    Sim& s = h->sim;
+    if (width) *width = s.W;
+    if (height) *height = s.H;
+    if (num_particles) *num_particles = s.num_particles;
+    if (num_chunks) *num_chunks = s.num_chunks;
+    if (num_levels) *num_levels = (uint32_t)(s.levels.size() - 1);
+    return APIC_OK;
+}
+
+apic_status apic_get_timestamp(apic_handle* h, uint32_t* timestamp, float* prev_deltatime) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    if (timestamp) *timestamp = h->sim.timestamp;
+    if (prev_deltatime) *prev_deltatime = h->sim.prev_dt;
+    return APIC_OK;
+}
+
+apic_status apic_set_timestamp(apic_handle* h, uint32_t timestamp, float prev_deltatime) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    h->sim.timestamp = timestamp;
+    h->sim.prev_dt = prev_deltatime;
+    return APIC_OK;
+}
+
+apic_status apic_field_length(apic_handle* h, apic_field field, uint32_t level, size_t* count) {
+    if (!h || !count) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    FieldDesc d;
+    if (!field_desc(h->sim, field, level, d)) return fail(APIC_ERR_INVALID_ARGUMENT, "unknown field or level");
+    *count = d.count;
+    return APIC_OK;
+}
+
+apic_status apic_read_field(apic_handle* h, apic_field field, uint32_t level, void* dst, size_t count) {
+    if (!h || (!dst && count)) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    Sim& s = h->sim;
+    FieldDesc d;
+    if (!field_desc(s, field, level, d)) return fail(APIC_ERR_INVALID_ARGUMENT, "unknown field or level");
+    if (count != d.count) return fail(APIC_ERR_INVALID_ARGUMENT, "count does not match the field length");
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    if (count) HIP_TRY(hipMemcpy(dst, d.ptr, count * d.elem, hipMemcpyDeviceToHost));
+    return APIC_OK;
+}
+
+apic_status apic_write_field(apic_handle* h, apic_field field, uint32_t level, const void* src, size_t count) {
+    if (!h || (!src && count)) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    Sim& s = h->sim;
+    FieldDesc d;
+    if (!field_desc(s, field, level, d)) return fail(APIC_ERR_INVALID_ARGUMENT, "unknown field or level");
+    if (count != d.count) return fail(APIC_ERR_INVALID_ARGUMENT, "count does not match the field length");
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    if (count) HIP_TRY(hipMemcpy(d.ptr, src, count * d.elem, hipMemcpyHostToDevice));
+    if (field == APIC_POS_X || field == APIC_POS_Y) s.keys_valid = false;
+    if (field == APIC_VEL_X || field == APIC_VEL_Y) s.maxv_valid = false;
+    if (field == APIC_SORT_KEY || field == APIC_SORT_VAL) s.keys_valid = true;
+    if (field >= APIC_EXT_FORCE_U && field <= APIC_POST_FORCE_V) s.forces_active = true;
+    if (field == APIC_FLUID && level == 0) {
+        launch_fluid_stats(s);
+        CHECK_LAUNCH();
+        HIP_TRY(hipStreamSynchronize(s.stream));
+    }
+    return APIC_OK;
+}
+
+apic_status apic_update_spatial_lookup(apic_handle* h) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    apic_status st = do_sort(h->sim);
+    if (st != APIC_OK) return st;
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    return APIC_OK;
+}
+
+apic_status apic_transfer_particles_to_grid(apic_handle* h, float deltatime, uint32_t timestamp) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    (void)deltatime; (void)timestamp;  // unused by the reference as well (:1218)
+    launch_p2g(h->sim);
+    launch_normalize_forces(h->sim, true, false, 0.0f);
+    launch_fluid_stats(h->sim);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    return APIC_OK;
+}
+
+apic_status apic_apply_external_forces(apic_handle* h, float deltatime) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    launch_normalize_forces(h->sim, false, true, deltatime);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    return APIC_OK;
+}
+
+apic_status apic_build_pressure_system(apic_handle* h) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    launch_build_pressure(h->sim, false);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    return APIC_OK;
+}
+
+apic_status apic_update_multilevel_grid(apic_handle* h) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    launch_mg_build(h->sim);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    return APIC_OK;
+}
+
+apic_status apic_apply_preconditioner(apic_handle* h) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    launch_pcg_reset(h->sim);
+    launch_vcycle(h->sim, 0);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    return APIC_OK;
+}
+
+apic_status apic_solve_pcg(apic_handle* h) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    apic_status st = do_pcg(h->sim, true);
+    if (st != APIC_OK) return st;
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    return APIC_OK;
+}
+
+apic_status apic_apply_pressure_gradient(apic_handle* h) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    launch_gradient(h->sim, false, 0.0f);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    return APIC_OK;
+}
+
+apic_status apic_extrapolate_velocity(apic_handle* h) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    launch_extrapolate(h->sim);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    return APIC_OK;
+}
+
+apic_status apic_transfer_grid_to_particles_and_advect(apic_handle* h, float deltatime) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    Sim& s = h->sim;
+    launch_g2p(s, deltatime, true);
+    CHECK_LAUNCH();
+    HIP_TRY(hipMemcpyAsync(s.h_maxv, s.maxv_bits, 4, hipMemcpyDeviceToHost, s.stream));
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    s.keys_valid = true;
+    s.maxv_valid = true;
+    return APIC_OK;
+}
+
+}  // extern "C"

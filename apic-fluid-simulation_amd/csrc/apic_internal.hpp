@@ -1,0 +1,162 @@
+// apic_internal.hpp -- device state of one simulation and the launchers of
+// every gfx950 kernel.  Host code only (plus plain-old-data structs shared
+// with the kernels).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/apic.h"
+
+namespace apic {
+
+// Particle fields in one SoA block: field f of particle i is at f*nalloc + i.
+enum ParticleField { P_X = 0, P_Y, V_X, V_Y, CU_X, CU_Y, CV_X, CV_Y, T_RES, NUM_PFIELDS };
+
+// One level of the multigrid hierarchy.  Level 0 is the base pressure grid.
+// Solver arrays are padded by one row on each side (index = raw + w), as in
+// MultiGridLevel (src/apic.rs:402-453), so that the 5-point stencil of any
+// interior row reads zeros past the grid instead of branching.
+struct LevelView {
+    uint32_t w, h, cells;
+    float* diag;
+    float* inv_diag;
+    float* px;
+    float* py;
+    float* r;
+    float* e;
+    uint8_t* fluid;  // raw index
+};
+
+// Reductions finish in the last block to arrive (ticket counter), which sums
+// the per-block partials in block order, so results are run-to-run stable.
+struct PcgState {
+    float gamma, delta, alpha, beta, maxerr;
+    int32_t done;        // nonzero: every kernel of the PCG loop returns at once
+    int32_t iters;
+    int32_t exit_reason;
+    int32_t first;       // 1 until the first search-direction update (s = z)
+    uint32_t ticket[4];  // last-block counters, reset by the block that finishes
+    double dgamma, ddelta, bnorm2, rnorm2;  // relative-mode scalars (FP64)
+    float maxv;          // unused padding for alignment
+    int32_t pad;
+};
+
+// Base-level fluid statistics needed by the padded-list quirk of prolongate
+// (src/apic.rs:1487-1494, 599-615): count and largest raw index.
+struct FluidStats {
+    uint32_t count;
+    uint32_t last_raw;  // largest fluid raw index + 1 (0 when there is none)
+};
+
+struct Sim;
+
+// ---- launchers (kernels live in sort.hip, transfer.hip, pressure.hip, grid.hip)
+void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
+void launch_fill_f32(hipStream_t s, float* p, float v, size_t n);
+
+// sort.hip
+void launch_keygen(Sim& sim);
+void launch_radix_sort(Sim& sim);  // keys/vals[0] -> sorted in keys/vals[0]
+void launch_permute(Sim& sim);
+size_t radix_scratch_words(size_t n);
+
+// transfer.hip
+void launch_p2g(Sim& sim);
+void launch_normalize_forces(Sim& sim, bool normalize, bool forces, float dt);
+void launch_g2p(Sim& sim, float dt, bool write_keys);
+void launch_max_velocity(Sim& sim);
+
+// grid.hip
+void launch_gradient(Sim& sim, bool post_force, float dt);
+void launch_post_force(Sim& sim, float dt);
+void launch_extrapolate(Sim& sim);
+void launch_radial_force(Sim& sim, float wx, float wy, float radius, float strength);
+void launch_open_faces(Sim& sim);
+
+// pressure.hip
+void launch_build_pressure(Sim& sim, bool zero_pressure);
+void launch_mg_build(Sim& sim);
+void launch_vcycle(Sim& sim, int mode);  // mode 0: plain, 1: PCG start (gamma), 2: PCG iteration (gamma')
+void launch_pcg_reset(Sim& sim);
+void launch_pcg_iteration_head(Sim& sim, int k);  // s update + A s + delta, then p/r update + max|r|
+void launch_zero_pressure(Sim& sim);
+void launch_fluid_stats(Sim& sim);
+
+struct Sim {
+    // dimensions and parameters
+    uint32_t W = 0, H = 0, cells = 0;
+    float cell = 1.0f;
+    apic_world_properties props{};
+    apic_solver_config solver{};
+    uint32_t num_particles = 0, num_chunks = 0;
+    size_t nalloc = 0;  // num_chunks * 8
+    uint32_t timestamp = 0;
+    float prev_dt = 1.0f / 60.0f;
+
+    hipStream_t stream = nullptr;
+    int device = 0;
+
+    // particles: two SoA blocks (sort double buffer)
+    float* pbuf[2] = {nullptr, nullptr};
+    int cur = 0;
+    // sort records and radix scratch
+    uint32_t* keys[2] = {nullptr, nullptr};
+    uint32_t* vals[2] = {nullptr, nullptr};
+    uint32_t* radix_scratch = nullptr;
+    bool keys_valid = false;  // keys[0]/vals[0] hold the current positions' keys
+
+    // MAC grid
+    float* mac_u = nullptr;   float* mac_v = nullptr;
+    float* old_u = nullptr;   float* old_v = nullptr;
+    uint8_t* valid_u = nullptr; uint8_t* valid_v = nullptr;
+    uint8_t* nvalid_u = nullptr; uint8_t* nvalid_v = nullptr;
+    float2* u_acc = nullptr;  float2* v_acc = nullptr;  // (momentum, weight) per face
+    float* density = nullptr;
+    float* smooth_a = nullptr; float* smooth_b = nullptr;  // smoothed density ends in smooth_a
+    float* pressure = nullptr;
+    uint8_t* obstacle = nullptr;
+    uint8_t* open_u = nullptr; uint8_t* open_v = nullptr;
+    float* ext_u = nullptr; float* ext_v = nullptr;
+    float* post_u = nullptr; float* post_v = nullptr;
+    bool forces_active = false;
+
+    // solver
+    std::vector<LevelView> levels;  // [0] = base grid, [k] = multigrid_levels[k-1]
+    size_t bottom_level = 0;        // first level solved by the one-workgroup kernel
+    float* search[2] = {nullptr, nullptr};
+    float* As = nullptr;
+    float* cg_s = nullptr; float* cg_ap = nullptr; float* cg_r = nullptr;  // coarsest level
+    float* rhs = nullptr;            // copy of b for the relative mode's residual recompute
+    PcgState* pcg = nullptr;
+    void* partials = nullptr;        // per-block reduction partials (double-sized)
+    size_t partials_cap = 0;
+    FluidStats* fstats = nullptr;
+    uint32_t* maxv_bits = nullptr;   // max |v| over all lanes (float bits)
+    bool maxv_valid = false;
+
+    // host-pinned mirrors for the per-substep reads
+    PcgState* h_pcg = nullptr;
+    uint32_t* h_maxv = nullptr;
+    FluidStats* h_fstats = nullptr;
+
+    apic_step_stats stats{};
+    int pcg_batch = 4;
+
+    // device allocations to free
+    std::vector<void*> allocations;
+
+    size_t nu() const { return (size_t)cells + H; }
+    size_t nv() const { return (size_t)cells + W; }
+    size_t padded(uint32_t w, uint32_t h) const { return (((size_t)w * h + 2 * (size_t)w) + 7) / 8 * 8; }
+    float* pfield(int f, int which = -1) const {
+        return pbuf[which < 0 ? cur : which] + (size_t)f * nalloc;
+    }
+};
+
+}  // namespace apic

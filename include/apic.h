@@ -1,0 +1,195 @@
+/*
+ * apic.h -- C ABI of the MI355X-native APIC time step (libapic_hip).
+ *
+ * The reference exposes a Rust module API, `FLIP::apic` (src/lib.rs:1, crate
+ * name Cargo.toml:2), with no FFI.  Each entry point below replaces one public
+ * item of that API; the comment names the item it stands in for.
+ *
+ * Conventions
+ *  - Every call returns an apic_status (0 = ok).  apic_last_error() gives the
+ *    message of the most recent failure on the calling thread.
+ *  - A handle owns all device memory of one simulation and one HIP stream.
+ *    One thread at a time per handle (the reference takes `&mut self`).
+ *  - Caller buffers stay caller-owned; reads and writes are synchronous copies.
+ *  - Grid indexing follows the reference: cell (x,y) -> y*W+x, u face (i,j) ->
+ *    j*(W+1)+i, v face (i,j) -> j*W+i (src/apic.rs:977-991).  Solver arrays
+ *    (DIAG..MATRIX_TIMES_SEARCH) are padded by one row on each side:
+ *    index = raw + W, length ceil8(W*H + 2W) (src/apic.rs:428-453).
+ *  - Particle arrays are flat SoA of length num_chunks*8 (num_chunks =
+ *    ceil(N/8)); particle i of the reference's AoSoA layout is chunk i/8,
+ *    lane i%8 (src/apic.rs:757-767, 876-898).
+ */
+#ifndef APIC_H
+#define APIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum apic_status {
+    APIC_OK = 0,
+    APIC_ERR_INVALID_ARGUMENT = 1,
+    APIC_ERR_UNSUPPORTED_SIZE = 2, /* no multigrid level can be built (the reference panics, src/apic.rs:910) */
+    APIC_ERR_OUT_OF_MEMORY = 3,
+    APIC_ERR_HIP = 4,
+    APIC_ERR_NO_DEVICE = 5
+} apic_status;
+
+typedef struct apic_handle apic_handle;
+
+/* WorldProperties (src/apic.rs:707-713). */
+typedef struct apic_world_properties {
+    float gravity;
+    float border_damping;
+    float cfl;
+    float solve_error;
+} apic_world_properties;
+
+/* DebugOptions (src/apic.rs:397-400). */
+typedef struct apic_debug_options {
+    uint8_t disable_particle_sort;
+} apic_debug_options;
+
+/* Pressure-solver rules.  mode 0 ("reference") reproduces solve_pcg exactly:
+ * absolute 1e-5 exits on r.z and s.As, infinity-norm residual test against
+ * WorldProperties.solve_error, 100-iteration cap, FP32 reductions
+ * (src/apic.rs:8, 1865-1956).  mode 1 ("relative") is an extension: stop when
+ * ||r||_2 <= rel_tol * ||b||_2 or after max_iterations; the dot products and
+ * the convergence test run in FP64 when fp64_reductions != 0, and the true
+ * residual b - A p is recomputed every recompute_every iterations (0 = never). */
+typedef struct apic_solver_config {
+    int32_t mode;
+    float rel_tol;
+    int32_t max_iterations;
+    int32_t fp64_reductions;
+    int32_t recompute_every;
+} apic_solver_config;
+
+/* Counters of the most recent apic_update call. */
+typedef struct apic_step_stats {
+    uint32_t substeps;              /* substeps taken by the last update */
+    uint32_t pcg_iterations;        /* PCG iterations of the last substep */
+    uint32_t pcg_iterations_total;  /* summed over the last update's substeps */
+    uint32_t pcg_exit_reason;       /* 0 running, 1 gamma<eps at start, 2 |delta|<eps,
+                                       3 residual below tolerance, 4 gamma'<eps, 5 iteration cap */
+    float pcg_max_residual;         /* infinity norm of r at exit */
+    double pcg_rel_residual;        /* ||r||2/||b||2 at exit (relative mode), else -1 */
+    uint32_t fluid_cells;           /* base-level fluid cells of the last substep */
+    uint32_t timestamp;             /* Apic::timestamp after the update */
+    float prev_deltatime;           /* Apic::prev_deltatime after the update */
+    float max_particle_velocity;    /* value that set the last substep's dt */
+} apic_step_stats;
+
+/* Fields readable / writable for host mirrors and per-stage tests.
+ * Element type F32 unless noted: U32 for the sort record, U8 (0/1) for flags. */
+typedef enum apic_field {
+    APIC_POS_X = 0, APIC_POS_Y, APIC_VEL_X, APIC_VEL_Y,
+    APIC_C_U_X, APIC_C_U_Y, APIC_C_V_X, APIC_C_V_Y, APIC_TIME_RESIDUAL, /* num_chunks*8 */
+    APIC_SORT_KEY,            /* U32, num_particles */
+    APIC_SORT_VAL,            /* U32, num_particles */
+    APIC_MAC_U,               /* (W+1)*H */
+    APIC_MAC_V,               /* W*(H+1) */
+    APIC_VALID_U,             /* U8, (W+1)*H */
+    APIC_VALID_V,             /* U8, W*(H+1) */
+    APIC_DENSITY,             /* W*H */
+    APIC_SMOOTHED_DENSITY,    /* W*H */
+    APIC_PRESSURE,            /* W*H */
+    APIC_FLUID,               /* U8, one per cell of the level */
+    APIC_OBSTACLE,            /* U8, W*H */
+    APIC_OPEN_U,              /* U8, (W+1)*H; the reference's mac_u_blocked bit (1 = open) */
+    APIC_OPEN_V,              /* U8, W*(H+1) */
+    APIC_DIAG, APIC_INV_DIAG, APIC_PLUS_X, APIC_PLUS_Y,
+    APIC_RESIDUAL, APIC_ERROR,                 /* padded, per level */
+    APIC_SEARCH, APIC_MATRIX_TIMES_SEARCH,     /* padded, base level */
+    APIC_EXT_FORCE_U, APIC_EXT_FORCE_V,        /* u faces, v faces */
+    APIC_POST_FORCE_U, APIC_POST_FORCE_V,
+    APIC_FIELD_COUNT
+} apic_field;
+
+/* ---- lifetime ---------------------------------------------------------- */
+
+/* Apic::new (src/apic.rs:779-916): walls one cell thick, 3x3 stratified
+ * particles in cells 4<x<W-4, 4<y<H/2, multigrid levels while both sides
+ * are even and > 4, open-face masks (update_world, :918-944). */
+apic_status apic_create(uint32_t width, uint32_t height, float cell_size,
+                        const apic_world_properties* world_properties,
+                        apic_handle** out);
+
+/* Apic::new followed by replacing the particle set with n explicit positions
+ * (xy interleaved x0,y0,x1,y1,...), zero velocity, C and time residual; the
+ * last chunk's tail lanes repeat the last particle as the constructor does
+ * (src/apic.rs:851-898).  The reference can only be driven this way by
+ * overwriting its pub particle fields after new. */
+apic_status apic_create_with_particles(uint32_t width, uint32_t height, float cell_size,
+                                       const apic_world_properties* world_properties,
+                                       const float* xy, uint32_t n,
+                                       apic_handle** out);
+
+/* Drop of Apic: frees device memory and the stream. */
+void apic_destroy(apic_handle* h);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* apic_last_error(void);
+
+/* ---- the step ------------------------------------------------------------ */
+
+/* Apic::update (src/apic.rs:2429-2474). */
+apic_status apic_update(apic_handle* h, const apic_debug_options* debug_options,
+                        float frame_deltatime);
+
+/* Apic::add_radial_force (src/apic.rs:2352-2410). */
+apic_status apic_add_radial_force(apic_handle* h, float world_x, float world_y,
+                                  float radius, float strength);
+
+/* Apic::get_max_particle_velocity (src/apic.rs:2412-2427). */
+apic_status apic_get_max_particle_velocity(apic_handle* h, float* out);
+
+/* Apic::is_fluid (src/apic.rs:1015-1022). */
+apic_status apic_is_fluid(apic_handle* h, int32_t i, int32_t j, uint64_t* out);
+
+/* Solver rules (extension; default is mode 0, the reference's rules). */
+apic_status apic_set_solver_config(apic_handle* h, const apic_solver_config* cfg);
+
+apic_status apic_get_stats(apic_handle* h, apic_step_stats* out);
+
+/* ---- pub fields (src/apic.rs:715-776) --------------------------------------- */
+
+/* width, height, num_particles, num_chunks, multigrid level count (excluding
+ * the base grid), fluid_cells, timestamp. */
+apic_status apic_get_dims(apic_handle* h, uint32_t* width, uint32_t* height,
+                          uint32_t* num_particles, uint32_t* num_chunks,
+                          uint32_t* num_levels);
+apic_status apic_get_timestamp(apic_handle* h, uint32_t* timestamp, float* prev_deltatime);
+apic_status apic_set_timestamp(apic_handle* h, uint32_t timestamp, float prev_deltatime);
+
+/* Element count of a field; level 0 is the base grid, level k>0 is
+ * multigrid_levels[k-1] (APIC_FLUID and the padded solver fields only). */
+apic_status apic_field_length(apic_handle* h, apic_field field, uint32_t level, size_t* count);
+apic_status apic_read_field(apic_handle* h, apic_field field, uint32_t level,
+                            void* dst, size_t count);
+apic_status apic_write_field(apic_handle* h, apic_field field, uint32_t level,
+                             const void* src, size_t count);
+
+/* ---- pub stage methods (run one stage on the device state) ------------------ */
+
+apic_status apic_update_spatial_lookup(apic_handle* h);                       /* :1043-1111 */
+apic_status apic_transfer_particles_to_grid(apic_handle* h, float deltatime,
+                                            uint32_t timestamp);              /* :1218-1497 */
+apic_status apic_apply_external_forces(apic_handle* h, float deltatime);     /* :1604-1626 */
+apic_status apic_build_pressure_system(apic_handle* h);                      /* :1640-1785 */
+apic_status apic_update_multilevel_grid(apic_handle* h);                     /* :1499-1602 */
+apic_status apic_apply_preconditioner(apic_handle* h);                       /* :1788-1821 */
+apic_status apic_solve_pcg(apic_handle* h);                                  /* :1865-1956 */
+apic_status apic_apply_pressure_gradient(apic_handle* h);                    /* :1958-2055 */
+apic_status apic_extrapolate_velocity(apic_handle* h);                       /* :2266-2350 */
+apic_status apic_transfer_grid_to_particles_and_advect(apic_handle* h,
+                                                       float deltatime);     /* :2057-2264 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* APIC_H */

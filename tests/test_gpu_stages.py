@@ -1,0 +1,198 @@
+"""Per-stage parity of the HIP path against the oracle.
+
+Each test steps the oracle to the point just before one stage, copies that
+stage's inputs into the GPU simulation through the C ABI, runs the stage on
+both sides and compares.  Stages without reductions must match bit for bit
+(the library keeps the reference's operation order and fused multiply-adds);
+the P2G scatter and the PCG sums are compared within a tolerance because their
+summation order differs (the reference's own order is nondeterministic).
+"""
+import numpy as np
+import pytest
+
+from helpers import PARTICLE_FIELDS, bit_equal, copy_state, first_mismatch, props, rel_l2, to_gpu
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {
+    # the reference constructor's scene (benches/fluid_benchmark.rs: cell 1.0), after impact
+    "ref64": dict(W=64, H=64, cell=1.0, cfg=None, dt=0.016, warm=60),
+    # the 128^2 dam break (dx = 1/128, 4 ppc, seed 1), mid-collapse
+    "dam128": dict(W=128, H=128, cell=1.0 / 128, cfg="cfg1_dam_break_128", dt=0.002, warm=40),
+}
+
+ORDER = ["update_spatial_lookup", "transfer_particles_to_grid", "apply_external_forces", "build_pressure_system",
+         "update_multilevel_grid", "solve_pcg", "apply_pressure_gradient", "apply_post_forces",
+         "extrapolate_velocity", "transfer_grid_to_particles_and_advect"]
+
+
+def make_pair(name):
+    from FLIP import scenes
+    from FLIP.apic import Apic
+    from oracle import pyoracle as po
+    s = SCENES[name]
+    xy = scenes.config_scene(s["cfg"]) if s["cfg"] else None
+    o = po.Oracle(s["W"], s["H"], s["cell"], po.bench_props(), xy=xy)
+    assert o.status == 0
+    for _ in range(s["warm"]):
+        o.update(s["dt"])
+    if xy is None:
+        g = Apic.new(s["W"], s["H"], s["cell"], props())
+    else:
+        g = Apic.with_particles(s["W"], s["H"], s["cell"], props(), xy)
+    assert g.num_particles == o.dims()["num_particles"]
+    copy_state(o, g)
+    return o, g, s
+
+
+def oracle_until(o, stage, dt):
+    for st in ORDER[:ORDER.index(stage)]:
+        o.stage(st, dt)
+
+
+def padded_fluid_mask(o, level=0):
+    fl = o.read("FLUID", level).astype(bool)
+    n = o.field_length("DIAG", level)
+    w = o.dims()["width"] >> level
+    m = np.zeros(n, dtype=bool)
+    m[w:w + fl.size] = fl
+    return m
+
+
+@pytest.fixture(params=list(SCENES))
+def pair(request):
+    o, g, s = make_pair(request.param)
+    yield o, g, s
+    g.close()
+
+
+def test_sort_bit_exact(pair):
+    o, g, s = pair
+    o.stage("update_spatial_lookup")
+    g.update_spatial_lookup()
+    for f in ("SORT_KEY", "SORT_VAL"):
+        assert np.array_equal(g.read(f), o.read(f)), f
+    keys = g.read("SORT_KEY")
+    assert np.all(keys[1:] >= keys[:-1])
+    assert np.array_equal(np.sort(g.read("SORT_VAL")), np.arange(keys.size, dtype=np.uint32))
+    for f in PARTICLE_FIELDS:
+        assert bit_equal(g.read(f), o.read(f)), (f, first_mismatch(g.read(f), o.read(f)))
+
+
+def test_p2g(pair):
+    o, g, s = pair
+    oracle_until(o, "transfer_particles_to_grid", s["dt"])
+    copy_state(o, g)
+    o.stage("transfer_particles_to_grid")
+    g.transfer_particles_to_grid()
+    for f in ("VALID_U", "VALID_V", "FLUID"):
+        assert np.array_equal(g.read(f), o.read(f)), f
+    # float sums in a different order: tolerance 1e-5 relative L2 (and 1e-4 max relative per face)
+    for f in ("MAC_U", "MAC_V", "DENSITY"):
+        a, b = g.read(f), o.read(f)
+        assert rel_l2(a, b) <= 1e-5, (f, rel_l2(a, b))
+
+
+def test_external_forces_bit_exact(pair):
+    o, g, s = pair
+    oracle_until(o, "apply_external_forces", s["dt"])
+    to_gpu(o, g, ["MAC_U", "MAC_V", "FLUID"])
+    o.stage("apply_external_forces", s["dt"])
+    g.apply_external_forces(s["dt"])
+    for f in ("MAC_U", "MAC_V"):
+        assert bit_equal(g.read(f), o.read(f)), (f, first_mismatch(g.read(f), o.read(f)))
+
+
+def test_build_pressure_bit_exact(pair):
+    o, g, s = pair
+    oracle_until(o, "build_pressure_system", s["dt"])
+    to_gpu(o, g, ["MAC_U", "MAC_V", "DENSITY", "FLUID"])
+    o.stage("build_pressure_system")
+    g.build_pressure_system()
+    assert bit_equal(g.read("SMOOTHED_DENSITY"), o.read("SMOOTHED_DENSITY"))
+    m = padded_fluid_mask(o)
+    for f in ("DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y"):
+        assert bit_equal(g.read(f), o.read(f)), (f, first_mismatch(g.read(f), o.read(f)))
+    assert bit_equal(g.read("RESIDUAL")[m], o.read("RESIDUAL")[m])
+
+
+def test_multilevel_grid_bit_exact(pair):
+    o, g, s = pair
+    oracle_until(o, "update_multilevel_grid", s["dt"])
+    to_gpu(o, g, ["FLUID", "DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y"])
+    o.stage("update_multilevel_grid")
+    g.update_multilevel_grid()
+    nl = o.dims()["num_levels"]
+    assert g.dims()["num_levels"] == nl
+    for lvl in range(1, nl + 1):
+        assert np.array_equal(g.read("FLUID", lvl), o.read("FLUID", lvl)), lvl
+        for f in ("DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y"):
+            assert bit_equal(g.read(f, lvl), o.read(f, lvl)), (lvl, f, first_mismatch(g.read(f, lvl), o.read(f, lvl)))
+
+
+def test_preconditioner_bit_exact(pair):
+    """apply_preconditioner has no reduction outside the coarse CG, whose sums the
+    library performs serially in the reference's order: the V-cycle is exact."""
+    o, g, s = pair
+    oracle_until(o, "solve_pcg", s["dt"])
+    nl = o.dims()["num_levels"]
+    for lvl in range(0, nl + 1):
+        to_gpu(o, g, ["FLUID", "DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y"], level=lvl)
+    to_gpu(o, g, ["RESIDUAL"], level=0)
+    o.stage("apply_preconditioner")
+    g.apply_preconditioner()
+    for lvl in range(0, nl + 1):
+        m = padded_fluid_mask(o, lvl)
+        a, b = g.read("ERROR", lvl)[m], o.read("ERROR", lvl)[m]
+        assert bit_equal(a, b), (lvl, first_mismatch(a, b))
+
+
+def test_solve_pcg(pair):
+    o, g, s = pair
+    oracle_until(o, "solve_pcg", s["dt"])
+    nl = o.dims()["num_levels"]
+    for lvl in range(0, nl + 1):
+        to_gpu(o, g, ["FLUID", "DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y"], level=lvl)
+    to_gpu(o, g, ["RESIDUAL", "PRESSURE"])
+    o.stage("solve_pcg")
+    g.solve_pcg()
+    so, sg = o.stats(), g.stats()
+    assert abs(int(so["pcg_iterations"]) - int(sg["pcg_iterations"])) <= 1, (so, sg)
+    err = rel_l2(g.read("PRESSURE"), o.read("PRESSURE"))
+    assert err <= 1e-4, err
+
+
+def test_pressure_gradient_bit_exact(pair):
+    o, g, s = pair
+    oracle_until(o, "apply_pressure_gradient", s["dt"])
+    to_gpu(o, g, ["PRESSURE", "SMOOTHED_DENSITY", "FLUID", "MAC_U", "MAC_V", "VALID_U", "VALID_V"])
+    o.stage("apply_pressure_gradient")
+    g.apply_pressure_gradient()
+    for f in ("MAC_U", "MAC_V"):
+        assert bit_equal(g.read(f), o.read(f)), (f, first_mismatch(g.read(f), o.read(f)))
+    for f in ("VALID_U", "VALID_V"):
+        assert np.array_equal(g.read(f), o.read(f)), f
+
+
+def test_extrapolate_bit_exact(pair):
+    o, g, s = pair
+    oracle_until(o, "extrapolate_velocity", s["dt"])
+    to_gpu(o, g, ["MAC_U", "MAC_V", "VALID_U", "VALID_V"])
+    o.stage("extrapolate_velocity")
+    g.extrapolate_velocity()
+    for f in ("MAC_U", "MAC_V"):
+        assert bit_equal(g.read(f), o.read(f)), (f, first_mismatch(g.read(f), o.read(f)))
+    for f in ("VALID_U", "VALID_V"):
+        assert np.array_equal(g.read(f), o.read(f)), f
+
+
+def test_g2p_bit_exact(pair):
+    o, g, s = pair
+    oracle_until(o, "transfer_grid_to_particles_and_advect", s["dt"])
+    copy_state(o, g)
+    to_gpu(o, g, ["MAC_U", "MAC_V"])
+    o.stage("transfer_grid_to_particles_and_advect", s["dt"])
+    g.transfer_grid_to_particles_and_advect(s["dt"])
+    for f in PARTICLE_FIELDS:
+        assert bit_equal(g.read(f), o.read(f)), (f, first_mismatch(g.read(f), o.read(f)))
+    assert g.get_max_particle_velocity() == np.float32(o.max_particle_velocity())
