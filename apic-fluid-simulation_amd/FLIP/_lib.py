@@ -29,8 +29,10 @@ SYMBOLS = [
     "apic_update_spatial_lookup", "apic_transfer_particles_to_grid", "apic_apply_external_forces",
     "apic_build_pressure_system", "apic_update_multilevel_grid", "apic_apply_preconditioner",
     "apic_solve_pcg", "apic_apply_pressure_gradient", "apic_extrapolate_velocity",
-    "apic_transfer_grid_to_particles_and_advect",
+    "apic_transfer_grid_to_particles_and_advect", "apic_set_profiling", "apic_get_timers",
 ]
+TIMERS = ["sort", "p2g", "pressure_build", "mg_build", "pcg", "gradient", "extrapolate", "g2p",
+          "kernel_p2g", "kernel_g2p"]
 
 
 class WorldPropertiesC(ctypes.Structure):
@@ -97,6 +99,8 @@ def lib():
     L.apic_transfer_particles_to_grid.argtypes = [vp, f32, u32]
     L.apic_apply_external_forces.argtypes = [vp, f32]
     L.apic_transfer_grid_to_particles_and_advect.argtypes = [vp, f32]
+    L.apic_set_profiling.argtypes = [vp, ctypes.c_int]
+    L.apic_get_timers.argtypes = [vp, ctypes.POINTER(ctypes.c_double), u32p, u32]
     for name in SYMBOLS:
         fn = getattr(L, name)
         if name not in ("apic_destroy", "apic_last_error"):

@@ -100,6 +100,18 @@ class Apic:
         check(lib().apic_get_stats(self._h, ctypes.byref(s)))
         return {k: getattr(s, k) for k, _ in _lib.StepStatsC._fields_}
 
+    def set_profiling(self, enable=True):
+        """Record HIP events around every stage (extension; resets the timers)."""
+        check(lib().apic_set_profiling(self._h, int(bool(enable))))
+
+    def timers(self):
+        """{timer: (total ms, count)} accumulated since profiling was enabled."""
+        n = len(_lib.TIMERS)
+        ms = (ctypes.c_double * n)()
+        cnt = (ctypes.c_uint32 * n)()
+        check(lib().apic_get_timers(self._h, ms, cnt, n))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(_lib.TIMERS)}
+
     # ---- pub fields --------------------------------------------------------
     def dims(self):
         v = [ctypes.c_uint32() for _ in range(5)]

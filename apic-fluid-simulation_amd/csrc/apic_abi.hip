@@ -68,6 +68,9 @@ apic_status dalloc(Sim& s, T*& p, size_t count) {
 
 void release(Sim& s) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
+    for (int t = 0; t < APIC_TIMER_COUNT; ++t)
+        for (int k = 0; k < 2; ++k)
+            if (s.ev[t][k]) { (void)hipEventDestroy(s.ev[t][k]); s.ev[t][k] = nullptr; }
     for (void* p : s.allocations) (void)hipFree(p);
     s.allocations.clear();
     if (s.h_pcg) (void)hipHostFree(s.h_pcg);
@@ -163,10 +166,10 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     size_t pcs = s.padded(C.w, C.h);
     ALLOC(s.cg_s, pcs); ALLOC(s.cg_ap, pcs); ALLOC(s.cg_r, pcs);
     ALLOC(s.pcg, 1);
-    s.partials_cap = (nc + 255) / 256 + 64;
-    double* partials = nullptr;
-    ALLOC(partials, s.partials_cap);
-    s.partials = partials;
+    ALLOC(s.part_gamma, kRedBlocks); ALLOC(s.part_delta, kRedBlocks); ALLOC(s.part_max, kRedBlocks);
+    ALLOC(s.part_fluid, 2 * kRedBlocks);
+    s.part_maxv_cap = 1024;
+    ALLOC(s.part_maxv, s.part_maxv_cap);
     ALLOC(s.fstats, 1);
     ALLOC(s.maxv_bits, 1);
     HIP_TRY(hipHostMalloc((void**)&s.h_pcg, sizeof(PcgState), hipHostMallocDefault));
@@ -226,56 +229,73 @@ apic_status do_sort(Sim& s) {
     return APIC_OK;
 }
 
-// solve_pcg (src/apic.rs:1865-1956) with the reference's rules.
+// solve_pcg (src/apic.rs:1865-1956) with the reference's rules.  Iterations
+// are queued in batches; the device-side state machine turns the kernels of
+// a finished solve into no-ops, and the host reads the state only at the end
+// of a batch.  The first batch is as long as the previous solve.
 apic_status do_pcg(Sim& s, bool zero_pressure) {
     launch_pcg_reset(s);
     if (zero_pressure) launch_zero_pressure(s);
     launch_vcycle(s, 1);
     CHECK_LAUNCH();
     int k = 0;
-    int next_check = std::max(1, s.pcg_batch);
-    bool done = false;
-    while (k < 100 && !done) {
-        launch_pcg_iteration_head(s, k);
-        launch_vcycle(s, 2);
-        CHECK_LAUNCH();
-        ++k;
-        if (k == next_check || k == 100) {
-            HIP_TRY(hipMemcpyAsync(s.h_pcg, s.pcg, sizeof(PcgState), hipMemcpyDeviceToHost, s.stream));
-            HIP_TRY(hipStreamSynchronize(s.stream));
-            done = s.h_pcg->done != 0;
-            next_check = k + 2;
+    int check_at = std::max(0, s.pcg_batch);
+    while (true) {
+        while (k < check_at && k < kMaxPcgIterations) {
+            launch_pcg_iteration_head(s, k);
+            launch_vcycle(s, 2);
+            ++k;
         }
-    }
-    if (!done) {
+        launch_pcg_decide(s, k);
+        CHECK_LAUNCH();
         HIP_TRY(hipMemcpyAsync(s.h_pcg, s.pcg, sizeof(PcgState), hipMemcpyDeviceToHost, s.stream));
         HIP_TRY(hipStreamSynchronize(s.stream));
+        if (s.h_pcg->done || k >= kMaxPcgIterations) break;
+        check_at = k + 2;
     }
     s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
     s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
     s.stats.pcg_max_residual = s.h_pcg->maxerr;
     s.stats.pcg_rel_residual = -1.0;
-    // predict the next solve's length so that the first check lands on it
-    s.pcg_batch = std::max(1, s.h_pcg->iters);
+    s.pcg_batch = s.h_pcg->iters;
     return APIC_OK;
 }
 
 apic_status substep(Sim& s, bool sort, float dt) {
     apic_status st;
-    if (sort && (st = do_sort(s)) != APIC_OK) return st;
+    if (sort) {
+        s.mark(APIC_T_SORT, 0);
+        if ((st = do_sort(s)) != APIC_OK) return st;
+        s.mark(APIC_T_SORT, 1);
+    }
+    s.mark(APIC_T_P2G, 0);
     launch_p2g(s);
     launch_normalize_forces(s, true, true, dt);
+    s.mark(APIC_T_P2G, 1);
+    s.mark(APIC_T_PRESSURE_BUILD, 0);
     launch_build_pressure(s, true);
+    s.mark(APIC_T_PRESSURE_BUILD, 1);
+    s.mark(APIC_T_MG_BUILD, 0);
     launch_mg_build(s);
+    s.mark(APIC_T_MG_BUILD, 1);
     CHECK_LAUNCH();
+    s.mark(APIC_T_PCG, 0);
     if ((st = do_pcg(s, false)) != APIC_OK) return st;
+    s.mark(APIC_T_PCG, 1);
+    s.mark(APIC_T_GRADIENT, 0);
     launch_gradient(s, true, dt);
+    s.mark(APIC_T_GRADIENT, 1);
+    s.mark(APIC_T_EXTRAPOLATE, 0);
     launch_extrapolate(s);
+    s.mark(APIC_T_EXTRAPOLATE, 1);
+    s.mark(APIC_T_G2P, 0);
     launch_g2p(s, dt, true);
+    s.mark(APIC_T_G2P, 1);
     CHECK_LAUNCH();
     HIP_TRY(hipMemcpyAsync(s.h_maxv, s.maxv_bits, 4, hipMemcpyDeviceToHost, s.stream));
     s.keys_valid = true;
     s.maxv_valid = true;
+    s.collect_timers();
     return APIC_OK;
 }
 
@@ -437,6 +457,30 @@ apic_status apic_set_solver_config(apic_handle* h, const apic_solver_config* cfg
     if (!h || !cfg) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
     if (cfg->mode != 0) return fail(APIC_ERR_INVALID_ARGUMENT, "only the reference solver rules (mode 0) are built");
     h->sim.solver = *cfg;
+    return APIC_OK;
+}
+
+apic_status apic_set_profiling(apic_handle* h, int enable) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    Sim& s = h->sim;
+    if (enable && !s.ev[0][0]) {
+        for (int t = 0; t < APIC_TIMER_COUNT; ++t)
+            for (int k = 0; k < 2; ++k) HIP_TRY(hipEventCreate(&s.ev[t][k]));
+    }
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    s.profiling = enable != 0;
+    for (int t = 0; t < APIC_TIMER_COUNT; ++t) { s.timer_ms[t] = 0.0; s.timer_count[t] = 0; s.ev_used[t] = false; }
+    return APIC_OK;
+}
+
+apic_status apic_get_timers(apic_handle* h, double* ms, uint32_t* counts, uint32_t n) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    if (n > APIC_TIMER_COUNT) return fail(APIC_ERR_INVALID_ARGUMENT, "n exceeds APIC_TIMER_COUNT");
+    h->sim.collect_timers();
+    for (uint32_t t = 0; t < n; ++t) {
+        if (ms) ms[t] = h->sim.timer_ms[t];
+        if (counts) counts[t] = h->sim.timer_count[t];
+    }
     return APIC_OK;
 }
 

@@ -33,18 +33,20 @@ struct LevelView {
     uint8_t* fluid;  // raw index
 };
 
-// Reductions finish in the last block to arrive (ticket counter), which sums
-// the per-block partials in block order, so results are run-to-run stable.
+// Scalars of one pressure solve.  The reductions write per-workgroup
+// partials; the kernel that needs a reduced scalar sums the partials itself,
+// in the same fixed order in every workgroup, and applies the reference's exit
+// test.  Workgroup 0 records the decision here for later kernels and the host.
+constexpr int kRedBlocks = 512;          // workgroups (and partials) of every solver reduction
+constexpr int kMaxPcgIterations = 100;   // solve_pcg's cap (src/apic.rs:1894)
 struct PcgState {
-    float gamma, delta, alpha, beta, maxerr;
-    int32_t done;        // nonzero: every kernel of the PCG loop returns at once
-    int32_t iters;
-    int32_t exit_reason;
-    int32_t first;       // 1 until the first search-direction update (s = z)
-    uint32_t ticket[4];  // last-block counters, reset by the block that finishes
-    double dgamma, ddelta, bnorm2, rnorm2;  // relative-mode scalars (FP64)
-    float maxv;          // unused padding for alignment
-    int32_t pad;
+    int32_t done;         // nonzero: every kernel of the PCG loop returns at once
+    int32_t iters;        // iterations entered
+    int32_t exit_reason;  // as apic_step_stats.pcg_exit_reason
+    int32_t pad0;
+    float maxerr, delta, alpha, pad1;
+    float gamma[kMaxPcgIterations + 4];  // gamma[k]: the r.z used by iteration k
+    double bnorm2, rnorm2;               // relative mode (FP64)
 };
 
 // Base-level fluid statistics needed by the padded-list quirk of prolongate
@@ -85,6 +87,7 @@ void launch_mg_build(Sim& sim);
 void launch_vcycle(Sim& sim, int mode);  // mode 0: plain, 1: PCG start (gamma), 2: PCG iteration (gamma')
 void launch_pcg_reset(Sim& sim);
 void launch_pcg_iteration_head(Sim& sim, int k);  // s update + A s + delta, then p/r update + max|r|
+void launch_pcg_decide(Sim& sim, int k);          // gamma test after k iterations
 void launch_zero_pressure(Sim& sim);
 void launch_fluid_stats(Sim& sim);
 
@@ -134,8 +137,12 @@ struct Sim {
     float* cg_s = nullptr; float* cg_ap = nullptr; float* cg_r = nullptr;  // coarsest level
     float* rhs = nullptr;            // copy of b for the relative mode's residual recompute
     PcgState* pcg = nullptr;
-    void* partials = nullptr;        // per-block reduction partials (double-sized)
-    size_t partials_cap = 0;
+    float* part_gamma = nullptr;     // kRedBlocks partials of r.z
+    float* part_delta = nullptr;     // kRedBlocks partials of s.As
+    float* part_max = nullptr;       // kRedBlocks partials of max |r|
+    uint32_t* part_fluid = nullptr;  // kRedBlocks (count, last+1) pairs of the base fluid cells
+    float* part_maxv = nullptr;      // per-workgroup max |v| of the G2P kernel
+    size_t part_maxv_cap = 0;
     FluidStats* fstats = nullptr;
     uint32_t* maxv_bits = nullptr;   // max |v| over all lanes (float bits)
     bool maxv_valid = false;
@@ -147,6 +154,31 @@ struct Sim {
 
     apic_step_stats stats{};
     int pcg_batch = 4;
+
+    // stage timers (HIP events on `stream`)
+    bool profiling = false;
+    hipEvent_t ev[APIC_TIMER_COUNT][2] = {};
+    bool ev_used[APIC_TIMER_COUNT] = {};
+    double timer_ms[APIC_TIMER_COUNT] = {};
+    uint32_t timer_count[APIC_TIMER_COUNT] = {};
+    void mark(int t, int which) {
+        if (!profiling) return;
+        (void)hipEventRecord(ev[t][which], stream);
+        if (which == 1) ev_used[t] = true;
+    }
+    void collect_timers() {
+        if (!profiling) return;
+        (void)hipStreamSynchronize(stream);
+        for (int t = 0; t < APIC_TIMER_COUNT; ++t) {
+            if (!ev_used[t]) continue;
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, ev[t][0], ev[t][1]) == hipSuccess) {
+                timer_ms[t] += ms;
+                timer_count[t] += 1;
+            }
+            ev_used[t] = false;
+        }
+    }
 
     // device allocations to free
     std::vector<void*> allocations;

@@ -18,11 +18,12 @@
 //    spmv_dot_kernel       s = z + beta*s fused with A s and s.As
 //    axpy_max_kernel       p += alpha s, r -= alpha As, max |r|
 //    the base level's final red sweep also computes r.z (gamma)
-//  Scalars live in a PcgState in device memory.  Each reduction finishes in
-//  the last workgroup to arrive, which sums the per-workgroup partials in a
-//  fixed order, decides the reference's exit tests and sets `done`; every
-//  kernel of the loop returns at once when `done` is set, so the host can
-//  queue several iterations between synchronisations.
+//  Scalars live in a PcgState in device memory.  A reduction kernel writes
+//  one partial per workgroup (plain stores, no device-wide fence or atomic);
+//  the next kernel of the chain sums those partials itself, in the same fixed
+//  order in every workgroup, applies the reference's exit test and skips its
+//  work when the test ends the solve.  Every kernel of the loop returns at once
+//  when `done` is set, so the host queues several iterations between reads.
 //
 // Every stencil keeps the reference's operation order and its fused
 // multiply-adds (mul_add -> fmaf); the library is compiled with
@@ -41,7 +42,6 @@ constexpr float kOmega = 1.2f;
 constexpr int kThreads = 256;
 constexpr int kBottomThreads = 1024;
 constexpr int kSerialCap = 4096;
-constexpr int kMaxIterations = 100;
 
 __device__ inline float atomic_load_f(const float* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -65,19 +65,40 @@ __global__ void smooth_density_kernel(const float* __restrict__ in, float* __res
     out[c] = sum / weight;
 }
 
-__global__ void assemble_kernel(const float* __restrict__ mac_u, const float* __restrict__ mac_v,
-                                const float* __restrict__ smoothed, const uint8_t* __restrict__ fluid,
-                                const uint8_t* __restrict__ obstacle, uint32_t W, uint32_t cells, float cell,
-                                LevelView L, float* __restrict__ pressure, int zero_pressure, FluidStats* fstats) {
-    __shared__ uint32_t s_cnt, s_last;
-    if (threadIdx.x == 0) { s_cnt = 0; s_last = 0; }
+// Per-workgroup (count, last raw index + 1) of the base fluid cells, reduced
+// by fstats_finalize_kernel: the padded-list quirk of prolongate needs both.
+__device__ inline void fluid_partial(bool fl, uint32_t c, uint32_t& cnt, uint32_t& last) {
+    if (fl) { cnt += 1; last = max(last, c + 1); }
+}
+
+__device__ inline void write_fluid_partial(uint32_t cnt, uint32_t last, uint32_t* part) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        cnt += __shfl_xor(cnt, o, 64);
+        last = max(last, (uint32_t)__shfl_xor(last, o, 64));
+    }
+    __shared__ uint32_t s_cnt[kThreads / 64], s_last[kThreads / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) { s_cnt[wave] = cnt; s_last[wave] = last; }
     __syncthreads();
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < cells) {
+    if (threadIdx.x == 0) {
+        uint32_t c = 0, l = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { c += s_cnt[w]; l = max(l, s_last[w]); }
+        part[2 * blockIdx.x] = c;
+        part[2 * blockIdx.x + 1] = l;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void assemble_kernel(
+    const float* __restrict__ mac_u, const float* __restrict__ mac_v, const float* __restrict__ smoothed,
+    const uint8_t* __restrict__ fluid, const uint8_t* __restrict__ obstacle, uint32_t W, uint32_t cells, float cell,
+    LevelView L, float* __restrict__ pressure, int zero_pressure, uint32_t* __restrict__ part_fluid) {
+    uint32_t cnt = 0, last = 0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
         size_t pc = (size_t)c + W;
-        if (fluid[c]) {
-            atomicAdd(&s_cnt, 1u);
-            atomicMax(&s_last, c + 1);
+        bool fl = fluid[c];
+        fluid_partial(fl, c, cnt, last);
+        if (fl) {
             uint32_t gx = c % W, gy = c / W;
             size_t ui = (size_t)gy * (W + 1) + gx;
             float ul = mac_u[ui], ur = mac_u[ui + 1], vb = mac_v[c], vt = mac_v[c + W];
@@ -87,9 +108,9 @@ __global__ void assemble_kernel(const float* __restrict__ mac_u, const float* __
             float rc = smoothed[c];
             const float target = 9.0f, noise = target * 1.15f, rate = 0.1f;
             float excess = fminf(fmaxf(rc - noise, 0.0f), target);
-            int fl = fluid[c - 1], fr = fluid[c + 1], fb = fluid[c - W], ft = fluid[c + W];
+            int fl_ = fluid[c - 1], fr = fluid[c + 1], fb = fluid[c - W], ft = fluid[c + W];
             float diag = 0.0f, px = 0.0f, py = 0.0f;
-            if (nol == 1) diag += fl ? 1.0f : 1.0f / theta(smoothed[c - 1], rc);
+            if (nol == 1) diag += fl_ ? 1.0f : 1.0f / theta(smoothed[c - 1], rc);
             if (nor == 1) {
                 if (fr) { diag += 1.0f; px = -1.0f; }
                 else diag += 1.0f / theta(smoothed[c + 1], rc);
@@ -113,25 +134,35 @@ __global__ void assemble_kernel(const float* __restrict__ mac_u, const float* __
             L.inv_diag[pc] = 0.0f;
         }
     }
+    write_fluid_partial(cnt, last, part_fluid);
+}
+
+__global__ __launch_bounds__(kThreads) void fluid_stats_kernel(const uint8_t* __restrict__ fluid, uint32_t cells,
+                                                                uint32_t* __restrict__ part_fluid) {
+    uint32_t cnt = 0, last = 0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x)
+        fluid_partial(fluid[c] != 0, c, cnt, last);
+    write_fluid_partial(cnt, last, part_fluid);
+}
+
+__global__ __launch_bounds__(kThreads) void fstats_finalize_kernel(const uint32_t* __restrict__ part_fluid,
+                                                                    FluidStats* fstats) {
+    uint32_t cnt = 0, last = 0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) {
+        cnt += part_fluid[2 * b];
+        last = max(last, part_fluid[2 * b + 1]);
+    }
+    __shared__ uint32_t s_cnt[kThreads], s_last[kThreads];
+    s_cnt[threadIdx.x] = cnt;
+    s_last[threadIdx.x] = last;
     __syncthreads();
-    if (threadIdx.x == 0 && s_cnt) {
-        atomicAdd(&fstats->count, s_cnt);
-        // last_raw holds raw+1 while accumulating (0 = none); finalized on read
-        atomicMax(&fstats->last_raw, s_last);
+    if (threadIdx.x == 0) {
+        uint32_t c = 0, l = 0;
+        for (int t = 0; t < (int)blockDim.x; ++t) { c += s_cnt[t]; l = max(l, s_last[t]); }
+        fstats->count = c;
+        fstats->last_raw = l;
     }
 }
-
-__global__ void fluid_stats_kernel(const uint8_t* __restrict__ fluid, uint32_t cells, FluidStats* fstats) {
-    __shared__ uint32_t s_cnt, s_last;
-    if (threadIdx.x == 0) { s_cnt = 0; s_last = 0; }
-    __syncthreads();
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < cells && fluid[c]) { atomicAdd(&s_cnt, 1u); atomicMax(&s_last, c + 1); }
-    __syncthreads();
-    if (threadIdx.x == 0 && s_cnt) { atomicAdd(&fstats->count, s_cnt); atomicMax(&fstats->last_raw, s_last); }
-}
-
-__global__ void reset_fstats_kernel(FluidStats* fstats) { fstats->count = 0; fstats->last_raw = 0; }
 
 __device__ inline void mg_build_cell(const LevelView& F, const LevelView& C, uint32_t c) {
     uint32_t cx = c % C.w, cy = c / C.w;
@@ -295,10 +326,27 @@ __device__ inline bool colour_cell(const LevelView& L, uint32_t t, int color, ui
     return true;
 }
 
-__global__ void red_zero_kernel(LevelView L, const int32_t* __restrict__ done) {
-    if (*done) return;
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < L.cells) red_zero_cell(L, c);
+__device__ inline float block_max(float v, float* scratch);
+
+// Zero e and the first red sweep.  In a PCG iteration this is the first kernel
+// after the residual update, so it also applies the residual test
+// (max |r| < solve_error, src/apic.rs:1930) from the axpy partials.
+__global__ __launch_bounds__(256) void red_zero_kernel(LevelView L, PcgState* st, const float* __restrict__ part_max,
+                                                       int check, float solve_error) {
+    __shared__ float scratch[256 / 64];
+    if (st->done) return;
+    if (check) {
+        float m = 0.0f;
+        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) m = fast_max(m, part_max[i]);
+        m = block_max(m, scratch);
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->maxerr = m;
+        if (m < solve_error) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 3; }
+            return;
+        }
+    }
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x)
+        red_zero_cell(L, c);
 }
 
 __global__ void sweep_kernel(LevelView L, Prolong P, int color, int mode, const int32_t* __restrict__ done) {
@@ -340,68 +388,38 @@ __device__ inline float block_max(float v, float* scratch) {
     return s;
 }
 
-// Last-workgroup finish: returns true in the workgroup that arrives last,
-// after which the partials of every workgroup are visible.
-__device__ inline bool arrive_last(uint32_t* ticket) {
-    __shared__ bool s_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        uint32_t t = atomicAdd(ticket, 1u);
-        s_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (s_last) __threadfence();
-    return s_last;
-}
-
-// Sum of partials[0..n) in a fixed order by one workgroup.
-__device__ inline float sum_partials(const float* partials, uint32_t n, float* scratch) {
+// Sum (or max) of the kRedBlocks partials of one reduction.  Every workgroup
+// that calls it uses kThreads threads and the same order, so all of them see
+// the same value.
+__device__ inline float sum_partials(const float* __restrict__ part, float* scratch) {
     float v = 0.0f;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) v += atomic_load_f(&partials[i]);
+    for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) v += part[i];
     return block_sum(v, scratch);
 }
-__device__ inline float max_partials(const float* partials, uint32_t n, float* scratch) {
+__device__ inline float max_partials(const float* __restrict__ part, float* scratch) {
     float v = 0.0f;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) v = fast_max(v, atomic_load_f(&partials[i]));
+    for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) v = fast_max(v, part[i]);
     return block_max(v, scratch);
 }
 
 // Base level, up sweep of the second colour (red), fused with r.z over the
-// whole padded interior (dot_product over [W, W+W*H), src/apic.rs:1888,1936).
-// mode 0: V-cycle only; 1: PCG start; 2: PCG iteration.
+// whole padded interior (dot_product over [W, W+W*H), src/apic.rs:1888,1936);
+// non-fluid cells have e = 0, so they add nothing.  mode 0: V-cycle alone.
 __global__ __launch_bounds__(kThreads) void base_red_up_dot_kernel(LevelView L, Prolong P, int mode,
-                                                                    PcgState* st, float* __restrict__ partials) {
+                                                                    const int32_t* __restrict__ done,
+                                                                    float* __restrict__ part_gamma) {
     __shared__ float scratch[kThreads / 64];
-    if (st->done) return;
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (*done) return;
     float prod = 0.0f;
-    if (c < L.cells) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
         uint32_t x = c % L.w, y = c / L.w;
         if (((x + y) & 1u) == 0) sor_cell(L, P, c, 2);
         size_t pc = (size_t)c + L.w;
-        prod = L.r[pc] * L.e[pc];
+        prod += L.r[pc] * L.e[pc];
     }
     if (mode == 0) return;
     float bs = block_sum(prod, scratch);
-    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
-    if (!arrive_last(&st->ticket[2])) return;
-    float g = sum_partials(partials, gridDim.x, scratch);
-    if (threadIdx.x == 0) {
-        st->ticket[2] = 0;
-        if (mode == 1) {
-            if (g < kSafety) { st->done = 1; st->exit_reason = 1; }
-            else { st->gamma = g; st->first = 1; }
-        } else {
-            if (g < kSafety) { st->done = 1; st->exit_reason = 4; }
-            else {
-                st->beta = g / st->gamma;
-                st->gamma = g;
-                st->first = 0;
-                if (st->iters >= kMaxIterations) { st->done = 1; st->exit_reason = 5; }
-            }
-        }
-    }
+    if (threadIdx.x == 0) part_gamma[blockIdx.x] = bs;
 }
 
 // ---- the one-workgroup bottom of the V-cycle
@@ -548,13 +566,11 @@ __global__ __launch_bounds__(kBottomThreads) void bottom_kernel(LevelSet ls, int
 // ---------------------------------------------------------------------- PCG
 
 __global__ void pcg_reset_kernel(PcgState* st) {
-    st->gamma = st->delta = st->alpha = st->beta = st->maxerr = 0.0f;
     st->done = 0;
     st->iters = 0;
     st->exit_reason = 0;
-    st->first = 1;
-    for (int k = 0; k < 4; ++k) st->ticket[k] = 0;
-    st->dgamma = st->ddelta = st->bnorm2 = st->rnorm2 = 0.0;
+    st->maxerr = st->delta = st->alpha = 0.0f;
+    st->bnorm2 = st->rnorm2 = 0.0;
 }
 
 __global__ void zero_pressure_kernel(const uint8_t* __restrict__ fluid, float* __restrict__ pressure,
@@ -563,19 +579,38 @@ __global__ void zero_pressure_kernel(const uint8_t* __restrict__ fluid, float* _
     if (c < cells && fluid[c]) pressure[c] = 0.0f;
 }
 
-// s = z + beta*s (s = z on the first iteration), As = A s, delta = s.As
-// (apply_matrix_and_dot, src/apic.rs:1823-1863; search update :1945-1954)
+// Exit tests on gamma = r.z before iteration k (src/apic.rs:1890, 1938) and
+// the iteration cap.  Returns true when the solve ends here.
+__device__ inline bool gamma_exit(PcgState* st, float g, int k) {
+    if (g < kSafety) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
+        return true;
+    }
+    if (k >= kMaxPcgIterations) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 5; st->iters = k; }
+        return true;
+    }
+    return false;
+}
+
+// Iteration k: gamma from the V-cycle's partials, s = z + beta*s (s = z when
+// k = 0), As = A s and the partials of delta = s.As (apply_matrix_and_dot,
+// src/apic.rs:1823-1863; search update :1945-1954).
 __global__ __launch_bounds__(kThreads) void spmv_dot_kernel(LevelView L, const float* __restrict__ s_old,
                                                              float* __restrict__ s_new, float* __restrict__ As,
-                                                             PcgState* st, float* __restrict__ partials) {
+                                                             PcgState* st, const float* __restrict__ part_gamma,
+                                                             float* __restrict__ part_delta, int k) {
     __shared__ float scratch[kThreads / 64];
     if (st->done) return;
-    const int first = st->first;
-    const float beta = st->beta;
+    const float g = sum_partials(part_gamma, scratch);
+    if (gamma_exit(st, g, k)) return;
+    const bool first = k == 0;
+    const float beta = first ? 0.0f : g / st->gamma[k - 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st->gamma[k] = g; st->iters = k + 1; }
     const size_t w = L.w;
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     float prod = 0.0f;
-    if (c < L.cells && L.fluid[c]) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+        if (!L.fluid[c]) continue;
         size_t pc = (size_t)c + w;
         auto sv = [&](size_t i) { return first ? L.e[i] : L.e[i] + (beta * s_old[i]); };
         float s0 = sv(pc);
@@ -586,47 +621,48 @@ __global__ __launch_bounds__(kThreads) void spmv_dot_kernel(LevelView L, const f
         v = fmaf(L.py[pc - w], sv(pc - w), v);
         s_new[pc] = s0;
         As[pc] = v;
-        prod = v * s0;
+        prod += v * s0;
     }
     float bs = block_sum(prod, scratch);
-    if (threadIdx.x == 0) partials[blockIdx.x] = bs;
-    if (!arrive_last(&st->ticket[0])) return;
-    float d = sum_partials(partials, gridDim.x, scratch);
-    if (threadIdx.x == 0) {
-        st->ticket[0] = 0;
-        st->iters += 1;
-        st->delta = d;
-        if (fabsf(d) < kSafety) { st->done = 1; st->exit_reason = 2; }
-        else st->alpha = st->gamma / d;
-    }
+    if (threadIdx.x == 0) part_delta[blockIdx.x] = bs;
 }
 
-// p += alpha*s; r = fma(-alpha, As, r); max |r| (src/apic.rs:1907-1932)
+// delta from the partials (exit when |delta| < 1e-5, src/apic.rs:1903), then
+// p += alpha*s; r = fma(-alpha, As, r); partials of max |r| (:1907-1928)
 __global__ __launch_bounds__(kThreads) void axpy_max_kernel(LevelView L, const float* __restrict__ s,
                                                              const float* __restrict__ As,
-                                                             float* __restrict__ pressure, float solve_error,
-                                                             PcgState* st, float* __restrict__ partials) {
+                                                             float* __restrict__ pressure, PcgState* st,
+                                                             const float* __restrict__ part_delta,
+                                                             float* __restrict__ part_max, int k) {
     __shared__ float scratch[kThreads / 64];
     if (st->done) return;
-    const float alpha = st->alpha;
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    const float d = sum_partials(part_delta, scratch);
+    if (fabsf(d) < kSafety) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 2; st->delta = d; }
+        return;
+    }
+    const float alpha = st->gamma[k] / d;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { st->delta = d; st->alpha = alpha; }
     float m = 0.0f;
-    if (c < L.cells && L.fluid[c]) {
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+        if (!L.fluid[c]) continue;
         size_t pc = (size_t)c + L.w;
         pressure[c] += alpha * s[pc];
         float nr = fmaf(-alpha, As[pc], L.r[pc]);
         L.r[pc] = nr;
-        m = fabsf(nr);
+        m = fast_max(m, fabsf(nr));
     }
     float bm = block_max(m, scratch);
-    if (threadIdx.x == 0) partials[blockIdx.x] = bm;
-    if (!arrive_last(&st->ticket[1])) return;
-    float mx = max_partials(partials, gridDim.x, scratch);
-    if (threadIdx.x == 0) {
-        st->ticket[1] = 0;
-        st->maxerr = mx;
-        if (mx < solve_error) { st->done = 1; st->exit_reason = 3; }
-    }
+    if (threadIdx.x == 0) part_max[blockIdx.x] = bm;
+}
+
+// The gamma test after k iterations, run at the host's check points so that
+// the recorded state is final when the host reads it.
+__global__ __launch_bounds__(kThreads) void decide_kernel(PcgState* st, const float* __restrict__ part_gamma, int k) {
+    __shared__ float scratch[kThreads / 64];
+    if (st->done) return;
+    const float g = sum_partials(part_gamma, scratch);
+    (void)gamma_exit(st, g, k);
 }
 
 unsigned blocks_for(size_t n, unsigned threads = kThreads) { return (unsigned)((n + threads - 1) / threads); }
@@ -651,24 +687,24 @@ Prolong base_prolong(const Sim& sim) {
 void launch_build_pressure(Sim& sim, bool zero_pressure) {
     const uint32_t cells = sim.cells;
     const uint8_t* fluid = sim.levels[0].fluid;
-    hipLaunchKernelGGL(reset_fstats_kernel, dim3(1), dim3(1), 0, sim.stream, sim.fstats);
-    // three smoothing passes: density -> b -> a -> b ... ending in smooth_a
+    // three smoothing passes: density -> a -> b -> a, so the result ends in smooth_a
     hipLaunchKernelGGL(smooth_density_kernel, dim3(blocks_for(cells)), dim3(kThreads), 0, sim.stream, sim.density,
                        sim.smooth_a, fluid, sim.W, cells);
     hipLaunchKernelGGL(smooth_density_kernel, dim3(blocks_for(cells)), dim3(kThreads), 0, sim.stream, sim.smooth_a,
                        sim.smooth_b, fluid, sim.W, cells);
     hipLaunchKernelGGL(smooth_density_kernel, dim3(blocks_for(cells)), dim3(kThreads), 0, sim.stream, sim.smooth_b,
                        sim.smooth_a, fluid, sim.W, cells);
-    hipLaunchKernelGGL(assemble_kernel, dim3(blocks_for(cells)), dim3(kThreads), 0, sim.stream, sim.mac_u, sim.mac_v,
+    hipLaunchKernelGGL(assemble_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, sim.mac_u, sim.mac_v,
                        sim.smooth_a, fluid, sim.obstacle, sim.W, cells, sim.cell, sim.levels[0], sim.pressure,
-                       (int)zero_pressure, sim.fstats);
+                       (int)zero_pressure, sim.part_fluid);
+    hipLaunchKernelGGL(fstats_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.part_fluid, sim.fstats);
     (void)hipMemcpyAsync(sim.h_fstats, sim.fstats, sizeof(FluidStats), hipMemcpyDeviceToHost, sim.stream);
 }
 
 void launch_fluid_stats(Sim& sim) {
-    hipLaunchKernelGGL(reset_fstats_kernel, dim3(1), dim3(1), 0, sim.stream, sim.fstats);
-    hipLaunchKernelGGL(fluid_stats_kernel, dim3(blocks_for(sim.cells)), dim3(kThreads), 0, sim.stream,
-                       sim.levels[0].fluid, sim.cells, sim.fstats);
+    hipLaunchKernelGGL(fluid_stats_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, sim.levels[0].fluid,
+                       sim.cells, sim.part_fluid);
+    hipLaunchKernelGGL(fstats_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.part_fluid, sim.fstats);
     (void)hipMemcpyAsync(sim.h_fstats, sim.fstats, sizeof(FluidStats), hipMemcpyDeviceToHost, sim.stream);
 }
 
@@ -689,14 +725,17 @@ void launch_zero_pressure(Sim& sim) {
                        sim.levels[0].fluid, sim.pressure, sim.cells);
 }
 
-// mode 0: apply_preconditioner alone; 1: PCG start (gamma); 2: PCG iteration (gamma')
+// mode 0: apply_preconditioner alone; 1: PCG start (gamma); 2: PCG iteration
+// (residual test in the first kernel, gamma' partials in the last).
 void launch_vcycle(Sim& sim, int mode) {
     const int32_t* done = &sim.pcg->done;
     const int n = (int)sim.levels.size();
     const LevelView& B = sim.levels[0];
     hipStream_t s = sim.stream;
+    auto capped = [](size_t cells) { return std::min<unsigned>(blocks_for(cells), 2048u); };
     // down: base
-    hipLaunchKernelGGL(red_zero_kernel, dim3(blocks_for(B.cells)), dim3(kThreads), 0, s, B, done);
+    hipLaunchKernelGGL(red_zero_kernel, dim3(capped(B.cells)), dim3(256), 0, s, B, sim.pcg, sim.part_max,
+                       (int)(mode == 2), sim.props.solve_error);
     Prolong none{};
     auto half_blocks = [](const LevelView& L) { return blocks_for((size_t)((L.w + 1) / 2) * L.h); };
     hipLaunchKernelGGL(sweep_kernel, dim3(half_blocks(B)), dim3(kThreads), 0, s, B, none, 1, 0, done);
@@ -724,19 +763,22 @@ void launch_vcycle(Sim& sim, int mode) {
     }
     Prolong P = base_prolong(sim);
     hipLaunchKernelGGL(sweep_kernel, dim3(half_blocks(B)), dim3(kThreads), 0, s, B, P, 1, 1, done);
-    hipLaunchKernelGGL(base_red_up_dot_kernel, dim3(blocks_for(B.cells)), dim3(kThreads), 0, s, B, P, mode, sim.pcg,
-                       static_cast<float*>(sim.partials));
+    hipLaunchKernelGGL(base_red_up_dot_kernel, dim3(kRedBlocks), dim3(kThreads), 0, s, B, P, mode, done,
+                       sim.part_gamma);
 }
 
 void launch_pcg_iteration_head(Sim& sim, int k) {
     const LevelView& B = sim.levels[0];
     float* s_old = sim.search[k & 1];
     float* s_new = sim.search[(k + 1) & 1];
-    float* partials = static_cast<float*>(sim.partials);
-    hipLaunchKernelGGL(spmv_dot_kernel, dim3(blocks_for(B.cells)), dim3(kThreads), 0, sim.stream, B, s_old, s_new,
-                       sim.As, sim.pcg, partials);
-    hipLaunchKernelGGL(axpy_max_kernel, dim3(blocks_for(B.cells)), dim3(kThreads), 0, sim.stream, B, s_new, sim.As,
-                       sim.pressure, sim.props.solve_error, sim.pcg, partials);
+    hipLaunchKernelGGL(spmv_dot_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_old, s_new, sim.As,
+                       sim.pcg, sim.part_gamma, sim.part_delta, k);
+    hipLaunchKernelGGL(axpy_max_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_new, sim.As,
+                       sim.pressure, sim.pcg, sim.part_delta, sim.part_max, k);
+}
+
+void launch_pcg_decide(Sim& sim, int k) {
+    hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.pcg, sim.part_gamma, k);
 }
 
 }  // namespace apic

@@ -2,13 +2,18 @@
 //
 //  p2g_kernel      transfer_particles_to_grid's scatter (src/apic.rs:1237-1426).
 //                  A workgroup takes 256 consecutive particles.  After the
-//                  Morton sort these occupy a compact patch of cells, so the
-//                  workgroup reduces the bounding box of every face and cell
-//                  its particles touch, accumulates momentum, weight and
-//                  density in LDS tiles over that box (ds_add_f32), and flushes
-//                  each nonzero tile entry with one global float atomic.  When
-//                  the box does not fit the tile (unsorted particles, sort
-//                  disabled) the workgroup scatters with global atomics.
+//                  Morton sort, particles of one cell are consecutive, so each
+//                  lane writes its contributions into a fixed window of slots
+//                  around its cell (2x3 u faces, 3x2 v faces, 3x3 density
+//                  cells) and a 64-lane segmented scan sums the window over
+//                  each run of same-cell lanes; only the last lane of a run
+//                  adds the run's sums into LDS tiles covering the
+//                  workgroup's bounding box.  The tiles are flushed with one
+//                  global float atomic per nonzero entry.  Particles next to
+//                  the domain edge (where the reference clamps stencil
+//                  indices) add their own contributions; when the box does
+//                  not fit the tiles (unsorted particles, sort disabled) the
+//                  workgroup scatters with global atomics.
 //  normalize_forces_kernel
 //                  u = momentum / weight and the valid flags (:1428-1459),
 //                  fused with apply_external_forces (:1604-1626).
@@ -50,6 +55,42 @@ __global__ void clear_grid_kernel(float2* __restrict__ u_acc, size_t nu, float2*
     if (t == 0) { fstats->count = 0; fstats->last_raw = 0; }
 }
 
+// Value of lane - n within the same 16-lane row (DPP row_shr:n, no LDS
+// traffic); lanes whose source would cross the row start get 0.
+template <int N>
+__device__ inline float row_shr(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 | N, 0xF, 0xF, true));
+}
+template <int N>
+__device__ inline int row_shr_i(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x110 | N, 0xF, 0xF, true);
+}
+
+// Segmented inclusive sum inside each 16-lane row: after the call, v holds
+// the sum of its segment's values from the segment head up to this lane.
+// Segments never cross a row boundary (the caller starts one at every row).
+template <int N>
+__device__ inline void row_segmented_scan(float (&v)[N], bool head, int lane) {
+    int f = head ? 1 : 0;
+    const int r = lane & 15;
+#define APIC_SEG_STEP(O)                                              \
+    {                                                                 \
+        int tf = row_shr_i<O>(f);                                     \
+        _Pragma("unroll") for (int k = 0; k < N; ++k) {               \
+            float t = row_shr<O>(v[k]);                               \
+            if (r >= O && !f) v[k] += t;                              \
+        }                                                             \
+        if (r >= O) f |= tf;                                          \
+    }
+    APIC_SEG_STEP(1)
+    APIC_SEG_STEP(2)
+    APIC_SEG_STEP(4)
+    APIC_SEG_STEP(8)
+#undef APIC_SEG_STEP
+}
+
+__device__ inline float pick2(int s, int base, float a, float b) { return s == base ? a : (s == base + 1 ? b : 0.0f); }
+
 __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(
     const float* __restrict__ pb, size_t nalloc, uint32_t np, float prev_dt, float cell, uint32_t W, uint32_t H,
     float2* __restrict__ u_acc, float2* __restrict__ v_acc, float* __restrict__ density,
@@ -60,9 +101,12 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(
     const uint32_t i = blockIdx.x * kP2GThreads + threadIdx.x;
     const bool active = i < np;
     const int32_t Wi = (int32_t)W, Hi = (int32_t)H;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 
     float uadd[4], uwt[4], vadd[4], vwt[4], cw[4];
     int32_t ux[2], uy[2], vx_[2], vy_[2], cx[2], cy[2];
+    int32_t kx = 0, ky = 0;
+    bool regular = false;
     int lo_x = INT32_MAX, hi_x = INT32_MIN, lo_y = INT32_MAX, hi_y = INT32_MIN;
     if (active) {
         const float x = pb[(size_t)P_X * nalloc + i], y = pb[(size_t)P_Y * nalloc + i];
@@ -120,11 +164,15 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(
             uint32_t fi = (uint32_t)((int32_t)cly * Wi + (int32_t)clx);
             if (!obstacle[fi]) fluid[fi] = 1;
         }
+        // inside [1, W-1) x [1, H-1) no stencil index is clamped, and every
+        // target lies in the 3x3 window around the particle's cell
+        regular = gx >= 1.0f && gx < (float)(Wi - 1) && gy >= 1.0f && gy < (float)(Hi - 1);
+        kx = (int32_t)floorf(gx);
+        ky = (int32_t)floorf(gy);
         lo_x = min(min(ux[0], vx_[0]), cx[0]); hi_x = max(max(ux[1], vx_[1]), cx[1]);
         lo_y = min(min(uy[0], vy_[0]), cy[0]); hi_y = max(max(uy[1], vy_[1]), cy[1]);
     }
     // workgroup bounding box of all targets
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     lo_x = wave_min(lo_x); hi_x = wave_max(hi_x); lo_y = wave_min(lo_y); hi_y = wave_max(hi_y);
     if (lane == 0) { s_box[0][wave] = lo_x; s_box[1][wave] = hi_x; s_box[2][wave] = lo_y; s_box[3][wave] = hi_y; }
     __syncthreads();
@@ -143,7 +191,74 @@ __global__ __launch_bounds__(kP2GThreads) void p2g_kernel(
             s_umom[e] = 0.0f; s_uw[e] = 0.0f; s_vmom[e] = 0.0f; s_vw[e] = 0.0f; s_den[e] = 0.0f;
         }
         __syncthreads();
-        if (active) {
+        // window slots around cell (kx, ky):
+        //   u faces (kx + sx, ky - 1 + sy), sx in 0..1, sy in 0..2   -> slot sx*3 + sy
+        //   v faces (kx - 1 + sx, ky + sy), sx in 0..2, sy in 0..1   -> slot sx*2 + sy
+        //   cells   (kx - 1 + sx, ky - 1 + sy), sx, sy in 0..2        -> slot sx*3 + sy
+        float v[33];
+        {
+            const int du = uy[0] - (ky - 1);   // 0 or 1
+            const int dv = vx_[0] - (kx - 1);  // 0 or 1
+            const int dcx = cx[0] - (kx - 1), dcy = cy[0] - (ky - 1);
+#pragma unroll
+            for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+                for (int sy = 0; sy < 3; ++sy) {
+                    v[sx * 3 + sy] = regular ? pick2(sy, du, uadd[sx], uadd[2 + sx]) : 0.0f;
+                    v[6 + sx * 3 + sy] = regular ? pick2(sy, du, uwt[sx], uwt[2 + sx]) : 0.0f;
+                }
+#pragma unroll
+            for (int sx = 0; sx < 3; ++sx)
+#pragma unroll
+                for (int sy = 0; sy < 2; ++sy) {
+                    v[12 + sx * 2 + sy] = regular ? pick2(sx, dv, vadd[2 * sy], vadd[2 * sy + 1]) : 0.0f;
+                    v[18 + sx * 2 + sy] = regular ? pick2(sx, dv, vwt[2 * sy], vwt[2 * sy + 1]) : 0.0f;
+                }
+#pragma unroll
+            for (int sx = 0; sx < 3; ++sx)
+#pragma unroll
+                for (int sy = 0; sy < 3; ++sy) {
+                    float a = pick2(sy, dcy, cw[0], cw[2]);
+                    float b = pick2(sy, dcy, cw[1], cw[3]);
+                    v[24 + sx * 3 + sy] = regular ? pick2(sx, dcx, a, b) : 0.0f;
+                }
+        }
+        // runs of same-cell regular lanes, cut at 16-lane row boundaries, form the segments
+        const uint32_t key = regular ? (uint32_t)(ky * Wi + kx) : (0x80000000u | (uint32_t)threadIdx.x);
+        const uint32_t prev_key = (uint32_t)row_shr_i<1>((int)key);
+        const uint32_t next_key = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)key, 0x100 | 1, 0xF, 0xF, true);
+        const bool head = (lane & 15) == 0 || prev_key != key;
+        const bool tail = (lane & 15) == 15 || next_key != key;
+        row_segmented_scan(v, head, lane);
+        if (regular && tail) {
+#pragma unroll
+            for (int sx = 0; sx < 2; ++sx)
+#pragma unroll
+                for (int sy = 0; sy < 3; ++sy) {
+                    int e = (ky - 1 + sy - lo_y) * tw + (kx + sx - lo_x);
+                    float m = v[sx * 3 + sy], w = v[6 + sx * 3 + sy];
+                    if (m != 0.0f) atomicAdd(&s_umom[e], m);
+                    if (w != 0.0f) atomicAdd(&s_uw[e], w);
+                }
+#pragma unroll
+            for (int sx = 0; sx < 3; ++sx)
+#pragma unroll
+                for (int sy = 0; sy < 2; ++sy) {
+                    int e = (ky + sy - lo_y) * tw + (kx - 1 + sx - lo_x);
+                    float m = v[12 + sx * 2 + sy], w = v[18 + sx * 2 + sy];
+                    if (m != 0.0f) atomicAdd(&s_vmom[e], m);
+                    if (w != 0.0f) atomicAdd(&s_vw[e], w);
+                }
+#pragma unroll
+            for (int sx = 0; sx < 3; ++sx)
+#pragma unroll
+                for (int sy = 0; sy < 3; ++sy) {
+                    int e = (ky - 1 + sy - lo_y) * tw + (kx - 1 + sx - lo_x);
+                    float d = v[24 + sx * 3 + sy];
+                    if (d != 0.0f) atomicAdd(&s_den[e], d);
+                }
+        }
+        if (active && !regular) {
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
                 int e = (uy[k >> 1] - lo_y) * tw + (ux[k & 1] - lo_x);
@@ -250,12 +365,11 @@ __device__ inline float masked_interp(const float* w, const uint8_t* open, const
 __global__ __launch_bounds__(256) void g2p_kernel(
     float* __restrict__ pb, size_t nalloc, uint32_t np, float dt, float cell, uint32_t W, uint32_t H,
     uint32_t timestamp, float border_damping, const float* __restrict__ mac_u, const float* __restrict__ mac_v,
-    const uint8_t* __restrict__ open_u, const uint8_t* __restrict__ open_v, uint32_t* __restrict__ maxv_bits,
-    int write_keys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+    const uint8_t* __restrict__ open_u, const uint8_t* __restrict__ open_v,
+    int write_keys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, float* __restrict__ part_maxv) {
     __shared__ float s_max[256 / 64];
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     float speed_new = 0.0f;
-    if (i < nalloc) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nalloc; i += (size_t)gridDim.x * blockDim.x) {
         const float x = pb[(size_t)P_X * nalloc + i], y = pb[(size_t)P_Y * nalloc + i];
         const float res = pb[(size_t)T_RES * nalloc + i];
         const float gx = x / cell, gy = y / cell;
@@ -326,7 +440,7 @@ __global__ __launch_bounds__(256) void g2p_kernel(
         pb[(size_t)CV_X * nalloc + i] = c_vx;
         pb[(size_t)CV_Y * nalloc + i] = c_vy;
         pb[(size_t)T_RES * nalloc + i] = new_res;
-        speed_new = sqrtf(nvx * nvx + nvy * nvy);
+        speed_new = fast_max(speed_new, sqrtf(nvx * nvx + nvy * nvy));
         if (write_keys && i < np) {
             // keygen of the next update_spatial_lookup (:1049-1064)
             float kx = nx / cell, ky = ny / cell;
@@ -345,16 +459,16 @@ __global__ __launch_bounds__(256) void g2p_kernel(
     if (threadIdx.x == 0) {
         float b = s_max[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = fast_max(b, s_max[w]);
-        atomicMax(maxv_bits, __float_as_uint(b));
+        part_maxv[blockIdx.x] = b;
     }
 }
 
 __global__ void max_velocity_kernel(const float* __restrict__ vx, const float* __restrict__ vy, size_t n,
-                                    uint32_t* __restrict__ maxv_bits) {
+                                    float* __restrict__ part_maxv) {
     __shared__ float s_max[256 / 64];
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     float m = 0.0f;
-    if (i < n) m = sqrtf(vx[i] * vx[i] + vy[i] * vy[i]);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        m = fast_max(m, sqrtf(vx[i] * vx[i] + vy[i] * vy[i]));
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = fast_max(m, __shfl_xor(m, o, 64));
     if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
@@ -362,8 +476,30 @@ __global__ void max_velocity_kernel(const float* __restrict__ vx, const float* _
     if (threadIdx.x == 0) {
         float b = s_max[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = fast_max(b, s_max[w]);
-        atomicMax(maxv_bits, __float_as_uint(b));
+        part_maxv[blockIdx.x] = b;
     }
+}
+
+// max over the per-workgroup maxima into the float bits read by the host
+__global__ __launch_bounds__(1024) void maxv_finalize_kernel(const float* __restrict__ part_maxv, uint32_t n,
+                                                             uint32_t* __restrict__ maxv_bits) {
+    __shared__ float s_max[1024 / 64];
+    float m = 0.0f;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = fast_max(m, part_maxv[i]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = fast_max(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float b = 0.0f;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b = fast_max(b, s_max[w]);
+        *maxv_bits = __float_as_uint(b);
+    }
+}
+
+unsigned particle_blocks(const Sim& sim) {
+    size_t b = (sim.nalloc + 255) / 256;
+    return (unsigned)std::min<size_t>(std::max<size_t>(b, 1), sim.part_maxv_cap);
 }
 
 }  // namespace
@@ -375,9 +511,11 @@ void launch_p2g(Sim& sim) {
                        sim.fstats);
     if (sim.num_particles == 0) return;
     uint32_t blocks = (sim.num_particles + kP2GThreads - 1) / kP2GThreads;
+    sim.mark(APIC_T_K_P2G, 0);
     hipLaunchKernelGGL(p2g_kernel, dim3(blocks), dim3(kP2GThreads), 0, sim.stream, sim.pbuf[sim.cur], sim.nalloc,
                        sim.num_particles, sim.prev_dt, sim.cell, sim.W, sim.H, sim.u_acc, sim.v_acc, sim.density,
                        sim.levels[0].fluid, sim.obstacle);
+    sim.mark(APIC_T_K_P2G, 1);
 }
 
 void launch_normalize_forces(Sim& sim, bool normalize, bool forces, float dt) {
@@ -390,21 +528,29 @@ void launch_normalize_forces(Sim& sim, bool normalize, bool forces, float dt) {
 }
 
 void launch_g2p(Sim& sim, float dt, bool write_keys) {
-    launch_fill_u32(sim.stream, sim.maxv_bits, 0u, 1);
-    if (sim.nalloc == 0) return;
-    size_t n = sim.nalloc;
-    hipLaunchKernelGGL(g2p_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sim.stream, sim.pbuf[sim.cur],
-                       sim.nalloc, sim.num_particles, dt, sim.cell, sim.W, sim.H, sim.timestamp,
-                       sim.props.border_damping, sim.mac_u, sim.mac_v, sim.open_u, sim.open_v, sim.maxv_bits,
-                       (int)write_keys, sim.keys[0], sim.vals[0]);
+    if (sim.nalloc == 0) {
+        launch_fill_u32(sim.stream, sim.maxv_bits, 0u, 1);
+        return;
+    }
+    const unsigned blocks = particle_blocks(sim);
+    sim.mark(APIC_T_K_G2P, 0);
+    hipLaunchKernelGGL(g2p_kernel, dim3(blocks), dim3(256), 0, sim.stream, sim.pbuf[sim.cur], sim.nalloc,
+                       sim.num_particles, dt, sim.cell, sim.W, sim.H, sim.timestamp, sim.props.border_damping,
+                       sim.mac_u, sim.mac_v, sim.open_u, sim.open_v, (int)write_keys, sim.keys[0], sim.vals[0],
+                       sim.part_maxv);
+    sim.mark(APIC_T_K_G2P, 1);
+    hipLaunchKernelGGL(maxv_finalize_kernel, dim3(1), dim3(1024), 0, sim.stream, sim.part_maxv, blocks, sim.maxv_bits);
 }
 
 void launch_max_velocity(Sim& sim) {
-    launch_fill_u32(sim.stream, sim.maxv_bits, 0u, 1);
-    if (sim.nalloc == 0) return;
-    size_t n = sim.nalloc;
-    hipLaunchKernelGGL(max_velocity_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sim.stream,
-                       sim.pfield(V_X), sim.pfield(V_Y), n, sim.maxv_bits);
+    if (sim.nalloc == 0) {
+        launch_fill_u32(sim.stream, sim.maxv_bits, 0u, 1);
+        return;
+    }
+    const unsigned blocks = particle_blocks(sim);
+    hipLaunchKernelGGL(max_velocity_kernel, dim3(blocks), dim3(256), 0, sim.stream, sim.pfield(V_X),
+                       sim.pfield(V_Y), sim.nalloc, sim.part_maxv);
+    hipLaunchKernelGGL(maxv_finalize_kernel, dim3(1), dim3(1024), 0, sim.stream, sim.part_maxv, blocks, sim.maxv_bits);
 }
 
 }  // namespace apic

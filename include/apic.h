@@ -155,6 +155,22 @@ apic_status apic_set_solver_config(apic_handle* h, const apic_solver_config* cfg
 
 apic_status apic_get_stats(apic_handle* h, apic_step_stats* out);
 
+/* Stage timers (extension, for benchmarking; no counterpart in the reference).
+ * With profiling on, the library records HIP events on its own stream around
+ * each stage of every substep, and around the two transfer kernels alone, and
+ * accumulates the elapsed milliseconds.  Enabling resets the accumulators. */
+typedef enum apic_timer {
+    APIC_T_SORT = 0, APIC_T_P2G, APIC_T_PRESSURE_BUILD, APIC_T_MG_BUILD, APIC_T_PCG,
+    APIC_T_GRADIENT, APIC_T_EXTRAPOLATE, APIC_T_G2P,
+    APIC_T_K_P2G,   /* the P2G scatter kernel alone */
+    APIC_T_K_G2P,   /* the G2P gather/advect kernel alone */
+    APIC_TIMER_COUNT
+} apic_timer;
+
+apic_status apic_set_profiling(apic_handle* h, int enable);
+/* ms[i] and counts[i] for i < n (n <= APIC_TIMER_COUNT); either may be null. */
+apic_status apic_get_timers(apic_handle* h, double* ms, uint32_t* counts, uint32_t n);
+
 /* ---- pub fields (src/apic.rs:715-776) --------------------------------------- */
 
 /* width, height, num_particles, num_chunks, multigrid level count (excluding

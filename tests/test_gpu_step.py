@@ -1,0 +1,179 @@
+"""Whole-step parity of Apic::update on the GPU against the oracle.
+
+Tolerances (relative L2 over all particles):
+  - 1 update, and 100 updates with the particle sort disabled: 1e-5.
+  - 100 updates with the sort on: the reference's step is chaotic here.  The
+    jitter of each particle is hashed from its post-sort index
+    (src/apic.rs:2181), so a rounding-level difference that moves one
+    particle across a cell edge re-orders thousands of indices.  The oracle
+    run twice with two valid P2G summation orders diverges by the same
+    amount, so the GPU's error is bounded by twice that self-spread (plus
+    1e-5), and aggregate statistics must agree to 1%.
+The FP64 oracle's error is reported beside the FP32 parity in each case.
+"""
+import numpy as np
+import pytest
+
+from helpers import PARTICLE_FIELDS, props, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def pair(name, precision=32):
+    from FLIP import scenes
+    from FLIP.apic import Apic
+    from oracle import pyoracle as po
+    if name == "ref64":
+        g = Apic.new(64, 64, 1.0, props())
+        o = po.Oracle(64, 64, 1.0, po.bench_props(), precision=precision)
+        return g, o, 0.016
+    xy = scenes.config_scene("cfg1_dam_break_128")
+    g = Apic.with_particles(128, 128, 1.0 / 128, props(), xy)
+    o = po.Oracle(128, 128, 1.0 / 128, po.bench_props(), xy=xy, precision=precision)
+    return g, o, 0.002
+
+
+def particle_err(g, o):
+    return max(rel_l2(g.read(f), o.read(f)) for f in ("POS_X", "POS_Y", "VEL_X", "VEL_Y"))
+
+
+@pytest.mark.parametrize("name", ["ref64", "dam128"])
+def test_one_update(name):
+    from FLIP.apic import DebugOptions
+    g, o, dt = pair(name)
+    o64 = pair(name, precision=64)[1]
+    g.update(DebugOptions(False), dt)
+    o.update(dt)
+    o64.update(dt)
+    for f in PARTICLE_FIELDS:
+        assert rel_l2(g.read(f), o.read(f)) <= 1e-5, f
+    for f in ("MAC_U", "MAC_V", "DENSITY"):
+        assert rel_l2(g.read(f), o.read(f)) <= 1e-5, f
+    assert g.stats()["pcg_iterations"] == o.stats()["pcg_iterations"]
+    assert g.timestamp == 1 and abs(g.prev_deltatime - dt) < 1e-9
+    print(f"{name}: 1 update, GPU vs FP32 oracle {particle_err(g, o):.2e}, vs FP64 oracle {particle_err(g, o64):.2e}")
+
+
+def test_100_updates_without_sort():
+    from FLIP.apic import DebugOptions
+    g, o, dt = pair("dam128")
+    o64 = pair("dam128", precision=64)[1]
+    iters_g, iters_o = [], []
+    for _ in range(100):
+        g.update(DebugOptions(True), dt)
+        o.update(dt, disable_sort=True)
+        o64.update(dt, disable_sort=True)
+        iters_g.append(g.stats()["pcg_iterations"])
+        iters_o.append(o.stats()["pcg_iterations"])
+    err = particle_err(g, o)
+    print(f"100 updates, sort off: GPU vs FP32 oracle {err:.2e}, vs FP64 oracle {particle_err(g, o64):.2e}; "
+          f"PCG iterations {sum(iters_g)} (GPU) vs {sum(iters_o)} (oracle)")
+    assert err <= 1e-5
+    assert max(abs(a - b) for a, b in zip(iters_g, iters_o)) <= 1
+    assert rel_l2(g.read("PRESSURE"), o.read("PRESSURE")) <= 1e-3
+
+
+def test_100_updates_with_sort_within_reference_spread():
+    from FLIP import scenes
+    from FLIP.apic import DebugOptions
+    from oracle import pyoracle as po
+    g, o, dt = pair("dam128")
+    xy = scenes.config_scene("cfg1_dam_break_128")
+    o2 = po.Oracle(128, 128, 1.0 / 128, po.bench_props(), xy=xy)
+    o2.set_p2g_shuffle(12345)  # another valid summation order of the reference's scatter
+    for _ in range(100):
+        g.update(DebugOptions(False), dt)
+        o.update(dt)
+        o2.update(dt)
+    spread = particle_err(o2, o)
+    err = particle_err(g, o)
+    print(f"100 updates, sort on: GPU vs oracle {err:.2e}; oracle self-spread {spread:.2e}")
+    assert err <= 2.0 * spread + 1e-5
+    n = o.dims()["num_particles"]
+    for f in ("POS_X", "POS_Y"):
+        a, b = g.read(f)[:n].mean(), o.read(f)[:n].mean()
+        assert abs(a - b) <= 0.01 * abs(b), f
+    sg = np.hypot(g.read("VEL_X")[:n], g.read("VEL_Y")[:n]).mean()
+    so = np.hypot(o.read("VEL_X")[:n], o.read("VEL_Y")[:n]).mean()
+    assert abs(sg - so) <= 0.01 * so
+
+
+def test_radial_force_parity():
+    """add_radial_force (src/apic.rs:2352-2410), both signs, then one update."""
+    from FLIP.apic import DebugOptions
+    g, o, dt = pair("dam128")
+    for args in ((0.2, 0.3, 0.1, 50.0), (0.3, 0.2, 0.08, -40.0)):
+        g.add_radial_force(*args)
+        o.add_radial_force(*args)
+    for f in ("EXT_FORCE_U", "EXT_FORCE_V", "POST_FORCE_U", "POST_FORCE_V"):
+        assert np.array_equal(g.read(f), o.read(f).astype(np.float32)), f
+    g.update(DebugOptions(False), dt)
+    o.update(dt)
+    assert particle_err(g, o) <= 1e-5
+    for f in ("EXT_FORCE_U", "POST_FORCE_V"):  # cleared after the update (src/apic.rs:2470-2473)
+        assert not g.read(f).any()
+
+
+def test_empty_scene_updates():
+    from FLIP.apic import Apic, DebugOptions
+    from oracle import pyoracle as po
+    g = Apic.with_particles(32, 32, 1.0, props(), np.zeros((0, 2), np.float32))
+    o = po.Oracle(32, 32, 1.0, po.bench_props(), xy=np.zeros((0, 2), np.float32))
+    assert g.num_particles == 0 and g.dims()["num_levels"] == 0
+    for _ in range(3):
+        g.update(DebugOptions(False), 0.016)
+        o.update(0.016)
+    assert g.timestamp == 3
+    assert np.array_equal(g.read("MAC_V"), o.read("MAC_V").astype(np.float32))
+    assert g.get_max_particle_velocity() == 0.0
+
+
+def test_ragged_particle_count_and_tail_lanes():
+    """N % 8 != 0: tail lanes repeat the last particle, are zeroed by the sort
+    and still advected (src/apic.rs:887-898, 1077-1103, 2079-2085)."""
+    from FLIP.apic import Apic, DebugOptions
+    from oracle import pyoracle as po
+    rng = np.random.default_rng(11)
+    xy = rng.uniform(4.0, 20.0, size=(1237, 2)).astype(np.float32)
+    g = Apic.with_particles(32, 32, 1.0, props(), xy)
+    o = po.Oracle(32, 32, 1.0, po.bench_props(), xy=xy)
+    assert g.num_chunks == (1237 + 7) // 8
+    assert np.array_equal(g.read("POS_X"), o.read("POS_X").astype(np.float32))
+    for _ in range(5):
+        g.update(DebugOptions(False), 0.016)
+        o.update(0.016)
+    assert particle_err(g, o) <= 1e-5
+    assert g.get_max_particle_velocity() == np.float32(o.max_particle_velocity())
+
+
+def test_unsupported_size_fails_like_the_reference():
+    from FLIP._lib import ApicError
+    from FLIP.apic import Apic
+    with pytest.raises(ApicError) as e:
+        Apic.new(97, 97, 1.0, props())
+    assert e.value.status == 2
+
+
+def test_full_size_sort_and_conservation():
+    """The 1024^2 dam break at full size (2,005,736 particles): bit-exact sort
+    against the oracle's stable sort, and the total particle-to-grid weight
+    equals the sum of temporal weights."""
+    from FLIP import scenes
+    from FLIP.apic import Apic, DebugOptions
+    from oracle import pyoracle as po
+    xy = scenes.config_scene("cfg2_dam_break_1024")
+    g = Apic.with_particles(1024, 1024, 1.0 / 1024, props(), xy)
+    o = po.Oracle(1024, 1024, 1.0 / 1024, po.bench_props(), xy=xy)
+    g.update(DebugOptions(False), 2.5e-4)
+    o.update(2.5e-4)
+    assert particle_err(g, o) <= 1e-5
+    o.stage("update_spatial_lookup")
+    g.update_spatial_lookup()
+    assert np.array_equal(g.read("SORT_KEY"), o.read("SORT_KEY"))
+    assert np.array_equal(g.read("SORT_VAL"), o.read("SORT_VAL"))
+    g.transfer_particles_to_grid()
+    n = xy.shape[0]
+    tres = g.read("TIME_RESIDUAL")[:n].astype(np.float64)
+    tau = -tres / g.prev_deltatime
+    wt = np.maximum(2.1875 * np.maximum(1.0 - (tau - 0.5) ** 2, 0.0) ** 3, 1e-5)
+    assert abs(g.read("DENSITY").sum(dtype=np.float64) - wt.sum()) <= 1e-4 * wt.sum()
