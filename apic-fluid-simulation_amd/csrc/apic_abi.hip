@@ -137,10 +137,10 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     ALLOC(s.keys[0], np); ALLOC(s.keys[1], np);
     ALLOC(s.vals[0], np); ALLOC(s.vals[1], np);
     ALLOC(s.radix_scratch, radix_scratch_words(np));
+    ALLOC(s.cell_range, cell_key_count(s));
     const size_t nu = s.nu(), nv = s.nv(), nc = s.cells;
     ALLOC(s.mac_u, nu); ALLOC(s.mac_v, nv); ALLOC(s.old_u, nu); ALLOC(s.old_v, nv);
     ALLOC(s.valid_u, nu); ALLOC(s.valid_v, nv); ALLOC(s.nvalid_u, nu); ALLOC(s.nvalid_v, nv);
-    ALLOC(s.u_acc, nu); ALLOC(s.v_acc, nv);
     ALLOC(s.density, nc); ALLOC(s.smooth_a, nc); ALLOC(s.smooth_b, nc); ALLOC(s.pressure, nc);
     ALLOC(s.obstacle, nc); ALLOC(s.open_u, nu); ALLOC(s.open_v, nv);
     ALLOC(s.ext_u, nu); ALLOC(s.ext_v, nv); ALLOC(s.post_u, nu); ALLOC(s.post_v, nv);
@@ -224,6 +224,8 @@ apic_status do_sort(Sim& s) {
     if (!s.keys_valid) launch_keygen(s);
     launch_radix_sort(s);
     launch_permute(s);
+    launch_cell_ranges(s);
+    s.cells_valid = true;
     s.keys_valid = false;
     CHECK_LAUNCH();
     return APIC_OK;
@@ -269,8 +271,7 @@ apic_status substep(Sim& s, bool sort, float dt) {
         s.mark(APIC_T_SORT, 1);
     }
     s.mark(APIC_T_P2G, 0);
-    launch_p2g(s);
-    launch_normalize_forces(s, true, true, dt);
+    launch_p2g(s, true, dt);
     s.mark(APIC_T_P2G, 1);
     s.mark(APIC_T_PRESSURE_BUILD, 0);
     launch_build_pressure(s, true);
@@ -290,6 +291,7 @@ apic_status substep(Sim& s, bool sort, float dt) {
     s.mark(APIC_T_EXTRAPOLATE, 1);
     s.mark(APIC_T_G2P, 0);
     launch_g2p(s, dt, true);
+    s.cells_valid = false;
     s.mark(APIC_T_G2P, 1);
     CHECK_LAUNCH();
     HIP_TRY(hipMemcpyAsync(s.h_maxv, s.maxv_bits, 4, hipMemcpyDeviceToHost, s.stream));
@@ -543,9 +545,11 @@ apic_status apic_write_field(apic_handle* h, apic_field field, uint32_t level, c
     if (count != d.count) return fail(APIC_ERR_INVALID_ARGUMENT, "count does not match the field length");
     HIP_TRY(hipStreamSynchronize(s.stream));
     if (count) HIP_TRY(hipMemcpy(d.ptr, src, count * d.elem, hipMemcpyHostToDevice));
+    if (field >= APIC_POS_X && field <= APIC_TIME_RESIDUAL) s.cells_valid = false;
     if (field == APIC_POS_X || field == APIC_POS_Y) s.keys_valid = false;
     if (field == APIC_VEL_X || field == APIC_VEL_Y) s.maxv_valid = false;
-    if (field == APIC_SORT_KEY || field == APIC_SORT_VAL) s.keys_valid = true;
+    // the reference recomputes the keys from the positions at every sort (:1050-1063)
+    if (field == APIC_SORT_KEY || field == APIC_SORT_VAL) s.keys_valid = false;
     if (field >= APIC_EXT_FORCE_U && field <= APIC_POST_FORCE_V) s.forces_active = true;
     if (field == APIC_FLUID && level == 0) {
         launch_fluid_stats(s);
@@ -566,8 +570,7 @@ apic_status apic_update_spatial_lookup(apic_handle* h) {
 apic_status apic_transfer_particles_to_grid(apic_handle* h, float deltatime, uint32_t timestamp) {
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
     (void)deltatime; (void)timestamp;  // unused by the reference as well (:1218)
-    launch_p2g(h->sim);
-    launch_normalize_forces(h->sim, true, false, 0.0f);
+    launch_p2g(h->sim, false, 0.0f);
     launch_fluid_stats(h->sim);
     CHECK_LAUNCH();
     HIP_TRY(hipStreamSynchronize(h->sim.stream));
@@ -576,7 +579,7 @@ apic_status apic_transfer_particles_to_grid(apic_handle* h, float deltatime, uin
 
 apic_status apic_apply_external_forces(apic_handle* h, float deltatime) {
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
-    launch_normalize_forces(h->sim, false, true, deltatime);
+    launch_forces(h->sim, deltatime);
     CHECK_LAUNCH();
     HIP_TRY(hipStreamSynchronize(h->sim.stream));
     return APIC_OK;
@@ -635,6 +638,7 @@ apic_status apic_transfer_grid_to_particles_and_advect(apic_handle* h, float del
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
     Sim& s = h->sim;
     launch_g2p(s, deltatime, true);
+    s.cells_valid = false;
     CHECK_LAUNCH();
     HIP_TRY(hipMemcpyAsync(s.h_maxv, s.maxv_bits, 4, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipStreamSynchronize(s.stream));

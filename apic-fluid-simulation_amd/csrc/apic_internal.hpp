@@ -69,8 +69,10 @@ void launch_permute(Sim& sim);
 size_t radix_scratch_words(size_t n);
 
 // transfer.hip
-void launch_p2g(Sim& sim);
-void launch_normalize_forces(Sim& sim, bool normalize, bool forces, float dt);
+size_t cell_key_count(const Sim& sim);            // Morton keys of the grid: morton(W-1, H-1) + 1
+void launch_cell_ranges(Sim& sim);                // per-cell [begin, end) of keys[0]'s order
+void launch_p2g(Sim& sim, bool forces, float dt);  // P2G + normalisation (+ external forces and gravity)
+void launch_forces(Sim& sim, float dt);            // apply_external_forces alone
 void launch_g2p(Sim& sim, float dt, bool write_keys);
 void launch_max_velocity(Sim& sim);
 
@@ -113,13 +115,14 @@ struct Sim {
     uint32_t* vals[2] = {nullptr, nullptr};
     uint32_t* radix_scratch = nullptr;
     bool keys_valid = false;  // keys[0]/vals[0] hold the current positions' keys
+    uint2* cell_range = nullptr;  // [begin, end) of every cell's particles in cell order, by Morton key
+    bool cells_valid = false;     // the particles are in cell order and cell_range describes it
 
     // MAC grid
     float* mac_u = nullptr;   float* mac_v = nullptr;
     float* old_u = nullptr;   float* old_v = nullptr;
     uint8_t* valid_u = nullptr; uint8_t* valid_v = nullptr;
     uint8_t* nvalid_u = nullptr; uint8_t* nvalid_v = nullptr;
-    float2* u_acc = nullptr;  float2* v_acc = nullptr;  // (momentum, weight) per face
     float* density = nullptr;
     float* smooth_a = nullptr; float* smooth_b = nullptr;  // smoothed density ends in smooth_a
     float* pressure = nullptr;

@@ -1,22 +1,16 @@
 // transfer.hip -- particle<->grid transfers of the APIC step.
 //
-//  p2g_kernel      transfer_particles_to_grid's scatter (src/apic.rs:1237-1426).
-//                  A workgroup takes 256 consecutive particles.  After the
-//                  Morton sort, particles of one cell are consecutive, so each
-//                  lane writes its contributions into a fixed window of slots
-//                  around its cell (2x3 u faces, 3x2 v faces, 3x3 density
-//                  cells) and a 64-lane segmented scan sums the window over
-//                  each run of same-cell lanes; only the last lane of a run
-//                  adds the run's sums into LDS tiles covering the
-//                  workgroup's bounding box.  The tiles are flushed with one
-//                  global float atomic per nonzero entry.  Particles next to
-//                  the domain edge (where the reference clamps stencil
-//                  indices) add their own contributions; when the box does
-//                  not fit the tiles (unsorted particles, sort disabled) the
-//                  workgroup scatters with global atomics.
-//  normalize_forces_kernel
-//                  u = momentum / weight and the valid flags (:1428-1459),
-//                  fused with apply_external_forces (:1604-1626).
+//  p2g_owner_kernel
+//                  transfer_particles_to_grid (src/apic.rs:1218-1497) over
+//                  particles grouped by cell: one thread per cell sums its own
+//                  particles into register partials, each face and cell adds
+//                  the partials of its neighbour cells, and the same kernel
+//                  normalises the faces and applies the external forces
+//                  (apply_external_forces, :1604-1626).  Deterministic, no
+//                  atomics.  The cell grouping comes from the Morton sort
+//                  (the particles themselves are sorted) or, when the sort is
+//                  disabled, from a sorted index.
+//  forces_kernel   apply_external_forces on its own (the stage entry point).
 //  g2p_kernel      transfer_grid_to_particles_and_advect (:2057-2264): masked
 //                  bilinear gather, jittered time step, wall clamp/reflect,
 //                  affine C rebuild; fused with the next sort's keygen and the
@@ -29,322 +23,327 @@ namespace apic {
 
 namespace {
 
-constexpr int kP2GThreads = 256;
-constexpr int kTileCap = 1024;
+// ---- cell-owner P2G -----------------------------------------------------------
+//
+// In cell order (the Morton-sorted particles, or the particles through a
+// sorted index `order`), the particles of one cell are the contiguous range
+// cell_range[key] = [begin, end).  A workgroup
+// takes a kOwnWin x kOwnWin window of cells: a kOwnTile x kOwnTile tile and
+// its one-cell ring.  Particles of the window are staged in LDS (coalesced
+// loads), then one thread per window cell sums the contributions of its own
+// particles into 33 register partials: the 2x3 u faces, 3x2 v faces and 3x3
+// cells that a particle of that cell can reach (src/apic.rs:1237-1426, with
+// u_index / v_index clamping, :1029-1041).  Every face and cell of the tile
+// then adds the partials of its 6 (faces) or 9 (cells) source cells in a
+// fixed order, so the result is deterministic and needs no atomics, and
+// finishes the target in the same kernel: u = momentum / weight and the valid
+// flag (:1428-1459), the fluid flag (:1461-1464) and, when asked,
+// apply_external_forces (:1604-1626).  The faces x = W and y = H belong to the
+// cells x = W-1 and y = H-1.
 
-__device__ inline int wave_min(int v) {
+constexpr int kOwnWin = 16;
+constexpr int kOwnTile = kOwnWin - 2;
+constexpr int kOwnCells = kOwnWin * kOwnWin;  // one thread per window cell
+constexpr int kOwnCap = 1024;                 // particles staged per pass
+constexpr int kOwnFields = 9;
+constexpr int kOwnSlots = 33;                 // 6 u (mom, w), 6 v (mom, w), 9 density
+static_assert(kOwnFields * kOwnCap >= kOwnSlots * kOwnCells, "partials alias the staging buffer");
+
+struct OwnerAcc {
+    float um[6], uw[6];  // u slot kx * 3 + sy: face (cx + kx, cy - 1 + sy)
+    float vm[6], vw[6];  // v slot sx * 2 + ky: face (cx - 1 + sx, cy + ky)
+    float d[9];          // cell slot sx * 3 + sy: cell (cx - 1 + sx, cy - 1 + sy)
+};
+
+template <int N>
+__device__ inline void slot_add(float (&acc)[N], int s, float v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
-    return v;
+    for (int t = 0; t < N; ++t)
+        if (s == t) acc[t] += v;
 }
-__device__ inline int wave_max(int v) {
+
+// Two stencil corners c0 (offset 0) and c1 (offset 1) added to three slots
+// whose first one is the stencil's start when h is false, one before it when
+// h is true.
+__device__ inline void shift3(bool h, float c0, float c1, float& s0, float& s1, float& s2) {
+    const float z = 0.0f;
+    s0 += h ? z : c0;
+    s1 += h ? c0 : c1;
+    s2 += h ? c1 : z;
+}
+
+// One particle of cell (cx, cy): the same arithmetic as p2g_contrib.
+__device__ inline void owner_accumulate(const float* __restrict__ sp, int k, float prev_dt, float cell, int32_t Wi,
+                                        int32_t Hi, int32_t cx, int32_t cy, OwnerAcc& a) {
+    const float x = sp[0 * kOwnCap + k], y = sp[1 * kOwnCap + k];
+    const float vx = sp[2 * kOwnCap + k], vy = sp[3 * kOwnCap + k];
+    const float cux = sp[4 * kOwnCap + k], cuy = sp[5 * kOwnCap + k];
+    const float cvx = sp[6 * kOwnCap + k], cvy = sp[7 * kOwnCap + k];
+    const float tres = sp[8 * kOwnCap + k];
+    const float tau = (-tres) / prev_dt;
+    const float wt = temporal_w(tau);
+    const float gx = x / cell, gy = y / cell;
+    const float cgx = gx - 0.5f, cgy = gy - 0.5f;
+    const float cbx = floorf(cgx), cby = floorf(cgy);
+    const float ctx = cgx - cbx, cty = cgy - cby;
+    float cw[4];
+    cw[0] = ((1.0f - ctx) * (1.0f - cty)) * wt;
+    cw[1] = (ctx * (1.0f - cty)) * wt;
+    cw[2] = ((1.0f - ctx) * cty) * wt;
+    cw[3] = (ctx * cty) * wt;
+    float uadd[4], uwt[4], vadd[4], vwt[4];
+    Stencil su, sv;
+    make_stencil(gx, gy - 0.5f, cell, su);
+    make_stencil(gx - 0.5f, gy, cell, sv);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-__global__ void clear_grid_kernel(float2* __restrict__ u_acc, size_t nu, float2* __restrict__ v_acc, size_t nv,
-                                  float* __restrict__ density, uint32_t* __restrict__ fluid_words, size_t cells,
-                                  size_t fluid_words_n, FluidStats* fstats) {
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (size_t i = t; i < nu; i += stride) u_acc[i] = make_float2(0.0f, 0.0f);
-    for (size_t i = t; i < nv; i += stride) v_acc[i] = make_float2(0.0f, 0.0f);
-    for (size_t i = t; i < cells; i += stride) density[i] = 0.0f;
-    for (size_t i = t; i < fluid_words_n; i += stride) fluid_words[i] = 0u;
-    if (t == 0) { fstats->count = 0; fstats->last_raw = 0; }
-}
-
-// Value of lane - n within the same 16-lane row (DPP row_shr:n, no LDS
-// traffic); lanes whose source would cross the row start get 0.
-template <int N>
-__device__ inline float row_shr(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x110 | N, 0xF, 0xF, true));
-}
-template <int N>
-__device__ inline int row_shr_i(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, 0x110 | N, 0xF, 0xF, true);
-}
-
-// Segmented inclusive sum inside each 16-lane row: after the call, v holds
-// the sum of its segment's values from the segment head up to this lane.
-// Segments never cross a row boundary (the caller starts one at every row).
-template <int N>
-__device__ inline void row_segmented_scan(float (&v)[N], bool head, int lane) {
-    int f = head ? 1 : 0;
-    const int r = lane & 15;
-#define APIC_SEG_STEP(O)                                              \
-    {                                                                 \
-        int tf = row_shr_i<O>(f);                                     \
-        _Pragma("unroll") for (int k = 0; k < N; ++k) {               \
-            float t = row_shr<O>(v[k]);                               \
-            if (r >= O && !f) v[k] += t;                              \
-        }                                                             \
-        if (r >= O) f |= tf;                                          \
+    for (int c = 0; c < 4; ++c) {
+        float ww = su.w[c] * wt;
+        uwt[c] = ww;
+        uadd[c] = ((vx + cux * su.dx[c]) + cuy * su.dy[c]) * ww;
+        ww = sv.w[c] * wt;
+        vwt[c] = ww;
+        vadd[c] = ((vy + cvx * sv.dx[c]) + cvy * sv.dy[c]) * ww;
     }
-    APIC_SEG_STEP(1)
-    APIC_SEG_STEP(2)
-    APIC_SEG_STEP(4)
-    APIC_SEG_STEP(8)
-#undef APIC_SEG_STEP
+    const bool regular = gx >= 1.0f && gx < (float)(Wi - 1) && gy >= 1.0f && gy < (float)(Hi - 1);
+    if (regular) {
+        // No index is clamped and the particle lies in (cx, cy): the stencils
+        // start at column cx - 1 + hx and row cy - 1 + hy.
+        const bool hx = (int32_t)cbx - (cx - 1) != 0;
+        const bool hy = (int32_t)cby - (cy - 1) != 0;
+        shift3(hy, uadd[0], uadd[2], a.um[0], a.um[1], a.um[2]);
+        shift3(hy, uwt[0], uwt[2], a.uw[0], a.uw[1], a.uw[2]);
+        shift3(hy, uadd[1], uadd[3], a.um[3], a.um[4], a.um[5]);
+        shift3(hy, uwt[1], uwt[3], a.uw[3], a.uw[4], a.uw[5]);
+        shift3(hx, vadd[0], vadd[1], a.vm[0], a.vm[2], a.vm[4]);
+        shift3(hx, vwt[0], vwt[1], a.vw[0], a.vw[2], a.vw[4]);
+        shift3(hx, vadd[2], vadd[3], a.vm[1], a.vm[3], a.vm[5]);
+        shift3(hx, vwt[2], vwt[3], a.vw[1], a.vw[3], a.vw[5]);
+        const float r00 = hx ? 0.0f : cw[0], r01 = hx ? cw[0] : cw[1], r02 = hx ? cw[1] : 0.0f;
+        const float r10 = hx ? 0.0f : cw[2], r11 = hx ? cw[2] : cw[3], r12 = hx ? cw[3] : 0.0f;
+        shift3(hy, r00, r10, a.d[0], a.d[1], a.d[2]);
+        shift3(hy, r01, r11, a.d[3], a.d[4], a.d[5]);
+        shift3(hy, r02, r12, a.d[6], a.d[7], a.d[8]);
+    } else {
+        // Clamped corners (a particle near or beyond the walls): slots from
+        // the clamped indices, which always stay within the cell's reach.
+        const int32_t ux0 = clampi(su.ix, 0, Wi) - cx, ux1 = clampi(su.ix + 1, 0, Wi) - cx;
+        const int32_t uy0 = clampi(su.iy, 0, Hi - 1) - (cy - 1), uy1 = clampi(su.iy + 1, 0, Hi - 1) - (cy - 1);
+        const int32_t vx0 = clampi(sv.ix, 0, Wi - 1) - (cx - 1), vx1 = clampi(sv.ix + 1, 0, Wi - 1) - (cx - 1);
+        const int32_t vy0 = clampi(sv.iy, 0, Hi) - cy, vy1 = clampi(sv.iy + 1, 0, Hi) - cy;
+        const int32_t bx = (int32_t)cbx, by = (int32_t)cby;
+        const int32_t dx0 = clampi(bx, 0, Wi - 1) - (cx - 1), dx1 = clampi(bx + 1, 0, Wi - 1) - (cx - 1);
+        const int32_t dy0 = clampi(by, 0, Hi - 1) - (cy - 1), dy1 = clampi(by + 1, 0, Hi - 1) - (cy - 1);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int32_t us = ((c & 1) ? ux1 : ux0) * 3 + ((c >> 1) ? uy1 : uy0);
+            const int32_t vs = ((c & 1) ? vx1 : vx0) * 2 + ((c >> 1) ? vy1 : vy0);
+            const int32_t ds = ((c & 1) ? dx1 : dx0) * 3 + ((c >> 1) ? dy1 : dy0);
+            slot_add(a.um, us, uadd[c]);
+            slot_add(a.uw, us, uwt[c]);
+            slot_add(a.vm, vs, vadd[c]);
+            slot_add(a.vw, vs, vwt[c]);
+            slot_add(a.d, ds, cw[c]);
+        }
+    }
 }
 
-__device__ inline float pick2(int s, int base, float a, float b) { return s == base ? a : (s == base + 1 ? b : 0.0f); }
-
-__global__ __launch_bounds__(kP2GThreads) void p2g_kernel(
-    const float* __restrict__ pb, size_t nalloc, uint32_t np, float prev_dt, float cell, uint32_t W, uint32_t H,
-    float2* __restrict__ u_acc, float2* __restrict__ v_acc, float* __restrict__ density,
-    uint8_t* __restrict__ fluid, const uint8_t* __restrict__ obstacle) {
-    __shared__ float s_umom[kTileCap], s_uw[kTileCap], s_vmom[kTileCap], s_vw[kTileCap], s_den[kTileCap];
-    __shared__ int s_box[4][kP2GThreads / 64];
-
-    const uint32_t i = blockIdx.x * kP2GThreads + threadIdx.x;
-    const bool active = i < np;
+__global__ __launch_bounds__(kOwnCells) void p2g_owner_kernel(
+    const float* __restrict__ pb, size_t nalloc, uint32_t np, const uint32_t* __restrict__ order, float prev_dt,
+    float cell, uint32_t W, uint32_t H, const uint2* __restrict__ cell_range, uint32_t tiles_x,
+    float* __restrict__ mac_u, float* __restrict__ mac_v, uint8_t* __restrict__ valid_u,
+    uint8_t* __restrict__ valid_v, float* __restrict__ density, uint8_t* __restrict__ fluid,
+    const uint8_t* __restrict__ obstacle, const float* __restrict__ ext_u, const float* __restrict__ ext_v,
+    int forces, int ext, float dt, float g) {
+    __shared__ float s_buf[kOwnFields * kOwnCap];  // staged particles, then the partials
+    __shared__ uint32_t s_off[kOwnCells + 1];      // window-order prefix of the cell counts
+    __shared__ uint32_t s_src[kOwnCells];          // sorted-order begin of each window cell
+    __shared__ uint32_t s_wave[kOwnCells / 64];
+    __shared__ uint8_t s_fl[kOwnCells];
     const int32_t Wi = (int32_t)W, Hi = (int32_t)H;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-
-    float uadd[4], uwt[4], vadd[4], vwt[4], cw[4];
-    int32_t ux[2], uy[2], vx_[2], vy_[2], cx[2], cy[2];
-    int32_t kx = 0, ky = 0;
-    bool regular = false;
-    int lo_x = INT32_MAX, hi_x = INT32_MIN, lo_y = INT32_MAX, hi_y = INT32_MIN;
-    if (active) {
-        const float x = pb[(size_t)P_X * nalloc + i], y = pb[(size_t)P_Y * nalloc + i];
-        const float vx = pb[(size_t)V_X * nalloc + i], vy = pb[(size_t)V_Y * nalloc + i];
-        const float cux = pb[(size_t)CU_X * nalloc + i], cuy = pb[(size_t)CU_Y * nalloc + i];
-        const float cvx = pb[(size_t)CV_X * nalloc + i], cvy = pb[(size_t)CV_Y * nalloc + i];
-        const float tres = pb[(size_t)T_RES * nalloc + i];
-        const float tau = (-tres) / prev_dt;
-        const float wt = temporal_w(tau);
-        const float gx = x / cell, gy = y / cell;
-        // density stencil (:1260-1284)
-        {
-            float cgx = gx - 0.5f, cgy = gy - 0.5f;
-            float cbx = floorf(cgx), cby = floorf(cgy);
-            float ctx = cgx - cbx, cty = cgy - cby;
-            cw[0] = ((1.0f - ctx) * (1.0f - cty)) * wt;
-            cw[1] = (ctx * (1.0f - cty)) * wt;
-            cw[2] = ((1.0f - ctx) * cty) * wt;
-            cw[3] = (ctx * cty) * wt;
-            int32_t bx = (int32_t)cbx, by = (int32_t)cby;
-            cx[0] = clampi(bx, 0, Wi - 1); cx[1] = clampi(bx + 1, 0, Wi - 1);
-            cy[0] = clampi(by, 0, Hi - 1); cy[1] = clampi(by + 1, 0, Hi - 1);
-        }
-        // u faces
-        {
-            Stencil s;
-            make_stencil(gx, gy - 0.5f, cell, s);
-            ux[0] = clampi(s.ix, 0, Wi); ux[1] = clampi(s.ix + 1, 0, Wi);
-            uy[0] = clampi(s.iy, 0, Hi - 1); uy[1] = clampi(s.iy + 1, 0, Hi - 1);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                float ww = s.w[k] * wt;
-                uwt[k] = ww;
-                uadd[k] = ((vx + cux * s.dx[k]) + cuy * s.dy[k]) * ww;
-            }
-        }
-        // v faces
-        {
-            Stencil s;
-            make_stencil(gx - 0.5f, gy, cell, s);
-            vx_[0] = clampi(s.ix, 0, Wi - 1); vx_[1] = clampi(s.ix + 1, 0, Wi - 1);
-            vy_[0] = clampi(s.iy, 0, Hi); vy_[1] = clampi(s.iy + 1, 0, Hi);
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                float ww = s.w[k] * wt;
-                vwt[k] = ww;
-                vadd[k] = ((vy + cvx * s.dx[k]) + cvy * s.dy[k]) * ww;
-            }
-        }
-        // fluid bit of the particle's cell (:1358-1360, 1409; obstacle cells are
-        // cleared from the fluid mask afterwards, :1464)
-        {
-            float clx = fast_max(fast_min(gx, (float)W - 1.0f), 0.0f);
-            float cly = fast_max(fast_min(gy, (float)H - 1.0f), 0.0f);
-            uint32_t fi = (uint32_t)((int32_t)cly * Wi + (int32_t)clx);
-            if (!obstacle[fi]) fluid[fi] = 1;
-        }
-        // inside [1, W-1) x [1, H-1) no stencil index is clamped, and every
-        // target lies in the 3x3 window around the particle's cell
-        regular = gx >= 1.0f && gx < (float)(Wi - 1) && gy >= 1.0f && gy < (float)(Hi - 1);
-        kx = (int32_t)floorf(gx);
-        ky = (int32_t)floorf(gy);
-        lo_x = min(min(ux[0], vx_[0]), cx[0]); hi_x = max(max(ux[1], vx_[1]), cx[1]);
-        lo_y = min(min(uy[0], vy_[0]), cy[0]); hi_y = max(max(uy[1], vy_[1]), cy[1]);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wx = tid % kOwnWin, wy = tid / kOwnWin;
+    const int32_t cx = (int32_t)(blockIdx.x % tiles_x) * kOwnTile - 1 + wx;
+    const int32_t cy = (int32_t)(blockIdx.x / tiles_x) * kOwnTile - 1 + wy;
+    const bool inside = cx >= 0 && cx < Wi && cy >= 0 && cy < Hi;
+    uint32_t b = 0, cnt = 0;
+    bool fl = false;
+    if (inside) {
+        const uint2 r = cell_range[morton((uint32_t)cx, (uint32_t)cy)];
+        b = r.x;
+        cnt = (r.y > r.x && r.y <= np) ? r.y - r.x : 0u;
+        fl = cnt > 0 && !obstacle[(size_t)cy * W + cx];
     }
-    // workgroup bounding box of all targets
-    lo_x = wave_min(lo_x); hi_x = wave_max(hi_x); lo_y = wave_min(lo_y); hi_y = wave_max(hi_y);
-    if (lane == 0) { s_box[0][wave] = lo_x; s_box[1][wave] = hi_x; s_box[2][wave] = lo_y; s_box[3][wave] = hi_y; }
+    // exclusive prefix of the counts in window order
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) s_wave[wave] = incl;
+    s_src[tid] = b;
+    s_fl[tid] = fl;
     __syncthreads();
-    lo_x = s_box[0][0]; hi_x = s_box[1][0]; lo_y = s_box[2][0]; hi_y = s_box[3][0];
+    uint32_t wbase = 0, total = 0;
 #pragma unroll
-    for (int w = 1; w < kP2GThreads / 64; ++w) {
-        lo_x = min(lo_x, s_box[0][w]); hi_x = max(hi_x, s_box[1][w]);
-        lo_y = min(lo_y, s_box[2][w]); hi_y = max(hi_y, s_box[3][w]);
+    for (int w = 0; w < kOwnCells / 64; ++w) {
+        if (w < wave) wbase += s_wave[w];
+        total += s_wave[w];
     }
-    if (hi_x < lo_x) return;  // no active particle in this workgroup (uniform)
-    const int tw = hi_x - lo_x + 1, th = hi_y - lo_y + 1;
-    const uint32_t W1 = W + 1;
-    if ((long long)tw * th <= kTileCap) {
-        const int n = tw * th;
-        for (int e = threadIdx.x; e < n; e += kP2GThreads) {
-            s_umom[e] = 0.0f; s_uw[e] = 0.0f; s_vmom[e] = 0.0f; s_vw[e] = 0.0f; s_den[e] = 0.0f;
+    const uint32_t my0 = wbase + incl - cnt, my1 = wbase + incl;
+    s_off[tid] = my0;
+    if (tid == 0) s_off[kOwnCells] = total;
+    __syncthreads();
+
+    OwnerAcc a;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) { a.um[s] = 0.0f; a.uw[s] = 0.0f; a.vm[s] = 0.0f; a.vw[s] = 0.0f; }
+#pragma unroll
+    for (int s = 0; s < 9; ++s) a.d[s] = 0.0f;
+    for (uint32_t base = 0; base < total; base += kOwnCap) {
+        const uint32_t n = min((uint32_t)kOwnCap, total - base);
+        for (uint32_t k = tid; k < n; k += kOwnCells) {
+            const uint32_t idx = base + k;
+            int lo = 0, hi = kOwnCells - 1;  // the window cell holding idx: last c with s_off[c] <= idx
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= idx) lo = mid; else hi = mid - 1;
+            }
+            const uint32_t q = s_src[lo] + (idx - s_off[lo]);  // position in cell order
+            const size_t src = order ? order[q] : q;
+            s_buf[0 * kOwnCap + k] = pb[(size_t)P_X * nalloc + src];
+            s_buf[1 * kOwnCap + k] = pb[(size_t)P_Y * nalloc + src];
+            s_buf[2 * kOwnCap + k] = pb[(size_t)V_X * nalloc + src];
+            s_buf[3 * kOwnCap + k] = pb[(size_t)V_Y * nalloc + src];
+            s_buf[4 * kOwnCap + k] = pb[(size_t)CU_X * nalloc + src];
+            s_buf[5 * kOwnCap + k] = pb[(size_t)CU_Y * nalloc + src];
+            s_buf[6 * kOwnCap + k] = pb[(size_t)CV_X * nalloc + src];
+            s_buf[7 * kOwnCap + k] = pb[(size_t)CV_Y * nalloc + src];
+            s_buf[8 * kOwnCap + k] = pb[(size_t)T_RES * nalloc + src];
         }
         __syncthreads();
-        // window slots around cell (kx, ky):
-        //   u faces (kx + sx, ky - 1 + sy), sx in 0..1, sy in 0..2   -> slot sx*3 + sy
-        //   v faces (kx - 1 + sx, ky + sy), sx in 0..2, sy in 0..1   -> slot sx*2 + sy
-        //   cells   (kx - 1 + sx, ky - 1 + sy), sx, sy in 0..2        -> slot sx*3 + sy
-        float v[33];
-        {
-            const int du = uy[0] - (ky - 1);   // 0 or 1
-            const int dv = vx_[0] - (kx - 1);  // 0 or 1
-            const int dcx = cx[0] - (kx - 1), dcy = cy[0] - (ky - 1);
-#pragma unroll
-            for (int sx = 0; sx < 2; ++sx)
-#pragma unroll
-                for (int sy = 0; sy < 3; ++sy) {
-                    v[sx * 3 + sy] = regular ? pick2(sy, du, uadd[sx], uadd[2 + sx]) : 0.0f;
-                    v[6 + sx * 3 + sy] = regular ? pick2(sy, du, uwt[sx], uwt[2 + sx]) : 0.0f;
-                }
-#pragma unroll
-            for (int sx = 0; sx < 3; ++sx)
-#pragma unroll
-                for (int sy = 0; sy < 2; ++sy) {
-                    v[12 + sx * 2 + sy] = regular ? pick2(sx, dv, vadd[2 * sy], vadd[2 * sy + 1]) : 0.0f;
-                    v[18 + sx * 2 + sy] = regular ? pick2(sx, dv, vwt[2 * sy], vwt[2 * sy + 1]) : 0.0f;
-                }
-#pragma unroll
-            for (int sx = 0; sx < 3; ++sx)
-#pragma unroll
-                for (int sy = 0; sy < 3; ++sy) {
-                    float a = pick2(sy, dcy, cw[0], cw[2]);
-                    float b = pick2(sy, dcy, cw[1], cw[3]);
-                    v[24 + sx * 3 + sy] = regular ? pick2(sx, dcx, a, b) : 0.0f;
-                }
-        }
-        // runs of same-cell regular lanes, cut at 16-lane row boundaries, form the segments
-        const uint32_t key = regular ? (uint32_t)(ky * Wi + kx) : (0x80000000u | (uint32_t)threadIdx.x);
-        const uint32_t prev_key = (uint32_t)row_shr_i<1>((int)key);
-        const uint32_t next_key = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)key, 0x100 | 1, 0xF, 0xF, true);
-        const bool head = (lane & 15) == 0 || prev_key != key;
-        const bool tail = (lane & 15) == 15 || next_key != key;
-        row_segmented_scan(v, head, lane);
-        if (regular && tail) {
-#pragma unroll
-            for (int sx = 0; sx < 2; ++sx)
-#pragma unroll
-                for (int sy = 0; sy < 3; ++sy) {
-                    int e = (ky - 1 + sy - lo_y) * tw + (kx + sx - lo_x);
-                    float m = v[sx * 3 + sy], w = v[6 + sx * 3 + sy];
-                    if (m != 0.0f) atomicAdd(&s_umom[e], m);
-                    if (w != 0.0f) atomicAdd(&s_uw[e], w);
-                }
-#pragma unroll
-            for (int sx = 0; sx < 3; ++sx)
-#pragma unroll
-                for (int sy = 0; sy < 2; ++sy) {
-                    int e = (ky + sy - lo_y) * tw + (kx - 1 + sx - lo_x);
-                    float m = v[12 + sx * 2 + sy], w = v[18 + sx * 2 + sy];
-                    if (m != 0.0f) atomicAdd(&s_vmom[e], m);
-                    if (w != 0.0f) atomicAdd(&s_vw[e], w);
-                }
-#pragma unroll
-            for (int sx = 0; sx < 3; ++sx)
-#pragma unroll
-                for (int sy = 0; sy < 3; ++sy) {
-                    int e = (ky - 1 + sy - lo_y) * tw + (kx - 1 + sx - lo_x);
-                    float d = v[24 + sx * 3 + sy];
-                    if (d != 0.0f) atomicAdd(&s_den[e], d);
-                }
-        }
-        if (active && !regular) {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                int e = (uy[k >> 1] - lo_y) * tw + (ux[k & 1] - lo_x);
-                atomicAdd(&s_umom[e], uadd[k]);
-                atomicAdd(&s_uw[e], uwt[k]);
-                e = (vy_[k >> 1] - lo_y) * tw + (vx_[k & 1] - lo_x);
-                atomicAdd(&s_vmom[e], vadd[k]);
-                atomicAdd(&s_vw[e], vwt[k]);
-                e = (cy[k >> 1] - lo_y) * tw + (cx[k & 1] - lo_x);
-                atomicAdd(&s_den[e], cw[k]);
-            }
-        }
+        const uint32_t lo = max(my0, base), hi = min(my1, base + n);
+        for (uint32_t idx = lo; idx < hi; ++idx)
+            owner_accumulate(s_buf, (int)(idx - base), prev_dt, cell, Wi, Hi, cx, cy, a);
         __syncthreads();
-        for (int e = threadIdx.x; e < n; e += kP2GThreads) {
-            const int ex = lo_x + e % tw, ey = lo_y + e / tw;
-            float m = s_umom[e], w = s_uw[e];
-            if ((m != 0.0f || w != 0.0f) && ex <= Wi && ey < Hi) {
-                float2* a = &u_acc[(size_t)ey * W1 + ex];
-                unsafeAtomicAdd(&a->x, m);
-                unsafeAtomicAdd(&a->y, w);
-            }
-            m = s_vmom[e]; w = s_vw[e];
-            if ((m != 0.0f || w != 0.0f) && ex < Wi && ey <= Hi) {
-                float2* a = &v_acc[(size_t)ey * W + ex];
-                unsafeAtomicAdd(&a->x, m);
-                unsafeAtomicAdd(&a->y, w);
-            }
-            float d = s_den[e];
-            if (d != 0.0f && ex < Wi && ey < Hi) unsafeAtomicAdd(&density[(size_t)ey * W + ex], d);
-        }
-    } else if (active) {
+    }
+    // partials of every window cell, slot-major
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (uadd[k] != 0.0f || uwt[k] != 0.0f) {
-                float2* a = &u_acc[(size_t)uy[k >> 1] * W1 + ux[k & 1]];
-                unsafeAtomicAdd(&a->x, uadd[k]);
-                unsafeAtomicAdd(&a->y, uwt[k]);
+    for (int s = 0; s < 6; ++s) {
+        s_buf[(0 + s) * kOwnCells + tid] = a.um[s];
+        s_buf[(6 + s) * kOwnCells + tid] = a.uw[s];
+        s_buf[(12 + s) * kOwnCells + tid] = a.vm[s];
+        s_buf[(18 + s) * kOwnCells + tid] = a.vw[s];
+    }
+#pragma unroll
+    for (int s = 0; s < 9; ++s) s_buf[(24 + s) * kOwnCells + tid] = a.d[s];
+    __syncthreads();
+    if (!inside || wx < 1 || wx > kOwnTile || wy < 1 || wy > kOwnTile) return;
+#define PART(slot, dwx, dwy) s_buf[(slot) * kOwnCells + tid + (dwy) * kOwnWin + (dwx)]
+    // u face (cx, cy): cells (cx - 1 + dx, cy - 1 + dy), slot (1 - dx) * 3 + (2 - dy)
+    {
+        float m = 0.0f, w = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int s = (1 - dx) * 3 + (2 - dy);
+                m += PART(0 + s, dx - 1, dy - 1);
+                w += PART(6 + s, dx - 1, dy - 1);
             }
-            if (vadd[k] != 0.0f || vwt[k] != 0.0f) {
-                float2* a = &v_acc[(size_t)vy_[k >> 1] * W + vx_[k & 1]];
-                unsafeAtomicAdd(&a->x, vadd[k]);
-                unsafeAtomicAdd(&a->y, vwt[k]);
+        const size_t f = (size_t)cy * (W + 1) + cx;
+        const bool ok = w > 0.0f;
+        float u = ok ? m / w : 0.0f;
+        if (ext) u += ext_u[f] * dt;
+        mac_u[f] = u;
+        valid_u[f] = ok;
+        if (cx == Wi - 1) {  // face x = W: only the cells x = W - 1 reach it
+            m = 0.0f; w = 0.0f;
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy) {
+                m += PART(0 + 3 + (2 - dy), 0, dy - 1);
+                w += PART(6 + 3 + (2 - dy), 0, dy - 1);
             }
-            if (cw[k] != 0.0f) unsafeAtomicAdd(&density[(size_t)cy[k >> 1] * W + cx[k & 1]], cw[k]);
+            const bool ok2 = w > 0.0f;
+            u = ok2 ? m / w : 0.0f;
+            if (ext) u += ext_u[f + 1] * dt;
+            mac_u[f + 1] = u;
+            valid_u[f + 1] = ok2;
         }
     }
+    // v face (cx, cy): cells (cx - 1 + dx, cy - 1 + dy), slot (2 - dx) * 2 + (1 - dy)
+    {
+        float m = 0.0f, w = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const int s = (2 - dx) * 2 + (1 - dy);
+                m += PART(12 + s, dx - 1, dy - 1);
+                w += PART(18 + s, dx - 1, dy - 1);
+            }
+        const size_t f = (size_t)cy * W + cx;
+        const bool ok = w > 0.0f;
+        float v = ok ? m / w : 0.0f;
+        if (ext) v += ext_v[f] * dt;
+        if (forces && ((cy >= 1 && s_fl[tid - kOwnWin]) || s_fl[tid])) v += g;
+        mac_v[f] = v;
+        valid_v[f] = ok;
+        if (cy == Hi - 1) {  // face y = H: only the cells y = H - 1 reach it
+            m = 0.0f; w = 0.0f;
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                m += PART(12 + (2 - dx) * 2 + 1, dx - 1, 0);
+                w += PART(18 + (2 - dx) * 2 + 1, dx - 1, 0);
+            }
+            const bool ok2 = w > 0.0f;
+            v = ok2 ? m / w : 0.0f;
+            if (ext) v += ext_v[f + W] * dt;
+            if (forces && s_fl[tid]) v += g;
+            mac_v[f + W] = v;
+            valid_v[f + W] = ok2;
+        }
+    }
+    // cell (cx, cy): cells (cx - 1 + dx, cy - 1 + dy), slot (2 - dx) * 3 + (2 - dy)
+    {
+        float dsum = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) dsum += PART(24 + (2 - dx) * 3 + (2 - dy), dx - 1, dy - 1);
+        const size_t c = (size_t)cy * W + cx;
+        density[c] = dsum;
+        fluid[c] = s_fl[tid];
+    }
+#undef PART
 }
 
-__global__ void normalize_forces_kernel(const float2* __restrict__ u_acc, const float2* __restrict__ v_acc,
-                                        float* __restrict__ mac_u, float* __restrict__ mac_v,
-                                        uint8_t* __restrict__ valid_u, uint8_t* __restrict__ valid_v,
-                                        const float* __restrict__ ext_u, const float* __restrict__ ext_v,
-                                        const uint8_t* __restrict__ fluid, uint32_t W, uint32_t H, int normalize,
-                                        int ext_forces, int gravity_on, float dt, float g) {
+// [begin, end) of every cell's particles in cell order, indexed by key
+__global__ void cell_ranges_kernel(const uint32_t* __restrict__ keys, uint32_t np, uint2* __restrict__ range,
+                                   uint32_t nkeys) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const uint32_t k = keys[i];
+    if (k >= nkeys) return;  // keys come from clamped cell coordinates; kept as a guard
+    if (i == 0 || keys[i - 1] != k) range[k].x = i;
+    if (i + 1 == np || keys[i + 1] != k) range[k].y = i + 1;
+}
+
+// apply_external_forces (src/apic.rs:1604-1626): u, v += force * dt on every
+// face, v += g * dt on faces with a fluid cell below or above.
+__global__ void forces_kernel(float* __restrict__ mac_u, float* __restrict__ mac_v, const float* __restrict__ ext_u,
+                              const float* __restrict__ ext_v, const uint8_t* __restrict__ fluid, uint32_t W,
+                              uint32_t H, int ext_forces, float dt, float g) {
     const size_t nu = (size_t)W * H + H, nv = (size_t)W * H + W;
     size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < nu) {
-        float u;
-        if (normalize) {
-            float2 a = u_acc[f];
-            bool ok = a.y > 0.0f;
-            u = ok ? a.x / a.y : 0.0f;
-            valid_u[f] = ok;
-        } else {
-            u = mac_u[f];
-        }
-        if (ext_forces) u += ext_u[f] * dt;
-        mac_u[f] = u;
-    }
+    if (f < nu && ext_forces) mac_u[f] = mac_u[f] + ext_u[f] * dt;
     if (f < nv) {
-        float v;
-        if (normalize) {
-            float2 a = v_acc[f];
-            bool ok = a.y > 0.0f;
-            v = ok ? a.x / a.y : 0.0f;
-            valid_v[f] = ok;
-        } else {
-            v = mac_v[f];
-        }
+        float v = mac_v[f];
         if (ext_forces) v += ext_v[f] * dt;
-        if (gravity_on) {
-            uint32_t x = (uint32_t)(f % W), y = (uint32_t)(f / W);
-            bool below = y >= 1 && fluid[(size_t)(y - 1) * W + x];
-            bool above = y < H && fluid[(size_t)y * W + x];
-            if (below || above) v += g;
-        }
+        uint32_t x = (uint32_t)(f % W), y = (uint32_t)(f / W);
+        bool below = y >= 1 && fluid[(size_t)(y - 1) * W + x];
+        bool above = y < H && fluid[(size_t)y * W + x];
+        if (below || above) v += g;
         mac_v[f] = v;
     }
 }
@@ -504,27 +503,48 @@ unsigned particle_blocks(const Sim& sim) {
 
 }  // namespace
 
-void launch_p2g(Sim& sim) {
-    size_t fw = (sim.cells + 3) / 4;
-    hipLaunchKernelGGL(clear_grid_kernel, dim3(1024), dim3(256), 0, sim.stream, sim.u_acc, sim.nu(), sim.v_acc,
-                       sim.nv(), sim.density, reinterpret_cast<uint32_t*>(sim.levels[0].fluid), (size_t)sim.cells, fw,
-                       sim.fstats);
+size_t cell_key_count(const Sim& sim) {
+    return (size_t)morton(sim.W > 0 ? sim.W - 1 : 0, sim.H > 0 ? sim.H - 1 : 0) + 1;
+}
+
+void launch_cell_ranges(Sim& sim) {
+    const size_t nk = cell_key_count(sim);
+    launch_fill_u32(sim.stream, reinterpret_cast<uint32_t*>(sim.cell_range), 0u, 2 * nk);
     if (sim.num_particles == 0) return;
-    uint32_t blocks = (sim.num_particles + kP2GThreads - 1) / kP2GThreads;
+    hipLaunchKernelGGL(cell_ranges_kernel, dim3((sim.num_particles + 255) / 256), dim3(256), 0, sim.stream,
+                       sim.keys[0], sim.num_particles, sim.cell_range, (uint32_t)nk);
+}
+
+void launch_p2g(Sim& sim, bool forces, float dt) {
+    const uint32_t* order = nullptr;
+    if (!sim.cells_valid) {
+        // The particles are not in cell order (sort disabled, or state written
+        // through the field API): sort an index by cell instead of the particles.
+        if (sim.num_particles > 0) {
+            if (!sim.keys_valid) launch_keygen(sim);
+            launch_radix_sort(sim);
+            sim.keys_valid = false;  // keys[0] now holds the sorted keys
+            order = sim.vals[0];
+        }
+        launch_cell_ranges(sim);
+    }
+    const uint32_t tx = (sim.W + kOwnTile - 1) / kOwnTile, ty = (sim.H + kOwnTile - 1) / kOwnTile;
+    const float g = sim.props.gravity * dt;
+    const int ext = forces && sim.forces_active;
     sim.mark(APIC_T_K_P2G, 0);
-    hipLaunchKernelGGL(p2g_kernel, dim3(blocks), dim3(kP2GThreads), 0, sim.stream, sim.pbuf[sim.cur], sim.nalloc,
-                       sim.num_particles, sim.prev_dt, sim.cell, sim.W, sim.H, sim.u_acc, sim.v_acc, sim.density,
-                       sim.levels[0].fluid, sim.obstacle);
+    hipLaunchKernelGGL(p2g_owner_kernel, dim3(tx * ty), dim3(kOwnCells), 0, sim.stream, sim.pbuf[sim.cur], sim.nalloc,
+                       sim.num_particles, order, sim.prev_dt, sim.cell, sim.W, sim.H, sim.cell_range, tx, sim.mac_u,
+                       sim.mac_v, sim.valid_u, sim.valid_v, sim.density, sim.levels[0].fluid, sim.obstacle,
+                       sim.ext_u, sim.ext_v, (int)forces, ext, dt, g);
     sim.mark(APIC_T_K_P2G, 1);
 }
 
-void launch_normalize_forces(Sim& sim, bool normalize, bool forces, float dt) {
+void launch_forces(Sim& sim, float dt) {
     size_t n = std::max(sim.nu(), sim.nv());
     float g = sim.props.gravity * dt;
-    int ext = forces && sim.forces_active;
-    hipLaunchKernelGGL(normalize_forces_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sim.stream,
-                       sim.u_acc, sim.v_acc, sim.mac_u, sim.mac_v, sim.valid_u, sim.valid_v, sim.ext_u, sim.ext_v,
-                       sim.levels[0].fluid, sim.W, sim.H, (int)normalize, ext, (int)forces, dt, g);
+    int ext = sim.forces_active;
+    hipLaunchKernelGGL(forces_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sim.stream, sim.mac_u,
+                       sim.mac_v, sim.ext_u, sim.ext_v, sim.levels[0].fluid, sim.W, sim.H, ext, dt, g);
 }
 
 void launch_g2p(Sim& sim, float dt, bool write_keys) {
