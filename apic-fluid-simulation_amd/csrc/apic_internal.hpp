@@ -33,6 +33,20 @@ struct LevelView {
     uint8_t* fluid;  // raw index
 };
 
+// The one-workgroup bottom of the V-cycle keeps its levels in LDS when they
+// fit: six float arrays over the padded range [0, cells + 2w) per level and
+// one flag byte per cell, plus the coarsest level's three CG vectors and its
+// serial-sum products.
+constexpr size_t kBottomLdsFloats = 13312;
+constexpr size_t kBottomLdsFlags = 4096;
+__host__ __device__ inline size_t bottom_padded(const LevelView& L) { return (size_t)L.cells + 2 * (size_t)L.w; }
+inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first) {
+    size_t fl = 0, fb = 0;
+    for (size_t l = first; l < lv.size(); ++l) { fl += 6 * bottom_padded(lv[l]); fb += lv[l].cells; }
+    fl += 3 * bottom_padded(lv.back()) + lv.back().cells;
+    return fl <= kBottomLdsFloats && fb <= kBottomLdsFlags;
+}
+
 // Scalars of one pressure solve.  The reductions write per-workgroup
 // partials; the kernel that needs a reduced scalar sums the partials itself,
 // in the same fixed order in every workgroup, and applies the reference's exit
@@ -135,6 +149,7 @@ struct Sim {
     // solver
     std::vector<LevelView> levels;  // [0] = base grid, [k] = multigrid_levels[k-1]
     size_t bottom_level = 0;        // first level solved by the one-workgroup kernel
+    bool bottom_in_lds = false;     // that kernel holds its levels in LDS
     float* search[2] = {nullptr, nullptr};
     float* As = nullptr;
     float* cg_s = nullptr; float* cg_ap = nullptr; float* cg_r = nullptr;  // coarsest level

@@ -450,17 +450,28 @@ __device__ void coarse_cg_block(const LevelView& L, float* cg_s, float* cg_ap, f
     }
     atomicAdd(&s_nf, nf_local);
     __syncthreads();
-    // sequential sums in ascending cell order, as the reference's serial loops
+    // Sequential sums in ascending cell order, as the reference's serial
+    // loops.  Terms of non-fluid cells are replaced by +0, which leaves the
+    // running sum unchanged (it starts at +0 and never becomes -0), so eight
+    // terms at a time are loaded before they are added in order.
+    int bc = 0;
     auto serial_sum = [&](auto term) -> float {
         if (threadIdx.x == 0) {
             float s = 0.0f;
-            for (uint32_t c = 0; c < n; ++c)
-                if (L.fluid[c]) s += term(c);
-            s_bcast[0] = s;
+            uint32_t c = 0;
+            for (; c + 8 <= n; c += 8) {
+                float t[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k) t[k] = L.fluid[c + k] ? term(c + k) : 0.0f;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) s += t[k];
+            }
+            for (; c < n; ++c) s += L.fluid[c] ? term(c) : 0.0f;
+            s_bcast[bc] = s;
         }
         __syncthreads();
-        float r = s_bcast[0];
-        __syncthreads();
+        float r = s_bcast[bc];
+        bc ^= 1;  // the next sum writes the other slot, so no second barrier is needed
         return r;
     };
     float gamma;
@@ -513,22 +524,113 @@ __device__ void coarse_cg_block(const LevelView& L, float* cg_s, float* cg_ap, f
     }
 }
 
+// Level views of the one-workgroup bottom: the levels themselves (global
+// memory), or copies of levels [first, count) in LDS, laid out level after
+// level as diag, inv_diag, px, py, r, e over the padded range
+// [0, cells + 2w), fluid flags after them, then the coarse CG vectors.
+struct GlobalViews {
+    const LevelSet* ls;
+    __device__ LevelView view(int l) const { return ls->lv[l]; }
+};
+
+struct LdsViews {
+    const LevelSet* ls;
+    int first;
+    float* f;
+    uint8_t* fl;
+    __device__ LevelView view(int l) const {
+        size_t fo = 0, bo = 0;
+        for (int k = first; k < l; ++k) {
+            fo += 6 * bottom_padded(ls->lv[k]);
+            bo += ls->lv[k].cells;
+        }
+        LevelView L = ls->lv[l];
+        const size_t p = bottom_padded(L);
+        L.diag = f + fo; L.inv_diag = f + fo + p; L.px = f + fo + 2 * p; L.py = f + fo + 3 * p;
+        L.r = f + fo + 4 * p; L.e = f + fo + 5 * p;
+        L.fluid = fl + bo;
+        return L;
+    }
+};
+
+// solve_coarse_exact (src/apic.rs:618-704) for a level of at most 64 cells,
+// run by one wave with one lane per cell: the vectors stay in registers,
+// neighbours come from other lanes, and the serial sums read the lanes in
+// ascending cell order, so the arithmetic is that of coarse_cg_block without a
+// workgroup barrier per phase.  Only e of the level is written.
+__device__ void coarse_cg_wave(const LevelView& L) {
+    const uint32_t n = L.cells;
+    const int w = (int)L.w;
+    const int c = (int)(threadIdx.x & 63);
+    const bool in = (uint32_t)c < n;
+    const size_t pc = (size_t)c + (size_t)w;
+    const bool fl = in && L.fluid[c];
+    const float diag = in ? L.diag[pc] : 0.0f, px = in ? L.px[pc] : 0.0f, py = in ? L.py[pc] : 0.0f;
+    const float pxm = in ? L.px[pc - 1] : 0.0f, pym = in ? L.py[pc - w] : 0.0f;
+    float r = in ? L.r[pc] : 0.0f;
+    float sv = r, e = 0.0f, ap = 0.0f;
+    // the padded vectors are zero outside [0, n)
+    auto nb = [&](int d) {
+        const int j = c + d;
+        const float v = __shfl(sv, j & 63, 64);
+        return (j >= 0 && (uint32_t)j < n) ? v : 0.0f;
+    };
+    auto serial_sum = [&](float term) {
+        const float t = fl ? term : 0.0f;  // +0 leaves the running sum unchanged
+        float acc = 0.0f;
+        for (uint32_t k = 0; k < n; ++k) acc += __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), (int)k));
+        return acc;
+    };
+    uint32_t nf = 0;
+    for (uint32_t k = 0; k < n; ++k) nf += (uint32_t)__builtin_amdgcn_readlane((int)fl, (int)k);
+    float gamma = serial_sum(r * r);
+    if (!(gamma < 1e-12f)) {
+        const uint32_t maxit = nf < 100u ? nf : 100u;
+        for (uint32_t it = 0; it < maxit; ++it) {
+            const float sr = nb(1), sl = nb(-1), st = nb(w), sb = nb(-w);
+            if (fl) {
+                float v = diag * sv;
+                v = fmaf(px, sr, v);
+                v = fmaf(pxm, sl, v);
+                v = fmaf(py, st, v);
+                v = fmaf(pym, sb, v);
+                ap = v;
+            }
+            const float delta = serial_sum(sv * ap);
+            if (fabsf(delta) < kSafety) break;
+            const float alpha = gamma / delta;
+            float m = 0.0f;
+            if (fl) {
+                e += alpha * sv;
+                r = r - alpha * ap;
+                m = fabsf(r);
+            }
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) m = fast_max(m, __shfl_xor(m, o, 64));
+            if (m < 1e-6f) break;
+            const float ng = serial_sum(r * r);
+            const float beta = ng / gamma;
+            gamma = ng;
+            if (fl) sv = r + (beta * sv);
+        }
+    }
+    if (in) L.e[pc] = e;
+}
+
 // Levels [first, count) of the V-cycle inside one workgroup.  On entry
 // level `first` holds r (restricted) and e after its fused first red sweep
 // (or e = 0 when it is the coarsest level).  On exit level `first` holds its
 // final post-smoothed error.
-__global__ __launch_bounds__(kBottomThreads) void bottom_kernel(LevelSet ls, int first, float* cg_s, float* cg_ap,
-                                                               float* cg_r, const int32_t* __restrict__ done) {
-    __shared__ float sprod[kSerialCap];
-    __shared__ float scratch[kBottomThreads / 64];
-    if (*done) return;
-    const int last = ls.count - 1;
+template <class Views>
+__device__ void bottom_body(const Views& V, int first, int last, float* cg_s, float* cg_ap, float* cg_r,
+                            float* sprod, float* scratch) {
     Prolong none{};
     // down
     for (int l = first; l < last; ++l) {
-        const LevelView& L = ls.lv[l];
+        const LevelView L = V.view(l);
         if (l > first) {
-            for (uint32_t c = threadIdx.x; c < L.cells; c += blockDim.x) restrict_cell(ls.lv[l - 1], L, c, true);
+            const LevelView F = V.view(l - 1);
+            for (uint32_t c = threadIdx.x; c < L.cells; c += blockDim.x) restrict_cell(F, L, c, true);
             __syncthreads();
         }
         for (uint32_t c = threadIdx.x; c < L.cells; c += blockDim.x) {
@@ -537,18 +639,23 @@ __global__ __launch_bounds__(kBottomThreads) void bottom_kernel(LevelSet ls, int
         }
         __syncthreads();
     }
+    const LevelView C = V.view(last);
     if (last > first) {
-        for (uint32_t c = threadIdx.x; c < ls.lv[last].cells; c += blockDim.x)
-            restrict_cell(ls.lv[last - 1], ls.lv[last], c, false);
+        const LevelView F = V.view(last - 1);
+        for (uint32_t c = threadIdx.x; c < C.cells; c += blockDim.x) restrict_cell(F, C, c, false);
         __syncthreads();
     }
-    coarse_cg_block(ls.lv[last], cg_s, cg_ap, cg_r, sprod, scratch);
+    if (C.cells <= 64) {
+        if (threadIdx.x < 64) coarse_cg_wave(C);
+    } else {
+        coarse_cg_block(C, cg_s, cg_ap, cg_r, sprod, scratch);
+    }
     __syncthreads();
     // up
     for (int l = last - 1; l >= first; --l) {
-        const LevelView& L = ls.lv[l];
+        const LevelView L = V.view(l);
         Prolong P{};
-        P.C = ls.lv[l + 1];
+        P.C = V.view(l + 1);
         P.has = 1;
         for (uint32_t c = threadIdx.x; c < L.cells; c += blockDim.x) {
             uint32_t x = c % L.w, y = c / L.w;
@@ -560,6 +667,62 @@ __global__ __launch_bounds__(kBottomThreads) void bottom_kernel(LevelSet ls, int
             if (((x + y) & 1u) == 0) sor_cell(L, P, c, 2);
         }
         __syncthreads();
+    }
+}
+
+// Bottom levels in global memory (used when they do not fit in LDS).
+__global__ __launch_bounds__(kBottomThreads) void bottom_kernel(LevelSet ls, int first, float* cg_s, float* cg_ap,
+                                                               float* cg_r, const int32_t* __restrict__ done) {
+    __shared__ float sprod[kSerialCap];
+    __shared__ float scratch[kBottomThreads / 64];
+    if (*done) return;
+    GlobalViews V{&ls};
+    bottom_body(V, first, ls.count - 1, cg_s, cg_ap, cg_r, sprod, scratch);
+}
+
+// Bottom levels copied to LDS: the same arithmetic without a global-memory
+// round trip between the dependent phases.  The errors and residuals of every
+// bottom level are copied back (the finer level's prolongation reads them).
+__global__ __launch_bounds__(kBottomThreads) void bottom_lds_kernel(LevelSet ls, int first,
+                                                                   const int32_t* __restrict__ done) {
+    __shared__ float f[kBottomLdsFloats];
+    __shared__ uint8_t fl[kBottomLdsFlags];
+    __shared__ float scratch[kBottomThreads / 64];
+    if (*done) return;
+    const int last = ls.count - 1;
+    LdsViews V{&ls, first, f, fl};
+    size_t fo = 0, bo = 0;
+    for (int l = first; l <= last; ++l) {
+        const LevelView& G = ls.lv[l];
+        const uint32_t p = (uint32_t)bottom_padded(G);
+        for (uint32_t i = threadIdx.x; i < p; i += blockDim.x) {
+            f[fo + i] = G.diag[i];
+            f[fo + p + i] = G.inv_diag[i];
+            f[fo + 2 * p + i] = G.px[i];
+            f[fo + 3 * p + i] = G.py[i];
+            f[fo + 4 * p + i] = G.r[i];
+            f[fo + 5 * p + i] = G.e[i];
+        }
+        for (uint32_t i = threadIdx.x; i < G.cells; i += blockDim.x) fl[bo + i] = G.fluid[i];
+        fo += 6 * (size_t)p;
+        bo += G.cells;
+    }
+    const size_t pl = bottom_padded(ls.lv[last]);
+    float* cg_s = f + fo;
+    float* cg_ap = cg_s + pl;
+    float* cg_r = cg_ap + pl;
+    float* sprod = cg_r + pl;
+    // the CG vectors' padding rows are read as neighbours: zero, as in global memory
+    for (uint32_t i = threadIdx.x; i < 3 * pl + ls.lv[last].cells; i += blockDim.x) cg_s[i] = 0.0f;
+    __syncthreads();
+    bottom_body(V, first, last, cg_s, cg_ap, cg_r, sprod, scratch);
+    for (int l = first; l <= last; ++l) {
+        const LevelView G = ls.lv[l];
+        const LevelView L = V.view(l);
+        for (uint32_t i = threadIdx.x; i < G.cells; i += blockDim.x) {
+            G.r[i + G.w] = L.r[i + G.w];
+            G.e[i + G.w] = L.e[i + G.w];
+        }
     }
 }
 
@@ -750,8 +913,11 @@ void launch_vcycle(Sim& sim, int mode) {
         }
         hipLaunchKernelGGL(restrict_kernel, dim3(blocks_for(sim.levels[bottom].cells)), dim3(kThreads), 0, s,
                            sim.levels[bottom - 1], sim.levels[bottom], (int)(bottom < last), done);
-        hipLaunchKernelGGL(bottom_kernel, dim3(1), dim3(kBottomThreads), 0, s, level_set(sim), bottom, sim.cg_s,
-                           sim.cg_ap, sim.cg_r, done);
+        if (sim.bottom_in_lds)
+            hipLaunchKernelGGL(bottom_lds_kernel, dim3(1), dim3(kBottomThreads), 0, s, level_set(sim), bottom, done);
+        else
+            hipLaunchKernelGGL(bottom_kernel, dim3(1), dim3(kBottomThreads), 0, s, level_set(sim), bottom, sim.cg_s,
+                               sim.cg_ap, sim.cg_r, done);
         for (int l = bottom - 1; l >= 1; --l) {
             Prolong P{};
             P.C = sim.levels[l + 1]; P.has = 1;
