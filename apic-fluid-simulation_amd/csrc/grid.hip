@@ -2,8 +2,9 @@
 //
 //  gradient_kernel     apply_pressure_gradient (src/apic.rs:1958-2055), fused
 //                      with the post-force add of Apic::update (:2453-2458)
-//  extrapolate_kernel  one Jacobi pass of extrapolate_velocity (:2266-2350)
-//                      over u and v faces at once; four ping-pong launches
+//  extrapolate_kernel  extrapolate_velocity (:2266-2350): the four Jacobi
+//                      passes of both components in one launch, on LDS tiles
+//                      with a four-face halo (each pass needs one more ring)
 //  radial_force_kernel add_radial_force (:2352-2410)
 //  open_faces_kernel   update_world's open-face masks (:918-944)
 #include "apic_internal.hpp"
@@ -88,44 +89,71 @@ __global__ void post_force_kernel(float* __restrict__ mac_u, float* __restrict__
     if (f < nv) mac_v[f] += post_v[f] * dt;
 }
 
-__global__ void extrapolate_kernel(const float* __restrict__ in_u, const uint8_t* __restrict__ in_vu,
-                                   float* __restrict__ out_u, uint8_t* __restrict__ out_vu,
-                                   const float* __restrict__ in_v, const uint8_t* __restrict__ in_vv,
-                                   float* __restrict__ out_v, uint8_t* __restrict__ out_vv, uint32_t W, uint32_t H) {
-    const size_t nu = (size_t)W * H + H, nv = (size_t)W * H + W;
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nu) {
-        const size_t W1 = W + 1;
-        size_t x = i % W1, y = i / W1;
-        if (in_vu[i]) {
-            out_u[i] = in_u[i];
-            out_vu[i] = 1;
-        } else {
-            float sum = 0.0f;
-            uint32_t cnt = 0;
-            if (x > 0 && in_vu[i - 1]) { sum += in_u[i - 1]; ++cnt; }
-            if (x < W && in_vu[i + 1]) { sum += in_u[i + 1]; ++cnt; }
-            if (y > 0 && in_vu[i - W1]) { sum += in_u[i - W1]; ++cnt; }
-            if (y < H - 1 && in_vu[i + W1]) { sum += in_u[i + W1]; ++cnt; }
-            if (cnt > 0) { out_u[i] = sum / (float)cnt; out_vu[i] = 1; }
-            else { out_u[i] = in_u[i]; out_vu[i] = 0; }
-        }
+constexpr int kExTX = 64, kExTY = 16, kExHalo = 4;  // four passes need a four-face halo
+constexpr int kExLW = kExTX + 2 * kExHalo, kExLH = kExTY + 2 * kExHalo;
+
+// One tile of one component (nx x ny faces: u is (W+1) x H, v is W x (H+1)).
+// Pass p recomputes the faces at least p + 1 faces inside the loaded region,
+// so after four passes the tile itself holds exactly what four whole-grid
+// passes give.  Per face: a valid face keeps its value; an invalid one takes
+// the mean of its valid neighbours, summed left, right, below, above
+// (src/apic.rs:2276-2310), and becomes valid.
+__global__ __launch_bounds__(256) void extrapolate_kernel(const float* __restrict__ in_u, const uint8_t* __restrict__ in_vu,
+                                                          float* __restrict__ out_u, uint8_t* __restrict__ out_vu,
+                                                          const float* __restrict__ in_v, const uint8_t* __restrict__ in_vv,
+                                                          float* __restrict__ out_v, uint8_t* __restrict__ out_vv,
+                                                          uint32_t W, uint32_t H, uint32_t tiles_u_x, uint32_t tiles_u,
+                                                          uint32_t tiles_v_x) {
+    __shared__ float val[2][kExLH * kExLW];
+    __shared__ uint8_t ok[2][kExLH * kExLW];
+    const bool is_u = blockIdx.x < tiles_u;
+    const uint32_t b = is_u ? blockIdx.x : blockIdx.x - tiles_u;
+    const uint32_t tx = is_u ? tiles_u_x : tiles_v_x;
+    const int32_t nx = (int32_t)(is_u ? W + 1 : W), ny = (int32_t)(is_u ? H : H + 1);
+    const float* __restrict__ in = is_u ? in_u : in_v;
+    const uint8_t* __restrict__ in_ok = is_u ? in_vu : in_vv;
+    float* __restrict__ out = is_u ? out_u : out_v;
+    uint8_t* __restrict__ out_ok = is_u ? out_vu : out_vv;
+    const int32_t X0 = (int32_t)(b % tx) * kExTX - kExHalo, Y0 = (int32_t)(b / tx) * kExTY - kExHalo;
+    for (int i = threadIdx.x; i < kExLH * kExLW; i += blockDim.x) {
+        const int32_t gx = X0 + i % kExLW, gy = Y0 + i / kExLW;
+        const bool inside = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+        const size_t g = (size_t)gy * nx + gx;
+        val[0][i] = inside ? in[g] : 0.0f;
+        ok[0][i] = inside ? in_ok[g] : 0;
     }
-    if (i < nv) {
-        size_t x = i % W, y = i / W;
-        if (in_vv[i]) {
-            out_v[i] = in_v[i];
-            out_vv[i] = 1;
-        } else {
-            float sum = 0.0f;
-            uint32_t cnt = 0;
-            if (x > 0 && in_vv[i - 1]) { sum += in_v[i - 1]; ++cnt; }
-            if (x < W - 1 && in_vv[i + 1]) { sum += in_v[i + 1]; ++cnt; }
-            if (y > 0 && in_vv[i - W]) { sum += in_v[i - W]; ++cnt; }
-            if (y < H && in_vv[i + W]) { sum += in_v[i + W]; ++cnt; }
-            if (cnt > 0) { out_v[i] = sum / (float)cnt; out_vv[i] = 1; }
-            else { out_v[i] = in_v[i]; out_vv[i] = 0; }
+    __syncthreads();
+#pragma unroll 1
+    for (int p = 0; p < 4; ++p) {
+        const int src = p & 1, dst = src ^ 1;
+        for (int i = threadIdx.x; i < kExLH * kExLW; i += blockDim.x) {
+            const int lx = i % kExLW, ly = i / kExLW;
+            if (lx <= p || lx >= kExLW - 1 - p || ly <= p || ly >= kExLH - 1 - p) continue;
+            const int32_t gx = X0 + lx, gy = Y0 + ly;
+            if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+            if (ok[src][i]) {
+                val[dst][i] = val[src][i];
+                ok[dst][i] = 1;
+            } else {
+                float sum = 0.0f;
+                uint32_t cnt = 0;
+                if (gx > 0 && ok[src][i - 1]) { sum += val[src][i - 1]; ++cnt; }
+                if (gx < nx - 1 && ok[src][i + 1]) { sum += val[src][i + 1]; ++cnt; }
+                if (gy > 0 && ok[src][i - kExLW]) { sum += val[src][i - kExLW]; ++cnt; }
+                if (gy < ny - 1 && ok[src][i + kExLW]) { sum += val[src][i + kExLW]; ++cnt; }
+                if (cnt > 0) { val[dst][i] = sum / (float)cnt; ok[dst][i] = 1; }
+                else { val[dst][i] = val[src][i]; ok[dst][i] = 0; }
+            }
         }
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < kExTY * kExTX; i += blockDim.x) {
+        const int lx = kExHalo + i % kExTX, ly = kExHalo + i / kExTX;
+        const int32_t gx = X0 + lx, gy = Y0 + ly;
+        if (gx >= nx || gy >= ny) continue;
+        const size_t g = (size_t)gy * nx + gx;
+        out[g] = val[0][ly * kExLW + lx];
+        out_ok[g] = ok[0][ly * kExLW + lx];
     }
 }
 
@@ -208,20 +236,16 @@ void launch_post_force(Sim& sim, float dt) {
 }
 
 void launch_extrapolate(Sim& sim) {
-    size_t n = std::max(sim.nu(), sim.nv());
-    for (int pass = 0; pass < 4; ++pass) {
-        bool fwd = (pass % 2) == 0;
-        float* iu = fwd ? sim.mac_u : sim.old_u;
-        float* ou = fwd ? sim.old_u : sim.mac_u;
-        uint8_t* ivu = fwd ? sim.valid_u : sim.nvalid_u;
-        uint8_t* ovu = fwd ? sim.nvalid_u : sim.valid_u;
-        float* iv = fwd ? sim.mac_v : sim.old_v;
-        float* ov = fwd ? sim.old_v : sim.mac_v;
-        uint8_t* ivv = fwd ? sim.valid_v : sim.nvalid_v;
-        uint8_t* ovv = fwd ? sim.nvalid_v : sim.valid_v;
-        hipLaunchKernelGGL(extrapolate_kernel, dim3(blocks_for(n)), dim3(256), 0, sim.stream, iu, ivu, ou, ovu, iv,
-                           ivv, ov, ovv, sim.W, sim.H);
-    }
+    // into the spare buffers, which then become the current ones
+    const uint32_t tux = (sim.W + 1 + kExTX - 1) / kExTX, tuy = (sim.H + kExTY - 1) / kExTY;
+    const uint32_t tvx = (sim.W + kExTX - 1) / kExTX, tvy = (sim.H + 1 + kExTY - 1) / kExTY;
+    hipLaunchKernelGGL(extrapolate_kernel, dim3(tux * tuy + tvx * tvy), dim3(256), 0, sim.stream, sim.mac_u,
+                       sim.valid_u, sim.old_u, sim.nvalid_u, sim.mac_v, sim.valid_v, sim.old_v, sim.nvalid_v, sim.W,
+                       sim.H, tux, tux * tuy, tvx);
+    std::swap(sim.mac_u, sim.old_u);
+    std::swap(sim.mac_v, sim.old_v);
+    std::swap(sim.valid_u, sim.nvalid_u);
+    std::swap(sim.valid_v, sim.nvalid_v);
 }
 
 void launch_radial_force(Sim& sim, float wx, float wy, float radius, float strength) {
