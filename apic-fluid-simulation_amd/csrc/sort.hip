@@ -3,15 +3,20 @@
 //
 // The reference sorts (key, index) records with rdst's unstable radix sort;
 // this one is stable, so equal keys keep ascending particle index and the
-// permutation equals std::stable_sort's.  Each pass ranks one 8-bit digit:
-//   upsweep   per-tile digit histogram in LDS  -> counts[digit][tile]
-//   scan      exclusive prefix over the digit-major counts (3 kernels)
-//   downsweep per-tile stable ranks: 64-lane ballots give each lane the mask
-//             of lower lanes holding the same digit, LDS carries per-wave and
-//             running per-digit counts, and the record is scattered to
-//             offset[digit][tile] + rank.
-// Only the digits that can be nonzero are sorted: keys are Morton codes of
-// cells clamped into the grid, so key <= morton(W-1, H-1).
+// permutation equals std::stable_sort's.  Keys are Morton codes of cells
+// clamped into the grid (key <= morton(W-1, H-1)), so only that many bits are
+// sorted, in as few passes of at most 11-bit digits as cover them (9 + 11
+// bits for a 1024^2 grid).  Each pass:
+//   upsweep   per-tile digit histogram in LDS, written as one contiguous row
+//             counts[tile][digit]
+//   scan      per digit, the exclusive prefix over tiles in chunks of 32
+//             tiles (coalesced along the rows), then one workgroup scans the
+//             chunk totals and the digit totals
+//   downsweep each wave ranks its own 1024 consecutive records (ballot peer
+//             groups and a wave-private LDS count per digit, no workgroup
+//             barrier per record), the workgroup adds the counts of lower
+//             waves, stages the tile in LDS in digit order and writes it out
+//             as runs: record -> start(digit, tile) + rank.
 #include "apic_internal.hpp"
 #include "device_math.hpp"
 
@@ -20,13 +25,14 @@ namespace apic {
 namespace {
 
 constexpr int kRadixThreads = 256;   // 4 waves of 64
+constexpr int kRadixWaves = kRadixThreads / 64;
 constexpr int kRadixItems = 16;      // keys per thread per tile
 constexpr int kRadixTile = kRadixThreads * kRadixItems;
-constexpr int kRadixBits = 8;
-constexpr int kRadixBuckets = 1 << kRadixBits;
+constexpr int kRadixWaveItems = kRadixTile / kRadixWaves;
+constexpr int kRadixMaxBits = 11;
+constexpr int kRadixMaxBuckets = 1 << kRadixMaxBits;
 constexpr int kScanThreads = 1024;
-constexpr int kScanItems = 4;
-constexpr int kScanTile = kScanThreads * kScanItems;
+constexpr int kScanChunkTiles = 32;  // tiles per chunk of the two-level column scan
 
 __global__ void keygen_kernel(const float* __restrict__ px, const float* __restrict__ py, uint32_t n,
                               float cell, float max_x, float max_y, uint32_t* __restrict__ keys,
@@ -40,21 +46,67 @@ __global__ void keygen_kernel(const float* __restrict__ px, const float* __restr
     vals[i] = i;
 }
 
+// Lanes of the wave whose digit equals this lane's (valid lanes only).  When
+// the wave's digits form few runs (the usual case for Morton-sorted particles
+// that moved little since the last sort), one ballot per distinct digit;
+// otherwise one ballot per digit bit.
+__device__ inline uint64_t digit_peers(uint32_t d, bool valid, int bits) {
+    const int lane = (int)(threadIdx.x & 63);
+    const uint32_t prev = (uint32_t)__shfl((int)d, (lane + 63) & 63, 64);
+    const uint64_t act = __ballot(valid);
+    const int runs = __popcll(__ballot(valid && (lane == 0 || prev != d)));
+    if (runs <= 12) {
+        uint64_t remaining = act, peers = 0;
+        while (remaining) {
+            const int leader = (int)__builtin_ctzll(remaining);
+            const uint32_t v = (uint32_t)__builtin_amdgcn_readlane((int)d, leader);
+            const uint64_t m = __ballot(d == v) & remaining;
+            if (valid && d == v) peers = m;
+            remaining &= ~m;
+        }
+        return peers;
+    }
+    uint64_t peers = act;
+    for (int b = 0; b < bits; ++b) {
+        const bool bit = (d >> b) & 1u;
+        const uint64_t bal = __ballot(bit);
+        peers &= bit ? bal : ~bal;
+    }
+    return valid ? peers : 0ull;
+}
+
+// A wave whose digits are all equal adds its count once; otherwise each lane
+// adds itself (LDS atomics).
 __global__ __launch_bounds__(kRadixThreads) void radix_upsweep(const uint32_t* __restrict__ keys, uint32_t n,
-                                                               int shift, uint32_t* __restrict__ counts,
-                                                               uint32_t ntiles) {
-    __shared__ uint32_t hist[kRadixBuckets];
-    const int tid = threadIdx.x;
-    hist[tid] = 0;
-    __syncthreads();
-    size_t base = (size_t)blockIdx.x * kRadixTile;
+                                                               int shift, int bits, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[kRadixMaxBuckets];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const uint32_t nb = 1u << bits, mask = nb - 1u;
+    for (uint32_t d = tid; d < nb; d += kRadixThreads) hist[d] = 0;
+    const size_t base = (size_t)blockIdx.x * kRadixTile;
+    uint32_t k[kRadixItems];
 #pragma unroll
     for (int j = 0; j < kRadixItems; ++j) {
-        size_t idx = base + (size_t)j * kRadixThreads + tid;
-        if (idx < n) atomicAdd(&hist[(keys[idx] >> shift) & (kRadixBuckets - 1)], 1u);
+        const size_t idx = base + (size_t)j * kRadixThreads + tid;
+        k[j] = idx < n ? keys[idx] : 0u;
     }
     __syncthreads();
-    counts[(size_t)tid * ntiles + blockIdx.x] = hist[tid];
+#pragma unroll
+    for (int j = 0; j < kRadixItems; ++j) {
+        const size_t idx = base + (size_t)j * kRadixThreads + tid;
+        const bool valid = idx < n;
+        const uint32_t d = (k[j] >> shift) & mask;
+        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+        const uint64_t act = __ballot(valid);
+        const uint64_t same = __ballot(valid && d == d0);
+        if (same == act) {
+            if (act != 0 && lane == (int)__builtin_ctzll(act)) atomicAdd(&hist[d0], (uint32_t)__popcll(act));
+        } else if (valid) {
+            atomicAdd(&hist[d], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t d = tid; d < nb; d += kRadixThreads) counts[(size_t)blockIdx.x * nb + d] = hist[d];
 }
 
 // Block-wide exclusive scan of one value per thread (1024 threads = 16 waves).
@@ -81,94 +133,171 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* wave_sums,
     return r;
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_tiles(uint32_t* __restrict__ data, size_t n,
-                                                           uint32_t* __restrict__ tile_sums) {
-    __shared__ uint32_t wave_sums[kScanThreads / 64 + 1];
-    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
-    uint32_t v[kScanItems];
-    uint32_t local = 0;
+// Column scan, level 1: for each chunk of kScanChunkTiles tiles and each
+// digit, the exclusive prefix of the tile counts within the chunk (in place)
+// and the chunk's total.  Threads of a block take consecutive digits, so every
+// access is coalesced along the tile-major rows.
+__global__ __launch_bounds__(256) void scan_columns(uint32_t* __restrict__ counts, uint32_t ntiles, uint32_t nb,
+                                                    uint32_t* __restrict__ chunk_sum) {
+    const uint32_t chunk = blockIdx.x;
+    const uint32_t t0 = chunk * kScanChunkTiles, t1 = min(t0 + (uint32_t)kScanChunkTiles, ntiles);
+    const uint32_t d = blockIdx.y * blockDim.x + threadIdx.x;
+    if (d >= nb) return;
+    uint32_t c[kScanChunkTiles];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = base + k < n ? data[base + k] : 0u;
-        local += v[k];
+    for (int q = 0; q < kScanChunkTiles; ++q) c[q] = t0 + q < t1 ? counts[(size_t)(t0 + q) * nb + d] : 0u;
+    uint32_t run = 0;
+#pragma unroll
+    for (int q = 0; q < kScanChunkTiles; ++q) {
+        if (t0 + q < t1) counts[(size_t)(t0 + q) * nb + d] = run;
+        run += c[q];
+    }
+    chunk_sum[(size_t)chunk * nb + d] = run;
+}
+
+// Column scan, level 2 (one workgroup): per digit the exclusive prefix over
+// chunks (in place), and each digit's start (exclusive scan over digits of
+// the digit totals).  A thread takes up to two consecutive digits.
+__global__ __launch_bounds__(kScanThreads) void scan_chunks(uint32_t* __restrict__ chunk_sum, uint32_t nchunks,
+                                                            uint32_t nb, uint32_t* __restrict__ digit_start) {
+    __shared__ uint32_t wave_sums[kScanThreads / 64 + 1];
+    const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;  // 1 or 2
+    const uint32_t d0 = threadIdx.x * per;
+    uint32_t run[2] = {0u, 0u};
+    for (uint32_t c0 = 0; c0 < nchunks; c0 += 16) {
+        uint32_t v[2][16];
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                const bool ok = (uint32_t)q < per && d0 + q < nb && c0 + c < nchunks;
+                v[q][c] = ok ? chunk_sum[(size_t)(c0 + c) * nb + d0 + q] : 0u;
+            }
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                if ((uint32_t)q < per && d0 + q < nb && c0 + c < nchunks) chunk_sum[(size_t)(c0 + c) * nb + d0 + q] = run[q];
+                run[q] += v[q][c];
+            }
     }
     uint32_t total;
-    uint32_t off = block_exclusive_scan(local, wave_sums, total);
+    uint32_t start = block_exclusive_scan(run[0] + run[1], wave_sums, total);
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < n) data[base + k] = off;
-        off += v[k];
-    }
-    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
-}
-
-// Exclusive scan of the tile sums by one block (looping over 1024-wide chunks).
-__global__ __launch_bounds__(kScanThreads) void scan_tile_sums(uint32_t* __restrict__ tile_sums, uint32_t n) {
-    __shared__ uint32_t wave_sums[kScanThreads / 64 + 1];
-    uint32_t carry = 0;
-    for (uint32_t b = 0; b < n; b += kScanThreads) {
-        uint32_t i = b + threadIdx.x;
-        uint32_t v = i < n ? tile_sums[i] : 0u;
-        uint32_t total;
-        uint32_t off = block_exclusive_scan(v, wave_sums, total);
-        if (i < n) tile_sums[i] = carry + off;
-        carry += total;
+    for (int q = 0; q < 2; ++q) {
+        if ((uint32_t)q < per && d0 + q < nb) digit_start[d0 + q] = start;
+        start += run[q];
     }
 }
 
-__global__ __launch_bounds__(kScanThreads) void scan_add(uint32_t* __restrict__ data, size_t n,
-                                                         const uint32_t* __restrict__ tile_sums) {
-    size_t base = (size_t)blockIdx.x * kScanTile + (size_t)threadIdx.x * kScanItems;
-    uint32_t add = tile_sums[blockIdx.x];
-#pragma unroll
-    for (int k = 0; k < kScanItems; ++k)
-        if (base + k < n) data[base + k] += add;
-}
-
+// prefix[tile][digit] + chunk_base[chunk of tile][digit] + digit_start[digit]
+// is the tile's first destination for the digit.
+// After ranking, the tile is reordered by digit in LDS, so that consecutive
+// threads write consecutive destinations of one digit.
 __global__ __launch_bounds__(kRadixThreads) void radix_downsweep(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
-    uint32_t* __restrict__ vals_out, uint32_t n, int shift, const uint32_t* __restrict__ offsets, uint32_t ntiles) {
-    constexpr int kWaves = kRadixThreads / 64;
-    __shared__ uint32_t goff[kRadixBuckets];
-    __shared__ uint32_t run[kRadixBuckets];
-    __shared__ uint32_t whist[kWaves][kRadixBuckets];
+    uint32_t* __restrict__ vals_out, uint32_t n, int shift, int bits, const uint32_t* __restrict__ prefix,
+    const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ digit_start) {
+    constexpr int kPerThread = kRadixMaxBuckets / kRadixThreads;  // digits per thread in the tile scan
+    __shared__ uint32_t whist[kRadixWaves][kRadixMaxBuckets];      // per-wave counts, then the staged tile
+    __shared__ uint32_t gdelta[kRadixMaxBuckets];                  // global start - tile start, per digit
+    __shared__ uint32_t wsum[kRadixWaves + 1];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    goff[tid] = offsets[(size_t)tid * ntiles + blockIdx.x];
-    run[tid] = 0;
+    const uint32_t nb = 1u << bits, mask = nb - 1u;
+    for (uint32_t d = tid; d < nb; d += kRadixThreads) {
 #pragma unroll
-    for (int w = 0; w < kWaves; ++w) whist[w][tid] = 0;
-    __syncthreads();
-    const uint64_t lanes_below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-    size_t base = (size_t)blockIdx.x * kRadixTile;
+        for (int w = 0; w < kRadixWaves; ++w) whist[w][d] = 0;
+    }
+    const size_t tile0 = (size_t)blockIdx.x * kRadixTile;
+    const size_t base = tile0 + (size_t)wave * kRadixWaveItems;
+    uint32_t k[kRadixItems], v[kRadixItems], r[kRadixItems];
+#pragma unroll
     for (int j = 0; j < kRadixItems; ++j) {
-        size_t idx = base + (size_t)j * kRadixThreads + tid;
-        bool valid = idx < n;
-        uint32_t k = valid ? keys_in[idx] : 0u;
-        uint32_t v = valid ? vals_in[idx] : 0u;
-        uint32_t d = (k >> shift) & (kRadixBuckets - 1);
-        uint64_t peers = __ballot(valid);
+        const size_t idx = base + (size_t)j * 64 + lane;
+        const bool valid = idx < n;
+        k[j] = valid ? keys_in[idx] : 0u;
+        v[j] = valid ? vals_in[idx] : 0u;
+    }
+    __syncthreads();
+    // ranks within the wave, in record order
+    const uint64_t lanes_below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t* wh = whist[wave];
 #pragma unroll
-        for (int b = 0; b < kRadixBits; ++b) {
-            bool bit = (d >> b) & 1u;
-            uint64_t bal = __ballot(bit);
-            peers &= bit ? bal : ~bal;
-        }
-        uint32_t rank = (uint32_t)__popcll(peers & lanes_below);
-        if (valid && rank == 0) whist[wave][d] = (uint32_t)__popcll(peers);
-        __syncthreads();
-        if (valid) {
-            uint32_t pos = run[d] + rank;
-            for (int w = 0; w < wave; ++w) pos += whist[w][d];
-            uint32_t dst = goff[d] + pos;
-            keys_out[dst] = k;
-            vals_out[dst] = v;
-        }
-        __syncthreads();
-        uint32_t add = 0;
+    for (int j = 0; j < kRadixItems; ++j) {
+        const size_t idx = base + (size_t)j * 64 + lane;
+        const bool valid = idx < n;
+        const uint32_t d = (k[j] >> shift) & mask;
+        const uint64_t peers = digit_peers(d, valid, bits);
+        const uint32_t below = (uint32_t)__popcll(peers & lanes_below);
+        const uint32_t before = valid ? wh[d] : 0u;
+        r[j] = before + below;
+        __builtin_amdgcn_wave_barrier();
+        if (valid && below == 0) wh[d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // tile-local start of every digit (exclusive scan over digits of the
+    // tile's totals); whist[w][d] becomes the tile-local start of wave w's
+    // records of digit d
+    uint32_t tot[kPerThread];
+    uint32_t local = 0;
 #pragma unroll
-        for (int w = 0; w < kWaves; ++w) { add += whist[w][tid]; whist[w][tid] = 0; }
-        run[tid] += add;
-        __syncthreads();
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t d = (uint32_t)tid * kPerThread + q;
+        uint32_t t = 0;
+        if (d < nb) {
+#pragma unroll
+            for (int w = 0; w < kRadixWaves; ++w) t += whist[w][d];
+        }
+        tot[q] = t;
+        local += t;
+    }
+    uint32_t incl = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    uint32_t start = incl - local;
+    for (int w = 0; w < wave; ++w) start += wsum[w];
+#pragma unroll
+    for (int q = 0; q < kPerThread; ++q) {
+        const uint32_t d = (uint32_t)tid * kPerThread + q;
+        if (d < nb) {
+            gdelta[d] = prefix[(size_t)blockIdx.x * nb + d] +
+                        chunk_base[(size_t)(blockIdx.x / kScanChunkTiles) * nb + d] + digit_start[d] - start;
+            uint32_t run = start;
+#pragma unroll
+            for (int w = 0; w < kRadixWaves; ++w) {
+                const uint32_t c = whist[w][d];
+                whist[w][d] = run;
+                run += c;
+            }
+        }
+        start += tot[q];
+    }
+    __syncthreads();
+    uint32_t pos[kRadixItems];
+#pragma unroll
+    for (int j = 0; j < kRadixItems; ++j) pos[j] = wh[(k[j] >> shift) & mask] + r[j];
+    __syncthreads();
+    // stage the tile in digit order (the count arrays are free now)
+    uint32_t* sk = &whist[0][0];
+    uint32_t* sv = &whist[kRadixWaves / 2][0];
+#pragma unroll
+    for (int j = 0; j < kRadixItems; ++j) {
+        const size_t idx = base + (size_t)j * 64 + lane;
+        if (idx < n) { sk[pos[j]] = k[j]; sv[pos[j]] = v[j]; }
+    }
+    __syncthreads();
+    const uint32_t count = (uint32_t)min((size_t)kRadixTile, (size_t)n - tile0);
+    for (uint32_t t = tid; t < count; t += kRadixThreads) {
+        const uint32_t key = sk[t];
+        const uint32_t dst = gdelta[(key >> shift) & mask] + t;
+        keys_out[dst] = key;
+        vals_out[dst] = sv[t];
     }
 }
 
@@ -199,9 +328,8 @@ uint32_t key_bits(const Sim& sim) {
 
 size_t radix_scratch_words(size_t n) {
     size_t ntiles = (n + kRadixTile - 1) / kRadixTile;
-    size_t counts = (size_t)kRadixBuckets * ntiles;
-    size_t scan_tiles_n = (counts + kScanTile - 1) / kScanTile;
-    return counts + scan_tiles_n + 64;
+    size_t nchunks = (ntiles + kScanChunkTiles - 1) / kScanChunkTiles;
+    return (size_t)kRadixMaxBuckets * (ntiles + nchunks + 1) + 64;
 }
 
 void launch_keygen(Sim& sim) {
@@ -215,23 +343,30 @@ void launch_keygen(Sim& sim) {
 void launch_radix_sort(Sim& sim) {
     uint32_t n = sim.num_particles;
     if (n <= 1) return;
-    uint32_t ntiles = (n + kRadixTile - 1) / kRadixTile;
-    size_t ncounts = (size_t)kRadixBuckets * ntiles;
-    uint32_t nscan = (uint32_t)((ncounts + kScanTile - 1) / kScanTile);
-    uint32_t* counts = sim.radix_scratch;
-    uint32_t* tile_sums = counts + ncounts;
-    int passes = (int)((key_bits(sim) + kRadixBits - 1) / kRadixBits);
-    int src = 0;
+    // Passes of at most kRadixMaxBits bits; the first (least significant)
+    // pass takes the remainder, so the later passes get the widest digits.
+    const int total_bits = (int)key_bits(sim);
+    const int passes = (total_bits + kRadixMaxBits - 1) / kRadixMaxBits;
+    const uint32_t ntiles = (n + kRadixTile - 1) / kRadixTile;
+    const uint32_t nchunks = (ntiles + kScanChunkTiles - 1) / kScanChunkTiles;
+    uint32_t* counts = sim.radix_scratch;                               // [tile][digit]
+    uint32_t* chunk_sum = counts + (size_t)kRadixMaxBuckets * ntiles;   // [chunk][digit]
+    uint32_t* digit_start = chunk_sum + (size_t)kRadixMaxBuckets * nchunks;
+    int src = 0, shift = 0;
     for (int p = 0; p < passes; ++p) {
-        int shift = p * kRadixBits;
+        const int bits = p == 0 ? total_bits - (passes - 1) * kRadixMaxBits : kRadixMaxBits;
+        const uint32_t nb = 1u << bits;
         hipLaunchKernelGGL(radix_upsweep, dim3(ntiles), dim3(kRadixThreads), 0, sim.stream, sim.keys[src], n, shift,
-                           counts, ntiles);
-        hipLaunchKernelGGL(scan_tiles, dim3(nscan), dim3(kScanThreads), 0, sim.stream, counts, ncounts, tile_sums);
-        hipLaunchKernelGGL(scan_tile_sums, dim3(1), dim3(kScanThreads), 0, sim.stream, tile_sums, nscan);
-        hipLaunchKernelGGL(scan_add, dim3(nscan), dim3(kScanThreads), 0, sim.stream, counts, ncounts, tile_sums);
+                           bits, counts);
+        hipLaunchKernelGGL(scan_columns, dim3(nchunks, (nb + 255) / 256), dim3(256), 0, sim.stream, counts, ntiles, nb,
+                           chunk_sum);
+        hipLaunchKernelGGL(scan_chunks, dim3(1), dim3(kScanThreads), 0, sim.stream, chunk_sum, nchunks, nb,
+                           digit_start);
         hipLaunchKernelGGL(radix_downsweep, dim3(ntiles), dim3(kRadixThreads), 0, sim.stream, sim.keys[src],
-                           sim.vals[src], sim.keys[1 - src], sim.vals[1 - src], n, shift, counts, ntiles);
+                           sim.vals[src], sim.keys[1 - src], sim.vals[1 - src], n, shift, bits, counts, chunk_sum,
+                           digit_start);
         src = 1 - src;
+        shift += bits;
     }
     if (src != 0) {
         std::swap(sim.keys[0], sim.keys[1]);
