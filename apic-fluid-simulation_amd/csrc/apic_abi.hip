@@ -68,15 +68,15 @@ apic_status dalloc(Sim& s, T*& p, size_t count) {
 
 void release(Sim& s) {
     if (s.stream) (void)hipStreamSynchronize(s.stream);
-    for (int t = 0; t < APIC_TIMER_COUNT; ++t)
-        for (int k = 0; k < 2; ++k)
-            if (s.ev[t][k]) { (void)hipEventDestroy(s.ev[t][k]); s.ev[t][k] = nullptr; }
+    s.free_events();
+    if (s.e_sync) { (void)hipEventDestroy(s.e_sync); s.e_sync = nullptr; }
     for (void* p : s.allocations) (void)hipFree(p);
     s.allocations.clear();
     if (s.h_pcg) (void)hipHostFree(s.h_pcg);
     if (s.h_maxv) (void)hipHostFree(s.h_maxv);
     if (s.h_fstats) (void)hipHostFree(s.h_fstats);
-    s.h_pcg = nullptr; s.h_maxv = nullptr; s.h_fstats = nullptr;
+    if (s.h_step) (void)hipHostFree(s.h_step);
+    s.h_pcg = nullptr; s.h_maxv = nullptr; s.h_fstats = nullptr; s.h_step = nullptr;
     if (s.stream) (void)hipStreamDestroy(s.stream);
     s.stream = nullptr;
 }
@@ -178,12 +178,27 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     ALLOC(s.part_maxv, s.part_maxv_cap);
     ALLOC(s.fstats, 1);
     ALLOC(s.maxv_bits, 1);
-    HIP_TRY(hipHostMalloc((void**)&s.h_pcg, sizeof(PcgState), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&s.h_maxv, sizeof(uint32_t), hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&s.h_fstats, sizeof(FluidStats), hipHostMallocDefault));
+    ALLOC(s.step, 1);
+    // coherent, mapped host memory: kernels write these mirrors directly
+    const unsigned hflags = hipHostMallocMapped | hipHostMallocCoherent;
+    HIP_TRY(hipHostMalloc((void**)&s.h_pcg, sizeof(PcgState), hflags));
+    HIP_TRY(hipHostMalloc((void**)&s.h_maxv, sizeof(uint32_t), hflags));
+    HIP_TRY(hipHostMalloc((void**)&s.h_fstats, sizeof(FluidStats), hflags));
+    HIP_TRY(hipHostMalloc((void**)&s.h_step, sizeof(StepState), hflags));
     std::memset(s.h_pcg, 0, sizeof(PcgState));
     *s.h_maxv = 0;
     std::memset(s.h_fstats, 0, sizeof(FluidStats));
+    std::memset(s.h_step, 0, sizeof(StepState));
+    HIP_TRY(hipHostGetDevicePointer((void**)&s.h_pcg_d, s.h_pcg, 0));
+    HIP_TRY(hipHostGetDevicePointer((void**)&s.h_maxv_d, s.h_maxv, 0));
+    HIP_TRY(hipHostGetDevicePointer((void**)&s.h_fstats_d, s.h_fstats, 0));
+    HIP_TRY(hipHostGetDevicePointer((void**)&s.h_step_d, s.h_step, 0));
+    HIP_TRY(hipEventCreateWithFlags(&s.e_sync, hipEventDisableTiming));
+    {
+        StepState st0{};
+        st0.prev_dt = s.prev_dt;  // 1/60 before the first update (:824)
+        HIP_TRY(hipMemcpy(s.step, &st0, sizeof(StepState), hipMemcpyHostToDevice));
+    }
 
     // upload particles (tail lanes of the last chunk repeat the last particle, :887-898)
     if (s.nalloc) {
@@ -216,9 +231,8 @@ float host_maxv(Sim& s) {
 
 apic_status refresh_maxv(Sim& s) {
     if (!s.maxv_valid) {
-        launch_max_velocity(s);
+        launch_max_velocity(s);  // the finalize kernel also writes the host mirror
         CHECK_LAUNCH();
-        HIP_TRY(hipMemcpyAsync(s.h_maxv, s.maxv_bits, 4, hipMemcpyDeviceToHost, s.stream));
         s.maxv_valid = true;
     }
     HIP_TRY(hipStreamSynchronize(s.stream));
@@ -254,9 +268,8 @@ apic_status do_pcg(Sim& s, bool zero_pressure) {
             launch_vcycle(s, 2);
             ++k;
         }
-        launch_pcg_decide(s, k);
+        launch_pcg_decide(s, k);  // also writes the host mirror
         CHECK_LAUNCH();
-        HIP_TRY(hipMemcpyAsync(s.h_pcg, s.pcg, sizeof(PcgState), hipMemcpyDeviceToHost, s.stream));
         HIP_TRY(hipStreamSynchronize(s.stream));
         if (s.h_pcg->done || k >= kMaxPcgIterations) break;
         check_at = k + 2;
@@ -269,15 +282,25 @@ apic_status do_pcg(Sim& s, bool zero_pressure) {
     return APIC_OK;
 }
 
-apic_status substep(Sim& s, bool sort, float dt) {
+// One substep of Apic::update (src/apic.rs:2444-2461), queued without
+// waiting for the device except once, for the pressure solve's state: the
+// substep's dt is computed on the device, and the stages after the solve are
+// queued right behind the first batch of PCG iterations, guarded by the
+// solve's done flag.  The host waits on the batch while the device goes on;
+// if the solve needs more iterations, the guarded stages have turned into
+// no-ops, the host-side bookkeeping they did is undone, and they are queued
+// again after the remaining iterations.
+apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     apic_status st;
+    s.step_src = s.step;
+    launch_substep_begin(s, first, frame_dt);
     if (sort) {
         s.mark(APIC_T_SORT, 0);
         if ((st = do_sort(s)) != APIC_OK) return st;
         s.mark(APIC_T_SORT, 1);
     }
     s.mark(APIC_T_P2G, 0);
-    launch_p2g(s, true, dt);
+    launch_p2g(s, true, 0.0f);
     s.mark(APIC_T_P2G, 1);
     s.mark(APIC_T_PRESSURE_BUILD, 0);
     launch_build_pressure(s, true);
@@ -286,24 +309,77 @@ apic_status substep(Sim& s, bool sort, float dt) {
     launch_mg_build(s);
     s.mark(APIC_T_MG_BUILD, 1);
     CHECK_LAUNCH();
+    // pressure solve: first batch as long as the previous solve
     s.mark(APIC_T_PCG, 0);
-    if ((st = do_pcg(s, false)) != APIC_OK) return st;
+    launch_pcg_reset(s);
+    launch_vcycle(s, 1);
+    int k = 0;
+    for (; k < std::max(0, s.pcg_batch) && k < kMaxPcgIterations; ++k) {
+        launch_pcg_iteration_head(s, k);
+        launch_vcycle(s, 2);
+    }
+    launch_pcg_decide(s, k);
     s.mark(APIC_T_PCG, 1);
-    s.mark(APIC_T_GRADIENT, 0);
-    launch_gradient(s, true, dt);
-    s.mark(APIC_T_GRADIENT, 1);
-    s.mark(APIC_T_EXTRAPOLATE, 0);
-    launch_extrapolate(s);
-    s.mark(APIC_T_EXTRAPOLATE, 1);
-    s.mark(APIC_T_G2P, 0);
-    launch_g2p(s, dt, true);
-    s.cells_valid = false;
-    s.mark(APIC_T_G2P, 1);
+    HIP_TRY(hipEventRecord(s.e_sync, s.stream));
     CHECK_LAUNCH();
-    HIP_TRY(hipMemcpyAsync(s.h_maxv, s.maxv_bits, 4, hipMemcpyDeviceToHost, s.stream));
-    s.keys_valid = true;
-    s.maxv_valid = true;
-    s.collect_timers();
+    // the rest of the substep, speculatively
+    struct Saved {
+        float *mac_u, *mac_v, *old_u, *old_v;
+        uint8_t *valid_u, *valid_v, *nvalid_u, *nvalid_v;
+        bool cells_valid, keys_valid, maxv_valid;
+        size_t marks;
+    } saved{s.mac_u, s.mac_v, s.old_u, s.old_v, s.valid_u, s.valid_v, s.nvalid_u, s.nvalid_v,
+            s.cells_valid, s.keys_valid, s.maxv_valid, s.ev_pending.size()};
+    auto tail = [&]() -> apic_status {
+        s.mark(APIC_T_GRADIENT, 0);
+        launch_gradient(s, true, 0.0f);
+        s.mark(APIC_T_GRADIENT, 1);
+        s.mark(APIC_T_EXTRAPOLATE, 0);
+        launch_extrapolate(s);
+        s.mark(APIC_T_EXTRAPOLATE, 1);
+        s.mark(APIC_T_G2P, 0);
+        launch_g2p(s, 0.0f, true);
+        s.mark(APIC_T_G2P, 1);
+        launch_substep_end(s);
+        s.cells_valid = false;
+        s.keys_valid = true;
+        s.maxv_valid = true;
+        CHECK_LAUNCH();
+        return APIC_OK;
+    };
+    s.tail_guard = &s.pcg->done;
+    st = tail();
+    s.tail_guard = nullptr;
+    if (st != APIC_OK) { s.step_src = nullptr; return st; }
+    HIP_TRY(hipEventSynchronize(s.e_sync));
+    if (!s.h_pcg->done && k < kMaxPcgIterations) {
+        s.mac_u = saved.mac_u; s.mac_v = saved.mac_v; s.old_u = saved.old_u; s.old_v = saved.old_v;
+        s.valid_u = saved.valid_u; s.valid_v = saved.valid_v; s.nvalid_u = saved.nvalid_u; s.nvalid_v = saved.nvalid_v;
+        s.cells_valid = saved.cells_valid; s.keys_valid = saved.keys_valid; s.maxv_valid = saved.maxv_valid;
+        s.drop_marks(saved.marks);
+        int check_at = k + 2;
+        while (true) {
+            while (k < check_at && k < kMaxPcgIterations) {
+                launch_pcg_iteration_head(s, k);
+                launch_vcycle(s, 2);
+                ++k;
+            }
+            launch_pcg_decide(s, k);
+            HIP_TRY(hipEventRecord(s.e_sync, s.stream));
+            CHECK_LAUNCH();
+            HIP_TRY(hipEventSynchronize(s.e_sync));
+            if (s.h_pcg->done || k >= kMaxPcgIterations) break;
+            check_at = k + 2;
+        }
+        if ((st = tail()) != APIC_OK) { s.step_src = nullptr; return st; }
+    }
+    s.step_src = nullptr;
+    s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
+    s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
+    s.stats.pcg_max_residual = s.h_pcg->maxerr;
+    s.stats.pcg_rel_residual = -1.0;
+    s.pcg_batch = s.h_pcg->iters;
+    s.collect_timers(false);
     return APIC_OK;
 }
 
@@ -400,20 +476,24 @@ apic_status apic_update(apic_handle* h, const apic_debug_options* opts, float fr
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
     Sim& s = h->sim;
     const bool disable_sort = opts && opts->disable_particle_sort;
+    // The substep loop of Apic::update (src/apic.rs:2431-2467).  The host keeps
+    // the same f32 sum of the dts the device computes, to know when the frame
+    // is done; it returns while the last substep may still run on the device.
     float t = 0.0f;
     uint32_t substeps = 0, total_it = 0;
     float last_maxv = 0.0f;
     apic_status st;
+    if (!s.maxv_valid) {
+        launch_max_velocity(s);
+        CHECK_LAUNCH();
+        s.maxv_valid = true;
+    }
     while (t < frame_dt) {
-        if ((st = refresh_maxv(s)) != APIC_OK) return st;
-        float maxv = host_maxv(s);
-        last_maxv = maxv;
-        float safe = maxv > 1e-5f ? (s.props.cfl * s.cell) / maxv : frame_dt;
-        float rem = frame_dt - t;
-        float dt = safe < rem ? safe : rem;  // f32::min
-        if ((st = substep(s, !disable_sort && t == 0.0f, dt)) != APIC_OK) return st;
-        t += dt;
-        s.prev_dt = dt;
+        if ((st = substep(s, !disable_sort && substeps == 0, substeps == 0, frame_dt)) != APIC_OK) return st;
+        const StepState hs = *s.h_step;  // complete: it was written before the solve the host waited for
+        last_maxv = hs.maxv;
+        t += hs.dt;
+        s.prev_dt = hs.dt;
         s.timestamp += 1;
         substeps += 1;
         total_it += s.stats.pcg_iterations;
@@ -471,20 +551,18 @@ apic_status apic_set_solver_config(apic_handle* h, const apic_solver_config* cfg
 apic_status apic_set_profiling(apic_handle* h, int enable) {
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
     Sim& s = h->sim;
-    if (enable && !s.ev[0][0]) {
-        for (int t = 0; t < APIC_TIMER_COUNT; ++t)
-            for (int k = 0; k < 2; ++k) HIP_TRY(hipEventCreate(&s.ev[t][k]));
-    }
     HIP_TRY(hipStreamSynchronize(s.stream));
-    s.profiling = enable != 0;
-    for (int t = 0; t < APIC_TIMER_COUNT; ++t) { s.timer_ms[t] = 0.0; s.timer_count[t] = 0; s.ev_used[t] = false; }
+    s.collect_timers(true);
+    s.profiling = enable < 0 ? 0 : (enable > 2 ? 2 : enable);
+    for (int t = 0; t < APIC_TIMER_COUNT; ++t) { s.timer_ms[t] = 0.0; s.timer_count[t] = 0; }
     return APIC_OK;
 }
 
 apic_status apic_get_timers(apic_handle* h, double* ms, uint32_t* counts, uint32_t n) {
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
     if (n > APIC_TIMER_COUNT) return fail(APIC_ERR_INVALID_ARGUMENT, "n exceeds APIC_TIMER_COUNT");
-    h->sim.collect_timers();
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    h->sim.collect_timers(true);
     for (uint32_t t = 0; t < n; ++t) {
         if (ms) ms[t] = h->sim.timer_ms[t];
         if (counts) counts[t] = h->sim.timer_count[t];
@@ -519,8 +597,14 @@ apic_status apic_get_timestamp(apic_handle* h, uint32_t* timestamp, float* prev_
 
 apic_status apic_set_timestamp(apic_handle* h, uint32_t timestamp, float prev_deltatime) {
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
-    h->sim.timestamp = timestamp;
-    h->sim.prev_dt = prev_deltatime;
+    Sim& s = h->sim;
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    s.timestamp = timestamp;
+    s.prev_dt = prev_deltatime;
+    StepState st{};
+    HIP_TRY(hipMemcpy(&st, s.step, sizeof(StepState), hipMemcpyDeviceToHost));
+    st.prev_dt = prev_deltatime;
+    HIP_TRY(hipMemcpy(s.step, &st, sizeof(StepState), hipMemcpyHostToDevice));
     return APIC_OK;
 }
 
@@ -646,7 +730,6 @@ apic_status apic_transfer_grid_to_particles_and_advect(apic_handle* h, float del
     launch_g2p(s, deltatime, true);
     s.cells_valid = false;
     CHECK_LAUNCH();
-    HIP_TRY(hipMemcpyAsync(s.h_maxv, s.maxv_bits, 4, hipMemcpyDeviceToHost, s.stream));
     HIP_TRY(hipStreamSynchronize(s.stream));
     s.keys_valid = true;
     s.maxv_valid = true;

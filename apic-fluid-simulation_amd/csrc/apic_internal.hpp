@@ -70,6 +70,15 @@ struct FluidStats {
     uint32_t last_raw;  // largest fluid raw index + 1 (0 when there is none)
 };
 
+// Time-step scalars of the current substep, kept on the device so that the
+// host can queue a substep before the previous one has finished.
+struct StepState {
+    float dt;        // this substep's dt: min(cfl * dx / max|v|, time left in the frame)
+    float prev_dt;   // the previous substep's dt (the temporal kernel's tau uses it)
+    float elapsed;   // time advanced in the current frame
+    float maxv;      // max |v| the dt was computed from
+};
+
 struct Sim;
 
 // ---- launchers (kernels live in sort.hip, transfer.hip, pressure.hip, grid.hip)
@@ -103,7 +112,9 @@ void launch_mg_build(Sim& sim);
 void launch_vcycle(Sim& sim, int mode);  // mode 0: plain, 1: PCG start (gamma), 2: PCG iteration (gamma')
 void launch_pcg_reset(Sim& sim);
 void launch_pcg_iteration_head(Sim& sim, int k);  // s update + A s + delta, then p/r update + max|r|
-void launch_pcg_decide(Sim& sim, int k);          // gamma test after k iterations
+void launch_pcg_decide(Sim& sim, int k);          // gamma test after k iterations; mirrors the state to the host
+void launch_substep_begin(Sim& sim, bool first, float frame_dt);  // dt of the substep, on the device
+void launch_substep_end(Sim& sim);                                // prev_dt = dt, elapsed += dt
 void launch_zero_pressure(Sim& sim);
 void launch_fluid_stats(Sim& sim);
 
@@ -165,37 +176,94 @@ struct Sim {
     uint32_t* maxv_bits = nullptr;   // max |v| over all lanes (float bits)
     bool maxv_valid = false;
 
-    // host-pinned mirrors for the per-substep reads
+    // Host-pinned mirrors, written by the kernels themselves (the *_d
+    // pointers are their device addresses), read by the host after an event.
     PcgState* h_pcg = nullptr;
     uint32_t* h_maxv = nullptr;
     FluidStats* h_fstats = nullptr;
+    StepState* h_step = nullptr;
+    PcgState* h_pcg_d = nullptr;
+    uint32_t* h_maxv_d = nullptr;
+    FluidStats* h_fstats_d = nullptr;
+    StepState* h_step_d = nullptr;
+    StepState* step = nullptr;          // device StepState
+    hipEvent_t e_sync = nullptr;        // host waits (timing disabled)
+    // Set while apic_update queues a substep: kernels take dt and prev_dt
+    // from `step_src`, and the kernels after the pressure solve return at
+    // once unless *tail_guard (the solve's done flag) is set.
+    const StepState* step_src = nullptr;
+    const int32_t* tail_guard = nullptr;
 
     apic_step_stats stats{};
     int pcg_batch = 4;
 
-    // stage timers (HIP events on `stream`)
-    bool profiling = false;
-    hipEvent_t ev[APIC_TIMER_COUNT][2] = {};
-    bool ev_used[APIC_TIMER_COUNT] = {};
+    // stage timers: pairs of HIP events on `stream`, collected once complete
+    struct TimerMark {
+        int t;
+        hipEvent_t a, b;
+    };
+    int profiling = 0;  // 0 off, 1 the transfer kernels only, 2 every timer
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<TimerMark> ev_pending;
+    size_t ev_first = 0;  // ev_pending[ev_first..] are not collected yet
+    hipEvent_t ev_open[APIC_TIMER_COUNT] = {};
     double timer_ms[APIC_TIMER_COUNT] = {};
     uint32_t timer_count[APIC_TIMER_COUNT] = {};
-    void mark(int t, int which) {
-        if (!profiling) return;
-        (void)hipEventRecord(ev[t][which], stream);
-        if (which == 1) ev_used[t] = true;
+    hipEvent_t take_event() {
+        if (!ev_pool.empty()) { hipEvent_t e = ev_pool.back(); ev_pool.pop_back(); return e; }
+        // timing only: no system-scope fence (no cache writeback around the kernels timed)
+        hipEvent_t e = nullptr;
+        (void)hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+        return e;
     }
-    void collect_timers() {
-        if (!profiling) return;
-        (void)hipStreamSynchronize(stream);
-        for (int t = 0; t < APIC_TIMER_COUNT; ++t) {
-            if (!ev_used[t]) continue;
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, ev[t][0], ev[t][1]) == hipSuccess) {
-                timer_ms[t] += ms;
-                timer_count[t] += 1;
-            }
-            ev_used[t] = false;
+    void mark(int t, int which) {
+        if (profiling == 0 || (profiling == 1 && t != APIC_T_K_P2G && t != APIC_T_K_G2P)) return;
+        hipEvent_t e = take_event();
+        (void)hipEventRecord(e, stream);
+        if (which == 0) {
+            ev_open[t] = e;
+        } else if (ev_open[t]) {
+            ev_pending.push_back(TimerMark{t, ev_open[t], e});
+            ev_open[t] = nullptr;
+        } else {
+            ev_pool.push_back(e);
         }
+    }
+    // Marks from ev_pending[from..] are dropped (their kernels were queued
+    // but will not run).
+    void drop_marks(size_t from) {
+        for (size_t i = std::max(from, ev_first); i < ev_pending.size(); ++i) {
+            ev_pool.push_back(ev_pending[i].a);
+            ev_pool.push_back(ev_pending[i].b);
+        }
+        if (ev_pending.size() > from) ev_pending.resize(std::max(from, ev_first));
+    }
+    // Adds the completed marks to the timers; with wait, waits for all.
+    void collect_timers(bool wait) {
+        while (ev_first < ev_pending.size()) {
+            TimerMark& m = ev_pending[ev_first];
+            if (wait) (void)hipEventSynchronize(m.b);
+            else if (hipEventQuery(m.b) != hipSuccess) break;
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, m.a, m.b) == hipSuccess) {
+                timer_ms[m.t] += ms;
+                timer_count[m.t] += 1;
+            }
+            ev_pool.push_back(m.a);
+            ev_pool.push_back(m.b);
+            ++ev_first;
+        }
+        if (ev_first == ev_pending.size()) { ev_pending.clear(); ev_first = 0; }
+    }
+    void free_events() {
+        for (auto& m : ev_pending) { (void)hipEventDestroy(m.a); (void)hipEventDestroy(m.b); }
+        for (size_t i = 0; i < ev_first && i < ev_pending.size(); ++i) {}
+        ev_pending.clear();
+        ev_first = 0;
+        for (int t = 0; t < APIC_TIMER_COUNT; ++t)
+            if (ev_open[t]) { (void)hipEventDestroy(ev_open[t]); ev_open[t] = nullptr; }
+        for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+        ev_pool.clear();
     }
 
     // device allocations to free

@@ -33,12 +33,39 @@ __device__ inline bool fluid_at(const uint8_t* f, int32_t i, int32_t j, uint32_t
     return f[(size_t)j * W + i] != 0;
 }
 
+// The substep's dt (the time-step rule of Apic::update, src/apic.rs:2433-2441):
+// min(cfl * dx / max|v|, frame time left), with cfl * dx / max|v| replaced by
+// the frame dt when max|v| <= 1e-5.  The host keeps the same sum of dts in f32
+// to know whether another substep follows.
+__global__ void substep_begin_kernel(StepState* ss, StepState* host, const uint32_t* __restrict__ maxv_bits,
+                                     float cfl_cell, float frame_dt, int first) {
+    StepState s = *ss;
+    if (first) s.elapsed = 0.0f;
+    const float maxv = __uint_as_float(*maxv_bits);
+    const float safe = maxv > 1e-5f ? cfl_cell / maxv : frame_dt;
+    const float rem = frame_dt - s.elapsed;
+    s.dt = safe < rem ? safe : rem;
+    s.maxv = maxv;
+    *ss = s;
+    if (host) *host = s;
+}
+
+__global__ void substep_end_kernel(StepState* ss, const int32_t* __restrict__ guard) {
+    if (guard && *guard == 0) return;
+    StepState s = *ss;
+    s.prev_dt = s.dt;
+    s.elapsed = s.elapsed + s.dt;
+    *ss = s;
+}
+
 __global__ void gradient_kernel(float* __restrict__ mac_u, float* __restrict__ mac_v, uint8_t* __restrict__ valid_u,
                                 uint8_t* __restrict__ valid_v, const float* __restrict__ pressure,
                                 const float* __restrict__ smoothed, const uint8_t* __restrict__ obstacle,
                                 const uint8_t* __restrict__ fluid, uint32_t W, uint32_t H, float inv_cell,
                                 int post, const float* __restrict__ post_u, const float* __restrict__ post_v,
-                                float dt) {
+                                float dt_arg, const StepState* __restrict__ ss, const int32_t* __restrict__ guard) {
+    if (guard && *guard == 0) return;
+    const float dt = ss ? ss->dt : dt_arg;
     const size_t nu = (size_t)W * H + H, nv = (size_t)W * H + W;
     size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (f < nu) {
@@ -103,9 +130,10 @@ __global__ __launch_bounds__(256) void extrapolate_kernel(const float* __restric
                                                           const float* __restrict__ in_v, const uint8_t* __restrict__ in_vv,
                                                           float* __restrict__ out_v, uint8_t* __restrict__ out_vv,
                                                           uint32_t W, uint32_t H, uint32_t tiles_u_x, uint32_t tiles_u,
-                                                          uint32_t tiles_v_x) {
+                                                          uint32_t tiles_v_x, const int32_t* __restrict__ guard) {
     __shared__ float val[2][kExLH * kExLW];
     __shared__ uint8_t ok[2][kExLH * kExLW];
+    if (guard && *guard == 0) return;
     const bool is_u = blockIdx.x < tiles_u;
     const uint32_t b = is_u ? blockIdx.x : blockIdx.x - tiles_u;
     const uint32_t tx = is_u ? tiles_u_x : tiles_v_x;
@@ -208,6 +236,16 @@ unsigned blocks_for(size_t n, unsigned threads = 256) { return (unsigned)((n + t
 
 }  // namespace
 
+void launch_substep_begin(Sim& sim, bool first, float frame_dt) {
+    const float cfl_cell = sim.props.cfl * sim.cell;
+    hipLaunchKernelGGL(substep_begin_kernel, dim3(1), dim3(1), 0, sim.stream, sim.step, sim.h_step_d, sim.maxv_bits,
+                       cfl_cell, frame_dt, (int)first);
+}
+
+void launch_substep_end(Sim& sim) {
+    hipLaunchKernelGGL(substep_end_kernel, dim3(1), dim3(1), 0, sim.stream, sim.step, sim.tail_guard);
+}
+
 void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {
     if (n == 0) return;
     unsigned b = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
@@ -226,7 +264,7 @@ void launch_gradient(Sim& sim, bool post_force, float dt) {
     int post = post_force && sim.forces_active;
     hipLaunchKernelGGL(gradient_kernel, dim3(blocks_for(n)), dim3(256), 0, sim.stream, sim.mac_u, sim.mac_v,
                        sim.valid_u, sim.valid_v, sim.pressure, sim.smooth_a, sim.obstacle, sim.levels[0].fluid,
-                       sim.W, sim.H, inv_cell, post, sim.post_u, sim.post_v, dt);
+                       sim.W, sim.H, inv_cell, post, sim.post_u, sim.post_v, dt, sim.step_src, sim.tail_guard);
 }
 
 void launch_post_force(Sim& sim, float dt) {
@@ -241,7 +279,7 @@ void launch_extrapolate(Sim& sim) {
     const uint32_t tvx = (sim.W + kExTX - 1) / kExTX, tvy = (sim.H + 1 + kExTY - 1) / kExTY;
     hipLaunchKernelGGL(extrapolate_kernel, dim3(tux * tuy + tvx * tvy), dim3(256), 0, sim.stream, sim.mac_u,
                        sim.valid_u, sim.old_u, sim.nvalid_u, sim.mac_v, sim.valid_v, sim.old_v, sim.nvalid_v, sim.W,
-                       sim.H, tux, tux * tuy, tvx);
+                       sim.H, tux, tux * tuy, tvx, sim.tail_guard);
     std::swap(sim.mac_u, sim.old_u);
     std::swap(sim.mac_v, sim.old_v);
     std::swap(sim.valid_u, sim.nvalid_u);

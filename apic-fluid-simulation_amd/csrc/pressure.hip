@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kThreads) void fluid_stats_kernel(const uint8_t* __
 }
 
 __global__ __launch_bounds__(kThreads) void fstats_finalize_kernel(const uint32_t* __restrict__ part_fluid,
-                                                                    FluidStats* fstats) {
+                                                                    FluidStats* fstats, FluidStats* host) {
     uint32_t cnt = 0, last = 0;
     for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) {
         cnt += part_fluid[2 * b];
@@ -161,6 +161,7 @@ __global__ __launch_bounds__(kThreads) void fstats_finalize_kernel(const uint32_
         for (int t = 0; t < (int)blockDim.x; ++t) { c += s_cnt[t]; l = max(l, s_last[t]); }
         fstats->count = c;
         fstats->last_raw = l;
+        if (host) { host->count = c; host->last_raw = l; }
     }
 }
 
@@ -821,11 +822,23 @@ __global__ __launch_bounds__(kThreads) void axpy_max_kernel(LevelView L, const f
 
 // The gamma test after k iterations, run at the host's check points so that
 // the recorded state is final when the host reads it.
-__global__ __launch_bounds__(kThreads) void decide_kernel(PcgState* st, const float* __restrict__ part_gamma, int k) {
+__global__ __launch_bounds__(kThreads) void decide_kernel(PcgState* st, const float* __restrict__ part_gamma, int k,
+                                                           PcgState* host) {
     __shared__ float scratch[kThreads / 64];
-    if (st->done) return;
-    const float g = sum_partials(part_gamma, scratch);
-    (void)gamma_exit(st, g, k);
+    if (!st->done) {
+        const float g = sum_partials(part_gamma, scratch);
+        (void)gamma_exit(st, g, k);
+    }
+    if (host && threadIdx.x == 0) {  // the scalars the host reads (not the gamma history)
+        host->done = st->done;
+        host->iters = st->iters;
+        host->exit_reason = st->exit_reason;
+        host->maxerr = st->maxerr;
+        host->delta = st->delta;
+        host->alpha = st->alpha;
+        host->bnorm2 = st->bnorm2;
+        host->rnorm2 = st->rnorm2;
+    }
 }
 
 unsigned blocks_for(size_t n, unsigned threads = kThreads) { return (unsigned)((n + threads - 1) / threads); }
@@ -860,15 +873,15 @@ void launch_build_pressure(Sim& sim, bool zero_pressure) {
     hipLaunchKernelGGL(assemble_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, sim.mac_u, sim.mac_v,
                        sim.smooth_a, fluid, sim.obstacle, sim.W, cells, sim.cell, sim.levels[0], sim.pressure,
                        (int)zero_pressure, sim.part_fluid);
-    hipLaunchKernelGGL(fstats_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.part_fluid, sim.fstats);
-    (void)hipMemcpyAsync(sim.h_fstats, sim.fstats, sizeof(FluidStats), hipMemcpyDeviceToHost, sim.stream);
+    hipLaunchKernelGGL(fstats_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.part_fluid, sim.fstats,
+                       sim.h_fstats_d);
 }
 
 void launch_fluid_stats(Sim& sim) {
     hipLaunchKernelGGL(fluid_stats_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, sim.levels[0].fluid,
                        sim.cells, sim.part_fluid);
-    hipLaunchKernelGGL(fstats_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.part_fluid, sim.fstats);
-    (void)hipMemcpyAsync(sim.h_fstats, sim.fstats, sizeof(FluidStats), hipMemcpyDeviceToHost, sim.stream);
+    hipLaunchKernelGGL(fstats_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.part_fluid, sim.fstats,
+                       sim.h_fstats_d);
 }
 
 void launch_mg_build(Sim& sim) {
@@ -944,7 +957,7 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
 }
 
 void launch_pcg_decide(Sim& sim, int k) {
-    hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.pcg, sim.part_gamma, k);
+    hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.pcg, sim.part_gamma, k, sim.h_pcg_d);
 }
 
 }  // namespace apic

@@ -148,13 +148,16 @@ __device__ inline void owner_accumulate(const float* __restrict__ sp, int k, flo
 }
 
 __global__ __launch_bounds__(kOwnCells) void p2g_owner_kernel(
-    const float* __restrict__ pb, size_t nalloc, uint32_t np, const uint32_t* __restrict__ order, float prev_dt,
+    const float* __restrict__ pb, size_t nalloc, uint32_t np, const uint32_t* __restrict__ order, float prev_dt_arg,
     float cell, uint32_t W, uint32_t H, const uint2* __restrict__ cell_range, uint32_t tiles_x,
     float* __restrict__ mac_u, float* __restrict__ mac_v, uint8_t* __restrict__ valid_u,
     uint8_t* __restrict__ valid_v, float* __restrict__ density, uint8_t* __restrict__ fluid,
     const uint8_t* __restrict__ obstacle, const float* __restrict__ ext_u, const float* __restrict__ ext_v,
-    int forces, int ext, float dt, float g) {
-    __shared__ float s_buf[kOwnFields * kOwnCap];  // staged particles, then the partials
+    int forces, int ext, float dt_arg, float gravity, const StepState* __restrict__ ss) {
+    __shared__ float s_buf[kOwnFields * kOwnCap];
+    const float prev_dt = ss ? ss->prev_dt : prev_dt_arg;
+    const float dt = ss ? ss->dt : dt_arg;
+    const float g = gravity * dt;  // staged particles, then the partials
     __shared__ uint32_t s_off[kOwnCells + 1];      // window-order prefix of the cell counts
     __shared__ uint32_t s_src[kOwnCells];          // sorted-order begin of each window cell
     __shared__ uint32_t s_wave[kOwnCells / 64];
@@ -362,11 +365,14 @@ __device__ inline float masked_interp(const float* w, const uint8_t* open, const
 }
 
 __global__ __launch_bounds__(256) void g2p_kernel(
-    float* __restrict__ pb, size_t nalloc, uint32_t np, float dt, float cell, uint32_t W, uint32_t H,
+    float* __restrict__ pb, size_t nalloc, uint32_t np, float dt_arg, float cell, uint32_t W, uint32_t H,
     uint32_t timestamp, float border_damping, const float* __restrict__ mac_u, const float* __restrict__ mac_v,
     const uint8_t* __restrict__ open_u, const uint8_t* __restrict__ open_v,
-    int write_keys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, float* __restrict__ part_maxv) {
+    int write_keys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, float* __restrict__ part_maxv,
+    const StepState* __restrict__ ss, const int32_t* __restrict__ guard) {
     __shared__ float s_max[256 / 64];
+    if (guard && *guard == 0) return;
+    const float dt = ss ? ss->dt : dt_arg;
     float speed_new = 0.0f;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nalloc; i += (size_t)gridDim.x * blockDim.x) {
         const float x = pb[(size_t)P_X * nalloc + i], y = pb[(size_t)P_Y * nalloc + i];
@@ -481,8 +487,11 @@ __global__ void max_velocity_kernel(const float* __restrict__ vx, const float* _
 
 // max over the per-workgroup maxima into the float bits read by the host
 __global__ __launch_bounds__(1024) void maxv_finalize_kernel(const float* __restrict__ part_maxv, uint32_t n,
-                                                             uint32_t* __restrict__ maxv_bits) {
+                                                             uint32_t* __restrict__ maxv_bits,
+                                                             uint32_t* __restrict__ host_bits,
+                                                             const int32_t* __restrict__ guard) {
     __shared__ float s_max[1024 / 64];
+    if (guard && *guard == 0) return;
     float m = 0.0f;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = fast_max(m, part_maxv[i]);
 #pragma unroll
@@ -493,6 +502,7 @@ __global__ __launch_bounds__(1024) void maxv_finalize_kernel(const float* __rest
         float b = 0.0f;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b = fast_max(b, s_max[w]);
         *maxv_bits = __float_as_uint(b);
+        if (host_bits) *host_bits = __float_as_uint(b);
     }
 }
 
@@ -529,13 +539,12 @@ void launch_p2g(Sim& sim, bool forces, float dt) {
         launch_cell_ranges(sim);
     }
     const uint32_t tx = (sim.W + kOwnTile - 1) / kOwnTile, ty = (sim.H + kOwnTile - 1) / kOwnTile;
-    const float g = sim.props.gravity * dt;
     const int ext = forces && sim.forces_active;
     sim.mark(APIC_T_K_P2G, 0);
     hipLaunchKernelGGL(p2g_owner_kernel, dim3(tx * ty), dim3(kOwnCells), 0, sim.stream, sim.pbuf[sim.cur], sim.nalloc,
                        sim.num_particles, order, sim.prev_dt, sim.cell, sim.W, sim.H, sim.cell_range, tx, sim.mac_u,
                        sim.mac_v, sim.valid_u, sim.valid_v, sim.density, sim.levels[0].fluid, sim.obstacle,
-                       sim.ext_u, sim.ext_v, (int)forces, ext, dt, g);
+                       sim.ext_u, sim.ext_v, (int)forces, ext, dt, sim.props.gravity, sim.step_src);
     sim.mark(APIC_T_K_P2G, 1);
 }
 
@@ -550,6 +559,7 @@ void launch_forces(Sim& sim, float dt) {
 void launch_g2p(Sim& sim, float dt, bool write_keys) {
     if (sim.nalloc == 0) {
         launch_fill_u32(sim.stream, sim.maxv_bits, 0u, 1);
+        if (sim.h_maxv) *sim.h_maxv = 0u;
         return;
     }
     const unsigned blocks = particle_blocks(sim);
@@ -557,20 +567,23 @@ void launch_g2p(Sim& sim, float dt, bool write_keys) {
     hipLaunchKernelGGL(g2p_kernel, dim3(blocks), dim3(256), 0, sim.stream, sim.pbuf[sim.cur], sim.nalloc,
                        sim.num_particles, dt, sim.cell, sim.W, sim.H, sim.timestamp, sim.props.border_damping,
                        sim.mac_u, sim.mac_v, sim.open_u, sim.open_v, (int)write_keys, sim.keys[0], sim.vals[0],
-                       sim.part_maxv);
+                       sim.part_maxv, sim.step_src, sim.tail_guard);
     sim.mark(APIC_T_K_G2P, 1);
-    hipLaunchKernelGGL(maxv_finalize_kernel, dim3(1), dim3(1024), 0, sim.stream, sim.part_maxv, blocks, sim.maxv_bits);
+    hipLaunchKernelGGL(maxv_finalize_kernel, dim3(1), dim3(1024), 0, sim.stream, sim.part_maxv, blocks, sim.maxv_bits,
+                       sim.h_maxv_d, sim.tail_guard);
 }
 
 void launch_max_velocity(Sim& sim) {
     if (sim.nalloc == 0) {
         launch_fill_u32(sim.stream, sim.maxv_bits, 0u, 1);
+        if (sim.h_maxv) *sim.h_maxv = 0u;
         return;
     }
     const unsigned blocks = particle_blocks(sim);
     hipLaunchKernelGGL(max_velocity_kernel, dim3(blocks), dim3(256), 0, sim.stream, sim.pfield(V_X),
                        sim.pfield(V_Y), sim.nalloc, sim.part_maxv);
-    hipLaunchKernelGGL(maxv_finalize_kernel, dim3(1), dim3(1024), 0, sim.stream, sim.part_maxv, blocks, sim.maxv_bits);
+    hipLaunchKernelGGL(maxv_finalize_kernel, dim3(1), dim3(1024), 0, sim.stream, sim.part_maxv, blocks, sim.maxv_bits,
+                       sim.h_maxv_d, nullptr);
 }
 
 }  // namespace apic

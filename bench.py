@@ -105,7 +105,7 @@ def main():
         g.update(opts, dt)
     print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
 
-    g.set_profiling(True)
+    g.set_profiling(1)  # HIP events around the two transfer kernels only
     g.get_max_particle_velocity()  # synchronises the library's stream
     substeps, iters, fluid = 0, [], []
     t0 = time.perf_counter()
@@ -119,6 +119,15 @@ def main():
     elapsed = time.perf_counter() - t0
     timers = g.timers()
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
+    # stage breakdown from a short separate pass with events around every stage
+    g.set_profiling(2)
+    stage_steps = min(args.steps, 20)
+    stage_substeps = 0
+    for _ in range(stage_steps):
+        g.update(opts, dt)
+        stage_substeps += g.stats()["substeps"]
+    stage_timers = g.timers()
+    g.set_profiling(0)
 
     value = n_p * substeps / elapsed
     n_alloc = g.field_length("POS_X")
@@ -132,7 +141,8 @@ def main():
                       "gbs": algo[k] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None}
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
     d = kernels[dom]
-    stages = {k: round(v[0] / max(substeps, 1), 4) for k, v in timers.items() if not k.startswith("kernel_")}
+    stages = {k: round(v[0] / max(stage_substeps, 1), 4) for k, v in stage_timers.items()
+              if not k.startswith("kernel_")}
 
     line = {
         "metric": METRIC,
