@@ -174,6 +174,8 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     ALLOC(s.pcg, 1);
     ALLOC(s.part_gamma, kRedBlocks); ALLOC(s.part_delta, kRedBlocks); ALLOC(s.part_max, kRedBlocks);
     ALLOC(s.part_fluid, 2 * kRedBlocks);
+    ALLOC(s.part_gamma_d, kRedBlocks); ALLOC(s.part_delta_d, kRedBlocks); ALLOC(s.part_rr_d, kRedBlocks);
+    ALLOC(s.part_rr2_d, kRedBlocks); ALLOC(s.pressure_d, nc);
     s.part_maxv_cap = 1024;
     ALLOC(s.part_maxv, s.part_maxv_cap);
     ALLOC(s.fstats, 1);
@@ -239,6 +241,13 @@ apic_status refresh_maxv(Sim& s) {
     return APIC_OK;
 }
 
+// ||r||_2 / ||b||_2 at the end of a relative-mode solve, else -1.
+double rel_residual(const Sim& s) {
+    if (s.solver.mode != 1) return -1.0;
+    const PcgState& p = *s.h_pcg;
+    return p.bnorm2 > 0.0 ? std::sqrt(p.rnorm2 / p.bnorm2) : 0.0;
+}
+
 apic_status do_sort(Sim& s) {
     if (s.num_particles == 0) return APIC_OK;
     if (!s.keys_valid) launch_keygen(s);
@@ -256,14 +265,14 @@ apic_status do_sort(Sim& s) {
 // a finished solve into no-ops, and the host reads the state only at the end
 // of a batch.  The first batch is as long as the previous solve.
 apic_status do_pcg(Sim& s, bool zero_pressure) {
-    launch_pcg_reset(s);
     if (zero_pressure) launch_zero_pressure(s);
+    launch_pcg_reset(s);
     launch_vcycle(s, 1);
     CHECK_LAUNCH();
     int k = 0;
     int check_at = std::max(0, s.pcg_batch);
     while (true) {
-        while (k < check_at && k < kMaxPcgIterations) {
+        while (k < check_at && k < pcg_cap(s)) {
             launch_pcg_iteration_head(s, k);
             launch_vcycle(s, 2);
             ++k;
@@ -271,13 +280,13 @@ apic_status do_pcg(Sim& s, bool zero_pressure) {
         launch_pcg_decide(s, k);  // also writes the host mirror
         CHECK_LAUNCH();
         HIP_TRY(hipStreamSynchronize(s.stream));
-        if (s.h_pcg->done || k >= kMaxPcgIterations) break;
+        if (s.h_pcg->done || k >= pcg_cap(s)) break;
         check_at = k + 2;
     }
     s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
     s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
     s.stats.pcg_max_residual = s.h_pcg->maxerr;
-    s.stats.pcg_rel_residual = -1.0;
+    s.stats.pcg_rel_residual = rel_residual(s);
     s.pcg_batch = s.h_pcg->iters;
     return APIC_OK;
 }
@@ -314,7 +323,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     launch_pcg_reset(s);
     launch_vcycle(s, 1);
     int k = 0;
-    for (; k < std::max(0, s.pcg_batch) && k < kMaxPcgIterations; ++k) {
+    for (; k < std::max(0, s.pcg_batch) && k < pcg_cap(s); ++k) {
         launch_pcg_iteration_head(s, k);
         launch_vcycle(s, 2);
     }
@@ -352,14 +361,14 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     s.tail_guard = nullptr;
     if (st != APIC_OK) { s.step_src = nullptr; return st; }
     HIP_TRY(hipEventSynchronize(s.e_sync));
-    if (!s.h_pcg->done && k < kMaxPcgIterations) {
+    if (!s.h_pcg->done && k < pcg_cap(s)) {
         s.mac_u = saved.mac_u; s.mac_v = saved.mac_v; s.old_u = saved.old_u; s.old_v = saved.old_v;
         s.valid_u = saved.valid_u; s.valid_v = saved.valid_v; s.nvalid_u = saved.nvalid_u; s.nvalid_v = saved.nvalid_v;
         s.cells_valid = saved.cells_valid; s.keys_valid = saved.keys_valid; s.maxv_valid = saved.maxv_valid;
         s.drop_marks(saved.marks);
         int check_at = k + 2;
         while (true) {
-            while (k < check_at && k < kMaxPcgIterations) {
+            while (k < check_at && k < pcg_cap(s)) {
                 launch_pcg_iteration_head(s, k);
                 launch_vcycle(s, 2);
                 ++k;
@@ -368,7 +377,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
             HIP_TRY(hipEventRecord(s.e_sync, s.stream));
             CHECK_LAUNCH();
             HIP_TRY(hipEventSynchronize(s.e_sync));
-            if (s.h_pcg->done || k >= kMaxPcgIterations) break;
+            if (s.h_pcg->done || k >= pcg_cap(s)) break;
             check_at = k + 2;
         }
         if ((st = tail()) != APIC_OK) { s.step_src = nullptr; return st; }
@@ -377,7 +386,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
     s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
     s.stats.pcg_max_residual = s.h_pcg->maxerr;
-    s.stats.pcg_rel_residual = -1.0;
+    s.stats.pcg_rel_residual = rel_residual(s);
     s.pcg_batch = s.h_pcg->iters;
     s.collect_timers(false);
     return APIC_OK;
@@ -543,7 +552,16 @@ apic_status apic_is_fluid(apic_handle* h, int32_t i, int32_t j, uint64_t* out) {
 
 apic_status apic_set_solver_config(apic_handle* h, const apic_solver_config* cfg) {
     if (!h || !cfg) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
-    if (cfg->mode != 0) return fail(APIC_ERR_INVALID_ARGUMENT, "only the reference solver rules (mode 0) are built");
+    if (cfg->mode != 0 && cfg->mode != 1) return fail(APIC_ERR_INVALID_ARGUMENT, "solver mode must be 0 or 1");
+    if (cfg->mode == 1) {
+        if (!(cfg->rel_tol > 0.0f)) return fail(APIC_ERR_INVALID_ARGUMENT, "rel_tol must be positive");
+        if (cfg->max_iterations < 1 || cfg->max_iterations > 100000)
+            return fail(APIC_ERR_INVALID_ARGUMENT, "max_iterations must be in [1, 100000]");
+        if (cfg->recompute_every < 0) return fail(APIC_ERR_INVALID_ARGUMENT, "recompute_every must be >= 0");
+        if (cfg->fp64_reductions == 0)
+            return fail(APIC_ERR_INVALID_ARGUMENT, "the relative mode runs its reductions in FP64 (fp64_reductions = 1)");
+    }
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
     h->sim.solver = *cfg;
     return APIC_OK;
 }

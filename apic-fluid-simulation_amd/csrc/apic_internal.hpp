@@ -60,7 +60,11 @@ struct PcgState {
     int32_t pad0;
     float maxerr, delta, alpha, pad1;
     float gamma[kMaxPcgIterations + 4];  // gamma[k]: the r.z used by iteration k
-    double bnorm2, rnorm2;               // relative mode (FP64)
+    // relative mode (apic_solver_config.mode 1): FP64 scalars
+    double bnorm2, rnorm2;   // ||b||^2 and the last ||r||^2
+    double tol2;             // rel_tol^2
+    double gamma_d[2];       // r.z of iteration k in gamma_d[k & 1]
+    int32_t max_iters, pad2;
 };
 
 // Base-level fluid statistics needed by the padded-list quirk of prolongate
@@ -110,7 +114,8 @@ void launch_open_faces(Sim& sim);
 void launch_build_pressure(Sim& sim, bool zero_pressure);
 void launch_mg_build(Sim& sim);
 void launch_vcycle(Sim& sim, int mode);  // mode 0: plain, 1: PCG start (gamma), 2: PCG iteration (gamma')
-void launch_pcg_reset(Sim& sim);
+void launch_pcg_reset(Sim& sim);                 // also the relative mode's ||b|| prologue
+int pcg_cap(const Sim& sim);                      // iteration cap of the current solver rules
 void launch_pcg_iteration_head(Sim& sim, int k);  // s update + A s + delta, then p/r update + max|r|
 void launch_pcg_decide(Sim& sim, int k);          // gamma test after k iterations; mirrors the state to the host
 void launch_substep_begin(Sim& sim, bool first, float frame_dt);  // dt of the substep, on the device
@@ -169,6 +174,11 @@ struct Sim {
     float* part_gamma = nullptr;     // kRedBlocks partials of r.z
     float* part_delta = nullptr;     // kRedBlocks partials of s.As
     float* part_max = nullptr;       // kRedBlocks partials of max |r|
+    double* part_gamma_d = nullptr;  // relative mode: FP64 partials of r.z, s.As and |r|^2
+    double* part_delta_d = nullptr;
+    double* part_rr_d = nullptr;     // ||r||^2 partials of the recurrence residual
+    double* part_rr2_d = nullptr;    // ... after the true-residual check
+    double* pressure_d = nullptr;    // relative mode: the pressure accumulated in FP64
     uint32_t* part_fluid = nullptr;  // kRedBlocks (count, last+1) pairs of the base fluid cells
     float* part_maxv = nullptr;      // per-workgroup max |v| of the G2P kernel
     size_t part_maxv_cap = 0;

@@ -328,15 +328,28 @@ __device__ inline bool colour_cell(const LevelView& L, uint32_t t, int color, ui
 }
 
 __device__ inline float block_max(float v, float* scratch);
+template <typename T>
+__device__ inline T block_sum(T v, T* scratch);
 
 // Zero e and the first red sweep.  In a PCG iteration this is the first kernel
 // after the residual update, so it also applies the residual test
 // (max |r| < solve_error, src/apic.rs:1930) from the axpy partials.
 __global__ __launch_bounds__(256) void red_zero_kernel(LevelView L, PcgState* st, const float* __restrict__ part_max,
-                                                       int check, float solve_error) {
+                                                       int check, float solve_error,
+                                                       const double* __restrict__ part_rr) {
     __shared__ float scratch[256 / 64];
+    __shared__ double dscratch[256 / 64];
     if (st->done) return;
-    if (check) {
+    if (check && part_rr) {  // relative mode: ||r||^2 <= rel_tol^2 ||b||^2
+        double rr = 0.0;
+        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) rr += part_rr[i];
+        rr = block_sum(rr, dscratch);
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->rnorm2 = rr;
+        if (rr <= st->tol2 * st->bnorm2) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 3; }
+            return;
+        }
+    } else if (check) {
         float m = 0.0f;
         for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) m = fast_max(m, part_max[i]);
         m = block_max(m, scratch);
@@ -408,17 +421,26 @@ __device__ inline float max_partials(const float* __restrict__ part, float* scra
 // non-fluid cells have e = 0, so they add nothing.  mode 0: V-cycle alone.
 __global__ __launch_bounds__(kThreads) void base_red_up_dot_kernel(LevelView L, Prolong P, int mode,
                                                                     const int32_t* __restrict__ done,
-                                                                    float* __restrict__ part_gamma) {
+                                                                    float* __restrict__ part_gamma,
+                                                                    double* __restrict__ part_gamma_d) {
     __shared__ float scratch[kThreads / 64];
+    __shared__ double dscratch[kThreads / 64];
     if (*done) return;
     float prod = 0.0f;
+    double dprod = 0.0;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
         uint32_t x = c % L.w, y = c / L.w;
         if (((x + y) & 1u) == 0) sor_cell(L, P, c, 2);
         size_t pc = (size_t)c + L.w;
-        prod += L.r[pc] * L.e[pc];
+        if (part_gamma_d) dprod += (double)L.r[pc] * (double)L.e[pc];
+        else prod += L.r[pc] * L.e[pc];
     }
     if (mode == 0) return;
+    if (part_gamma_d) {
+        double bs = block_sum(dprod, dscratch);
+        if (threadIdx.x == 0) part_gamma_d[blockIdx.x] = bs;
+        return;
+    }
     float bs = block_sum(prod, scratch);
     if (threadIdx.x == 0) part_gamma[blockIdx.x] = bs;
 }
@@ -729,18 +751,101 @@ __global__ __launch_bounds__(kBottomThreads) void bottom_lds_kernel(LevelSet ls,
 
 // ---------------------------------------------------------------------- PCG
 
-__global__ void pcg_reset_kernel(PcgState* st) {
+__global__ void pcg_reset_kernel(PcgState* st, double tol2, int max_iters) {
     st->done = 0;
     st->iters = 0;
     st->exit_reason = 0;
     st->maxerr = st->delta = st->alpha = 0.0f;
     st->bnorm2 = st->rnorm2 = 0.0;
+    st->tol2 = tol2;
+    st->gamma_d[0] = st->gamma_d[1] = 0.0;
+    st->max_iters = max_iters;
+}
+
+// Relative mode, before the first V-cycle: b = r is kept for the residual
+// recomputation, the FP64 pressure starts from the FP32 one, and the partials
+// of ||b||^2 over the fluid cells.
+__global__ __launch_bounds__(kThreads) void pcg_bnorm_kernel(LevelView L, float* __restrict__ rhs,
+                                                              double* __restrict__ part,
+                                                              const float* __restrict__ pressure,
+                                                              double* __restrict__ pd) {
+    __shared__ double dscratch[kThreads / 64];
+    double acc = 0.0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+        pd[c] = (double)pressure[c];
+        if (!L.fluid[c]) continue;
+        const size_t pc = (size_t)c + L.w;
+        const float b = L.r[pc];
+        rhs[pc] = b;
+        acc += (double)b * (double)b;
+    }
+    acc = block_sum(acc, dscratch);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(kThreads) void pcg_bnorm_finalize_kernel(PcgState* st, const double* __restrict__ part) {
+    __shared__ double dscratch[kThreads / 64];
+    double b = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) b += part[i];
+    b = block_sum(b, dscratch);
+    if (threadIdx.x == 0) {
+        st->bnorm2 = b;
+        if (!(b > 0.0)) { st->done = 1; st->exit_reason = 1; }  // b = 0: p = 0 is the solution
+    }
+}
+
+// Relative mode, after iteration k's update, every recompute_every
+// iterations (due): r is replaced by the true residual b - A p evaluated in
+// FP64 against the FP64 pressure (the product with the same fused operations
+// as apply_matrix_and_dot, zero outside the grid).  part_out receives the
+// partials of ||r||^2 that the residual test reads (part_in's when not due).
+__global__ __launch_bounds__(kThreads) void true_residual_kernel(LevelView L, const float* __restrict__ rhs,
+                                                                  const double* __restrict__ pd, const PcgState* st,
+                                                                  const double* __restrict__ part_in,
+                                                                  double* __restrict__ part_out, int due) {
+    __shared__ double dscratch[kThreads / 64];
+    if (st->done) return;
+    if (!due) {
+        if (threadIdx.x == 0) part_out[blockIdx.x] = part_in[blockIdx.x];
+        return;
+    }
+    const int64_t w = L.w, n = L.cells;
+    auto pa = [&](int64_t raw) { return raw >= 0 && raw < n ? pd[raw] : 0.0; };
+    double acc = 0.0;
+    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+        if (!L.fluid[c]) continue;
+        const size_t pc = (size_t)c + L.w;
+        const int64_t ci = c;
+        double v = (double)L.diag[pc] * pd[c];
+        v = fma((double)L.px[pc], pa(ci + 1), v);
+        v = fma((double)L.px[pc - 1], pa(ci - 1), v);
+        v = fma((double)L.py[pc], pa(ci + w), v);
+        v = fma((double)L.py[pc - L.w], pa(ci - w), v);
+        const double t = (double)rhs[pc] - v;
+        L.r[pc] = (float)t;
+        acc += t * t;
+    }
+    acc = block_sum(acc, dscratch);
+    if (threadIdx.x == 0) part_out[blockIdx.x] = acc;
 }
 
 __global__ void zero_pressure_kernel(const uint8_t* __restrict__ fluid, float* __restrict__ pressure,
                                      uint32_t cells) {
     uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c < cells && fluid[c]) pressure[c] = 0.0f;
+}
+
+// Relative mode: breakdown when r.z <= 0 (or not finite), the iteration cap.
+__device__ inline bool gamma_exit_rel(PcgState* st, double g, int k) {
+    if (!(g > 0.0)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
+        return true;
+    }
+    if (k >= st->max_iters) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 5; st->iters = k; }
+        return true;
+    }
+    return false;
 }
 
 // Exit tests on gamma = r.z before iteration k (src/apic.rs:1890, 1938) and
@@ -763,16 +868,30 @@ __device__ inline bool gamma_exit(PcgState* st, float g, int k) {
 __global__ __launch_bounds__(kThreads) void spmv_dot_kernel(LevelView L, const float* __restrict__ s_old,
                                                              float* __restrict__ s_new, float* __restrict__ As,
                                                              PcgState* st, const float* __restrict__ part_gamma,
-                                                             float* __restrict__ part_delta, int k) {
+                                                             float* __restrict__ part_delta, int k,
+                                                             const double* __restrict__ part_gamma_d,
+                                                             double* __restrict__ part_delta_d) {
     __shared__ float scratch[kThreads / 64];
+    __shared__ double dscratch[kThreads / 64];
     if (st->done) return;
-    const float g = sum_partials(part_gamma, scratch);
-    if (gamma_exit(st, g, k)) return;
     const bool first = k == 0;
-    const float beta = first ? 0.0f : g / st->gamma[k - 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st->gamma[k] = g; st->iters = k + 1; }
+    float beta;
+    if (part_gamma_d) {  // relative mode: FP64 scalars
+        double g = 0.0;
+        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) g += part_gamma_d[i];
+        g = block_sum(g, dscratch);
+        if (gamma_exit_rel(st, g, k)) return;
+        beta = first ? 0.0f : (float)(g / st->gamma_d[(k - 1) & 1]);
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->gamma_d[k & 1] = g; st->iters = k + 1; }
+    } else {
+        const float g = sum_partials(part_gamma, scratch);
+        if (gamma_exit(st, g, k)) return;
+        beta = first ? 0.0f : g / st->gamma[k - 1];
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->gamma[k] = g; st->iters = k + 1; }
+    }
     const size_t w = L.w;
     float prod = 0.0f;
+    double dprod = 0.0;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
         if (!L.fluid[c]) continue;
         size_t pc = (size_t)c + w;
@@ -785,7 +904,13 @@ __global__ __launch_bounds__(kThreads) void spmv_dot_kernel(LevelView L, const f
         v = fmaf(L.py[pc - w], sv(pc - w), v);
         s_new[pc] = s0;
         As[pc] = v;
-        prod += v * s0;
+        if (part_delta_d) dprod += (double)s0 * (double)v;
+        else prod += v * s0;
+    }
+    if (part_delta_d) {
+        const double bs = block_sum(dprod, dscratch);
+        if (threadIdx.x == 0) part_delta_d[blockIdx.x] = bs;
+        return;
     }
     float bs = block_sum(prod, scratch);
     if (threadIdx.x == 0) part_delta[blockIdx.x] = bs;
@@ -797,9 +922,39 @@ __global__ __launch_bounds__(kThreads) void axpy_max_kernel(LevelView L, const f
                                                              const float* __restrict__ As,
                                                              float* __restrict__ pressure, PcgState* st,
                                                              const float* __restrict__ part_delta,
-                                                             float* __restrict__ part_max, int k) {
+                                                             float* __restrict__ part_max, int k,
+                                                             const double* __restrict__ part_delta_d,
+                                                             double* __restrict__ part_rr_d,
+                                                             double* __restrict__ pd) {
     __shared__ float scratch[kThreads / 64];
+    __shared__ double dscratch[kThreads / 64];
     if (st->done) return;
+    if (part_delta_d) {  // relative mode: alpha = r.z / s.As in FP64, partials of ||r||^2
+        double d = 0.0;
+        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) d += part_delta_d[i];
+        d = block_sum(d, dscratch);
+        if (!(d > 0.0)) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 2; st->delta = (float)d; }
+            return;
+        }
+        const double alpha_d = st->gamma_d[k & 1] / d;
+        const float alpha = (float)alpha_d;
+        if (blockIdx.x == 0 && threadIdx.x == 0) { st->delta = (float)d; st->alpha = alpha; }
+        double rr = 0.0;
+        for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+            if (!L.fluid[c]) continue;
+            size_t pc = (size_t)c + L.w;
+            const double np = pd[c] + alpha_d * (double)s[pc];
+            pd[c] = np;
+            pressure[c] = (float)np;
+            float nr = fmaf(-alpha, As[pc], L.r[pc]);
+            L.r[pc] = nr;
+            rr += (double)nr * (double)nr;
+        }
+        rr = block_sum(rr, dscratch);
+        if (threadIdx.x == 0) part_rr_d[blockIdx.x] = rr;
+        return;
+    }
     const float d = sum_partials(part_delta, scratch);
     if (fabsf(d) < kSafety) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 2; st->delta = d; }
@@ -823,11 +978,19 @@ __global__ __launch_bounds__(kThreads) void axpy_max_kernel(LevelView L, const f
 // The gamma test after k iterations, run at the host's check points so that
 // the recorded state is final when the host reads it.
 __global__ __launch_bounds__(kThreads) void decide_kernel(PcgState* st, const float* __restrict__ part_gamma, int k,
-                                                           PcgState* host) {
+                                                           PcgState* host, const double* __restrict__ part_gamma_d) {
     __shared__ float scratch[kThreads / 64];
+    __shared__ double dscratch[kThreads / 64];
     if (!st->done) {
-        const float g = sum_partials(part_gamma, scratch);
-        (void)gamma_exit(st, g, k);
+        if (part_gamma_d) {
+            double g = 0.0;
+            for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) g += part_gamma_d[i];
+            g = block_sum(g, dscratch);
+            (void)gamma_exit_rel(st, g, k);
+        } else {
+            const float g = sum_partials(part_gamma, scratch);
+            (void)gamma_exit(st, g, k);
+        }
     }
     if (host && threadIdx.x == 0) {  // the scalars the host reads (not the gamma history)
         host->done = st->done;
@@ -894,7 +1057,20 @@ void launch_mg_build(Sim& sim) {
         hipLaunchKernelGGL(mg_build_bottom_kernel, dim3(1), dim3(kBottomThreads), 0, sim.stream, level_set(sim), l);
 }
 
-void launch_pcg_reset(Sim& sim) { hipLaunchKernelGGL(pcg_reset_kernel, dim3(1), dim3(1), 0, sim.stream, sim.pcg); }
+int pcg_cap(const Sim& sim) { return sim.solver.mode == 1 ? sim.solver.max_iterations : kMaxPcgIterations; }
+
+// Before the solve: the state, and in the relative mode b = r (kept for the
+// residual recomputation) and ||b||^2.
+void launch_pcg_reset(Sim& sim) {
+    const bool rel = sim.solver.mode == 1;
+    const double tol = (double)sim.solver.rel_tol;
+    hipLaunchKernelGGL(pcg_reset_kernel, dim3(1), dim3(1), 0, sim.stream, sim.pcg, tol * tol, pcg_cap(sim));
+    if (rel) {
+        hipLaunchKernelGGL(pcg_bnorm_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, sim.levels[0], sim.rhs,
+                           sim.part_rr_d, sim.pressure, sim.pressure_d);
+        hipLaunchKernelGGL(pcg_bnorm_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.pcg, sim.part_rr_d);
+    }
+}
 
 void launch_zero_pressure(Sim& sim) {
     hipLaunchKernelGGL(zero_pressure_kernel, dim3(blocks_for(sim.cells)), dim3(kThreads), 0, sim.stream,
@@ -910,8 +1086,9 @@ void launch_vcycle(Sim& sim, int mode) {
     hipStream_t s = sim.stream;
     auto capped = [](size_t cells) { return std::min<unsigned>(blocks_for(cells), 2048u); };
     // down: base
+    const bool rel = sim.solver.mode == 1 && mode != 0;
     hipLaunchKernelGGL(red_zero_kernel, dim3(capped(B.cells)), dim3(256), 0, s, B, sim.pcg, sim.part_max,
-                       (int)(mode == 2), sim.props.solve_error);
+                       (int)(mode == 2), sim.props.solve_error, rel ? sim.part_rr2_d : nullptr);
     Prolong none{};
     auto half_blocks = [](const LevelView& L) { return blocks_for((size_t)((L.w + 1) / 2) * L.h); };
     hipLaunchKernelGGL(sweep_kernel, dim3(half_blocks(B)), dim3(kThreads), 0, s, B, none, 1, 0, done);
@@ -943,21 +1120,31 @@ void launch_vcycle(Sim& sim, int mode) {
     Prolong P = base_prolong(sim);
     hipLaunchKernelGGL(sweep_kernel, dim3(half_blocks(B)), dim3(kThreads), 0, s, B, P, 1, 1, done);
     hipLaunchKernelGGL(base_red_up_dot_kernel, dim3(kRedBlocks), dim3(kThreads), 0, s, B, P, mode, done,
-                       sim.part_gamma);
+                       sim.part_gamma, rel ? sim.part_gamma_d : nullptr);
 }
 
 void launch_pcg_iteration_head(Sim& sim, int k) {
     const LevelView& B = sim.levels[0];
     float* s_old = sim.search[k & 1];
     float* s_new = sim.search[(k + 1) & 1];
+    const bool rel = sim.solver.mode == 1;
     hipLaunchKernelGGL(spmv_dot_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_old, s_new, sim.As,
-                       sim.pcg, sim.part_gamma, sim.part_delta, k);
+                       sim.pcg, sim.part_gamma, sim.part_delta, k, rel ? sim.part_gamma_d : nullptr,
+                       rel ? sim.part_delta_d : nullptr);
     hipLaunchKernelGGL(axpy_max_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_new, sim.As,
-                       sim.pressure, sim.pcg, sim.part_delta, sim.part_max, k);
+                       sim.pressure, sim.pcg, sim.part_delta, sim.part_max, k, rel ? sim.part_delta_d : nullptr,
+                       rel ? sim.part_rr_d : nullptr, sim.pressure_d);
+    if (rel) {
+        const int every = sim.solver.recompute_every;
+        const int due = every > 0 && (k + 1) % every == 0;
+        hipLaunchKernelGGL(true_residual_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, sim.rhs,
+                           sim.pressure_d, sim.pcg, sim.part_rr_d, sim.part_rr2_d, due);
+    }
 }
 
 void launch_pcg_decide(Sim& sim, int k) {
-    hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.pcg, sim.part_gamma, k, sim.h_pcg_d);
+    hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.pcg, sim.part_gamma, k, sim.h_pcg_d,
+                       sim.solver.mode == 1 ? sim.part_gamma_d : nullptr);
 }
 
 }  // namespace apic

@@ -53,13 +53,19 @@ typedef struct apic_debug_options {
     uint8_t disable_particle_sort;
 } apic_debug_options;
 
-/* Pressure-solver rules.  mode 0 ("reference") reproduces solve_pcg exactly:
- * absolute 1e-5 exits on r.z and s.As, infinity-norm residual test against
- * WorldProperties.solve_error, 100-iteration cap, FP32 reductions
- * (src/apic.rs:8, 1865-1956).  mode 1 ("relative") is an extension: stop when
- * ||r||_2 <= rel_tol * ||b||_2 or after max_iterations; the dot products and
- * the convergence test run in FP64 when fp64_reductions != 0, and the true
- * residual b - A p is recomputed every recompute_every iterations (0 = never). */
+/* Pressure-solver rules.  mode 0 ("reference", the default) reproduces
+ * solve_pcg exactly: absolute 1e-5 exits on r.z and s.As, infinity-norm
+ * residual test against WorldProperties.solve_error, 100-iteration cap, FP32
+ * reductions (src/apic.rs:8, 1865-1956).  mode 1 ("relative") is an extension
+ * the reference cannot express: the same multigrid-preconditioned CG, stopped
+ * when ||r||_2 <= rel_tol * ||b||_2 or after max_iterations (1..100000), with
+ * breakdown exits when s.As <= 0 or r.z <= 0; vectors stay FP32, the dot
+ * products, ||r||, ||b|| and the step scalars are FP64 (fp64_reductions must
+ * be 1), the pressure is accumulated in FP64, and the true residual b - A p
+ * (FP64) replaces r every recompute_every iterations (0 = never).
+ * pcg_rel_residual reports ||r||_2 / ||b||_2 of the residual the test used.
+ * With FP32 SpMV the true residual levels off near eps32 * || |A| |p| || / ||b||,
+ * so a rel_tol below that is met by the recurrence residual only. */
 typedef struct apic_solver_config {
     int32_t mode;
     float rel_tol;

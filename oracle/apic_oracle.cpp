@@ -716,6 +716,99 @@ template <class R> struct Sim {
         stats.pcg_exit_reason = 5;
     }
 
+    // Solver rules of the "relative" mode, an extension the reference cannot
+    // express (its solve_pcg, src/apic.rs:1865-1956, hardcodes absolute exits
+    // and 100 iterations): preconditioned CG with the same V-cycle and FP32
+    // vectors s, As, r, z; the pressure is accumulated in FP64 (written to the
+    // FP32 pressure as it goes); dot products, norms and step scalars in A
+    // (FP64 when fp64); stop when the residual meets ||r||_2 <= rel_tol *
+    // ||b||_2, or after max_iterations; every recompute_every iterations r is
+    // replaced by the true residual b - A p evaluated in FP64; breakdown when
+    // s.As <= 0 or r.z <= 0.  Sums run in ascending cell order.  With FP32
+    // SpMV the true residual cannot fall much below eps32 || |A| |p| || / ||b||.
+    int solver_mode = 0;
+    double solver_rel_tol = 1e-6;
+    int solver_max_it = 500, solver_fp64 = 1, solver_recompute = 25;
+    template <class A>
+    void solve_pcg_relative_acc() {
+        const size_t w = W, start = W, end = start + cells;
+        stats.pcg_iterations = 0;
+        stats.pcg_exit_reason = 0;
+        stats.pcg_max_residual = 0;
+        stats.pcg_rel_residual = 0;
+        for (uint32_t k = 0; k < fluid_cells; ++k) pressure[base.fluid_idx[k] - W] = 0;
+        std::vector<double> pd(pressure.begin(), pressure.end());
+        std::vector<R> b(base.r);
+        auto fluid_sum = [&](auto term) {
+            A acc = 0;
+            for (size_t raw = 0; raw < cells; ++raw)
+                if (base.fluid[raw]) acc += term(raw + w);
+            return acc;
+        };
+        const A bnorm2 = fluid_sum([&](size_t i) { return A(base.r[i]) * A(base.r[i]); });
+        if (!(bnorm2 > 0)) { stats.pcg_exit_reason = 1; return; }
+        apply_preconditioner();
+        std::copy(base.e.begin() + start, base.e.begin() + end, search.begin() + start);
+        A gamma = 0;
+        for (size_t i = start; i < end; ++i) gamma += A(base.r[i]) * A(base.e[i]);
+        if (!(gamma > 0)) { stats.pcg_exit_reason = 1; return; }
+        const A tol2 = A(solver_rel_tol) * A(solver_rel_tol);
+        auto pd_at = [&](int64_t raw) { return raw >= 0 && raw < (int64_t)cells ? pd[(size_t)raw] : 0.0; };
+        for (int it = 0; it < solver_max_it; ++it) {
+            stats.pcg_iterations = it + 1;
+            apply_matrix_and_dot(search, As);
+            const A delta = fluid_sum([&](size_t i) { return A(search[i]) * A(As[i]); });
+            if (!(delta > 0)) { stats.pcg_exit_reason = 2; return; }
+            const A alpha_a = gamma / delta;
+            const R alpha = R(alpha_a);
+            for (size_t raw = 0; raw < cells; ++raw) {
+                if (!base.fluid[raw]) continue;
+                size_t pi = raw + w;
+                pd[raw] += double(alpha_a) * double(search[pi]);
+                pressure[raw] = R(pd[raw]);
+                base.r[pi] = std::fma(-alpha, As[pi], base.r[pi]);
+            }
+            A rr = fluid_sum([&](size_t i) { return A(base.r[i]) * A(base.r[i]); });
+            const bool due = solver_recompute > 0 && (it + 1) % solver_recompute == 0;
+            if (due) {  // true residual, FP64
+                double acc = 0;
+                for (size_t raw = 0; raw < cells; ++raw) {
+                    if (!base.fluid[raw]) continue;
+                    size_t pi = raw + w;
+                    const int64_t c = (int64_t)raw, ww = (int64_t)w;
+                    double v = double(base.diag[pi]) * pd[raw];
+                    v = std::fma(double(base.px[pi]), pd_at(c + 1), v);
+                    v = std::fma(double(base.px[pi - 1]), pd_at(c - 1), v);
+                    v = std::fma(double(base.py[pi]), pd_at(c + ww), v);
+                    v = std::fma(double(base.py[pi - w]), pd_at(c - ww), v);
+                    const double t = double(b[pi]) - v;
+                    base.r[pi] = R(t);
+                    acc += t * t;
+                }
+                rr = A(acc);
+            }
+            stats.pcg_rel_residual = (double)std::sqrt((double)rr / (double)bnorm2);
+            if (rr <= tol2 * bnorm2) { stats.pcg_exit_reason = 3; return; }
+            apply_preconditioner();
+            A ng = 0;
+            for (size_t i = start; i < end; ++i) ng += A(base.r[i]) * A(base.e[i]);
+            if (!(ng > 0)) { stats.pcg_exit_reason = 4; return; }
+            const R beta = R(ng / gamma);
+            gamma = ng;
+            for (size_t raw = 0; raw < cells; ++raw) {
+                if (!base.fluid[raw]) continue;
+                size_t i = raw + w;
+                search[i] = base.e[i] + (beta * search[i]);
+            }
+        }
+        stats.pcg_exit_reason = 5;
+    }
+    void solve_pressure() {
+        if (solver_mode == 0) solve_pcg();
+        else if (solver_fp64) solve_pcg_relative_acc<double>();
+        else solve_pcg_relative_acc<R>();
+    }
+
     // apply_pressure_gradient, src/apic.rs:1958-2055
     void apply_pressure_gradient() {
         const R target = 9, inv_cell = R(1) / cell;
@@ -940,7 +1033,7 @@ template <class R> struct Sim {
             apply_external_forces(dt);
             build_pressure_system();
             update_multilevel_grid();
-            solve_pcg();
+            solve_pressure();
             apply_pressure_gradient();
             apply_post_forces(dt);
             extrapolate_velocity();
@@ -959,7 +1052,7 @@ template <class R> struct Sim {
         stats.timestamp = timestamp;
         stats.prev_deltatime = (float)prev_dt;
         stats.max_particle_velocity = (float)last_maxv;
-        stats.pcg_rel_residual = -1;
+        if (solver_mode == 0) stats.pcg_rel_residual = -1;
     }
 
     // ---- field access ----------------------------------------------------
@@ -1090,7 +1183,7 @@ template <class R> int stage_t(Sim<R>* s, int stage, float dt, uint32_t ts) {
         case 3: s->build_pressure_system(); break;
         case 4: s->update_multilevel_grid(); break;
         case 5: s->apply_preconditioner(); break;
-        case 6: s->solve_pcg(); break;
+        case 6: s->solve_pressure(); break;
         case 7: s->apply_pressure_gradient(); break;
         case 8: s->apply_post_forces((R)dt); break;
         case 9: s->extrapolate_velocity(); break;
@@ -1172,6 +1265,15 @@ void orc_set_timestamp(void* hv, uint32_t ts, double prev_dt) {
     Handle* h = static_cast<Handle*>(hv);
     if (h->precision == 64) { S<double>(h)->timestamp = ts; S<double>(h)->prev_dt = prev_dt; }
     else { S<float>(h)->timestamp = ts; S<float>(h)->prev_dt = (float)prev_dt; }
+}
+
+void orc_set_solver(void* hv, int mode, double rel_tol, int max_it, int fp64, int recompute) {
+    auto* h = static_cast<Handle*>(hv);
+    auto set = [&](auto* sim) {
+        sim->solver_mode = mode; sim->solver_rel_tol = rel_tol; sim->solver_max_it = max_it;
+        sim->solver_fp64 = fp64; sim->solver_recompute = recompute;
+    };
+    if (h->precision == 64) set(S<double>(h)); else set(S<float>(h));
 }
 
 void orc_get_stats(void* hv, apic_step_stats* out) {

@@ -72,6 +72,7 @@ def lib():
         L.orc_set_timestamp.argtypes = [vp, ctypes.c_uint32, ctypes.c_double]
         L.orc_get_stats.argtypes = [vp, ctypes.POINTER(StepStats)]
         L.orc_set_p2g_shuffle.argtypes = [vp, ctypes.c_uint32]
+        L.orc_set_solver.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_max_particle_velocity.restype = ctypes.c_double
         L.orc_max_particle_velocity.argtypes = [vp]
         L.orc_add_radial_force.argtypes = [vp] + [ctypes.c_float] * 4
@@ -155,6 +156,10 @@ class Oracle:
         pd = ctypes.c_double()
         lib().orc_get_timestamp(self._h, ctypes.byref(ts), ctypes.byref(pd))
         return ts.value, pd.value
+
+    def set_solver(self, mode=1, rel_tol=1e-6, max_iterations=500, fp64=True, recompute_every=25):
+        """Solver rules: mode 0 the reference's, mode 1 relative (FP64 dots)."""
+        lib().orc_set_solver(self._h, mode, rel_tol, max_iterations, int(fp64), recompute_every)
 
     def set_timestamp(self, ts, prev_dt):
         lib().orc_set_timestamp(self._h, ts, prev_dt)
