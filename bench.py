@@ -1,7 +1,9 @@
 #!/usr/bin/env python3
 """Throughput of the APIC step on one MI355X: particle-steps/s on the 1024^2
 dam break (dx = 1/1024, 8 jittered particles per fluid cell, 2,005,736
-particles, dt = 2.5e-4), the reference's own solver rules.
+particles, dt = 2.5e-4, PCG tolerance 1e-6 relative: solver mode 1 with FP64
+reductions and a 500-iteration cap).  The same workload under the
+reference's own solver rules (absolute 1e-5 exits) is reported beside it.
 
 A step is one Apic::update(dt) (src/apic.rs:2429) over all particles, with
 every array resident in HBM.  W untimed warm-up updates, then K timed updates
@@ -13,6 +15,7 @@ baseline is the oracle (FP32, one thread) on the first steps of the same
 workload, timed on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
+                       [--solver relative|reference]
 """
 import argparse
 import json
@@ -51,11 +54,24 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c):
 KERNEL_NAMES = {"kernel_g2p": "g2p_kernel", "kernel_p2g": "p2g_owner_kernel"}
 
 
-def cpu_baseline(xy, cfg, budget_s):
+SOLVERS = {
+    # the workload's "PCG tol 1e-6": relative 2-norm, FP64 reductions, 500-iteration cap
+    "relative": dict(mode=1, rel_tol=1e-6, max_iterations=500, fp64_reductions=True, recompute_every=25),
+    # the reference's own rules (absolute 1e-5 exits, 100-iteration cap)
+    "reference": dict(mode=0),
+}
+SOLVER_NAMES = {"relative": "relative 1e-6 (FP64 reductions, residual recompute every 25, cap 500)",
+                "reference": "reference rules (absolute 1e-5, 100-iteration cap)"}
+
+
+def cpu_baseline(xy, cfg, budget_s, solver):
     """The oracle (FP32 restatement of the reference step, one thread) on the
     first steps of the same workload; steps chosen to fill about budget_s."""
     from oracle import pyoracle as po
     o = po.Oracle(cfg["width"], cfg["height"], cfg["dx"], po.bench_props(), xy=xy)
+    sv = SOLVERS[solver]
+    if sv["mode"] == 1:
+        o.set_solver(1, sv["rel_tol"], sv["max_iterations"], sv["fp64_reductions"], sv["recompute_every"])
     n = xy.shape[0]
     t0 = time.perf_counter()
     o.update(cfg["dt"])
@@ -72,9 +88,47 @@ def cpu_baseline(xy, cfg, budget_s):
         "unit": "particle-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{WORKLOAD}: first {steps} updates of dt={cfg['dt']} from the seeded scene, "
-                  f"oracle FP32 restatement, one thread, {platform.processor() or platform.machine()}",
+        "sample": f"{WORKLOAD}: first {steps} updates of dt={cfg['dt']} from the seeded scene, solver "
+                  f"{SOLVER_NAMES[solver]}, oracle FP32 restatement, one thread, "
+                  f"{platform.processor() or platform.machine()}",
     }
+
+
+def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_steps):
+    """W warm-up updates, K timed updates (HIP events around the two transfer
+    kernels only), then a short pass with events around every stage."""
+    import numpy as np
+    g = Apic.with_particles(cfg["width"], cfg["height"], cfg["dx"], props, xy)
+    g.set_solver_config(**SOLVERS[solver])
+    opts = DebugOptions(False)
+    dt = cfg["dt"]
+    for _ in range(warmup):
+        g.update(opts, dt)
+    g.set_profiling(1)
+    g.get_max_particle_velocity()  # synchronises the library's stream
+    substeps, iters, fluid, rel = 0, [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.update(opts, dt)
+        st = g.stats()
+        substeps += st["substeps"]
+        iters.append(st["pcg_iterations_total"])
+        fluid.append(st["fluid_cells"])
+        rel.append(st["pcg_rel_residual"])
+    g.get_max_particle_velocity()
+    elapsed = time.perf_counter() - t0
+    timers = g.timers()
+    g.set_profiling(2)
+    stage_substeps = 0
+    for _ in range(stage_steps):
+        g.update(opts, dt)
+        stage_substeps += g.stats()["substeps"]
+    stage_timers = g.timers()
+    g.set_profiling(0)
+    n_alloc = g.field_length("POS_X")
+    g.close()
+    return dict(elapsed=elapsed, substeps=substeps, iters=iters, fluid=fluid, rel=rel, timers=timers,
+                stage_timers=stage_timers, stage_substeps=stage_substeps, n_alloc=n_alloc)
 
 
 def main():
@@ -84,6 +138,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=15)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--solver", choices=sorted(SOLVERS), default="relative")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one simulation per GPU and no "
@@ -98,51 +153,28 @@ def main():
     xy = scenes.config_scene(WORKLOAD)
     n_p = xy.shape[0]
     props = WorldProperties(-9.81, 0.1, 5.0, 1e-5)
-    g = Apic.with_particles(cfg["width"], cfg["height"], cfg["dx"], props, xy)
-    opts = DebugOptions(False)
-    dt = cfg["dt"]
-    for _ in range(args.warmup):
-        g.update(opts, dt)
-    print(f"[bench] warmup {args.warmup} done", file=sys.stderr, flush=True)
+    r = timed_run(Apic, DebugOptions, props, cfg, xy, args.solver, args.steps, args.warmup, min(args.steps, 20))
+    print(f"[bench] timed {args.steps} steps in {r['elapsed']:.3f}s ({args.solver} solver)", file=sys.stderr,
+          flush=True)
+    other = "reference" if args.solver == "relative" else "relative"
+    o_steps = min(args.steps, 50)
+    ro = timed_run(Apic, DebugOptions, props, cfg, xy, other, o_steps, args.warmup, 0)
+    print(f"[bench] {other} solver: {o_steps} steps in {ro['elapsed']:.3f}s", file=sys.stderr, flush=True)
 
-    g.set_profiling(1)  # HIP events around the two transfer kernels only
-    g.get_max_particle_velocity()  # synchronises the library's stream
-    substeps, iters, fluid = 0, [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        g.update(opts, dt)
-        s = g.stats()
-        substeps += s["substeps"]
-        iters.append(s["pcg_iterations_total"])
-        fluid.append(s["fluid_cells"])
-    g.get_max_particle_velocity()
-    elapsed = time.perf_counter() - t0
-    timers = g.timers()
-    print(f"[bench] timed {args.steps} steps in {elapsed:.3f}s", file=sys.stderr, flush=True)
-    # stage breakdown from a short separate pass with events around every stage
-    g.set_profiling(2)
-    stage_steps = min(args.steps, 20)
-    stage_substeps = 0
-    for _ in range(stage_steps):
-        g.update(opts, dt)
-        stage_substeps += g.stats()["substeps"]
-    stage_timers = g.timers()
-    g.set_profiling(0)
-
-    value = n_p * substeps / elapsed
-    n_alloc = g.field_length("POS_X")
-    n_f = float(np.mean(fluid))
-    algo = algorithmic_bytes(n_p, n_alloc, n_f, cfg["width"] * cfg["height"])
+    value = n_p * r["substeps"] / r["elapsed"]
+    n_f = float(np.mean(r["fluid"]))
+    algo = algorithmic_bytes(n_p, r["n_alloc"], n_f, cfg["width"] * cfg["height"])
     kernels = {}
     for k in ("kernel_g2p", "kernel_p2g"):
-        ms, cnt = timers[k]
+        ms, cnt = r["timers"][k]
         avg_ms = ms / max(cnt, 1)
         kernels[k] = {"avg_ms": avg_ms, "launches": cnt, "bytes": algo[k],
                       "gbs": algo[k] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None}
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
     d = kernels[dom]
-    stages = {k: round(v[0] / max(stage_substeps, 1), 4) for k, v in stage_timers.items()
+    stages = {k: round(v[0] / max(r["stage_substeps"], 1), 4) for k, v in r["stage_timers"].items()
               if not k.startswith("kernel_")}
+    rels = [x for x in r["rel"] if x >= 0]
 
     line = {
         "metric": METRIC,
@@ -151,27 +183,31 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": r["elapsed"] / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded dam break (SplitMix64 jitter), no dataset",
         "config": {"workload": WORKLOAD, "grid": [cfg["width"], cfg["height"]], "dx": cfg["dx"],
-                   "particles": n_p, "ppc": cfg["ppc"], "dt": dt, "seed": cfg["seed"],
-                   "solver": "reference rules (abs 1e-5, 100-iteration cap)", "parallelism": "single GPU"},
+                   "particles": n_p, "ppc": cfg["ppc"], "dt": cfg["dt"], "seed": cfg["seed"],
+                   "solver": SOLVER_NAMES[args.solver], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": d["gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (d["gbs"] / HBM_PEAK_GBS) if d["gbs"] else None, "traffic": None,
                      "bytes_per_launch": d["bytes"], "avg_launch_ms": d["avg_ms"]},
         "kernels": kernels,
         "stage_ms_per_step": stages,
-        "pcg_iterations_mean": float(np.mean(iters)),
-        "substeps": substeps,
+        "pcg_iterations_mean": float(np.mean(r["iters"])),
+        "pcg_rel_residual_max": max(rels) if rels else None,
+        "substeps": r["substeps"],
+        "other_solver": {"solver": SOLVER_NAMES[other], "steps": o_steps,
+                         "value": n_p * ro["substeps"] / ro["elapsed"],
+                         "ms_per_step": ro["elapsed"] / o_steps * 1e3,
+                         "pcg_iterations_mean": float(np.mean(ro["iters"]))},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(xy, cfg, args.cpu_seconds)
-    g.close()
+        line["cpu_baseline"] = cpu_baseline(xy, cfg, args.cpu_seconds, args.solver)
     print(json.dumps(line))
 
 
