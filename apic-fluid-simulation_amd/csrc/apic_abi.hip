@@ -172,9 +172,10 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     size_t pcs = s.padded(C.w, C.h);
     ALLOC(s.cg_s, pcs); ALLOC(s.cg_ap, pcs); ALLOC(s.cg_r, pcs);
     ALLOC(s.pcg, 1);
-    ALLOC(s.part_gamma, kRedBlocks); ALLOC(s.part_delta, kRedBlocks); ALLOC(s.part_max, kRedBlocks);
+    const size_t ngam = std::max<size_t>(kRedBlocks, gamma_partials(s));
+    ALLOC(s.part_gamma, ngam); ALLOC(s.part_delta, kRedBlocks); ALLOC(s.part_max, kRedBlocks);
     ALLOC(s.part_fluid, 2 * kRedBlocks);
-    ALLOC(s.part_gamma_d, kRedBlocks); ALLOC(s.part_delta_d, kRedBlocks); ALLOC(s.part_rr_d, kRedBlocks);
+    ALLOC(s.part_gamma_d, ngam); ALLOC(s.part_delta_d, kRedBlocks); ALLOC(s.part_rr_d, kRedBlocks);
     ALLOC(s.part_rr2_d, kRedBlocks); ALLOC(s.pressure_d, nc);
     s.part_maxv_cap = 1024;
     ALLOC(s.part_maxv, s.part_maxv_cap);

@@ -116,6 +116,7 @@ void launch_mg_build(Sim& sim);
 void launch_vcycle(Sim& sim, int mode);  // mode 0: plain, 1: PCG start (gamma), 2: PCG iteration (gamma')
 void launch_pcg_reset(Sim& sim);                 // also the relative mode's ||b|| prologue
 int pcg_cap(const Sim& sim);                      // iteration cap of the current solver rules
+size_t gamma_partials(const Sim& sim);            // partials of r.z the V-cycle writes
 void launch_pcg_iteration_head(Sim& sim, int k);  // s update + A s + delta, then p/r update + max|r|
 void launch_pcg_decide(Sim& sim, int k);          // gamma test after k iterations; mirrors the state to the host
 void launch_substep_begin(Sim& sim, bool first, float frame_dt);  // dt of the substep, on the device

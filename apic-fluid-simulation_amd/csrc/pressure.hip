@@ -5,19 +5,24 @@
 //  assemble_kernel         ghost-fluid Laplacian + RHS (:1698-1784)
 //  mg_build_kernel         one coarse level of update_multilevel_grid
 //                          (:1499-1602), dense over coarse cells
-//  V-cycle (apply_preconditioner, :1788-1821) with red-black SOR, omega=1.2:
-//    red_zero_kernel       zero e + first red sweep (e starts at 0, so the
-//                          sweep reduces to e = omega*(r*inv_diag))
-//    sweep_kernel          one colour of smooth_level (:471-535), optionally
-//                          fused with prolongate (:596-616) on the way up
-//    restrict_kernel       restrict (:537-594), fused with the next level's
-//                          first red sweep
-//    bottom_kernel         one workgroup runs every level with <= 4096 cells
-//                          and the coarsest-level CG (:618-704)
+//  V-cycle (apply_preconditioner, :1788-1821) with red-black SOR, omega=1.2,
+//  one launch per level each way, on LDS tiles with a two-cell ring:
+//    vdown_kernel          (base: e = 0 and the first red sweep) the black
+//                          sweep of smooth_level (:471-535), then restrict
+//                          (:537-594) into the next level fused with its
+//                          first red sweep; on the base level in a PCG
+//                          iteration it first applies the residual test
+//    bottom_lds_kernel     one workgroup runs the coarse levels that fit in
+//    (bottom_kernel)       LDS (else every level with <= 4096 cells, in global
+//                          memory) and the coarsest-level CG (:618-704)
+//    vup_kernel            prolongate (:596-616) fused with both colours of the
+//                          reverse sweep; on the base level it also computes
+//                          r.z (gamma), one partial per tile
 //  PCG (solve_pcg, :1865-1956), reference rules:
 //    spmv_dot_kernel       s = z + beta*s fused with A s and s.As
 //    axpy_max_kernel       p += alpha s, r -= alpha As, max |r|
-//    the base level's final red sweep also computes r.z (gamma)
+//  and the relative mode (FP64 scalars and sums, FP64 pressure accumulation,
+//  periodic true residual: pcg_bnorm_kernel, true_residual_kernel).
 //  Scalars live in a PcgState in device memory.  A reduction kernel writes
 //  one partial per workgroup (plain stores, no device-wide fence or atomic);
 //  the next kernel of the chain sums those partials itself, in the same fixed
@@ -316,64 +321,9 @@ __device__ inline void restrict_cell(const LevelView& F, const LevelView& C, uin
     C.e[pcc] = (fuse_red && red && id != 0.0f) ? kOmega * (sum * id) : 0.0f;
 }
 
-// cell of colour `color` addressed by thread t (half the cells per colour)
-__device__ inline bool colour_cell(const LevelView& L, uint32_t t, int color, uint32_t& c) {
-    uint32_t hw = (L.w + 1) / 2;
-    uint32_t y = t / hw, k = t % hw;
-    if (y >= L.h) return false;
-    uint32_t x = 2 * k + ((y + (uint32_t)color) & 1u);
-    if (x >= L.w) return false;
-    c = y * L.w + x;
-    return true;
-}
-
 __device__ inline float block_max(float v, float* scratch);
 template <typename T>
 __device__ inline T block_sum(T v, T* scratch);
-
-// Zero e and the first red sweep.  In a PCG iteration this is the first kernel
-// after the residual update, so it also applies the residual test
-// (max |r| < solve_error, src/apic.rs:1930) from the axpy partials.
-__global__ __launch_bounds__(256) void red_zero_kernel(LevelView L, PcgState* st, const float* __restrict__ part_max,
-                                                       int check, float solve_error,
-                                                       const double* __restrict__ part_rr) {
-    __shared__ float scratch[256 / 64];
-    __shared__ double dscratch[256 / 64];
-    if (st->done) return;
-    if (check && part_rr) {  // relative mode: ||r||^2 <= rel_tol^2 ||b||^2
-        double rr = 0.0;
-        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) rr += part_rr[i];
-        rr = block_sum(rr, dscratch);
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->rnorm2 = rr;
-        if (rr <= st->tol2 * st->bnorm2) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 3; }
-            return;
-        }
-    } else if (check) {
-        float m = 0.0f;
-        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) m = fast_max(m, part_max[i]);
-        m = block_max(m, scratch);
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->maxerr = m;
-        if (m < solve_error) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 3; }
-            return;
-        }
-    }
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x)
-        red_zero_cell(L, c);
-}
-
-__global__ void sweep_kernel(LevelView L, Prolong P, int color, int mode, const int32_t* __restrict__ done) {
-    if (*done) return;
-    uint32_t c;
-    if (colour_cell(L, blockIdx.x * blockDim.x + threadIdx.x, color, c)) sor_cell(L, P, c, mode);
-}
-
-__global__ void restrict_kernel(LevelView F, LevelView C, int fuse_red, const int32_t* __restrict__ done) {
-    if (*done) return;
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < C.cells) restrict_cell(F, C, c, fuse_red != 0);
-}
 
 // ---- block-level reductions (fixed order, deterministic)
 
@@ -405,44 +355,15 @@ __device__ inline float block_max(float v, float* scratch) {
 // Sum (or max) of the kRedBlocks partials of one reduction.  Every workgroup
 // that calls it uses kThreads threads and the same order, so all of them see
 // the same value.
-__device__ inline float sum_partials(const float* __restrict__ part, float* scratch) {
+__device__ inline float sum_partials(const float* __restrict__ part, float* scratch, int n = kRedBlocks) {
     float v = 0.0f;
-    for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) v += part[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
     return block_sum(v, scratch);
 }
 __device__ inline float max_partials(const float* __restrict__ part, float* scratch) {
     float v = 0.0f;
     for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) v = fast_max(v, part[i]);
     return block_max(v, scratch);
-}
-
-// Base level, up sweep of the second colour (red), fused with r.z over the
-// whole padded interior (dot_product over [W, W+W*H), src/apic.rs:1888,1936);
-// non-fluid cells have e = 0, so they add nothing.  mode 0: V-cycle alone.
-__global__ __launch_bounds__(kThreads) void base_red_up_dot_kernel(LevelView L, Prolong P, int mode,
-                                                                    const int32_t* __restrict__ done,
-                                                                    float* __restrict__ part_gamma,
-                                                                    double* __restrict__ part_gamma_d) {
-    __shared__ float scratch[kThreads / 64];
-    __shared__ double dscratch[kThreads / 64];
-    if (*done) return;
-    float prod = 0.0f;
-    double dprod = 0.0;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
-        uint32_t x = c % L.w, y = c / L.w;
-        if (((x + y) & 1u) == 0) sor_cell(L, P, c, 2);
-        size_t pc = (size_t)c + L.w;
-        if (part_gamma_d) dprod += (double)L.r[pc] * (double)L.e[pc];
-        else prod += L.r[pc] * L.e[pc];
-    }
-    if (mode == 0) return;
-    if (part_gamma_d) {
-        double bs = block_sum(dprod, dscratch);
-        if (threadIdx.x == 0) part_gamma_d[blockIdx.x] = bs;
-        return;
-    }
-    float bs = block_sum(prod, scratch);
-    if (threadIdx.x == 0) part_gamma[blockIdx.x] = bs;
 }
 
 // ---- the one-workgroup bottom of the V-cycle
@@ -870,7 +791,7 @@ __global__ __launch_bounds__(kThreads) void spmv_dot_kernel(LevelView L, const f
                                                              PcgState* st, const float* __restrict__ part_gamma,
                                                              float* __restrict__ part_delta, int k,
                                                              const double* __restrict__ part_gamma_d,
-                                                             double* __restrict__ part_delta_d) {
+                                                             double* __restrict__ part_delta_d, int ngam) {
     __shared__ float scratch[kThreads / 64];
     __shared__ double dscratch[kThreads / 64];
     if (st->done) return;
@@ -878,13 +799,13 @@ __global__ __launch_bounds__(kThreads) void spmv_dot_kernel(LevelView L, const f
     float beta;
     if (part_gamma_d) {  // relative mode: FP64 scalars
         double g = 0.0;
-        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) g += part_gamma_d[i];
+        for (int i = threadIdx.x; i < ngam; i += blockDim.x) g += part_gamma_d[i];
         g = block_sum(g, dscratch);
         if (gamma_exit_rel(st, g, k)) return;
         beta = first ? 0.0f : (float)(g / st->gamma_d[(k - 1) & 1]);
         if (blockIdx.x == 0 && threadIdx.x == 0) { st->gamma_d[k & 1] = g; st->iters = k + 1; }
     } else {
-        const float g = sum_partials(part_gamma, scratch);
+        const float g = sum_partials(part_gamma, scratch, ngam);
         if (gamma_exit(st, g, k)) return;
         beta = first ? 0.0f : g / st->gamma[k - 1];
         if (blockIdx.x == 0 && threadIdx.x == 0) { st->gamma[k] = g; st->iters = k + 1; }
@@ -978,17 +899,18 @@ __global__ __launch_bounds__(kThreads) void axpy_max_kernel(LevelView L, const f
 // The gamma test after k iterations, run at the host's check points so that
 // the recorded state is final when the host reads it.
 __global__ __launch_bounds__(kThreads) void decide_kernel(PcgState* st, const float* __restrict__ part_gamma, int k,
-                                                           PcgState* host, const double* __restrict__ part_gamma_d) {
+                                                           PcgState* host, const double* __restrict__ part_gamma_d,
+                                                           int ngam) {
     __shared__ float scratch[kThreads / 64];
     __shared__ double dscratch[kThreads / 64];
     if (!st->done) {
         if (part_gamma_d) {
             double g = 0.0;
-            for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) g += part_gamma_d[i];
+            for (int i = threadIdx.x; i < ngam; i += blockDim.x) g += part_gamma_d[i];
             g = block_sum(g, dscratch);
             (void)gamma_exit_rel(st, g, k);
         } else {
-            const float g = sum_partials(part_gamma, scratch);
+            const float g = sum_partials(part_gamma, scratch, ngam);
             (void)gamma_exit(st, g, k);
         }
     }
@@ -1001,6 +923,250 @@ __global__ __launch_bounds__(kThreads) void decide_kernel(PcgState* st, const fl
         host->alpha = st->alpha;
         host->bnorm2 = st->bnorm2;
         host->rnorm2 = st->rnorm2;
+    }
+}
+
+// ---- fused V-cycle kernels: one launch per level on the way down and one on
+// the way up.  A workgroup takes a kVT x kVT tile of the level and loads it
+// with a kVH-cell ring into LDS, then recomputes what the ring needs, so that
+// the per-cell arithmetic (operation order, fused multiply-adds) is exactly
+// that of red_zero_cell, sor_cell and restrict_cell above.  Ring positions
+// outside the grid are loaded by their padded flat index, the address the
+// global kernels read for a row's first and last cell; their coefficients
+// there are zero.
+
+constexpr int kVT = 32;                 // tile side (cells of the level)
+constexpr int kVH = 2;                  // ring: the first colour of a sweep needs two
+constexpr int kVR = kVT + 2 * kVH;      // region side
+constexpr int kVN = kVR * kVR;
+constexpr int kVC = kVT / 2 + 2;        // parent cells covering the region
+
+struct VTile {
+    float e[kVN], r[kVN], id[kVN], px[kVN], py[kVN], dg[kVN];
+    uint8_t fl[kVN];  // fluid flag; 0 outside the grid
+};
+
+// All loads of a thread are issued before its LDS stores (compile-time
+// stride: the workgroup has 256 threads).
+__device__ inline void vtile_load(const LevelView& L, int32_t X0, int32_t Y0, VTile& t, bool load_e, bool load_dg) {
+    constexpr int kPer = (kVN + 255) / 256;
+    const int64_t w = L.w, h = L.h, np = (int64_t)L.cells + 2 * w;
+    float r[kPer], id[kPer], px[kPer], py[kPer], dg[kPer], e[kPer];
+    uint8_t fl[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
+        const int32_t x = X0 - kVH + i % kVR, y = Y0 - kVH + i / kVR;
+        const int64_t p = (int64_t)(y + 1) * w + x;
+        const bool inb = i < kVN && p >= 0 && p < np;
+        const bool real = i < kVN && x >= 0 && x < w && y >= 0 && y < h;
+        r[k] = inb ? L.r[p] : 0.0f;
+        id[k] = inb ? L.inv_diag[p] : 0.0f;
+        px[k] = inb ? L.px[p] : 0.0f;
+        py[k] = inb ? L.py[p] : 0.0f;
+        dg[k] = (load_dg && inb) ? L.diag[p] : 0.0f;
+        e[k] = (load_e && inb) ? L.e[p] : 0.0f;
+        fl[k] = real ? L.fluid[(size_t)y * w + x] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int i = (int)threadIdx.x + 256 * k;
+        if (i >= kVN) continue;
+        t.r[i] = r[k]; t.id[i] = id[k]; t.px[i] = px[k]; t.py[i] = py[k]; t.dg[i] = dg[k]; t.e[i] = e[k];
+        t.fl[i] = fl[k];
+    }
+}
+
+// SOR update of cell i from its own value and four neighbour values (sor_cell).
+__device__ inline float vsor(const VTile& t, int i, float own, float er, float et, float el, float eb) {
+    const float id = t.id[i];
+    if (id != 0.0f) {
+        float sum = t.r[i];
+        sum = fmaf(-t.px[i], er, sum);
+        sum = fmaf(-t.py[i], et, sum);
+        sum = fmaf(-t.px[i - 1], el, sum);
+        sum = fmaf(-t.py[i - kVR], eb, sum);
+        const float gs = sum * id;
+        own = own + (kOmega * (gs - own));
+    }
+    return own;
+}
+
+// Down: (base: e = 0 and the first red sweep) the black sweep of level F,
+// then the restriction of b - A e into C fused with C's first red sweep.  On
+// the base level, in a PCG iteration, every workgroup first applies the
+// residual test of the previous update (max |r| < solve_error,
+// src/apic.rs:1930, or the relative test).
+__global__ __launch_bounds__(256) void vdown_kernel(LevelView F, LevelView C, int base, int fuse_red,
+                                                    const int32_t* __restrict__ done, uint32_t tiles_x,
+                                                    PcgState* st, int check, float solve_error,
+                                                    const float* __restrict__ part_max,
+                                                    const double* __restrict__ part_rr) {
+    __shared__ VTile t;
+    __shared__ float scratch[256 / 64];
+    __shared__ double dscratch[256 / 64];
+    if (*done) return;
+    if (check && part_rr) {
+        double rr = 0.0;
+        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) rr += part_rr[i];
+        rr = block_sum(rr, dscratch);
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->rnorm2 = rr;
+        if (rr <= st->tol2 * st->bnorm2) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 3; }
+            return;
+        }
+    } else if (check) {
+        float m = 0.0f;
+        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) m = fast_max(m, part_max[i]);
+        m = block_max(m, scratch);
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->maxerr = m;
+        if (m < solve_error) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 3; }
+            return;
+        }
+    }
+    const int32_t X0 = (int32_t)(blockIdx.x % tiles_x) * kVT, Y0 = (int32_t)(blockIdx.x / tiles_x) * kVT;
+    const int32_t w = (int32_t)F.w, h = (int32_t)F.h;
+    const int32_t cw = (int32_t)C.w, ch = (int32_t)C.h;
+    // the coarse cells' flags and inverse diagonals, loaded with the tile
+    constexpr int kCPer = (kVT / 2) * (kVT / 2) / 256;
+    uint8_t cfl[kCPer];
+    float cid[kCPer];
+#pragma unroll
+    for (int q = 0; q < kCPer; ++q) {
+        const int k = (int)threadIdx.x + 256 * q;
+        const int32_t cx = X0 / 2 + k % (kVT / 2), cy = Y0 / 2 + k / (kVT / 2);
+        const bool in = cx < cw && cy < ch;
+        cfl[q] = in ? C.fluid[(size_t)cy * cw + cx] : 0;
+        cid[q] = in ? C.inv_diag[(size_t)(cy + 1) * cw + cx] : 0.0f;
+    }
+    vtile_load(F, X0, Y0, t, !base, true);
+    __syncthreads();
+    if (base) {  // red_zero_cell over the region
+        for (int i = threadIdx.x; i < kVN; i += blockDim.x) {
+            const int32_t x = X0 - kVH + i % kVR, y = Y0 - kVH + i / kVR;
+            const bool red = ((x + y) & 1) == 0;
+            t.e[i] = (t.fl[i] && red && t.id[i] != 0.0f) ? kOmega * (t.r[i] * t.id[i]) : 0.0f;
+        }
+        __syncthreads();
+    }
+    // black cells of the tile and its one-cell ring (they read red cells only)
+    for (int i = threadIdx.x; i < kVN; i += blockDim.x) {
+        const int lx = i % kVR, ly = i / kVR;
+        if (lx < 1 || lx >= kVR - 1 || ly < 1 || ly >= kVR - 1) continue;
+        const int32_t x = X0 - kVH + lx, y = Y0 - kVH + ly;
+        if (!t.fl[i] || ((x + y) & 1) == 0) continue;
+        t.e[i] = vsor(t, i, t.e[i], t.e[i + 1], t.e[i + kVR], t.e[i - 1], t.e[i - kVR]);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kVT * kVT; i += blockDim.x) {
+        const int32_t x = X0 + i % kVT, y = Y0 + i / kVT;
+        if (x >= w || y >= h) continue;
+        if (!base && ((x + y) & 1) == 0) continue;  // red values are already in memory
+        const int li = (y - Y0 + kVH) * kVR + (x - X0 + kVH);
+        F.e[(size_t)(y + 1) * w + x] = t.e[li];
+    }
+    // restriction into the tile's coarse cells (restrict_cell)
+#pragma unroll
+    for (int q = 0; q < kCPer; ++q) {
+        const int k = (int)threadIdx.x + 256 * q;
+        const int32_t cx = X0 / 2 + k % (kVT / 2), cy = Y0 / 2 + k / (kVT / 2);
+        if (cx >= cw || cy >= ch) continue;
+        const size_t c = (size_t)cy * cw + cx, pcc = c + cw;
+        if (!cfl[q]) { C.r[pcc] = 0.0f; C.e[pcc] = 0.0f; continue; }
+        float sum = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int li = (2 * cy + dy - Y0 + kVH) * kVR + (2 * cx + dx - X0 + kVH);
+                if (t.fl[li]) {
+                    const float b = t.r[li];
+                    float ax = t.dg[li] * t.e[li];
+                    ax += t.px[li] * t.e[li + 1];
+                    ax += t.py[li] * t.e[li + kVR];
+                    ax += t.px[li - 1] * t.e[li - 1];
+                    ax += t.py[li - kVR] * t.e[li - kVR];
+                    sum += b - ax;
+                }
+            }
+        C.r[pcc] = sum;
+        const float id = cid[q];
+        const bool red = ((cx + cy) & 1) == 0;
+        C.e[pcc] = (fuse_red && red && id != 0.0f) ? kOmega * (sum * id) : 0.0f;
+    }
+}
+
+// Up: the two colour sweeps of smooth_level's reverse pass on level L with
+// C's error prolongated on the fly (prolongate + sor_cell modes 1 and 2); on
+// the base level, with the padded-list repeats of prolongate and the partials
+// of r.z over the tiles a workgroup takes (dot: 1 FP32, 2 FP64).
+__global__ __launch_bounds__(256) void vup_kernel(LevelView L, LevelView C, const FluidStats* __restrict__ fs,
+                                                  const int32_t* __restrict__ done, uint32_t tiles_x,
+                                                  int dot, float* __restrict__ part_gamma,
+                                                  double* __restrict__ part_gamma_d) {
+    __shared__ VTile t;
+    __shared__ float ce[kVC * kVC];
+    __shared__ float scratch[256 / 64];
+    __shared__ double dscratch[256 / 64];
+    if (*done) return;
+    const int32_t w = (int32_t)L.w, h = (int32_t)L.h, cw = (int32_t)C.w, chh = (int32_t)C.h;
+    const uint32_t last1 = fs ? fs->last_raw : 0u, cnt = fs ? fs->count : 0u;
+    float prod = 0.0f;
+    double dprod = 0.0;
+    {
+        const uint32_t tile = blockIdx.x;
+        const int32_t X0 = (int32_t)(tile % tiles_x) * kVT, Y0 = (int32_t)(tile / tiles_x) * kVT;
+        const int32_t CX0 = X0 / 2 - 1, CY0 = Y0 / 2 - 1;
+        vtile_load(L, X0, Y0, t, true, false);
+        for (int j = threadIdx.x; j < kVC * kVC; j += blockDim.x) {
+            const int32_t cx = CX0 + j % kVC, cy = CY0 + j / kVC;
+            ce[j] = (cx >= 0 && cx < cw && cy >= 0 && cy < chh) ? C.e[(size_t)(cy + 1) * cw + cx] : 0.0f;
+        }
+        __syncthreads();
+        // e plus the parent's error (prolonged); non-fluid cells keep e
+        auto pv = [&](int i) -> float {
+            float v = t.e[i];
+            if (!t.fl[i]) return v;
+            const int32_t x = X0 - kVH + i % kVR, y = Y0 - kVH + i / kVR;
+            const float pc = ce[((y >> 1) - CY0) * kVC + ((x >> 1) - CX0)];
+            int reps = 1;
+            if (fs) {
+                const uint32_t raw = (uint32_t)(y * w + x);
+                if (last1 != 0 && raw == last1 - 1 && (cnt & 7u) != 0) reps = 1 + (int)(8u - (cnt & 7u));
+            }
+            for (int k = 0; k < reps; ++k) v = v + pc;
+            return v;
+        };
+        // first colour (black): own and neighbours prolongated
+        for (int i = threadIdx.x; i < kVN; i += blockDim.x) {
+            const int lx = i % kVR, ly = i / kVR;
+            if (lx < 1 || lx >= kVR - 1 || ly < 1 || ly >= kVR - 1) continue;
+            const int32_t x = X0 - kVH + lx, y = Y0 - kVH + ly;
+            if (!t.fl[i] || ((x + y) & 1) == 0) continue;
+            const float own = pv(i);
+            t.e[i] = vsor(t, i, own, pv(i + 1), pv(i + kVR), pv(i - 1), pv(i - kVR));
+        }
+        __syncthreads();
+        // second colour (red) of the tile: own prolongated, neighbours final
+        for (int i = threadIdx.x; i < kVT * kVT; i += blockDim.x) {
+            const int32_t x = X0 + i % kVT, y = Y0 + i / kVT;
+            if (x >= w || y >= h) continue;
+            const int li = (y - Y0 + kVH) * kVR + (x - X0 + kVH);
+            float e = t.e[li];
+            if (t.fl[li] && ((x + y) & 1) == 0)
+                e = vsor(t, li, pv(li), t.e[li + 1], t.e[li + kVR], t.e[li - 1], t.e[li - kVR]);
+            L.e[(size_t)(y + 1) * w + x] = e;
+            if (dot == 2) dprod += (double)t.r[li] * (double)e;
+            else if (dot == 1) prod += t.r[li] * e;
+        }
+    }
+    if (dot == 2) {
+        const double bs = block_sum(dprod, dscratch);
+        if (threadIdx.x == 0) part_gamma_d[blockIdx.x] = bs;
+    } else if (dot == 1) {
+        const float bs = block_sum(prod, scratch);
+        if (threadIdx.x == 0) part_gamma[blockIdx.x] = bs;
     }
 }
 
@@ -1057,6 +1223,12 @@ void launch_mg_build(Sim& sim) {
         hipLaunchKernelGGL(mg_build_bottom_kernel, dim3(1), dim3(kBottomThreads), 0, sim.stream, level_set(sim), l);
 }
 
+// The base level's up-sweep writes one partial of r.z per tile.
+size_t gamma_partials(const Sim& sim) {
+    const LevelView& B = sim.levels[0];
+    return (size_t)((B.w + kVT - 1) / kVT) * ((B.h + kVT - 1) / kVT);
+}
+
 int pcg_cap(const Sim& sim) { return sim.solver.mode == 1 ? sim.solver.max_iterations : kMaxPcgIterations; }
 
 // Before the solve: the state, and in the relative mode b = r (kept for the
@@ -1081,46 +1253,37 @@ void launch_zero_pressure(Sim& sim) {
 // (residual test in the first kernel, gamma' partials in the last).
 void launch_vcycle(Sim& sim, int mode) {
     const int32_t* done = &sim.pcg->done;
-    const int n = (int)sim.levels.size();
-    const LevelView& B = sim.levels[0];
-    hipStream_t s = sim.stream;
-    auto capped = [](size_t cells) { return std::min<unsigned>(blocks_for(cells), 2048u); };
-    // down: base
+    const int last = (int)sim.levels.size() - 1;
+    const int bottom = (int)sim.bottom_level;  // 1 <= bottom <= last
     const bool rel = sim.solver.mode == 1 && mode != 0;
-    hipLaunchKernelGGL(red_zero_kernel, dim3(capped(B.cells)), dim3(256), 0, s, B, sim.pcg, sim.part_max,
-                       (int)(mode == 2), sim.props.solve_error, rel ? sim.part_rr2_d : nullptr);
-    Prolong none{};
-    auto half_blocks = [](const LevelView& L) { return blocks_for((size_t)((L.w + 1) / 2) * L.h); };
-    hipLaunchKernelGGL(sweep_kernel, dim3(half_blocks(B)), dim3(kThreads), 0, s, B, none, 1, 0, done);
-    if (n > 1) {
-        const int last = n - 1;
-        const int bottom = (int)sim.bottom_level;  // 1 <= bottom <= last
-        for (int l = 1; l < bottom; ++l) {
-            hipLaunchKernelGGL(restrict_kernel, dim3(blocks_for(sim.levels[l].cells)), dim3(kThreads), 0, s,
-                               sim.levels[l - 1], sim.levels[l], 1, done);
-            hipLaunchKernelGGL(sweep_kernel, dim3(half_blocks(sim.levels[l])), dim3(kThreads), 0, s, sim.levels[l],
-                               none, 1, 0, done);
-        }
-        hipLaunchKernelGGL(restrict_kernel, dim3(blocks_for(sim.levels[bottom].cells)), dim3(kThreads), 0, s,
-                           sim.levels[bottom - 1], sim.levels[bottom], (int)(bottom < last), done);
-        if (sim.bottom_in_lds)
-            hipLaunchKernelGGL(bottom_lds_kernel, dim3(1), dim3(kBottomThreads), 0, s, level_set(sim), bottom, done);
-        else
-            hipLaunchKernelGGL(bottom_kernel, dim3(1), dim3(kBottomThreads), 0, s, level_set(sim), bottom, sim.cg_s,
-                               sim.cg_ap, sim.cg_r, done);
-        for (int l = bottom - 1; l >= 1; --l) {
-            Prolong P{};
-            P.C = sim.levels[l + 1]; P.has = 1;
-            hipLaunchKernelGGL(sweep_kernel, dim3(half_blocks(sim.levels[l])), dim3(kThreads), 0, s, sim.levels[l], P,
-                               1, 1, done);
-            hipLaunchKernelGGL(sweep_kernel, dim3(half_blocks(sim.levels[l])), dim3(kThreads), 0, s, sim.levels[l], P,
-                               0, 2, done);
-        }
+    hipStream_t s = sim.stream;
+    auto tiles = [](const LevelView& L, uint32_t& tx) {
+        tx = (L.w + kVT - 1) / kVT;
+        return tx * ((L.h + kVT - 1) / kVT);
+    };
+    for (int l = 0; l < bottom; ++l) {
+        const LevelView& F = sim.levels[l];
+        const int fuse_red = l + 1 < bottom ? 1 : (int)(bottom < last);
+        uint32_t tx;
+        const uint32_t nt = tiles(F, tx);
+        hipLaunchKernelGGL(vdown_kernel, dim3(nt), dim3(256), 0, s, F, sim.levels[l + 1], (int)(l == 0), fuse_red,
+                           done, tx, sim.pcg, (int)(l == 0 && mode == 2), sim.props.solve_error, sim.part_max,
+                           (l == 0 && rel) ? sim.part_rr2_d : nullptr);
     }
-    Prolong P = base_prolong(sim);
-    hipLaunchKernelGGL(sweep_kernel, dim3(half_blocks(B)), dim3(kThreads), 0, s, B, P, 1, 1, done);
-    hipLaunchKernelGGL(base_red_up_dot_kernel, dim3(kRedBlocks), dim3(kThreads), 0, s, B, P, mode, done,
-                       sim.part_gamma, rel ? sim.part_gamma_d : nullptr);
+    if (sim.bottom_in_lds)
+        hipLaunchKernelGGL(bottom_lds_kernel, dim3(1), dim3(kBottomThreads), 0, s, level_set(sim), bottom, done);
+    else
+        hipLaunchKernelGGL(bottom_kernel, dim3(1), dim3(kBottomThreads), 0, s, level_set(sim), bottom, sim.cg_s,
+                           sim.cg_ap, sim.cg_r, done);
+    for (int l = bottom - 1; l >= 0; --l) {
+        const LevelView& L = sim.levels[l];
+        uint32_t tx;
+        const uint32_t nt = tiles(L, tx);
+        const int dot = (l == 0 && mode != 0) ? (rel ? 2 : 1) : 0;  // one partial of r.z per tile
+        hipLaunchKernelGGL(vup_kernel, dim3(nt), dim3(256), 0, s, L, sim.levels[l + 1],
+                           l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot, sim.part_gamma,
+                           sim.part_gamma_d);
+    }
 }
 
 void launch_pcg_iteration_head(Sim& sim, int k) {
@@ -1130,7 +1293,7 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     const bool rel = sim.solver.mode == 1;
     hipLaunchKernelGGL(spmv_dot_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_old, s_new, sim.As,
                        sim.pcg, sim.part_gamma, sim.part_delta, k, rel ? sim.part_gamma_d : nullptr,
-                       rel ? sim.part_delta_d : nullptr);
+                       rel ? sim.part_delta_d : nullptr, (int)gamma_partials(sim));
     hipLaunchKernelGGL(axpy_max_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_new, sim.As,
                        sim.pressure, sim.pcg, sim.part_delta, sim.part_max, k, rel ? sim.part_delta_d : nullptr,
                        rel ? sim.part_rr_d : nullptr, sim.pressure_d);
@@ -1144,7 +1307,7 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
 
 void launch_pcg_decide(Sim& sim, int k) {
     hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.pcg, sim.part_gamma, k, sim.h_pcg_d,
-                       sim.solver.mode == 1 ? sim.part_gamma_d : nullptr);
+                       sim.solver.mode == 1 ? sim.part_gamma_d : nullptr, (int)gamma_partials(sim));
 }
 
 }  // namespace apic
