@@ -26,6 +26,7 @@ SYMBOLS = [
     "apic_add_radial_force", "apic_get_max_particle_velocity", "apic_is_fluid",
     "apic_set_solver_config", "apic_get_stats", "apic_get_dims", "apic_get_timestamp",
     "apic_set_timestamp", "apic_field_length", "apic_read_field", "apic_write_field",
+    "apic_read_particles_aosoa", "apic_write_particles_aosoa",
     "apic_update_spatial_lookup", "apic_transfer_particles_to_grid", "apic_apply_external_forces",
     "apic_build_pressure_system", "apic_update_multilevel_grid", "apic_apply_preconditioner",
     "apic_solve_pcg", "apic_apply_pressure_gradient", "apic_extrapolate_velocity",
@@ -92,6 +93,8 @@ def lib():
     L.apic_field_length.argtypes = [vp, ctypes.c_int, u32, ctypes.POINTER(ctypes.c_size_t)]
     L.apic_read_field.argtypes = [vp, ctypes.c_int, u32, vp, ctypes.c_size_t]
     L.apic_write_field.argtypes = [vp, ctypes.c_int, u32, vp, ctypes.c_size_t]
+    L.apic_read_particles_aosoa.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
+    L.apic_write_particles_aosoa.argtypes = [vp, ctypes.c_int, vp, ctypes.c_size_t]
     for name in ("apic_update_spatial_lookup", "apic_build_pressure_system", "apic_update_multilevel_grid",
                  "apic_apply_preconditioner", "apic_solve_pcg", "apic_apply_pressure_gradient",
                  "apic_extrapolate_velocity"):

@@ -176,6 +176,27 @@ class Apic:
             raise ValueError(f"{name}: expected {n} elements, got {arr.size}")
         check(lib().apic_write_field(self._h, _lib.FIELD_ID[name], level, arr.ctypes.data if n else None, n))
 
+    # ---- the reference's pub particle vectors, in its chunk layout -------------
+    AOSOA_GROUPS = {"part_positions": 0, "part_velocities": 1, "part_c_u": 2, "part_c_v": 3,
+                    "part_time_residual": 4}
+
+    def read_particles_aosoa(self, group):
+        """Vec<Vec2x8> as (num_chunks, 2, 8) [x lanes then y lanes], or
+        Vec<f32x8> as (num_chunks, 8) for part_time_residual (src/apic.rs:757-767)."""
+        gid = self.AOSOA_GROUPS[group]
+        nc = self.num_chunks
+        shape = (nc, 8) if gid == 4 else (nc, 2, 8)
+        out = np.zeros(shape, dtype=np.float32)
+        check(lib().apic_read_particles_aosoa(self._h, gid, out.ctypes.data if out.size else None, out.size))
+        return out
+
+    def write_particles_aosoa(self, group, values):
+        gid = self.AOSOA_GROUPS[group]
+        nc = self.num_chunks
+        shape = (nc, 8) if gid == 4 else (nc, 2, 8)
+        arr = np.ascontiguousarray(np.asarray(values, dtype=np.float32).reshape(shape))
+        check(lib().apic_write_particles_aosoa(self._h, gid, arr.ctypes.data if arr.size else None, arr.size))
+
     # ---- pub stage methods ---------------------------------------------------
     def update_spatial_lookup(self):
         check(lib().apic_update_spatial_lookup(self._h))

@@ -668,6 +668,48 @@ apic_status apic_write_field(apic_handle* h, apic_field field, uint32_t level, c
     return APIC_OK;
 }
 
+static apic_status aosoa_prepare(Sim& s, int group, size_t count) {
+    if (group < APIC_PART_POSITIONS || group > APIC_PART_TIME_RESIDUAL)
+        return fail(APIC_ERR_INVALID_ARGUMENT, "unknown particle group");
+    if (count != aosoa_length(s, group)) return fail(APIC_ERR_INVALID_ARGUMENT, "count does not match the group length");
+    if (!s.aosoa_staging && s.nalloc) {
+        apic_status st = dalloc(s, s.aosoa_staging, 2 * s.nalloc);
+        if (st != APIC_OK) return st;
+    }
+    return APIC_OK;
+}
+
+apic_status apic_read_particles_aosoa(apic_handle* h, apic_particle_group group, float* dst, size_t count) {
+    if (!h || (!dst && count)) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    Sim& s = h->sim;
+    apic_status st = aosoa_prepare(s, (int)group, count);
+    if (st != APIC_OK) return st;
+    if (!count) return APIC_OK;
+    launch_pack_aosoa(s, (int)group, s.aosoa_staging);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    HIP_TRY(hipMemcpy(dst, s.aosoa_staging, count * 4, hipMemcpyDeviceToHost));
+    return APIC_OK;
+}
+
+apic_status apic_write_particles_aosoa(apic_handle* h, apic_particle_group group, const float* src, size_t count) {
+    if (!h || (!src && count)) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    Sim& s = h->sim;
+    apic_status st = aosoa_prepare(s, (int)group, count);
+    if (st != APIC_OK) return st;
+    if (!count) return APIC_OK;
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    HIP_TRY(hipMemcpy(s.aosoa_staging, src, count * 4, hipMemcpyHostToDevice));
+    launch_unpack_aosoa(s, (int)group, s.aosoa_staging);
+    CHECK_LAUNCH();
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    // the same bookkeeping as a field write (apic_write_field)
+    s.cells_valid = false;
+    if (group == APIC_PART_POSITIONS) s.keys_valid = false;
+    if (group == APIC_PART_VELOCITIES) s.maxv_valid = false;
+    return APIC_OK;
+}
+
 apic_status apic_update_spatial_lookup(apic_handle* h) {
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
     apic_status st = do_sort(h->sim);

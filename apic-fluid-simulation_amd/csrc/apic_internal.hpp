@@ -117,6 +117,11 @@ void launch_vcycle(Sim& sim, int mode);  // mode 0: plain, 1: PCG start (gamma),
 void launch_pcg_reset(Sim& sim);                 // also the relative mode's ||b|| prologue
 int pcg_cap(const Sim& sim);                      // iteration cap of the current solver rules
 size_t gamma_partials(const Sim& sim);            // partials of r.z the V-cycle writes
+
+// mirror.hip
+size_t aosoa_length(const Sim& sim, int group);   // floats of a group in the reference's chunk layout
+void launch_pack_aosoa(Sim& sim, int group, float* staging);
+void launch_unpack_aosoa(Sim& sim, int group, const float* staging);
 void launch_pcg_iteration_head(Sim& sim, int k);  // s update + A s + delta, then p/r update + max|r|
 void launch_pcg_decide(Sim& sim, int k);          // gamma test after k iterations; mirrors the state to the host
 void launch_substep_begin(Sim& sim, bool first, float frame_dt);  // dt of the substep, on the device
@@ -180,6 +185,7 @@ struct Sim {
     double* part_rr_d = nullptr;     // ||r||^2 partials of the recurrence residual
     double* part_rr2_d = nullptr;    // ... after the true-residual check
     double* pressure_d = nullptr;    // relative mode: the pressure accumulated in FP64
+    float* aosoa_staging = nullptr;  // chunk-layout mirrors (2 * nalloc floats, allocated on first use)
     uint32_t* part_fluid = nullptr;  // kRedBlocks (count, last+1) pairs of the base fluid cells
     float* part_maxv = nullptr;      // per-workgroup max |v| of the G2P kernel
     size_t part_maxv_cap = 0;

@@ -156,6 +156,22 @@ apic_status apic_get_max_particle_velocity(apic_handle* h, float* out);
 /* Apic::is_fluid (src/apic.rs:1015-1022). */
 apic_status apic_is_fluid(apic_handle* h, int32_t i, int32_t j, uint64_t* out);
 
+/* The reference's pub particle fields in its own layout (src/apic.rs:757-767):
+ * Vec<Vec2x8> for positions, velocities, c_u and c_v -- per chunk of eight
+ * particles eight x then eight y, 16 floats -- and Vec<f32x8> for the time
+ * residual, 8 floats per chunk.  count = 16 * num_chunks or 8 * num_chunks.
+ * Lanes past num_particles in the last chunk are included, as in the
+ * reference's vectors.  Reads synchronise the library's stream. */
+typedef enum apic_particle_group {
+    APIC_PART_POSITIONS = 0,
+    APIC_PART_VELOCITIES = 1,
+    APIC_PART_C_U = 2,
+    APIC_PART_C_V = 3,
+    APIC_PART_TIME_RESIDUAL = 4
+} apic_particle_group;
+apic_status apic_read_particles_aosoa(apic_handle* h, apic_particle_group group, float* dst, size_t count);
+apic_status apic_write_particles_aosoa(apic_handle* h, apic_particle_group group, const float* src, size_t count);
+
 /* Solver rules (extension; default is mode 0, the reference's rules). */
 apic_status apic_set_solver_config(apic_handle* h, const apic_solver_config* cfg);
 
