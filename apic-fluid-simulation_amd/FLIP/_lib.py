@@ -33,7 +33,7 @@ SYMBOLS = [
     "apic_transfer_grid_to_particles_and_advect", "apic_set_profiling", "apic_get_timers",
 ]
 TIMERS = ["sort", "p2g", "pressure_build", "mg_build", "pcg", "gradient", "extrapolate", "g2p",
-          "kernel_p2g", "kernel_g2p"]
+          "kernel_p2g", "kernel_g2p", "kernel_spmv"]
 
 
 class WorldPropertiesC(ctypes.Structure):

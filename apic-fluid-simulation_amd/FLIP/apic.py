@@ -101,9 +101,9 @@ class Apic:
         return {k: getattr(s, k) for k, _ in _lib.StepStatsC._fields_}
 
     def set_profiling(self, enable=True):
-        """Record HIP events around every stage (True or 2), around the two
-        transfer kernels only (1), or not at all (False or 0).  Extension;
-        resets the timers."""
+        """Record HIP events around every stage (True or 2), around the timed
+        kernels only (1: P2G, G2P, SpMV), or not at all (False or 0).
+        Extension; resets the timers."""
         level = 2 if enable is True else int(enable)
         check(lib().apic_set_profiling(self._h, level))
 

@@ -234,7 +234,7 @@ struct Sim {
         return e;
     }
     void mark(int t, int which) {
-        if (profiling == 0 || (profiling == 1 && t != APIC_T_K_P2G && t != APIC_T_K_G2P)) return;
+        if (profiling == 0 || (profiling == 1 && t < APIC_T_K_P2G)) return;
         hipEvent_t e = take_event();
         (void)hipEventRecord(e, stream);
         if (which == 0) {

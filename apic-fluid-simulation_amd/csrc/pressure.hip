@@ -1291,9 +1291,11 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     float* s_old = sim.search[k & 1];
     float* s_new = sim.search[(k + 1) & 1];
     const bool rel = sim.solver.mode == 1;
+    sim.mark(APIC_T_K_SPMV, 0);
     hipLaunchKernelGGL(spmv_dot_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_old, s_new, sim.As,
                        sim.pcg, sim.part_gamma, sim.part_delta, k, rel ? sim.part_gamma_d : nullptr,
                        rel ? sim.part_delta_d : nullptr, (int)gamma_partials(sim));
+    sim.mark(APIC_T_K_SPMV, 1);
     hipLaunchKernelGGL(axpy_max_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_new, sim.As,
                        sim.pressure, sim.pcg, sim.part_delta, sim.part_max, k, rel ? sim.part_delta_d : nullptr,
                        rel ? sim.part_rr_d : nullptr, sim.pressure_d);

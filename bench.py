@@ -8,9 +8,11 @@ reference's own solver rules (absolute 1e-5 exits) is reported beside it.
 A step is one Apic::update(dt) (src/apic.rs:2429) over all particles, with
 every array resident in HBM.  W untimed warm-up updates, then K timed updates
 bracketed by device synchronisation; value = sum of particles x substeps / wall
-seconds.  HIP events recorded by the library on its own stream time every
-stage and the two transfer kernels inside the timed region; the dominant
-kernel's achieved bandwidth is reported against the 8 TB/s HBM peak.  The CPU
+seconds.  HIP events recorded by the library on its own stream time the P2G,
+G2P and SpMV kernels inside the timed region; the one with the largest share
+of the step is the roofline kernel, its achieved bandwidth (algorithmic bytes
+per launch / average launch time) reported against the 8 TB/s HBM peak.  A
+short second pass times every stage.  The CPU
 baseline is the oracle (FP32, one thread) on the first steps of the same
 workload, timed on this host.
 
@@ -44,14 +46,18 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c):
     write its u face and v face (4 + 4 B), their valid flags (1 + 1 B), the
     density (4 B) and the fluid flag (1 B) -- the reference rewrites every
     face and cell in this stage.
+    spmv_dot_kernel: per fluid cell read z, the old search vector, diag,
+    plus_x, plus_y (20 B) and write the new search vector and A s (8 B); the
+    fluid flag of every cell (1 B).
     """
     return {
         "kernel_g2p": 48 * n_alloc + 8 * n_p + 10 * n_f,
         "kernel_p2g": 36 * n_p + 15 * n_c,
+        "kernel_spmv": 28 * n_f + n_c,
     }
 
 
-KERNEL_NAMES = {"kernel_g2p": "g2p_kernel", "kernel_p2g": "p2g_owner_kernel"}
+KERNEL_NAMES = {"kernel_g2p": "g2p_kernel", "kernel_p2g": "p2g_owner_kernel", "kernel_spmv": "spmv_dot_kernel"}
 
 
 SOLVERS = {
@@ -104,7 +110,7 @@ def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_s
     dt = cfg["dt"]
     for _ in range(warmup):
         g.update(opts, dt)
-    g.set_profiling(1)
+    g.set_profiling(1)  # HIP events around the timed kernels only
     g.get_max_particle_velocity()  # synchronises the library's stream
     substeps, iters, fluid, rel = 0, [], [], []
     t0 = time.perf_counter()
@@ -165,12 +171,13 @@ def main():
     n_f = float(np.mean(r["fluid"]))
     algo = algorithmic_bytes(n_p, r["n_alloc"], n_f, cfg["width"] * cfg["height"])
     kernels = {}
-    for k in ("kernel_g2p", "kernel_p2g"):
+    for k in ("kernel_g2p", "kernel_p2g", "kernel_spmv"):
         ms, cnt = r["timers"][k]
         avg_ms = ms / max(cnt, 1)
-        kernels[k] = {"avg_ms": avg_ms, "launches": cnt, "bytes": algo[k],
+        kernels[k] = {"avg_ms": avg_ms, "launches": cnt, "ms_per_step": ms / args.steps, "bytes": algo[k],
                       "gbs": algo[k] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None}
-    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    # the dominant kernel: the largest share of the step among the timed ones
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     d = kernels[dom]
     stages = {k: round(v[0] / max(r["stage_substeps"], 1), 4) for k, v in r["stage_timers"].items()
               if not k.startswith("kernel_")}

@@ -186,11 +186,12 @@ typedef enum apic_timer {
     APIC_T_GRADIENT, APIC_T_EXTRAPOLATE, APIC_T_G2P,
     APIC_T_K_P2G,   /* the P2G kernel alone */
     APIC_T_K_G2P,   /* the G2P gather/advect kernel alone */
+    APIC_T_K_SPMV,  /* the PCG's SpMV + dot kernel alone (every launch) */
     APIC_TIMER_COUNT
 } apic_timer;
 
-/* enable: 0 off, 1 the two transfer kernels only (APIC_T_K_P2G, APIC_T_K_G2P),
- * 2 every timer.  Resets the timers. */
+/* enable: 0 off, 1 the kernel timers only (APIC_T_K_P2G, APIC_T_K_G2P,
+ * APIC_T_K_SPMV), 2 every timer.  Resets the timers. */
 apic_status apic_set_profiling(apic_handle* h, int enable);
 /* ms[i] and counts[i] for i < n (n <= APIC_TIMER_COUNT); either may be null. */
 apic_status apic_get_timers(apic_handle* h, double* ms, uint32_t* counts, uint32_t n);
