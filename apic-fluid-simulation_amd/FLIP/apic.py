@@ -95,6 +95,16 @@ class Apic:
         cfg = _lib.SolverConfigC(mode, rel_tol, max_iterations, int(fp64_reductions), recompute_every)
         check(lib().apic_set_solver_config(self._h, ctypes.byref(cfg)))
 
+    def set_gravity_x(self, gx):
+        """Horizontal gravity added to u faces next to fluid (extension; 0 = the reference)."""
+        check(lib().apic_set_gravity_x(self._h, gx))
+
+    def device_bytes(self):
+        """Device memory held by this simulation."""
+        n = ctypes.c_size_t()
+        check(lib().apic_device_bytes(self._h, ctypes.byref(n)))
+        return n.value
+
     def stats(self):
         s = _lib.StepStatsC()
         check(lib().apic_get_stats(self._h, ctypes.byref(s)))
@@ -227,3 +237,56 @@ class Apic:
 
     def transfer_grid_to_particles_and_advect(self, deltatime):
         check(lib().apic_transfer_grid_to_particles_and_advect(self._h, deltatime))
+
+
+class Batch:
+    """Simulations on one stream, stepped together (extension; apic_batch_* of
+    include/apic.h).  Each simulation of a batch is an ordinary `Apic`; a batch
+    update of several of them computes, for each, exactly what `Apic.update`
+    computes alone."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().apic_batch_create(ctypes.byref(h)))
+        self._h = h
+
+    def create_sim(self, width, height, cell_size, world_properties, xy=None):
+        """Apic.new (xy None) or Apic.with_particles (xy: positions) inside the batch."""
+        h = ctypes.c_void_p()
+        if xy is None:
+            check(lib().apic_batch_create_sim(self._h, width, height, cell_size,
+                                              ctypes.byref(world_properties._c()), None, 0, ctypes.byref(h)))
+        else:
+            xy = np.ascontiguousarray(np.asarray(xy, dtype=np.float32).reshape(-1))
+            n = xy.size // 2
+            if n == 0:
+                xy = np.zeros(2, dtype=np.float32)  # a non-null pointer with n = 0: no particles
+            ptr = ctypes.c_void_p(xy.ctypes.data)
+            check(lib().apic_batch_create_sim(self._h, width, height, cell_size,
+                                              ctypes.byref(world_properties._c()), ptr, n, ctypes.byref(h)))
+        sim = Apic(h)
+        sim._batch = self  # the batch outlives its simulations
+        return sim
+
+    def update(self, sims, debug_options, frame_deltatime):
+        """Apic.update(debug_options, dt) of every simulation in `sims`, together;
+        frame_deltatime is one value or one per simulation."""
+        n = len(sims)
+        if n == 0:
+            return
+        dts = np.broadcast_to(np.asarray(frame_deltatime, dtype=np.float32), (n,)).copy()
+        hs = (ctypes.c_void_p * n)(*[s._h.value for s in sims])
+        opts = _lib.DebugOptionsC(1 if debug_options.disable_particle_sort else 0)
+        check(lib().apic_batch_update(self._h, hs, n, ctypes.byref(opts),
+                                      dts.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            check(lib().apic_batch_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

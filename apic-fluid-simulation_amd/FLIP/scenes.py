@@ -67,6 +67,40 @@ def falling_block(width, height, dx, ppc=8, seed=3, pool=0.5, block=512):
     return seed_cells(xs[order], ys[order], ppc, dx, seed)
 
 
+def dam_break_fill(width, height, dx, fill, ppc, seed):
+    """Dam break whose block covers the fraction `fill` of the domain with the
+    2:3 aspect of the 0.4 x 0.6 block (fill = 0.24 gives that block)."""
+    fx = float(np.sqrt(fill * 2.0 / 3.0))
+    fy = float(np.sqrt(fill * 3.0 / 2.0))
+    return dam_break(width, height, dx, fx, fy, ppc, seed)
+
+
+def instance_mix(count, side_min, side_max, seed, side_multiple=32):
+    """Independent dam breaks of mixed sizes: square sides log-uniform in
+    [side_min, side_max] rounded to a multiple of `side_multiple`, fill
+    fraction U[0.2, 0.6], 4-8 particles per cell, gravity angle U[-15, 15]
+    degrees.  A side that is a multiple of 32 gives every instance a multigrid
+    hierarchy down to at most 15 x 15 cells (a side of 2 x odd would stop the
+    hierarchy at (side/2)^2 cells, whose serial coarse CG dominates any step).
+    Returns a list of dicts (width, height, dx, ppc, fill, angle_deg, gravity,
+    gravity_x, seed, xy)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        side = float(np.exp(rng.uniform(np.log(side_min), np.log(side_max))))
+        side = int(min(max(round(side / side_multiple) * side_multiple, side_min), side_max))
+        fill = float(rng.uniform(0.2, 0.6))
+        ppc = int(rng.integers(4, 9))
+        angle = float(rng.uniform(-15.0, 15.0))
+        g = 9.81
+        inst_seed = int(seed) * 100003 + i
+        dx = 1.0 / side
+        out.append(dict(width=side, height=side, dx=dx, ppc=ppc, fill=fill, angle_deg=angle,
+                        gravity=float(-g * np.cos(np.radians(angle))), gravity_x=float(g * np.sin(np.radians(angle))),
+                        seed=inst_seed, xy=dam_break_fill(side, side, dx, fill, ppc, inst_seed)))
+    return out
+
+
 CONFIGS = {
     # (width, height, dx, dt, steps, ppc, seed, fill_x, fill_y)
     "cfg1_dam_break_128": dict(width=128, height=128, dx=1.0 / 128, dt=0.002, steps=100, ppc=4, seed=1),

@@ -19,6 +19,24 @@ using namespace apic;
 
 struct apic_handle {
     Sim sim;
+    apic_batch* batch = nullptr;  // the batch whose stream the simulation shares
+};
+
+// Several simulations on one stream, stepped together (launch.hpp).  The
+// launch tables of a substep go to the device in one copy before its first
+// kernel; the pinned staging buffer is rewritten only after that copy.
+struct apic_batch {
+    hipStream_t stream = nullptr;
+    uint32_t members = 0;
+    hipEvent_t e_sync = nullptr;
+    hipEvent_t e_copied = nullptr;
+    bool copy_pending = false;
+    apic::Recorder rec;
+    std::vector<uint8_t> host;
+    uint8_t* pinned = nullptr;
+    size_t pinned_cap = 0;
+    uint8_t* dev = nullptr;
+    size_t dev_cap = 0;
 };
 
 namespace {
@@ -54,6 +72,7 @@ apic_status dalloc(Sim& s, T*& p, size_t count) {
         return fail(e == hipErrorOutOfMemory ? APIC_ERR_OUT_OF_MEMORY : APIC_ERR_HIP,
                     std::string("hipMalloc: ") + hipGetErrorString(e));
     s.allocations.push_back(q);
+    s.device_bytes += bytes;
     e = hipMemset(q, 0, bytes);
     if (e != hipSuccess) return fail(APIC_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
     p = static_cast<T*>(q);
@@ -77,13 +96,13 @@ void release(Sim& s) {
     if (s.h_fstats) (void)hipHostFree(s.h_fstats);
     if (s.h_step) (void)hipHostFree(s.h_step);
     s.h_pcg = nullptr; s.h_maxv = nullptr; s.h_fstats = nullptr; s.h_step = nullptr;
-    if (s.stream) (void)hipStreamDestroy(s.stream);
+    if (s.stream && s.own_stream) (void)hipStreamDestroy(s.stream);
     s.stream = nullptr;
 }
 
 // Apic::new (src/apic.rs:779-916) with optional explicit particles.
 apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_properties& wp, bool explicit_set,
-                  const float* xy, uint32_t n) {
+                  const float* xy, uint32_t n, hipStream_t shared = nullptr) {
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(APIC_ERR_NO_DEVICE, "no HIP device found");
     if (W < 3 || H < 3) return fail(APIC_ERR_INVALID_ARGUMENT, "grid sides must be at least 3");
@@ -91,6 +110,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     if ((uint64_t)W * H > (1ull << 31)) return fail(APIC_ERR_INVALID_ARGUMENT, "grid too large");
     HIP_TRY(hipGetDevice(&s.device));
     s.W = W; s.H = H; s.cells = W * H; s.cell = cell; s.props = wp;
+    s.nred = reduction_workgroups(s.cells);
     s.solver = apic_solver_config{0, 1e-6f, 500, 1, 25};
     s.timestamp = 0;
     s.prev_dt = 1.0f / 60.0f;
@@ -129,7 +149,12 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
                         "no multigrid level: both grid sides must be even and > 4 (the reference panics here)");
     }
 
-    HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    if (shared) {
+        s.stream = shared;
+        s.own_stream = false;
+    } else {
+        HIP_TRY(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking));
+    }
     const size_t np = std::max<uint32_t>(s.num_particles, 1);
     const size_t na = std::max<size_t>(s.nalloc, 8);
     ALLOC(s.pbuf[0], NUM_PFIELDS * na);
@@ -443,19 +468,205 @@ bool field_desc(Sim& s, int f, uint32_t level, FieldDesc& d) {
 }
 
 apic_status create(uint32_t width, uint32_t height, float cell_size, const apic_world_properties* world_properties,
-                   bool explicit_set, const float* xy, uint32_t n, apic_handle** out) {
+                   bool explicit_set, const float* xy, uint32_t n, apic_handle** out, apic_batch* batch = nullptr) {
     if (!out || !world_properties) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;
     if (xy == nullptr && n != 0) return fail(APIC_ERR_INVALID_ARGUMENT, "xy is null but n > 0");
     apic_handle* h = new (std::nothrow) apic_handle();
     if (!h) return fail(APIC_ERR_OUT_OF_MEMORY, "host allocation failed");
-    apic_status st = build(h->sim, width, height, cell_size, *world_properties, explicit_set, xy, n);
+    h->batch = batch;
+    apic_status st = build(h->sim, width, height, cell_size, *world_properties, explicit_set, xy, n,
+                           batch ? batch->stream : nullptr);
     if (st != APIC_OK) {
         release(h->sim);
         delete h;
         return st;
     }
     *out = h;
+    if (batch) batch->members += 1;
+    return APIC_OK;
+}
+
+// ---- batched stepping
+
+// Uploads host[0, used) to the batch's device arena (in stream order, before
+// the kernels that read it).
+apic_status batch_upload(apic_batch& B, size_t used) {
+    if (B.copy_pending) {
+        HIP_TRY(hipEventSynchronize(B.e_copied));
+        B.copy_pending = false;
+    }
+    if (used > B.pinned_cap) {
+        if (B.pinned) HIP_TRY(hipHostFree(B.pinned));
+        B.pinned = nullptr;
+        B.pinned_cap = 0;
+        const size_t cap = std::max<size_t>(used + used / 2, 1 << 20);
+        HIP_TRY(hipHostMalloc((void**)&B.pinned, cap, hipHostMallocDefault));
+        B.pinned_cap = cap;
+    }
+    if (used > B.dev_cap) {
+        HIP_TRY(hipStreamSynchronize(B.stream));  // kernels may still read the old arena
+        if (B.dev) HIP_TRY(hipFree(B.dev));
+        B.dev = nullptr;
+        B.dev_cap = 0;
+        const size_t cap = std::max<size_t>(used + used / 2, 1 << 20);
+        HIP_TRY(hipMalloc((void**)&B.dev, cap));
+        B.dev_cap = cap;
+    }
+    std::memcpy(B.pinned, B.host.data(), used);
+    HIP_TRY(hipMemcpyAsync(B.dev, B.pinned, used, hipMemcpyHostToDevice, B.stream));
+    HIP_TRY(hipEventRecord(B.e_copied, B.stream));
+    B.copy_pending = true;
+    return APIC_OK;
+}
+
+// Compiles what the recorder holds into B.host at `used`.
+apic_status batch_compile(apic_batch& B, size_t& used, std::vector<Recorder::Launch>& out) {
+    out = B.rec.compile(B.host, used);
+    if (out.empty() && !B.rec.empty())
+        return fail(APIC_ERR_HIP, "batched launches: the simulations' launch orders conflict");
+    B.rec.clear();
+    return APIC_OK;
+}
+
+#define HIP_LAUNCH_TRY(expr)                                                                   \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) return fail(APIC_ERR_HIP, std::string("batched launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Apic::update for several simulations at once: the substeps of all of them
+// that still have frame time left are stepped together.  Per substep:
+// the head (dt, sort, P2G, pressure system, hierarchy, PCG start), the PCG
+// iterations in lockstep until every solve has ended (each simulation's
+// kernels return at once after its own exit), then the tail (gradient,
+// extrapolation, G2P).  The launches of one simulation are those of substep()
+// in the same order, so it computes the same bits as when stepped alone.
+apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disable_sort,
+                         const std::vector<float>& frame_dt) {
+    const size_t n = sims.size();
+    for (Sim* s : sims)
+        if (!s->maxv_valid) {
+            launch_max_velocity(*s);
+            s->maxv_valid = true;
+        }
+    CHECK_LAUNCH();
+    std::vector<float> t(n, 0.0f), last_maxv(n, 0.0f);
+    std::vector<uint32_t> substeps(n, 0), total_it(n, 0);
+    for (uint32_t round = 0;; ++round) {
+        std::vector<size_t> act;
+        for (size_t i = 0; i < n; ++i)
+            if (t[i] < frame_dt[i]) act.push_back(i);
+        if (act.empty()) break;
+        const bool first = round == 0;
+        const bool sort = !disable_sort && first;
+        std::vector<Recorder::Launch> head, iter, decide, tail;
+        size_t used = 0;
+        apic_status st;
+        // head
+        for (size_t r = 0; r < act.size(); ++r) {
+            Sim& s = *sims[act[r]];
+            B.rec.begin((uint32_t)r);
+            s.rec = &B.rec;
+            s.step_src = s.step;
+            launch_substep_begin(s, first, frame_dt[act[r]]);
+            if (sort) do_sort(s);
+            launch_p2g(s, true, 0.0f);
+            launch_build_pressure(s, true);
+            launch_mg_build(s);
+            launch_pcg_reset(s);
+            launch_vcycle(s, 1);
+            s.rec = nullptr;
+        }
+        if ((st = batch_compile(B, used, head)) != APIC_OK) return st;
+        // one PCG iteration (issued with k = 0, 1, ...) and the check
+        for (size_t r = 0; r < act.size(); ++r) {
+            Sim& s = *sims[act[r]];
+            B.rec.begin((uint32_t)r);
+            s.rec = &B.rec;
+            launch_pcg_iteration_head(s, 0);
+            launch_vcycle(s, 2);
+            s.rec = nullptr;
+        }
+        if ((st = batch_compile(B, used, iter)) != APIC_OK) return st;
+        for (size_t r = 0; r < act.size(); ++r) {
+            Sim& s = *sims[act[r]];
+            B.rec.begin((uint32_t)r);
+            s.rec = &B.rec;
+            launch_pcg_decide(s, 0);
+            s.rec = nullptr;
+        }
+        if ((st = batch_compile(B, used, decide)) != APIC_OK) return st;
+        // tail (issued once every solve has ended)
+        for (size_t r = 0; r < act.size(); ++r) {
+            Sim& s = *sims[act[r]];
+            B.rec.begin((uint32_t)r);
+            s.rec = &B.rec;
+            s.tail_guard = nullptr;
+            launch_gradient(s, true, 0.0f);
+            launch_extrapolate(s);
+            launch_g2p(s, 0.0f, true);
+            launch_substep_end(s);
+            s.cells_valid = false;
+            s.keys_valid = true;
+            s.maxv_valid = true;
+            s.rec = nullptr;
+        }
+        if ((st = batch_compile(B, used, tail)) != APIC_OK) return st;
+        if ((st = batch_upload(B, used)) != APIC_OK) return st;
+        HIP_LAUNCH_TRY(issue(head, B.dev, B.stream));
+        int batch_len = 0, cap = 0;
+        for (size_t i : act) {
+            batch_len = std::max(batch_len, sims[i]->pcg_batch);
+            cap = std::max(cap, pcg_cap(*sims[i]));
+        }
+        int k = 0, check_at = batch_len;
+        while (true) {
+            for (; k < check_at && k < cap; ++k) HIP_LAUNCH_TRY(issue(iter, B.dev, B.stream, k));
+            HIP_LAUNCH_TRY(issue(decide, B.dev, B.stream, k));
+            HIP_TRY(hipEventRecord(B.e_sync, B.stream));
+            HIP_TRY(hipEventSynchronize(B.e_sync));
+            bool all = true;
+            for (size_t i : act)
+                if (!sims[i]->h_pcg->done && k < pcg_cap(*sims[i])) all = false;
+            if (all || k >= cap) break;
+            check_at = k + 2;
+        }
+        HIP_LAUNCH_TRY(issue(tail, B.dev, B.stream));
+        for (size_t i : act) {
+            Sim& s = *sims[i];
+            s.step_src = nullptr;
+            s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
+            s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
+            s.stats.pcg_max_residual = s.h_pcg->maxerr;
+            s.stats.pcg_rel_residual = rel_residual(s);
+            s.pcg_batch = s.h_pcg->iters;
+            const StepState hs = *s.h_step;  // written by the head, complete after the wait
+            last_maxv[i] = hs.maxv;
+            t[i] += hs.dt;
+            s.prev_dt = hs.dt;
+            s.timestamp += 1;
+            substeps[i] += 1;
+            total_it[i] += s.stats.pcg_iterations;
+        }
+    }
+    for (size_t i = 0; i < n; ++i) {
+        Sim& s = *sims[i];
+        if (s.forces_active) {
+            launch_fill_f32(s, s.ext_u, 0.0f, s.nu());
+            launch_fill_f32(s, s.ext_v, 0.0f, s.nv());
+            launch_fill_f32(s, s.post_u, 0.0f, s.nu());
+            launch_fill_f32(s, s.post_v, 0.0f, s.nv());
+            s.forces_active = false;
+        }
+        s.stats.substeps = substeps[i];
+        s.stats.pcg_iterations_total = total_it[i];
+        s.stats.fluid_cells = s.h_fstats->count;
+        s.stats.timestamp = s.timestamp;
+        s.stats.prev_deltatime = s.prev_dt;
+        s.stats.max_particle_velocity = last_maxv[i];
+    }
+    CHECK_LAUNCH();
     return APIC_OK;
 }
 
@@ -479,7 +690,79 @@ apic_status apic_create(uint32_t width, uint32_t height, float cell_size,
 void apic_destroy(apic_handle* h) {
     if (!h) return;
     release(h->sim);
+    if (h->batch) h->batch->members -= 1;
     delete h;
+}
+
+apic_status apic_set_gravity_x(apic_handle* h, float gx) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    if (!std::isfinite(gx)) return fail(APIC_ERR_INVALID_ARGUMENT, "gx must be finite");
+    h->sim.gravity_x = gx;
+    return APIC_OK;
+}
+
+apic_status apic_device_bytes(apic_handle* h, size_t* bytes) {
+    if (!h || !bytes) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    *bytes = h->sim.device_bytes;
+    return APIC_OK;
+}
+
+apic_status apic_batch_create(apic_batch** out) {
+    if (!out) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(APIC_ERR_NO_DEVICE, "no HIP device found");
+    apic_batch* b = new (std::nothrow) apic_batch();
+    if (!b) return fail(APIC_ERR_OUT_OF_MEMORY, "host allocation failed");
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->e_sync, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->e_copied, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (b->e_sync) (void)hipEventDestroy(b->e_sync);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return fail(APIC_ERR_HIP, std::string("batch: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return APIC_OK;
+}
+
+apic_status apic_batch_destroy(apic_batch* b) {
+    if (!b) return APIC_OK;
+    if (b->members) return fail(APIC_ERR_INVALID_ARGUMENT, "the batch still has simulations");
+    (void)hipStreamSynchronize(b->stream);
+    if (b->pinned) (void)hipHostFree(b->pinned);
+    if (b->dev) (void)hipFree(b->dev);
+    (void)hipEventDestroy(b->e_sync);
+    (void)hipEventDestroy(b->e_copied);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+    return APIC_OK;
+}
+
+apic_status apic_batch_create_sim(apic_batch* b, uint32_t width, uint32_t height, float cell_size,
+                                  const apic_world_properties* world_properties, const float* xy, uint32_t n,
+                                  apic_handle** out) {
+    if (!b) return fail(APIC_ERR_INVALID_ARGUMENT, "null batch");
+    return create(width, height, cell_size, world_properties, xy != nullptr, xy, xy ? n : 0, out, b);
+}
+
+apic_status apic_batch_update(apic_batch* b, apic_handle* const* sims, uint32_t count,
+                              const apic_debug_options* opts, const float* frame_dt) {
+    if (!b) return fail(APIC_ERR_INVALID_ARGUMENT, "null batch");
+    if (count == 0) return APIC_OK;
+    if (!sims || !frame_dt) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    if (count > kMaxBatchRows) return fail(APIC_ERR_INVALID_ARGUMENT, "more than 4096 simulations in one update");
+    std::vector<Sim*> v(count);
+    std::vector<float> fdt(frame_dt, frame_dt + count);
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!sims[i] || sims[i]->batch != b)
+            return fail(APIC_ERR_INVALID_ARGUMENT, "a simulation is not part of this batch");
+        v[i] = &sims[i]->sim;
+        for (uint32_t j = 0; j < i; ++j)
+            if (v[j] == v[i]) return fail(APIC_ERR_INVALID_ARGUMENT, "a simulation is listed twice");
+    }
+    return batch_update(*b, v, opts && opts->disable_particle_sort, fdt);
 }
 
 apic_status apic_update(apic_handle* h, const apic_debug_options* opts, float frame_dt) {
@@ -509,10 +792,10 @@ apic_status apic_update(apic_handle* h, const apic_debug_options* opts, float fr
         total_it += s.stats.pcg_iterations;
     }
     if (s.forces_active) {
-        launch_fill_f32(s.stream, s.ext_u, 0.0f, s.nu());
-        launch_fill_f32(s.stream, s.ext_v, 0.0f, s.nv());
-        launch_fill_f32(s.stream, s.post_u, 0.0f, s.nu());
-        launch_fill_f32(s.stream, s.post_v, 0.0f, s.nv());
+        launch_fill_f32(s, s.ext_u, 0.0f, s.nu());
+        launch_fill_f32(s, s.ext_v, 0.0f, s.nv());
+        launch_fill_f32(s, s.post_u, 0.0f, s.nu());
+        launch_fill_f32(s, s.post_v, 0.0f, s.nv());
         CHECK_LAUNCH();
         s.forces_active = false;
     }

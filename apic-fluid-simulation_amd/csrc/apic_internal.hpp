@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "../../include/apic.h"
+#include "launch.hpp"
 
 namespace apic {
 
@@ -51,7 +52,11 @@ inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first) {
 // partials; the kernel that needs a reduced scalar sums the partials itself,
 // in the same fixed order in every workgroup, and applies the reference's exit
 // test.  Workgroup 0 records the decision here for later kernels and the host.
-constexpr int kRedBlocks = 512;          // workgroups (and partials) of every solver reduction
+constexpr int kRedBlocks = 512;          // most workgroups (and partials) of a solver reduction
+constexpr int kRedCells = 2048;          // cells per reduction workgroup below that
+inline uint32_t reduction_workgroups(uint32_t cells) {
+    return std::max<uint32_t>(1, std::min<uint32_t>(kRedBlocks, (cells + kRedCells - 1) / kRedCells));
+}
 constexpr int kMaxPcgIterations = 100;   // solve_pcg's cap (src/apic.rs:1894)
 struct PcgState {
     int32_t done;         // nonzero: every kernel of the PCG loop returns at once
@@ -86,8 +91,8 @@ struct StepState {
 struct Sim;
 
 // ---- launchers (kernels live in sort.hip, transfer.hip, pressure.hip, grid.hip)
-void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n);
-void launch_fill_f32(hipStream_t s, float* p, float v, size_t n);
+void launch_fill_u32(Sim& sim, uint32_t* p, uint32_t v, size_t n);
+void launch_fill_f32(Sim& sim, float* p, float v, size_t n);
 
 // sort.hip
 void launch_keygen(Sim& sim);
@@ -134,6 +139,7 @@ struct Sim {
     uint32_t W = 0, H = 0, cells = 0;
     float cell = 1.0f;
     apic_world_properties props{};
+    float gravity_x = 0.0f;  // horizontal gravity (an extension; the reference has none)
     apic_solver_config solver{};
     uint32_t num_particles = 0, num_chunks = 0;
     size_t nalloc = 0;  // num_chunks * 8
@@ -141,7 +147,10 @@ struct Sim {
     float prev_dt = 1.0f / 60.0f;
 
     hipStream_t stream = nullptr;
+    bool own_stream = true;   // false: the stream belongs to a batch this simulation is part of
     int device = 0;
+    Recorder* rec = nullptr;  // set while a batch records this simulation's launches
+    uint32_t nred = 1;        // workgroups (and partials) of the base-level solver reductions
 
     // particles: two SoA blocks (sort double buffer)
     float* pbuf[2] = {nullptr, nullptr};
@@ -234,7 +243,7 @@ struct Sim {
         return e;
     }
     void mark(int t, int which) {
-        if (profiling == 0 || (profiling == 1 && t < APIC_T_K_P2G)) return;
+        if (rec || profiling == 0 || (profiling == 1 && t < APIC_T_K_P2G)) return;
         hipEvent_t e = take_event();
         (void)hipEventRecord(e, stream);
         if (which == 0) {
@@ -285,6 +294,7 @@ struct Sim {
 
     // device allocations to free
     std::vector<void*> allocations;
+    size_t device_bytes = 0;
 
     size_t nu() const { return (size_t)cells + H; }
     size_t nv() const { return (size_t)cells + W; }
@@ -293,5 +303,17 @@ struct Sim {
         return pbuf[which < 0 ? cur : which] + (size_t)f * nalloc;
     }
 };
+
+// Launches kernel K for `s`: at once on its stream, or into the recorder of
+// the batch it is being stepped in.
+template <class K>
+inline void emit(Sim& s, const typename K::Args& a, uint32_t nwg, int k = 0) {
+    if (nwg == 0) return;
+    if (s.rec) {
+        s.rec->add(reinterpret_cast<const void*>(&batch_kernel<K>), sizeof(a), &a, K::kThreads, nwg, k);
+        return;
+    }
+    hipLaunchKernelGGL(one_kernel<K>, dim3(nwg), dim3(K::kThreads), 0, s.stream, a, k);
+}
 
 }  // namespace apic

@@ -1,0 +1,123 @@
+// batch.hip -- the recorder that turns the launches of several simulations
+// into one launch per kind (launch.hpp), and the issue of compiled launches.
+#include <algorithm>
+#include <queue>
+
+#include "launch.hpp"
+
+namespace apic {
+
+void Recorder::clear() {
+    groups_.clear();
+    chains_.clear();
+    cur_ = 0;
+}
+
+void Recorder::begin(uint32_t sim) {
+    cur_ = sim;
+    if (chains_.size() <= sim) chains_.resize(sim + 1);
+}
+
+int Recorder::find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k) {
+    for (size_t g = 0; g < groups_.size(); ++g) {
+        const Group& G = groups_[g];
+        if (G.fn == fn && G.occurrence == occurrence && G.k == k) return (int)g;
+    }
+    Group G;
+    G.fn = fn;
+    G.occurrence = occurrence;
+    G.arg_size = arg_size;
+    G.threads = threads;
+    G.k = k;
+    groups_.push_back(std::move(G));
+    return (int)groups_.size() - 1;
+}
+
+void Recorder::add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k) {
+    std::vector<int>& chain = chains_[cur_];
+    uint32_t occurrence = 0;
+    for (int g : chain)
+        if (groups_[g].fn == fn) ++occurrence;
+    const int g = find_or_add(fn, occurrence, arg_size, threads, k);
+    Group& G = groups_[g];
+    const uint8_t* p = static_cast<const uint8_t*>(args);
+    G.args.insert(G.args.end(), p, p + arg_size);
+    G.nwg.push_back(nwg);
+    chain.push_back(g);
+}
+
+std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size_t& used) {
+    std::vector<Launch> out;
+    const size_t ng = groups_.size();
+    if (ng == 0) return out;
+    // topological order of the groups over every simulation's chain, ties by
+    // first appearance (lowest group index)
+    std::vector<std::vector<int>> succ(ng);
+    std::vector<int> indeg(ng, 0);
+    for (const auto& chain : chains_)
+        for (size_t i = 1; i < chain.size(); ++i) {
+            succ[chain[i - 1]].push_back(chain[i]);
+            ++indeg[chain[i]];
+        }
+    std::priority_queue<int, std::vector<int>, std::greater<int>> ready;
+    for (size_t g = 0; g < ng; ++g)
+        if (indeg[g] == 0) ready.push((int)g);
+    std::vector<int> order;
+    order.reserve(ng);
+    while (!ready.empty()) {
+        const int g = ready.top();
+        ready.pop();
+        order.push_back(g);
+        for (int s : succ[g])
+            if (--indeg[s] == 0) ready.push(s);
+    }
+    if (order.size() != ng) return out;
+    auto align = [](size_t v) { return (v + 255) / 256 * 256; };
+    size_t total = align(used);
+    for (int g : order) {
+        const Group& G = groups_[g];
+        size_t nblk = 0;
+        for (uint32_t n : G.nwg) nblk += n;
+        total = align(total) + G.args.size();
+        total = align(total) + nblk * sizeof(uint32_t);
+    }
+    total = align(total);
+    if (host.size() < total) host.resize(total);
+    size_t off = used;
+    for (int g : order) {
+        const Group& G = groups_[g];
+        Launch L{};
+        L.fn = G.fn;
+        L.threads = G.threads;
+        L.k = G.k;
+        off = align(off);
+        L.tab_off = off;
+        std::memcpy(host.data() + off, G.args.data(), G.args.size());
+        off += G.args.size();
+        off = align(off);
+        L.map_off = off;
+        uint32_t* map = reinterpret_cast<uint32_t*>(host.data() + off);
+        size_t n = 0;
+        for (size_t row = 0; row < G.nwg.size(); ++row)
+            for (uint32_t wg = 0; wg < G.nwg[row]; ++wg) map[n++] = ((uint32_t)row << kMapWgBits) | wg;
+        off += n * sizeof(uint32_t);
+        L.grid = (uint32_t)n;
+        out.push_back(L);
+    }
+    used = align(off);
+    return out;
+}
+
+hipError_t issue(const std::vector<Recorder::Launch>& launches, const uint8_t* dev, hipStream_t stream, int k) {
+    for (const Recorder::Launch& L : launches) {
+        const void* tab = dev + L.tab_off;
+        const void* map = dev + L.map_off;
+        int kk = k >= 0 ? k : L.k;
+        void* params[3] = {&tab, &map, &kk};
+        hipError_t e = hipLaunchKernel(L.fn, dim3(L.grid), dim3(L.threads), params, 0, stream);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace apic

@@ -14,15 +14,20 @@ namespace apic {
 
 namespace {
 
-__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t v, size_t n) {
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
-}
-
-__global__ void fill_f32_kernel(float* __restrict__ p, float v, size_t n) {
-    size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
-}
+template <typename T>
+struct Fill {
+    static constexpr int kThreads = 256;
+    struct Args {
+        T* p;
+        T v;
+        size_t n;
+        uint32_t nwg;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        const size_t stride = (size_t)a.nwg * kThreads;
+        for (size_t i = (size_t)wg * kThreads + threadIdx.x; i < a.n; i += stride) a.p[i] = a.v;
+    }
+};
 
 __device__ inline bool obst(const uint8_t* o, int32_t i, int32_t j, uint32_t W, uint32_t H) {
     if (i < 0 || i >= (int32_t)W || j < 0 || j >= (int32_t)H) return true;
@@ -37,76 +42,115 @@ __device__ inline bool fluid_at(const uint8_t* f, int32_t i, int32_t j, uint32_t
 // min(cfl * dx / max|v|, frame time left), with cfl * dx / max|v| replaced by
 // the frame dt when max|v| <= 1e-5.  The host keeps the same sum of dts in f32
 // to know whether another substep follows.
-__global__ void substep_begin_kernel(StepState* ss, StepState* host, const uint32_t* __restrict__ maxv_bits,
-                                     float cfl_cell, float frame_dt, int first) {
-    StepState s = *ss;
-    if (first) s.elapsed = 0.0f;
-    const float maxv = __uint_as_float(*maxv_bits);
-    const float safe = maxv > 1e-5f ? cfl_cell / maxv : frame_dt;
-    const float rem = frame_dt - s.elapsed;
-    s.dt = safe < rem ? safe : rem;
-    s.maxv = maxv;
-    *ss = s;
-    if (host) *host = s;
-}
-
-__global__ void substep_end_kernel(StepState* ss, const int32_t* __restrict__ guard) {
-    if (guard && *guard == 0) return;
-    StepState s = *ss;
-    s.prev_dt = s.dt;
-    s.elapsed = s.elapsed + s.dt;
-    *ss = s;
-}
-
-__global__ void gradient_kernel(float* __restrict__ mac_u, float* __restrict__ mac_v, uint8_t* __restrict__ valid_u,
-                                uint8_t* __restrict__ valid_v, const float* __restrict__ pressure,
-                                const float* __restrict__ smoothed, const uint8_t* __restrict__ obstacle,
-                                const uint8_t* __restrict__ fluid, uint32_t W, uint32_t H, float inv_cell,
-                                int post, const float* __restrict__ post_u, const float* __restrict__ post_v,
-                                float dt_arg, const StepState* __restrict__ ss, const int32_t* __restrict__ guard) {
-    if (guard && *guard == 0) return;
-    const float dt = ss ? ss->dt : dt_arg;
-    const size_t nu = (size_t)W * H + H, nv = (size_t)W * H + W;
-    size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < nu) {
-        int32_t y = (int32_t)(f / (W + 1)), x = (int32_t)(f % (W + 1));
-        float u;
-        bool lo = obst(obstacle, x - 1, y, W, H), ro = obst(obstacle, x, y, W, H);
-        if (lo != ro) {
-            u = 0.0f;
-            valid_u[f] = 1;
-        } else {
-            bool lf = fluid_at(fluid, x - 1, y, W, H), rf = fluid_at(fluid, x, y, W, H);
-            float pd = 0.0f;
-            size_t li = (size_t)y * W + (x - 1), ri = (size_t)y * W + x;
-            if (lf && rf) pd = pressure[ri] - pressure[li];
-            else if (lf) pd = (-pressure[li]) / theta(smoothed[ri], smoothed[li]);
-            else if (rf) pd = pressure[ri] / theta(smoothed[li], smoothed[ri]);
-            u = mac_u[f] - pd * inv_cell;
-        }
-        if (post) u += post_u[f] * dt;
-        mac_u[f] = u;
+struct SubstepBegin {
+    static constexpr int kThreads = 64;
+    struct Args {
+        StepState* ss;
+        StepState* host;
+        const uint32_t* maxv_bits;
+        float cfl_cell, frame_dt;
+        int first;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        if (threadIdx.x != 0) return;
+        StepState s = *a.ss;
+        if (a.first) s.elapsed = 0.0f;
+        const float maxv = __uint_as_float(*a.maxv_bits);
+        const float safe = maxv > 1e-5f ? a.cfl_cell / maxv : a.frame_dt;
+        const float rem = a.frame_dt - s.elapsed;
+        s.dt = safe < rem ? safe : rem;
+        s.maxv = maxv;
+        *a.ss = s;
+        if (a.host) *a.host = s;
     }
-    if (f < nv) {
-        int32_t y = (int32_t)(f / W), x = (int32_t)(f % W);
-        float v;
-        bool bo = obst(obstacle, x, y - 1, W, H), to = obst(obstacle, x, y, W, H);
-        if (bo != to) {
-            v = 0.0f;
-            valid_v[f] = 1;
-        } else {
-            bool bf = fluid_at(fluid, x, y - 1, W, H), tf = fluid_at(fluid, x, y, W, H);
-            float pd = 0.0f;
-            size_t bi = (size_t)(y - 1) * W + x, ti = (size_t)y * W + x;
-            if (bf && tf) pd = pressure[ti] - pressure[bi];
-            else if (bf) pd = (-pressure[bi]) / theta(smoothed[ti], smoothed[bi]);
-            else if (tf) pd = pressure[ti] / theta(smoothed[bi], smoothed[ti]);
-            v = mac_v[f] - pd * inv_cell;
-        }
-        if (post) v += post_v[f] * dt;
-        mac_v[f] = v;
+};
+
+struct SubstepEnd {
+    static constexpr int kThreads = 64;
+    struct Args {
+        StepState* ss;
+        const int32_t* guard;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        if (threadIdx.x != 0) return;
+        if (a.guard && *a.guard == 0) return;
+        StepState s = *a.ss;
+        s.prev_dt = s.dt;
+        s.elapsed = s.elapsed + s.dt;
+        *a.ss = s;
     }
-}
+};
+
+struct Gradient {
+    static constexpr int kThreads = 256;
+    struct Args {
+        float* mac_u;
+        float* mac_v;
+        uint8_t* valid_u;
+        uint8_t* valid_v;
+        const float* pressure;
+        const float* smoothed;
+        const uint8_t* obstacle;
+        const uint8_t* fluid;
+        uint32_t W, H;
+        float inv_cell;
+        int post;
+        const float* post_u;
+        const float* post_v;
+        float dt_arg;
+        const StepState* ss;
+        const int32_t* guard;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        if (a.guard && *a.guard == 0) return;
+        const uint32_t W = a.W, H = a.H;
+        const uint8_t* __restrict__ obstacle = a.obstacle;
+        const uint8_t* __restrict__ fluid = a.fluid;
+        const float* __restrict__ pressure = a.pressure;
+        const float* __restrict__ smoothed = a.smoothed;
+        const float dt = a.ss ? a.ss->dt : a.dt_arg;
+        const size_t nu = (size_t)W * H + H, nv = (size_t)W * H + W;
+        size_t f = (size_t)wg * kThreads + threadIdx.x;
+        if (f < nu) {
+            int32_t y = (int32_t)(f / (W + 1)), x = (int32_t)(f % (W + 1));
+            float u;
+            bool lo = obst(obstacle, x - 1, y, W, H), ro = obst(obstacle, x, y, W, H);
+            if (lo != ro) {
+                u = 0.0f;
+                a.valid_u[f] = 1;
+            } else {
+                bool lf = fluid_at(fluid, x - 1, y, W, H), rf = fluid_at(fluid, x, y, W, H);
+                float pd = 0.0f;
+                size_t li = (size_t)y * W + (x - 1), ri = (size_t)y * W + x;
+                if (lf && rf) pd = pressure[ri] - pressure[li];
+                else if (lf) pd = (-pressure[li]) / theta(smoothed[ri], smoothed[li]);
+                else if (rf) pd = pressure[ri] / theta(smoothed[li], smoothed[ri]);
+                u = a.mac_u[f] - pd * a.inv_cell;
+            }
+            if (a.post) u += a.post_u[f] * dt;
+            a.mac_u[f] = u;
+        }
+        if (f < nv) {
+            int32_t y = (int32_t)(f / W), x = (int32_t)(f % W);
+            float v;
+            bool bo = obst(obstacle, x, y - 1, W, H), to = obst(obstacle, x, y, W, H);
+            if (bo != to) {
+                v = 0.0f;
+                a.valid_v[f] = 1;
+            } else {
+                bool bf = fluid_at(fluid, x, y - 1, W, H), tf = fluid_at(fluid, x, y, W, H);
+                float pd = 0.0f;
+                size_t bi = (size_t)(y - 1) * W + x, ti = (size_t)y * W + x;
+                if (bf && tf) pd = pressure[ti] - pressure[bi];
+                else if (bf) pd = (-pressure[bi]) / theta(smoothed[ti], smoothed[bi]);
+                else if (tf) pd = pressure[ti] / theta(smoothed[bi], smoothed[ti]);
+                v = a.mac_v[f] - pd * a.inv_cell;
+            }
+            if (a.post) v += a.post_v[f] * dt;
+            a.mac_v[f] = v;
+        }
+    }
+};
 
 __global__ void post_force_kernel(float* __restrict__ mac_u, float* __restrict__ mac_v,
                                   const float* __restrict__ post_u, const float* __restrict__ post_v, size_t nu,
@@ -125,65 +169,76 @@ constexpr int kExLW = kExTX + 2 * kExHalo, kExLH = kExTY + 2 * kExHalo;
 // passes give.  Per face: a valid face keeps its value; an invalid one takes
 // the mean of its valid neighbours, summed left, right, below, above
 // (src/apic.rs:2276-2310), and becomes valid.
-__global__ __launch_bounds__(256) void extrapolate_kernel(const float* __restrict__ in_u, const uint8_t* __restrict__ in_vu,
-                                                          float* __restrict__ out_u, uint8_t* __restrict__ out_vu,
-                                                          const float* __restrict__ in_v, const uint8_t* __restrict__ in_vv,
-                                                          float* __restrict__ out_v, uint8_t* __restrict__ out_vv,
-                                                          uint32_t W, uint32_t H, uint32_t tiles_u_x, uint32_t tiles_u,
-                                                          uint32_t tiles_v_x, const int32_t* __restrict__ guard) {
-    __shared__ float val[2][kExLH * kExLW];
-    __shared__ uint8_t ok[2][kExLH * kExLW];
-    if (guard && *guard == 0) return;
-    const bool is_u = blockIdx.x < tiles_u;
-    const uint32_t b = is_u ? blockIdx.x : blockIdx.x - tiles_u;
-    const uint32_t tx = is_u ? tiles_u_x : tiles_v_x;
-    const int32_t nx = (int32_t)(is_u ? W + 1 : W), ny = (int32_t)(is_u ? H : H + 1);
-    const float* __restrict__ in = is_u ? in_u : in_v;
-    const uint8_t* __restrict__ in_ok = is_u ? in_vu : in_vv;
-    float* __restrict__ out = is_u ? out_u : out_v;
-    uint8_t* __restrict__ out_ok = is_u ? out_vu : out_vv;
-    const int32_t X0 = (int32_t)(b % tx) * kExTX - kExHalo, Y0 = (int32_t)(b / tx) * kExTY - kExHalo;
-    for (int i = threadIdx.x; i < kExLH * kExLW; i += blockDim.x) {
-        const int32_t gx = X0 + i % kExLW, gy = Y0 + i / kExLW;
-        const bool inside = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
-        const size_t g = (size_t)gy * nx + gx;
-        val[0][i] = inside ? in[g] : 0.0f;
-        ok[0][i] = inside ? in_ok[g] : 0;
-    }
-    __syncthreads();
-#pragma unroll 1
-    for (int p = 0; p < 4; ++p) {
-        const int src = p & 1, dst = src ^ 1;
+struct Extrapolate {
+    static constexpr int kThreads = 256;
+    struct Args {
+        const float* in_u;
+        const uint8_t* in_vu;
+        float* out_u;
+        uint8_t* out_vu;
+        const float* in_v;
+        const uint8_t* in_vv;
+        float* out_v;
+        uint8_t* out_vv;
+        uint32_t W, H, tiles_u_x, tiles_u, tiles_v_x;
+        const int32_t* guard;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        __shared__ float val[2][kExLH * kExLW];
+        __shared__ uint8_t ok[2][kExLH * kExLW];
+        if (a.guard && *a.guard == 0) return;
+        const uint32_t W = a.W, H = a.H;
+        const bool is_u = wg < a.tiles_u;
+        const uint32_t b = is_u ? wg : wg - a.tiles_u;
+        const uint32_t tx = is_u ? a.tiles_u_x : a.tiles_v_x;
+        const int32_t nx = (int32_t)(is_u ? W + 1 : W), ny = (int32_t)(is_u ? H : H + 1);
+        const float* __restrict__ in = is_u ? a.in_u : a.in_v;
+        const uint8_t* __restrict__ in_ok = is_u ? a.in_vu : a.in_vv;
+        float* __restrict__ out = is_u ? a.out_u : a.out_v;
+        uint8_t* __restrict__ out_ok = is_u ? a.out_vu : a.out_vv;
+        const int32_t X0 = (int32_t)(b % tx) * kExTX - kExHalo, Y0 = (int32_t)(b / tx) * kExTY - kExHalo;
         for (int i = threadIdx.x; i < kExLH * kExLW; i += blockDim.x) {
-            const int lx = i % kExLW, ly = i / kExLW;
-            if (lx <= p || lx >= kExLW - 1 - p || ly <= p || ly >= kExLH - 1 - p) continue;
-            const int32_t gx = X0 + lx, gy = Y0 + ly;
-            if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-            if (ok[src][i]) {
-                val[dst][i] = val[src][i];
-                ok[dst][i] = 1;
-            } else {
-                float sum = 0.0f;
-                uint32_t cnt = 0;
-                if (gx > 0 && ok[src][i - 1]) { sum += val[src][i - 1]; ++cnt; }
-                if (gx < nx - 1 && ok[src][i + 1]) { sum += val[src][i + 1]; ++cnt; }
-                if (gy > 0 && ok[src][i - kExLW]) { sum += val[src][i - kExLW]; ++cnt; }
-                if (gy < ny - 1 && ok[src][i + kExLW]) { sum += val[src][i + kExLW]; ++cnt; }
-                if (cnt > 0) { val[dst][i] = sum / (float)cnt; ok[dst][i] = 1; }
-                else { val[dst][i] = val[src][i]; ok[dst][i] = 0; }
-            }
+            const int32_t gx = X0 + i % kExLW, gy = Y0 + i / kExLW;
+            const bool inside = gx >= 0 && gx < nx && gy >= 0 && gy < ny;
+            const size_t g = (size_t)gy * nx + gx;
+            val[0][i] = inside ? in[g] : 0.0f;
+            ok[0][i] = inside ? in_ok[g] : 0;
         }
         __syncthreads();
+#pragma unroll 1
+        for (int p = 0; p < 4; ++p) {
+            const int src = p & 1, dst = src ^ 1;
+            for (int i = threadIdx.x; i < kExLH * kExLW; i += blockDim.x) {
+                const int lx = i % kExLW, ly = i / kExLW;
+                if (lx <= p || lx >= kExLW - 1 - p || ly <= p || ly >= kExLH - 1 - p) continue;
+                const int32_t gx = X0 + lx, gy = Y0 + ly;
+                if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
+                if (ok[src][i]) {
+                    val[dst][i] = val[src][i];
+                    ok[dst][i] = 1;
+                } else {
+                    float sum = 0.0f;
+                    uint32_t cnt = 0;
+                    if (gx > 0 && ok[src][i - 1]) { sum += val[src][i - 1]; ++cnt; }
+                    if (gx < nx - 1 && ok[src][i + 1]) { sum += val[src][i + 1]; ++cnt; }
+                    if (gy > 0 && ok[src][i - kExLW]) { sum += val[src][i - kExLW]; ++cnt; }
+                    if (gy < ny - 1 && ok[src][i + kExLW]) { sum += val[src][i + kExLW]; ++cnt; }
+                    if (cnt > 0) { val[dst][i] = sum / (float)cnt; ok[dst][i] = 1; }
+                    else { val[dst][i] = val[src][i]; ok[dst][i] = 0; }
+                }
+            }
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < kExTY * kExTX; i += blockDim.x) {
+            const int lx = kExHalo + i % kExTX, ly = kExHalo + i / kExTX;
+            const int32_t gx = X0 + lx, gy = Y0 + ly;
+            if (gx >= nx || gy >= ny) continue;
+            const size_t g = (size_t)gy * nx + gx;
+            out[g] = val[0][ly * kExLW + lx];
+            out_ok[g] = ok[0][ly * kExLW + lx];
+        }
     }
-    for (int i = threadIdx.x; i < kExTY * kExTX; i += blockDim.x) {
-        const int lx = kExHalo + i % kExTX, ly = kExHalo + i / kExTX;
-        const int32_t gx = X0 + lx, gy = Y0 + ly;
-        if (gx >= nx || gy >= ny) continue;
-        const size_t g = (size_t)gy * nx + gx;
-        out[g] = val[0][ly * kExLW + lx];
-        out_ok[g] = ok[0][ly * kExLW + lx];
-    }
-}
+};
 
 __global__ void radial_force_kernel(float* __restrict__ ext_u, float* __restrict__ ext_v,
                                     float* __restrict__ post_u, float* __restrict__ post_v, uint32_t W,
@@ -237,34 +292,29 @@ unsigned blocks_for(size_t n, unsigned threads = 256) { return (unsigned)((n + t
 }  // namespace
 
 void launch_substep_begin(Sim& sim, bool first, float frame_dt) {
-    const float cfl_cell = sim.props.cfl * sim.cell;
-    hipLaunchKernelGGL(substep_begin_kernel, dim3(1), dim3(1), 0, sim.stream, sim.step, sim.h_step_d, sim.maxv_bits,
-                       cfl_cell, frame_dt, (int)first);
+    emit<SubstepBegin>(sim, {sim.step, sim.h_step_d, sim.maxv_bits, sim.props.cfl * sim.cell, frame_dt, (int)first}, 1);
 }
 
-void launch_substep_end(Sim& sim) {
-    hipLaunchKernelGGL(substep_end_kernel, dim3(1), dim3(1), 0, sim.stream, sim.step, sim.tail_guard);
-}
+void launch_substep_end(Sim& sim) { emit<SubstepEnd>(sim, {sim.step, sim.tail_guard}, 1); }
 
-void launch_fill_u32(hipStream_t s, uint32_t* p, uint32_t v, size_t n) {
+void launch_fill_u32(Sim& sim, uint32_t* p, uint32_t v, size_t n) {
     if (n == 0) return;
-    unsigned b = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(b), dim3(256), 0, s, p, v, n);
+    const uint32_t b = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
+    emit<Fill<uint32_t>>(sim, {p, v, n, b}, b);
 }
 
-void launch_fill_f32(hipStream_t s, float* p, float v, size_t n) {
+void launch_fill_f32(Sim& sim, float* p, float v, size_t n) {
     if (n == 0) return;
-    unsigned b = (unsigned)std::min<size_t>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(fill_f32_kernel, dim3(b), dim3(256), 0, s, p, v, n);
+    const uint32_t b = (uint32_t)std::min<size_t>((n + 255) / 256, 4096);
+    emit<Fill<float>>(sim, {p, v, n, b}, b);
 }
 
 void launch_gradient(Sim& sim, bool post_force, float dt) {
     size_t n = std::max(sim.nu(), sim.nv());
-    float inv_cell = 1.0f / sim.cell;
-    int post = post_force && sim.forces_active;
-    hipLaunchKernelGGL(gradient_kernel, dim3(blocks_for(n)), dim3(256), 0, sim.stream, sim.mac_u, sim.mac_v,
-                       sim.valid_u, sim.valid_v, sim.pressure, sim.smooth_a, sim.obstacle, sim.levels[0].fluid,
-                       sim.W, sim.H, inv_cell, post, sim.post_u, sim.post_v, dt, sim.step_src, sim.tail_guard);
+    Gradient::Args a{sim.mac_u, sim.mac_v, sim.valid_u, sim.valid_v, sim.pressure, sim.smooth_a, sim.obstacle,
+                     sim.levels[0].fluid, sim.W, sim.H, 1.0f / sim.cell, (int)(post_force && sim.forces_active),
+                     sim.post_u, sim.post_v, dt, sim.step_src, sim.tail_guard};
+    emit<Gradient>(sim, a, blocks_for(n));
 }
 
 void launch_post_force(Sim& sim, float dt) {
@@ -277,9 +327,9 @@ void launch_extrapolate(Sim& sim) {
     // into the spare buffers, which then become the current ones
     const uint32_t tux = (sim.W + 1 + kExTX - 1) / kExTX, tuy = (sim.H + kExTY - 1) / kExTY;
     const uint32_t tvx = (sim.W + kExTX - 1) / kExTX, tvy = (sim.H + 1 + kExTY - 1) / kExTY;
-    hipLaunchKernelGGL(extrapolate_kernel, dim3(tux * tuy + tvx * tvy), dim3(256), 0, sim.stream, sim.mac_u,
-                       sim.valid_u, sim.old_u, sim.nvalid_u, sim.mac_v, sim.valid_v, sim.old_v, sim.nvalid_v, sim.W,
-                       sim.H, tux, tux * tuy, tvx, sim.tail_guard);
+    Extrapolate::Args a{sim.mac_u, sim.valid_u, sim.old_u, sim.nvalid_u, sim.mac_v, sim.valid_v, sim.old_v,
+                        sim.nvalid_v, sim.W, sim.H, tux, tux * tuy, tvx, sim.tail_guard};
+    emit<Extrapolate>(sim, a, tux * tuy + tvx * tvy);
     std::swap(sim.mac_u, sim.old_u);
     std::swap(sim.mac_v, sim.old_v);
     std::swap(sim.valid_u, sim.nvalid_u);

@@ -1,0 +1,97 @@
+// launch.hpp -- one kernel body for one simulation or for many at once.
+//
+// Every kernel of the library is a struct K with
+//   static constexpr int kThreads;     workgroup size
+//   struct Args;                       all that one simulation's launch reads
+//   static __device__ void run(const Args& a, uint32_t wg, int k);
+//                                      the body for workgroup wg of that launch
+//                                      (k: the PCG iteration, else unused)
+// one_kernel<K> runs one simulation's launch with the arguments by value.
+// batch_kernel<K> runs the launches of many simulations as one grid: an
+// argument table holds one Args per simulation and a map gives, per
+// workgroup of the grid, (table row << kMapWgBits) | workgroup of that
+// simulation's launch.  The body cannot tell the two apart, so a simulation
+// computes the same bits alone and in a batch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstring>
+#include <vector>
+
+namespace apic {
+
+constexpr int kMapWgBits = 20;                           // workgroups per simulation launch: < 2^20
+constexpr uint32_t kMapWgMask = (1u << kMapWgBits) - 1;
+constexpr uint32_t kMaxBatchRows = 1u << (32 - kMapWgBits);  // simulations per batched launch
+
+template <class K>
+__global__ __launch_bounds__(K::kThreads) void one_kernel(typename K::Args a, int k) {
+    K::run(a, blockIdx.x, k);
+}
+
+// The argument table is read through the constant address space, as kernel
+// arguments are: the rows are never written while a launch runs, and the
+// loads of a workgroup's (uniform) row then go through the scalar unit.
+template <class K>
+__global__ __launch_bounds__(K::kThreads) void batch_kernel(const typename K::Args* __restrict__ tab,
+                                                            const uint32_t* __restrict__ map, int k) {
+    using ConstArgs = const __attribute__((address_space(4))) typename K::Args;
+    const uint32_t m = map[blockIdx.x];
+    const ConstArgs* row = (const ConstArgs*)tab + (m >> kMapWgBits);
+    K::run(*(const typename K::Args*)row, m & kMapWgMask, k);
+}
+
+// Collects the launches that the host-side launchers make for several
+// simulations, then issues each kind of launch once for all of them.
+//
+// Launches are grouped by (kernel, how many launches of that kernel the
+// simulation made before): the n-th smoothing pass of every simulation is one
+// group, the n-th V-cycle level on the way down another.  Groups are issued
+// in an order that keeps every simulation's own launch order (a topological
+// order of the groups, ties broken by first appearance), so a group depends
+// only on groups issued before it.
+class Recorder {
+public:
+    struct Launch {
+        const void* fn;      // batch_kernel<K>
+        uint32_t threads;
+        uint32_t grid;       // workgroups over all simulations of the group
+        size_t tab_off;      // argument table: offset in the uploaded arena
+        size_t map_off;      // workgroup map: offset in the uploaded arena
+        int k;               // uniform argument recorded with the group
+    };
+
+    void begin(uint32_t sim);  // the following launches belong to simulation `sim`
+    void add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k);
+    bool empty() const { return groups_.empty(); }
+    // Appends the tables and maps to host[used, ...) (growing it as needed,
+    // advancing `used`) and returns the launches in issue order, with offsets
+    // from host[0].  An empty result with launches recorded means that the
+    // simulations' launch orders conflict.  The caller uploads host[0, used)
+    // to a device arena and issues the launches with `issue`.
+    std::vector<Launch> compile(std::vector<uint8_t>& host, size_t& used);
+    void clear();
+
+private:
+    struct Group {
+        const void* fn;
+        uint32_t occurrence;
+        size_t arg_size;
+        uint32_t threads;
+        int k;
+        std::vector<uint8_t> args;
+        std::vector<uint32_t> nwg;  // per table row
+    };
+    std::vector<Group> groups_;
+    std::vector<std::vector<int>> chains_;  // per simulation: its groups in launch order
+    uint32_t cur_ = 0;
+    int find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k);
+};
+
+// Issues compiled launches whose tables were uploaded to `dev`.  k >= 0
+// replaces the recorded uniform argument (the PCG iteration).
+hipError_t issue(const std::vector<Recorder::Launch>& launches, const uint8_t* dev, hipStream_t stream, int k = -1);
+
+}  // namespace apic

@@ -45,6 +45,7 @@ namespace {
 constexpr float kSafety = 1e-5f;  // SAFTY, src/apic.rs:8
 constexpr float kOmega = 1.2f;
 constexpr int kThreads = 256;
+constexpr int kSolverThreads = kThreads;
 constexpr int kBottomThreads = 1024;
 constexpr int kSerialCap = 4096;
 
@@ -54,21 +55,32 @@ __device__ inline float atomic_load_f(const float* p) {
 
 // ---------------------------------------------------------------- assembly
 
-__global__ void smooth_density_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                      const uint8_t* __restrict__ fluid, uint32_t W, uint32_t cells) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cells) return;
-    if (!fluid[c]) { out[c] = in[c]; return; }
-    float sum = in[c] * 4.0f;
-    float weight = 4.0f;
-    float fl = (float)fluid[c - 1], fr = (float)fluid[c + 1], fb = (float)fluid[c - W], ft = (float)fluid[c + W];
-    sum += in[c - 1] * fl;
-    sum += in[c + 1] * fr;
-    sum += in[c - W] * fb;
-    sum += in[c + W] * ft;
-    weight += ((fl + fr) + fb) + ft;
-    out[c] = sum / weight;
-}
+struct SmoothDensity {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        const float* in;
+        float* out;
+        const uint8_t* fluid;
+        uint32_t W, cells;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        const uint32_t c = wg * kThreads + threadIdx.x;
+        if (c >= a.cells) return;
+        const float* __restrict__ in = a.in;
+        const uint8_t* __restrict__ fluid = a.fluid;
+        const uint32_t W = a.W;
+        if (!fluid[c]) { a.out[c] = in[c]; return; }
+        float sum = in[c] * 4.0f;
+        float weight = 4.0f;
+        float fl = (float)fluid[c - 1], fr = (float)fluid[c + 1], fb = (float)fluid[c - W], ft = (float)fluid[c + W];
+        sum += in[c - 1] * fl;
+        sum += in[c + 1] * fr;
+        sum += in[c - W] * fb;
+        sum += in[c + W] * ft;
+        weight += ((fl + fr) + fb) + ft;
+        a.out[c] = sum / weight;
+    }
+};
 
 // Per-workgroup (count, last raw index + 1) of the base fluid cells, reduced
 // by fstats_finalize_kernel: the padded-list quirk of prolongate needs both.
@@ -76,7 +88,7 @@ __device__ inline void fluid_partial(bool fl, uint32_t c, uint32_t& cnt, uint32_
     if (fl) { cnt += 1; last = max(last, c + 1); }
 }
 
-__device__ inline void write_fluid_partial(uint32_t cnt, uint32_t last, uint32_t* part) {
+__device__ inline void write_fluid_partial(uint32_t cnt, uint32_t last, uint32_t* part, uint32_t wg) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         cnt += __shfl_xor(cnt, o, 64);
@@ -89,17 +101,18 @@ __device__ inline void write_fluid_partial(uint32_t cnt, uint32_t last, uint32_t
     if (threadIdx.x == 0) {
         uint32_t c = 0, l = 0;
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { c += s_cnt[w]; l = max(l, s_last[w]); }
-        part[2 * blockIdx.x] = c;
-        part[2 * blockIdx.x + 1] = l;
+        part[2 * wg] = c;
+        part[2 * wg + 1] = l;
     }
 }
 
-__global__ __launch_bounds__(kThreads) void assemble_kernel(
+__device__ __forceinline__ void assemble(
     const float* __restrict__ mac_u, const float* __restrict__ mac_v, const float* __restrict__ smoothed,
     const uint8_t* __restrict__ fluid, const uint8_t* __restrict__ obstacle, uint32_t W, uint32_t cells, float cell,
-    LevelView L, float* __restrict__ pressure, int zero_pressure, uint32_t* __restrict__ part_fluid) {
+    const LevelView& L, float* __restrict__ pressure, int zero_pressure, uint32_t* __restrict__ part_fluid, uint32_t wg,
+    uint32_t nwg) {
     uint32_t cnt = 0, last = 0;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+    for (uint32_t c = wg * blockDim.x + threadIdx.x; c < cells; c += nwg * blockDim.x) {
         size_t pc = (size_t)c + W;
         bool fl = fluid[c];
         fluid_partial(fl, c, cnt, last);
@@ -139,36 +152,66 @@ __global__ __launch_bounds__(kThreads) void assemble_kernel(
             L.inv_diag[pc] = 0.0f;
         }
     }
-    write_fluid_partial(cnt, last, part_fluid);
+    write_fluid_partial(cnt, last, part_fluid, wg);
 }
+
+struct Assemble {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        const float* mac_u;
+        const float* mac_v;
+        const float* smoothed;
+        const uint8_t* fluid;
+        const uint8_t* obstacle;
+        uint32_t W, cells;
+        float cell;
+        LevelView L;
+        float* pressure;
+        int zero_pressure;
+        uint32_t* part_fluid;
+        uint32_t nwg;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        assemble(a.mac_u, a.mac_v, a.smoothed, a.fluid, a.obstacle, a.W, a.cells, a.cell, a.L, a.pressure,
+                 a.zero_pressure, a.part_fluid, wg, a.nwg);
+    }
+};
 
 __global__ __launch_bounds__(kThreads) void fluid_stats_kernel(const uint8_t* __restrict__ fluid, uint32_t cells,
                                                                 uint32_t* __restrict__ part_fluid) {
     uint32_t cnt = 0, last = 0;
     for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x)
         fluid_partial(fluid[c] != 0, c, cnt, last);
-    write_fluid_partial(cnt, last, part_fluid);
+    write_fluid_partial(cnt, last, part_fluid, blockIdx.x);
 }
 
-__global__ __launch_bounds__(kThreads) void fstats_finalize_kernel(const uint32_t* __restrict__ part_fluid,
-                                                                    FluidStats* fstats, FluidStats* host) {
-    uint32_t cnt = 0, last = 0;
-    for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) {
-        cnt += part_fluid[2 * b];
-        last = max(last, part_fluid[2 * b + 1]);
+struct FstatsFinalize {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        const uint32_t* part_fluid;
+        uint32_t n;
+        FluidStats* fstats;
+        FluidStats* host;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        uint32_t cnt = 0, last = 0;
+        for (uint32_t b = threadIdx.x; b < a.n; b += blockDim.x) {
+            cnt += a.part_fluid[2 * b];
+            last = max(last, a.part_fluid[2 * b + 1]);
+        }
+        __shared__ uint32_t s_cnt[kThreads], s_last[kThreads];
+        s_cnt[threadIdx.x] = cnt;
+        s_last[threadIdx.x] = last;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t c = 0, l = 0;
+            for (int t = 0; t < (int)blockDim.x; ++t) { c += s_cnt[t]; l = max(l, s_last[t]); }
+            a.fstats->count = c;
+            a.fstats->last_raw = l;
+            if (a.host) { a.host->count = c; a.host->last_raw = l; }
+        }
     }
-    __shared__ uint32_t s_cnt[kThreads], s_last[kThreads];
-    s_cnt[threadIdx.x] = cnt;
-    s_last[threadIdx.x] = last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t c = 0, l = 0;
-        for (int t = 0; t < (int)blockDim.x; ++t) { c += s_cnt[t]; l = max(l, s_last[t]); }
-        fstats->count = c;
-        fstats->last_raw = l;
-        if (host) { host->count = c; host->last_raw = l; }
-    }
-}
+};
 
 __device__ inline void mg_build_cell(const LevelView& F, const LevelView& C, uint32_t c) {
     uint32_t cx = c % C.w, cy = c / C.w;
@@ -204,24 +247,37 @@ __device__ inline void mg_build_cell(const LevelView& F, const LevelView& C, uin
     }
 }
 
-__global__ void mg_build_kernel(LevelView F, LevelView C) {
-    uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < C.cells) mg_build_cell(F, C, c);
-}
+struct MgBuild {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        LevelView F, C;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        const uint32_t c = wg * kThreads + threadIdx.x;
+        if (c < a.C.cells) mg_build_cell(a.F, a.C, c);
+    }
+};
 
 struct LevelSet {
     LevelView lv[16];
     int count;  // number of levels in lv (lv[0] = base)
 };
 
-__global__ __launch_bounds__(kBottomThreads) void mg_build_bottom_kernel(LevelSet ls, int first) {
-    for (int l = first; l < ls.count; ++l) {
-        const LevelView F = ls.lv[l - 1];
-        const LevelView C = ls.lv[l];
-        for (uint32_t c = threadIdx.x; c < C.cells; c += blockDim.x) mg_build_cell(F, C, c);
-        __syncthreads();
+struct MgBuildBottom {
+    static constexpr int kThreads = kBottomThreads;
+    struct Args {
+        LevelSet ls;
+        int first;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        for (int l = a.first; l < a.ls.count; ++l) {
+            const LevelView F = a.ls.lv[l - 1];
+            const LevelView C = a.ls.lv[l];
+            for (uint32_t c = threadIdx.x; c < C.cells; c += blockDim.x) mg_build_cell(F, C, c);
+            __syncthreads();
+        }
     }
-}
+};
 
 // ------------------------------------------------------------------ V-cycle
 
@@ -352,18 +408,12 @@ __device__ inline float block_max(float v, float* scratch) {
     return s;
 }
 
-// Sum (or max) of the kRedBlocks partials of one reduction.  Every workgroup
-// that calls it uses kThreads threads and the same order, so all of them see
-// the same value.
-__device__ inline float sum_partials(const float* __restrict__ part, float* scratch, int n = kRedBlocks) {
+// Sum of the n partials of one reduction.  Every workgroup that calls it uses
+// kThreads threads and the same order, so all of them see the same value.
+__device__ inline float sum_partials(const float* __restrict__ part, float* scratch, int n) {
     float v = 0.0f;
     for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
     return block_sum(v, scratch);
-}
-__device__ inline float max_partials(const float* __restrict__ part, float* scratch) {
-    float v = 0.0f;
-    for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) v = fast_max(v, part[i]);
-    return block_max(v, scratch);
 }
 
 // ---- the one-workgroup bottom of the V-cycle
@@ -615,20 +665,29 @@ __device__ void bottom_body(const Views& V, int first, int last, float* cg_s, fl
 }
 
 // Bottom levels in global memory (used when they do not fit in LDS).
-__global__ __launch_bounds__(kBottomThreads) void bottom_kernel(LevelSet ls, int first, float* cg_s, float* cg_ap,
-                                                               float* cg_r, const int32_t* __restrict__ done) {
-    __shared__ float sprod[kSerialCap];
-    __shared__ float scratch[kBottomThreads / 64];
-    if (*done) return;
-    GlobalViews V{&ls};
-    bottom_body(V, first, ls.count - 1, cg_s, cg_ap, cg_r, sprod, scratch);
-}
+struct BottomGlobal {
+    static constexpr int kThreads = kBottomThreads;
+    struct Args {
+        LevelSet ls;
+        int first;
+        float* cg_s;
+        float* cg_ap;
+        float* cg_r;
+        const int32_t* done;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        __shared__ float sprod[kSerialCap];
+        __shared__ float scratch[kBottomThreads / 64];
+        if (*a.done) return;
+        GlobalViews V{&a.ls};
+        bottom_body(V, a.first, a.ls.count - 1, a.cg_s, a.cg_ap, a.cg_r, sprod, scratch);
+    }
+};
 
 // Bottom levels copied to LDS: the same arithmetic without a global-memory
 // round trip between the dependent phases.  The errors and residuals of every
 // bottom level are copied back (the finer level's prolongation reads them).
-__global__ __launch_bounds__(kBottomThreads) void bottom_lds_kernel(LevelSet ls, int first,
-                                                                   const int32_t* __restrict__ done) {
+__device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const int32_t* __restrict__ done) {
     __shared__ float f[kBottomLdsFloats];
     __shared__ uint8_t fl[kBottomLdsFlags];
     __shared__ float scratch[kBottomThreads / 64];
@@ -670,29 +729,48 @@ __global__ __launch_bounds__(kBottomThreads) void bottom_lds_kernel(LevelSet ls,
     }
 }
 
+struct BottomLds {
+    static constexpr int kThreads = kBottomThreads;
+    struct Args {
+        LevelSet ls;
+        int first;
+        const int32_t* done;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) { bottom_lds(a.ls, a.first, a.done); }
+};
+
 // ---------------------------------------------------------------------- PCG
 
-__global__ void pcg_reset_kernel(PcgState* st, double tol2, int max_iters) {
-    st->done = 0;
-    st->iters = 0;
-    st->exit_reason = 0;
-    st->maxerr = st->delta = st->alpha = 0.0f;
-    st->bnorm2 = st->rnorm2 = 0.0;
-    st->tol2 = tol2;
-    st->gamma_d[0] = st->gamma_d[1] = 0.0;
-    st->max_iters = max_iters;
-}
+struct PcgReset {
+    static constexpr int kThreads = 64;
+    struct Args {
+        PcgState* st;
+        double tol2;
+        int max_iters;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        if (threadIdx.x != 0) return;
+        PcgState* st = a.st;
+        st->done = 0;
+        st->iters = 0;
+        st->exit_reason = 0;
+        st->maxerr = st->delta = st->alpha = 0.0f;
+        st->bnorm2 = st->rnorm2 = 0.0;
+        st->tol2 = a.tol2;
+        st->gamma_d[0] = st->gamma_d[1] = 0.0;
+        st->max_iters = a.max_iters;
+    }
+};
 
 // Relative mode, before the first V-cycle: b = r is kept for the residual
 // recomputation, the FP64 pressure starts from the FP32 one, and the partials
 // of ||b||^2 over the fluid cells.
-__global__ __launch_bounds__(kThreads) void pcg_bnorm_kernel(LevelView L, float* __restrict__ rhs,
-                                                              double* __restrict__ part,
-                                                              const float* __restrict__ pressure,
-                                                              double* __restrict__ pd) {
+__device__ __forceinline__ void pcg_bnorm(const LevelView& L, float* __restrict__ rhs, double* __restrict__ part,
+                                 const float* __restrict__ pressure, double* __restrict__ pd, uint32_t wg,
+                                 uint32_t nwg) {
     __shared__ double dscratch[kThreads / 64];
     double acc = 0.0;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+    for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
         pd[c] = (double)pressure[c];
         if (!L.fluid[c]) continue;
         const size_t pc = (size_t)c + L.w;
@@ -701,39 +779,60 @@ __global__ __launch_bounds__(kThreads) void pcg_bnorm_kernel(LevelView L, float*
         acc += (double)b * (double)b;
     }
     acc = block_sum(acc, dscratch);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part[wg] = acc;
 }
 
-__global__ __launch_bounds__(kThreads) void pcg_bnorm_finalize_kernel(PcgState* st, const double* __restrict__ part) {
-    __shared__ double dscratch[kThreads / 64];
-    double b = 0.0;
-    for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) b += part[i];
-    b = block_sum(b, dscratch);
-    if (threadIdx.x == 0) {
-        st->bnorm2 = b;
-        if (!(b > 0.0)) { st->done = 1; st->exit_reason = 1; }  // b = 0: p = 0 is the solution
+struct PcgBnorm {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        LevelView L;
+        float* rhs;
+        double* part;
+        const float* pressure;
+        double* pd;
+        uint32_t nwg;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) { pcg_bnorm(a.L, a.rhs, a.part, a.pressure, a.pd, wg, a.nwg); }
+};
+
+struct PcgBnormFinalize {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        PcgState* st;
+        const double* part;
+        uint32_t n;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        __shared__ double dscratch[kThreads / 64];
+        double b = 0.0;
+        for (uint32_t i = threadIdx.x; i < a.n; i += blockDim.x) b += a.part[i];
+        b = block_sum(b, dscratch);
+        if (threadIdx.x == 0) {
+            a.st->bnorm2 = b;
+            if (!(b > 0.0)) { a.st->done = 1; a.st->exit_reason = 1; }  // b = 0: p = 0 is the solution
+        }
     }
-}
+};
 
 // Relative mode, after iteration k's update, every recompute_every
 // iterations (due): r is replaced by the true residual b - A p evaluated in
 // FP64 against the FP64 pressure (the product with the same fused operations
 // as apply_matrix_and_dot, zero outside the grid).  part_out receives the
 // partials of ||r||^2 that the residual test reads (part_in's when not due).
-__global__ __launch_bounds__(kThreads) void true_residual_kernel(LevelView L, const float* __restrict__ rhs,
-                                                                  const double* __restrict__ pd, const PcgState* st,
-                                                                  const double* __restrict__ part_in,
-                                                                  double* __restrict__ part_out, int due) {
+__device__ __forceinline__ void true_residual(const LevelView& L, const float* __restrict__ rhs,
+                                     const double* __restrict__ pd, const PcgState* st,
+                                     const double* __restrict__ part_in, double* __restrict__ part_out, int due,
+                                     uint32_t wg, uint32_t nwg) {
     __shared__ double dscratch[kThreads / 64];
     if (st->done) return;
     if (!due) {
-        if (threadIdx.x == 0) part_out[blockIdx.x] = part_in[blockIdx.x];
+        if (threadIdx.x == 0) part_out[wg] = part_in[wg];
         return;
     }
     const int64_t w = L.w, n = L.cells;
     auto pa = [&](int64_t raw) { return raw >= 0 && raw < n ? pd[raw] : 0.0; };
     double acc = 0.0;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+    for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
         if (!L.fluid[c]) continue;
         const size_t pc = (size_t)c + L.w;
         const int64_t ci = c;
@@ -747,8 +846,27 @@ __global__ __launch_bounds__(kThreads) void true_residual_kernel(LevelView L, co
         acc += t * t;
     }
     acc = block_sum(acc, dscratch);
-    if (threadIdx.x == 0) part_out[blockIdx.x] = acc;
+    if (threadIdx.x == 0) part_out[wg] = acc;
 }
+
+// Relative mode, after the update of iteration k (the uniform argument).
+struct TrueResidual {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        LevelView L;
+        const float* rhs;
+        const double* pd;
+        const PcgState* st;
+        const double* part_in;
+        double* part_out;
+        int every;  // recompute every `every` iterations (0: never)
+        uint32_t nwg;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
+        const int due = a.every > 0 && (k + 1) % a.every == 0;
+        true_residual(a.L, a.rhs, a.pd, a.st, a.part_in, a.part_out, due, wg, a.nwg);
+    }
+};
 
 __global__ void zero_pressure_kernel(const uint8_t* __restrict__ fluid, float* __restrict__ pressure,
                                      uint32_t cells) {
@@ -757,13 +875,13 @@ __global__ void zero_pressure_kernel(const uint8_t* __restrict__ fluid, float* _
 }
 
 // Relative mode: breakdown when r.z <= 0 (or not finite), the iteration cap.
-__device__ inline bool gamma_exit_rel(PcgState* st, double g, int k) {
+__device__ inline bool gamma_exit_rel(PcgState* st, double g, int k, bool lead) {
     if (!(g > 0.0)) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
+        if (lead) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
         return true;
     }
     if (k >= st->max_iters) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 5; st->iters = k; }
+        if (lead) { st->done = 1; st->exit_reason = 5; st->iters = k; }
         return true;
     }
     return false;
@@ -771,13 +889,13 @@ __device__ inline bool gamma_exit_rel(PcgState* st, double g, int k) {
 
 // Exit tests on gamma = r.z before iteration k (src/apic.rs:1890, 1938) and
 // the iteration cap.  Returns true when the solve ends here.
-__device__ inline bool gamma_exit(PcgState* st, float g, int k) {
+__device__ inline bool gamma_exit(PcgState* st, float g, int k, bool lead) {
     if (g < kSafety) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
+        if (lead) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
         return true;
     }
     if (k >= kMaxPcgIterations) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 5; st->iters = k; }
+        if (lead) { st->done = 1; st->exit_reason = 5; st->iters = k; }
         return true;
     }
     return false;
@@ -786,12 +904,11 @@ __device__ inline bool gamma_exit(PcgState* st, float g, int k) {
 // Iteration k: gamma from the V-cycle's partials, s = z + beta*s (s = z when
 // k = 0), As = A s and the partials of delta = s.As (apply_matrix_and_dot,
 // src/apic.rs:1823-1863; search update :1945-1954).
-__global__ __launch_bounds__(kThreads) void spmv_dot_kernel(LevelView L, const float* __restrict__ s_old,
-                                                             float* __restrict__ s_new, float* __restrict__ As,
-                                                             PcgState* st, const float* __restrict__ part_gamma,
-                                                             float* __restrict__ part_delta, int k,
-                                                             const double* __restrict__ part_gamma_d,
-                                                             double* __restrict__ part_delta_d, int ngam) {
+__device__ __forceinline__ void spmv_dot(const LevelView& L, const float* __restrict__ s_old, float* __restrict__ s_new,
+                                float* __restrict__ As, PcgState* st, const float* __restrict__ part_gamma,
+                                float* __restrict__ part_delta, int k, const double* __restrict__ part_gamma_d,
+                                double* __restrict__ part_delta_d, int ngam, uint32_t wg, uint32_t nwg) {
+    const bool lead = wg == 0 && threadIdx.x == 0;
     __shared__ float scratch[kThreads / 64];
     __shared__ double dscratch[kThreads / 64];
     if (st->done) return;
@@ -801,19 +918,19 @@ __global__ __launch_bounds__(kThreads) void spmv_dot_kernel(LevelView L, const f
         double g = 0.0;
         for (int i = threadIdx.x; i < ngam; i += blockDim.x) g += part_gamma_d[i];
         g = block_sum(g, dscratch);
-        if (gamma_exit_rel(st, g, k)) return;
+        if (gamma_exit_rel(st, g, k, lead)) return;
         beta = first ? 0.0f : (float)(g / st->gamma_d[(k - 1) & 1]);
-        if (blockIdx.x == 0 && threadIdx.x == 0) { st->gamma_d[k & 1] = g; st->iters = k + 1; }
+        if (lead) { st->gamma_d[k & 1] = g; st->iters = k + 1; }
     } else {
         const float g = sum_partials(part_gamma, scratch, ngam);
-        if (gamma_exit(st, g, k)) return;
+        if (gamma_exit(st, g, k, lead)) return;
         beta = first ? 0.0f : g / st->gamma[k - 1];
-        if (blockIdx.x == 0 && threadIdx.x == 0) { st->gamma[k] = g; st->iters = k + 1; }
+        if (lead) { st->gamma[k] = g; st->iters = k + 1; }
     }
     const size_t w = L.w;
     float prod = 0.0f;
     double dprod = 0.0;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+    for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
         if (!L.fluid[c]) continue;
         size_t pc = (size_t)c + w;
         auto sv = [&](size_t i) { return first ? L.e[i] : L.e[i] + (beta * s_old[i]); };
@@ -830,39 +947,58 @@ __global__ __launch_bounds__(kThreads) void spmv_dot_kernel(LevelView L, const f
     }
     if (part_delta_d) {
         const double bs = block_sum(dprod, dscratch);
-        if (threadIdx.x == 0) part_delta_d[blockIdx.x] = bs;
+        if (threadIdx.x == 0) part_delta_d[wg] = bs;
         return;
     }
     float bs = block_sum(prod, scratch);
-    if (threadIdx.x == 0) part_delta[blockIdx.x] = bs;
+    if (threadIdx.x == 0) part_delta[wg] = bs;
 }
+
+// Iteration k (the uniform argument): the search vectors alternate between
+// search[0] and search[1].
+struct SpmvDot {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        LevelView L;
+        float* search[2];
+        float* As;
+        PcgState* st;
+        const float* part_gamma;
+        float* part_delta;
+        const double* part_gamma_d;  // relative mode (else null)
+        double* part_delta_d;
+        int ngam;
+        uint32_t nwg;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
+        spmv_dot(a.L, a.search[k & 1], a.search[(k + 1) & 1], a.As, a.st, a.part_gamma, a.part_delta, k,
+                 a.part_gamma_d, a.part_delta_d, a.ngam, wg, a.nwg);
+    }
+};
 
 // delta from the partials (exit when |delta| < 1e-5, src/apic.rs:1903), then
 // p += alpha*s; r = fma(-alpha, As, r); partials of max |r| (:1907-1928)
-__global__ __launch_bounds__(kThreads) void axpy_max_kernel(LevelView L, const float* __restrict__ s,
-                                                             const float* __restrict__ As,
-                                                             float* __restrict__ pressure, PcgState* st,
-                                                             const float* __restrict__ part_delta,
-                                                             float* __restrict__ part_max, int k,
-                                                             const double* __restrict__ part_delta_d,
-                                                             double* __restrict__ part_rr_d,
-                                                             double* __restrict__ pd) {
+__device__ __forceinline__ void axpy_max(const LevelView& L, const float* __restrict__ s, const float* __restrict__ As,
+                                float* __restrict__ pressure, PcgState* st, const float* __restrict__ part_delta,
+                                float* __restrict__ part_max, int k, const double* __restrict__ part_delta_d,
+                                double* __restrict__ part_rr_d, double* __restrict__ pd, uint32_t wg, uint32_t nwg) {
+    const bool lead = wg == 0 && threadIdx.x == 0;
     __shared__ float scratch[kThreads / 64];
     __shared__ double dscratch[kThreads / 64];
     if (st->done) return;
     if (part_delta_d) {  // relative mode: alpha = r.z / s.As in FP64, partials of ||r||^2
         double d = 0.0;
-        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) d += part_delta_d[i];
+        for (uint32_t i = threadIdx.x; i < nwg; i += blockDim.x) d += part_delta_d[i];
         d = block_sum(d, dscratch);
         if (!(d > 0.0)) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 2; st->delta = (float)d; }
+            if (lead) { st->done = 1; st->exit_reason = 2; st->delta = (float)d; }
             return;
         }
         const double alpha_d = st->gamma_d[k & 1] / d;
         const float alpha = (float)alpha_d;
-        if (blockIdx.x == 0 && threadIdx.x == 0) { st->delta = (float)d; st->alpha = alpha; }
+        if (lead) { st->delta = (float)d; st->alpha = alpha; }
         double rr = 0.0;
-        for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+        for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
             if (!L.fluid[c]) continue;
             size_t pc = (size_t)c + L.w;
             const double np = pd[c] + alpha_d * (double)s[pc];
@@ -873,18 +1009,18 @@ __global__ __launch_bounds__(kThreads) void axpy_max_kernel(LevelView L, const f
             rr += (double)nr * (double)nr;
         }
         rr = block_sum(rr, dscratch);
-        if (threadIdx.x == 0) part_rr_d[blockIdx.x] = rr;
+        if (threadIdx.x == 0) part_rr_d[wg] = rr;
         return;
     }
-    const float d = sum_partials(part_delta, scratch);
+    const float d = sum_partials(part_delta, scratch, (int)nwg);
     if (fabsf(d) < kSafety) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 2; st->delta = d; }
+        if (lead) { st->done = 1; st->exit_reason = 2; st->delta = d; }
         return;
     }
     const float alpha = st->gamma[k] / d;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { st->delta = d; st->alpha = alpha; }
+    if (lead) { st->delta = d; st->alpha = alpha; }
     float m = 0.0f;
-    for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < L.cells; c += gridDim.x * blockDim.x) {
+    for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
         if (!L.fluid[c]) continue;
         size_t pc = (size_t)c + L.w;
         pressure[c] += alpha * s[pc];
@@ -893,38 +1029,70 @@ __global__ __launch_bounds__(kThreads) void axpy_max_kernel(LevelView L, const f
         m = fast_max(m, fabsf(nr));
     }
     float bm = block_max(m, scratch);
-    if (threadIdx.x == 0) part_max[blockIdx.x] = bm;
+    if (threadIdx.x == 0) part_max[wg] = bm;
 }
+
+struct AxpyMax {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        LevelView L;
+        float* search[2];
+        const float* As;
+        float* pressure;
+        PcgState* st;
+        const float* part_delta;
+        float* part_max;
+        const double* part_delta_d;  // relative mode (else null)
+        double* part_rr_d;
+        double* pd;
+        uint32_t nwg;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
+        axpy_max(a.L, a.search[(k + 1) & 1], a.As, a.pressure, a.st, a.part_delta, a.part_max, k, a.part_delta_d,
+                 a.part_rr_d, a.pd, wg, a.nwg);
+    }
+};
 
 // The gamma test after k iterations, run at the host's check points so that
 // the recorded state is final when the host reads it.
-__global__ __launch_bounds__(kThreads) void decide_kernel(PcgState* st, const float* __restrict__ part_gamma, int k,
-                                                           PcgState* host, const double* __restrict__ part_gamma_d,
-                                                           int ngam) {
-    __shared__ float scratch[kThreads / 64];
-    __shared__ double dscratch[kThreads / 64];
-    if (!st->done) {
-        if (part_gamma_d) {
-            double g = 0.0;
-            for (int i = threadIdx.x; i < ngam; i += blockDim.x) g += part_gamma_d[i];
-            g = block_sum(g, dscratch);
-            (void)gamma_exit_rel(st, g, k);
-        } else {
-            const float g = sum_partials(part_gamma, scratch, ngam);
-            (void)gamma_exit(st, g, k);
+struct Decide {
+    static constexpr int kThreads = kSolverThreads;
+    struct Args {
+        PcgState* st;
+        const float* part_gamma;
+        PcgState* host;
+        const double* part_gamma_d;  // relative mode (else null)
+        int ngam;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int k) {
+        __shared__ float scratch[kThreads / 64];
+        __shared__ double dscratch[kThreads / 64];
+        PcgState* st = a.st;
+        const bool lead = threadIdx.x == 0;
+        if (!st->done) {
+            if (a.part_gamma_d) {
+                double g = 0.0;
+                for (int i = threadIdx.x; i < a.ngam; i += blockDim.x) g += a.part_gamma_d[i];
+                g = block_sum(g, dscratch);
+                (void)gamma_exit_rel(st, g, k, lead);
+            } else {
+                const float g = sum_partials(a.part_gamma, scratch, a.ngam);
+                (void)gamma_exit(st, g, k, lead);
+            }
+        }
+        if (a.host && threadIdx.x == 0) {  // the scalars the host reads (not the gamma history)
+            PcgState* host = a.host;
+            host->done = st->done;
+            host->iters = st->iters;
+            host->exit_reason = st->exit_reason;
+            host->maxerr = st->maxerr;
+            host->delta = st->delta;
+            host->alpha = st->alpha;
+            host->bnorm2 = st->bnorm2;
+            host->rnorm2 = st->rnorm2;
         }
     }
-    if (host && threadIdx.x == 0) {  // the scalars the host reads (not the gamma history)
-        host->done = st->done;
-        host->iters = st->iters;
-        host->exit_reason = st->exit_reason;
-        host->maxerr = st->maxerr;
-        host->delta = st->delta;
-        host->alpha = st->alpha;
-        host->bnorm2 = st->bnorm2;
-        host->rnorm2 = st->rnorm2;
-    }
-}
+};
 
 // ---- fused V-cycle kernels: one launch per level on the way down and one on
 // the way up.  A workgroup takes a kVT x kVT tile of the level and loads it
@@ -997,35 +1165,35 @@ __device__ inline float vsor(const VTile& t, int i, float own, float er, float e
 // the base level, in a PCG iteration, every workgroup first applies the
 // residual test of the previous update (max |r| < solve_error,
 // src/apic.rs:1930, or the relative test).
-__global__ __launch_bounds__(256) void vdown_kernel(LevelView F, LevelView C, int base, int fuse_red,
-                                                    const int32_t* __restrict__ done, uint32_t tiles_x,
-                                                    PcgState* st, int check, float solve_error,
-                                                    const float* __restrict__ part_max,
-                                                    const double* __restrict__ part_rr) {
+__device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, int base, int fuse_red,
+                             const int32_t* __restrict__ done, uint32_t tiles_x, PcgState* st, int check,
+                             float solve_error, const float* __restrict__ part_max, const double* __restrict__ part_rr,
+                             uint32_t nred, uint32_t wg) {
     __shared__ VTile t;
     __shared__ float scratch[256 / 64];
     __shared__ double dscratch[256 / 64];
     if (*done) return;
+    const bool lead = wg == 0 && threadIdx.x == 0;
     if (check && part_rr) {
         double rr = 0.0;
-        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) rr += part_rr[i];
+        for (uint32_t i = threadIdx.x; i < nred; i += blockDim.x) rr += part_rr[i];
         rr = block_sum(rr, dscratch);
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->rnorm2 = rr;
+        if (lead) st->rnorm2 = rr;
         if (rr <= st->tol2 * st->bnorm2) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 3; }
+            if (lead) { st->done = 1; st->exit_reason = 3; }
             return;
         }
     } else if (check) {
         float m = 0.0f;
-        for (int i = threadIdx.x; i < kRedBlocks; i += blockDim.x) m = fast_max(m, part_max[i]);
+        for (uint32_t i = threadIdx.x; i < nred; i += blockDim.x) m = fast_max(m, part_max[i]);
         m = block_max(m, scratch);
-        if (blockIdx.x == 0 && threadIdx.x == 0) st->maxerr = m;
+        if (lead) st->maxerr = m;
         if (m < solve_error) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 1; st->exit_reason = 3; }
+            if (lead) { st->done = 1; st->exit_reason = 3; }
             return;
         }
     }
-    const int32_t X0 = (int32_t)(blockIdx.x % tiles_x) * kVT, Y0 = (int32_t)(blockIdx.x / tiles_x) * kVT;
+    const int32_t X0 = (int32_t)(wg % tiles_x) * kVT, Y0 = (int32_t)(wg / tiles_x) * kVT;
     const int32_t w = (int32_t)F.w, h = (int32_t)F.h;
     const int32_t cw = (int32_t)C.w, ch = (int32_t)C.h;
     // the coarse cells' flags and inverse diagonals, loaded with the tile
@@ -1097,14 +1265,35 @@ __global__ __launch_bounds__(256) void vdown_kernel(LevelView F, LevelView C, in
     }
 }
 
+// One level on the way down.  On the base level of a PCG iteration (check)
+// the residual test reads the nred partials of the previous update.
+struct VDown {
+    static constexpr int kThreads = 256;
+    struct Args {
+        LevelView F, C;
+        int base, fuse_red;
+        const int32_t* done;
+        uint32_t tiles_x;
+        PcgState* st;
+        int check;
+        float solve_error;
+        const float* part_max;
+        const double* part_rr;  // relative mode (else null)
+        uint32_t nred;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        vdown(a.F, a.C, a.base, a.fuse_red, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max, a.part_rr,
+              a.nred, wg);
+    }
+};
+
 // Up: the two colour sweeps of smooth_level's reverse pass on level L with
 // C's error prolongated on the fly (prolongate + sor_cell modes 1 and 2); on
 // the base level, with the padded-list repeats of prolongate and the partials
 // of r.z over the tiles a workgroup takes (dot: 1 FP32, 2 FP64).
-__global__ __launch_bounds__(256) void vup_kernel(LevelView L, LevelView C, const FluidStats* __restrict__ fs,
-                                                  const int32_t* __restrict__ done, uint32_t tiles_x,
-                                                  int dot, float* __restrict__ part_gamma,
-                                                  double* __restrict__ part_gamma_d) {
+__device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, const FluidStats* __restrict__ fs,
+                           const int32_t* __restrict__ done, uint32_t tiles_x, int dot, float* __restrict__ part_gamma,
+                           double* __restrict__ part_gamma_d, uint32_t wg) {
     __shared__ VTile t;
     __shared__ float ce[kVC * kVC];
     __shared__ float scratch[256 / 64];
@@ -1115,7 +1304,7 @@ __global__ __launch_bounds__(256) void vup_kernel(LevelView L, LevelView C, cons
     float prod = 0.0f;
     double dprod = 0.0;
     {
-        const uint32_t tile = blockIdx.x;
+        const uint32_t tile = wg;
         const int32_t X0 = (int32_t)(tile % tiles_x) * kVT, Y0 = (int32_t)(tile / tiles_x) * kVT;
         const int32_t CX0 = X0 / 2 - 1, CY0 = Y0 / 2 - 1;
         vtile_load(L, X0, Y0, t, true, false);
@@ -1163,12 +1352,28 @@ __global__ __launch_bounds__(256) void vup_kernel(LevelView L, LevelView C, cons
     }
     if (dot == 2) {
         const double bs = block_sum(dprod, dscratch);
-        if (threadIdx.x == 0) part_gamma_d[blockIdx.x] = bs;
+        if (threadIdx.x == 0) part_gamma_d[wg] = bs;
     } else if (dot == 1) {
         const float bs = block_sum(prod, scratch);
-        if (threadIdx.x == 0) part_gamma[blockIdx.x] = bs;
+        if (threadIdx.x == 0) part_gamma[wg] = bs;
     }
 }
+
+struct VUp {
+    static constexpr int kThreads = 256;
+    struct Args {
+        LevelView L, C;
+        const FluidStats* fs;  // base level only
+        const int32_t* done;
+        uint32_t tiles_x;
+        int dot;
+        float* part_gamma;
+        double* part_gamma_d;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        vup(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, wg);
+    }
+};
 
 unsigned blocks_for(size_t n, unsigned threads = kThreads) { return (unsigned)((n + threads - 1) / threads); }
 
@@ -1193,34 +1398,27 @@ void launch_build_pressure(Sim& sim, bool zero_pressure) {
     const uint32_t cells = sim.cells;
     const uint8_t* fluid = sim.levels[0].fluid;
     // three smoothing passes: density -> a -> b -> a, so the result ends in smooth_a
-    hipLaunchKernelGGL(smooth_density_kernel, dim3(blocks_for(cells)), dim3(kThreads), 0, sim.stream, sim.density,
-                       sim.smooth_a, fluid, sim.W, cells);
-    hipLaunchKernelGGL(smooth_density_kernel, dim3(blocks_for(cells)), dim3(kThreads), 0, sim.stream, sim.smooth_a,
-                       sim.smooth_b, fluid, sim.W, cells);
-    hipLaunchKernelGGL(smooth_density_kernel, dim3(blocks_for(cells)), dim3(kThreads), 0, sim.stream, sim.smooth_b,
-                       sim.smooth_a, fluid, sim.W, cells);
-    hipLaunchKernelGGL(assemble_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, sim.mac_u, sim.mac_v,
-                       sim.smooth_a, fluid, sim.obstacle, sim.W, cells, sim.cell, sim.levels[0], sim.pressure,
-                       (int)zero_pressure, sim.part_fluid);
-    hipLaunchKernelGGL(fstats_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.part_fluid, sim.fstats,
-                       sim.h_fstats_d);
+    emit<SmoothDensity>(sim, {sim.density, sim.smooth_a, fluid, sim.W, cells}, blocks_for(cells));
+    emit<SmoothDensity>(sim, {sim.smooth_a, sim.smooth_b, fluid, sim.W, cells}, blocks_for(cells));
+    emit<SmoothDensity>(sim, {sim.smooth_b, sim.smooth_a, fluid, sim.W, cells}, blocks_for(cells));
+    Assemble::Args a{sim.mac_u, sim.mac_v, sim.smooth_a, fluid, sim.obstacle, sim.W, cells, sim.cell, sim.levels[0],
+                     sim.pressure, (int)zero_pressure, sim.part_fluid, sim.nred};
+    emit<Assemble>(sim, a, sim.nred);
+    emit<FstatsFinalize>(sim, {sim.part_fluid, sim.nred, sim.fstats, sim.h_fstats_d}, 1);
 }
 
 void launch_fluid_stats(Sim& sim) {
-    hipLaunchKernelGGL(fluid_stats_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, sim.levels[0].fluid,
+    hipLaunchKernelGGL(fluid_stats_kernel, dim3(sim.nred), dim3(kThreads), 0, sim.stream, sim.levels[0].fluid,
                        sim.cells, sim.part_fluid);
-    hipLaunchKernelGGL(fstats_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.part_fluid, sim.fstats,
-                       sim.h_fstats_d);
+    emit<FstatsFinalize>(sim, {sim.part_fluid, sim.nred, sim.fstats, sim.h_fstats_d}, 1);
 }
 
 void launch_mg_build(Sim& sim) {
     const int n = (int)sim.levels.size();
     int l = 1;
     for (; l < n && l < (int)sim.bottom_level; ++l)
-        hipLaunchKernelGGL(mg_build_kernel, dim3(blocks_for(sim.levels[l].cells)), dim3(kThreads), 0, sim.stream,
-                           sim.levels[l - 1], sim.levels[l]);
-    if (l < n)
-        hipLaunchKernelGGL(mg_build_bottom_kernel, dim3(1), dim3(kBottomThreads), 0, sim.stream, level_set(sim), l);
+        emit<MgBuild>(sim, {sim.levels[l - 1], sim.levels[l]}, blocks_for(sim.levels[l].cells));
+    if (l < n) emit<MgBuildBottom>(sim, {level_set(sim), l}, 1);
 }
 
 // The base level's up-sweep writes one partial of r.z per tile.
@@ -1236,11 +1434,10 @@ int pcg_cap(const Sim& sim) { return sim.solver.mode == 1 ? sim.solver.max_itera
 void launch_pcg_reset(Sim& sim) {
     const bool rel = sim.solver.mode == 1;
     const double tol = (double)sim.solver.rel_tol;
-    hipLaunchKernelGGL(pcg_reset_kernel, dim3(1), dim3(1), 0, sim.stream, sim.pcg, tol * tol, pcg_cap(sim));
+    emit<PcgReset>(sim, {sim.pcg, tol * tol, pcg_cap(sim)}, 1);
     if (rel) {
-        hipLaunchKernelGGL(pcg_bnorm_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, sim.levels[0], sim.rhs,
-                           sim.part_rr_d, sim.pressure, sim.pressure_d);
-        hipLaunchKernelGGL(pcg_bnorm_finalize_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.pcg, sim.part_rr_d);
+        emit<PcgBnorm>(sim, {sim.levels[0], sim.rhs, sim.part_rr_d, sim.pressure, sim.pressure_d, sim.nred}, sim.nred);
+        emit<PcgBnormFinalize>(sim, {sim.pcg, sim.part_rr_d, sim.nred}, 1);
     }
 }
 
@@ -1256,7 +1453,6 @@ void launch_vcycle(Sim& sim, int mode) {
     const int last = (int)sim.levels.size() - 1;
     const int bottom = (int)sim.bottom_level;  // 1 <= bottom <= last
     const bool rel = sim.solver.mode == 1 && mode != 0;
-    hipStream_t s = sim.stream;
     auto tiles = [](const LevelView& L, uint32_t& tx) {
         tx = (L.w + kVT - 1) / kVT;
         return tx * ((L.h + kVT - 1) / kVT);
@@ -1266,50 +1462,48 @@ void launch_vcycle(Sim& sim, int mode) {
         const int fuse_red = l + 1 < bottom ? 1 : (int)(bottom < last);
         uint32_t tx;
         const uint32_t nt = tiles(F, tx);
-        hipLaunchKernelGGL(vdown_kernel, dim3(nt), dim3(256), 0, s, F, sim.levels[l + 1], (int)(l == 0), fuse_red,
-                           done, tx, sim.pcg, (int)(l == 0 && mode == 2), sim.props.solve_error, sim.part_max,
-                           (l == 0 && rel) ? sim.part_rr2_d : nullptr);
+        VDown::Args a{F, sim.levels[l + 1], (int)(l == 0), fuse_red, done, tx, sim.pcg, (int)(l == 0 && mode == 2),
+                      sim.props.solve_error, sim.part_max, (l == 0 && rel) ? sim.part_rr2_d : nullptr, sim.nred};
+        emit<VDown>(sim, a, nt);
     }
     if (sim.bottom_in_lds)
-        hipLaunchKernelGGL(bottom_lds_kernel, dim3(1), dim3(kBottomThreads), 0, s, level_set(sim), bottom, done);
+        emit<BottomLds>(sim, {level_set(sim), bottom, done}, 1);
     else
-        hipLaunchKernelGGL(bottom_kernel, dim3(1), dim3(kBottomThreads), 0, s, level_set(sim), bottom, sim.cg_s,
-                           sim.cg_ap, sim.cg_r, done);
+        emit<BottomGlobal>(sim, {level_set(sim), bottom, sim.cg_s, sim.cg_ap, sim.cg_r, done}, 1);
     for (int l = bottom - 1; l >= 0; --l) {
         const LevelView& L = sim.levels[l];
         uint32_t tx;
         const uint32_t nt = tiles(L, tx);
         const int dot = (l == 0 && mode != 0) ? (rel ? 2 : 1) : 0;  // one partial of r.z per tile
-        hipLaunchKernelGGL(vup_kernel, dim3(nt), dim3(256), 0, s, L, sim.levels[l + 1],
-                           l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot, sim.part_gamma,
-                           sim.part_gamma_d);
+        VUp::Args a{L, sim.levels[l + 1], l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot,
+                    sim.part_gamma, sim.part_gamma_d};
+        emit<VUp>(sim, a, nt);
     }
 }
 
 void launch_pcg_iteration_head(Sim& sim, int k) {
     const LevelView& B = sim.levels[0];
-    float* s_old = sim.search[k & 1];
-    float* s_new = sim.search[(k + 1) & 1];
     const bool rel = sim.solver.mode == 1;
     sim.mark(APIC_T_K_SPMV, 0);
-    hipLaunchKernelGGL(spmv_dot_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_old, s_new, sim.As,
-                       sim.pcg, sim.part_gamma, sim.part_delta, k, rel ? sim.part_gamma_d : nullptr,
-                       rel ? sim.part_delta_d : nullptr, (int)gamma_partials(sim));
+    SpmvDot::Args a{B, {sim.search[0], sim.search[1]}, sim.As, sim.pcg, sim.part_gamma, sim.part_delta,
+                    rel ? sim.part_gamma_d : nullptr, rel ? sim.part_delta_d : nullptr, (int)gamma_partials(sim),
+                    sim.nred};
+    emit<SpmvDot>(sim, a, sim.nred, k);
     sim.mark(APIC_T_K_SPMV, 1);
-    hipLaunchKernelGGL(axpy_max_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, s_new, sim.As,
-                       sim.pressure, sim.pcg, sim.part_delta, sim.part_max, k, rel ? sim.part_delta_d : nullptr,
-                       rel ? sim.part_rr_d : nullptr, sim.pressure_d);
+    AxpyMax::Args b{B, {sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pcg, sim.part_delta, sim.part_max,
+                    rel ? sim.part_delta_d : nullptr, rel ? sim.part_rr_d : nullptr, sim.pressure_d, sim.nred};
+    emit<AxpyMax>(sim, b, sim.nred, k);
     if (rel) {
-        const int every = sim.solver.recompute_every;
-        const int due = every > 0 && (k + 1) % every == 0;
-        hipLaunchKernelGGL(true_residual_kernel, dim3(kRedBlocks), dim3(kThreads), 0, sim.stream, B, sim.rhs,
-                           sim.pressure_d, sim.pcg, sim.part_rr_d, sim.part_rr2_d, due);
+        TrueResidual::Args c{B, sim.rhs, sim.pressure_d, sim.pcg, sim.part_rr_d, sim.part_rr2_d,
+                             sim.solver.recompute_every, sim.nred};
+        emit<TrueResidual>(sim, c, sim.nred, k);
     }
 }
 
 void launch_pcg_decide(Sim& sim, int k) {
-    hipLaunchKernelGGL(decide_kernel, dim3(1), dim3(kThreads), 0, sim.stream, sim.pcg, sim.part_gamma, k, sim.h_pcg_d,
-                       sim.solver.mode == 1 ? sim.part_gamma_d : nullptr, (int)gamma_partials(sim));
+    emit<Decide>(sim, {sim.pcg, sim.part_gamma, sim.h_pcg_d, sim.solver.mode == 1 ? sim.part_gamma_d : nullptr,
+                       (int)gamma_partials(sim)},
+                 1, k);
 }
 
 }  // namespace apic

@@ -34,17 +34,26 @@ constexpr int kRadixMaxBuckets = 1 << kRadixMaxBits;
 constexpr int kScanThreads = 1024;
 constexpr int kScanChunkTiles = 32;  // tiles per chunk of the two-level column scan
 
-__global__ void keygen_kernel(const float* __restrict__ px, const float* __restrict__ py, uint32_t n,
-                              float cell, float max_x, float max_y, uint32_t* __restrict__ keys,
-                              uint32_t* __restrict__ vals) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    float gx = px[i] / cell, gy = py[i] / cell;
-    uint32_t ix = (uint32_t)(int32_t)fast_min(fast_max(gx, 0.0f), max_x);
-    uint32_t iy = (uint32_t)(int32_t)fast_min(fast_max(gy, 0.0f), max_y);
-    keys[i] = morton(ix, iy);
-    vals[i] = i;
-}
+struct Keygen {
+    static constexpr int kThreads = 256;
+    struct Args {
+        const float* px;
+        const float* py;
+        uint32_t n;
+        float cell, max_x, max_y;
+        uint32_t* keys;
+        uint32_t* vals;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        uint32_t i = wg * kThreads + threadIdx.x;
+        if (i >= a.n) return;
+        float gx = a.px[i] / a.cell, gy = a.py[i] / a.cell;
+        uint32_t ix = (uint32_t)(int32_t)fast_min(fast_max(gx, 0.0f), a.max_x);
+        uint32_t iy = (uint32_t)(int32_t)fast_min(fast_max(gy, 0.0f), a.max_y);
+        a.keys[i] = morton(ix, iy);
+        a.vals[i] = i;
+    }
+};
 
 // Lanes of the wave whose digit equals this lane's (valid lanes only).  When
 // the wave's digits form few runs (the usual case for Morton-sorted particles
@@ -77,37 +86,47 @@ __device__ inline uint64_t digit_peers(uint32_t d, bool valid, int bits) {
 
 // A wave whose digits are all equal adds its count once; otherwise each lane
 // adds itself (LDS atomics).
-__global__ __launch_bounds__(kRadixThreads) void radix_upsweep(const uint32_t* __restrict__ keys, uint32_t n,
-                                                               int shift, int bits, uint32_t* __restrict__ counts) {
-    __shared__ uint32_t hist[kRadixMaxBuckets];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const uint32_t nb = 1u << bits, mask = nb - 1u;
-    for (uint32_t d = tid; d < nb; d += kRadixThreads) hist[d] = 0;
-    const size_t base = (size_t)blockIdx.x * kRadixTile;
-    uint32_t k[kRadixItems];
+struct RadixUpsweep {
+    static constexpr int kThreads = kRadixThreads;
+    struct Args {
+        const uint32_t* keys;
+        uint32_t n;
+        int shift, bits;
+        uint32_t* counts;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        __shared__ uint32_t hist[kRadixMaxBuckets];
+        const int tid = threadIdx.x, lane = tid & 63;
+        const uint32_t n = a.n;
+        const int shift = a.shift;
+        const uint32_t nb = 1u << a.bits, mask = nb - 1u;
+        for (uint32_t d = tid; d < nb; d += kRadixThreads) hist[d] = 0;
+        const size_t base = (size_t)wg * kRadixTile;
+        uint32_t k[kRadixItems];
 #pragma unroll
-    for (int j = 0; j < kRadixItems; ++j) {
-        const size_t idx = base + (size_t)j * kRadixThreads + tid;
-        k[j] = idx < n ? keys[idx] : 0u;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kRadixItems; ++j) {
-        const size_t idx = base + (size_t)j * kRadixThreads + tid;
-        const bool valid = idx < n;
-        const uint32_t d = (k[j] >> shift) & mask;
-        const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
-        const uint64_t act = __ballot(valid);
-        const uint64_t same = __ballot(valid && d == d0);
-        if (same == act) {
-            if (act != 0 && lane == (int)__builtin_ctzll(act)) atomicAdd(&hist[d0], (uint32_t)__popcll(act));
-        } else if (valid) {
-            atomicAdd(&hist[d], 1u);
+        for (int j = 0; j < kRadixItems; ++j) {
+            const size_t idx = base + (size_t)j * kRadixThreads + tid;
+            k[j] = idx < n ? a.keys[idx] : 0u;
         }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kRadixItems; ++j) {
+            const size_t idx = base + (size_t)j * kRadixThreads + tid;
+            const bool valid = idx < n;
+            const uint32_t d = (k[j] >> shift) & mask;
+            const uint32_t d0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)d);
+            const uint64_t act = __ballot(valid);
+            const uint64_t same = __ballot(valid && d == d0);
+            if (same == act) {
+                if (act != 0 && lane == (int)__builtin_ctzll(act)) atomicAdd(&hist[d0], (uint32_t)__popcll(act));
+            } else if (valid) {
+                atomicAdd(&hist[d], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t d = tid; d < nb; d += kRadixThreads) a.counts[(size_t)wg * nb + d] = hist[d];
     }
-    __syncthreads();
-    for (uint32_t d = tid; d < nb; d += kRadixThreads) counts[(size_t)blockIdx.x * nb + d] = hist[d];
-}
+};
 
 // Block-wide exclusive scan of one value per thread (1024 threads = 16 waves).
 __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* wave_sums, uint32_t& total) {
@@ -137,67 +156,86 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t v, uint32_t* wave_sums,
 // digit, the exclusive prefix of the tile counts within the chunk (in place)
 // and the chunk's total.  Threads of a block take consecutive digits, so every
 // access is coalesced along the tile-major rows.
-__global__ __launch_bounds__(256) void scan_columns(uint32_t* __restrict__ counts, uint32_t ntiles, uint32_t nb,
-                                                    uint32_t* __restrict__ chunk_sum) {
-    const uint32_t chunk = blockIdx.x;
-    const uint32_t t0 = chunk * kScanChunkTiles, t1 = min(t0 + (uint32_t)kScanChunkTiles, ntiles);
-    const uint32_t d = blockIdx.y * blockDim.x + threadIdx.x;
-    if (d >= nb) return;
-    uint32_t c[kScanChunkTiles];
+struct ScanColumns {
+    static constexpr int kThreads = 256;
+    struct Args {
+        uint32_t* counts;
+        uint32_t ntiles, nb, nchunks;
+        uint32_t* chunk_sum;
+    };
+    // workgroup wg: chunk wg % nchunks, digits from 256 * (wg / nchunks)
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        const uint32_t nb = a.nb, ntiles = a.ntiles;
+        uint32_t* __restrict__ counts = a.counts;
+        const uint32_t chunk = wg % a.nchunks;
+        const uint32_t t0 = chunk * kScanChunkTiles, t1 = min(t0 + (uint32_t)kScanChunkTiles, ntiles);
+        const uint32_t d = (wg / a.nchunks) * kThreads + threadIdx.x;
+        if (d >= nb) return;
+        uint32_t c[kScanChunkTiles];
 #pragma unroll
-    for (int q = 0; q < kScanChunkTiles; ++q) c[q] = t0 + q < t1 ? counts[(size_t)(t0 + q) * nb + d] : 0u;
-    uint32_t run = 0;
+        for (int q = 0; q < kScanChunkTiles; ++q) c[q] = t0 + q < t1 ? counts[(size_t)(t0 + q) * nb + d] : 0u;
+        uint32_t run = 0;
 #pragma unroll
-    for (int q = 0; q < kScanChunkTiles; ++q) {
-        if (t0 + q < t1) counts[(size_t)(t0 + q) * nb + d] = run;
-        run += c[q];
+        for (int q = 0; q < kScanChunkTiles; ++q) {
+            if (t0 + q < t1) counts[(size_t)(t0 + q) * nb + d] = run;
+            run += c[q];
+        }
+        a.chunk_sum[(size_t)chunk * nb + d] = run;
     }
-    chunk_sum[(size_t)chunk * nb + d] = run;
-}
+};
 
 // Column scan, level 2 (one workgroup): per digit the exclusive prefix over
 // chunks (in place), and each digit's start (exclusive scan over digits of
 // the digit totals).  A thread takes up to two consecutive digits.
-__global__ __launch_bounds__(kScanThreads) void scan_chunks(uint32_t* __restrict__ chunk_sum, uint32_t nchunks,
-                                                            uint32_t nb, uint32_t* __restrict__ digit_start) {
-    __shared__ uint32_t wave_sums[kScanThreads / 64 + 1];
-    const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;  // 1 or 2
-    const uint32_t d0 = threadIdx.x * per;
-    uint32_t run[2] = {0u, 0u};
-    for (uint32_t c0 = 0; c0 < nchunks; c0 += 16) {
-        uint32_t v[2][16];
+struct ScanChunks {
+    static constexpr int kThreads = kScanThreads;
+    struct Args {
+        uint32_t* chunk_sum;
+        uint32_t nchunks, nb;
+        uint32_t* digit_start;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        __shared__ uint32_t wave_sums[kScanThreads / 64 + 1];
+        const uint32_t nb = a.nb, nchunks = a.nchunks;
+        uint32_t* __restrict__ chunk_sum = a.chunk_sum;
+        const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;  // 1 or 2
+        const uint32_t d0 = threadIdx.x * per;
+        uint32_t run[2] = {0u, 0u};
+        for (uint32_t c0 = 0; c0 < nchunks; c0 += 16) {
+            uint32_t v[2][16];
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+            for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                const bool ok = (uint32_t)q < per && d0 + q < nb && c0 + c < nchunks;
-                v[q][c] = ok ? chunk_sum[(size_t)(c0 + c) * nb + d0 + q] : 0u;
-            }
+                for (int c = 0; c < 16; ++c) {
+                    const bool ok = (uint32_t)q < per && d0 + q < nb && c0 + c < nchunks;
+                    v[q][c] = ok ? chunk_sum[(size_t)(c0 + c) * nb + d0 + q] : 0u;
+                }
 #pragma unroll
-        for (int q = 0; q < 2; ++q)
+            for (int q = 0; q < 2; ++q)
 #pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                if ((uint32_t)q < per && d0 + q < nb && c0 + c < nchunks) chunk_sum[(size_t)(c0 + c) * nb + d0 + q] = run[q];
-                run[q] += v[q][c];
-            }
+                for (int c = 0; c < 16; ++c) {
+                    if ((uint32_t)q < per && d0 + q < nb && c0 + c < nchunks) chunk_sum[(size_t)(c0 + c) * nb + d0 + q] = run[q];
+                    run[q] += v[q][c];
+                }
+        }
+        uint32_t total;
+        uint32_t start = block_exclusive_scan(run[0] + run[1], wave_sums, total);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if ((uint32_t)q < per && d0 + q < nb) a.digit_start[d0 + q] = start;
+            start += run[q];
+        }
     }
-    uint32_t total;
-    uint32_t start = block_exclusive_scan(run[0] + run[1], wave_sums, total);
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        if ((uint32_t)q < per && d0 + q < nb) digit_start[d0 + q] = start;
-        start += run[q];
-    }
-}
+};
 
 // prefix[tile][digit] + chunk_base[chunk of tile][digit] + digit_start[digit]
 // is the tile's first destination for the digit.
 // After ranking, the tile is reordered by digit in LDS, so that consecutive
 // threads write consecutive destinations of one digit.
-__global__ __launch_bounds__(kRadixThreads) void radix_downsweep(
+__device__ __forceinline__ void radix_downsweep(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out, uint32_t n, int shift, int bits, const uint32_t* __restrict__ prefix,
-    const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ digit_start) {
+    const uint32_t* __restrict__ chunk_base, const uint32_t* __restrict__ digit_start, uint32_t wg) {
     constexpr int kPerThread = kRadixMaxBuckets / kRadixThreads;  // digits per thread in the tile scan
     __shared__ uint32_t whist[kRadixWaves][kRadixMaxBuckets];      // per-wave counts, then the staged tile
     __shared__ uint32_t gdelta[kRadixMaxBuckets];                  // global start - tile start, per digit
@@ -208,7 +246,7 @@ __global__ __launch_bounds__(kRadixThreads) void radix_downsweep(
 #pragma unroll
         for (int w = 0; w < kRadixWaves; ++w) whist[w][d] = 0;
     }
-    const size_t tile0 = (size_t)blockIdx.x * kRadixTile;
+    const size_t tile0 = (size_t)wg * kRadixTile;
     const size_t base = tile0 + (size_t)wave * kRadixWaveItems;
     uint32_t k[kRadixItems], v[kRadixItems], r[kRadixItems];
 #pragma unroll
@@ -266,8 +304,8 @@ __global__ __launch_bounds__(kRadixThreads) void radix_downsweep(
     for (int q = 0; q < kPerThread; ++q) {
         const uint32_t d = (uint32_t)tid * kPerThread + q;
         if (d < nb) {
-            gdelta[d] = prefix[(size_t)blockIdx.x * nb + d] +
-                        chunk_base[(size_t)(blockIdx.x / kScanChunkTiles) * nb + d] + digit_start[d] - start;
+            gdelta[d] = prefix[(size_t)wg * nb + d] +
+                        chunk_base[(size_t)(wg / kScanChunkTiles) * nb + d] + digit_start[d] - start;
             uint32_t run = start;
 #pragma unroll
             for (int w = 0; w < kRadixWaves; ++w) {
@@ -301,21 +339,51 @@ __global__ __launch_bounds__(kRadixThreads) void radix_downsweep(
     }
 }
 
-__global__ void permute_kernel(const float* __restrict__ in, float* __restrict__ out,
-                               const uint32_t* __restrict__ vals, uint32_t np, size_t nalloc) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nalloc) return;
-    if (i < np) {
-        size_t j = vals[i];
-        if (j >= np) j = np - 1;  // records written through the field API are not trusted
-#pragma unroll
-        for (int f = 0; f < NUM_PFIELDS; ++f) out[(size_t)f * nalloc + i] = in[(size_t)f * nalloc + j];
-    } else {
-        // the reference's gather leaves tail lanes of the last chunk zero (src/apic.rs:1077-1103)
-#pragma unroll
-        for (int f = 0; f < NUM_PFIELDS; ++f) out[(size_t)f * nalloc + i] = 0.0f;
+struct RadixDownsweep {
+    static constexpr int kThreads = kRadixThreads;
+    struct Args {
+        const uint32_t* keys_in;
+        const uint32_t* vals_in;
+        uint32_t* keys_out;
+        uint32_t* vals_out;
+        uint32_t n;
+        int shift, bits;
+        const uint32_t* prefix;
+        const uint32_t* chunk_base;
+        const uint32_t* digit_start;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        radix_downsweep(a.keys_in, a.vals_in, a.keys_out, a.vals_out, a.n, a.shift, a.bits, a.prefix, a.chunk_base,
+                        a.digit_start, wg);
     }
-}
+};
+
+struct Permute {
+    static constexpr int kThreads = 256;
+    struct Args {
+        const float* in;
+        float* out;
+        const uint32_t* vals;
+        uint32_t np;
+        size_t nalloc;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        const size_t nalloc = a.nalloc;
+        const uint32_t np = a.np;
+        size_t i = (size_t)wg * kThreads + threadIdx.x;
+        if (i >= nalloc) return;
+        if (i < np) {
+            size_t j = a.vals[i];
+            if (j >= np) j = np - 1;  // records written through the field API are not trusted
+#pragma unroll
+            for (int f = 0; f < NUM_PFIELDS; ++f) a.out[(size_t)f * nalloc + i] = a.in[(size_t)f * nalloc + j];
+        } else {
+            // the reference's gather leaves tail lanes of the last chunk zero (src/apic.rs:1077-1103)
+#pragma unroll
+            for (int f = 0; f < NUM_PFIELDS; ++f) a.out[(size_t)f * nalloc + i] = 0.0f;
+        }
+    }
+};
 
 uint32_t key_bits(const Sim& sim) {
     uint32_t mk = morton(sim.W > 0 ? sim.W - 1 : 0, sim.H > 0 ? sim.H - 1 : 0);
@@ -336,8 +404,8 @@ void launch_keygen(Sim& sim) {
     if (sim.num_particles == 0) return;
     uint32_t n = sim.num_particles;
     float max_x = (float)(sim.W > 0 ? sim.W - 1 : 0), max_y = (float)(sim.H > 0 ? sim.H - 1 : 0);
-    hipLaunchKernelGGL(keygen_kernel, dim3((n + 255) / 256), dim3(256), 0, sim.stream, sim.pfield(P_X),
-                       sim.pfield(P_Y), n, sim.cell, max_x, max_y, sim.keys[0], sim.vals[0]);
+    emit<Keygen>(sim, {sim.pfield(P_X), sim.pfield(P_Y), n, sim.cell, max_x, max_y, sim.keys[0], sim.vals[0]},
+                 (n + 255) / 256);
 }
 
 void launch_radix_sort(Sim& sim) {
@@ -356,15 +424,12 @@ void launch_radix_sort(Sim& sim) {
     for (int p = 0; p < passes; ++p) {
         const int bits = p == 0 ? total_bits - (passes - 1) * kRadixMaxBits : kRadixMaxBits;
         const uint32_t nb = 1u << bits;
-        hipLaunchKernelGGL(radix_upsweep, dim3(ntiles), dim3(kRadixThreads), 0, sim.stream, sim.keys[src], n, shift,
-                           bits, counts);
-        hipLaunchKernelGGL(scan_columns, dim3(nchunks, (nb + 255) / 256), dim3(256), 0, sim.stream, counts, ntiles, nb,
-                           chunk_sum);
-        hipLaunchKernelGGL(scan_chunks, dim3(1), dim3(kScanThreads), 0, sim.stream, chunk_sum, nchunks, nb,
-                           digit_start);
-        hipLaunchKernelGGL(radix_downsweep, dim3(ntiles), dim3(kRadixThreads), 0, sim.stream, sim.keys[src],
-                           sim.vals[src], sim.keys[1 - src], sim.vals[1 - src], n, shift, bits, counts, chunk_sum,
-                           digit_start);
+        emit<RadixUpsweep>(sim, {sim.keys[src], n, shift, bits, counts}, ntiles);
+        emit<ScanColumns>(sim, {counts, ntiles, nb, nchunks, chunk_sum}, nchunks * ((nb + 255) / 256));
+        emit<ScanChunks>(sim, {chunk_sum, nchunks, nb, digit_start}, 1);
+        emit<RadixDownsweep>(sim, {sim.keys[src], sim.vals[src], sim.keys[1 - src], sim.vals[1 - src], n, shift, bits,
+                                   counts, chunk_sum, digit_start},
+                             ntiles);
         src = 1 - src;
         shift += bits;
     }
@@ -377,8 +442,8 @@ void launch_radix_sort(Sim& sim) {
 void launch_permute(Sim& sim) {
     if (sim.nalloc == 0) return;
     size_t n = sim.nalloc;
-    hipLaunchKernelGGL(permute_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sim.stream, sim.pbuf[sim.cur],
-                       sim.pbuf[1 - sim.cur], sim.vals[0], sim.num_particles, n);
+    emit<Permute>(sim, {sim.pbuf[sim.cur], sim.pbuf[1 - sim.cur], sim.vals[0], sim.num_particles, n},
+                  (uint32_t)((n + 255) / 256));
     sim.cur = 1 - sim.cur;
 }
 

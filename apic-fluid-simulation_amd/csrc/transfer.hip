@@ -147,17 +147,19 @@ __device__ inline void owner_accumulate(const float* __restrict__ sp, int k, flo
     }
 }
 
-__global__ __launch_bounds__(kOwnCells) void p2g_owner_kernel(
+__device__ __forceinline__ void p2g_owner(
     const float* __restrict__ pb, size_t nalloc, uint32_t np, const uint32_t* __restrict__ order, float prev_dt_arg,
     float cell, uint32_t W, uint32_t H, const uint2* __restrict__ cell_range, uint32_t tiles_x,
     float* __restrict__ mac_u, float* __restrict__ mac_v, uint8_t* __restrict__ valid_u,
     uint8_t* __restrict__ valid_v, float* __restrict__ density, uint8_t* __restrict__ fluid,
     const uint8_t* __restrict__ obstacle, const float* __restrict__ ext_u, const float* __restrict__ ext_v,
-    int forces, int ext, float dt_arg, float gravity, const StepState* __restrict__ ss) {
+    int forces, int ext, float dt_arg, float gravity, float gravity_x, const StepState* __restrict__ ss,
+    uint32_t wg) {
     __shared__ float s_buf[kOwnFields * kOwnCap];
     const float prev_dt = ss ? ss->prev_dt : prev_dt_arg;
     const float dt = ss ? ss->dt : dt_arg;
-    const float g = gravity * dt;  // staged particles, then the partials
+    const float g = gravity * dt;
+    const float gx = gravity_x * dt;  // horizontal gravity (an extension; zero in the reference)
     __shared__ uint32_t s_off[kOwnCells + 1];      // window-order prefix of the cell counts
     __shared__ uint32_t s_src[kOwnCells];          // sorted-order begin of each window cell
     __shared__ uint32_t s_wave[kOwnCells / 64];
@@ -165,8 +167,8 @@ __global__ __launch_bounds__(kOwnCells) void p2g_owner_kernel(
     const int32_t Wi = (int32_t)W, Hi = (int32_t)H;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wx = tid % kOwnWin, wy = tid / kOwnWin;
-    const int32_t cx = (int32_t)(blockIdx.x % tiles_x) * kOwnTile - 1 + wx;
-    const int32_t cy = (int32_t)(blockIdx.x / tiles_x) * kOwnTile - 1 + wy;
+    const int32_t cx = (int32_t)(wg % tiles_x) * kOwnTile - 1 + wx;
+    const int32_t cy = (int32_t)(wg / tiles_x) * kOwnTile - 1 + wy;
     const bool inside = cx >= 0 && cx < Wi && cy >= 0 && cy < Hi;
     uint32_t b = 0, cnt = 0;
     bool fl = false;
@@ -258,6 +260,7 @@ __global__ __launch_bounds__(kOwnCells) void p2g_owner_kernel(
         const bool ok = w > 0.0f;
         float u = ok ? m / w : 0.0f;
         if (ext) u += ext_u[f] * dt;
+        if (forces && gravity_x != 0.0f && (s_fl[tid - 1] || s_fl[tid])) u += gx;
         mac_u[f] = u;
         valid_u[f] = ok;
         if (cx == Wi - 1) {  // face x = W: only the cells x = W - 1 reach it
@@ -270,6 +273,7 @@ __global__ __launch_bounds__(kOwnCells) void p2g_owner_kernel(
             const bool ok2 = w > 0.0f;
             u = ok2 ? m / w : 0.0f;
             if (ext) u += ext_u[f + 1] * dt;
+            if (forces && gravity_x != 0.0f && s_fl[tid]) u += gx;
             mac_u[f + 1] = u;
             valid_u[f + 1] = ok2;
         }
@@ -321,25 +325,77 @@ __global__ __launch_bounds__(kOwnCells) void p2g_owner_kernel(
 #undef PART
 }
 
+struct P2gOwner {
+    static constexpr int kThreads = kOwnCells;
+    struct Args {
+        const float* pb;
+        size_t nalloc;
+        uint32_t np;
+        const uint32_t* order;
+        float prev_dt;
+        float cell;
+        uint32_t W, H;
+        const uint2* cell_range;
+        uint32_t tiles_x;
+        float* mac_u;
+        float* mac_v;
+        uint8_t* valid_u;
+        uint8_t* valid_v;
+        float* density;
+        uint8_t* fluid;
+        const uint8_t* obstacle;
+        const float* ext_u;
+        const float* ext_v;
+        int forces, ext;
+        float dt, gravity, gravity_x;
+        const StepState* ss;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        p2g_owner(a.pb, a.nalloc, a.np, a.order, a.prev_dt, a.cell, a.W, a.H, a.cell_range, a.tiles_x, a.mac_u,
+                  a.mac_v, a.valid_u, a.valid_v, a.density, a.fluid, a.obstacle, a.ext_u, a.ext_v, a.forces, a.ext,
+                  a.dt, a.gravity, a.gravity_x, a.ss, wg);
+    }
+};
+
 // [begin, end) of every cell's particles in cell order, indexed by key
-__global__ void cell_ranges_kernel(const uint32_t* __restrict__ keys, uint32_t np, uint2* __restrict__ range,
-                                   uint32_t nkeys) {
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const uint32_t k = keys[i];
-    if (k >= nkeys) return;  // keys come from clamped cell coordinates; kept as a guard
-    if (i == 0 || keys[i - 1] != k) range[k].x = i;
-    if (i + 1 == np || keys[i + 1] != k) range[k].y = i + 1;
-}
+struct CellRanges {
+    static constexpr int kThreads = 256;
+    struct Args {
+        const uint32_t* keys;
+        uint32_t np;
+        uint2* range;
+        uint32_t nkeys;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        const uint32_t i = wg * kThreads + threadIdx.x;
+        const uint32_t np = a.np;
+        if (i >= np) return;
+        const uint32_t* __restrict__ keys = a.keys;
+        const uint32_t k = keys[i];
+        if (k >= a.nkeys) return;  // keys come from clamped cell coordinates; kept as a guard
+        if (i == 0 || keys[i - 1] != k) a.range[k].x = i;
+        if (i + 1 == np || keys[i + 1] != k) a.range[k].y = i + 1;
+    }
+};
 
 // apply_external_forces (src/apic.rs:1604-1626): u, v += force * dt on every
 // face, v += g * dt on faces with a fluid cell below or above.
 __global__ void forces_kernel(float* __restrict__ mac_u, float* __restrict__ mac_v, const float* __restrict__ ext_u,
                               const float* __restrict__ ext_v, const uint8_t* __restrict__ fluid, uint32_t W,
-                              uint32_t H, int ext_forces, float dt, float g) {
+                              uint32_t H, int ext_forces, float dt, float g, float gx) {
     const size_t nu = (size_t)W * H + H, nv = (size_t)W * H + W;
     size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (f < nu && ext_forces) mac_u[f] = mac_u[f] + ext_u[f] * dt;
+    if (f < nu) {
+        float u = mac_u[f];
+        if (ext_forces) u = u + ext_u[f] * dt;
+        if (gx != 0.0f) {  // horizontal gravity (an extension): faces with a fluid cell left or right
+            uint32_t x = (uint32_t)(f % (W + 1)), y = (uint32_t)(f / (W + 1));
+            bool left = x >= 1 && fluid[(size_t)y * W + x - 1];
+            bool right = x < W && fluid[(size_t)y * W + x];
+            if (left || right) u += gx;
+        }
+        mac_u[f] = u;
+    }
     if (f < nv) {
         float v = mac_v[f];
         if (ext_forces) v += ext_v[f] * dt;
@@ -364,17 +420,17 @@ __device__ inline float masked_interp(const float* w, const uint8_t* open, const
     return vel;
 }
 
-__global__ __launch_bounds__(256) void g2p_kernel(
+__device__ __forceinline__ void g2p(
     float* __restrict__ pb, size_t nalloc, uint32_t np, float dt_arg, float cell, uint32_t W, uint32_t H,
     uint32_t timestamp, float border_damping, const float* __restrict__ mac_u, const float* __restrict__ mac_v,
     const uint8_t* __restrict__ open_u, const uint8_t* __restrict__ open_v,
     int write_keys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, float* __restrict__ part_maxv,
-    const StepState* __restrict__ ss, const int32_t* __restrict__ guard) {
+    const StepState* __restrict__ ss, const int32_t* __restrict__ guard, uint32_t wg, uint32_t nwg) {
     __shared__ float s_max[256 / 64];
     if (guard && *guard == 0) return;
     const float dt = ss ? ss->dt : dt_arg;
     float speed_new = 0.0f;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < nalloc; i += (size_t)gridDim.x * blockDim.x) {
+    for (size_t i = (size_t)wg * blockDim.x + threadIdx.x; i < nalloc; i += (size_t)nwg * blockDim.x) {
         const float x = pb[(size_t)P_X * nalloc + i], y = pb[(size_t)P_Y * nalloc + i];
         const float res = pb[(size_t)T_RES * nalloc + i];
         const float gx = x / cell, gy = y / cell;
@@ -464,9 +520,37 @@ __global__ __launch_bounds__(256) void g2p_kernel(
     if (threadIdx.x == 0) {
         float b = s_max[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = fast_max(b, s_max[w]);
-        part_maxv[blockIdx.x] = b;
+        part_maxv[wg] = b;
     }
 }
+
+struct G2p {
+    static constexpr int kThreads = 256;
+    struct Args {
+        float* pb;
+        size_t nalloc;
+        uint32_t np;
+        float dt;
+        float cell;
+        uint32_t W, H, timestamp;
+        float border_damping;
+        const float* mac_u;
+        const float* mac_v;
+        const uint8_t* open_u;
+        const uint8_t* open_v;
+        int write_keys;
+        uint32_t* keys;
+        uint32_t* vals;
+        float* part_maxv;
+        const StepState* ss;
+        const int32_t* guard;
+        uint32_t nwg;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        g2p(a.pb, a.nalloc, a.np, a.dt, a.cell, a.W, a.H, a.timestamp, a.border_damping, a.mac_u, a.mac_v, a.open_u,
+            a.open_v, a.write_keys, a.keys, a.vals, a.part_maxv, a.ss, a.guard, wg, a.nwg);
+    }
+};
 
 __global__ void max_velocity_kernel(const float* __restrict__ vx, const float* __restrict__ vy, size_t n,
                                     float* __restrict__ part_maxv) {
@@ -486,25 +570,32 @@ __global__ void max_velocity_kernel(const float* __restrict__ vx, const float* _
 }
 
 // max over the per-workgroup maxima into the float bits read by the host
-__global__ __launch_bounds__(1024) void maxv_finalize_kernel(const float* __restrict__ part_maxv, uint32_t n,
-                                                             uint32_t* __restrict__ maxv_bits,
-                                                             uint32_t* __restrict__ host_bits,
-                                                             const int32_t* __restrict__ guard) {
-    __shared__ float s_max[1024 / 64];
-    if (guard && *guard == 0) return;
-    float m = 0.0f;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) m = fast_max(m, part_maxv[i]);
+struct MaxvFinalize {
+    static constexpr int kThreads = 1024;
+    struct Args {
+        const float* part_maxv;
+        uint32_t n;
+        uint32_t* maxv_bits;
+        uint32_t* host_bits;
+        const int32_t* guard;
+    };
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        __shared__ float s_max[1024 / 64];
+        if (a.guard && *a.guard == 0) return;
+        float m = 0.0f;
+        for (uint32_t i = threadIdx.x; i < a.n; i += blockDim.x) m = fast_max(m, a.part_maxv[i]);
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) m = fast_max(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        float b = 0.0f;
-        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b = fast_max(b, s_max[w]);
-        *maxv_bits = __float_as_uint(b);
-        if (host_bits) *host_bits = __float_as_uint(b);
+        for (int o = 32; o > 0; o >>= 1) m = fast_max(m, __shfl_xor(m, o, 64));
+        if ((threadIdx.x & 63) == 0) s_max[threadIdx.x >> 6] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float b = 0.0f;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b = fast_max(b, s_max[w]);
+            *a.maxv_bits = __float_as_uint(b);
+            if (a.host_bits) *a.host_bits = __float_as_uint(b);
+        }
     }
-}
+};
 
 unsigned particle_blocks(const Sim& sim) {
     size_t b = (sim.nalloc + 255) / 256;
@@ -519,10 +610,10 @@ size_t cell_key_count(const Sim& sim) {
 
 void launch_cell_ranges(Sim& sim) {
     const size_t nk = cell_key_count(sim);
-    launch_fill_u32(sim.stream, reinterpret_cast<uint32_t*>(sim.cell_range), 0u, 2 * nk);
+    launch_fill_u32(sim, reinterpret_cast<uint32_t*>(sim.cell_range), 0u, 2 * nk);
     if (sim.num_particles == 0) return;
-    hipLaunchKernelGGL(cell_ranges_kernel, dim3((sim.num_particles + 255) / 256), dim3(256), 0, sim.stream,
-                       sim.keys[0], sim.num_particles, sim.cell_range, (uint32_t)nk);
+    emit<CellRanges>(sim, {sim.keys[0], sim.num_particles, sim.cell_range, (uint32_t)nk},
+                     (sim.num_particles + 255) / 256);
 }
 
 void launch_p2g(Sim& sim, bool forces, float dt) {
@@ -541,10 +632,11 @@ void launch_p2g(Sim& sim, bool forces, float dt) {
     const uint32_t tx = (sim.W + kOwnTile - 1) / kOwnTile, ty = (sim.H + kOwnTile - 1) / kOwnTile;
     const int ext = forces && sim.forces_active;
     sim.mark(APIC_T_K_P2G, 0);
-    hipLaunchKernelGGL(p2g_owner_kernel, dim3(tx * ty), dim3(kOwnCells), 0, sim.stream, sim.pbuf[sim.cur], sim.nalloc,
-                       sim.num_particles, order, sim.prev_dt, sim.cell, sim.W, sim.H, sim.cell_range, tx, sim.mac_u,
-                       sim.mac_v, sim.valid_u, sim.valid_v, sim.density, sim.levels[0].fluid, sim.obstacle,
-                       sim.ext_u, sim.ext_v, (int)forces, ext, dt, sim.props.gravity, sim.step_src);
+    P2gOwner::Args a{sim.pbuf[sim.cur], sim.nalloc, sim.num_particles, order, sim.prev_dt, sim.cell, sim.W, sim.H,
+                     sim.cell_range, tx, sim.mac_u, sim.mac_v, sim.valid_u, sim.valid_v, sim.density,
+                     sim.levels[0].fluid, sim.obstacle, sim.ext_u, sim.ext_v, (int)forces, ext, dt,
+                     sim.props.gravity, sim.gravity_x, sim.step_src};
+    emit<P2gOwner>(sim, a, tx * ty);
     sim.mark(APIC_T_K_P2G, 1);
 }
 
@@ -553,37 +645,36 @@ void launch_forces(Sim& sim, float dt) {
     float g = sim.props.gravity * dt;
     int ext = sim.forces_active;
     hipLaunchKernelGGL(forces_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, sim.stream, sim.mac_u,
-                       sim.mac_v, sim.ext_u, sim.ext_v, sim.levels[0].fluid, sim.W, sim.H, ext, dt, g);
+                       sim.mac_v, sim.ext_u, sim.ext_v, sim.levels[0].fluid, sim.W, sim.H, ext, dt, g,
+                       sim.gravity_x * dt);
 }
 
 void launch_g2p(Sim& sim, float dt, bool write_keys) {
     if (sim.nalloc == 0) {
-        launch_fill_u32(sim.stream, sim.maxv_bits, 0u, 1);
+        launch_fill_u32(sim, sim.maxv_bits, 0u, 1);
         if (sim.h_maxv) *sim.h_maxv = 0u;
         return;
     }
     const unsigned blocks = particle_blocks(sim);
     sim.mark(APIC_T_K_G2P, 0);
-    hipLaunchKernelGGL(g2p_kernel, dim3(blocks), dim3(256), 0, sim.stream, sim.pbuf[sim.cur], sim.nalloc,
-                       sim.num_particles, dt, sim.cell, sim.W, sim.H, sim.timestamp, sim.props.border_damping,
-                       sim.mac_u, sim.mac_v, sim.open_u, sim.open_v, (int)write_keys, sim.keys[0], sim.vals[0],
-                       sim.part_maxv, sim.step_src, sim.tail_guard);
+    G2p::Args a{sim.pbuf[sim.cur], sim.nalloc, sim.num_particles, dt, sim.cell, sim.W, sim.H, sim.timestamp,
+                sim.props.border_damping, sim.mac_u, sim.mac_v, sim.open_u, sim.open_v, (int)write_keys,
+                sim.keys[0], sim.vals[0], sim.part_maxv, sim.step_src, sim.tail_guard, blocks};
+    emit<G2p>(sim, a, blocks);
     sim.mark(APIC_T_K_G2P, 1);
-    hipLaunchKernelGGL(maxv_finalize_kernel, dim3(1), dim3(1024), 0, sim.stream, sim.part_maxv, blocks, sim.maxv_bits,
-                       sim.h_maxv_d, sim.tail_guard);
+    emit<MaxvFinalize>(sim, {sim.part_maxv, blocks, sim.maxv_bits, sim.h_maxv_d, sim.tail_guard}, 1);
 }
 
 void launch_max_velocity(Sim& sim) {
     if (sim.nalloc == 0) {
-        launch_fill_u32(sim.stream, sim.maxv_bits, 0u, 1);
+        launch_fill_u32(sim, sim.maxv_bits, 0u, 1);
         if (sim.h_maxv) *sim.h_maxv = 0u;
         return;
     }
     const unsigned blocks = particle_blocks(sim);
     hipLaunchKernelGGL(max_velocity_kernel, dim3(blocks), dim3(256), 0, sim.stream, sim.pfield(V_X),
                        sim.pfield(V_Y), sim.nalloc, sim.part_maxv);
-    hipLaunchKernelGGL(maxv_finalize_kernel, dim3(1), dim3(1024), 0, sim.stream, sim.part_maxv, blocks, sim.maxv_bits,
-                       sim.h_maxv_d, nullptr);
+    emit<MaxvFinalize>(sim, {sim.part_maxv, blocks, sim.maxv_bits, sim.h_maxv_d, nullptr}, 1);
 }
 
 }  // namespace apic

@@ -196,6 +196,41 @@ apic_status apic_set_profiling(apic_handle* h, int enable);
 /* ms[i] and counts[i] for i < n (n <= APIC_TIMER_COUNT); either may be null. */
 apic_status apic_get_timers(apic_handle* h, double* ms, uint32_t* counts, uint32_t n);
 
+/* Horizontal gravity (extension; the reference applies gravity along y only,
+ * src/apic.rs:1604-1626).  gx is added, times dt, to every u face with a fluid
+ * cell on either side, as WorldProperties.gravity is to v faces; 0 (the
+ * default) leaves the step identical to the reference's. */
+apic_status apic_set_gravity_x(apic_handle* h, float gx);
+
+/* ---- batched stepping (extension; no counterpart in the reference, whose
+ * simulations are stepped one at a time) ------------------------------------- */
+
+/* A batch owns one HIP stream.  Simulations created in it share that stream
+ * and accept every apic_* call; apic_batch_update steps any subset of them
+ * together: each kind of kernel launch of the step is issued once for all
+ * of them, over every simulation's workgroups.  Each simulation computes
+ * exactly what it computes when stepped alone with apic_update (the same
+ * kernels on the same data; the PCG iterations run in lockstep, and a
+ * simulation whose solve has ended skips the rest).  Calls on one batch and
+ * its simulations must come from one thread at a time. */
+typedef struct apic_batch apic_batch;
+
+apic_status apic_batch_create(apic_batch** out);
+/* Fails with APIC_ERR_INVALID_ARGUMENT while simulations of the batch exist. */
+apic_status apic_batch_destroy(apic_batch* b);
+/* apic_create / apic_create_with_particles in the batch (xy null: the
+ * reference scene, n ignored). */
+apic_status apic_batch_create_sim(apic_batch* b, uint32_t width, uint32_t height, float cell_size,
+                                  const apic_world_properties* world_properties, const float* xy,
+                                  uint32_t n, apic_handle** out);
+/* Apic::update(opts, frame_dt[i]) for sims[0..count), count <= 4096, every
+ * simulation of this batch and none twice.  All must use the same solver
+ * mode.  Returns when the last substep's work is queued (as apic_update). */
+apic_status apic_batch_update(apic_batch* b, apic_handle* const* sims, uint32_t count,
+                              const apic_debug_options* opts, const float* frame_dt);
+/* Device memory (bytes) the simulation holds, for admission by free HBM. */
+apic_status apic_device_bytes(apic_handle* h, size_t* bytes);
+
 /* ---- pub fields (src/apic.rs:715-776) --------------------------------------- */
 
 /* width, height, num_particles, num_chunks, multigrid level count (excluding
