@@ -62,10 +62,23 @@ apic_status fail(apic_status st, const std::string& msg) {
         if (e_ != hipSuccess) return fail(APIC_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
     } while (0)
 
+// Device arrays of a simulation are carved out of one allocation (a sizing
+// pass, then an assigning pass over the same sequence); arrays allocated
+// later get their own.
 template <class T>
 apic_status dalloc(Sim& s, T*& p, size_t count) {
     size_t bytes = std::max<size_t>(count * sizeof(T), 16);
     bytes = (bytes + 255) / 256 * 256;
+    if (s.slab_pass == 1) {
+        s.slab_size += bytes;
+        p = nullptr;
+        return APIC_OK;
+    }
+    if (s.slab_pass == 2) {
+        p = reinterpret_cast<T*>(s.slab + s.slab_used);
+        s.slab_used += bytes;
+        return APIC_OK;
+    }
     void* q = nullptr;
     hipError_t e = hipMalloc(&q, bytes);
     if (e != hipSuccess)
@@ -91,10 +104,8 @@ void release(Sim& s) {
     if (s.e_sync) { (void)hipEventDestroy(s.e_sync); s.e_sync = nullptr; }
     for (void* p : s.allocations) (void)hipFree(p);
     s.allocations.clear();
-    if (s.h_pcg) (void)hipHostFree(s.h_pcg);
-    if (s.h_maxv) (void)hipHostFree(s.h_maxv);
-    if (s.h_fstats) (void)hipHostFree(s.h_fstats);
-    if (s.h_step) (void)hipHostFree(s.h_step);
+    if (s.h_block) (void)hipHostFree(s.h_block);
+    s.h_block = nullptr;
     s.h_pcg = nullptr; s.h_maxv = nullptr; s.h_fstats = nullptr; s.h_step = nullptr;
     if (s.stream && s.own_stream) (void)hipStreamDestroy(s.stream);
     s.stream = nullptr;
@@ -157,6 +168,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     }
     const size_t np = std::max<uint32_t>(s.num_particles, 1);
     const size_t na = std::max<size_t>(s.nalloc, 8);
+    auto carve = [&]() -> apic_status {
     ALLOC(s.pbuf[0], NUM_PFIELDS * na);
     ALLOC(s.pbuf[1], NUM_PFIELDS * na);
     ALLOC(s.keys[0], np); ALLOC(s.keys[1], np);
@@ -200,27 +212,55 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     const size_t ngam = std::max<size_t>(kRedBlocks, gamma_partials(s));
     ALLOC(s.part_gamma, ngam); ALLOC(s.part_delta, kRedBlocks); ALLOC(s.part_max, kRedBlocks);
     ALLOC(s.part_fluid, 2 * kRedBlocks);
-    ALLOC(s.part_gamma_d, ngam); ALLOC(s.part_delta_d, kRedBlocks); ALLOC(s.part_rr_d, kRedBlocks);
+    ALLOC(s.part_gamma_d, ngam); ALLOC(s.part_asz_d, ngam);
+    ALLOC(s.part_delta_d, kRedBlocks); ALLOC(s.part_rr_d, kRedBlocks);
     ALLOC(s.part_rr2_d, kRedBlocks); ALLOC(s.pressure_d, nc);
     s.part_maxv_cap = 1024;
     ALLOC(s.part_maxv, s.part_maxv_cap);
     ALLOC(s.fstats, 1);
     ALLOC(s.maxv_bits, 1);
     ALLOC(s.step, 1);
-    // coherent, mapped host memory: kernels write these mirrors directly
-    const unsigned hflags = hipHostMallocMapped | hipHostMallocCoherent;
-    HIP_TRY(hipHostMalloc((void**)&s.h_pcg, sizeof(PcgState), hflags));
-    HIP_TRY(hipHostMalloc((void**)&s.h_maxv, sizeof(uint32_t), hflags));
-    HIP_TRY(hipHostMalloc((void**)&s.h_fstats, sizeof(FluidStats), hflags));
-    HIP_TRY(hipHostMalloc((void**)&s.h_step, sizeof(StepState), hflags));
-    std::memset(s.h_pcg, 0, sizeof(PcgState));
-    *s.h_maxv = 0;
-    std::memset(s.h_fstats, 0, sizeof(FluidStats));
-    std::memset(s.h_step, 0, sizeof(StepState));
-    HIP_TRY(hipHostGetDevicePointer((void**)&s.h_pcg_d, s.h_pcg, 0));
-    HIP_TRY(hipHostGetDevicePointer((void**)&s.h_maxv_d, s.h_maxv, 0));
-    HIP_TRY(hipHostGetDevicePointer((void**)&s.h_fstats_d, s.h_fstats, 0));
-    HIP_TRY(hipHostGetDevicePointer((void**)&s.h_step_d, s.h_step, 0));
+    return APIC_OK;
+    };
+    apic_status cst;
+    s.slab_pass = 1;
+    s.slab_size = 0;
+    if ((cst = carve()) != APIC_OK) return cst;
+    {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, s.slab_size);
+        if (e != hipSuccess)
+            return fail(e == hipErrorOutOfMemory ? APIC_ERR_OUT_OF_MEMORY : APIC_ERR_HIP,
+                        std::string("hipMalloc: ") + hipGetErrorString(e));
+        s.allocations.push_back(q);
+        s.device_bytes += s.slab_size;
+        s.slab = static_cast<uint8_t*>(q);
+        HIP_TRY(hipMemset(q, 0, s.slab_size));
+    }
+    s.slab_pass = 2;
+    s.slab_used = 0;
+    cst = carve();
+    s.slab_pass = 0;
+    if (cst != APIC_OK) return cst;
+    // coherent, mapped host memory (one block): kernels write these mirrors directly
+    {
+        const unsigned hflags = hipHostMallocMapped | hipHostMallocCoherent;
+        auto al = [](size_t v) { return (v + 63) / 64 * 64; };
+        const size_t o_maxv = al(sizeof(PcgState)), o_fs = o_maxv + al(sizeof(uint32_t));
+        const size_t o_step = o_fs + al(sizeof(FluidStats)), total = o_step + al(sizeof(StepState));
+        HIP_TRY(hipHostMalloc((void**)&s.h_block, total, hflags));
+        std::memset(s.h_block, 0, total);
+        uint8_t* hd = nullptr;
+        HIP_TRY(hipHostGetDevicePointer((void**)&hd, s.h_block, 0));
+        s.h_pcg = reinterpret_cast<PcgState*>(s.h_block);
+        s.h_maxv = reinterpret_cast<uint32_t*>(s.h_block + o_maxv);
+        s.h_fstats = reinterpret_cast<FluidStats*>(s.h_block + o_fs);
+        s.h_step = reinterpret_cast<StepState*>(s.h_block + o_step);
+        s.h_pcg_d = reinterpret_cast<PcgState*>(hd);
+        s.h_maxv_d = reinterpret_cast<uint32_t*>(hd + o_maxv);
+        s.h_fstats_d = reinterpret_cast<FluidStats*>(hd + o_fs);
+        s.h_step_d = reinterpret_cast<StepState*>(hd + o_step);
+    }
     HIP_TRY(hipEventCreateWithFlags(&s.e_sync, hipEventDisableTiming));
     {
         StepState st0{};
@@ -239,7 +279,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
         HIP_TRY(hipMemcpy(s.pfield(P_X, 0), px.data(), s.nalloc * 4, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(s.pfield(P_Y, 0), py.data(), s.nalloc * 4, hipMemcpyHostToDevice));
     }
-    HIP_TRY(hipMemcpy(s.obstacle, obstacle.data(), nc, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(s.obstacle, obstacle.data(), s.cells, hipMemcpyHostToDevice));
     // update_world (:913) runs only when the scene has particles
     if (s.num_particles > 0) launch_open_faces(s);
     CHECK_LAUNCH();

@@ -69,6 +69,7 @@ struct PcgState {
     double bnorm2, rnorm2;   // ||b||^2 and the last ||r||^2
     double tol2;             // rel_tol^2
     double gamma_d[2];       // r.z of iteration k in gamma_d[k & 1]
+    double alpha_d;          // the last step length in FP64 (the flexible beta uses it)
     int32_t max_iters, pad2;
 };
 
@@ -193,6 +194,7 @@ struct Sim {
     double* part_delta_d = nullptr;
     double* part_rr_d = nullptr;     // ||r||^2 partials of the recurrence residual
     double* part_rr2_d = nullptr;    // ... after the true-residual check
+    double* part_asz_d = nullptr;    // relative mode: partials of (A s).z for the flexible beta
     double* pressure_d = nullptr;    // relative mode: the pressure accumulated in FP64
     float* aosoa_staging = nullptr;  // chunk-layout mirrors (2 * nalloc floats, allocated on first use)
     uint32_t* part_fluid = nullptr;  // kRedBlocks (count, last+1) pairs of the base fluid cells
@@ -295,6 +297,10 @@ struct Sim {
     // device allocations to free
     std::vector<void*> allocations;
     size_t device_bytes = 0;
+    uint8_t* slab = nullptr;  // the allocation the construction-time arrays are carved from
+    size_t slab_size = 0, slab_used = 0;
+    int slab_pass = 0;        // 1: sizing, 2: assigning, 0: separate allocations
+    uint8_t* h_block = nullptr;  // the pinned allocation holding the host mirrors
 
     size_t nu() const { return (size_t)cells + H; }
     size_t nv() const { return (size_t)cells + W; }

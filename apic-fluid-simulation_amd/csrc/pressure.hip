@@ -758,6 +758,7 @@ struct PcgReset {
         st->bnorm2 = st->rnorm2 = 0.0;
         st->tol2 = a.tol2;
         st->gamma_d[0] = st->gamma_d[1] = 0.0;
+        st->alpha_d = 0.0;
         st->max_iters = a.max_iters;
     }
 };
@@ -907,19 +908,30 @@ __device__ inline bool gamma_exit(PcgState* st, float g, int k, bool lead) {
 __device__ __forceinline__ void spmv_dot(const LevelView& L, const float* __restrict__ s_old, float* __restrict__ s_new,
                                 float* __restrict__ As, PcgState* st, const float* __restrict__ part_gamma,
                                 float* __restrict__ part_delta, int k, const double* __restrict__ part_gamma_d,
-                                double* __restrict__ part_delta_d, int ngam, uint32_t wg, uint32_t nwg) {
+                                const double* __restrict__ part_asz_d, double* __restrict__ part_delta_d, int ngam,
+                                uint32_t wg, uint32_t nwg) {
     const bool lead = wg == 0 && threadIdx.x == 0;
     __shared__ float scratch[kThreads / 64];
     __shared__ double dscratch[kThreads / 64];
     if (st->done) return;
     const bool first = k == 0;
     float beta;
-    if (part_gamma_d) {  // relative mode: FP64 scalars
-        double g = 0.0;
-        for (int i = threadIdx.x; i < ngam; i += blockDim.x) g += part_gamma_d[i];
+    if (part_gamma_d) {  // relative mode: FP64 scalars, flexible beta
+        double g = 0.0, asz = 0.0;
+        for (int i = threadIdx.x; i < ngam; i += blockDim.x) {
+            g += part_gamma_d[i];
+            asz += part_asz_d[i];
+        }
         g = block_sum(g, dscratch);
         if (gamma_exit_rel(st, g, k, lead)) return;
-        beta = first ? 0.0f : (float)(g / st->gamma_d[(k - 1) & 1]);
+        // Polak-Ribiere: r_{k}.(z_{k} - z_{k-1}) / r_{k-1}.z_{k-1}, where
+        // r_{k-1}.z_k = r_k.z_k + alpha_{k-1} (A s_{k-1}).z_k
+        if (!first) {
+            asz = block_sum(asz, dscratch);
+            beta = (float)(-st->alpha_d * asz / st->gamma_d[(k - 1) & 1]);
+        } else {
+            beta = 0.0f;
+        }
         if (lead) { st->gamma_d[k & 1] = g; st->iters = k + 1; }
     } else {
         const float g = sum_partials(part_gamma, scratch, ngam);
@@ -966,13 +978,14 @@ struct SpmvDot {
         const float* part_gamma;
         float* part_delta;
         const double* part_gamma_d;  // relative mode (else null)
+        const double* part_asz_d;
         double* part_delta_d;
         int ngam;
         uint32_t nwg;
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         spmv_dot(a.L, a.search[k & 1], a.search[(k + 1) & 1], a.As, a.st, a.part_gamma, a.part_delta, k,
-                 a.part_gamma_d, a.part_delta_d, a.ngam, wg, a.nwg);
+                 a.part_gamma_d, a.part_asz_d, a.part_delta_d, a.ngam, wg, a.nwg);
     }
 };
 
@@ -996,7 +1009,7 @@ __device__ __forceinline__ void axpy_max(const LevelView& L, const float* __rest
         }
         const double alpha_d = st->gamma_d[k & 1] / d;
         const float alpha = (float)alpha_d;
-        if (lead) { st->delta = (float)d; st->alpha = alpha; }
+        if (lead) { st->delta = (float)d; st->alpha = alpha; st->alpha_d = alpha_d; }
         double rr = 0.0;
         for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
             if (!L.fluid[c]) continue;
@@ -1293,7 +1306,8 @@ struct VDown {
 // of r.z over the tiles a workgroup takes (dot: 1 FP32, 2 FP64).
 __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, const FluidStats* __restrict__ fs,
                            const int32_t* __restrict__ done, uint32_t tiles_x, int dot, float* __restrict__ part_gamma,
-                           double* __restrict__ part_gamma_d, uint32_t wg) {
+                           double* __restrict__ part_gamma_d, const float* __restrict__ As,
+                           double* __restrict__ part_asz_d, uint32_t wg) {
     __shared__ VTile t;
     __shared__ float ce[kVC * kVC];
     __shared__ float scratch[256 / 64];
@@ -1302,7 +1316,7 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     const int32_t w = (int32_t)L.w, h = (int32_t)L.h, cw = (int32_t)C.w, chh = (int32_t)C.h;
     const uint32_t last1 = fs ? fs->last_raw : 0u, cnt = fs ? fs->count : 0u;
     float prod = 0.0f;
-    double dprod = 0.0;
+    double dprod = 0.0, dasz = 0.0;
     {
         const uint32_t tile = wg;
         const int32_t X0 = (int32_t)(tile % tiles_x) * kVT, Y0 = (int32_t)(tile / tiles_x) * kVT;
@@ -1345,14 +1359,23 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
             float e = t.e[li];
             if (t.fl[li] && ((x + y) & 1) == 0)
                 e = vsor(t, li, pv(li), t.e[li + 1], t.e[li + kVR], t.e[li - 1], t.e[li - kVR]);
-            L.e[(size_t)(y + 1) * w + x] = e;
-            if (dot == 2) dprod += (double)t.r[li] * (double)e;
-            else if (dot == 1) prod += t.r[li] * e;
+            const size_t pc = (size_t)(y + 1) * w + x;
+            L.e[pc] = e;
+            if (dot == 2) {
+                dprod += (double)t.r[li] * (double)e;
+                if (t.fl[li]) dasz += (double)As[pc] * (double)e;
+            } else if (dot == 1) {
+                prod += t.r[li] * e;
+            }
         }
     }
     if (dot == 2) {
         const double bs = block_sum(dprod, dscratch);
-        if (threadIdx.x == 0) part_gamma_d[wg] = bs;
+        const double ba = block_sum(dasz, dscratch);
+        if (threadIdx.x == 0) {
+            part_gamma_d[wg] = bs;
+            part_asz_d[wg] = ba;
+        }
     } else if (dot == 1) {
         const float bs = block_sum(prod, scratch);
         if (threadIdx.x == 0) part_gamma[wg] = bs;
@@ -1369,9 +1392,11 @@ struct VUp {
         int dot;
         float* part_gamma;
         double* part_gamma_d;
+        const float* As;      // relative mode: A s of the last iteration and
+        double* part_asz_d;   // the partials of (A s).z
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        vup(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, wg);
+        vup(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, wg);
     }
 };
 
@@ -1476,7 +1501,7 @@ void launch_vcycle(Sim& sim, int mode) {
         const uint32_t nt = tiles(L, tx);
         const int dot = (l == 0 && mode != 0) ? (rel ? 2 : 1) : 0;  // one partial of r.z per tile
         VUp::Args a{L, sim.levels[l + 1], l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot,
-                    sim.part_gamma, sim.part_gamma_d};
+                    sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d};
         emit<VUp>(sim, a, nt);
     }
 }
@@ -1486,8 +1511,8 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     const bool rel = sim.solver.mode == 1;
     sim.mark(APIC_T_K_SPMV, 0);
     SpmvDot::Args a{B, {sim.search[0], sim.search[1]}, sim.As, sim.pcg, sim.part_gamma, sim.part_delta,
-                    rel ? sim.part_gamma_d : nullptr, rel ? sim.part_delta_d : nullptr, (int)gamma_partials(sim),
-                    sim.nred};
+                    rel ? sim.part_gamma_d : nullptr, sim.part_asz_d, rel ? sim.part_delta_d : nullptr,
+                    (int)gamma_partials(sim), sim.nred};
     emit<SpmvDot>(sim, a, sim.nred, k);
     sim.mark(APIC_T_K_SPMV, 1);
     AxpyMax::Args b{B, {sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pcg, sim.part_delta, sim.part_max,

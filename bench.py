@@ -39,14 +39,14 @@ WORKLOAD = "cfg2_dam_break_1024"
 def algorithmic_bytes(n_p, n_alloc, n_f, n_c):
     """Compulsory HBM bytes per launch of the two transfer kernels.
 
-    g2p_kernel: per particle lane read x, y, time residual (12 B), write the 9
+    G2p: per particle lane read x, y, time residual (12 B), write the 9
     particle fields (36 B) and the next sort record (8 B); per fluid cell read
     one u and one v face value (8 B) and their open flags (2 B).
-    p2g_owner_kernel: per particle read the 9 fields (36 B); per grid cell
+    P2gOwner: per particle read the 9 fields (36 B); per grid cell
     write its u face and v face (4 + 4 B), their valid flags (1 + 1 B), the
     density (4 B) and the fluid flag (1 B) -- the reference rewrites every
     face and cell in this stage.
-    spmv_dot_kernel: per fluid cell read z, the old search vector, diag,
+    SpmvDot: per fluid cell read z, the old search vector, diag,
     plus_x, plus_y (20 B) and write the new search vector and A s (8 B); the
     fluid flag of every cell (1 B).
     """
@@ -57,7 +57,9 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c):
     }
 
 
-KERNEL_NAMES = {"kernel_g2p": "g2p_kernel", "kernel_p2g": "p2g_owner_kernel", "kernel_spmv": "spmv_dot_kernel"}
+KERNEL_NAMES = {"kernel_g2p": "one_kernel<G2p>", "kernel_p2g": "one_kernel<P2gOwner>",
+                "kernel_spmv": "one_kernel<SpmvDot>"}
+BATCH_WORKLOAD = "cfg4_batched_256"
 
 
 SOLVERS = {
@@ -137,6 +139,94 @@ def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_s
                 stage_timers=stage_timers, stage_substeps=stage_substeps, n_alloc=n_alloc)
 
 
+def batched_run(count=256, side_min=64, side_max=512, seed=4, steps=10, warmup=3, seq_rounds=2, dt=0.002):
+    """Independent dam breaks of mixed sizes stepped together by one batch
+    update per step, against the same simulations stepped one at a time with
+    Apic.update (each update finished before the next simulation starts).
+    Solver: relative 1e-6 with FP64 reductions.  Per step: wall time of the
+    update plus a stream synchronisation; value = sum of particles x substeps
+    / the summed step times."""
+    import numpy as np
+    from FLIP import scenes
+    from FLIP.apic import Apic, Batch, DebugOptions, WorldProperties
+    t_setup = time.perf_counter()
+    mix = scenes.instance_mix(count, side_min, side_max, seed)
+    n_p = np.array([m["xy"].shape[0] for m in mix], dtype=np.int64)
+    print(f"[bench] batched workload: {count} simulations, {int(n_p.sum())} particles "
+          f"({time.perf_counter() - t_setup:.1f}s to seed)", file=sys.stderr, flush=True)
+
+    def make(create, m):
+        s = create(m["width"], m["height"], m["dx"], WorldProperties(m["gravity"], 0.1, 5.0, 1e-5), m["xy"])
+        s.set_gravity_x(m["gravity_x"])
+        s.set_solver_config(**SOLVERS["relative"])
+        return s
+
+    opts = DebugOptions(False)
+    batch = Batch()
+    t0 = time.perf_counter()
+    sims = []
+    for i, m in enumerate(mix):
+        sims.append(make(batch.create_sim, m))
+        if (i + 1) % 64 == 0 or i + 1 == count:
+            print(f"[bench] created {i + 1}/{count} ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    for w in range(warmup):
+        t0 = time.perf_counter()
+        batch.update(sims, opts, dt)
+        sims[0].get_max_particle_velocity()  # synchronises the batch's stream
+        print(f"[bench] batched warm-up step {w}: {time.perf_counter() - t0:.3f}s", file=sys.stderr, flush=True)
+    lat, ps, iters = [], 0, []
+    for k in range(steps):
+        t0 = time.perf_counter()
+        batch.update(sims, opts, dt)
+        sims[0].get_max_particle_velocity()
+        lat.append(time.perf_counter() - t0)
+        st = [s.stats() for s in sims]
+        ps += int(sum(n * x["substeps"] for n, x in zip(n_p, st)))
+        iters.append(max(x["pcg_iterations_total"] for x in st))
+        print(f"[bench] batched step {k}: {lat[-1] * 1e3:.1f} ms, most PCG iterations {iters[-1]}",
+              file=sys.stderr, flush=True)
+    b_elapsed = float(sum(lat))
+    for s in sims:
+        s.close()
+    batch.close()
+    print(f"[bench] batched: {count} simulations, {steps} steps in {b_elapsed:.3f}s", file=sys.stderr, flush=True)
+
+    alone = [make(lambda w, h, dx, wp, xy: Apic.with_particles(w, h, dx, wp, xy), m) for m in mix]
+    for s in alone:
+        s.update(opts, dt)
+        s.get_max_particle_velocity()
+    print(f"[bench] one at a time: created and warmed up", file=sys.stderr, flush=True)
+    s_elapsed, s_ps = 0.0, 0
+    for r in range(seq_rounds):
+        for s, n in zip(alone, n_p):
+            t0 = time.perf_counter()
+            s.update(opts, dt)
+            s.get_max_particle_velocity()  # this update has finished
+            s_elapsed += time.perf_counter() - t0
+            s_ps += int(n) * s.stats()["substeps"]
+        print(f"[bench] one at a time: round {r} done ({s_elapsed:.2f}s so far)", file=sys.stderr, flush=True)
+    for s in alone:
+        s.close()
+    print(f"[bench] one at a time: {seq_rounds} rounds in {s_elapsed:.3f}s", file=sys.stderr, flush=True)
+    lat_ms = np.array(lat) * 1e3
+    b_value, s_value = ps / b_elapsed, s_ps / s_elapsed
+    sides = [m["width"] for m in mix]
+    return {
+        "workload": BATCH_WORKLOAD,
+        "instances": count,
+        "particles": int(n_p.sum()),
+        "sides": [int(min(sides)), int(max(sides))],
+        "dt": dt,
+        "solver": SOLVER_NAMES["relative"],
+        "batched": {"value": b_value, "unit": "particle-steps/s", "steps": steps, "warmup": warmup,
+                    "ms_per_step": float(lat_ms.mean()), "p50_ms": float(np.percentile(lat_ms, 50)),
+                    "p99_ms": float(np.percentile(lat_ms, 99)), "pcg_iterations_max_mean": float(np.mean(iters))},
+        "one_at_a_time": {"value": s_value, "unit": "particle-steps/s", "rounds": seq_rounds,
+                          "ms_per_round": s_elapsed / seq_rounds * 1e3},
+        "speedup": b_value / s_value,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +235,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solver", choices=sorted(SOLVERS), default="relative")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-size workload")
+    ap.add_argument("--batch-only", action="store_true", help="print only the batched workload's result")
+    ap.add_argument("--batch-count", type=int, default=256, help="simulations in the batched workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one simulation per GPU and no "
@@ -155,6 +248,9 @@ def main():
     from FLIP import scenes
     from FLIP.apic import Apic, DebugOptions, WorldProperties
 
+    if args.batch_only:
+        print(json.dumps(batched_run(count=args.batch_count)))
+        return
     cfg = scenes.CONFIGS[WORKLOAD]
     xy = scenes.config_scene(WORKLOAD)
     n_p = xy.shape[0]
@@ -213,6 +309,8 @@ def main():
                          "ms_per_step": ro["elapsed"] / o_steps * 1e3,
                          "pcg_iterations_mean": float(np.mean(ro["iters"]))},
     }
+    if not args.no_batch:
+        line["batched"] = batched_run(count=args.batch_count)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(xy, cfg, args.cpu_seconds, args.solver)
     print(json.dumps(line))

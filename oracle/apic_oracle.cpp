@@ -218,6 +218,7 @@ enum Pf { PX_ = 0, PY_, VX_, VY_, CUX_, CUY_, CVX_, CVY_, TR_, NPF_ };
 
 template <class R> struct Sim {
     R gravity, border_damping, cfl, solve_error;
+    R gravity_x = R(0);  // horizontal gravity: the library's extension (zero in the reference)
     uint32_t W = 0, H = 0, cells = 0;
     R cell = 1;
     uint32_t num_chunks = 0, num_particles = 0;
@@ -570,6 +571,15 @@ template <class R> struct Sim {
                 uint32_t f = v_index(x, y);
                 if (is_fluid(x, y - 1) == 1 || is_fluid(x, y) == 1) mac_v[f] += g;
             }
+        // the library's horizontal gravity (an extension): the same rule on u faces
+        if (gravity_x != R(0)) {
+            R gx = gravity_x * dt;
+            for (int32_t y = 0; y < (int32_t)H; ++y)
+                for (int32_t x = 0; x <= (int32_t)W; ++x) {
+                    uint32_t f = u_index(x, y);
+                    if (is_fluid(x - 1, y) == 1 || is_fluid(x, y) == 1) mac_u[f] += gx;
+                }
+        }
     }
 
     // build_pressure_system, src/apic.rs:1640-1785
@@ -790,10 +800,16 @@ template <class R> struct Sim {
             stats.pcg_rel_residual = (double)std::sqrt((double)rr / (double)bnorm2);
             if (rr <= tol2 * bnorm2) { stats.pcg_exit_reason = 3; return; }
             apply_preconditioner();
-            A ng = 0;
-            for (size_t i = start; i < end; ++i) ng += A(base.r[i]) * A(base.e[i]);
+            // flexible (Polak-Ribiere) beta = r_{k+1}.(z_{k+1} - z_k) / r_k.z_k, with
+            // r_k.z_{k+1} = r_{k+1}.z_{k+1} + alpha (A s).z_{k+1}: robust when the
+            // V-cycle (a coarse CG with absolute exits) is not a fixed linear operator
+            A ng = 0, asz = 0;
+            for (size_t i = start; i < end; ++i) {
+                ng += A(base.r[i]) * A(base.e[i]);
+                asz += A(As[i]) * A(base.e[i]);
+            }
             if (!(ng > 0)) { stats.pcg_exit_reason = 4; return; }
-            const R beta = R(ng / gamma);
+            const R beta = R(-alpha_a * asz / gamma);
             gamma = ng;
             for (size_t raw = 0; raw < cells; ++raw) {
                 if (!base.fluid[raw]) continue;
@@ -1274,6 +1290,11 @@ void orc_set_solver(void* hv, int mode, double rel_tol, int max_it, int fp64, in
         sim->solver_fp64 = fp64; sim->solver_recompute = recompute;
     };
     if (h->precision == 64) set(S<double>(h)); else set(S<float>(h));
+}
+
+void orc_set_gravity_x(void* hv, double gx) {
+    auto* h = static_cast<Handle*>(hv);
+    if (h->precision == 64) S<double>(h)->gravity_x = gx; else S<float>(h)->gravity_x = (float)gx;
 }
 
 void orc_get_stats(void* hv, apic_step_stats* out) {

@@ -73,6 +73,7 @@ def lib():
         L.orc_get_stats.argtypes = [vp, ctypes.POINTER(StepStats)]
         L.orc_set_p2g_shuffle.argtypes = [vp, ctypes.c_uint32]
         L.orc_set_solver.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_set_gravity_x.argtypes = [vp, ctypes.c_double]
         L.orc_max_particle_velocity.restype = ctypes.c_double
         L.orc_max_particle_velocity.argtypes = [vp]
         L.orc_add_radial_force.argtypes = [vp] + [ctypes.c_float] * 4
@@ -163,6 +164,10 @@ class Oracle:
 
     def set_timestamp(self, ts, prev_dt):
         lib().orc_set_timestamp(self._h, ts, prev_dt)
+
+    def set_gravity_x(self, gx):
+        """The library's horizontal-gravity extension (0 = the reference)."""
+        lib().orc_set_gravity_x(self._h, gx)
 
     def stats(self):
         s = StepStats()

@@ -25,6 +25,8 @@ SPECS = [
     (160, 160, 0.40, 8, 14, 3.0, 0, 0.002),
     (64, 128, 0.60, 5, 15, 0.0, 1, 0.03),   # several substeps per update (CFL-limited dt)
     (32, 32, 0.40, 4, 16, 0.0, 1, 0.002),   # one sort pass (10-bit keys)
+    (80, 72, 0.40, 6, 17, 5.0, 0, 0.002),   # coarsest level 10 x 9: workgroup-wide coarse CG
+    (320, 320, 0.30, 4, 18, -4.0, 1, 0.002),  # four levels above the one-workgroup bottom
 ]
 
 

@@ -21,6 +21,12 @@ SCENES = {
     "ref64": dict(W=64, H=64, cell=1.0, cfg=None, dt=0.016, warm=60),
     # the 128^2 dam break (dx = 1/128, 4 ppc, seed 1), mid-collapse
     "dam128": dict(W=128, H=128, cell=1.0 / 128, cfg="cfg1_dam_break_128", dt=0.002, warm=40),
+    # coarsest level 10 x 9 = 90 cells: the workgroup-wide coarse CG, with the
+    # coarse levels in LDS
+    "ref80x72": dict(W=80, H=72, cell=1.0, cfg=None, dt=0.016, warm=40),
+    # one coarse level of 39 x 39 cells, too large for LDS: the bottom solve
+    # in global memory
+    "ref78": dict(W=78, H=78, cell=1.0, cfg=None, dt=0.016, warm=40),
 }
 
 ORDER = ["update_spatial_lookup", "transfer_particles_to_grid", "apply_external_forces", "build_pressure_system",
