@@ -32,7 +32,7 @@ SYMBOLS = [
     "apic_solve_pcg", "apic_apply_pressure_gradient", "apic_extrapolate_velocity",
     "apic_transfer_grid_to_particles_and_advect", "apic_set_profiling", "apic_get_timers",
     "apic_set_gravity_x", "apic_device_bytes", "apic_batch_create", "apic_batch_destroy",
-    "apic_batch_create_sim", "apic_batch_update",
+    "apic_batch_create_sim", "apic_batch_update", "apic_device_memory",
 ]
 TIMERS = ["sort", "p2g", "pressure_build", "mg_build", "pcg", "gradient", "extrapolate", "g2p",
           "kernel_p2g", "kernel_g2p", "kernel_spmv"]
@@ -109,6 +109,7 @@ def lib():
     L.apic_set_gravity_x.argtypes = [vp, f32]
     L.apic_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_size_t)]
     L.apic_batch_create.argtypes = [ctypes.POINTER(vp)]
+    L.apic_device_memory.argtypes = [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]
     L.apic_batch_destroy.argtypes = [vp]
     L.apic_batch_create_sim.argtypes = [vp, u32, u32, f32, ctypes.POINTER(WorldPropertiesC), vp, u32,
                                         ctypes.POINTER(vp)]

@@ -747,6 +747,16 @@ apic_status apic_device_bytes(apic_handle* h, size_t* bytes) {
     return APIC_OK;
 }
 
+apic_status apic_device_memory(size_t* free_bytes, size_t* total_bytes) {
+    size_t f = 0, t = 0;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(APIC_ERR_NO_DEVICE, "no HIP device found");
+    HIP_TRY(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+    return APIC_OK;
+}
+
 apic_status apic_batch_create(apic_batch** out) {
     if (!out) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
     *out = nullptr;

@@ -227,6 +227,64 @@ def batched_run(count=256, side_min=64, side_max=512, seed=4, steps=10, warmup=3
     }
 
 
+def served_run(pool=128, n_requests=400, rates=(50.0, 200.0, 800.0), seed=5, side_min=64, side_max=1024,
+               dt=0.002):
+    """The served workload: requests of 1-20 updates for simulations of a
+    resident pool (sides 64-1024, 4-8 ppc, relative 1e-6 solver) arriving as a
+    Poisson process, replayed in real time through the continuous-batching
+    server and, for the same trace, through plain Apic.update one request at a
+    time.  Per arrival rate: aggregate particle-steps/s (first arrival to last
+    completion) and request latency (completion - arrival) percentiles."""
+    import copy
+
+    import numpy as np
+    from FLIP import scenes
+    from FLIP.apic import Apic, WorldProperties
+    from FLIP.serve import Server, SimSpec, poisson_trace, run_one_at_a_time, run_served
+    t0 = time.perf_counter()
+    mix = scenes.instance_mix(pool, side_min, side_max, seed)
+    specs = [SimSpec(m["width"], m["height"], m["dx"], m["xy"], gravity=m["gravity"], gravity_x=m["gravity_x"],
+                     solver=SOLVERS["relative"]) for m in mix]
+    n_particles = int(sum(sp.xy.shape[0] for sp in specs))
+    print(f"[bench] served workload: pool of {pool} simulations, {n_particles} particles "
+          f"({time.perf_counter() - t0:.1f}s to seed)", file=sys.stderr, flush=True)
+    out = []
+    for rate in rates:
+        trace = poisson_trace(n_requests, rate, pool, seed * 1000 + int(rate))
+        base_trace = copy.deepcopy(trace)
+        server = Server(specs, frame_dt=dt)
+        server.preload(range(pool))
+        wall = run_served(server, trace)
+        lat = np.array([r.t_done - r.t_arrival for r in trace]) * 1e3
+        served = {"value": server.particle_steps / wall, "unit": "particle-steps/s", "wall_s": wall,
+                  "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                  "ticks": server.ticks, "mean_sims_per_tick": server.stepped / max(server.ticks, 1),
+                  "offloads": server.offloads}
+        server.close()
+        print(f"[bench] served at {rate} req/s: {wall:.2f}s, p50 {served['p50_ms']:.1f} ms, "
+              f"p99 {served['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+        sims = {}
+        for sid, sp in enumerate(specs):
+            s = Apic.with_particles(sp.width, sp.height, sp.dx,
+                                    WorldProperties(sp.gravity, sp.border_damping, sp.cfl, sp.solve_error), sp.xy)
+            s.set_gravity_x(sp.gravity_x)
+            s.set_solver_config(**sp.solver)
+            sims[sid] = s
+        wall2, ps2 = run_one_at_a_time(sims, specs, base_trace, frame_dt=dt)
+        for s in sims.values():
+            s.close()
+        lat2 = np.array([r.t_done - r.t_arrival for r in base_trace]) * 1e3
+        alone = {"value": ps2 / wall2, "unit": "particle-steps/s", "wall_s": wall2,
+                 "p50_ms": float(np.percentile(lat2, 50)), "p99_ms": float(np.percentile(lat2, 99))}
+        print(f"[bench] one at a time at {rate} req/s: {wall2:.2f}s, p50 {alone['p50_ms']:.1f} ms, "
+              f"p99 {alone['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+        out.append({"rate_per_s": rate, "requests": n_requests, "served": served, "one_at_a_time": alone,
+                    "throughput_ratio": served["value"] / alone["value"]})
+    return {"workload": "cfg5_served_pool", "pool": pool, "particles": n_particles,
+            "sides": [int(min(sp.width for sp in specs)), int(max(sp.width for sp in specs))], "dt": dt,
+            "steps_per_request": [1, 20], "solver": SOLVER_NAMES["relative"], "rates": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,6 +296,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched mixed-size workload")
     ap.add_argument("--batch-only", action="store_true", help="print only the batched workload's result")
     ap.add_argument("--batch-count", type=int, default=256, help="simulations in the batched workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
+    ap.add_argument("--serve-only", action="store_true", help="print only the served workload's result")
+    ap.add_argument("--serve-pool", type=int, default=128, help="simulations in the served pool")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one simulation per GPU and no "
@@ -250,6 +311,9 @@ def main():
 
     if args.batch_only:
         print(json.dumps(batched_run(count=args.batch_count)))
+        return
+    if args.serve_only:
+        print(json.dumps(served_run(pool=args.serve_pool)))
         return
     cfg = scenes.CONFIGS[WORKLOAD]
     xy = scenes.config_scene(WORKLOAD)
@@ -311,6 +375,8 @@ def main():
     }
     if not args.no_batch:
         line["batched"] = batched_run(count=args.batch_count)
+    if not args.no_serve:
+        line["served"] = served_run(pool=args.serve_pool)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(xy, cfg, args.cpu_seconds, args.solver)
     print(json.dumps(line))

@@ -224,12 +224,14 @@ apic_status apic_batch_create_sim(apic_batch* b, uint32_t width, uint32_t height
                                   const apic_world_properties* world_properties, const float* xy,
                                   uint32_t n, apic_handle** out);
 /* Apic::update(opts, frame_dt[i]) for sims[0..count), count <= 4096, every
- * simulation of this batch and none twice.  All must use the same solver
- * mode.  Returns when the last substep's work is queued (as apic_update). */
+ * simulation of this batch and none twice.  Returns when the last substep's
+ * work is queued (as apic_update). */
 apic_status apic_batch_update(apic_batch* b, apic_handle* const* sims, uint32_t count,
                               const apic_debug_options* opts, const float* frame_dt);
 /* Device memory (bytes) the simulation holds, for admission by free HBM. */
 apic_status apic_device_bytes(apic_handle* h, size_t* bytes);
+/* Free and total memory of the current HIP device (hipMemGetInfo). */
+apic_status apic_device_memory(size_t* free_bytes, size_t* total_bytes);
 
 /* ---- pub fields (src/apic.rs:715-776) --------------------------------------- */
 
