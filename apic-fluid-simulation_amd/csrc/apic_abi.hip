@@ -340,7 +340,7 @@ apic_status do_pcg(Sim& s, bool zero_pressure) {
     while (true) {
         while (k < check_at && k < pcg_cap(s)) {
             launch_pcg_iteration_head(s, k);
-            launch_vcycle(s, 2);
+            launch_vcycle(s, 2, k);
             ++k;
         }
         launch_pcg_decide(s, k);  // also writes the host mirror
@@ -391,7 +391,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     int k = 0;
     for (; k < std::max(0, s.pcg_batch) && k < pcg_cap(s); ++k) {
         launch_pcg_iteration_head(s, k);
-        launch_vcycle(s, 2);
+        launch_vcycle(s, 2, k);
     }
     launch_pcg_decide(s, k);
     s.mark(APIC_T_PCG, 1);
@@ -436,7 +436,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
         while (true) {
             while (k < check_at && k < pcg_cap(s)) {
                 launch_pcg_iteration_head(s, k);
-                launch_vcycle(s, 2);
+                launch_vcycle(s, 2, k);
                 ++k;
             }
             launch_pcg_decide(s, k);

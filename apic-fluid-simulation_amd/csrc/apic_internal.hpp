@@ -37,9 +37,10 @@ struct LevelView {
 // The one-workgroup bottom of the V-cycle keeps its levels in LDS when they
 // fit: six float arrays over the padded range [0, cells + 2w) per level and
 // one flag byte per cell, plus the coarsest level's three CG vectors and its
-// serial-sum products.
-constexpr size_t kBottomLdsFloats = 13312;
-constexpr size_t kBottomLdsFlags = 4096;
+// serial-sum products.  144 KB of gfx950's 160 KB per workgroup: enough for
+// the levels from 64 x 64 cells down.
+constexpr size_t kBottomLdsFloats = 34816;
+constexpr size_t kBottomLdsFlags = 8192;
 __host__ __device__ inline size_t bottom_padded(const LevelView& L) { return (size_t)L.cells + 2 * (size_t)L.w; }
 inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first) {
     size_t fl = 0, fb = 0;
@@ -119,7 +120,8 @@ void launch_open_faces(Sim& sim);
 // pressure.hip
 void launch_build_pressure(Sim& sim, bool zero_pressure);
 void launch_mg_build(Sim& sim);
-void launch_vcycle(Sim& sim, int mode);  // mode 0: plain, 1: PCG start (gamma), 2: PCG iteration (gamma')
+// mode 0: plain, 1: PCG start (gamma), 2: after PCG iteration k (residual test, gamma')
+void launch_vcycle(Sim& sim, int mode, int k = 0);
 void launch_pcg_reset(Sim& sim);                 // also the relative mode's ||b|| prologue
 int pcg_cap(const Sim& sim);                      // iteration cap of the current solver rules
 size_t gamma_partials(const Sim& sim);            // partials of r.z the V-cycle writes

@@ -819,17 +819,14 @@ struct PcgBnormFinalize {
 // iterations (due): r is replaced by the true residual b - A p evaluated in
 // FP64 against the FP64 pressure (the product with the same fused operations
 // as apply_matrix_and_dot, zero outside the grid).  part_out receives the
-// partials of ||r||^2 that the residual test reads (part_in's when not due).
+// partials of ||r||^2 that the residual test then reads (the update's own,
+// part_in, when not due).
 __device__ __forceinline__ void true_residual(const LevelView& L, const float* __restrict__ rhs,
                                      const double* __restrict__ pd, const PcgState* st,
                                      const double* __restrict__ part_in, double* __restrict__ part_out, int due,
                                      uint32_t wg, uint32_t nwg) {
     __shared__ double dscratch[kThreads / 64];
-    if (st->done) return;
-    if (!due) {
-        if (threadIdx.x == 0) part_out[wg] = part_in[wg];
-        return;
-    }
+    if (st->done || !due) return;
     const int64_t w = L.w, n = L.cells;
     auto pa = [&](int64_t raw) { return raw >= 0 && raw < n ? pd[raw] : 0.0; };
     double acc = 0.0;
@@ -1291,12 +1288,16 @@ struct VDown {
         int check;
         float solve_error;
         const float* part_max;
-        const double* part_rr;  // relative mode (else null)
+        const double* part_rr;   // relative mode (else null): ||r||^2 partials of the update
+        const double* part_rr2;  // ... and of the true residual, on the iterations that recompute it
+        int every;               // the true residual is recomputed after iteration k when (k + 1) % every == 0
         uint32_t nred;
     };
-    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        vdown(a.F, a.C, a.base, a.fuse_red, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max, a.part_rr,
-              a.nred, wg);
+    // k: the PCG iteration whose update the residual test checks
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
+        const bool due = a.every > 0 && (k + 1) % a.every == 0;
+        vdown(a.F, a.C, a.base, a.fuse_red, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max,
+              due ? a.part_rr2 : a.part_rr, a.nred, wg);
     }
 };
 
@@ -1473,7 +1474,7 @@ void launch_zero_pressure(Sim& sim) {
 
 // mode 0: apply_preconditioner alone; 1: PCG start (gamma); 2: PCG iteration
 // (residual test in the first kernel, gamma' partials in the last).
-void launch_vcycle(Sim& sim, int mode) {
+void launch_vcycle(Sim& sim, int mode, int k) {
     const int32_t* done = &sim.pcg->done;
     const int last = (int)sim.levels.size() - 1;
     const int bottom = (int)sim.bottom_level;  // 1 <= bottom <= last
@@ -1487,9 +1488,11 @@ void launch_vcycle(Sim& sim, int mode) {
         const int fuse_red = l + 1 < bottom ? 1 : (int)(bottom < last);
         uint32_t tx;
         const uint32_t nt = tiles(F, tx);
+        const bool rel_check = l == 0 && rel;
         VDown::Args a{F, sim.levels[l + 1], (int)(l == 0), fuse_red, done, tx, sim.pcg, (int)(l == 0 && mode == 2),
-                      sim.props.solve_error, sim.part_max, (l == 0 && rel) ? sim.part_rr2_d : nullptr, sim.nred};
-        emit<VDown>(sim, a, nt);
+                      sim.props.solve_error, sim.part_max, rel_check ? sim.part_rr_d : nullptr,
+                      rel_check ? sim.part_rr2_d : nullptr, rel_check ? sim.solver.recompute_every : 0, sim.nred};
+        emit<VDown>(sim, a, nt, k);
     }
     if (sim.bottom_in_lds)
         emit<BottomLds>(sim, {level_set(sim), bottom, done}, 1);
@@ -1518,9 +1521,11 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     AxpyMax::Args b{B, {sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pcg, sim.part_delta, sim.part_max,
                     rel ? sim.part_delta_d : nullptr, rel ? sim.part_rr_d : nullptr, sim.pressure_d, sim.nred};
     emit<AxpyMax>(sim, b, sim.nred, k);
-    if (rel) {
-        TrueResidual::Args c{B, sim.rhs, sim.pressure_d, sim.pcg, sim.part_rr_d, sim.part_rr2_d,
-                             sim.solver.recompute_every, sim.nred};
+    // the true residual, on the iterations that recompute it (a batch records
+    // one iteration for all k: there the kernel itself skips the others)
+    const int every = sim.solver.recompute_every;
+    if (rel && (sim.rec || (every > 0 && (k + 1) % every == 0))) {
+        TrueResidual::Args c{B, sim.rhs, sim.pressure_d, sim.pcg, sim.part_rr_d, sim.part_rr2_d, every, sim.nred};
         emit<TrueResidual>(sim, c, sim.nred, k);
     }
 }

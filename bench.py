@@ -285,6 +285,37 @@ def served_run(pool=128, n_requests=400, rates=(50.0, 200.0, 800.0), seed=5, sid
             "steps_per_request": [1, 20], "solver": SOLVER_NAMES["relative"], "rates": out}
 
 
+def copy_calibration(gib=1.0, repeats=3):
+    """A plain device-to-device copy of known size (the runtime's copy kernel,
+    through torch), for scaling the FETCH_SIZE and WRITE_SIZE counters: run
+    under rocprofv3 --pmc, its dispatches read and write `gib` GiB each, far
+    more than the L2 and Infinity Cache hold."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(repeats):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / repeats
+    print(json.dumps({"copy_bytes": 4 * n, "repeats": repeats, "ms": dt * 1e3,
+                      "gbs": 2 * 4 * n / dt / 1e9}))
+
+
+def traffic_summary():
+    """HBM bytes per launch of the roofline kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each divided by what the
+    counter reads per byte of a plain copy), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_dam_break_1024.json")
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,6 +330,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
     ap.add_argument("--serve-only", action="store_true", help="print only the served workload's result")
     ap.add_argument("--serve-pool", type=int, default=128, help="simulations in the served pool")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only a 1 GiB device copy (for scaling the HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs one simulation per GPU and no "
@@ -309,6 +342,9 @@ def main():
     from FLIP import scenes
     from FLIP.apic import Apic, DebugOptions, WorldProperties
 
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.batch_only:
         print(json.dumps(batched_run(count=args.batch_count)))
         return
@@ -342,6 +378,10 @@ def main():
     stages = {k: round(v[0] / max(r["stage_substeps"], 1), 4) for k, v in r["stage_timers"].items()
               if not k.startswith("kernel_")}
     rels = [x for x in r["rel"] if x >= 0]
+    tr = traffic_summary()
+    traffic = None
+    if tr and tr.get("kernel") == KERNEL_NAMES[dom]:
+        traffic = tr["hbm_bytes_per_launch"]
 
     line = {
         "metric": METRIC,
@@ -361,7 +401,7 @@ def main():
                    "solver": SOLVER_NAMES[args.solver], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": d["gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (d["gbs"] / HBM_PEAK_GBS) if d["gbs"] else None, "traffic": None,
+                     "frac": (d["gbs"] / HBM_PEAK_GBS) if d["gbs"] else None, "traffic": traffic,
                      "bytes_per_launch": d["bytes"], "avg_launch_ms": d["avg_ms"]},
         "kernels": kernels,
         "stage_ms_per_step": stages,
