@@ -562,6 +562,10 @@ apic_status batch_upload(apic_batch& B, size_t used) {
 
 // Compiles what the recorder holds into B.host at `used`.
 apic_status batch_compile(apic_batch& B, size_t& used, std::vector<Recorder::Launch>& out) {
+    if (B.rec.overflow()) {
+        B.rec.clear();
+        return fail(APIC_ERR_UNSUPPORTED_SIZE, "batched launches: a simulation needs more than 2^20 - 1 workgroups");
+    }
     out = B.rec.compile(B.host, used);
     if (out.empty() && !B.rec.empty())
         return fail(APIC_ERR_HIP, "batched launches: the simulations' launch orders conflict");

@@ -318,7 +318,8 @@ template <class K>
 inline void emit(Sim& s, const typename K::Args& a, uint32_t nwg, int k = 0) {
     if (nwg == 0) return;
     if (s.rec) {
-        s.rec->add(reinterpret_cast<const void*>(&batch_kernel<K>), sizeof(a), &a, K::kThreads, nwg, k);
+        s.rec->add(reinterpret_cast<const void*>(&batch_kernel<K>), sizeof(a), &a, K::kThreads, nwg, k,
+                   !round_robin<K>::value);
         return;
     }
     hipLaunchKernelGGL(one_kernel<K>, dim3(nwg), dim3(K::kThreads), 0, s.stream, a, k);

@@ -11,6 +11,7 @@ void Recorder::clear() {
     groups_.clear();
     chains_.clear();
     cur_ = 0;
+    overflow_ = false;
 }
 
 void Recorder::begin(uint32_t sim) {
@@ -18,7 +19,8 @@ void Recorder::begin(uint32_t sim) {
     if (chains_.size() <= sim) chains_.resize(sim + 1);
 }
 
-int Recorder::find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k) {
+int Recorder::find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k,
+                          bool bands) {
     for (size_t g = 0; g < groups_.size(); ++g) {
         const Group& G = groups_[g];
         if (G.fn == fn && G.occurrence == occurrence && G.k == k) return (int)g;
@@ -29,16 +31,19 @@ int Recorder::find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, 
     G.arg_size = arg_size;
     G.threads = threads;
     G.k = k;
+    G.bands = bands;
     groups_.push_back(std::move(G));
     return (int)groups_.size() - 1;
 }
 
-void Recorder::add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k) {
+void Recorder::add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k,
+                   bool bands) {
+    if (nwg > kMapWgMask || cur_ >= kMaxBatchRows) overflow_ = true;  // kMapWgMask itself stays unused (kMapSkip)
     std::vector<int>& chain = chains_[cur_];
     uint32_t occurrence = 0;
     for (int g : chain)
         if (groups_[g].fn == fn) ++occurrence;
-    const int g = find_or_add(fn, occurrence, arg_size, threads, k);
+    const int g = find_or_add(fn, occurrence, arg_size, threads, k, bands);
     Group& G = groups_[g];
     const uint8_t* p = static_cast<const uint8_t*>(args);
     G.args.insert(G.args.end(), p, p + arg_size);
@@ -49,7 +54,7 @@ void Recorder::add(const void* fn, size_t arg_size, const void* args, uint32_t t
 std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size_t& used) {
     std::vector<Launch> out;
     const size_t ng = groups_.size();
-    if (ng == 0) return out;
+    if (ng == 0 || overflow_) return out;
     // topological order of the groups over every simulation's chain, ties by
     // first appearance (lowest group index)
     std::vector<std::vector<int>> succ(ng);
@@ -73,13 +78,41 @@ std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size
     }
     if (order.size() != ng) return out;
     auto align = [](size_t v) { return (v + 255) / 256 * 256; };
-    size_t total = align(used);
+    // XCD bands: workgroup w of a simulation with n workgroups goes to the
+    // XCD whose band holds w (as xcd_band maps them for one simulation),
+    // rotated by the workgroups laid out before it so that the queues stay
+    // even; the grid interleaves the eight queues (position 8 i + x runs the
+    // i-th entry of XCD x), padded with entries that run nothing.
+    std::vector<std::vector<uint32_t>> maps(ng);
     for (int g : order) {
         const Group& G = groups_[g];
-        size_t nblk = 0;
-        for (uint32_t n : G.nwg) nblk += n;
-        total = align(total) + G.args.size();
-        total = align(total) + nblk * sizeof(uint32_t);
+        std::vector<uint32_t>& m = maps[g];
+        if (!G.bands) {
+            for (size_t row = 0; row < G.nwg.size(); ++row)
+                for (uint32_t wg = 0; wg < G.nwg[row]; ++wg) m.push_back(((uint32_t)row << kMapWgBits) | wg);
+            continue;
+        }
+        std::vector<uint32_t> q[kXcds];
+        uint32_t rot = 0;
+        for (size_t row = 0; row < G.nwg.size(); ++row) {
+            const uint32_t n = G.nwg[row], qn = n / kXcds, rn = n % kXcds;
+            for (uint32_t x = 0; x < kXcds; ++x) {
+                const uint32_t len = qn + (x < rn ? 1u : 0u), first = x < rn ? x * (qn + 1) : rn * (qn + 1) + (x - rn) * qn;
+                std::vector<uint32_t>& dst = q[(x + rot) % kXcds];
+                for (uint32_t j = 0; j < len; ++j) dst.push_back(((uint32_t)row << kMapWgBits) | (first + j));
+            }
+            rot = (rot + rn) % kXcds;
+        }
+        size_t longest = 0;
+        for (uint32_t x = 0; x < kXcds; ++x) longest = std::max(longest, q[x].size());
+        m.assign(longest * kXcds, kMapSkip);
+        for (uint32_t x = 0; x < kXcds; ++x)
+            for (size_t i = 0; i < q[x].size(); ++i) m[i * kXcds + x] = q[x][i];
+    }
+    size_t total = align(used);
+    for (int g : order) {
+        total = align(total) + groups_[g].args.size();
+        total = align(total) + maps[g].size() * sizeof(uint32_t);
     }
     total = align(total);
     if (host.size() < total) host.resize(total);
@@ -96,12 +129,9 @@ std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size
         off += G.args.size();
         off = align(off);
         L.map_off = off;
-        uint32_t* map = reinterpret_cast<uint32_t*>(host.data() + off);
-        size_t n = 0;
-        for (size_t row = 0; row < G.nwg.size(); ++row)
-            for (uint32_t wg = 0; wg < G.nwg[row]; ++wg) map[n++] = ((uint32_t)row << kMapWgBits) | wg;
-        off += n * sizeof(uint32_t);
-        L.grid = (uint32_t)n;
+        std::memcpy(host.data() + off, maps[g].data(), maps[g].size() * sizeof(uint32_t));
+        off += maps[g].size() * sizeof(uint32_t);
+        L.grid = (uint32_t)maps[g].size();
         out.push_back(L);
     }
     used = align(off);

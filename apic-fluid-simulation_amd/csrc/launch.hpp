@@ -18,6 +18,7 @@
 #include <stdint.h>
 
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace apic {
@@ -26,19 +27,42 @@ constexpr int kMapWgBits = 20;                           // workgroups per simul
 constexpr uint32_t kMapWgMask = (1u << kMapWgBits) - 1;
 constexpr uint32_t kMaxBatchRows = 1u << (32 - kMapWgBits);  // simulations per batched launch
 
+// Workgroup i of a grid runs on XCD i % kXcds (round-robin dispatch), and
+// every XCD has its own L2.  The logical workgroup index gives each XCD a
+// contiguous band of the grid's logical workgroups, so that the neighbouring
+// rows a stencil reads mostly come from the same L2.  A bijection of
+// [0, n): every logical workgroup still runs exactly once.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_band(uint32_t hw, uint32_t n) {
+    const uint32_t xcd = hw % kXcds, slot = hw / kXcds;
+    const uint32_t q = n / kXcds, r = n % kXcds;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// A kernel whose work per workgroup follows the particles (uneven over the
+// grid) declares kRoundRobin = true and keeps the dispatch order: bands would
+// leave the XCDs that hold empty regions idle.
+template <class K, class = void>
+struct round_robin : std::false_type {};
+template <class K>
+struct round_robin<K, std::void_t<decltype(K::kRoundRobin)>> : std::bool_constant<K::kRoundRobin> {};
+
 template <class K>
 __global__ __launch_bounds__(K::kThreads) void one_kernel(typename K::Args a, int k) {
-    K::run(a, blockIdx.x, k);
+    K::run(a, round_robin<K>::value ? blockIdx.x : xcd_band(blockIdx.x, gridDim.x), k);
 }
 
 // The argument table is read through the constant address space, as kernel
 // arguments are: the rows are never written while a launch runs, and the
 // loads of a workgroup's (uniform) row then go through the scalar unit.
+constexpr uint32_t kMapSkip = 0xFFFFFFFFu;  // a map entry that runs nothing (XCD queue padding)
+
 template <class K>
 __global__ __launch_bounds__(K::kThreads) void batch_kernel(const typename K::Args* __restrict__ tab,
                                                             const uint32_t* __restrict__ map, int k) {
     using ConstArgs = const __attribute__((address_space(4))) typename K::Args;
     const uint32_t m = map[blockIdx.x];
+    if (m == kMapSkip) return;
     const ConstArgs* row = (const ConstArgs*)tab + (m >> kMapWgBits);
     K::run(*(const typename K::Args*)row, m & kMapWgMask, k);
 }
@@ -64,8 +88,12 @@ public:
     };
 
     void begin(uint32_t sim);  // the following launches belong to simulation `sim`
-    void add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k);
+    // bands: lay each simulation's workgroups out in XCD bands (as one_kernel
+    // does), else in dispatch order
+    void add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k, bool bands);
     bool empty() const { return groups_.empty(); }
+    // a simulation's launch had more workgroups than a map entry can name
+    bool overflow() const { return overflow_; }
     // Appends the tables and maps to host[used, ...) (growing it as needed,
     // advancing `used`) and returns the launches in issue order, with offsets
     // from host[0].  An empty result with launches recorded means that the
@@ -81,13 +109,15 @@ private:
         size_t arg_size;
         uint32_t threads;
         int k;
+        bool bands;
         std::vector<uint8_t> args;
         std::vector<uint32_t> nwg;  // per table row
     };
     std::vector<Group> groups_;
     std::vector<std::vector<int>> chains_;  // per simulation: its groups in launch order
     uint32_t cur_ = 0;
-    int find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k);
+    bool overflow_ = false;
+    int find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k, bool bands);
 };
 
 // Issues compiled launches whose tables were uploaded to `dev`.  k >= 0

@@ -327,6 +327,7 @@ __device__ __forceinline__ void p2g_owner(
 
 struct P2gOwner {
     static constexpr int kThreads = kOwnCells;
+    static constexpr bool kRoundRobin = true;  // work follows the particles
     struct Args {
         const float* pb;
         size_t nalloc;
