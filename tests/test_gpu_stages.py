@@ -27,6 +27,9 @@ SCENES = {
     # one coarse level of 39 x 39 cells, too large for LDS: the bottom solve
     # in global memory
     "ref78": dict(W=78, H=78, cell=1.0, cfg=None, dt=0.016, warm=40),
+    # 256^2: two V-cycle levels above the one-workgroup bottom, whose LDS
+    # holds the levels from 64^2 down
+    "ref256": dict(W=256, H=256, cell=1.0, cfg=None, dt=0.016, warm=8),
 }
 
 ORDER = ["update_spatial_lookup", "transfer_particles_to_grid", "apply_external_forces", "build_pressure_system",
