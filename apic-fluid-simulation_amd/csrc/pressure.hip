@@ -1218,6 +1218,13 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         cfl[q] = in ? C.fluid[(size_t)cy * cw + cx] : 0;
         cid[q] = in ? C.inv_diag[(size_t)(cy + 1) * cw + cx] : 0.0f;
     }
+    // A tile without a fluid cell (none of its coarse cells is fluid) has
+    // nothing to compute: the values of non-fluid cells are only ever read
+    // times zero coefficients, so it leaves them as they are.
+    bool any = false;
+#pragma unroll
+    for (int q = 0; q < kCPer; ++q) any = any || cfl[q] != 0;
+    if (!__syncthreads_or(any)) return;
     vtile_load(F, X0, Y0, t, !base, true);
     __syncthreads();
     if (base) {  // red_zero_cell over the region
@@ -1315,6 +1322,19 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     __shared__ double dscratch[256 / 64];
     if (*done) return;
     const int32_t w = (int32_t)L.w, h = (int32_t)L.h, cw = (int32_t)C.w, chh = (int32_t)C.h;
+    {
+        // a tile without a fluid cell (see vdown) only reports a zero partial
+        const int32_t X0 = (int32_t)(wg % tiles_x) * kVT, Y0 = (int32_t)(wg / tiles_x) * kVT;
+        const int32_t ccx = X0 / 2 + (int32_t)(threadIdx.x % (kVT / 2)), ccy = Y0 / 2 + (int32_t)(threadIdx.x / (kVT / 2));
+        const bool cf = threadIdx.x < (kVT / 2) * (kVT / 2) && ccx < cw && ccy < chh && C.fluid[(size_t)ccy * cw + ccx];
+        if (!__syncthreads_or(cf)) {
+            if (threadIdx.x == 0) {
+                if (dot == 2) { part_gamma_d[wg] = 0.0; part_asz_d[wg] = 0.0; }
+                else if (dot == 1) part_gamma[wg] = 0.0f;
+            }
+            return;
+        }
+    }
     const uint32_t last1 = fs ? fs->last_raw : 0u, cnt = fs ? fs->count : 0u;
     float prod = 0.0f;
     double dprod = 0.0, dasz = 0.0;
