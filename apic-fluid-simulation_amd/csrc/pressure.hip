@@ -1124,34 +1124,56 @@ struct VTile {
     uint8_t fl[kVN];  // fluid flag; 0 outside the grid
 };
 
-// All loads of a thread are issued before its LDS stores (compile-time
-// stride: the workgroup has 256 threads).
-__device__ inline void vtile_load(const LevelView& L, int32_t X0, int32_t Y0, VTile& t, bool load_e, bool load_dg) {
-    constexpr int kPer = (kVN + 255) / 256;
-    const int64_t w = L.w, h = L.h, np = (int64_t)L.cells + 2 * w;
+// A thread's share of a region, held in registers between the loads and the
+// LDS stores, so that all of the loads go out before any of them is waited
+// for (compile-time stride: the workgroup has 256 threads).
+struct VRegs {
+    static constexpr int kPer = (kVN + 255) / 256;
     float r[kPer], id[kPer], px[kPer], py[kPer], dg[kPer], e[kPer];
     uint8_t fl[kPer];
+    uint32_t inb, real;  // bit k: slot k is in the padded range / a cell of the grid
+};
+
+// Loads at clamped addresses (no branch per load, so that they all issue back
+// to back); the values outside the range are replaced by zeros in vtile_put,
+// after the loads the caller issues next.
+__device__ __forceinline__ void vtile_fetch(const LevelView& L, int32_t X0, int32_t Y0, VRegs& g, bool load_e,
+                                            bool load_dg) {
+    const int64_t w = L.w, h = L.h, np = (int64_t)L.cells + 2 * w;
+    g.inb = g.real = 0u;
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
+    for (int k = 0; k < VRegs::kPer; ++k) {
         const int i = (int)threadIdx.x + 256 * k;
         const int32_t x = X0 - kVH + i % kVR, y = Y0 - kVH + i / kVR;
         const int64_t p = (int64_t)(y + 1) * w + x;
         const bool inb = i < kVN && p >= 0 && p < np;
         const bool real = i < kVN && x >= 0 && x < w && y >= 0 && y < h;
-        r[k] = inb ? L.r[p] : 0.0f;
-        id[k] = inb ? L.inv_diag[p] : 0.0f;
-        px[k] = inb ? L.px[p] : 0.0f;
-        py[k] = inb ? L.py[p] : 0.0f;
-        dg[k] = (load_dg && inb) ? L.diag[p] : 0.0f;
-        e[k] = (load_e && inb) ? L.e[p] : 0.0f;
-        fl[k] = real ? L.fluid[(size_t)y * w + x] : 0;
+        g.inb |= (uint32_t)inb << k;
+        g.real |= (uint32_t)real << k;
+        const int64_t q = inb ? p : 0;
+        g.r[k] = L.r[q];
+        g.id[k] = L.inv_diag[q];
+        g.px[k] = L.px[q];
+        g.py[k] = L.py[q];
+        g.dg[k] = load_dg ? L.diag[q] : 0.0f;
+        g.e[k] = load_e ? L.e[q] : 0.0f;
+        g.fl[k] = L.fluid[real ? (size_t)y * w + x : 0];
     }
+}
+
+__device__ __forceinline__ void vtile_put(const VRegs& g, VTile& t) {
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
+    for (int k = 0; k < VRegs::kPer; ++k) {
         const int i = (int)threadIdx.x + 256 * k;
         if (i >= kVN) continue;
-        t.r[i] = r[k]; t.id[i] = id[k]; t.px[i] = px[k]; t.py[i] = py[k]; t.dg[i] = dg[k]; t.e[i] = e[k];
-        t.fl[i] = fl[k];
+        const bool inb = (g.inb >> k) & 1u, real = (g.real >> k) & 1u;
+        t.r[i] = inb ? g.r[k] : 0.0f;
+        t.id[i] = inb ? g.id[k] : 0.0f;
+        t.px[i] = inb ? g.px[k] : 0.0f;
+        t.py[i] = inb ? g.py[k] : 0.0f;
+        t.dg[i] = inb ? g.dg[k] : 0.0f;
+        t.e[i] = inb ? g.e[k] : 0.0f;
+        t.fl[i] = real ? g.fl[k] : 0;
     }
 }
 
@@ -1175,18 +1197,58 @@ __device__ inline float vsor(const VTile& t, int i, float own, float er, float e
 // the base level, in a PCG iteration, every workgroup first applies the
 // residual test of the previous update (max |r| < solve_error,
 // src/apic.rs:1930, or the relative test).
-__device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, int base, int fuse_red,
+__device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, int base, int fuse_red, int spec,
                              const int32_t* __restrict__ done, uint32_t tiles_x, PcgState* st, int check,
                              float solve_error, const float* __restrict__ part_max, const double* __restrict__ part_rr,
                              uint32_t nred, uint32_t wg) {
     __shared__ VTile t;
     __shared__ float scratch[256 / 64];
     __shared__ double dscratch[256 / 64];
-    if (*done) return;
     const bool lead = wg == 0 && threadIdx.x == 0;
+    const int32_t X0 = (int32_t)(wg % tiles_x) * kVT, Y0 = (int32_t)(wg / tiles_x) * kVT;
+    const int32_t w = (int32_t)F.w, h = (int32_t)F.h;
+    const int32_t cw = (int32_t)C.w, ch = (int32_t)C.h;
+    // Loads that do not depend on each other go out together, so that the
+    // workgroup waits for them once: the done flag, the residual test's
+    // partials (nred <= kRedBlocks), the coarse cells' flags and inverse
+    // diagonals and, with spec, the region itself before it is known whether
+    // the tile has work (a coarse level has few tiles: its time is the latency
+    // of these loads, not their bytes).
+    const int32_t dn = *done;
+    VRegs g;
+    if (spec) vtile_fetch(F, X0, Y0, g, !base, true);
+    constexpr int kCPer = (kVT / 2) * (kVT / 2) / 256;
+    uint8_t cfl[kCPer];
+    float cid[kCPer];
+    bool cin[kCPer];
+#pragma unroll
+    for (int q = 0; q < kCPer; ++q) {
+        const int k = (int)threadIdx.x + 256 * q;
+        const int32_t cx = X0 / 2 + k % (kVT / 2), cy = Y0 / 2 + k / (kVT / 2);
+        const bool in = cx < cw && cy < ch;  // (the restriction below skips the others)
+        const size_t c = in ? (size_t)cy * cw + cx : 0;
+        cfl[q] = C.fluid[c];
+        cid[q] = C.inv_diag[c + cw];
+        cin[q] = in;
+    }
+    constexpr int kRPer = kRedBlocks / 256;
+    double prr[kRPer];
+    float pmx[kRPer];
+#pragma unroll
+    for (int q = 0; q < kRPer; ++q) {
+        const uint32_t i = threadIdx.x + 256u * q, ic = i < nred ? i : 0u;
+        prr[q] = 0.0;
+        pmx[q] = 0.0f;
+        if (check && part_rr) prr[q] = part_rr[ic];
+        else if (check) pmx[q] = part_max[ic];
+    }
+    if (spec) vtile_put(g, t);  // read after the barrier of the tile test below
+    if (dn) return;
     if (check && part_rr) {
         double rr = 0.0;
-        for (uint32_t i = threadIdx.x; i < nred; i += blockDim.x) rr += part_rr[i];
+#pragma unroll
+        for (int q = 0; q < kRPer; ++q)
+            if (threadIdx.x + 256u * q < nred) rr += prr[q];
         rr = block_sum(rr, dscratch);
         if (lead) st->rnorm2 = rr;
         if (rr <= st->tol2 * st->bnorm2) {
@@ -1195,7 +1257,9 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         }
     } else if (check) {
         float m = 0.0f;
-        for (uint32_t i = threadIdx.x; i < nred; i += blockDim.x) m = fast_max(m, part_max[i]);
+#pragma unroll
+        for (int q = 0; q < kRPer; ++q)
+            if (threadIdx.x + 256u * q < nred) m = fast_max(m, pmx[q]);
         m = block_max(m, scratch);
         if (lead) st->maxerr = m;
         if (m < solve_error) {
@@ -1203,30 +1267,18 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
             return;
         }
     }
-    const int32_t X0 = (int32_t)(wg % tiles_x) * kVT, Y0 = (int32_t)(wg / tiles_x) * kVT;
-    const int32_t w = (int32_t)F.w, h = (int32_t)F.h;
-    const int32_t cw = (int32_t)C.w, ch = (int32_t)C.h;
-    // the coarse cells' flags and inverse diagonals, loaded with the tile
-    constexpr int kCPer = (kVT / 2) * (kVT / 2) / 256;
-    uint8_t cfl[kCPer];
-    float cid[kCPer];
-#pragma unroll
-    for (int q = 0; q < kCPer; ++q) {
-        const int k = (int)threadIdx.x + 256 * q;
-        const int32_t cx = X0 / 2 + k % (kVT / 2), cy = Y0 / 2 + k / (kVT / 2);
-        const bool in = cx < cw && cy < ch;
-        cfl[q] = in ? C.fluid[(size_t)cy * cw + cx] : 0;
-        cid[q] = in ? C.inv_diag[(size_t)(cy + 1) * cw + cx] : 0.0f;
-    }
     // A tile without a fluid cell (none of its coarse cells is fluid) has
     // nothing to compute: the values of non-fluid cells are only ever read
     // times zero coefficients, so it leaves them as they are.
     bool any = false;
 #pragma unroll
-    for (int q = 0; q < kCPer; ++q) any = any || cfl[q] != 0;
+    for (int q = 0; q < kCPer; ++q) any = any || (cin[q] && cfl[q] != 0);
     if (!__syncthreads_or(any)) return;
-    vtile_load(F, X0, Y0, t, !base, true);
-    __syncthreads();
+    if (!spec) {
+        vtile_fetch(F, X0, Y0, g, !base, true);
+        vtile_put(g, t);
+        __syncthreads();
+    }
     if (base) {  // red_zero_cell over the region
         for (int i = threadIdx.x; i < kVN; i += blockDim.x) {
             const int32_t x = X0 - kVH + i % kVR, y = Y0 - kVH + i / kVR;
@@ -1289,6 +1341,7 @@ struct VDown {
     struct Args {
         LevelView F, C;
         int base, fuse_red;
+        int spec;                // load the region with the flags (levels of few tiles)
         const int32_t* done;
         uint32_t tiles_x;
         PcgState* st;
@@ -1303,7 +1356,7 @@ struct VDown {
     // k: the PCG iteration whose update the residual test checks
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         const bool due = a.every > 0 && (k + 1) % a.every == 0;
-        vdown(a.F, a.C, a.base, a.fuse_red, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max,
+        vdown(a.F, a.C, a.base, a.fuse_red, a.spec, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max,
               due ? a.part_rr2 : a.part_rr, a.nred, wg);
     }
 };
@@ -1315,39 +1368,74 @@ struct VDown {
 __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, const FluidStats* __restrict__ fs,
                            const int32_t* __restrict__ done, uint32_t tiles_x, int dot, float* __restrict__ part_gamma,
                            double* __restrict__ part_gamma_d, const float* __restrict__ As,
-                           double* __restrict__ part_asz_d, uint32_t wg) {
+                           double* __restrict__ part_asz_d, int spec, uint32_t wg) {
     __shared__ VTile t;
     __shared__ float ce[kVC * kVC];
     __shared__ float scratch[256 / 64];
     __shared__ double dscratch[256 / 64];
-    if (*done) return;
     const int32_t w = (int32_t)L.w, h = (int32_t)L.h, cw = (int32_t)C.w, chh = (int32_t)C.h;
-    {
-        // a tile without a fluid cell (see vdown) only reports a zero partial
-        const int32_t X0 = (int32_t)(wg % tiles_x) * kVT, Y0 = (int32_t)(wg / tiles_x) * kVT;
-        const int32_t ccx = X0 / 2 + (int32_t)(threadIdx.x % (kVT / 2)), ccy = Y0 / 2 + (int32_t)(threadIdx.x / (kVT / 2));
-        const bool cf = threadIdx.x < (kVT / 2) * (kVT / 2) && ccx < cw && ccy < chh && C.fluid[(size_t)ccy * cw + ccx];
-        if (!__syncthreads_or(cf)) {
-            if (threadIdx.x == 0) {
-                if (dot == 2) { part_gamma_d[wg] = 0.0; part_asz_d[wg] = 0.0; }
-                else if (dot == 1) part_gamma[wg] = 0.0f;
-            }
-            return;
+    const int32_t X0 = (int32_t)(wg % tiles_x) * kVT, Y0 = (int32_t)(wg / tiles_x) * kVT;
+    const int32_t CX0 = X0 / 2 - 1, CY0 = Y0 / 2 - 1;
+    // Loads that do not depend on each other go out together (as in vdown):
+    // the done flag, the tile's coarse flags, the parent's error over the
+    // region, the fluid statistics and, with spec, the region itself.
+    const int32_t dn = *done;
+    // A s of the thread's tile cells (the partials of (A s).z)
+    constexpr int kTPer = kVT * kVT / 256;
+    float asv[kTPer];
+    auto fetch_as = [&]() {
+#pragma unroll
+        for (int q = 0; q < kTPer; ++q) {
+            const int i = (int)threadIdx.x + 256 * q;
+            const int32_t x = X0 + i % kVT, y = Y0 + i / kVT;
+            const bool in = x < w && y < h;  // (the sweep below skips the others)
+            asv[q] = dot == 2 ? As[in ? (size_t)(y + 1) * w + x : 0] : 0.0f;
         }
+    };
+    VRegs g;
+    if (spec) {
+        vtile_fetch(L, X0, Y0, g, true, false);
+        fetch_as();
     }
+    const int32_t ccx = X0 / 2 + (int32_t)(threadIdx.x % (kVT / 2)), ccy = Y0 / 2 + (int32_t)(threadIdx.x / (kVT / 2));
+    constexpr int kEPer = (kVC * kVC + 255) / 256;
+    float cev[kEPer];
+    uint32_t cin_mask = 0u;
+#pragma unroll
+    for (int q = 0; q < kEPer; ++q) {
+        const int j = (int)threadIdx.x + 256 * q;
+        const int32_t cx = CX0 + j % kVC, cy = CY0 + j / kVC;
+        const bool in = j < kVC * kVC && cx >= 0 && cx < cw && cy >= 0 && cy < chh;
+        cin_mask |= (uint32_t)in << q;
+        cev[q] = C.e[in ? (size_t)(cy + 1) * cw + cx : 0];
+    }
+    const bool cin = threadIdx.x < (kVT / 2) * (kVT / 2) && ccx < cw && ccy < chh;
+    const uint8_t cfv = C.fluid[cin ? (size_t)ccy * cw + ccx : 0];
     const uint32_t last1 = fs ? fs->last_raw : 0u, cnt = fs ? fs->count : 0u;
+    if (spec) vtile_put(g, t);  // read after the barrier of the tile test below
+#pragma unroll
+    for (int q = 0; q < kEPer; ++q) {
+        const int j = (int)threadIdx.x + 256 * q;
+        if (j < kVC * kVC) ce[j] = ((cin_mask >> q) & 1u) ? cev[q] : 0.0f;
+    }
+    if (dn) return;
+    // a tile without a fluid cell (see vdown) only reports a zero partial
+    if (!__syncthreads_or(cin && cfv != 0)) {
+        if (threadIdx.x == 0) {
+            if (dot == 2) { part_gamma_d[wg] = 0.0; part_asz_d[wg] = 0.0; }
+            else if (dot == 1) part_gamma[wg] = 0.0f;
+        }
+        return;
+    }
+    if (!spec) {
+        vtile_fetch(L, X0, Y0, g, true, false);
+        fetch_as();
+        vtile_put(g, t);
+        __syncthreads();
+    }
     float prod = 0.0f;
     double dprod = 0.0, dasz = 0.0;
     {
-        const uint32_t tile = wg;
-        const int32_t X0 = (int32_t)(tile % tiles_x) * kVT, Y0 = (int32_t)(tile / tiles_x) * kVT;
-        const int32_t CX0 = X0 / 2 - 1, CY0 = Y0 / 2 - 1;
-        vtile_load(L, X0, Y0, t, true, false);
-        for (int j = threadIdx.x; j < kVC * kVC; j += blockDim.x) {
-            const int32_t cx = CX0 + j % kVC, cy = CY0 + j / kVC;
-            ce[j] = (cx >= 0 && cx < cw && cy >= 0 && cy < chh) ? C.e[(size_t)(cy + 1) * cw + cx] : 0.0f;
-        }
-        __syncthreads();
         // e plus the parent's error (prolonged); non-fluid cells keep e
         auto pv = [&](int i) -> float {
             float v = t.e[i];
@@ -1373,7 +1461,9 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         }
         __syncthreads();
         // second colour (red) of the tile: own prolongated, neighbours final
-        for (int i = threadIdx.x; i < kVT * kVT; i += blockDim.x) {
+#pragma unroll
+        for (int q = 0; q < kTPer; ++q) {
+            const int i = (int)threadIdx.x + 256 * q;
             const int32_t x = X0 + i % kVT, y = Y0 + i / kVT;
             if (x >= w || y >= h) continue;
             const int li = (y - Y0 + kVH) * kVR + (x - X0 + kVH);
@@ -1384,7 +1474,7 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
             L.e[pc] = e;
             if (dot == 2) {
                 dprod += (double)t.r[li] * (double)e;
-                if (t.fl[li]) dasz += (double)As[pc] * (double)e;
+                if (t.fl[li]) dasz += (double)asv[q] * (double)e;
             } else if (dot == 1) {
                 prod += t.r[li] * e;
             }
@@ -1415,9 +1505,11 @@ struct VUp {
         double* part_gamma_d;
         const float* As;      // relative mode: A s of the last iteration and
         double* part_asz_d;   // the partials of (A s).z
+        int spec;             // load the region with the flags (levels of few tiles)
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        vup(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, wg);
+        vup(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, a.spec,
+            wg);
     }
 };
 
@@ -1509,7 +1601,7 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         uint32_t tx;
         const uint32_t nt = tiles(F, tx);
         const bool rel_check = l == 0 && rel;
-        VDown::Args a{F, sim.levels[l + 1], (int)(l == 0), fuse_red, done, tx, sim.pcg, (int)(l == 0 && mode == 2),
+        VDown::Args a{F, sim.levels[l + 1], (int)(l == 0), fuse_red, (int)(l > 0), done, tx, sim.pcg, (int)(l == 0 && mode == 2),
                       sim.props.solve_error, sim.part_max, rel_check ? sim.part_rr_d : nullptr,
                       rel_check ? sim.part_rr2_d : nullptr, rel_check ? sim.solver.recompute_every : 0, sim.nred};
         emit<VDown>(sim, a, nt, k);
@@ -1524,7 +1616,7 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         const uint32_t nt = tiles(L, tx);
         const int dot = (l == 0 && mode != 0) ? (rel ? 2 : 1) : 0;  // one partial of r.z per tile
         VUp::Args a{L, sim.levels[l + 1], l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot,
-                    sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d};
+                    sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d, (int)(l > 0)};
         emit<VUp>(sim, a, nt);
     }
 }
