@@ -1105,30 +1105,49 @@ struct Decide {
 };
 
 // ---- fused V-cycle kernels: one launch per level on the way down and one on
-// the way up.  A workgroup takes a kVT x kVT tile of the level and loads it
-// with a kVH-cell ring into LDS, then recomputes what the ring needs, so that
-// the per-cell arithmetic (operation order, fused multiply-adds) is exactly
-// that of red_zero_cell, sor_cell and restrict_cell above.  Ring positions
-// outside the grid are loaded by their padded flat index, the address the
-// global kernels read for a row's first and last cell; their coefficients
-// there are zero.
+// the way up.  A workgroup takes a T x T tile of the level and loads it with a
+// kVH-cell ring into LDS, then recomputes what the ring needs, so that the
+// per-cell arithmetic (operation order, fused multiply-adds) is exactly that
+// of red_zero_cell, sor_cell and restrict_cell above, whatever the tile side.
+// Ring positions outside the grid are loaded by their padded flat index, the
+// address the global kernels read for a row's first and last cell; their
+// coefficients there are zero.
+//
+// Tiles are 32 x 32 on a level with many of them and 16 x 16 on a level with
+// few: there a launch fills a fraction of the CUs with one workgroup each, its
+// time is the latency of the workgroup's phases, and a smaller tile spreads
+// the level over four times as many CUs.
 
-constexpr int kVT = 32;                 // tile side (cells of the level)
-constexpr int kVH = 2;                  // ring: the first colour of a sweep needs two
-constexpr int kVR = kVT + 2 * kVH;      // region side
-constexpr int kVN = kVR * kVR;
-constexpr int kVC = kVT / 2 + 2;        // parent cells covering the region
+constexpr int kVH = 2;         // ring: the first colour of a sweep needs two
+constexpr int kBaseTile = 32;  // tile side on the base level (one partial of r.z per tile)
 
+template <int T>
+struct VGeom {
+    static constexpr int kT = T;                        // tile side (cells of the level)
+    static constexpr int kR = T + 2 * kVH;              // region side
+    static constexpr int kN = kR * kR;                  // region cells
+    static constexpr int kC = T / 2 + 2;                // parent cells across the region
+    static constexpr int kPer = (kN + 255) / 256;       // region cells per thread
+    static constexpr int kCN = (T / 2) * (T / 2);       // the tile's coarse cells
+    static constexpr int kCPer = (kCN + 255) / 256;
+    static constexpr int kTPer = (T * T + 255) / 256;   // tile cells per thread
+    static constexpr int kEPer = (kC * kC + 255) / 256;  // parent errors per thread
+    static_assert(kPer <= 32, "region slots are masks of 32 bits");
+};
+
+template <int T>
 struct VTile {
-    float e[kVN], r[kVN], id[kVN], px[kVN], py[kVN], dg[kVN];
-    uint8_t fl[kVN];  // fluid flag; 0 outside the grid
+    using G = VGeom<T>;
+    float e[G::kN], r[G::kN], id[G::kN], px[G::kN], py[G::kN], dg[G::kN];
+    uint8_t fl[G::kN];  // fluid flag; 0 outside the grid
 };
 
 // A thread's share of a region, held in registers between the loads and the
 // LDS stores, so that all of the loads go out before any of them is waited
 // for (compile-time stride: the workgroup has 256 threads).
+template <int T>
 struct VRegs {
-    static constexpr int kPer = (kVN + 255) / 256;
+    static constexpr int kPer = VGeom<T>::kPer;
     float r[kPer], id[kPer], px[kPer], py[kPer], dg[kPer], e[kPer];
     uint8_t fl[kPer];
     uint32_t inb, real;  // bit k: slot k is in the padded range / a cell of the grid
@@ -1137,17 +1156,19 @@ struct VRegs {
 // Loads at clamped addresses (no branch per load, so that they all issue back
 // to back); the values outside the range are replaced by zeros in vtile_put,
 // after the loads the caller issues next.
-__device__ __forceinline__ void vtile_fetch(const LevelView& L, int32_t X0, int32_t Y0, VRegs& g, bool load_e,
+template <int T>
+__device__ __forceinline__ void vtile_fetch(const LevelView& L, int32_t X0, int32_t Y0, VRegs<T>& g, bool load_e,
                                             bool load_dg) {
+    using G = VGeom<T>;
     const int64_t w = L.w, h = L.h, np = (int64_t)L.cells + 2 * w;
     g.inb = g.real = 0u;
 #pragma unroll
-    for (int k = 0; k < VRegs::kPer; ++k) {
+    for (int k = 0; k < G::kPer; ++k) {
         const int i = (int)threadIdx.x + 256 * k;
-        const int32_t x = X0 - kVH + i % kVR, y = Y0 - kVH + i / kVR;
+        const int32_t x = X0 - kVH + i % G::kR, y = Y0 - kVH + i / G::kR;
         const int64_t p = (int64_t)(y + 1) * w + x;
-        const bool inb = i < kVN && p >= 0 && p < np;
-        const bool real = i < kVN && x >= 0 && x < w && y >= 0 && y < h;
+        const bool inb = i < G::kN && p >= 0 && p < np;
+        const bool real = i < G::kN && x >= 0 && x < w && y >= 0 && y < h;
         g.inb |= (uint32_t)inb << k;
         g.real |= (uint32_t)real << k;
         const int64_t q = inb ? p : 0;
@@ -1161,11 +1182,13 @@ __device__ __forceinline__ void vtile_fetch(const LevelView& L, int32_t X0, int3
     }
 }
 
-__device__ __forceinline__ void vtile_put(const VRegs& g, VTile& t) {
+template <int T>
+__device__ __forceinline__ void vtile_put(const VRegs<T>& g, VTile<T>& t) {
+    using G = VGeom<T>;
 #pragma unroll
-    for (int k = 0; k < VRegs::kPer; ++k) {
+    for (int k = 0; k < G::kPer; ++k) {
         const int i = (int)threadIdx.x + 256 * k;
-        if (i >= kVN) continue;
+        if (i >= G::kN) continue;
         const bool inb = (g.inb >> k) & 1u, real = (g.real >> k) & 1u;
         t.r[i] = inb ? g.r[k] : 0.0f;
         t.id[i] = inb ? g.id[k] : 0.0f;
@@ -1178,14 +1201,15 @@ __device__ __forceinline__ void vtile_put(const VRegs& g, VTile& t) {
 }
 
 // SOR update of cell i from its own value and four neighbour values (sor_cell).
-__device__ inline float vsor(const VTile& t, int i, float own, float er, float et, float el, float eb) {
+template <int T>
+__device__ inline float vsor(const VTile<T>& t, int i, float own, float er, float et, float el, float eb) {
     const float id = t.id[i];
     if (id != 0.0f) {
         float sum = t.r[i];
         sum = fmaf(-t.px[i], er, sum);
         sum = fmaf(-t.py[i], et, sum);
         sum = fmaf(-t.px[i - 1], el, sum);
-        sum = fmaf(-t.py[i - kVR], eb, sum);
+        sum = fmaf(-t.py[i - VGeom<T>::kR], eb, sum);
         const float gs = sum * id;
         own = own + (kOmega * (gs - own));
     }
@@ -1197,41 +1221,44 @@ __device__ inline float vsor(const VTile& t, int i, float own, float er, float e
 // the base level, in a PCG iteration, every workgroup first applies the
 // residual test of the previous update (max |r| < solve_error,
 // src/apic.rs:1930, or the relative test).
+template <int T>
 __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, int base, int fuse_red, int spec,
                              const int32_t* __restrict__ done, uint32_t tiles_x, PcgState* st, int check,
                              float solve_error, const float* __restrict__ part_max, const double* __restrict__ part_rr,
                              uint32_t nred, uint32_t wg) {
-    __shared__ VTile t;
+    using G = VGeom<T>;
+    constexpr int kR = G::kR;
+    __shared__ VTile<T> t;
     __shared__ float scratch[256 / 64];
     __shared__ double dscratch[256 / 64];
     const bool lead = wg == 0 && threadIdx.x == 0;
-    const int32_t X0 = (int32_t)(wg % tiles_x) * kVT, Y0 = (int32_t)(wg / tiles_x) * kVT;
+    const int32_t X0 = (int32_t)(wg % tiles_x) * T, Y0 = (int32_t)(wg / tiles_x) * T;
     const int32_t w = (int32_t)F.w, h = (int32_t)F.h;
     const int32_t cw = (int32_t)C.w, ch = (int32_t)C.h;
     // Loads that do not depend on each other go out together, so that the
-    // workgroup waits for them once: the done flag, the residual test's
-    // partials (nred <= kRedBlocks), the coarse cells' flags and inverse
-    // diagonals and, with spec, the region itself before it is known whether
-    // the tile has work (a coarse level has few tiles: its time is the latency
-    // of these loads, not their bytes).
+    // workgroup waits for them once: the done flag, the coarse cells' flags
+    // and inverse diagonals, the residual test's partials (nred <=
+    // kRedBlocks) and, with spec, the region itself before it is known
+    // whether the tile has work (a coarse level has few tiles: its time is the
+    // latency of these loads, not their bytes).
     const int32_t dn = *done;
-    VRegs g;
-    if (spec) vtile_fetch(F, X0, Y0, g, !base, true);
-    constexpr int kCPer = (kVT / 2) * (kVT / 2) / 256;
-    uint8_t cfl[kCPer];
-    float cid[kCPer];
-    bool cin[kCPer];
+    VRegs<T> g;
+    if (spec) vtile_fetch<T>(F, X0, Y0, g, !base, true);
+    uint8_t cfl[G::kCPer];
+    float cid[G::kCPer];
+    bool cin[G::kCPer];
 #pragma unroll
-    for (int q = 0; q < kCPer; ++q) {
+    for (int q = 0; q < G::kCPer; ++q) {
         const int k = (int)threadIdx.x + 256 * q;
-        const int32_t cx = X0 / 2 + k % (kVT / 2), cy = Y0 / 2 + k / (kVT / 2);
-        const bool in = cx < cw && cy < ch;  // (the restriction below skips the others)
+        const int32_t cx = X0 / 2 + k % (T / 2), cy = Y0 / 2 + k / (T / 2);
+        const bool in = k < G::kCN && cx < cw && cy < ch;  // (the restriction below skips the others)
         const size_t c = in ? (size_t)cy * cw + cx : 0;
         cfl[q] = C.fluid[c];
         cid[q] = C.inv_diag[c + cw];
         cin[q] = in;
     }
     constexpr int kRPer = kRedBlocks / 256;
+    static_assert(kRedBlocks % 256 == 0, "partials per thread");
     double prr[kRPer];
     float pmx[kRPer];
 #pragma unroll
@@ -1242,7 +1269,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         if (check && part_rr) prr[q] = part_rr[ic];
         else if (check) pmx[q] = part_max[ic];
     }
-    if (spec) vtile_put(g, t);  // read after the barrier of the tile test below
+    if (spec) vtile_put<T>(g, t);  // read after the barrier of the tile test below
     if (dn) return;
     if (check && part_rr) {
         double rr = 0.0;
@@ -1272,43 +1299,45 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     // times zero coefficients, so it leaves them as they are.
     bool any = false;
 #pragma unroll
-    for (int q = 0; q < kCPer; ++q) any = any || (cin[q] && cfl[q] != 0);
+    for (int q = 0; q < G::kCPer; ++q) any = any || (cin[q] && cfl[q] != 0);
     if (!__syncthreads_or(any)) return;
     if (!spec) {
-        vtile_fetch(F, X0, Y0, g, !base, true);
-        vtile_put(g, t);
+        vtile_fetch<T>(F, X0, Y0, g, !base, true);
+        vtile_put<T>(g, t);
         __syncthreads();
     }
     if (base) {  // red_zero_cell over the region
-        for (int i = threadIdx.x; i < kVN; i += blockDim.x) {
-            const int32_t x = X0 - kVH + i % kVR, y = Y0 - kVH + i / kVR;
+        for (int i = threadIdx.x; i < G::kN; i += blockDim.x) {
+            const int32_t x = X0 - kVH + i % kR, y = Y0 - kVH + i / kR;
             const bool red = ((x + y) & 1) == 0;
             t.e[i] = (t.fl[i] && red && t.id[i] != 0.0f) ? kOmega * (t.r[i] * t.id[i]) : 0.0f;
         }
         __syncthreads();
     }
     // black cells of the tile and its one-cell ring (they read red cells only)
-    for (int i = threadIdx.x; i < kVN; i += blockDim.x) {
-        const int lx = i % kVR, ly = i / kVR;
-        if (lx < 1 || lx >= kVR - 1 || ly < 1 || ly >= kVR - 1) continue;
+    for (int i = threadIdx.x; i < G::kN; i += blockDim.x) {
+        const int lx = i % kR, ly = i / kR;
+        if (lx < 1 || lx >= kR - 1 || ly < 1 || ly >= kR - 1) continue;
         const int32_t x = X0 - kVH + lx, y = Y0 - kVH + ly;
         if (!t.fl[i] || ((x + y) & 1) == 0) continue;
-        t.e[i] = vsor(t, i, t.e[i], t.e[i + 1], t.e[i + kVR], t.e[i - 1], t.e[i - kVR]);
+        t.e[i] = vsor<T>(t, i, t.e[i], t.e[i + 1], t.e[i + kR], t.e[i - 1], t.e[i - kR]);
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < kVT * kVT; i += blockDim.x) {
-        const int32_t x = X0 + i % kVT, y = Y0 + i / kVT;
-        if (x >= w || y >= h) continue;
+#pragma unroll
+    for (int q = 0; q < G::kTPer; ++q) {
+        const int i = (int)threadIdx.x + 256 * q;
+        const int32_t x = X0 + i % T, y = Y0 + i / T;
+        if (i >= T * T || x >= w || y >= h) continue;
         if (!base && ((x + y) & 1) == 0) continue;  // red values are already in memory
-        const int li = (y - Y0 + kVH) * kVR + (x - X0 + kVH);
+        const int li = (y - Y0 + kVH) * kR + (x - X0 + kVH);
         F.e[(size_t)(y + 1) * w + x] = t.e[li];
     }
     // restriction into the tile's coarse cells (restrict_cell)
 #pragma unroll
-    for (int q = 0; q < kCPer; ++q) {
+    for (int q = 0; q < G::kCPer; ++q) {
         const int k = (int)threadIdx.x + 256 * q;
-        const int32_t cx = X0 / 2 + k % (kVT / 2), cy = Y0 / 2 + k / (kVT / 2);
-        if (cx >= cw || cy >= ch) continue;
+        const int32_t cx = X0 / 2 + k % (T / 2), cy = Y0 / 2 + k / (T / 2);
+        if (!cin[q]) continue;
         const size_t c = (size_t)cy * cw + cx, pcc = c + cw;
         if (!cfl[q]) { C.r[pcc] = 0.0f; C.e[pcc] = 0.0f; continue; }
         float sum = 0.0f;
@@ -1316,14 +1345,14 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
-                const int li = (2 * cy + dy - Y0 + kVH) * kVR + (2 * cx + dx - X0 + kVH);
+                const int li = (2 * cy + dy - Y0 + kVH) * kR + (2 * cx + dx - X0 + kVH);
                 if (t.fl[li]) {
                     const float b = t.r[li];
                     float ax = t.dg[li] * t.e[li];
                     ax += t.px[li] * t.e[li + 1];
-                    ax += t.py[li] * t.e[li + kVR];
+                    ax += t.py[li] * t.e[li + kR];
                     ax += t.px[li - 1] * t.e[li - 1];
-                    ax += t.py[li - kVR] * t.e[li - kVR];
+                    ax += t.py[li - kR] * t.e[li - kR];
                     sum += b - ax;
                 }
             }
@@ -1336,28 +1365,31 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
 
 // One level on the way down.  On the base level of a PCG iteration (check)
 // the residual test reads the nred partials of the previous update.
+struct VDownArgs {
+    LevelView F, C;
+    int base, fuse_red;
+    int spec;                // load the region with the flags (levels of few tiles)
+    const int32_t* done;
+    uint32_t tiles_x;
+    PcgState* st;
+    int check;
+    float solve_error;
+    const float* part_max;
+    const double* part_rr;   // relative mode (else null): ||r||^2 partials of the update
+    const double* part_rr2;  // ... and of the true residual, on the iterations that recompute it
+    int every;               // the true residual is recomputed after iteration k when (k + 1) % every == 0
+    uint32_t nred;
+};
+
+template <int T>
 struct VDown {
     static constexpr int kThreads = 256;
-    struct Args {
-        LevelView F, C;
-        int base, fuse_red;
-        int spec;                // load the region with the flags (levels of few tiles)
-        const int32_t* done;
-        uint32_t tiles_x;
-        PcgState* st;
-        int check;
-        float solve_error;
-        const float* part_max;
-        const double* part_rr;   // relative mode (else null): ||r||^2 partials of the update
-        const double* part_rr2;  // ... and of the true residual, on the iterations that recompute it
-        int every;               // the true residual is recomputed after iteration k when (k + 1) % every == 0
-        uint32_t nred;
-    };
+    using Args = VDownArgs;
     // k: the PCG iteration whose update the residual test checks
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         const bool due = a.every > 0 && (k + 1) % a.every == 0;
-        vdown(a.F, a.C, a.base, a.fuse_red, a.spec, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max,
-              due ? a.part_rr2 : a.part_rr, a.nred, wg);
+        vdown<T>(a.F, a.C, a.base, a.fuse_red, a.spec, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max,
+                 due ? a.part_rr2 : a.part_rr, a.nred, wg);
     }
 };
 
@@ -1365,58 +1397,59 @@ struct VDown {
 // C's error prolongated on the fly (prolongate + sor_cell modes 1 and 2); on
 // the base level, with the padded-list repeats of prolongate and the partials
 // of r.z over the tiles a workgroup takes (dot: 1 FP32, 2 FP64).
+template <int T>
 __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, const FluidStats* __restrict__ fs,
                            const int32_t* __restrict__ done, uint32_t tiles_x, int dot, float* __restrict__ part_gamma,
                            double* __restrict__ part_gamma_d, const float* __restrict__ As,
                            double* __restrict__ part_asz_d, int spec, uint32_t wg) {
-    __shared__ VTile t;
-    __shared__ float ce[kVC * kVC];
+    using G = VGeom<T>;
+    constexpr int kR = G::kR, kC = G::kC;
+    __shared__ VTile<T> t;
+    __shared__ float ce[kC * kC];
     __shared__ float scratch[256 / 64];
     __shared__ double dscratch[256 / 64];
     const int32_t w = (int32_t)L.w, h = (int32_t)L.h, cw = (int32_t)C.w, chh = (int32_t)C.h;
-    const int32_t X0 = (int32_t)(wg % tiles_x) * kVT, Y0 = (int32_t)(wg / tiles_x) * kVT;
+    const int32_t X0 = (int32_t)(wg % tiles_x) * T, Y0 = (int32_t)(wg / tiles_x) * T;
     const int32_t CX0 = X0 / 2 - 1, CY0 = Y0 / 2 - 1;
     // Loads that do not depend on each other go out together (as in vdown):
-    // the done flag, the tile's coarse flags, the parent's error over the
-    // region, the fluid statistics and, with spec, the region itself.
+    // the done flag, the region (with spec) and A s, the parent's error over
+    // the region, the tile's coarse flags and the fluid statistics.
     const int32_t dn = *done;
     // A s of the thread's tile cells (the partials of (A s).z)
-    constexpr int kTPer = kVT * kVT / 256;
-    float asv[kTPer];
+    float asv[G::kTPer];
     auto fetch_as = [&]() {
 #pragma unroll
-        for (int q = 0; q < kTPer; ++q) {
+        for (int q = 0; q < G::kTPer; ++q) {
             const int i = (int)threadIdx.x + 256 * q;
-            const int32_t x = X0 + i % kVT, y = Y0 + i / kVT;
-            const bool in = x < w && y < h;  // (the sweep below skips the others)
+            const int32_t x = X0 + i % T, y = Y0 + i / T;
+            const bool in = i < T * T && x < w && y < h;  // (the sweep below skips the others)
             asv[q] = dot == 2 ? As[in ? (size_t)(y + 1) * w + x : 0] : 0.0f;
         }
     };
-    VRegs g;
+    VRegs<T> g;
     if (spec) {
-        vtile_fetch(L, X0, Y0, g, true, false);
+        vtile_fetch<T>(L, X0, Y0, g, true, false);
         fetch_as();
     }
-    const int32_t ccx = X0 / 2 + (int32_t)(threadIdx.x % (kVT / 2)), ccy = Y0 / 2 + (int32_t)(threadIdx.x / (kVT / 2));
-    constexpr int kEPer = (kVC * kVC + 255) / 256;
-    float cev[kEPer];
+    float cev[G::kEPer];
     uint32_t cin_mask = 0u;
 #pragma unroll
-    for (int q = 0; q < kEPer; ++q) {
+    for (int q = 0; q < G::kEPer; ++q) {
         const int j = (int)threadIdx.x + 256 * q;
-        const int32_t cx = CX0 + j % kVC, cy = CY0 + j / kVC;
-        const bool in = j < kVC * kVC && cx >= 0 && cx < cw && cy >= 0 && cy < chh;
+        const int32_t cx = CX0 + j % kC, cy = CY0 + j / kC;
+        const bool in = j < kC * kC && cx >= 0 && cx < cw && cy >= 0 && cy < chh;
         cin_mask |= (uint32_t)in << q;
         cev[q] = C.e[in ? (size_t)(cy + 1) * cw + cx : 0];
     }
-    const bool cin = threadIdx.x < (kVT / 2) * (kVT / 2) && ccx < cw && ccy < chh;
+    const int32_t ccx = X0 / 2 + (int32_t)(threadIdx.x % (T / 2)), ccy = Y0 / 2 + (int32_t)(threadIdx.x / (T / 2));
+    const bool cin = threadIdx.x < G::kCN && ccx < cw && ccy < chh;
     const uint8_t cfv = C.fluid[cin ? (size_t)ccy * cw + ccx : 0];
     const uint32_t last1 = fs ? fs->last_raw : 0u, cnt = fs ? fs->count : 0u;
-    if (spec) vtile_put(g, t);  // read after the barrier of the tile test below
+    if (spec) vtile_put<T>(g, t);  // read after the barrier of the tile test below
 #pragma unroll
-    for (int q = 0; q < kEPer; ++q) {
+    for (int q = 0; q < G::kEPer; ++q) {
         const int j = (int)threadIdx.x + 256 * q;
-        if (j < kVC * kVC) ce[j] = ((cin_mask >> q) & 1u) ? cev[q] : 0.0f;
+        if (j < kC * kC) ce[j] = ((cin_mask >> q) & 1u) ? cev[q] : 0.0f;
     }
     if (dn) return;
     // a tile without a fluid cell (see vdown) only reports a zero partial
@@ -1428,56 +1461,57 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         return;
     }
     if (!spec) {
-        vtile_fetch(L, X0, Y0, g, true, false);
+        vtile_fetch<T>(L, X0, Y0, g, true, false);
         fetch_as();
-        vtile_put(g, t);
+        vtile_put<T>(g, t);
         __syncthreads();
     }
-    float prod = 0.0f;
-    double dprod = 0.0, dasz = 0.0;
-    {
-        // e plus the parent's error (prolonged); non-fluid cells keep e
-        auto pv = [&](int i) -> float {
-            float v = t.e[i];
-            if (!t.fl[i]) return v;
-            const int32_t x = X0 - kVH + i % kVR, y = Y0 - kVH + i / kVR;
-            const float pc = ce[((y >> 1) - CY0) * kVC + ((x >> 1) - CX0)];
+    // e plus the parent's error (prolonged) of every region cell, once, into
+    // the diagonal's slot (not read on the way up); non-fluid cells keep e
+    for (int i = threadIdx.x; i < G::kN; i += blockDim.x) {
+        float v = t.e[i];
+        if (t.fl[i]) {
+            const int32_t x = X0 - kVH + i % kR, y = Y0 - kVH + i / kR;
+            const float pc = ce[((y >> 1) - CY0) * kC + ((x >> 1) - CX0)];
             int reps = 1;
             if (fs) {
                 const uint32_t raw = (uint32_t)(y * w + x);
                 if (last1 != 0 && raw == last1 - 1 && (cnt & 7u) != 0) reps = 1 + (int)(8u - (cnt & 7u));
             }
             for (int k = 0; k < reps; ++k) v = v + pc;
-            return v;
-        };
-        // first colour (black): own and neighbours prolongated
-        for (int i = threadIdx.x; i < kVN; i += blockDim.x) {
-            const int lx = i % kVR, ly = i / kVR;
-            if (lx < 1 || lx >= kVR - 1 || ly < 1 || ly >= kVR - 1) continue;
-            const int32_t x = X0 - kVH + lx, y = Y0 - kVH + ly;
-            if (!t.fl[i] || ((x + y) & 1) == 0) continue;
-            const float own = pv(i);
-            t.e[i] = vsor(t, i, own, pv(i + 1), pv(i + kVR), pv(i - 1), pv(i - kVR));
         }
-        __syncthreads();
-        // second colour (red) of the tile: own prolongated, neighbours final
+        t.dg[i] = v;
+    }
+    __syncthreads();
+    // first colour (black): own and neighbours prolongated
+    for (int i = threadIdx.x; i < G::kN; i += blockDim.x) {
+        const int lx = i % kR, ly = i / kR;
+        if (lx < 1 || lx >= kR - 1 || ly < 1 || ly >= kR - 1) continue;
+        const int32_t x = X0 - kVH + lx, y = Y0 - kVH + ly;
+        if (!t.fl[i] || ((x + y) & 1) == 0) continue;
+        t.e[i] = vsor<T>(t, i, t.dg[i], t.dg[i + 1], t.dg[i + kR], t.dg[i - 1], t.dg[i - kR]);
+    }
+    __syncthreads();
+    // second colour (red) of the tile: own prolongated (the first colour left
+    // red cells as they were), neighbours final
+    float prod = 0.0f;
+    double dprod = 0.0, dasz = 0.0;
 #pragma unroll
-        for (int q = 0; q < kTPer; ++q) {
-            const int i = (int)threadIdx.x + 256 * q;
-            const int32_t x = X0 + i % kVT, y = Y0 + i / kVT;
-            if (x >= w || y >= h) continue;
-            const int li = (y - Y0 + kVH) * kVR + (x - X0 + kVH);
-            float e = t.e[li];
-            if (t.fl[li] && ((x + y) & 1) == 0)
-                e = vsor(t, li, pv(li), t.e[li + 1], t.e[li + kVR], t.e[li - 1], t.e[li - kVR]);
-            const size_t pc = (size_t)(y + 1) * w + x;
-            L.e[pc] = e;
-            if (dot == 2) {
-                dprod += (double)t.r[li] * (double)e;
-                if (t.fl[li]) dasz += (double)asv[q] * (double)e;
-            } else if (dot == 1) {
-                prod += t.r[li] * e;
-            }
+    for (int q = 0; q < G::kTPer; ++q) {
+        const int i = (int)threadIdx.x + 256 * q;
+        const int32_t x = X0 + i % T, y = Y0 + i / T;
+        if (i >= T * T || x >= w || y >= h) continue;
+        const int li = (y - Y0 + kVH) * kR + (x - X0 + kVH);
+        float e = t.e[li];
+        if (t.fl[li] && ((x + y) & 1) == 0)
+            e = vsor<T>(t, li, t.dg[li], t.e[li + 1], t.e[li + kR], t.e[li - 1], t.e[li - kR]);
+        const size_t pc = (size_t)(y + 1) * w + x;
+        L.e[pc] = e;
+        if (dot == 2) {
+            dprod += (double)t.r[li] * (double)e;
+            if (t.fl[li]) dasz += (double)asv[q] * (double)e;
+        } else if (dot == 1) {
+            prod += t.r[li] * e;
         }
     }
     if (dot == 2) {
@@ -1493,23 +1527,26 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     }
 }
 
+struct VUpArgs {
+    LevelView L, C;
+    const FluidStats* fs;  // base level only
+    const int32_t* done;
+    uint32_t tiles_x;
+    int dot;
+    float* part_gamma;
+    double* part_gamma_d;
+    const float* As;      // relative mode: A s of the last iteration and
+    double* part_asz_d;   // the partials of (A s).z
+    int spec;             // load the region with the flags (levels of few tiles)
+};
+
+template <int T>
 struct VUp {
     static constexpr int kThreads = 256;
-    struct Args {
-        LevelView L, C;
-        const FluidStats* fs;  // base level only
-        const int32_t* done;
-        uint32_t tiles_x;
-        int dot;
-        float* part_gamma;
-        double* part_gamma_d;
-        const float* As;      // relative mode: A s of the last iteration and
-        double* part_asz_d;   // the partials of (A s).z
-        int spec;             // load the region with the flags (levels of few tiles)
-    };
+    using Args = VUpArgs;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        vup(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, a.spec,
-            wg);
+        vup<T>(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, a.spec,
+               wg);
     }
 };
 
@@ -1562,7 +1599,7 @@ void launch_mg_build(Sim& sim) {
 // The base level's up-sweep writes one partial of r.z per tile.
 size_t gamma_partials(const Sim& sim) {
     const LevelView& B = sim.levels[0];
-    return (size_t)((B.w + kVT - 1) / kVT) * ((B.h + kVT - 1) / kVT);
+    return (size_t)((B.w + kBaseTile - 1) / kBaseTile) * ((B.h + kBaseTile - 1) / kBaseTile);
 }
 
 int pcg_cap(const Sim& sim) { return sim.solver.mode == 1 ? sim.solver.max_iterations : kMaxPcgIterations; }
@@ -1591,20 +1628,30 @@ void launch_vcycle(Sim& sim, int mode, int k) {
     const int last = (int)sim.levels.size() - 1;
     const int bottom = (int)sim.bottom_level;  // 1 <= bottom <= last
     const bool rel = sim.solver.mode == 1 && mode != 0;
-    auto tiles = [](const LevelView& L, uint32_t& tx) {
-        tx = (L.w + kVT - 1) / kVT;
-        return tx * ((L.h + kVT - 1) / kVT);
+    auto tiles = [](const LevelView& L, uint32_t side, uint32_t& tx) {
+        tx = (L.w + side - 1) / side;
+        return tx * ((L.h + side - 1) / side);
+    };
+    // 16 x 16 tiles on a level above the base with at most kFewTiles 32 x 32
+    // ones (the base keeps one partial of r.z per 32 x 32 tile)
+    constexpr uint32_t kFewTiles = 1024;
+    auto small = [&](int l) {
+        uint32_t tx;
+        return l > 0 && tiles(sim.levels[l], kBaseTile, tx) <= kFewTiles;
     };
     for (int l = 0; l < bottom; ++l) {
         const LevelView& F = sim.levels[l];
         const int fuse_red = l + 1 < bottom ? 1 : (int)(bottom < last);
+        const bool sm = small(l);
         uint32_t tx;
-        const uint32_t nt = tiles(F, tx);
+        const uint32_t nt = tiles(F, sm ? 16 : kBaseTile, tx);
         const bool rel_check = l == 0 && rel;
-        VDown::Args a{F, sim.levels[l + 1], (int)(l == 0), fuse_red, (int)(l > 0), done, tx, sim.pcg, (int)(l == 0 && mode == 2),
-                      sim.props.solve_error, sim.part_max, rel_check ? sim.part_rr_d : nullptr,
-                      rel_check ? sim.part_rr2_d : nullptr, rel_check ? sim.solver.recompute_every : 0, sim.nred};
-        emit<VDown>(sim, a, nt, k);
+        VDownArgs a{F, sim.levels[l + 1], (int)(l == 0), fuse_red, (int)(l > 0), done, tx, sim.pcg,
+                          (int)(l == 0 && mode == 2), sim.props.solve_error, sim.part_max,
+                          rel_check ? sim.part_rr_d : nullptr, rel_check ? sim.part_rr2_d : nullptr,
+                          rel_check ? sim.solver.recompute_every : 0, sim.nred};
+        if (sm) emit<VDown<16>>(sim, a, nt, k);
+        else emit<VDown<kBaseTile>>(sim, a, nt, k);
     }
     if (sim.bottom_in_lds)
         emit<BottomLds>(sim, {level_set(sim), bottom, done}, 1);
@@ -1612,12 +1659,14 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         emit<BottomGlobal>(sim, {level_set(sim), bottom, sim.cg_s, sim.cg_ap, sim.cg_r, done}, 1);
     for (int l = bottom - 1; l >= 0; --l) {
         const LevelView& L = sim.levels[l];
+        const bool sm = small(l);
         uint32_t tx;
-        const uint32_t nt = tiles(L, tx);
+        const uint32_t nt = tiles(L, sm ? 16 : kBaseTile, tx);
         const int dot = (l == 0 && mode != 0) ? (rel ? 2 : 1) : 0;  // one partial of r.z per tile
-        VUp::Args a{L, sim.levels[l + 1], l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot,
-                    sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d, (int)(l > 0)};
-        emit<VUp>(sim, a, nt);
+        VUpArgs a{L, sim.levels[l + 1], l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot,
+                        sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d, (int)(l > 0)};
+        if (sm) emit<VUp<16>>(sim, a, nt);
+        else emit<VUp<kBaseTile>>(sim, a, nt);
     }
 }
 
