@@ -192,16 +192,23 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
         s.levels.push_back(L);
     }
     if (s.levels.size() > 16) return fail(APIC_ERR_UNSUPPORTED_SIZE, "more than 15 multigrid levels");
-    // The one-workgroup bottom solver takes the coarse levels that fit in LDS;
-    // when even the coarsest does not, it works in global memory on every
-    // level with <= 4096 cells.
+    // The one-workgroup bottom solver takes the coarse levels of at most
+    // kBottomFirstCells cells that fit in LDS (a finer level runs faster as
+    // tiles over several CUs); when even the coarsest does not fit, it works
+    // in global memory on every level with <= 4096 cells.
     s.bottom_level = s.levels.size() > 1 ? s.levels.size() - 1 : 0;
     s.bottom_in_lds = false;
     for (size_t l = 1; l < s.levels.size(); ++l)
-        if (bottom_fits_lds(s.levels, l)) { s.bottom_level = l; s.bottom_in_lds = true; break; }
+        if (s.levels[l].cells <= kBottomFirstCells && bottom_fits_lds(s.levels, l)) {
+            s.bottom_level = l;
+            s.bottom_in_lds = true;
+            break;
+        }
     if (!s.bottom_in_lds)
         for (size_t l = 1; l < s.levels.size(); ++l)
             if (s.levels[l].cells <= 4096) { s.bottom_level = l; break; }
+    s.down_e.assign(s.bottom_level, nullptr);
+    for (size_t l = 0; l < s.bottom_level; ++l) ALLOC(s.down_e[l], s.padded(s.levels[l].w, s.levels[l].h));
     const LevelView& B = s.levels[0];
     size_t pb = s.padded(B.w, B.h);
     ALLOC(s.search[0], pb); ALLOC(s.search[1], pb); ALLOC(s.As, pb); ALLOC(s.rhs, pb);

@@ -42,6 +42,8 @@ struct LevelView {
 constexpr size_t kBottomLdsFloats = 34816;
 constexpr size_t kBottomLdsFlags = 8192;
 __host__ __device__ inline size_t bottom_padded(const LevelView& L) { return (size_t)L.cells + 2 * (size_t)L.w; }
+constexpr uint32_t kBottomFirstCells = 1024;  // cells of the one-workgroup bottom's finest level
+
 inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first) {
     size_t fl = 0, fb = 0;
     for (size_t l = first; l < lv.size(); ++l) { fl += 6 * bottom_padded(lv[l]); fb += lv[l].cells; }
@@ -183,6 +185,10 @@ struct Sim {
     // solver
     std::vector<LevelView> levels;  // [0] = base grid, [k] = multigrid_levels[k-1]
     size_t bottom_level = 0;        // first level solved by the one-workgroup kernel
+    // the V-cycle's down-sweep errors of the levels above the bottom (the
+    // up-sweep reads them and writes the level's e, so that no launch reads
+    // a tile ring that another workgroup of the same launch writes)
+    std::vector<float*> down_e;
     bool bottom_in_lds = false;     // that kernel holds its levels in LDS
     float* search[2] = {nullptr, nullptr};
     float* As = nullptr;

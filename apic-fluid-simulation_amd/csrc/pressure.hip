@@ -408,13 +408,6 @@ __device__ inline float block_max(float v, float* scratch) {
     return s;
 }
 
-// Sum of the n partials of one reduction.  Every workgroup that calls it uses
-// kThreads threads and the same order, so all of them see the same value.
-__device__ inline float sum_partials(const float* __restrict__ part, float* scratch, int n) {
-    float v = 0.0f;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
-    return block_sum(v, scratch);
-}
 
 // ---- the one-workgroup bottom of the V-cycle
 
@@ -899,107 +892,197 @@ __device__ inline bool gamma_exit(PcgState* st, float g, int k, bool lead) {
     return false;
 }
 
+// SpmvDot and AxpyMax take their cells in batches: a thread first loads the
+// fluid flags of a batch of its cells (with the reduction partials it needs),
+// then every operand of the batch's fluid cells at once (a non-fluid cell
+// reads the row's first cell, a line every lane of the wave shares), so that
+// a workgroup waits twice per batch instead of twice per cell.  With FP64
+// reductions they run kWideThreads threads per workgroup (nred workgroups:
+// about two cells per thread); with the FP32 reductions of the reference's
+// rules 256, which keeps each partial's cells and summation order.
+constexpr int kWideThreads = 1024;
+constexpr int kWideBatch = 2;
+
+// The n partials of one reduction summed in the order of a 256-thread
+// workgroup (thread t < 256 adds part[t], part[t + 256], ... in turn; the
+// four wave sums are added in wave order), whatever the size of the calling
+// workgroup: every kernel that reduces the same partials obtains the same
+// value.  Four loads per thread are in flight at a time.
+template <typename T>
+__device__ inline T sum_partials256(const T* __restrict__ part, int n, T* scratch) {
+    T v = 0;
+    if (threadIdx.x < 256) {
+        for (int i0 = (int)threadIdx.x; i0 < n; i0 += 1024) {
+            T x[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int i = i0 + 256 * j;
+                x[j] = part[i < n ? i : 0];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (i0 + 256 * j < n) v += x[j];
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && threadIdx.x < 256) scratch[threadIdx.x >> 6] = v;
+    __syncthreads();
+    T s = scratch[0];
+    for (int w = 1; w < 4; ++w) s += scratch[w];
+    return s;
+}
+
 // Iteration k: gamma from the V-cycle's partials, s = z + beta*s (s = z when
 // k = 0), As = A s and the partials of delta = s.As (apply_matrix_and_dot,
 // src/apic.rs:1823-1863; search update :1945-1954).
+template <int NT>
 __device__ __forceinline__ void spmv_dot(const LevelView& L, const float* __restrict__ s_old, float* __restrict__ s_new,
                                 float* __restrict__ As, PcgState* st, const float* __restrict__ part_gamma,
                                 float* __restrict__ part_delta, int k, const double* __restrict__ part_gamma_d,
                                 const double* __restrict__ part_asz_d, double* __restrict__ part_delta_d, int ngam,
                                 uint32_t wg, uint32_t nwg) {
     const bool lead = wg == 0 && threadIdx.x == 0;
-    __shared__ float scratch[kThreads / 64];
-    __shared__ double dscratch[kThreads / 64];
+    __shared__ float scratch[NT / 64];
+    __shared__ double dscratch[NT / 64];
     if (st->done) return;
     const bool first = k == 0;
+    const size_t w = L.w;
+    const uint32_t cells = L.cells, stride = nwg * NT;
+    uint32_t c0 = wg * NT + threadIdx.x;
+    // the first batch's flags go out with the partials
+    uint8_t fl[kWideBatch];
+    auto fetch_flags = [&]() {
+#pragma unroll
+        for (int j = 0; j < kWideBatch; ++j) {
+            const uint32_t c = c0 + j * stride;
+            fl[j] = c < cells ? L.fluid[c < cells ? c : 0] : 0;
+        }
+    };
+    fetch_flags();
     float beta;
     if (part_gamma_d) {  // relative mode: FP64 scalars, flexible beta
-        double g = 0.0, asz = 0.0;
-        for (int i = threadIdx.x; i < ngam; i += blockDim.x) {
-            g += part_gamma_d[i];
-            asz += part_asz_d[i];
-        }
-        g = block_sum(g, dscratch);
+        const double g = sum_partials256(part_gamma_d, ngam, dscratch);
         if (gamma_exit_rel(st, g, k, lead)) return;
         // Polak-Ribiere: r_{k}.(z_{k} - z_{k-1}) / r_{k-1}.z_{k-1}, where
         // r_{k-1}.z_k = r_k.z_k + alpha_{k-1} (A s_{k-1}).z_k
         if (!first) {
-            asz = block_sum(asz, dscratch);
+            const double asz = sum_partials256(part_asz_d, ngam, dscratch);
             beta = (float)(-st->alpha_d * asz / st->gamma_d[(k - 1) & 1]);
         } else {
             beta = 0.0f;
         }
         if (lead) { st->gamma_d[k & 1] = g; st->iters = k + 1; }
     } else {
-        const float g = sum_partials(part_gamma, scratch, ngam);
+        const float g = sum_partials256(part_gamma, ngam, scratch);
         if (gamma_exit(st, g, k, lead)) return;
         beta = first ? 0.0f : g / st->gamma[k - 1];
         if (lead) { st->gamma[k] = g; st->iters = k + 1; }
     }
-    const size_t w = L.w;
     float prod = 0.0f;
     double dprod = 0.0;
-    for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
-        if (!L.fluid[c]) continue;
-        size_t pc = (size_t)c + w;
-        auto sv = [&](size_t i) { return first ? L.e[i] : L.e[i] + (beta * s_old[i]); };
-        float s0 = sv(pc);
-        float v = L.diag[pc] * s0;
-        v = fmaf(L.px[pc], sv(pc + 1), v);
-        v = fmaf(L.px[pc - 1], sv(pc - 1), v);
-        v = fmaf(L.py[pc], sv(pc + w), v);
-        v = fmaf(L.py[pc - w], sv(pc - w), v);
-        s_new[pc] = s0;
-        As[pc] = v;
-        if (part_delta_d) dprod += (double)s0 * (double)v;
-        else prod += v * s0;
+    for (; c0 < cells; c0 += kWideBatch * stride) {
+        // operands of the batch's cells: own, right, left, top, bottom
+        float e[kWideBatch][5], so[kWideBatch][5], dg[kWideBatch], px[kWideBatch][2], py[kWideBatch][2];
+#pragma unroll
+        for (int j = 0; j < kWideBatch; ++j) {
+            const size_t pc = fl[j] ? (size_t)(c0 + j * stride) + w : w;
+            const size_t nb[5] = {pc, pc + 1, pc - 1, pc + w, pc - w};
+#pragma unroll
+            for (int d = 0; d < 5; ++d) {
+                e[j][d] = L.e[nb[d]];
+                so[j][d] = first ? 0.0f : s_old[nb[d]];
+            }
+            dg[j] = L.diag[pc];
+            px[j][0] = L.px[pc]; px[j][1] = L.px[pc - 1];
+            py[j][0] = L.py[pc]; py[j][1] = L.py[pc - w];
+        }
+        uint8_t fl_now[kWideBatch];
+#pragma unroll
+        for (int j = 0; j < kWideBatch; ++j) fl_now[j] = fl[j];
+        // the next batch's flags, in flight during this batch's arithmetic
+        c0 += kWideBatch * stride;
+        fetch_flags();
+        c0 -= kWideBatch * stride;
+#pragma unroll
+        for (int j = 0; j < kWideBatch; ++j) {
+            if (!fl_now[j]) continue;
+            const size_t pc = (size_t)(c0 + j * stride) + w;
+            float sv[5];
+#pragma unroll
+            for (int d = 0; d < 5; ++d) sv[d] = first ? e[j][d] : e[j][d] + (beta * so[j][d]);
+            float v = dg[j] * sv[0];
+            v = fmaf(px[j][0], sv[1], v);
+            v = fmaf(px[j][1], sv[2], v);
+            v = fmaf(py[j][0], sv[3], v);
+            v = fmaf(py[j][1], sv[4], v);
+            s_new[pc] = sv[0];
+            As[pc] = v;
+            if (part_delta_d) dprod += (double)sv[0] * (double)v;
+            else prod += v * sv[0];
+        }
     }
     if (part_delta_d) {
         const double bs = block_sum(dprod, dscratch);
         if (threadIdx.x == 0) part_delta_d[wg] = bs;
         return;
     }
-    float bs = block_sum(prod, scratch);
+    const float bs = block_sum(prod, scratch);
     if (threadIdx.x == 0) part_delta[wg] = bs;
 }
 
 // Iteration k (the uniform argument): the search vectors alternate between
 // search[0] and search[1].
+struct SpmvDotArgs {
+    LevelView L;
+    float* search[2];
+    float* As;
+    PcgState* st;
+    const float* part_gamma;
+    float* part_delta;
+    const double* part_gamma_d;  // relative mode (else null)
+    const double* part_asz_d;
+    double* part_delta_d;
+    int ngam;
+    uint32_t nwg;
+};
+
+template <int NT>
 struct SpmvDot {
-    static constexpr int kThreads = kSolverThreads;
-    struct Args {
-        LevelView L;
-        float* search[2];
-        float* As;
-        PcgState* st;
-        const float* part_gamma;
-        float* part_delta;
-        const double* part_gamma_d;  // relative mode (else null)
-        const double* part_asz_d;
-        double* part_delta_d;
-        int ngam;
-        uint32_t nwg;
-    };
+    static constexpr int kThreads = NT;
+    using Args = SpmvDotArgs;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
-        spmv_dot(a.L, a.search[k & 1], a.search[(k + 1) & 1], a.As, a.st, a.part_gamma, a.part_delta, k,
-                 a.part_gamma_d, a.part_asz_d, a.part_delta_d, a.ngam, wg, a.nwg);
+        spmv_dot<NT>(a.L, a.search[k & 1], a.search[(k + 1) & 1], a.As, a.st, a.part_gamma, a.part_delta, k,
+                     a.part_gamma_d, a.part_asz_d, a.part_delta_d, a.ngam, wg, a.nwg);
     }
 };
 
 // delta from the partials (exit when |delta| < 1e-5, src/apic.rs:1903), then
 // p += alpha*s; r = fma(-alpha, As, r); partials of max |r| (:1907-1928)
+template <int NT>
 __device__ __forceinline__ void axpy_max(const LevelView& L, const float* __restrict__ s, const float* __restrict__ As,
                                 float* __restrict__ pressure, PcgState* st, const float* __restrict__ part_delta,
                                 float* __restrict__ part_max, int k, const double* __restrict__ part_delta_d,
                                 double* __restrict__ part_rr_d, double* __restrict__ pd, uint32_t wg, uint32_t nwg) {
     const bool lead = wg == 0 && threadIdx.x == 0;
-    __shared__ float scratch[kThreads / 64];
-    __shared__ double dscratch[kThreads / 64];
+    __shared__ float scratch[NT / 64];
+    __shared__ double dscratch[NT / 64];
     if (st->done) return;
+    const uint32_t cells = L.cells, stride = nwg * NT;
+    const size_t w = L.w;
+    uint32_t c0 = wg * NT + threadIdx.x;
+    uint8_t fl[kWideBatch];
+    auto fetch_flags = [&]() {
+#pragma unroll
+        for (int j = 0; j < kWideBatch; ++j) {
+            const uint32_t c = c0 + j * stride;
+            fl[j] = c < cells ? L.fluid[c < cells ? c : 0] : 0;
+        }
+    };
+    fetch_flags();  // with the partials
     if (part_delta_d) {  // relative mode: alpha = r.z / s.As in FP64, partials of ||r||^2
-        double d = 0.0;
-        for (uint32_t i = threadIdx.x; i < nwg; i += blockDim.x) d += part_delta_d[i];
-        d = block_sum(d, dscratch);
+        const double d = sum_partials256(part_delta_d, (int)nwg, dscratch);
         if (!(d > 0.0)) {
             if (lead) { st->done = 1; st->exit_reason = 2; st->delta = (float)d; }
             return;
@@ -1008,21 +1091,40 @@ __device__ __forceinline__ void axpy_max(const LevelView& L, const float* __rest
         const float alpha = (float)alpha_d;
         if (lead) { st->delta = (float)d; st->alpha = alpha; st->alpha_d = alpha_d; }
         double rr = 0.0;
-        for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
-            if (!L.fluid[c]) continue;
-            size_t pc = (size_t)c + L.w;
-            const double np = pd[c] + alpha_d * (double)s[pc];
-            pd[c] = np;
-            pressure[c] = (float)np;
-            float nr = fmaf(-alpha, As[pc], L.r[pc]);
-            L.r[pc] = nr;
-            rr += (double)nr * (double)nr;
+        for (; c0 < cells; c0 += kWideBatch * stride) {
+            double p[kWideBatch];
+            float sv[kWideBatch], av[kWideBatch], rv[kWideBatch];
+            uint8_t fl_now[kWideBatch];
+#pragma unroll
+            for (int j = 0; j < kWideBatch; ++j) {
+                const uint32_t c = fl[j] ? c0 + j * stride : 0;
+                p[j] = pd[c];
+                sv[j] = s[c + w];
+                av[j] = As[c + w];
+                rv[j] = L.r[c + w];
+                fl_now[j] = fl[j];
+            }
+            c0 += kWideBatch * stride;
+            fetch_flags();  // the next batch's, in flight during this one
+            c0 -= kWideBatch * stride;
+#pragma unroll
+            for (int j = 0; j < kWideBatch; ++j) {
+                if (!fl_now[j]) continue;
+                const uint32_t c = c0 + j * stride;
+                const size_t pc = (size_t)c + w;
+                const double np = p[j] + alpha_d * (double)sv[j];
+                pd[c] = np;
+                pressure[c] = (float)np;
+                const float nr = fmaf(-alpha, av[j], rv[j]);
+                L.r[pc] = nr;
+                rr += (double)nr * (double)nr;
+            }
         }
         rr = block_sum(rr, dscratch);
         if (threadIdx.x == 0) part_rr_d[wg] = rr;
         return;
     }
-    const float d = sum_partials(part_delta, scratch, (int)nwg);
+    const float d = sum_partials256(part_delta, (int)nwg, scratch);
     if (fabsf(d) < kSafety) {
         if (lead) { st->done = 1; st->exit_reason = 2; st->delta = d; }
         return;
@@ -1030,36 +1132,57 @@ __device__ __forceinline__ void axpy_max(const LevelView& L, const float* __rest
     const float alpha = st->gamma[k] / d;
     if (lead) { st->delta = d; st->alpha = alpha; }
     float m = 0.0f;
-    for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
-        if (!L.fluid[c]) continue;
-        size_t pc = (size_t)c + L.w;
-        pressure[c] += alpha * s[pc];
-        float nr = fmaf(-alpha, As[pc], L.r[pc]);
-        L.r[pc] = nr;
-        m = fast_max(m, fabsf(nr));
+    for (; c0 < cells; c0 += kWideBatch * stride) {
+        float p[kWideBatch], sv[kWideBatch], av[kWideBatch], rv[kWideBatch];
+        uint8_t fl_now[kWideBatch];
+#pragma unroll
+        for (int j = 0; j < kWideBatch; ++j) {
+            const uint32_t c = fl[j] ? c0 + j * stride : 0;
+            p[j] = pressure[c];
+            sv[j] = s[c + w];
+            av[j] = As[c + w];
+            rv[j] = L.r[c + w];
+            fl_now[j] = fl[j];
+        }
+        c0 += kWideBatch * stride;
+        fetch_flags();
+        c0 -= kWideBatch * stride;
+#pragma unroll
+        for (int j = 0; j < kWideBatch; ++j) {
+            if (!fl_now[j]) continue;
+            const uint32_t c = c0 + j * stride;
+            const size_t pc = (size_t)c + w;
+            pressure[c] = p[j] + alpha * sv[j];
+            const float nr = fmaf(-alpha, av[j], rv[j]);
+            L.r[pc] = nr;
+            m = fast_max(m, fabsf(nr));
+        }
     }
-    float bm = block_max(m, scratch);
+    const float bm = block_max(m, scratch);
     if (threadIdx.x == 0) part_max[wg] = bm;
 }
 
+struct AxpyMaxArgs {
+    LevelView L;
+    float* search[2];
+    const float* As;
+    float* pressure;
+    PcgState* st;
+    const float* part_delta;
+    float* part_max;
+    const double* part_delta_d;  // relative mode (else null)
+    double* part_rr_d;
+    double* pd;
+    uint32_t nwg;
+};
+
+template <int NT>
 struct AxpyMax {
-    static constexpr int kThreads = kSolverThreads;
-    struct Args {
-        LevelView L;
-        float* search[2];
-        const float* As;
-        float* pressure;
-        PcgState* st;
-        const float* part_delta;
-        float* part_max;
-        const double* part_delta_d;  // relative mode (else null)
-        double* part_rr_d;
-        double* pd;
-        uint32_t nwg;
-    };
+    static constexpr int kThreads = NT;
+    using Args = AxpyMaxArgs;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
-        axpy_max(a.L, a.search[(k + 1) & 1], a.As, a.pressure, a.st, a.part_delta, a.part_max, k, a.part_delta_d,
-                 a.part_rr_d, a.pd, wg, a.nwg);
+        axpy_max<NT>(a.L, a.search[(k + 1) & 1], a.As, a.pressure, a.st, a.part_delta, a.part_max, k, a.part_delta_d,
+                     a.part_rr_d, a.pd, wg, a.nwg);
     }
 };
 
@@ -1081,12 +1204,10 @@ struct Decide {
         const bool lead = threadIdx.x == 0;
         if (!st->done) {
             if (a.part_gamma_d) {
-                double g = 0.0;
-                for (int i = threadIdx.x; i < a.ngam; i += blockDim.x) g += a.part_gamma_d[i];
-                g = block_sum(g, dscratch);
+                const double g = sum_partials256(a.part_gamma_d, a.ngam, dscratch);
                 (void)gamma_exit_rel(st, g, k, lead);
             } else {
-                const float g = sum_partials(a.part_gamma, scratch, a.ngam);
+                const float g = sum_partials256(a.part_gamma, a.ngam, scratch);
                 (void)gamma_exit(st, g, k, lead);
             }
         }
@@ -1314,13 +1435,17 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         }
         __syncthreads();
     }
-    // black cells of the tile and its one-cell ring (they read red cells only)
+    // black cells of the tile and its one-cell ring (they read red cells
+    // only).  A black cell's own value before its sweep is zero (the red
+    // sweep before it leaves it as the restriction set it); it is not taken
+    // from memory, where a neighbouring tile may already have written its
+    // result for the ring's cells.
     for (int i = threadIdx.x; i < G::kN; i += blockDim.x) {
         const int lx = i % kR, ly = i / kR;
         if (lx < 1 || lx >= kR - 1 || ly < 1 || ly >= kR - 1) continue;
         const int32_t x = X0 - kVH + lx, y = Y0 - kVH + ly;
         if (!t.fl[i] || ((x + y) & 1) == 0) continue;
-        t.e[i] = vsor<T>(t, i, t.e[i], t.e[i + 1], t.e[i + kR], t.e[i - 1], t.e[i - kR]);
+        t.e[i] = vsor<T>(t, i, 0.0f, t.e[i + 1], t.e[i + kR], t.e[i - 1], t.e[i - kR]);
     }
     __syncthreads();
 #pragma unroll
@@ -1396,12 +1521,14 @@ struct VDown {
 // Up: the two colour sweeps of smooth_level's reverse pass on level L with
 // C's error prolongated on the fly (prolongate + sor_cell modes 1 and 2); on
 // the base level, with the padded-list repeats of prolongate and the partials
-// of r.z over the tiles a workgroup takes (dot: 1 FP32, 2 FP64).
+// of r.z over the tiles a workgroup takes (dot: 1 FP32, 2 FP64).  L.e is the
+// level's down-sweep buffer and the result goes to e_out: a workgroup's ring
+// must not see a neighbouring tile's result.
 template <int T>
 __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, const FluidStats* __restrict__ fs,
                            const int32_t* __restrict__ done, uint32_t tiles_x, int dot, float* __restrict__ part_gamma,
                            double* __restrict__ part_gamma_d, const float* __restrict__ As,
-                           double* __restrict__ part_asz_d, int spec, uint32_t wg) {
+                           double* __restrict__ part_asz_d, int spec, float* __restrict__ e_out, uint32_t wg) {
     using G = VGeom<T>;
     constexpr int kR = G::kR, kC = G::kC;
     __shared__ VTile<T> t;
@@ -1506,7 +1633,7 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         if (t.fl[li] && ((x + y) & 1) == 0)
             e = vsor<T>(t, li, t.dg[li], t.e[li + 1], t.e[li + kR], t.e[li - 1], t.e[li - kR]);
         const size_t pc = (size_t)(y + 1) * w + x;
-        L.e[pc] = e;
+        e_out[pc] = e;
         if (dot == 2) {
             dprod += (double)t.r[li] * (double)e;
             if (t.fl[li]) dasz += (double)asv[q] * (double)e;
@@ -1538,6 +1665,7 @@ struct VUpArgs {
     const float* As;      // relative mode: A s of the last iteration and
     double* part_asz_d;   // the partials of (A s).z
     int spec;             // load the region with the flags (levels of few tiles)
+    float* e_out;         // the level's e; L.e is its down-sweep buffer
 };
 
 template <int T>
@@ -1546,7 +1674,7 @@ struct VUp {
     using Args = VUpArgs;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
         vup<T>(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, a.spec,
-               wg);
+               a.e_out, wg);
     }
 };
 
@@ -1639,14 +1767,22 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         uint32_t tx;
         return l > 0 && tiles(sim.levels[l], kBaseTile, tx) <= kFewTiles;
     };
+    // A level above the bottom with its down-sweep buffer as e: the way down
+    // reads and writes it, the way up reads it and writes the level's e
+    // (the bottom's levels keep theirs: one workgroup runs them)
+    auto down = [&](int l) {
+        LevelView v = sim.levels[l];
+        if (l < bottom) v.e = sim.down_e[l];
+        return v;
+    };
     for (int l = 0; l < bottom; ++l) {
-        const LevelView& F = sim.levels[l];
+        const LevelView F = down(l);
         const int fuse_red = l + 1 < bottom ? 1 : (int)(bottom < last);
         const bool sm = small(l);
         uint32_t tx;
         const uint32_t nt = tiles(F, sm ? 16 : kBaseTile, tx);
         const bool rel_check = l == 0 && rel;
-        VDownArgs a{F, sim.levels[l + 1], (int)(l == 0), fuse_red, (int)(l > 0), done, tx, sim.pcg,
+        VDownArgs a{F, down(l + 1), (int)(l == 0), fuse_red, (int)(l > 0), done, tx, sim.pcg,
                           (int)(l == 0 && mode == 2), sim.props.solve_error, sim.part_max,
                           rel_check ? sim.part_rr_d : nullptr, rel_check ? sim.part_rr2_d : nullptr,
                           rel_check ? sim.solver.recompute_every : 0, sim.nred};
@@ -1658,13 +1794,13 @@ void launch_vcycle(Sim& sim, int mode, int k) {
     else
         emit<BottomGlobal>(sim, {level_set(sim), bottom, sim.cg_s, sim.cg_ap, sim.cg_r, done}, 1);
     for (int l = bottom - 1; l >= 0; --l) {
-        const LevelView& L = sim.levels[l];
+        const LevelView L = down(l);
         const bool sm = small(l);
         uint32_t tx;
         const uint32_t nt = tiles(L, sm ? 16 : kBaseTile, tx);
         const int dot = (l == 0 && mode != 0) ? (rel ? 2 : 1) : 0;  // one partial of r.z per tile
         VUpArgs a{L, sim.levels[l + 1], l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot,
-                        sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d, (int)(l > 0)};
+                        sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d, (int)(l > 0), sim.levels[l].e};
         if (sm) emit<VUp<16>>(sim, a, nt);
         else emit<VUp<kBaseTile>>(sim, a, nt);
     }
@@ -1674,14 +1810,16 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     const LevelView& B = sim.levels[0];
     const bool rel = sim.solver.mode == 1;
     sim.mark(APIC_T_K_SPMV, 0);
-    SpmvDot::Args a{B, {sim.search[0], sim.search[1]}, sim.As, sim.pcg, sim.part_gamma, sim.part_delta,
-                    rel ? sim.part_gamma_d : nullptr, sim.part_asz_d, rel ? sim.part_delta_d : nullptr,
-                    (int)gamma_partials(sim), sim.nred};
-    emit<SpmvDot>(sim, a, sim.nred, k);
+    SpmvDotArgs a{B, {sim.search[0], sim.search[1]}, sim.As, sim.pcg, sim.part_gamma, sim.part_delta,
+                  rel ? sim.part_gamma_d : nullptr, sim.part_asz_d, rel ? sim.part_delta_d : nullptr,
+                  (int)gamma_partials(sim), sim.nred};
+    if (rel) emit<SpmvDot<kWideThreads>>(sim, a, sim.nred, k);
+    else emit<SpmvDot<kSolverThreads>>(sim, a, sim.nred, k);
     sim.mark(APIC_T_K_SPMV, 1);
-    AxpyMax::Args b{B, {sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pcg, sim.part_delta, sim.part_max,
-                    rel ? sim.part_delta_d : nullptr, rel ? sim.part_rr_d : nullptr, sim.pressure_d, sim.nred};
-    emit<AxpyMax>(sim, b, sim.nred, k);
+    AxpyMaxArgs b{B, {sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pcg, sim.part_delta, sim.part_max,
+                  rel ? sim.part_delta_d : nullptr, rel ? sim.part_rr_d : nullptr, sim.pressure_d, sim.nred};
+    if (rel) emit<AxpyMax<kWideThreads>>(sim, b, sim.nred, k);
+    else emit<AxpyMax<kSolverThreads>>(sim, b, sim.nred, k);
     // the true residual, on the iterations that recompute it (a batch records
     // one iteration for all k: there the kernel itself skips the others)
     const int every = sim.solver.recompute_every;

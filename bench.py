@@ -58,7 +58,7 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c):
 
 
 KERNEL_NAMES = {"kernel_g2p": "one_kernel<G2p>", "kernel_p2g": "one_kernel<P2gOwner>",
-                "kernel_spmv": "one_kernel<SpmvDot>"}
+                "kernel_spmv": "one_kernel<SpmvDot<1024> >"}
 BATCH_WORKLOAD = "cfg4_batched_256"
 
 
