@@ -43,10 +43,15 @@ constexpr size_t kBottomLdsFloats = 34816;
 constexpr size_t kBottomLdsFlags = 8192;
 __host__ __device__ inline size_t bottom_padded(const LevelView& L) { return (size_t)L.cells + 2 * (size_t)L.w; }
 constexpr uint32_t kBottomFirstCells = 1024;  // cells of the one-workgroup bottom's finest level
+constexpr size_t kBottomPaddedMax = 2048;      // padded length of an LDS bottom level (two per thread)
 
 inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first) {
     size_t fl = 0, fb = 0;
-    for (size_t l = first; l < lv.size(); ++l) { fl += 6 * bottom_padded(lv[l]); fb += lv[l].cells; }
+    for (size_t l = first; l < lv.size(); ++l) {
+        if (bottom_padded(lv[l]) > kBottomPaddedMax) return false;
+        fl += 6 * bottom_padded(lv[l]);
+        fb += lv[l].cells;
+    }
     fl += 3 * bottom_padded(lv.back()) + lv.back().cells;
     return fl <= kBottomLdsFloats && fb <= kBottomLdsFlags;
 }

@@ -678,46 +678,208 @@ struct BottomGlobal {
 };
 
 // Bottom levels copied to LDS: the same arithmetic without a global-memory
-// round trip between the dependent phases.  The errors and residuals of every
-// bottom level are copied back (the finer level's prolongation reads them).
+// round trip between the dependent phases.  Every level of an LDS bottom has
+// at most kBottomFirstCells <= kBottomThreads cells, so thread t owns cell t
+// of each level: the sweeps and restrictions of sor_cell, restrict_cell and
+// prolonged run with the cell's coordinates computed once per level and
+// direct LDS indexing.  The errors and residuals of every bottom level are
+// copied back (the finer level's prolongation reads them).
+static_assert(kBottomFirstCells <= (uint32_t)kBottomThreads, "one cell per thread on an LDS bottom");
+
+// Offsets of a bottom level's arrays in the LDS image.
+struct BLev {
+    int32_t w, h, n;    // sides and cells
+    uint32_t dg, id, px, py, r, e;  // float offsets of the padded arrays
+    uint32_t fl;        // offset of the fluid flags
+};
+
 __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const int32_t* __restrict__ done) {
     __shared__ float f[kBottomLdsFloats];
     __shared__ uint8_t fl[kBottomLdsFlags];
     __shared__ float scratch[kBottomThreads / 64];
-    if (*done) return;
+    __shared__ BLev tab[16];  // the levels' offsets, by level index
+    const int32_t dn = *done;
     const int last = ls.count - 1;
-    LdsViews V{&ls, first, f, fl};
-    size_t fo = 0, bo = 0;
-    for (int l = first; l <= last; ++l) {
-        const LevelView& G = ls.lv[l];
-        const uint32_t p = (uint32_t)bottom_padded(G);
-        for (uint32_t i = threadIdx.x; i < p; i += blockDim.x) {
-            f[fo + i] = G.diag[i];
-            f[fo + p + i] = G.inv_diag[i];
-            f[fo + 2 * p + i] = G.px[i];
-            f[fo + 3 * p + i] = G.py[i];
-            f[fo + 4 * p + i] = G.r[i];
-            f[fo + 5 * p + i] = G.e[i];
+    const int c = (int)threadIdx.x;
+    // Copy-in, level after level, into the image (diag, inv_diag, px, py, r,
+    // e over the padded range, flags apart): elements t and t + kBottomThreads
+    // of each array and flag t per thread, with the next level's loads in
+    // flight while a level's values are stored.
+    {
+        constexpr int kPer = 2;
+        static_assert(kBottomPaddedMax <= (size_t)kPer * kBottomThreads, "padded length per thread");
+        struct Share {
+            float v[6][kPer];
+            uint8_t flag;
+        };
+        auto fetch = [&](int l, Share& sh) {
+            const LevelView& G = ls.lv[l];
+            const uint32_t p = (uint32_t)bottom_padded(G);
+            const float* src[6] = {G.diag, G.inv_diag, G.px, G.py, G.r, G.e};
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int k = 0; k < kPer; ++k) {
+                    const uint32_t i = threadIdx.x + k * kBottomThreads;
+                    sh.v[a][k] = src[a][i < p ? i : 0];
+                }
+            sh.flag = G.fluid[threadIdx.x < G.cells ? threadIdx.x : 0];
+        };
+        Share cur, nxt;
+        fetch(first, cur);
+        uint32_t o = 0, fo = 0;
+        for (int l = first; l <= last; ++l) {
+            if (l < last) fetch(l + 1, nxt);
+            const LevelView& G = ls.lv[l];
+            const uint32_t p = (uint32_t)bottom_padded(G);
+#pragma unroll
+            for (int a = 0; a < 6; ++a)
+#pragma unroll
+                for (int k = 0; k < kPer; ++k) {
+                    const uint32_t i = threadIdx.x + k * kBottomThreads;
+                    if (i < p) f[o + a * p + i] = cur.v[a][k];
+                }
+            if (threadIdx.x < G.cells) fl[fo + threadIdx.x] = cur.flag;
+            if (threadIdx.x == 0)
+                tab[l] = {(int32_t)G.w, (int32_t)G.h, (int32_t)G.cells, o, o + p, o + 2 * p, o + 3 * p, o + 4 * p,
+                          o + 5 * p, fo};
+            o += 6 * p;
+            fo += G.cells;
+            cur = nxt;
         }
-        for (uint32_t i = threadIdx.x; i < G.cells; i += blockDim.x) fl[bo + i] = G.fluid[i];
-        fo += 6 * (size_t)p;
-        bo += G.cells;
     }
+    if (dn) return;
+    LdsViews V{&ls, first, f, fl};
     const size_t pl = bottom_padded(ls.lv[last]);
-    float* cg_s = f + fo;
+    float* cg_s = V.view(last).e + pl;  // after the image: the coarse CG vectors
     float* cg_ap = cg_s + pl;
     float* cg_r = cg_ap + pl;
     float* sprod = cg_r + pl;
     // the CG vectors' padding rows are read as neighbours: zero, as in global memory
     for (uint32_t i = threadIdx.x; i < 3 * pl + ls.lv[last].cells; i += blockDim.x) cg_s[i] = 0.0f;
     __syncthreads();
-    bottom_body(V, first, last, cg_s, cg_ap, cg_r, sprod, scratch);
+
+    // sor_cell's update from the cell's coefficients, own value and neighbour values
+    auto sor = [](float own, float id, float r, float px, float py, float pxm, float pym, float er, float et,
+                  float el, float eb) {
+        if (id != 0.0f) {
+            float sum = r;
+            sum = fmaf(-px, er, sum);
+            sum = fmaf(-py, et, sum);
+            sum = fmaf(-pxm, el, sum);
+            sum = fmaf(-pym, eb, sum);
+            const float gs = sum * id;
+            own = own + (kOmega * (gs - own));
+        }
+        return own;
+    };
+    // restrict_cell: cell c = (cx, cy) of C from level F
+    auto restrict_to = [&](const BLev& F, const BLev& C, int cx, int cy, bool fuse_red) {
+        if (c >= C.n) return;
+        const int pcc = c + C.w;
+        if (!fl[C.fl + c]) { f[C.r + pcc] = 0.0f; f[C.e + pcc] = 0.0f; return; }
+        float sum = 0.0f;
+#pragma unroll
+        for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+            for (int dx = 0; dx < 2; ++dx) {
+                const int rf = (cy * 2 + dy) * F.w + cx * 2 + dx, pf = rf + F.w;
+                if (fl[F.fl + rf]) {
+                    const float b = f[F.r + pf];
+                    float ax = f[F.dg + pf] * f[F.e + pf];
+                    ax += f[F.px + pf] * f[F.e + pf + 1];
+                    ax += f[F.py + pf] * f[F.e + pf + F.w];
+                    ax += f[F.px + pf - 1] * f[F.e + pf - 1];
+                    ax += f[F.py + pf - F.w] * f[F.e + pf - F.w];
+                    sum += b - ax;
+                }
+            }
+        f[C.r + pcc] = sum;
+        const float id = f[C.id + pcc];
+        const bool red = ((cx + cy) & 1) == 0;
+        f[C.e + pcc] = (fuse_red && red && id != 0.0f) ? kOmega * (sum * id) : 0.0f;
+    };
+    auto coords = [&](const BLev& L, int& x, int& y) {
+        const bool mine = c < L.n;
+        x = mine ? c % L.w : 0;
+        y = mine ? c / L.w : 0;
+        return mine;
+    };
+
+    // down: restriction into level l, then its black sweep (the red one was
+    // fused into the restriction)
+    for (int l = first; l < last; ++l) {
+        const BLev L = tab[l];
+        int x, y;
+        const bool mine = coords(L, x, y);
+        if (l > first) {
+            restrict_to(tab[l - 1], L, x, y, true);
+            __syncthreads();
+        }
+        if (mine && fl[L.fl + c] && ((x + y) & 1) == 1) {
+            const int pc = c + L.w;
+            f[L.e + pc] = sor(f[L.e + pc], f[L.id + pc], f[L.r + pc], f[L.px + pc], f[L.py + pc], f[L.px + pc - 1],
+                              f[L.py + pc - L.w], f[L.e + pc + 1], f[L.e + pc + L.w], f[L.e + pc - 1],
+                              f[L.e + pc - L.w]);
+        }
+        __syncthreads();
+    }
+    const LevelView C = V.view(last);
+    if (last > first) {
+        int x, y;
+        coords(tab[last], x, y);
+        restrict_to(tab[last - 1], tab[last], x, y, false);
+        __syncthreads();
+    }
+    if (C.cells <= 64) {
+        if (threadIdx.x < 64) coarse_cg_wave(C);
+    } else {
+        coarse_cg_block(C, cg_s, cg_ap, cg_r, sprod, scratch);
+    }
+    __syncthreads();
+    // up: the two colours of each level with the parent's error prolongated
+    for (int l = last - 1; l >= first; --l) {
+        const BLev L = tab[l], P = tab[l + 1];
+        int x, y;
+        const bool mine = coords(L, x, y);
+        const int pc = (mine ? c : 0) + L.w;
+        const bool work = mine && fl[L.fl + c];
+        // e + the parent's error at raw index raw = (nx, ny) when it is a
+        // fluid cell of the level; e alone outside [0, n) or for other cells
+        // (rows wrap as in the flat arrays).  The loads are unconditional
+        // (clamped), so that they go out together.
+        auto nb = [&](int raw, int nx, int ny) {
+            const bool in = raw >= 0 && raw < L.n;
+            const float v = f[L.e + raw + L.w];
+            const uint8_t fv = fl[L.fl + (in ? raw : c)];
+            const float pe = f[P.e + ((in ? ny : y) / 2) * P.w + (in ? nx : x) / 2 + P.w];
+            return (in && fv) ? v + pe : v;
+        };
+        // the cell's own prolongated value and coefficients, before the first
+        // colour (it leaves the second colour's cells as they are)
+        const float own = f[L.e + pc] + f[P.e + (y / 2) * P.w + x / 2 + P.w];
+        const float id = f[L.id + pc], r = f[L.r + pc], px = f[L.px + pc], py = f[L.py + pc];
+        const float pxm = f[L.px + pc - 1], pym = f[L.py + pc - L.w];
+        const bool black = ((x + y) & 1) == 1;
+        if (work && black) {
+            const float er = x + 1 < L.w ? nb(c + 1, x + 1, y) : nb(c + 1, 0, y + 1);
+            const float et = nb(c + L.w, x, y + 1);
+            const float el = x > 0 ? nb(c - 1, x - 1, y) : nb(c - 1, L.w - 1, y - 1);
+            const float eb = nb(c - L.w, x, y - 1);
+            f[L.e + pc] = sor(own, id, r, px, py, pxm, pym, er, et, el, eb);
+        }
+        __syncthreads();
+        if (work && !black)
+            f[L.e + pc] = sor(own, id, r, px, py, pxm, pym, f[L.e + pc + 1], f[L.e + pc + L.w], f[L.e + pc - 1],
+                              f[L.e + pc - L.w]);
+        __syncthreads();
+    }
     for (int l = first; l <= last; ++l) {
         const LevelView G = ls.lv[l];
-        const LevelView L = V.view(l);
-        for (uint32_t i = threadIdx.x; i < G.cells; i += blockDim.x) {
-            G.r[i + G.w] = L.r[i + G.w];
-            G.e[i + G.w] = L.e[i + G.w];
+        const BLev L = tab[l];
+        if (c < L.n) {
+            G.r[c + G.w] = f[L.r + c + L.w];
+            G.e[c + G.w] = f[L.e + c + L.w];
         }
     }
 }
