@@ -1059,11 +1059,17 @@ __device__ inline bool gamma_exit(PcgState* st, float g, int k, bool lead) {
 // then every operand of the batch's fluid cells at once (a non-fluid cell
 // reads the row's first cell, a line every lane of the wave shares), so that
 // a workgroup waits twice per batch instead of twice per cell.  With FP64
-// reductions they run kWideThreads threads per workgroup (nred workgroups:
-// about two cells per thread); with the FP32 reductions of the reference's
-// rules 256, which keeps each partial's cells and summation order.
+// reductions on a grid of at least kWideMinRed reduction workgroups they run
+// kWideThreads threads per workgroup (about two cells per thread): one such
+// grid's launch is bound by latency, which four times the waves hide.
+// Smaller grids, and batches of them (whose launches fill the GPU with the
+// workgroups of all their simulations), keep 256 threads and more cells per
+// wave, and so do the FP32 reductions of the reference's rules (which keeps
+// each partial's cells and summation order).  The choice depends on the
+// simulation alone, so a batched simulation computes what it computes alone.
 constexpr int kWideThreads = 1024;
 constexpr int kWideBatch = 2;
+constexpr uint32_t kWideMinRed = 256;
 
 // The n partials of one reduction summed in the order of a 256-thread
 // workgroup (thread t < 256 adds part[t], part[t + 256], ... in turn; the
@@ -1975,12 +1981,13 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     SpmvDotArgs a{B, {sim.search[0], sim.search[1]}, sim.As, sim.pcg, sim.part_gamma, sim.part_delta,
                   rel ? sim.part_gamma_d : nullptr, sim.part_asz_d, rel ? sim.part_delta_d : nullptr,
                   (int)gamma_partials(sim), sim.nred};
-    if (rel) emit<SpmvDot<kWideThreads>>(sim, a, sim.nred, k);
+    const bool wide = rel && sim.nred >= kWideMinRed;
+    if (wide) emit<SpmvDot<kWideThreads>>(sim, a, sim.nred, k);
     else emit<SpmvDot<kSolverThreads>>(sim, a, sim.nred, k);
     sim.mark(APIC_T_K_SPMV, 1);
     AxpyMaxArgs b{B, {sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pcg, sim.part_delta, sim.part_max,
                   rel ? sim.part_delta_d : nullptr, rel ? sim.part_rr_d : nullptr, sim.pressure_d, sim.nred};
-    if (rel) emit<AxpyMax<kWideThreads>>(sim, b, sim.nred, k);
+    if (wide) emit<AxpyMax<kWideThreads>>(sim, b, sim.nred, k);
     else emit<AxpyMax<kSolverThreads>>(sim, b, sim.nred, k);
     // the true residual, on the iterations that recompute it (a batch records
     // one iteration for all k: there the kernel itself skips the others)
