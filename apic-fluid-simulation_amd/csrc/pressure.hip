@@ -1409,18 +1409,19 @@ struct Decide {
 
 constexpr int kVH = 2;         // ring: the first colour of a sweep needs two
 constexpr int kBaseTile = 32;  // tile side on the base level (one partial of r.z per tile)
+constexpr int kWideTileThreads = 512;
 
-template <int T>
+template <int T, int NT = 256>
 struct VGeom {
     static constexpr int kT = T;                        // tile side (cells of the level)
     static constexpr int kR = T + 2 * kVH;              // region side
     static constexpr int kN = kR * kR;                  // region cells
     static constexpr int kC = T / 2 + 2;                // parent cells across the region
-    static constexpr int kPer = (kN + 255) / 256;       // region cells per thread
+    static constexpr int kPer = (kN + NT - 1) / NT;     // region cells per thread
     static constexpr int kCN = (T / 2) * (T / 2);       // the tile's coarse cells
-    static constexpr int kCPer = (kCN + 255) / 256;
-    static constexpr int kTPer = (T * T + 255) / 256;   // tile cells per thread
-    static constexpr int kEPer = (kC * kC + 255) / 256;  // parent errors per thread
+    static constexpr int kCPer = (kCN + NT - 1) / NT;
+    static constexpr int kTPer = (T * T + NT - 1) / NT;  // tile cells per thread
+    static constexpr int kEPer = (kC * kC + NT - 1) / NT;  // parent errors per thread
     static_assert(kPer <= 32, "region slots are masks of 32 bits");
 };
 
@@ -1433,10 +1434,10 @@ struct VTile {
 
 // A thread's share of a region, held in registers between the loads and the
 // LDS stores, so that all of the loads go out before any of them is waited
-// for (compile-time stride: the workgroup has 256 threads).
-template <int T>
+// for (compile-time stride: the workgroup has NT threads).
+template <int T, int NT>
 struct VRegs {
-    static constexpr int kPer = VGeom<T>::kPer;
+    static constexpr int kPer = VGeom<T, NT>::kPer;
     float r[kPer], id[kPer], px[kPer], py[kPer], dg[kPer], e[kPer];
     uint8_t fl[kPer];
     uint32_t inb, real;  // bit k: slot k is in the padded range / a cell of the grid
@@ -1445,15 +1446,15 @@ struct VRegs {
 // Loads at clamped addresses (no branch per load, so that they all issue back
 // to back); the values outside the range are replaced by zeros in vtile_put,
 // after the loads the caller issues next.
-template <int T>
-__device__ __forceinline__ void vtile_fetch(const LevelView& L, int32_t X0, int32_t Y0, VRegs<T>& g, bool load_e,
+template <int T, int NT>
+__device__ __forceinline__ void vtile_fetch(const LevelView& L, int32_t X0, int32_t Y0, VRegs<T, NT>& g, bool load_e,
                                             bool load_dg) {
-    using G = VGeom<T>;
+    using G = VGeom<T, NT>;
     const int64_t w = L.w, h = L.h, np = (int64_t)L.cells + 2 * w;
     g.inb = g.real = 0u;
 #pragma unroll
     for (int k = 0; k < G::kPer; ++k) {
-        const int i = (int)threadIdx.x + 256 * k;
+        const int i = (int)threadIdx.x + NT * k;
         const int32_t x = X0 - kVH + i % G::kR, y = Y0 - kVH + i / G::kR;
         const int64_t p = (int64_t)(y + 1) * w + x;
         const bool inb = i < G::kN && p >= 0 && p < np;
@@ -1471,12 +1472,12 @@ __device__ __forceinline__ void vtile_fetch(const LevelView& L, int32_t X0, int3
     }
 }
 
-template <int T>
-__device__ __forceinline__ void vtile_put(const VRegs<T>& g, VTile<T>& t) {
-    using G = VGeom<T>;
+template <int T, int NT>
+__device__ __forceinline__ void vtile_put(const VRegs<T, NT>& g, VTile<T>& t) {
+    using G = VGeom<T, NT>;
 #pragma unroll
     for (int k = 0; k < G::kPer; ++k) {
-        const int i = (int)threadIdx.x + 256 * k;
+        const int i = (int)threadIdx.x + NT * k;
         if (i >= G::kN) continue;
         const bool inb = (g.inb >> k) & 1u, real = (g.real >> k) & 1u;
         t.r[i] = inb ? g.r[k] : 0.0f;
@@ -1510,16 +1511,16 @@ __device__ inline float vsor(const VTile<T>& t, int i, float own, float er, floa
 // the base level, in a PCG iteration, every workgroup first applies the
 // residual test of the previous update (max |r| < solve_error,
 // src/apic.rs:1930, or the relative test).
-template <int T>
+template <int T, int NT>
 __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, int base, int fuse_red, int spec,
                              const int32_t* __restrict__ done, uint32_t tiles_x, PcgState* st, int check,
                              float solve_error, const float* __restrict__ part_max, const double* __restrict__ part_rr,
                              uint32_t nred, uint32_t wg) {
-    using G = VGeom<T>;
+    using G = VGeom<T, NT>;
     constexpr int kR = G::kR;
     __shared__ VTile<T> t;
-    __shared__ float scratch[256 / 64];
-    __shared__ double dscratch[256 / 64];
+    __shared__ float scratch[NT / 64];
+    __shared__ double dscratch[NT / 64];
     const bool lead = wg == 0 && threadIdx.x == 0;
     const int32_t X0 = (int32_t)(wg % tiles_x) * T, Y0 = (int32_t)(wg / tiles_x) * T;
     const int32_t w = (int32_t)F.w, h = (int32_t)F.h;
@@ -1531,14 +1532,14 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     // whether the tile has work (a coarse level has few tiles: its time is the
     // latency of these loads, not their bytes).
     const int32_t dn = *done;
-    VRegs<T> g;
-    if (spec) vtile_fetch<T>(F, X0, Y0, g, !base, true);
+    VRegs<T, NT> g;
+    if (spec) vtile_fetch<T, NT>(F, X0, Y0, g, !base, true);
     uint8_t cfl[G::kCPer];
     float cid[G::kCPer];
     bool cin[G::kCPer];
 #pragma unroll
     for (int q = 0; q < G::kCPer; ++q) {
-        const int k = (int)threadIdx.x + 256 * q;
+        const int k = (int)threadIdx.x + NT * q;
         const int32_t cx = X0 / 2 + k % (T / 2), cy = Y0 / 2 + k / (T / 2);
         const bool in = k < G::kCN && cx < cw && cy < ch;  // (the restriction below skips the others)
         const size_t c = in ? (size_t)cy * cw + cx : 0;
@@ -1546,25 +1547,24 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         cid[q] = C.inv_diag[c + cw];
         cin[q] = in;
     }
-    constexpr int kRPer = kRedBlocks / 256;
-    static_assert(kRedBlocks % 256 == 0, "partials per thread");
+    constexpr int kRPer = (kRedBlocks + NT - 1) / NT;
     double prr[kRPer];
     float pmx[kRPer];
 #pragma unroll
     for (int q = 0; q < kRPer; ++q) {
-        const uint32_t i = threadIdx.x + 256u * q, ic = i < nred ? i : 0u;
+        const uint32_t i = threadIdx.x + (uint32_t)NT * q, ic = i < nred ? i : 0u;
         prr[q] = 0.0;
         pmx[q] = 0.0f;
         if (check && part_rr) prr[q] = part_rr[ic];
         else if (check) pmx[q] = part_max[ic];
     }
-    if (spec) vtile_put<T>(g, t);  // read after the barrier of the tile test below
+    if (spec) vtile_put<T, NT>(g, t);  // read after the barrier of the tile test below
     if (dn) return;
     if (check && part_rr) {
         double rr = 0.0;
 #pragma unroll
         for (int q = 0; q < kRPer; ++q)
-            if (threadIdx.x + 256u * q < nred) rr += prr[q];
+            if (threadIdx.x + (uint32_t)NT * q < nred) rr += prr[q];
         rr = block_sum(rr, dscratch);
         if (lead) st->rnorm2 = rr;
         if (rr <= st->tol2 * st->bnorm2) {
@@ -1575,7 +1575,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         float m = 0.0f;
 #pragma unroll
         for (int q = 0; q < kRPer; ++q)
-            if (threadIdx.x + 256u * q < nred) m = fast_max(m, pmx[q]);
+            if (threadIdx.x + (uint32_t)NT * q < nred) m = fast_max(m, pmx[q]);
         m = block_max(m, scratch);
         if (lead) st->maxerr = m;
         if (m < solve_error) {
@@ -1591,8 +1591,8 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     for (int q = 0; q < G::kCPer; ++q) any = any || (cin[q] && cfl[q] != 0);
     if (!__syncthreads_or(any)) return;
     if (!spec) {
-        vtile_fetch<T>(F, X0, Y0, g, !base, true);
-        vtile_put<T>(g, t);
+        vtile_fetch<T, NT>(F, X0, Y0, g, !base, true);
+        vtile_put<T, NT>(g, t);
         __syncthreads();
     }
     if (base) {  // red_zero_cell over the region
@@ -1618,7 +1618,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < G::kTPer; ++q) {
-        const int i = (int)threadIdx.x + 256 * q;
+        const int i = (int)threadIdx.x + NT * q;
         const int32_t x = X0 + i % T, y = Y0 + i / T;
         if (i >= T * T || x >= w || y >= h) continue;
         if (!base && ((x + y) & 1) == 0) continue;  // red values are already in memory
@@ -1628,7 +1628,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     // restriction into the tile's coarse cells (restrict_cell)
 #pragma unroll
     for (int q = 0; q < G::kCPer; ++q) {
-        const int k = (int)threadIdx.x + 256 * q;
+        const int k = (int)threadIdx.x + NT * q;
         const int32_t cx = X0 / 2 + k % (T / 2), cy = Y0 / 2 + k / (T / 2);
         if (!cin[q]) continue;
         const size_t c = (size_t)cy * cw + cx, pcc = c + cw;
@@ -1674,14 +1674,14 @@ struct VDownArgs {
     uint32_t nred;
 };
 
-template <int T>
+template <int T, int NT = 256>
 struct VDown {
-    static constexpr int kThreads = 256;
+    static constexpr int kThreads = NT;
     using Args = VDownArgs;
     // k: the PCG iteration whose update the residual test checks
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         const bool due = a.every > 0 && (k + 1) % a.every == 0;
-        vdown<T>(a.F, a.C, a.base, a.fuse_red, a.spec, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max,
+        vdown<T, NT>(a.F, a.C, a.base, a.fuse_red, a.spec, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max,
                  due ? a.part_rr2 : a.part_rr, a.nred, wg);
     }
 };
@@ -1692,17 +1692,17 @@ struct VDown {
 // of r.z over the tiles a workgroup takes (dot: 1 FP32, 2 FP64).  L.e is the
 // level's down-sweep buffer and the result goes to e_out: a workgroup's ring
 // must not see a neighbouring tile's result.
-template <int T>
+template <int T, int NT>
 __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, const FluidStats* __restrict__ fs,
                            const int32_t* __restrict__ done, uint32_t tiles_x, int dot, float* __restrict__ part_gamma,
                            double* __restrict__ part_gamma_d, const float* __restrict__ As,
                            double* __restrict__ part_asz_d, int spec, float* __restrict__ e_out, uint32_t wg) {
-    using G = VGeom<T>;
+    using G = VGeom<T, NT>;
     constexpr int kR = G::kR, kC = G::kC;
     __shared__ VTile<T> t;
     __shared__ float ce[kC * kC];
-    __shared__ float scratch[256 / 64];
-    __shared__ double dscratch[256 / 64];
+    __shared__ float scratch[NT / 64];
+    __shared__ double dscratch[NT / 64];
     const int32_t w = (int32_t)L.w, h = (int32_t)L.h, cw = (int32_t)C.w, chh = (int32_t)C.h;
     const int32_t X0 = (int32_t)(wg % tiles_x) * T, Y0 = (int32_t)(wg / tiles_x) * T;
     const int32_t CX0 = X0 / 2 - 1, CY0 = Y0 / 2 - 1;
@@ -1715,22 +1715,22 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     auto fetch_as = [&]() {
 #pragma unroll
         for (int q = 0; q < G::kTPer; ++q) {
-            const int i = (int)threadIdx.x + 256 * q;
+            const int i = (int)threadIdx.x + NT * q;
             const int32_t x = X0 + i % T, y = Y0 + i / T;
             const bool in = i < T * T && x < w && y < h;  // (the sweep below skips the others)
             asv[q] = dot == 2 ? As[in ? (size_t)(y + 1) * w + x : 0] : 0.0f;
         }
     };
-    VRegs<T> g;
+    VRegs<T, NT> g;
     if (spec) {
-        vtile_fetch<T>(L, X0, Y0, g, true, false);
+        vtile_fetch<T, NT>(L, X0, Y0, g, true, false);
         fetch_as();
     }
     float cev[G::kEPer];
     uint32_t cin_mask = 0u;
 #pragma unroll
     for (int q = 0; q < G::kEPer; ++q) {
-        const int j = (int)threadIdx.x + 256 * q;
+        const int j = (int)threadIdx.x + NT * q;
         const int32_t cx = CX0 + j % kC, cy = CY0 + j / kC;
         const bool in = j < kC * kC && cx >= 0 && cx < cw && cy >= 0 && cy < chh;
         cin_mask |= (uint32_t)in << q;
@@ -1740,10 +1740,10 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     const bool cin = threadIdx.x < G::kCN && ccx < cw && ccy < chh;
     const uint8_t cfv = C.fluid[cin ? (size_t)ccy * cw + ccx : 0];
     const uint32_t last1 = fs ? fs->last_raw : 0u, cnt = fs ? fs->count : 0u;
-    if (spec) vtile_put<T>(g, t);  // read after the barrier of the tile test below
+    if (spec) vtile_put<T, NT>(g, t);  // read after the barrier of the tile test below
 #pragma unroll
     for (int q = 0; q < G::kEPer; ++q) {
-        const int j = (int)threadIdx.x + 256 * q;
+        const int j = (int)threadIdx.x + NT * q;
         if (j < kC * kC) ce[j] = ((cin_mask >> q) & 1u) ? cev[q] : 0.0f;
     }
     if (dn) return;
@@ -1756,9 +1756,9 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         return;
     }
     if (!spec) {
-        vtile_fetch<T>(L, X0, Y0, g, true, false);
+        vtile_fetch<T, NT>(L, X0, Y0, g, true, false);
         fetch_as();
-        vtile_put<T>(g, t);
+        vtile_put<T, NT>(g, t);
         __syncthreads();
     }
     // e plus the parent's error (prolonged) of every region cell, once, into
@@ -1793,7 +1793,7 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     double dprod = 0.0, dasz = 0.0;
 #pragma unroll
     for (int q = 0; q < G::kTPer; ++q) {
-        const int i = (int)threadIdx.x + 256 * q;
+        const int i = (int)threadIdx.x + NT * q;
         const int32_t x = X0 + i % T, y = Y0 + i / T;
         if (i >= T * T || x >= w || y >= h) continue;
         const int li = (y - Y0 + kVH) * kR + (x - X0 + kVH);
@@ -1836,12 +1836,12 @@ struct VUpArgs {
     float* e_out;         // the level's e; L.e is its down-sweep buffer
 };
 
-template <int T>
+template <int T, int NT = 256>
 struct VUp {
-    static constexpr int kThreads = 256;
+    static constexpr int kThreads = NT;
     using Args = VUpArgs;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        vup<T>(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, a.spec,
+        vup<T, NT>(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, a.spec,
                a.e_out, wg);
     }
 };
@@ -1935,6 +1935,11 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         uint32_t tx;
         return l > 0 && tiles(sim.levels[l], kBaseTile, tx) <= kFewTiles;
     };
+    // The base level of a large grid (see kWideMinRed) runs kWideTileThreads
+    // threads per tile: four tiles a CU are then 32 waves.  The FP32 partials
+    // of r.z under the reference's rules (dot 1) keep 256 threads and their
+    // summation order.
+    const bool wide_base = sim.nred >= kWideMinRed;
     // A level above the bottom with its down-sweep buffer as e: the way down
     // reads and writes it, the way up reads it and writes the level's e
     // (the bottom's levels keep theirs: one workgroup runs them)
@@ -1955,6 +1960,7 @@ void launch_vcycle(Sim& sim, int mode, int k) {
                           rel_check ? sim.part_rr_d : nullptr, rel_check ? sim.part_rr2_d : nullptr,
                           rel_check ? sim.solver.recompute_every : 0, sim.nred};
         if (sm) emit<VDown<16>>(sim, a, nt, k);
+        else if (l == 0 && wide_base) emit<VDown<kBaseTile, kWideTileThreads>>(sim, a, nt, k);
         else emit<VDown<kBaseTile>>(sim, a, nt, k);
     }
     if (sim.bottom_in_lds)
@@ -1970,6 +1976,7 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         VUpArgs a{L, sim.levels[l + 1], l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot,
                         sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d, (int)(l > 0), sim.levels[l].e};
         if (sm) emit<VUp<16>>(sim, a, nt);
+        else if (l == 0 && wide_base && dot != 1) emit<VUp<kBaseTile, kWideTileThreads>>(sim, a, nt);
         else emit<VUp<kBaseTile>>(sim, a, nt);
     }
 }
