@@ -7,6 +7,7 @@
 // max |v| for the substep dt (once per substep), and the PCG exit flag,
 // read every few iterations while the device-side state machine predicates
 // the queued iterations.
+#include <algorithm>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -667,21 +668,55 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
         if ((st = batch_upload(B, used)) != APIC_OK) return st;
         HIP_LAUNCH_TRY(issue(head, B.dev, B.stream));
         int batch_len = 0, cap = 0;
+        std::vector<int> prev;
         for (size_t i : act) {
             batch_len = std::max(batch_len, sims[i]->pcg_batch);
             cap = std::max(cap, pcg_cap(*sims[i]));
+            prev.push_back(sims[i]->pcg_batch);
         }
-        int k = 0, check_at = batch_len;
+        // Checks: where half of the solves ended in the last substep (so that
+        // the ended ones can leave the program early), where the last one did,
+        // then every two iterations.
+        std::nth_element(prev.begin(), prev.begin() + prev.size() / 2, prev.end());
+        const int median = prev[prev.size() / 2];
+        int k = 0, check_at = median > 0 && median < batch_len ? median : batch_len;
+        size_t in_program = act.size();
         while (true) {
             for (; k < check_at && k < cap; ++k) HIP_LAUNCH_TRY(issue(iter, B.dev, B.stream, k));
             HIP_LAUNCH_TRY(issue(decide, B.dev, B.stream, k));
             HIP_TRY(hipEventRecord(B.e_sync, B.stream));
             HIP_TRY(hipEventSynchronize(B.e_sync));
-            bool all = true;
+            std::vector<size_t> live;
             for (size_t i : act)
-                if (!sims[i]->h_pcg->done && k < pcg_cap(*sims[i])) all = false;
-            if (all || k >= cap) break;
-            check_at = k + 2;
+                if (!sims[i]->h_pcg->done && k < pcg_cap(*sims[i])) live.push_back(i);
+            if (live.empty() || k >= cap) break;
+            check_at = k < batch_len ? batch_len : k + 2;
+            // Once a quarter of the program's solves have ended, the iteration
+            // and check programs are recorded again for the others alone: an
+            // ended solve's workgroups would still be dispatched, each to
+            // return after one load.  The GPU is idle here, and a simulation's
+            // launches are the same in either program, so its bits are too.
+            if (live.size() * 4 <= in_program * 3) {
+                for (size_t r = 0; r < live.size(); ++r) {
+                    Sim& s = *sims[live[r]];
+                    B.rec.begin((uint32_t)r);
+                    s.rec = &B.rec;
+                    launch_pcg_iteration_head(s, 0);
+                    launch_vcycle(s, 2);
+                    s.rec = nullptr;
+                }
+                if ((st = batch_compile(B, used, iter)) != APIC_OK) return st;
+                for (size_t r = 0; r < live.size(); ++r) {
+                    Sim& s = *sims[live[r]];
+                    B.rec.begin((uint32_t)r);
+                    s.rec = &B.rec;
+                    launch_pcg_decide(s, 0);
+                    s.rec = nullptr;
+                }
+                if ((st = batch_compile(B, used, decide)) != APIC_OK) return st;
+                if ((st = batch_upload(B, used)) != APIC_OK) return st;
+                in_program = live.size();
+            }
         }
         HIP_LAUNCH_TRY(issue(tail, B.dev, B.stream));
         for (size_t i : act) {
