@@ -186,10 +186,6 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     for (auto& d : dims) {
         LevelView L{};
         L.w = d.first; L.h = d.second; L.cells = d.first * d.second;
-        size_t pa = s.padded(L.w, L.h);
-        ALLOC(L.diag, pa); ALLOC(L.inv_diag, pa); ALLOC(L.px, pa); ALLOC(L.py, pa);
-        ALLOC(L.r, pa); ALLOC(L.e, pa);
-        ALLOC(L.fluid, ((size_t)L.cells + 3) / 4 * 4);
         s.levels.push_back(L);
     }
     if (s.levels.size() > 16) return fail(APIC_ERR_UNSUPPORTED_SIZE, "more than 15 multigrid levels");
@@ -208,6 +204,40 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     if (!s.bottom_in_lds)
         for (size_t l = 1; l < s.levels.size(); ++l)
             if (s.levels[l].cells <= 4096) { s.bottom_level = l; break; }
+    // The arrays of an LDS bottom's levels lie in two blocks laid out as the
+    // bottom's LDS image (per level: diag, inv_diag, plus_x, plus_y, r, e over
+    // the padded range; the flag bytes apart), so that it copies them in and
+    // out in one stride; every other level has its own arrays.
+    const size_t lds_first = s.bottom_in_lds ? s.bottom_level : s.levels.size();
+    for (size_t l = 0; l < lds_first; ++l) {
+        LevelView& L = s.levels[l];
+        const size_t pa = s.padded(L.w, L.h);
+        ALLOC(L.diag, pa); ALLOC(L.inv_diag, pa); ALLOC(L.px, pa); ALLOC(L.py, pa);
+        ALLOC(L.r, pa); ALLOC(L.e, pa);
+        ALLOC(L.fluid, bottom_flags(L));
+    }
+    if (lds_first < s.levels.size()) {
+        size_t nf = 0, nb = 0;
+        for (size_t l = lds_first; l < s.levels.size(); ++l) {
+            nf += 6 * bottom_padded(s.levels[l]);
+            nb += bottom_flags(s.levels[l]);
+        }
+        float* img = nullptr;
+        uint8_t* flags = nullptr;
+        ALLOC(img, nf);
+        ALLOC(flags, nb);
+        for (size_t l = lds_first; l < s.levels.size(); ++l) {
+            LevelView& L = s.levels[l];
+            const size_t pa = bottom_padded(L);
+            if (img) {
+                L.diag = img; L.inv_diag = img + pa; L.px = img + 2 * pa; L.py = img + 3 * pa;
+                L.r = img + 4 * pa; L.e = img + 5 * pa;
+                L.fluid = flags;
+                img += 6 * pa;
+                flags += bottom_flags(L);
+            }
+        }
+    }
     s.down_e.assign(s.bottom_level, nullptr);
     for (size_t l = 0; l < s.bottom_level; ++l) ALLOC(s.down_e[l], s.padded(s.levels[l].w, s.levels[l].h));
     const LevelView& B = s.levels[0];

@@ -35,22 +35,26 @@ struct LevelView {
 };
 
 // The one-workgroup bottom of the V-cycle keeps its levels in LDS when they
-// fit: six float arrays over the padded range [0, cells + 2w) per level and
-// one flag byte per cell, plus the coarsest level's three CG vectors and its
-// serial-sum products.  144 KB of gfx950's 160 KB per workgroup: enough for
-// the levels from 64 x 64 cells down.
+// fit: six float arrays over the padded range per level and the flag bytes,
+// plus the coarsest level's three CG vectors and its serial-sum products.
+// 144 KB of gfx950's 160 KB per workgroup: room for the levels from 64 x 64
+// cells down (the bottom starts at kBottomFirstCells cells or fewer).
 constexpr size_t kBottomLdsFloats = 34816;
 constexpr size_t kBottomLdsFlags = 8192;
-__host__ __device__ inline size_t bottom_padded(const LevelView& L) { return (size_t)L.cells + 2 * (size_t)L.w; }
+// A level's padded array length (ceil8(cells + 2w), that of every level
+// array) and its flag bytes rounded to a multiple of 4.  The LDS image of the
+// bottom levels and their arrays in global memory share this layout.
+__host__ __device__ inline size_t bottom_padded(const LevelView& L) {
+    return ((size_t)L.cells + 2 * (size_t)L.w + 7) / 8 * 8;
+}
+__host__ __device__ inline size_t bottom_flags(const LevelView& L) { return ((size_t)L.cells + 3) / 4 * 4; }
 constexpr uint32_t kBottomFirstCells = 1024;  // cells of the one-workgroup bottom's finest level
-constexpr size_t kBottomPaddedMax = 2048;      // padded length of an LDS bottom level (two per thread)
 
 inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first) {
     size_t fl = 0, fb = 0;
     for (size_t l = first; l < lv.size(); ++l) {
-        if (bottom_padded(lv[l]) > kBottomPaddedMax) return false;
         fl += 6 * bottom_padded(lv[l]);
-        fb += lv[l].cells;
+        fb += bottom_flags(lv[l]);
     }
     fl += 3 * bottom_padded(lv.back()) + lv.back().cells;
     return fl <= kBottomLdsFloats && fb <= kBottomLdsFlags;

@@ -529,7 +529,7 @@ struct LdsViews {
         size_t fo = 0, bo = 0;
         for (int k = first; k < l; ++k) {
             fo += 6 * bottom_padded(ls->lv[k]);
-            bo += ls->lv[k].cells;
+            bo += bottom_flags(ls->lv[k]);
         }
         LevelView L = ls->lv[l];
         const size_t p = bottom_padded(L);
@@ -695,58 +695,49 @@ struct BLev {
 
 __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const int32_t* __restrict__ done) {
     __shared__ float f[kBottomLdsFloats];
-    __shared__ uint8_t fl[kBottomLdsFlags];
+    __shared__ __attribute__((aligned(16))) uint8_t fl[kBottomLdsFlags];  // copied in as words
     __shared__ float scratch[kBottomThreads / 64];
     __shared__ BLev tab[16];  // the levels' offsets, by level index
     const int32_t dn = *done;
     const int last = ls.count - 1;
     const int c = (int)threadIdx.x;
-    // Copy-in, level after level, into the image (diag, inv_diag, px, py, r,
-    // e over the padded range, flags apart): elements t and t + kBottomThreads
-    // of each array and flag t per thread, with the next level's loads in
-    // flight while a level's values are stored.
+    // Copy-in: the levels' arrays lie in global memory in the image's layout
+    // (two blocks from the first level's diag and flags), so the image is one
+    // stride of floats and one of flag words; a thread's loads all go out
+    // before its stores.
     {
-        constexpr int kPer = 2;
-        static_assert(kBottomPaddedMax <= (size_t)kPer * kBottomThreads, "padded length per thread");
-        struct Share {
-            float v[6][kPer];
-            uint8_t flag;
-        };
-        auto fetch = [&](int l, Share& sh) {
-            const LevelView& G = ls.lv[l];
-            const uint32_t p = (uint32_t)bottom_padded(G);
-            const float* src[6] = {G.diag, G.inv_diag, G.px, G.py, G.r, G.e};
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int k = 0; k < kPer; ++k) {
-                    const uint32_t i = threadIdx.x + k * kBottomThreads;
-                    sh.v[a][k] = src[a][i < p ? i : 0];
-                }
-            sh.flag = G.fluid[threadIdx.x < G.cells ? threadIdx.x : 0];
-        };
-        Share cur, nxt;
-        fetch(first, cur);
-        uint32_t o = 0, fo = 0;
+        uint32_t total = 0, ftotal = 0;
         for (int l = first; l <= last; ++l) {
-            if (l < last) fetch(l + 1, nxt);
             const LevelView& G = ls.lv[l];
-            const uint32_t p = (uint32_t)bottom_padded(G);
-#pragma unroll
-            for (int a = 0; a < 6; ++a)
-#pragma unroll
-                for (int k = 0; k < kPer; ++k) {
-                    const uint32_t i = threadIdx.x + k * kBottomThreads;
-                    if (i < p) f[o + a * p + i] = cur.v[a][k];
-                }
-            if (threadIdx.x < G.cells) fl[fo + threadIdx.x] = cur.flag;
-            if (threadIdx.x == 0)
-                tab[l] = {(int32_t)G.w, (int32_t)G.h, (int32_t)G.cells, o, o + p, o + 2 * p, o + 3 * p, o + 4 * p,
-                          o + 5 * p, fo};
-            o += 6 * p;
-            fo += G.cells;
-            cur = nxt;
+            if (threadIdx.x == 0) {
+                const uint32_t p = (uint32_t)bottom_padded(G);
+                tab[l] = {(int32_t)G.w, (int32_t)G.h, (int32_t)G.cells, total, total + p, total + 2 * p,
+                          total + 3 * p, total + 4 * p, total + 5 * p, ftotal};
+            }
+            total += 6 * (uint32_t)bottom_padded(G);
+            ftotal += (uint32_t)bottom_flags(G);
         }
+        const float* __restrict__ img = ls.lv[first].diag;
+        const uint32_t* __restrict__ fimg = reinterpret_cast<const uint32_t*>(ls.lv[first].fluid);
+        uint32_t* fw = reinterpret_cast<uint32_t*>(fl);
+        constexpr int kPer = 12;
+        for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kPer * kBottomThreads) {
+            float v[kPer];
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const uint32_t i = i0 + j * kBottomThreads;
+                v[j] = img[i < total ? i : 0];
+            }
+            uint32_t fv = 0;
+            if (i0 == threadIdx.x && threadIdx.x < ftotal / 4) fv = fimg[threadIdx.x];
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) {
+                const uint32_t i = i0 + j * kBottomThreads;
+                if (i < total) f[i] = v[j];
+            }
+            if (i0 == threadIdx.x && threadIdx.x < ftotal / 4) fw[threadIdx.x] = fv;
+        }
+        for (uint32_t i = threadIdx.x + kBottomThreads; i < ftotal / 4; i += kBottomThreads) fw[i] = fimg[i];
     }
     if (dn) return;
     LdsViews V{&ls, first, f, fl};
@@ -874,13 +865,12 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
                               f[L.e + pc - L.w]);
         __syncthreads();
     }
+    // copy-back: each level's r and e are one stride of the image
     for (int l = first; l <= last; ++l) {
-        const LevelView G = ls.lv[l];
         const BLev L = tab[l];
-        if (c < L.n) {
-            G.r[c + G.w] = f[L.r + c + L.w];
-            G.e[c + G.w] = f[L.e + c + L.w];
-        }
+        float* __restrict__ dst = ls.lv[l].r;  // e follows r
+        const uint32_t n2 = 2 * (L.e - L.r);
+        for (uint32_t i = threadIdx.x; i < n2; i += kBottomThreads) dst[i] = f[L.r + i];
     }
 }
 
