@@ -1676,6 +1676,240 @@ struct VDown {
     }
 };
 
+// ---- two levels of the way down in one launch: levels l and l + 1, both
+// above the bottom and neither the base.  A workgroup takes a kP2T x kP2T
+// tile of level l + 1 and the 2 kP2T x 2 kP2T tile of level l under it.  It
+// loads level l with a kP2FR-cell ring, so that everything the coarse tile's
+// black sweep and restriction read is computed inside the workgroup: level
+// l's black sweep out to ring kP2FR - 1, the restriction into level l + 1's
+// cells out to ring kP2CR (with their fused red sweep), level l + 1's black
+// sweep out to ring 1, and the restriction into level l + 2 under the coarse
+// tile.  Each value is computed with exactly the arithmetic of vdown, so the
+// result is that of the two launches.  Cells of level l + 1 outside its grid
+// enter only times zero coefficients (the grid's boundary couplings are zero:
+// its outermost cells are walls on the base level), so they are taken as
+// zero.  The tile is skipped as a whole when none of its level l + 1 cells
+// is fluid (both launches skip it then).
+constexpr int kP2T = 16;                    // tile side on level l + 1
+constexpr int kP2FR = 6;                    // ring on level l
+constexpr int kP2F = 2 * kP2T + 2 * kP2FR;  // region side on level l
+constexpr int kP2CR = 2;                    // ring on level l + 1
+constexpr int kP2C = kP2T + 2 * kP2CR;      // region side on level l + 1
+constexpr int kP2Threads = 512;
+
+template <int S>
+struct Region {
+    static constexpr int kN = S * S;
+    float e[kN], r[kN], id[kN], px[kN], py[kN], dg[kN];
+    uint8_t fl[kN];  // fluid flag; 0 outside the grid
+};
+
+// A thread's share of a region of side S (stride NT) between its loads and
+// its LDS stores, as VRegs for the tiles.
+template <int S, int NT>
+struct RegionRegs {
+    static constexpr int kPer = (S * S + NT - 1) / NT;
+    float r[kPer], id[kPer], px[kPer], py[kPer], dg[kPer], e[kPer];
+    uint8_t fl[kPer];
+    uint32_t inb, real;
+};
+
+// Clamped loads of the region at (X0, Y0) by padded flat index (as
+// vtile_fetch); r and e only when asked.
+template <int S, int NT>
+__device__ __forceinline__ void region_fetch(const LevelView& L, int32_t X0, int32_t Y0, RegionRegs<S, NT>& g,
+                                             bool load_r, bool load_e) {
+    const int64_t w = L.w, h = L.h, np = (int64_t)L.cells + 2 * w;
+    g.inb = g.real = 0u;
+#pragma unroll
+    for (int k = 0; k < RegionRegs<S, NT>::kPer; ++k) {
+        const int i = (int)threadIdx.x + NT * k;
+        const int32_t x = X0 + i % S, y = Y0 + i / S;
+        const int64_t p = (int64_t)(y + 1) * w + x;
+        const bool inb = i < S * S && p >= 0 && p < np;
+        const bool real = i < S * S && x >= 0 && x < w && y >= 0 && y < h;
+        g.inb |= (uint32_t)inb << k;
+        g.real |= (uint32_t)real << k;
+        const int64_t q = inb ? p : 0;
+        g.r[k] = load_r ? L.r[q] : 0.0f;
+        g.id[k] = L.inv_diag[q];
+        g.px[k] = L.px[q];
+        g.py[k] = L.py[q];
+        g.dg[k] = L.diag[q];
+        g.e[k] = load_e ? L.e[q] : 0.0f;
+        g.fl[k] = L.fluid[real ? (size_t)y * w + x : 0];
+    }
+}
+
+template <int S, int NT>
+__device__ __forceinline__ void region_put(const RegionRegs<S, NT>& g, Region<S>& t) {
+#pragma unroll
+    for (int k = 0; k < RegionRegs<S, NT>::kPer; ++k) {
+        const int i = (int)threadIdx.x + NT * k;
+        if (i >= S * S) continue;
+        const bool inb = (g.inb >> k) & 1u, real = (g.real >> k) & 1u;
+        t.r[i] = inb ? g.r[k] : 0.0f;
+        t.id[i] = inb ? g.id[k] : 0.0f;
+        t.px[i] = inb ? g.px[k] : 0.0f;
+        t.py[i] = inb ? g.py[k] : 0.0f;
+        t.dg[i] = inb ? g.dg[k] : 0.0f;
+        t.e[i] = inb ? g.e[k] : 0.0f;
+        t.fl[i] = real ? g.fl[k] : 0;
+    }
+}
+
+// sor_cell's update of region cell i of side S
+template <int S>
+__device__ __forceinline__ float rsor(const Region<S>& t, int i, float own, float er, float et, float el, float eb) {
+    const float id = t.id[i];
+    if (id != 0.0f) {
+        float sum = t.r[i];
+        sum = fmaf(-t.px[i], er, sum);
+        sum = fmaf(-t.py[i], et, sum);
+        sum = fmaf(-t.px[i - 1], el, sum);
+        sum = fmaf(-t.py[i - S], eb, sum);
+        const float gs = sum * id;
+        own = own + (kOmega * (gs - own));
+    }
+    return own;
+}
+
+// restrict_cell's sum b - A e over the four children of a coarse cell whose
+// lower-left child is region cell i0 of side S
+template <int S>
+__device__ __forceinline__ float rrestrict(const Region<S>& t, int i0) {
+    float sum = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 2; ++dx) {
+            const int li = i0 + dy * S + dx;
+            if (t.fl[li]) {
+                const float b = t.r[li];
+                float ax = t.dg[li] * t.e[li];
+                ax += t.px[li] * t.e[li + 1];
+                ax += t.py[li] * t.e[li + S];
+                ax += t.px[li - 1] * t.e[li - 1];
+                ax += t.py[li - S] * t.e[li - S];
+                sum += b - ax;
+            }
+        }
+    return sum;
+}
+
+__device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, const LevelView& D, int fuse_red_d,
+                                       const int32_t* __restrict__ done, uint32_t tiles_x, uint32_t wg) {
+    constexpr int NT = kP2Threads;
+    __shared__ Region<kP2F> tf;
+    __shared__ Region<kP2C> tc;
+    const int32_t w = (int32_t)F.w, h = (int32_t)F.h, cw = (int32_t)C.w, ch = (int32_t)C.h;
+    const int32_t dw = (int32_t)D.w, dh = (int32_t)D.h;
+    const int32_t cx0 = (int32_t)(wg % tiles_x) * kP2T, cy0 = (int32_t)(wg / tiles_x) * kP2T;
+    const int32_t FX0 = 2 * cx0 - kP2FR, FY0 = 2 * cy0 - kP2FR;  // region origins
+    const int32_t CX0 = cx0 - kP2CR, CY0 = cy0 - kP2CR;
+    // one batch of loads: the done flag, both regions, the coarse tile's
+    // children's flags and inverse diagonals on level l + 2
+    const int32_t dn = *done;
+    RegionRegs<kP2F, NT> gf;
+    RegionRegs<kP2C, NT> gc;
+    region_fetch<kP2F, NT>(F, FX0, FY0, gf, true, true);
+    region_fetch<kP2C, NT>(C, CX0, CY0, gc, false, false);
+    constexpr int kDT = kP2T / 2;
+    const int32_t ddx = (int32_t)threadIdx.x % kDT, ddy = (int32_t)threadIdx.x / kDT;
+    const int32_t dcx = cx0 / 2 + ddx, dcy = cy0 / 2 + ddy;
+    const bool din = (int)threadIdx.x < kDT * kDT && dcx < dw && dcy < dh;
+    const size_t dc = din ? (size_t)dcy * dw + dcx : 0;
+    const uint8_t dfl = D.fluid[dc];
+    const float did = D.inv_diag[dc + dw];
+    region_put<kP2F, NT>(gf, tf);
+    region_put<kP2C, NT>(gc, tc);
+    if (dn) return;
+    // skip: no fluid cell of level l + 1 in the tile
+    bool any = false;
+    if ((int)threadIdx.x < kP2T * kP2T) {
+        const int i = ((int)threadIdx.x / kP2T + kP2CR) * kP2C + (int)threadIdx.x % kP2T + kP2CR;
+        any = tc.fl[i] != 0;
+    }
+    if (!__syncthreads_or(any)) return;
+    // level l: black cells out to ring kP2FR - 1 (own value zero before the sweep)
+    for (int i = threadIdx.x; i < kP2F * kP2F; i += NT) {
+        const int lx = i % kP2F, ly = i / kP2F;
+        if (lx < 1 || lx >= kP2F - 1 || ly < 1 || ly >= kP2F - 1) continue;
+        const int32_t x = FX0 + lx, y = FY0 + ly;
+        if (!tf.fl[i] || ((x + y) & 1) == 0) continue;
+        tf.e[i] = rsor<kP2F>(tf, i, 0.0f, tf.e[i + 1], tf.e[i + kP2F], tf.e[i - 1], tf.e[i - kP2F]);
+    }
+    __syncthreads();
+    // level l + 1 out to ring kP2CR: restriction with the fused red sweep
+    for (int i = threadIdx.x; i < kP2C * kP2C; i += NT) {
+        const int lx = i % kP2C, ly = i / kP2C;
+        const int32_t cx = CX0 + lx, cy = CY0 + ly;
+        float sum = 0.0f, e = 0.0f;
+        if (cx >= 0 && cx < cw && cy >= 0 && cy < ch && tc.fl[i]) {
+            sum = rrestrict<kP2F>(tf, (2 * ly + kP2FR - 2 * kP2CR) * kP2F + 2 * lx + kP2FR - 2 * kP2CR);
+            const float id = tc.id[i];
+            const bool red = ((cx + cy) & 1) == 0;
+            e = (red && id != 0.0f) ? kOmega * (sum * id) : 0.0f;
+        }
+        tc.r[i] = sum;
+        tc.e[i] = e;
+    }
+    __syncthreads();
+    // level l + 1: black cells of the tile and its one-cell ring
+    for (int i = threadIdx.x; i < kP2C * kP2C; i += NT) {
+        const int lx = i % kP2C, ly = i / kP2C;
+        if (lx < 1 || lx >= kP2C - 1 || ly < 1 || ly >= kP2C - 1) continue;
+        const int32_t cx = CX0 + lx, cy = CY0 + ly;
+        if (!tc.fl[i] || ((cx + cy) & 1) == 0) continue;
+        tc.e[i] = rsor<kP2C>(tc, i, 0.0f, tc.e[i + 1], tc.e[i + kP2C], tc.e[i - 1], tc.e[i - kP2C]);
+    }
+    // level l: the tile's black cells (its red cells are already in memory)
+    for (int i = threadIdx.x; i < 4 * kP2T * kP2T; i += NT) {
+        const int32_t x = 2 * cx0 + i % (2 * kP2T), y = 2 * cy0 + i / (2 * kP2T);
+        if (x >= w || y >= h || ((x + y) & 1) == 0) continue;
+        const int li = (y - FY0) * kP2F + (x - FX0);
+        F.e[(size_t)(y + 1) * w + x] = tf.e[li];
+    }
+    __syncthreads();
+    // level l + 1: the tile's residual and error
+    for (int i = threadIdx.x; i < kP2T * kP2T; i += NT) {
+        const int32_t cx = cx0 + i % kP2T, cy = cy0 + i / kP2T;
+        if (cx >= cw || cy >= ch) continue;
+        const int li = (cy - CY0) * kP2C + (cx - CX0);
+        const size_t pc = (size_t)(cy + 1) * cw + cx;
+        C.r[pc] = tc.r[li];
+        C.e[pc] = tc.e[li];
+    }
+    // level l + 2 under the tile (restrict_cell with its fused red sweep)
+    if (din) {
+        const size_t pdc = dc + dw;
+        if (!dfl) {
+            D.r[pdc] = 0.0f;
+            D.e[pdc] = 0.0f;
+        } else {
+            const float sum = rrestrict<kP2C>(tc, (2 * dcy - CY0) * kP2C + 2 * dcx - CX0);
+            D.r[pdc] = sum;
+            const bool red = ((dcx + dcy) & 1) == 0;
+            D.e[pdc] = (fuse_red_d && red && did != 0.0f) ? kOmega * (sum * did) : 0.0f;
+        }
+    }
+}
+
+struct VDown2Args {
+    LevelView F, C, D;  // levels l, l + 1, l + 2, each with its down-sweep error
+    int fuse_red_d;
+    const int32_t* done;
+    uint32_t tiles_x;  // tiles of level l + 1 in a row
+};
+
+struct VDown2 {
+    static constexpr int kThreads = kP2Threads;
+    using Args = VDown2Args;
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        vdown2(a.F, a.C, a.D, a.fuse_red_d, a.done, a.tiles_x, wg);
+    }
+};
+
 // Up: the two colour sweeps of smooth_level's reverse pass on level L with
 // C's error prolongated on the fly (prolongate + sor_cell modes 1 and 2); on
 // the base level, with the padded-list repeats of prolongate and the partials
@@ -1939,6 +2173,19 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         return v;
     };
     for (int l = 0; l < bottom; ++l) {
+        // Two levels of few tiles above the bottom (not the base): one launch
+        // when the simulation steps alone, where launches are latency-bound.
+        // A batch's launches are bound by throughput, which the pair's
+        // recomputed ring costs more than the launch saves; the pair computes
+        // the bits of the two launches, so either path gives the same result.
+        if (!sim.rec && l >= 1 && l + 1 < bottom && small(l)) {
+            uint32_t tx;
+            const uint32_t nt = tiles(sim.levels[l + 1], kP2T, tx);
+            const int fuse_red_d = l + 2 < bottom ? 1 : (int)(bottom < last);
+            emit<VDown2>(sim, VDown2Args{down(l), down(l + 1), down(l + 2), fuse_red_d, done, tx}, nt, k);
+            ++l;
+            continue;
+        }
         const LevelView F = down(l);
         const int fuse_red = l + 1 < bottom ? 1 : (int)(bottom < last);
         const bool sm = small(l);
