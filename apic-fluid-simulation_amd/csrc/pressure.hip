@@ -1758,6 +1758,22 @@ __device__ __forceinline__ void region_put(const RegionRegs<S, NT>& g, Region<S>
     }
 }
 
+// Whether one of the thread's fetched cells of a region of side S is a fluid
+// cell of the square [lo, lo + T)^2 of region coordinates (from its own
+// registers: the region's LDS copy is complete only after a barrier).
+template <int S, int NT, int T>
+__device__ __forceinline__ bool region_any_fluid(const RegionRegs<S, NT>& g, int lo) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < RegionRegs<S, NT>::kPer; ++k) {
+        const int i = (int)threadIdx.x + NT * k;
+        const int lx = i % S, ly = i / S;
+        const bool tile = i < S * S && lx >= lo && lx < lo + T && ly >= lo && ly < lo + T;
+        any = any || (tile && ((g.real >> k) & 1u) && g.fl[k] != 0);
+    }
+    return any;
+}
+
 // sor_cell's update of region cell i of side S
 template <int S>
 __device__ __forceinline__ float rsor(const Region<S>& t, int i, float own, float er, float et, float el, float eb) {
@@ -1825,12 +1841,7 @@ __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, c
     region_put<kP2C, NT>(gc, tc);
     if (dn) return;
     // skip: no fluid cell of level l + 1 in the tile
-    bool any = false;
-    if ((int)threadIdx.x < kP2T * kP2T) {
-        const int i = ((int)threadIdx.x / kP2T + kP2CR) * kP2C + (int)threadIdx.x % kP2T + kP2CR;
-        any = tc.fl[i] != 0;
-    }
-    if (!__syncthreads_or(any)) return;
+    if (!__syncthreads_or(region_any_fluid<kP2C, NT, kP2T>(gc, kP2CR))) return;
     // level l: black cells out to ring kP2FR - 1 (own value zero before the sweep)
     for (int i = threadIdx.x; i < kP2F * kP2F; i += NT) {
         const int lx = i % kP2F, ly = i / kP2F;
@@ -1907,6 +1918,144 @@ struct VDown2 {
     using Args = VDown2Args;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
         vdown2(a.F, a.C, a.D, a.fuse_red_d, a.done, a.tiles_x, wg);
+    }
+};
+
+// ---- two levels of the way up in one launch: level l + 1 with level l + 2's
+// final error prolongated, then level l with level l + 1's.  A workgroup
+// takes a kP2T x kP2T tile of level l + 1 and the tile of level l under it.
+// Level l + 1 is loaded with a three-cell ring: its black sweep out to ring 2
+// and its red sweep out to ring 1 give the final error of every cell whose
+// value level l's region prolongates; level l then runs as in vup.  Every
+// value is computed with the arithmetic of vup, so the result is that of the
+// two launches; the tile is skipped as a whole when none of its level l + 1
+// cells is fluid (both launches skip it then).
+constexpr int kU2CR = 3;                    // ring on level l + 1
+constexpr int kU2C = kP2T + 2 * kU2CR;      // its region side
+constexpr int kU2D = kP2T / 2 + 4;          // level l + 2 cells across that region
+constexpr int kU2F = 2 * kP2T + 2 * kVH;    // level l's region side (ring kVH)
+
+__device__ __forceinline__ void vup2(const LevelView& F, const LevelView& C, const LevelView& D,
+                                     float* __restrict__ f_out, float* __restrict__ c_out,
+                                     const int32_t* __restrict__ done, uint32_t tiles_x, uint32_t wg) {
+    constexpr int NT = kP2Threads;
+    __shared__ Region<kU2F> tf;
+    __shared__ Region<kU2C> tc;
+    __shared__ float de[kU2D * kU2D];
+    const int32_t w = (int32_t)F.w, h = (int32_t)F.h, cw = (int32_t)C.w, ch = (int32_t)C.h;
+    const int32_t dw = (int32_t)D.w, dh = (int32_t)D.h;
+    const int32_t cx0 = (int32_t)(wg % tiles_x) * kP2T, cy0 = (int32_t)(wg / tiles_x) * kP2T;
+    const int32_t CX0 = cx0 - kU2CR, CY0 = cy0 - kU2CR;        // level l + 1 region
+    const int32_t DX0 = CX0 >> 1, DY0 = CY0 >> 1;                // level l + 2 parents
+    const int32_t FX0 = 2 * cx0 - kVH, FY0 = 2 * cy0 - kVH;    // level l region
+    // one batch of loads: the done flag, both regions (down-sweep errors) and
+    // level l + 2's final error under level l + 1's region
+    const int32_t dn = *done;
+    RegionRegs<kU2F, NT> gf;
+    RegionRegs<kU2C, NT> gc;
+    region_fetch<kU2F, NT>(F, FX0, FY0, gf, true, true);
+    region_fetch<kU2C, NT>(C, CX0, CY0, gc, true, true);
+    constexpr int kDPer = (kU2D * kU2D + NT - 1) / NT;
+    float dv[kDPer];
+    uint32_t dmask = 0u;
+#pragma unroll
+    for (int q = 0; q < kDPer; ++q) {
+        const int j = (int)threadIdx.x + NT * q;
+        const int32_t dx = DX0 + j % kU2D, dy = DY0 + j / kU2D;
+        const bool in = j < kU2D * kU2D && dx >= 0 && dx < dw && dy >= 0 && dy < dh;
+        dmask |= (uint32_t)in << q;
+        dv[q] = D.e[in ? (size_t)(dy + 1) * dw + dx : 0];
+    }
+    region_put<kU2F, NT>(gf, tf);
+    region_put<kU2C, NT>(gc, tc);
+#pragma unroll
+    for (int q = 0; q < kDPer; ++q) {
+        const int j = (int)threadIdx.x + NT * q;
+        if (j < kU2D * kU2D) de[j] = ((dmask >> q) & 1u) ? dv[q] : 0.0f;
+    }
+    if (dn) return;
+    // skip: no fluid cell of level l + 1 in the tile
+    if (!__syncthreads_or(region_any_fluid<kU2C, NT, kP2T>(gc, kU2CR))) return;
+    // level l + 1: e plus the parent's error over the region (the diagonal's slot)
+    for (int i = threadIdx.x; i < kU2C * kU2C; i += NT) {
+        float v = tc.e[i];
+        if (tc.fl[i]) {
+            const int32_t x = CX0 + i % kU2C, y = CY0 + i / kU2C;
+            v = v + de[((y >> 1) - DY0) * kU2D + ((x >> 1) - DX0)];
+        }
+        tc.dg[i] = v;
+    }
+    __syncthreads();
+    // level l + 1: black cells out to ring 2
+    for (int i = threadIdx.x; i < kU2C * kU2C; i += NT) {
+        const int lx = i % kU2C, ly = i / kU2C;
+        if (lx < 1 || lx >= kU2C - 1 || ly < 1 || ly >= kU2C - 1) continue;
+        const int32_t x = CX0 + lx, y = CY0 + ly;
+        if (!tc.fl[i] || ((x + y) & 1) == 0) continue;
+        tc.e[i] = rsor<kU2C>(tc, i, tc.dg[i], tc.dg[i + 1], tc.dg[i + kU2C], tc.dg[i - 1], tc.dg[i - kU2C]);
+    }
+    __syncthreads();
+    // level l + 1: red cells out to ring 1, written in place (red cells read
+    // black neighbours only); non-fluid cells keep e
+    for (int i = threadIdx.x; i < kU2C * kU2C; i += NT) {
+        const int lx = i % kU2C, ly = i / kU2C;
+        if (lx < 2 || lx >= kU2C - 2 || ly < 2 || ly >= kU2C - 2) continue;
+        const int32_t x = CX0 + lx, y = CY0 + ly;
+        if (!tc.fl[i] || ((x + y) & 1) != 0) continue;
+        tc.e[i] = rsor<kU2C>(tc, i, tc.dg[i], tc.e[i + 1], tc.e[i + kU2C], tc.e[i - 1], tc.e[i - kU2C]);
+    }
+    __syncthreads();
+    // level l + 1: the tile's final error
+    for (int i = threadIdx.x; i < kP2T * kP2T; i += NT) {
+        const int32_t x = cx0 + i % kP2T, y = cy0 + i / kP2T;
+        if (x >= cw || y >= ch) continue;
+        c_out[(size_t)(y + 1) * cw + x] = tc.e[(y - CY0) * kU2C + (x - CX0)];
+    }
+    // level l: e plus the parent's final error (the parent of a fluid cell is
+    // a cell of the grid, within ring 1 of the level l + 1 tile)
+    for (int i = threadIdx.x; i < kU2F * kU2F; i += NT) {
+        float v = tf.e[i];
+        if (tf.fl[i]) {
+            const int32_t x = FX0 + i % kU2F, y = FY0 + i / kU2F;
+            v = v + tc.e[((y >> 1) - CY0) * kU2C + ((x >> 1) - CX0)];
+        }
+        tf.dg[i] = v;
+    }
+    __syncthreads();
+    // level l: black cells of the tile and its one-cell ring
+    for (int i = threadIdx.x; i < kU2F * kU2F; i += NT) {
+        const int lx = i % kU2F, ly = i / kU2F;
+        if (lx < 1 || lx >= kU2F - 1 || ly < 1 || ly >= kU2F - 1) continue;
+        const int32_t x = FX0 + lx, y = FY0 + ly;
+        if (!tf.fl[i] || ((x + y) & 1) == 0) continue;
+        tf.e[i] = rsor<kU2F>(tf, i, tf.dg[i], tf.dg[i + 1], tf.dg[i + kU2F], tf.dg[i - 1], tf.dg[i - kU2F]);
+    }
+    __syncthreads();
+    // level l: red cells of the tile, then the tile's final error
+    for (int i = threadIdx.x; i < 4 * kP2T * kP2T; i += NT) {
+        const int32_t x = 2 * cx0 + i % (2 * kP2T), y = 2 * cy0 + i / (2 * kP2T);
+        if (x >= w || y >= h) continue;
+        const int li = (y - FY0) * kU2F + (x - FX0);
+        float e = tf.e[li];
+        if (tf.fl[li] && ((x + y) & 1) == 0)
+            e = rsor<kU2F>(tf, li, tf.dg[li], tf.e[li + 1], tf.e[li + kU2F], tf.e[li - 1], tf.e[li - kU2F]);
+        f_out[(size_t)(y + 1) * w + x] = e;
+    }
+}
+
+struct VUp2Args {
+    LevelView F, C, D;  // levels l, l + 1 (their down-sweep errors) and l + 2 (its final error)
+    float* f_out;       // the final errors of levels l and l + 1
+    float* c_out;
+    const int32_t* done;
+    uint32_t tiles_x;  // tiles of level l + 1 in a row
+};
+
+struct VUp2 {
+    static constexpr int kThreads = kP2Threads;
+    using Args = VUp2Args;
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        vup2(a.F, a.C, a.D, a.f_out, a.c_out, a.done, a.tiles_x, wg);
     }
 };
 
@@ -2205,6 +2354,16 @@ void launch_vcycle(Sim& sim, int mode, int k) {
     else
         emit<BottomGlobal>(sim, {level_set(sim), bottom, sim.cg_s, sim.cg_ap, sim.cg_r, done}, 1);
     for (int l = bottom - 1; l >= 0; --l) {
+        // levels l - 1 and l in one launch, as on the way down
+        if (!sim.rec && l - 1 >= 1 && small(l - 1)) {
+            uint32_t tx;
+            const uint32_t nt = tiles(sim.levels[l], kP2T, tx);
+            emit<VUp2>(sim, VUp2Args{down(l - 1), down(l), sim.levels[l + 1], sim.levels[l - 1].e, sim.levels[l].e,
+                                     done, tx},
+                       nt);
+            --l;
+            continue;
+        }
         const LevelView L = down(l);
         const bool sm = small(l);
         uint32_t tx;
