@@ -242,7 +242,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     for (size_t l = 0; l < s.bottom_level; ++l) ALLOC(s.down_e[l], s.padded(s.levels[l].w, s.levels[l].h));
     const LevelView& B = s.levels[0];
     size_t pb = s.padded(B.w, B.h);
-    ALLOC(s.search[0], pb); ALLOC(s.search[1], pb); ALLOC(s.As, pb); ALLOC(s.rhs, pb);
+    ALLOC(s.search[0], pb); ALLOC(s.search[1], pb); ALLOC(s.As, pb); ALLOC(s.rhs, pb); ALLOC(s.r_alt, pb);
     const LevelView& C = s.levels.back();
     size_t pcs = s.padded(C.w, C.h);
     ALLOC(s.cg_s, pcs); ALLOC(s.cg_ap, pcs); ALLOC(s.cg_r, pcs);
@@ -250,7 +250,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     const size_t ngam = std::max<size_t>(kRedBlocks, gamma_partials(s));
     ALLOC(s.part_gamma, ngam); ALLOC(s.part_delta, kRedBlocks); ALLOC(s.part_max, kRedBlocks);
     ALLOC(s.part_fluid, 2 * kRedBlocks);
-    ALLOC(s.part_gamma_d, ngam); ALLOC(s.part_asz_d, ngam);
+    ALLOC(s.part_gamma_d, ngam); ALLOC(s.part_asz_d, ngam); ALLOC(s.part_rr_t, ngam);
     ALLOC(s.part_delta_d, kRedBlocks); ALLOC(s.part_rr_d, kRedBlocks);
     ALLOC(s.part_rr2_d, kRedBlocks); ALLOC(s.pressure_d, nc);
     s.part_maxv_cap = 1024;

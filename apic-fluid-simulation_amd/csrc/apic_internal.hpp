@@ -74,7 +74,7 @@ struct PcgState {
     int32_t done;         // nonzero: every kernel of the PCG loop returns at once
     int32_t iters;        // iterations entered
     int32_t exit_reason;  // as apic_step_stats.pcg_exit_reason
-    int32_t pad0;
+    int32_t r_in_alt;     // the solve ended with the base residual in Sim::r_alt (copied home by the up-sweep)
     float maxerr, delta, alpha, pad1;
     float gamma[kMaxPcgIterations + 4];  // gamma[k]: the r.z used by iteration k
     // relative mode (apic_solver_config.mode 1): FP64 scalars
@@ -203,13 +203,15 @@ struct Sim {
     float* As = nullptr;
     float* cg_s = nullptr; float* cg_ap = nullptr; float* cg_r = nullptr;  // coarsest level
     float* rhs = nullptr;            // copy of b for the relative mode's residual recompute
+    float* r_alt = nullptr;          // the base residual's second buffer (the fused update writes it)
     PcgState* pcg = nullptr;
     float* part_gamma = nullptr;     // kRedBlocks partials of r.z
     float* part_delta = nullptr;     // kRedBlocks partials of s.As
     float* part_max = nullptr;       // kRedBlocks partials of max |r|
     double* part_gamma_d = nullptr;  // relative mode: FP64 partials of r.z, s.As and |r|^2
     double* part_delta_d = nullptr;
-    double* part_rr_d = nullptr;     // ||r||^2 partials of the recurrence residual
+    double* part_rr_d = nullptr;     // ||r||^2 partials of ||b||^2 (the relative mode's prologue)
+    double* part_rr_t = nullptr;     // ||r||^2 partials of the recurrence residual, one per base tile
     double* part_rr2_d = nullptr;    // ... after the true-residual check
     double* part_asz_d = nullptr;    // relative mode: partials of (A s).z for the flexible beta
     double* pressure_d = nullptr;    // relative mode: the pressure accumulated in FP64

@@ -899,6 +899,7 @@ struct PcgReset {
         st->done = 0;
         st->iters = 0;
         st->exit_reason = 0;
+        st->r_in_alt = 0;
         st->maxerr = st->delta = st->alpha = 0.0f;
         st->bnorm2 = st->rnorm2 = 0.0;
         st->tol2 = a.tol2;
@@ -1216,16 +1217,16 @@ struct SpmvDot {
     }
 };
 
-// delta from the partials (exit when |delta| < 1e-5, src/apic.rs:1903), then
-// p += alpha*s; r = fma(-alpha, As, r); partials of max |r| (:1907-1928)
+// The reference's rules (FP32): delta from the partials (exit when |delta| <
+// 1e-5, src/apic.rs:1903), then p += alpha*s; r = fma(-alpha, As, r);
+// partials of max |r| (:1907-1928).  The relative mode's update is
+// axpy_tile.
 template <int NT>
 __device__ __forceinline__ void axpy_max(const LevelView& L, const float* __restrict__ s, const float* __restrict__ As,
                                 float* __restrict__ pressure, PcgState* st, const float* __restrict__ part_delta,
-                                float* __restrict__ part_max, int k, const double* __restrict__ part_delta_d,
-                                double* __restrict__ part_rr_d, double* __restrict__ pd, uint32_t wg, uint32_t nwg) {
+                                float* __restrict__ part_max, int k, uint32_t wg, uint32_t nwg) {
     const bool lead = wg == 0 && threadIdx.x == 0;
     __shared__ float scratch[NT / 64];
-    __shared__ double dscratch[NT / 64];
     if (st->done) return;
     const uint32_t cells = L.cells, stride = nwg * NT;
     const size_t w = L.w;
@@ -1239,49 +1240,6 @@ __device__ __forceinline__ void axpy_max(const LevelView& L, const float* __rest
         }
     };
     fetch_flags();  // with the partials
-    if (part_delta_d) {  // relative mode: alpha = r.z / s.As in FP64, partials of ||r||^2
-        const double d = sum_partials256(part_delta_d, (int)nwg, dscratch);
-        if (!(d > 0.0)) {
-            if (lead) { st->done = 1; st->exit_reason = 2; st->delta = (float)d; }
-            return;
-        }
-        const double alpha_d = st->gamma_d[k & 1] / d;
-        const float alpha = (float)alpha_d;
-        if (lead) { st->delta = (float)d; st->alpha = alpha; st->alpha_d = alpha_d; }
-        double rr = 0.0;
-        for (; c0 < cells; c0 += kWideBatch * stride) {
-            double p[kWideBatch];
-            float sv[kWideBatch], av[kWideBatch], rv[kWideBatch];
-            uint8_t fl_now[kWideBatch];
-#pragma unroll
-            for (int j = 0; j < kWideBatch; ++j) {
-                const uint32_t c = fl[j] ? c0 + j * stride : 0;
-                p[j] = pd[c];
-                sv[j] = s[c + w];
-                av[j] = As[c + w];
-                rv[j] = L.r[c + w];
-                fl_now[j] = fl[j];
-            }
-            c0 += kWideBatch * stride;
-            fetch_flags();  // the next batch's, in flight during this one
-            c0 -= kWideBatch * stride;
-#pragma unroll
-            for (int j = 0; j < kWideBatch; ++j) {
-                if (!fl_now[j]) continue;
-                const uint32_t c = c0 + j * stride;
-                const size_t pc = (size_t)c + w;
-                const double np = p[j] + alpha_d * (double)sv[j];
-                pd[c] = np;
-                pressure[c] = (float)np;
-                const float nr = fmaf(-alpha, av[j], rv[j]);
-                L.r[pc] = nr;
-                rr += (double)nr * (double)nr;
-            }
-        }
-        rr = block_sum(rr, dscratch);
-        if (threadIdx.x == 0) part_rr_d[wg] = rr;
-        return;
-    }
     const float d = sum_partials256(part_delta, (int)nwg, scratch);
     if (fabsf(d) < kSafety) {
         if (lead) { st->done = 1; st->exit_reason = 2; st->delta = d; }
@@ -1303,7 +1261,7 @@ __device__ __forceinline__ void axpy_max(const LevelView& L, const float* __rest
             fl_now[j] = fl[j];
         }
         c0 += kWideBatch * stride;
-        fetch_flags();
+        fetch_flags();  // the next batch's, in flight during this one
         c0 -= kWideBatch * stride;
 #pragma unroll
         for (int j = 0; j < kWideBatch; ++j) {
@@ -1328,9 +1286,6 @@ struct AxpyMaxArgs {
     PcgState* st;
     const float* part_delta;
     float* part_max;
-    const double* part_delta_d;  // relative mode (else null)
-    double* part_rr_d;
-    double* pd;
     uint32_t nwg;
 };
 
@@ -1339,8 +1294,7 @@ struct AxpyMax {
     static constexpr int kThreads = NT;
     using Args = AxpyMaxArgs;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
-        axpy_max<NT>(a.L, a.search[(k + 1) & 1], a.As, a.pressure, a.st, a.part_delta, a.part_max, k, a.part_delta_d,
-                     a.part_rr_d, a.pd, wg, a.nwg);
+        axpy_max<NT>(a.L, a.search[(k + 1) & 1], a.As, a.pressure, a.st, a.part_delta, a.part_max, k, wg, a.nwg);
     }
 };
 
@@ -1496,16 +1450,116 @@ __device__ inline float vsor(const VTile<T>& t, int i, float own, float er, floa
     return own;
 }
 
+// alpha = gamma / delta of iteration k in FP64 (src/apic.rs:1903-1907; the
+// relative mode ends the solve when delta <= 0); false when the solve ends.
+__device__ __forceinline__ bool pcg_alpha(PcgState* st, double d, double gam, bool lead, double& alpha_d) {
+    if (!(d > 0.0)) {
+        if (lead) { st->done = 1; st->exit_reason = 2; st->delta = (float)d; }
+        return false;
+    }
+    alpha_d = gam / d;
+    if (lead) { st->delta = (float)d; st->alpha = (float)alpha_d; st->alpha_d = alpha_d; }
+    return true;
+}
+
+// Iteration k's update in the relative mode (src/apic.rs:1907-1928 with FP64
+// scalars and pressure): p += alpha s, r = fma(-alpha, As, r) on the fluid
+// cells, and the partials of ||r||^2, one per 32 x 32 base tile: a workgroup
+// of NT threads takes one tile, thread t its cells t, t + NT, ... in turn,
+// and the workgroup adds its threads' sums with block_sum.  The base
+// down-sweep that fuses the update (VDownAx, the same NT) forms the same
+// partials in the same order, so both paths give the same residual norm.
+struct AxIn {
+    float* search[2];
+    const float* As;
+    float* pressure;
+    double* pd;                 // the FP64 pressure
+    const double* part_delta_d; // SpmvDot's partials of delta
+    uint32_t ndelta;
+    float* r_out;               // the updated residual (the old one is read from the level's r)
+    double* part_rr_out;        // one partial of ||r||^2 per tile
+};
+
+template <int NT>
+__device__ __forceinline__ void axpy_tile(const LevelView& L, const AxIn& ax, PcgState* st, uint32_t tiles_x, int k,
+                                          uint32_t wg) {
+    constexpr int T = kBaseTile, kPer = (T * T + NT - 1) / NT;
+    __shared__ double dscratch[NT / 64];
+    const bool lead = wg == 0 && threadIdx.x == 0;
+    const int32_t w = (int32_t)L.w, h = (int32_t)L.h;
+    const int32_t X0 = (int32_t)(wg % tiles_x) * T, Y0 = (int32_t)(wg / tiles_x) * T;
+    const float* __restrict__ s = ax.search[(k + 1) & 1];
+    const int32_t dn = st->done;
+    const double gam = st->gamma_d[k & 1];
+    uint8_t fl[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int i = (int)threadIdx.x + NT * q;
+        const int32_t x = X0 + i % T, y = Y0 + i / T;
+        const bool in = i < T * T && x < w && y < h;
+        const uint8_t f = L.fluid[in ? (size_t)y * w + x : 0];
+        fl[q] = in ? f : 0;
+    }
+    const double d = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
+    if (dn) return;
+    double alpha_d;
+    if (!pcg_alpha(st, d, gam, lead, alpha_d)) return;
+    const float alpha = (float)alpha_d;
+    double pv[kPer];
+    float sv[kPer], av[kPer], rv[kPer];
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        const int i = (int)threadIdx.x + NT * q;
+        const size_t c = fl[q] ? (size_t)(Y0 + i / T) * w + X0 + i % T : 0;
+        pv[q] = ax.pd[c];
+        sv[q] = s[c + w];
+        av[q] = ax.As[c + w];
+        rv[q] = L.r[c + w];
+    }
+    double rr = 0.0;
+#pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+        if (!fl[q]) continue;
+        const int i = (int)threadIdx.x + NT * q;
+        const size_t c = (size_t)(Y0 + i / T) * w + X0 + i % T;
+        const double np = pv[q] + alpha_d * (double)sv[q];
+        ax.pd[c] = np;
+        ax.pressure[c] = (float)np;
+        const float nr = fmaf(-alpha, av[q], rv[q]);
+        ax.r_out[c + w] = nr;
+        rr += (double)nr * (double)nr;
+    }
+    rr = block_sum(rr, dscratch);
+    if (threadIdx.x == 0) ax.part_rr_out[wg] = rr;
+}
+
+struct AxpyTileArgs {
+    LevelView L;  // the base level (r read and written in place)
+    AxIn ax;
+    PcgState* st;
+    uint32_t tiles_x;
+};
+
+template <int NT>
+struct AxpyTile {
+    static constexpr int kThreads = NT;
+    using Args = AxpyTileArgs;
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
+        axpy_tile<NT>(a.L, a.ax, a.st, a.tiles_x, k, wg);
+    }
+};
+
 // Down: (base: e = 0 and the first red sweep) the black sweep of level F,
 // then the restriction of b - A e into C fused with C's first red sweep.  On
 // the base level, in a PCG iteration, every workgroup first applies the
 // residual test of the previous update (max |r| < solve_error,
 // src/apic.rs:1930, or the relative test).
-template <int T, int NT>
+template <int T, int NT, bool AX = false>
 __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, int base, int fuse_red, int spec,
                              const int32_t* __restrict__ done, uint32_t tiles_x, PcgState* st, int check,
                              float solve_error, const float* __restrict__ part_max, const double* __restrict__ part_rr,
-                             uint32_t nred, uint32_t wg) {
+                             const double* __restrict__ part_rr_t, uint32_t ntiles, uint32_t nred, const AxIn& ax,
+                             int k, uint32_t wg) {
     using G = VGeom<T, NT>;
     constexpr int kR = G::kR;
     __shared__ VTile<T> t;
@@ -1517,10 +1571,11 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     const int32_t cw = (int32_t)C.w, ch = (int32_t)C.h;
     // Loads that do not depend on each other go out together, so that the
     // workgroup waits for them once: the done flag, the coarse cells' flags
-    // and inverse diagonals, the residual test's partials (nred <=
-    // kRedBlocks) and, with spec, the region itself before it is known
-    // whether the tile has work (a coarse level has few tiles: its time is the
-    // latency of these loads, not their bytes).
+    // and inverse diagonals, the residual test's partials and scalars, with
+    // spec the region itself before it is known whether the tile has work (a
+    // coarse level has few tiles: its time is the latency of these loads, not
+    // their bytes), and with AX the flags of the tile and its region and the
+    // partials of delta.
     const int32_t dn = *done;
     VRegs<T, NT> g;
     if (spec) vtile_fetch<T, NT>(F, X0, Y0, g, !base, true);
@@ -1546,18 +1601,50 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         prr[q] = 0.0;
         pmx[q] = 0.0f;
         if (check && part_rr) prr[q] = part_rr[ic];
-        else if (check) pmx[q] = part_max[ic];
+        else if (check && !part_rr_t) pmx[q] = part_max[ic];
     }
+    const bool rel_test = check && (part_rr || part_rr_t);
+    const double tol2 = rel_test ? st->tol2 : 0.0, bn2 = rel_test ? st->bnorm2 : 0.0;
+    double gam = 0.0;
+    uint8_t ofl[G::kTPer], rfl[G::kPer];
+    if constexpr (AX) {
+        gam = st->gamma_d[k & 1];
+#pragma unroll
+        for (int q = 0; q < G::kTPer; ++q) {
+            const int i = (int)threadIdx.x + NT * q;
+            const int32_t x = X0 + i % T, y = Y0 + i / T;
+            const bool in = i < T * T && x < w && y < h;
+            const uint8_t f = F.fluid[in ? (size_t)y * w + x : 0];
+            ofl[q] = in ? f : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < G::kPer; ++q) {
+            const int i = (int)threadIdx.x + NT * q;
+            const int32_t x = X0 - kVH + i % kR, y = Y0 - kVH + i / kR;
+            const bool real = i < G::kN && x >= 0 && x < w && y >= 0 && y < h;
+            const uint8_t f = F.fluid[real ? (size_t)y * w + x : 0];
+            rfl[q] = real ? f : 0;
+        }
+    }
+    const double rr_t = (check && part_rr_t) ? sum_partials256(part_rr_t, (int)ntiles, dscratch) : 0.0;
+    double dsum = 0.0;
+    if constexpr (AX) dsum = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
     if (spec) vtile_put<T, NT>(g, t);  // read after the barrier of the tile test below
     if (dn) return;
-    if (check && part_rr) {
+    if (check && part_rr_t) {  // the update's partials, one per base tile
+        if (lead) st->rnorm2 = rr_t;
+        if (rr_t <= tol2 * bn2) {
+            if (lead) { st->done = 1; st->exit_reason = 3; }
+            return;
+        }
+    } else if (check && part_rr) {  // the true residual's
         double rr = 0.0;
 #pragma unroll
         for (int q = 0; q < kRPer; ++q)
             if (threadIdx.x + (uint32_t)NT * q < nred) rr += prr[q];
         rr = block_sum(rr, dscratch);
         if (lead) st->rnorm2 = rr;
-        if (rr <= st->tol2 * st->bnorm2) {
+        if (rr <= tol2 * bn2) {
             if (lead) { st->done = 1; st->exit_reason = 3; }
             return;
         }
@@ -1576,14 +1663,69 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     // A tile without a fluid cell (none of its coarse cells is fluid) has
     // nothing to compute: the values of non-fluid cells are only ever read
     // times zero coefficients, so it leaves them as they are.
+    double alpha_d = 0.0;
+    if constexpr (AX) {
+        if (!pcg_alpha(st, dsum, gam, lead, alpha_d)) return;
+    }
     bool any = false;
 #pragma unroll
     for (int q = 0; q < G::kCPer; ++q) any = any || (cin[q] && cfl[q] != 0);
-    if (!__syncthreads_or(any)) return;
+    if (!__syncthreads_or(any)) {
+        if constexpr (AX) {  // no fluid cell in the tile: a zero partial, as axpy_tile's
+            if (threadIdx.x == 0) ax.part_rr_out[wg] = 0.0;
+        }
+        return;
+    }
     if (!spec) {
         vtile_fetch<T, NT>(F, X0, Y0, g, !base, true);
-        vtile_put<T, NT>(g, t);
-        __syncthreads();
+        if constexpr (AX) {
+            // the update's operands: A s over the region's fluid cells, and
+            // the tile's fluid cells' p, s, r and A s (non-fluid slots read
+            // cell 0)
+            const float* __restrict__ sv_in = ax.search[(k + 1) & 1];
+            float asr[G::kPer];
+#pragma unroll
+            for (int q = 0; q < G::kPer; ++q) {
+                const int i = (int)threadIdx.x + NT * q;
+                const int32_t x = X0 - kVH + i % kR, y = Y0 - kVH + i / kR;
+                asr[q] = ax.As[rfl[q] ? (size_t)(y + 1) * w + x : 0];
+            }
+            double pv[G::kTPer];
+            float sv[G::kTPer], av[G::kTPer], rv[G::kTPer];
+#pragma unroll
+            for (int q = 0; q < G::kTPer; ++q) {
+                const int i = (int)threadIdx.x + NT * q;
+                const size_t c = ofl[q] ? (size_t)(Y0 + i / T) * w + X0 + i % T : 0;
+                pv[q] = ax.pd[c];
+                sv[q] = sv_in[c + w];
+                av[q] = ax.As[c + w];
+                rv[q] = F.r[c + w];
+            }
+            const float alpha = (float)alpha_d;
+#pragma unroll
+            for (int q = 0; q < G::kPer; ++q)
+                if (rfl[q]) g.r[q] = fmaf(-alpha, asr[q], g.r[q]);
+            vtile_put<T, NT>(g, t);
+            // the tile's own cells, as axpy_tile updates them
+            double rr = 0.0;
+#pragma unroll
+            for (int q = 0; q < G::kTPer; ++q) {
+                if (!ofl[q]) continue;
+                const int i = (int)threadIdx.x + NT * q;
+                const size_t c = (size_t)(Y0 + i / T) * w + X0 + i % T;
+                const double np = pv[q] + alpha_d * (double)sv[q];
+                ax.pd[c] = np;
+                ax.pressure[c] = (float)np;
+                const float nr = fmaf(-alpha, av[q], rv[q]);
+                ax.r_out[c + w] = nr;
+                rr += (double)nr * (double)nr;
+            }
+            rr = block_sum(rr, dscratch);  // (its barriers also order the region's LDS stores)
+            if (threadIdx.x == 0) ax.part_rr_out[wg] = rr;
+        } else {
+            vtile_put<T, NT>(g, t);
+            __syncthreads();
+        }
     }
     if (base) {  // red_zero_cell over the region
         for (int i = threadIdx.x; i < G::kN; i += blockDim.x) {
@@ -1658,23 +1800,32 @@ struct VDownArgs {
     int check;
     float solve_error;
     const float* part_max;
-    const double* part_rr;   // relative mode (else null): ||r||^2 partials of the update
-    const double* part_rr2;  // ... and of the true residual, on the iterations that recompute it
-    int every;               // the true residual is recomputed after iteration k when (k + 1) % every == 0
+    const double* part_rr_t;  // relative mode (else null): ||r||^2 partials of the update, one per base tile
+    uint32_t ntiles;
+    const double* part_rr2;   // ... and of the true residual (nred), on the iterations that recompute it
+    int every;                // the true residual is recomputed after iteration k when (k + 1) % every == 0
     uint32_t nred;
+    AxIn ax;                  // VDownAx: the iteration's update
 };
 
-template <int T, int NT = 256>
+template <int T, int NT = 256, bool AX = false>
 struct VDown {
     static constexpr int kThreads = NT;
     using Args = VDownArgs;
     // k: the PCG iteration whose update the residual test checks
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         const bool due = a.every > 0 && (k + 1) % a.every == 0;
-        vdown<T, NT>(a.F, a.C, a.base, a.fuse_red, a.spec, a.done, a.tiles_x, a.st, a.check, a.solve_error, a.part_max,
-                 due ? a.part_rr2 : a.part_rr, a.nred, wg);
+        vdown<T, NT, AX>(a.F, a.C, a.base, a.fuse_red, a.spec, a.done, a.tiles_x, a.st, a.check, a.solve_error,
+                         a.part_max, due ? a.part_rr2 : nullptr, due ? nullptr : a.part_rr_t, a.ntiles, a.nred, a.ax,
+                         k, wg);
     }
 };
+// The base level's down-sweep with iteration k's update (axpy_tile) applied
+// on the way in: the relative mode's iterations that do not recompute the
+// true residual, for a simulation stepped alone.  The residual test of the
+// update moves to the next launch (VDown2's check).
+template <int NT>
+using VDownAx = VDown<kBaseTile, NT, true>;
 
 // ---- two levels of the way down in one launch: levels l and l + 1, both
 // above the bottom and neither the base.  A workgroup takes a kP2T x kP2T
@@ -1814,10 +1965,12 @@ __device__ __forceinline__ float rrestrict(const Region<S>& t, int i0) {
 }
 
 __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, const LevelView& D, int fuse_red_d,
-                                       const int32_t* __restrict__ done, uint32_t tiles_x, uint32_t wg) {
+                                       const int32_t* __restrict__ done, uint32_t tiles_x, PcgState* st,
+                                       const double* __restrict__ part_rr_t, uint32_t ntiles, uint32_t wg) {
     constexpr int NT = kP2Threads;
     __shared__ Region<kP2F> tf;
     __shared__ Region<kP2C> tc;
+    __shared__ double dscratch[NT / 64];
     const int32_t w = (int32_t)F.w, h = (int32_t)F.h, cw = (int32_t)C.w, ch = (int32_t)C.h;
     const int32_t dw = (int32_t)D.w, dh = (int32_t)D.h;
     const int32_t cx0 = (int32_t)(wg % tiles_x) * kP2T, cy0 = (int32_t)(wg / tiles_x) * kP2T;
@@ -1837,9 +1990,23 @@ __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, c
     const size_t dc = din ? (size_t)dcy * dw + dcx : 0;
     const uint8_t dfl = D.fluid[dc];
     const float did = D.inv_diag[dc + dw];
+    // with part_rr_t, the residual test of the update that the base level's
+    // launch before this one fused (as vdown's)
+    const double tol2 = part_rr_t ? st->tol2 : 0.0, bn2 = part_rr_t ? st->bnorm2 : 0.0;
+    const double rr = part_rr_t ? sum_partials256(part_rr_t, (int)ntiles, dscratch) : 0.0;
     region_put<kP2F, NT>(gf, tf);
     region_put<kP2C, NT>(gc, tc);
     if (dn) return;
+    if (part_rr_t) {
+        const bool lead = wg == 0 && threadIdx.x == 0;
+        if (lead) st->rnorm2 = rr;
+        if (rr <= tol2 * bn2) {
+            // the new residual is in the base level's second buffer: the
+            // base up-sweep copies it home
+            if (lead) { st->done = 1; st->exit_reason = 3; st->r_in_alt = 1; }
+            return;
+        }
+    }
     // skip: no fluid cell of level l + 1 in the tile
     if (!__syncthreads_or(region_any_fluid<kP2C, NT, kP2T>(gc, kP2CR))) return;
     // level l: black cells out to ring kP2FR - 1 (own value zero before the sweep)
@@ -1911,13 +2078,16 @@ struct VDown2Args {
     int fuse_red_d;
     const int32_t* done;
     uint32_t tiles_x;  // tiles of level l + 1 in a row
+    PcgState* st;
+    const double* part_rr_t;  // after VDownAx: the update's ||r||^2 partials (else null)
+    uint32_t ntiles;
 };
 
 struct VDown2 {
     static constexpr int kThreads = kP2Threads;
     using Args = VDown2Args;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        vdown2(a.F, a.C, a.D, a.fuse_red_d, a.done, a.tiles_x, wg);
+        vdown2(a.F, a.C, a.D, a.fuse_red_d, a.done, a.tiles_x, a.st, a.part_rr_t, a.ntiles, wg);
     }
 };
 
@@ -2069,7 +2239,8 @@ template <int T, int NT>
 __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, const FluidStats* __restrict__ fs,
                            const int32_t* __restrict__ done, uint32_t tiles_x, int dot, float* __restrict__ part_gamma,
                            double* __restrict__ part_gamma_d, const float* __restrict__ As,
-                           double* __restrict__ part_asz_d, int spec, float* __restrict__ e_out, uint32_t wg) {
+                           double* __restrict__ part_asz_d, int spec, float* __restrict__ e_out,
+                           float* __restrict__ r_home, const int32_t* __restrict__ r_in_alt, uint32_t wg) {
     using G = VGeom<T, NT>;
     constexpr int kR = G::kR, kC = G::kC;
     __shared__ VTile<T> t;
@@ -2113,13 +2284,28 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     const bool cin = threadIdx.x < G::kCN && ccx < cw && ccy < chh;
     const uint8_t cfv = C.fluid[cin ? (size_t)ccy * cw + ccx : 0];
     const uint32_t last1 = fs ? fs->last_raw : 0u, cnt = fs ? fs->count : 0u;
+    const int32_t ria = r_home ? *r_in_alt : 0;
     if (spec) vtile_put<T, NT>(g, t);  // read after the barrier of the tile test below
 #pragma unroll
     for (int q = 0; q < G::kEPer; ++q) {
         const int j = (int)threadIdx.x + NT * q;
         if (j < kC * kC) ce[j] = ((cin_mask >> q) & 1u) ? cev[q] : 0.0f;
     }
-    if (dn) return;
+    if (dn) {
+        // the solve ended at the residual test of a fused update: its
+        // residual (L.r, the second buffer) goes home over the fluid cells
+        if (ria) {
+#pragma unroll
+            for (int q = 0; q < G::kTPer; ++q) {
+                const int i = (int)threadIdx.x + NT * q;
+                const int32_t x = X0 + i % T, y = Y0 + i / T;
+                if (i >= T * T || x >= w || y >= h || !L.fluid[(size_t)y * w + x]) continue;
+                const size_t pc = (size_t)(y + 1) * w + x;
+                r_home[pc] = L.r[pc];
+            }
+        }
+        return;
+    }
     // a tile without a fluid cell (see vdown) only reports a zero partial
     if (!__syncthreads_or(cin && cfv != 0)) {
         if (threadIdx.x == 0) {
@@ -2131,6 +2317,13 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     if (!spec) {
         vtile_fetch<T, NT>(L, X0, Y0, g, true, false);
         fetch_as();
+        if (r_home) {
+            // r from the second buffer, whose non-fluid cells are never
+            // written: zero there (they enter the r.z partials times e = 0)
+#pragma unroll
+            for (int q = 0; q < G::kPer; ++q)
+                if (!(((g.real >> q) & 1u) && g.fl[q])) g.r[q] = 0.0f;
+        }
         vtile_put<T, NT>(g, t);
         __syncthreads();
     }
@@ -2175,6 +2368,7 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
             e = vsor<T>(t, li, t.dg[li], t.e[li + 1], t.e[li + kR], t.e[li - 1], t.e[li - kR]);
         const size_t pc = (size_t)(y + 1) * w + x;
         e_out[pc] = e;
+        if (r_home && t.fl[li]) r_home[pc] = t.r[li];
         if (dot == 2) {
             dprod += (double)t.r[li] * (double)e;
             if (t.fl[li]) dasz += (double)asv[q] * (double)e;
@@ -2207,6 +2401,8 @@ struct VUpArgs {
     double* part_asz_d;   // the partials of (A s).z
     int spec;             // load the region with the flags (levels of few tiles)
     float* e_out;         // the level's e; L.e is its down-sweep buffer
+    float* r_home;        // base level after VDownAx (L.r its second buffer): the residual's own buffer
+    const int32_t* r_in_alt;
 };
 
 template <int T, int NT = 256>
@@ -2215,7 +2411,7 @@ struct VUp {
     using Args = VUpArgs;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
         vup<T, NT>(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, a.spec,
-               a.e_out, wg);
+               a.e_out, a.r_home, a.r_in_alt, wg);
     }
 };
 
@@ -2273,6 +2469,53 @@ size_t gamma_partials(const Sim& sim) {
 
 int pcg_cap(const Sim& sim) { return sim.solver.mode == 1 ? sim.solver.max_iterations : kMaxPcgIterations; }
 
+namespace {
+
+// 16 x 16 tiles on a level above the base with at most kFewTiles 32 x 32
+// ones (the base keeps one partial of r.z per 32 x 32 tile)
+constexpr uint32_t kFewTiles = 1024;
+
+uint32_t tiles_of(const LevelView& L, uint32_t side, uint32_t& tx) {
+    tx = (L.w + side - 1) / side;
+    return tx * ((L.h + side - 1) / side);
+}
+
+bool few_tiles(const Sim& sim, int l) {
+    uint32_t tx;
+    return l > 0 && tiles_of(sim.levels[l], kBaseTile, tx) <= kFewTiles;
+}
+
+// Two levels of few tiles above the bottom (not the base), l and l + 1, in
+// one launch each way when the simulation steps alone, where launches are
+// latency-bound.  A batch's launches are bound by throughput, which the
+// pair's recomputed ring costs more than the launch saves; the pair computes
+// the bits of the two launches, so either path gives the same result.
+bool level_pair(const Sim& sim, int l) {
+    return !sim.rec && l >= 1 && l + 1 < (int)sim.bottom_level && few_tiles(sim, l);
+}
+
+// The relative mode's update of iteration k fused into the base level's
+// down-sweep (VDownAx, one launch fewer): a simulation stepped alone whose
+// level pair (1, 2) follows the base (it takes over the residual test), on
+// the iterations that do not recompute the true residual.
+bool fused_update(const Sim& sim, int k) {
+    const int every = sim.solver.recompute_every;
+    const bool due = every > 0 && (k + 1) % every == 0;
+    return sim.solver.mode == 1 && !due && level_pair(sim, 1);
+}
+
+// The base tiles' thread count (see kWideMinRed): four tiles a CU are then 32
+// waves.  AxpyTile takes the same count, so that its partials are VDownAx's.
+bool wide_base(const Sim& sim) { return sim.nred >= kWideMinRed; }
+
+// The update's operands (AxpyTile, VDownAx), r_out receiving the residual.
+AxIn update_operands(const Sim& sim, float* r_out) {
+    return AxIn{{sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pressure_d, sim.part_delta_d, sim.nred,
+                r_out, sim.part_rr_t};
+}
+
+}  // namespace
+
 // Before the solve: the state, and in the relative mode b = r (kept for the
 // residual recomputation) and ||b||^2.
 void launch_pcg_reset(Sim& sim) {
@@ -2297,22 +2540,15 @@ void launch_vcycle(Sim& sim, int mode, int k) {
     const int last = (int)sim.levels.size() - 1;
     const int bottom = (int)sim.bottom_level;  // 1 <= bottom <= last
     const bool rel = sim.solver.mode == 1 && mode != 0;
-    auto tiles = [](const LevelView& L, uint32_t side, uint32_t& tx) {
-        tx = (L.w + side - 1) / side;
-        return tx * ((L.h + side - 1) / side);
-    };
-    // 16 x 16 tiles on a level above the base with at most kFewTiles 32 x 32
-    // ones (the base keeps one partial of r.z per 32 x 32 tile)
-    constexpr uint32_t kFewTiles = 1024;
-    auto small = [&](int l) {
-        uint32_t tx;
-        return l > 0 && tiles(sim.levels[l], kBaseTile, tx) <= kFewTiles;
-    };
-    // The base level of a large grid (see kWideMinRed) runs kWideTileThreads
-    // threads per tile: four tiles a CU are then 32 waves.  The FP32 partials
-    // of r.z under the reference's rules (dot 1) keep 256 threads and their
-    // summation order.
-    const bool wide_base = sim.nred >= kWideMinRed;
+    auto tiles = tiles_of;
+    auto small = [&](int l) { return few_tiles(sim, l); };
+    // The base level of a large grid runs kWideTileThreads threads per tile.
+    // The FP32 partials of r.z under the reference's rules (dot 1) keep 256
+    // threads and their summation order.
+    const bool wide = wide_base(sim);
+    const bool fused = mode == 2 && fused_update(sim, k);
+    const uint32_t ntiles = (uint32_t)gamma_partials(sim);
+    const AxIn ax = update_operands(sim, fused ? sim.r_alt : sim.levels[0].r);
     // A level above the bottom with its down-sweep buffer as e: the way down
     // reads and writes it, the way up reads it and writes the level's e
     // (the bottom's levels keep theirs: one workgroup runs them)
@@ -2322,16 +2558,14 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         return v;
     };
     for (int l = 0; l < bottom; ++l) {
-        // Two levels of few tiles above the bottom (not the base): one launch
-        // when the simulation steps alone, where launches are latency-bound.
-        // A batch's launches are bound by throughput, which the pair's
-        // recomputed ring costs more than the launch saves; the pair computes
-        // the bits of the two launches, so either path gives the same result.
-        if (!sim.rec && l >= 1 && l + 1 < bottom && small(l)) {
+        if (level_pair(sim, l)) {
             uint32_t tx;
             const uint32_t nt = tiles(sim.levels[l + 1], kP2T, tx);
             const int fuse_red_d = l + 2 < bottom ? 1 : (int)(bottom < last);
-            emit<VDown2>(sim, VDown2Args{down(l), down(l + 1), down(l + 2), fuse_red_d, done, tx}, nt, k);
+            const bool test = fused && l == 1;  // the fused update's residual test
+            emit<VDown2>(sim, VDown2Args{down(l), down(l + 1), down(l + 2), fuse_red_d, done, tx, sim.pcg,
+                                         test ? sim.part_rr_t : nullptr, ntiles},
+                         nt, k);
             ++l;
             continue;
         }
@@ -2341,12 +2575,15 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         uint32_t tx;
         const uint32_t nt = tiles(F, sm ? 16 : kBaseTile, tx);
         const bool rel_check = l == 0 && rel;
+        const bool ax_here = l == 0 && fused;
         VDownArgs a{F, down(l + 1), (int)(l == 0), fuse_red, (int)(l > 0), done, tx, sim.pcg,
-                          (int)(l == 0 && mode == 2), sim.props.solve_error, sim.part_max,
-                          rel_check ? sim.part_rr_d : nullptr, rel_check ? sim.part_rr2_d : nullptr,
-                          rel_check ? sim.solver.recompute_every : 0, sim.nred};
-        if (sm) emit<VDown<16>>(sim, a, nt, k);
-        else if (l == 0 && wide_base) emit<VDown<kBaseTile, kWideTileThreads>>(sim, a, nt, k);
+                          (int)(l == 0 && mode == 2 && !fused), sim.props.solve_error, sim.part_max,
+                          rel_check ? sim.part_rr_t : nullptr, ntiles, rel_check ? sim.part_rr2_d : nullptr,
+                          rel_check ? sim.solver.recompute_every : 0, sim.nred, ax};
+        if (ax_here && wide) emit<VDownAx<kWideTileThreads>>(sim, a, nt, k);
+        else if (ax_here) emit<VDownAx<kSolverThreads>>(sim, a, nt, k);
+        else if (sm) emit<VDown<16>>(sim, a, nt, k);
+        else if (l == 0 && wide) emit<VDown<kBaseTile, kWideTileThreads>>(sim, a, nt, k);
         else emit<VDown<kBaseTile>>(sim, a, nt, k);
     }
     if (sim.bottom_in_lds)
@@ -2355,7 +2592,7 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         emit<BottomGlobal>(sim, {level_set(sim), bottom, sim.cg_s, sim.cg_ap, sim.cg_r, done}, 1);
     for (int l = bottom - 1; l >= 0; --l) {
         // levels l - 1 and l in one launch, as on the way down
-        if (!sim.rec && l - 1 >= 1 && small(l - 1)) {
+        if (level_pair(sim, l - 1)) {
             uint32_t tx;
             const uint32_t nt = tiles(sim.levels[l], kP2T, tx);
             emit<VUp2>(sim, VUp2Args{down(l - 1), down(l), sim.levels[l + 1], sim.levels[l - 1].e, sim.levels[l].e,
@@ -2364,15 +2601,18 @@ void launch_vcycle(Sim& sim, int mode, int k) {
             --l;
             continue;
         }
-        const LevelView L = down(l);
+        LevelView L = down(l);
+        const bool home = l == 0 && fused;  // the residual is in the second buffer
+        if (home) L.r = sim.r_alt;
         const bool sm = small(l);
         uint32_t tx;
         const uint32_t nt = tiles(L, sm ? 16 : kBaseTile, tx);
         const int dot = (l == 0 && mode != 0) ? (rel ? 2 : 1) : 0;  // one partial of r.z per tile
         VUpArgs a{L, sim.levels[l + 1], l == 0 ? (const FluidStats*)sim.fstats : nullptr, done, tx, dot,
-                        sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d, (int)(l > 0), sim.levels[l].e};
+                        sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d, (int)(l > 0), sim.levels[l].e,
+                        home ? sim.levels[0].r : nullptr, &sim.pcg->r_in_alt};
         if (sm) emit<VUp<16>>(sim, a, nt);
-        else if (l == 0 && wide_base && dot != 1) emit<VUp<kBaseTile, kWideTileThreads>>(sim, a, nt);
+        else if (l == 0 && wide && dot != 1) emit<VUp<kBaseTile, kWideTileThreads>>(sim, a, nt);
         else emit<VUp<kBaseTile>>(sim, a, nt);
     }
 }
@@ -2388,10 +2628,20 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     if (wide) emit<SpmvDot<kWideThreads>>(sim, a, sim.nred, k);
     else emit<SpmvDot<kSolverThreads>>(sim, a, sim.nred, k);
     sim.mark(APIC_T_K_SPMV, 1);
-    AxpyMaxArgs b{B, {sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pcg, sim.part_delta, sim.part_max,
-                  rel ? sim.part_delta_d : nullptr, rel ? sim.part_rr_d : nullptr, sim.pressure_d, sim.nred};
-    if (wide) emit<AxpyMax<kWideThreads>>(sim, b, sim.nred, k);
-    else emit<AxpyMax<kSolverThreads>>(sim, b, sim.nred, k);
+    if (rel) {
+        // the update, unless the base down-sweep fuses it
+        if (!fused_update(sim, k)) {
+            uint32_t tx;
+            const uint32_t nt = tiles_of(B, kBaseTile, tx);
+            AxpyTileArgs b{B, update_operands(sim, B.r), sim.pcg, tx};
+            if (wide_base(sim)) emit<AxpyTile<kWideTileThreads>>(sim, b, nt, k);
+            else emit<AxpyTile<kSolverThreads>>(sim, b, nt, k);
+        }
+    } else {
+        AxpyMaxArgs b{B, {sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pcg, sim.part_delta, sim.part_max,
+                      sim.nred};
+        emit<AxpyMax<kSolverThreads>>(sim, b, sim.nred, k);
+    }
     // the true residual, on the iterations that recompute it (a batch records
     // one iteration for all k: there the kernel itself skips the others)
     const int every = sim.solver.recompute_every;
