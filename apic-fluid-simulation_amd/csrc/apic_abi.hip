@@ -451,9 +451,8 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
         launch_extrapolate(s);
         s.mark(APIC_T_EXTRAPOLATE, 1);
         s.mark(APIC_T_G2P, 0);
-        launch_g2p(s, 0.0f, true);
+        launch_g2p(s, 0.0f, true, true);  // and the substep's end
         s.mark(APIC_T_G2P, 1);
-        launch_substep_end(s);
         s.cells_valid = false;
         s.keys_valid = true;
         s.maxv_valid = true;
@@ -687,8 +686,7 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
             s.tail_guard = nullptr;
             launch_gradient(s, true, 0.0f);
             launch_extrapolate(s);
-            launch_g2p(s, 0.0f, true);
-            launch_substep_end(s);
+            launch_g2p(s, 0.0f, true, true);  // and the substep's end
             s.cells_valid = false;
             s.keys_valid = true;
             s.maxv_valid = true;

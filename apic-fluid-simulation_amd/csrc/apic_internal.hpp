@@ -101,6 +101,14 @@ struct StepState {
     float maxv;      // max |v| the dt was computed from
 };
 
+// The end of a substep (src/apic.rs:2444-2461): prev_dt = dt, elapsed += dt.
+__device__ inline void substep_end(StepState* ss) {
+    StepState s = *ss;
+    s.prev_dt = s.dt;
+    s.elapsed = s.elapsed + s.dt;
+    *ss = s;
+}
+
 struct Sim;
 
 // ---- launchers (kernels live in sort.hip, transfer.hip, pressure.hip, grid.hip)
@@ -118,7 +126,7 @@ size_t cell_key_count(const Sim& sim);            // Morton keys of the grid: mo
 void launch_cell_ranges(Sim& sim);                // per-cell [begin, end) of keys[0]'s order
 void launch_p2g(Sim& sim, bool forces, float dt);  // P2G + normalisation (+ external forces and gravity)
 void launch_forces(Sim& sim, float dt);            // apply_external_forces alone
-void launch_g2p(Sim& sim, float dt, bool write_keys);
+void launch_g2p(Sim& sim, float dt, bool write_keys, bool end_substep = false);  // then launch_substep_end's work
 void launch_max_velocity(Sim& sim);
 
 // grid.hip

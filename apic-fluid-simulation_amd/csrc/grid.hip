@@ -74,10 +74,7 @@ struct SubstepEnd {
     static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
         if (threadIdx.x != 0) return;
         if (a.guard && *a.guard == 0) return;
-        StepState s = *a.ss;
-        s.prev_dt = s.dt;
-        s.elapsed = s.elapsed + s.dt;
-        *a.ss = s;
+        substep_end(a.ss);
     }
 };
 

@@ -942,10 +942,12 @@ struct PcgBnorm {
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) { pcg_bnorm(a.L, a.rhs, a.part, a.pressure, a.pd, wg, a.nwg); }
 };
 
+// ||b||^2 from PcgBnorm's partials; it also resets the state (PcgReset's
+// work: no kernel before it reads the state), one launch instead of two.
 struct PcgBnormFinalize {
     static constexpr int kThreads = kSolverThreads;
     struct Args {
-        PcgState* st;
+        PcgReset::Args reset;
         const double* part;
         uint32_t n;
     };
@@ -954,9 +956,11 @@ struct PcgBnormFinalize {
         double b = 0.0;
         for (uint32_t i = threadIdx.x; i < a.n; i += blockDim.x) b += a.part[i];
         b = block_sum(b, dscratch);
+        PcgReset::run(a.reset, 0, 0);  // (thread 0 alone)
         if (threadIdx.x == 0) {
-            a.st->bnorm2 = b;
-            if (!(b > 0.0)) { a.st->done = 1; a.st->exit_reason = 1; }  // b = 0: p = 0 is the solution
+            PcgState* st = a.reset.st;
+            st->bnorm2 = b;
+            if (!(b > 0.0)) { st->done = 1; st->exit_reason = 1; }  // b = 0: p = 0 is the solution
         }
     }
 };
@@ -2521,10 +2525,12 @@ AxIn update_operands(const Sim& sim, float* r_out) {
 void launch_pcg_reset(Sim& sim) {
     const bool rel = sim.solver.mode == 1;
     const double tol = (double)sim.solver.rel_tol;
-    emit<PcgReset>(sim, {sim.pcg, tol * tol, pcg_cap(sim)}, 1);
+    const PcgReset::Args reset{sim.pcg, tol * tol, pcg_cap(sim)};
     if (rel) {
         emit<PcgBnorm>(sim, {sim.levels[0], sim.rhs, sim.part_rr_d, sim.pressure, sim.pressure_d, sim.nred}, sim.nred);
-        emit<PcgBnormFinalize>(sim, {sim.pcg, sim.part_rr_d, sim.nred}, 1);
+        emit<PcgBnormFinalize>(sim, {reset, sim.part_rr_d, sim.nred}, 1);
+    } else {
+        emit<PcgReset>(sim, reset, 1);
     }
 }
 

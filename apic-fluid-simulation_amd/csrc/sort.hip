@@ -24,9 +24,12 @@ namespace apic {
 
 namespace {
 
-constexpr int kRadixThreads = 256;   // 4 waves of 64
+// A tile of 4096 keys per workgroup of 8 waves (a 2M-particle sort is about
+// 500 tiles, two a CU: more waves per tile hide more of the scatter's latency
+// than more, smaller tiles, whose count and scan arrays grow)
+constexpr int kRadixThreads = 512;   // 8 waves of 64
 constexpr int kRadixWaves = kRadixThreads / 64;
-constexpr int kRadixItems = 16;      // keys per thread per tile
+constexpr int kRadixItems = 8;       // keys per thread per tile
 constexpr int kRadixTile = kRadixThreads * kRadixItems;
 constexpr int kRadixWaveItems = kRadixTile / kRadixWaves;
 constexpr int kRadixMaxBits = 11;

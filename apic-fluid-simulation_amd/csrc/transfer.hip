@@ -570,7 +570,9 @@ __global__ void max_velocity_kernel(const float* __restrict__ vx, const float* _
     }
 }
 
-// max over the per-workgroup maxima into the float bits read by the host
+// max over the per-workgroup maxima into the float bits read by the host;
+// with ss, also the end of the substep (SubstepEnd's bookkeeping, under the
+// same guard), one launch instead of two
 struct MaxvFinalize {
     static constexpr int kThreads = 1024;
     struct Args {
@@ -579,6 +581,7 @@ struct MaxvFinalize {
         uint32_t* maxv_bits;
         uint32_t* host_bits;
         const int32_t* guard;
+        StepState* ss;
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
         __shared__ float s_max[1024 / 64];
@@ -594,6 +597,7 @@ struct MaxvFinalize {
             for (int w = 0; w < (int)(blockDim.x >> 6); ++w) b = fast_max(b, s_max[w]);
             *a.maxv_bits = __float_as_uint(b);
             if (a.host_bits) *a.host_bits = __float_as_uint(b);
+            if (a.ss) substep_end(a.ss);
         }
     }
 };
@@ -650,10 +654,11 @@ void launch_forces(Sim& sim, float dt) {
                        sim.gravity_x * dt);
 }
 
-void launch_g2p(Sim& sim, float dt, bool write_keys) {
+void launch_g2p(Sim& sim, float dt, bool write_keys, bool end_substep) {
     if (sim.nalloc == 0) {
         launch_fill_u32(sim, sim.maxv_bits, 0u, 1);
         if (sim.h_maxv) *sim.h_maxv = 0u;
+        if (end_substep) launch_substep_end(sim);
         return;
     }
     const unsigned blocks = particle_blocks(sim);
@@ -663,7 +668,9 @@ void launch_g2p(Sim& sim, float dt, bool write_keys) {
                 sim.keys[0], sim.vals[0], sim.part_maxv, sim.step_src, sim.tail_guard, blocks};
     emit<G2p>(sim, a, blocks);
     sim.mark(APIC_T_K_G2P, 1);
-    emit<MaxvFinalize>(sim, {sim.part_maxv, blocks, sim.maxv_bits, sim.h_maxv_d, sim.tail_guard}, 1);
+    emit<MaxvFinalize>(sim, {sim.part_maxv, blocks, sim.maxv_bits, sim.h_maxv_d, sim.tail_guard,
+                             end_substep ? sim.step : nullptr},
+                       1);
 }
 
 void launch_max_velocity(Sim& sim) {
