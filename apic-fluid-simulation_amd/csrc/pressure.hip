@@ -1850,7 +1850,9 @@ constexpr int kP2FR = 6;                    // ring on level l
 constexpr int kP2F = 2 * kP2T + 2 * kP2FR;  // region side on level l
 constexpr int kP2CR = 2;                    // ring on level l + 1
 constexpr int kP2C = kP2T + 2 * kP2CR;      // region side on level l + 1
-constexpr int kP2Threads = 512;
+// A level pair's tiles are few and most are fluid-free: the active ones run
+// alone on their CUs, and 16 waves hide more of their latency than 8.
+constexpr int kP2Threads = 1024;
 
 template <int S>
 struct Region {
