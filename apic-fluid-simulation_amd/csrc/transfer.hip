@@ -164,6 +164,8 @@ __device__ __forceinline__ void p2g_owner(
     __shared__ uint32_t s_src[kOwnCells];          // sorted-order begin of each window cell
     __shared__ uint32_t s_wave[kOwnCells / 64];
     __shared__ uint8_t s_fl[kOwnCells];
+    __shared__ uint8_t s_cell[kOwnCap];            // the window cell of each staged particle
+    static_assert(kOwnCells <= 256, "window cells are bytes");
     const int32_t Wi = (int32_t)W, Hi = (int32_t)H;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wx = tid % kOwnWin, wy = tid / kOwnWin;
@@ -207,13 +209,14 @@ __device__ __forceinline__ void p2g_owner(
     for (int s = 0; s < 9; ++s) a.d[s] = 0.0f;
     for (uint32_t base = 0; base < total; base += kOwnCap) {
         const uint32_t n = min((uint32_t)kOwnCap, total - base);
+        {  // each cell marks its particles' slots of this pass
+            const uint32_t lo = max(my0, base), hi = min(my1, base + n);
+            for (uint32_t idx = lo; idx < hi; ++idx) s_cell[idx - base] = (uint8_t)tid;
+        }
+        __syncthreads();
         for (uint32_t k = tid; k < n; k += kOwnCells) {
             const uint32_t idx = base + k;
-            int lo = 0, hi = kOwnCells - 1;  // the window cell holding idx: last c with s_off[c] <= idx
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] <= idx) lo = mid; else hi = mid - 1;
-            }
+            const int lo = s_cell[k];  // the window cell holding idx
             const uint32_t q = s_src[lo] + (idx - s_off[lo]);  // position in cell order
             const size_t src = order ? order[q] : q;
             s_buf[0 * kOwnCap + k] = pb[(size_t)P_X * nalloc + src];
