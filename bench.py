@@ -8,12 +8,13 @@ reference's own solver rules (absolute 1e-5 exits) is reported beside it.
 A step is one Apic::update(dt) (src/apic.rs:2429) over all particles, with
 every array resident in HBM.  W untimed warm-up updates, then K timed updates
 bracketed by device synchronisation; value = sum of particles x substeps / wall
-seconds.  HIP events recorded by the library on its own stream time the P2G,
-G2P and SpMV kernels inside the timed region; the one with the largest share
-of the step is the roofline kernel, its achieved bandwidth (algorithmic bytes
-per launch / average launch time) reported against the 8 TB/s HBM peak.  A
-short second pass times every stage.  The CPU
-baseline is the oracle (FP32, one thread) on the first steps of the same
+seconds.  HIP events recorded by the library on its own stream time the P2G
+and G2P kernels and one SpMV launch per solve (iteration 1; events around all
+of them would add their own cost to the step) inside the timed region; the one
+with the largest share of the step is the roofline kernel, its achieved
+bandwidth (algorithmic bytes per launch / average launch time) reported
+against the 8 TB/s HBM peak.  A short second pass times every stage.  The
+CPU baseline is the oracle (FP32, one thread) on the first steps of the same
 workload, timed on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
@@ -104,7 +105,8 @@ def cpu_baseline(xy, cfg, budget_s, solver):
 
 def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_steps):
     """W warm-up updates, K timed updates (HIP events around the two transfer
-    kernels only), then a short pass with events around every stage."""
+    kernels and one SpMV per solve only), then a short pass with events around
+    every stage."""
     import numpy as np
     g = Apic.with_particles(cfg["width"], cfg["height"], cfg["dx"], props, xy)
     g.set_solver_config(**SOLVERS[solver])
@@ -369,10 +371,15 @@ def main():
     n_f = float(np.mean(r["fluid"]))
     algo = algorithmic_bytes(n_p, r["n_alloc"], n_f, cfg["width"] * cfg["height"])
     kernels = {}
+    # the SpMV's timer samples one launch per solve (iteration 1): its share
+    # of the step is its average times the launches per step, one per
+    # PCG iteration
+    spmv_per_step = float(np.sum(r["iters"])) / args.steps
     for k in ("kernel_g2p", "kernel_p2g", "kernel_spmv"):
         ms, cnt = r["timers"][k]
         avg_ms = ms / max(cnt, 1)
-        kernels[k] = {"avg_ms": avg_ms, "launches": cnt, "ms_per_step": ms / args.steps, "bytes": algo[k],
+        per_step = avg_ms * spmv_per_step if k == "kernel_spmv" else ms / args.steps
+        kernels[k] = {"avg_ms": avg_ms, "launches": cnt, "ms_per_step": per_step, "bytes": algo[k],
                       "gbs": algo[k] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None}
     # the dominant kernel: the largest share of the step among the timed ones
     dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])

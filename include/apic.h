@@ -186,7 +186,7 @@ typedef enum apic_timer {
     APIC_T_GRADIENT, APIC_T_EXTRAPOLATE, APIC_T_G2P,
     APIC_T_K_P2G,   /* the P2G kernel alone */
     APIC_T_K_G2P,   /* the G2P gather/advect kernel alone */
-    APIC_T_K_SPMV,  /* the PCG's SpMV + dot kernel alone (every launch) */
+    APIC_T_K_SPMV,  /* the PCG's SpMV + dot kernel alone (iteration 1 of each solve) */
     APIC_TIMER_COUNT
 } apic_timer;
 
