@@ -218,7 +218,7 @@ struct Sim {
     float* part_max = nullptr;       // kRedBlocks partials of max |r|
     double* part_gamma_d = nullptr;  // relative mode: FP64 partials of r.z, s.As and |r|^2
     double* part_delta_d = nullptr;
-    double* part_rr_d = nullptr;     // ||r||^2 partials of ||b||^2 (the relative mode's prologue)
+    double* part_rr_d = nullptr;     // partials of ||b||^2 (the relative mode's prologue)
     double* part_rr_t = nullptr;     // ||r||^2 partials of the recurrence residual, one per base tile
     double* part_rr2_d = nullptr;    // ... after the true-residual check
     double* part_asz_d = nullptr;    // relative mode: partials of (A s).z for the flexible beta

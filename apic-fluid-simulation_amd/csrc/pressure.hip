@@ -969,12 +969,11 @@ struct PcgBnormFinalize {
 // iterations (due): r is replaced by the true residual b - A p evaluated in
 // FP64 against the FP64 pressure (the product with the same fused operations
 // as apply_matrix_and_dot, zero outside the grid).  part_out receives the
-// partials of ||r||^2 that the residual test then reads (the update's own,
-// part_in, when not due).
+// partials of ||r||^2 that the residual test reads on those iterations (on
+// the others it reads the update's per-tile partials).
 __device__ __forceinline__ void true_residual(const LevelView& L, const float* __restrict__ rhs,
                                      const double* __restrict__ pd, const PcgState* st,
-                                     const double* __restrict__ part_in, double* __restrict__ part_out, int due,
-                                     uint32_t wg, uint32_t nwg) {
+                                     double* __restrict__ part_out, int due, uint32_t wg, uint32_t nwg) {
     __shared__ double dscratch[kThreads / 64];
     if (st->done || !due) return;
     const int64_t w = L.w, n = L.cells;
@@ -1005,14 +1004,13 @@ struct TrueResidual {
         const float* rhs;
         const double* pd;
         const PcgState* st;
-        const double* part_in;
         double* part_out;
         int every;  // recompute every `every` iterations (0: never)
         uint32_t nwg;
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         const int due = a.every > 0 && (k + 1) % a.every == 0;
-        true_residual(a.L, a.rhs, a.pd, a.st, a.part_in, a.part_out, due, wg, a.nwg);
+        true_residual(a.L, a.rhs, a.pd, a.st, a.part_out, due, wg, a.nwg);
     }
 };
 
@@ -1054,14 +1052,15 @@ __device__ inline bool gamma_exit(PcgState* st, float g, int k, bool lead) {
 // then every operand of the batch's fluid cells at once (a non-fluid cell
 // reads the row's first cell, a line every lane of the wave shares), so that
 // a workgroup waits twice per batch instead of twice per cell.  With FP64
-// reductions on a grid of at least kWideMinRed reduction workgroups they run
-// kWideThreads threads per workgroup (about two cells per thread): one such
-// grid's launch is bound by latency, which four times the waves hide.
+// reductions on a grid of at least kWideMinRed reduction workgroups SpmvDot
+// runs kWideThreads threads per workgroup (about two cells per thread): one
+// such grid's launch is bound by latency, which four times the waves hide.
 // Smaller grids, and batches of them (whose launches fill the GPU with the
 // workgroups of all their simulations), keep 256 threads and more cells per
-// wave, and so do the FP32 reductions of the reference's rules (which keeps
-// each partial's cells and summation order).  The choice depends on the
-// simulation alone, so a batched simulation computes what it computes alone.
+// wave, and so do the FP32 reductions of the reference's rules (AxpyMax,
+// which keeps each partial's cells and summation order).  The choice depends
+// on the simulation alone, so a batched simulation computes what it computes
+// alone.  (The relative mode's update is axpy_tile, over the base tiles.)
 constexpr int kWideThreads = 1024;
 constexpr int kWideBatch = 2;
 constexpr uint32_t kWideMinRed = 256;
@@ -2658,7 +2657,7 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     // one iteration for all k: there the kernel itself skips the others)
     const int every = sim.solver.recompute_every;
     if (rel && (sim.rec || (every > 0 && (k + 1) % every == 0))) {
-        TrueResidual::Args c{B, sim.rhs, sim.pressure_d, sim.pcg, sim.part_rr_d, sim.part_rr2_d, every, sim.nred};
+        TrueResidual::Args c{B, sim.rhs, sim.pressure_d, sim.pcg, sim.part_rr2_d, every, sim.nred};
         emit<TrueResidual>(sim, c, sim.nred, k);
     }
 }
