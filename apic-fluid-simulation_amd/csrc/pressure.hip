@@ -775,15 +775,16 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
                 const int rf = (cy * 2 + dy) * F.w + cx * 2 + dx, pf = rf + F.w;
-                if (fl[F.fl + rf]) {
-                    const float b = f[F.r + pf];
-                    float ax = f[F.dg + pf] * f[F.e + pf];
-                    ax += f[F.px + pf] * f[F.e + pf + 1];
-                    ax += f[F.py + pf] * f[F.e + pf + F.w];
-                    ax += f[F.px + pf - 1] * f[F.e + pf - 1];
-                    ax += f[F.py + pf - F.w] * f[F.e + pf - F.w];
-                    sum += b - ax;
-                }
+            // every child's operands load unconditionally (they go out
+            // together); a non-fluid child adds +0, which leaves the sum as
+            // skipping it would: the sum starts at +0 and never becomes -0
+                const float b = f[F.r + pf];
+                float ax = f[F.dg + pf] * f[F.e + pf];
+                ax += f[F.px + pf] * f[F.e + pf + 1];
+                ax += f[F.py + pf] * f[F.e + pf + F.w];
+                ax += f[F.px + pf - 1] * f[F.e + pf - 1];
+                ax += f[F.py + pf - F.w] * f[F.e + pf - F.w];
+                sum += fl[F.fl + rf] ? b - ax : 0.0f;
             }
         f[C.r + pcc] = sum;
         const float id = f[C.id + pcc];
@@ -1774,15 +1775,16 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
                 const int li = (2 * cy + dy - Y0 + kVH) * kR + (2 * cx + dx - X0 + kVH);
-                if (t.fl[li]) {
-                    const float b = t.r[li];
-                    float ax = t.dg[li] * t.e[li];
-                    ax += t.px[li] * t.e[li + 1];
-                    ax += t.py[li] * t.e[li + kR];
-                    ax += t.px[li - 1] * t.e[li - 1];
-                    ax += t.py[li - kR] * t.e[li - kR];
-                    sum += b - ax;
-                }
+            // every child's operands load unconditionally (they go out
+            // together); a non-fluid child adds +0, which leaves the sum as
+            // skipping it would: the sum starts at +0 and never becomes -0
+                const float b = t.r[li];
+                float ax = t.dg[li] * t.e[li];
+                ax += t.px[li] * t.e[li + 1];
+                ax += t.py[li] * t.e[li + kR];
+                ax += t.px[li - 1] * t.e[li - 1];
+                ax += t.py[li - kR] * t.e[li - kR];
+                sum += t.fl[li] ? b - ax : 0.0f;
             }
         C.r[pcc] = sum;
         const float id = cid[q];
@@ -1956,15 +1958,15 @@ __device__ __forceinline__ float rrestrict(const Region<S>& t, int i0) {
 #pragma unroll
         for (int dx = 0; dx < 2; ++dx) {
             const int li = i0 + dy * S + dx;
-            if (t.fl[li]) {
-                const float b = t.r[li];
-                float ax = t.dg[li] * t.e[li];
-                ax += t.px[li] * t.e[li + 1];
-                ax += t.py[li] * t.e[li + S];
-                ax += t.px[li - 1] * t.e[li - 1];
-                ax += t.py[li - S] * t.e[li - S];
-                sum += b - ax;
-            }
+            // (as vdown's restriction: unconditional loads, +0 for a
+            // non-fluid child)
+            const float b = t.r[li];
+            float ax = t.dg[li] * t.e[li];
+            ax += t.px[li] * t.e[li + 1];
+            ax += t.py[li] * t.e[li + S];
+            ax += t.px[li - 1] * t.e[li - 1];
+            ax += t.py[li - S] * t.e[li - S];
+            sum += t.fl[li] ? b - ax : 0.0f;
         }
     return sum;
 }
