@@ -808,11 +808,12 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
             restrict_to(tab[l - 1], L, x, y, true);
             __syncthreads();
         }
-        if (mine && fl[L.fl + c] && ((x + y) & 1) == 1) {
+        if (mine && ((x + y) & 1) == 1) {  // (the colour alone gates the loads)
             const int pc = c + L.w;
-            f[L.e + pc] = sor(f[L.e + pc], f[L.id + pc], f[L.r + pc], f[L.px + pc], f[L.py + pc], f[L.px + pc - 1],
-                              f[L.py + pc - L.w], f[L.e + pc + 1], f[L.e + pc + L.w], f[L.e + pc - 1],
-                              f[L.e + pc - L.w]);
+            const float v = sor(f[L.e + pc], f[L.id + pc], f[L.r + pc], f[L.px + pc], f[L.py + pc],
+                                f[L.px + pc - 1], f[L.py + pc - L.w], f[L.e + pc + 1], f[L.e + pc + L.w],
+                                f[L.e + pc - 1], f[L.e + pc - L.w]);
+            if (fl[L.fl + c]) f[L.e + pc] = v;
         }
         __syncthreads();
     }
@@ -853,17 +854,20 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
         const float id = f[L.id + pc], r = f[L.r + pc], px = f[L.px + pc], py = f[L.py + pc];
         const float pxm = f[L.px + pc - 1], pym = f[L.py + pc - L.w];
         const bool black = ((x + y) & 1) == 1;
-        if (work && black) {
+        if (mine && black) {
             const float er = x + 1 < L.w ? nb(c + 1, x + 1, y) : nb(c + 1, 0, y + 1);
             const float et = nb(c + L.w, x, y + 1);
             const float el = x > 0 ? nb(c - 1, x - 1, y) : nb(c - 1, L.w - 1, y - 1);
             const float eb = nb(c - L.w, x, y - 1);
-            f[L.e + pc] = sor(own, id, r, px, py, pxm, pym, er, et, el, eb);
+            const float v = sor(own, id, r, px, py, pxm, pym, er, et, el, eb);
+            if (work) f[L.e + pc] = v;
         }
         __syncthreads();
-        if (work && !black)
-            f[L.e + pc] = sor(own, id, r, px, py, pxm, pym, f[L.e + pc + 1], f[L.e + pc + L.w], f[L.e + pc - 1],
-                              f[L.e + pc - L.w]);
+        if (mine && !black) {
+            const float v = sor(own, id, r, px, py, pxm, pym, f[L.e + pc + 1], f[L.e + pc + L.w], f[L.e + pc - 1],
+                                f[L.e + pc - L.w]);
+            if (work) f[L.e + pc] = v;
+        }
         __syncthreads();
     }
     // copy-back: each level's r and e are one stride of the image
@@ -1441,17 +1445,16 @@ __device__ __forceinline__ void vtile_put(const VRegs<T, NT>& g, VTile<T>& t) {
 // SOR update of cell i from its own value and four neighbour values (sor_cell).
 template <int T>
 __device__ inline float vsor(const VTile<T>& t, int i, float own, float er, float et, float el, float eb) {
+    // every operand loads at once; a cell with id = 0 keeps its value (the
+    // select discards what was computed from its operands)
     const float id = t.id[i];
-    if (id != 0.0f) {
-        float sum = t.r[i];
-        sum = fmaf(-t.px[i], er, sum);
-        sum = fmaf(-t.py[i], et, sum);
-        sum = fmaf(-t.px[i - 1], el, sum);
-        sum = fmaf(-t.py[i - VGeom<T>::kR], eb, sum);
-        const float gs = sum * id;
-        own = own + (kOmega * (gs - own));
-    }
-    return own;
+    float sum = t.r[i];
+    sum = fmaf(-t.px[i], er, sum);
+    sum = fmaf(-t.py[i], et, sum);
+    sum = fmaf(-t.px[i - 1], el, sum);
+    sum = fmaf(-t.py[i - VGeom<T>::kR], eb, sum);
+    const float gs = sum * id;
+    return id != 0.0f ? own + (kOmega * (gs - own)) : own;
 }
 
 // alpha = gamma / delta of iteration k in FP64 (src/apic.rs:1903-1907; the
@@ -1748,8 +1751,9 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         const int lx = i % kR, ly = i / kR;
         if (lx < 1 || lx >= kR - 1 || ly < 1 || ly >= kR - 1) continue;
         const int32_t x = X0 - kVH + lx, y = Y0 - kVH + ly;
-        if (!t.fl[i] || ((x + y) & 1) == 0) continue;
-        t.e[i] = vsor<T>(t, i, 0.0f, t.e[i + 1], t.e[i + kR], t.e[i - 1], t.e[i - kR]);
+        if (((x + y) & 1) == 0) continue;  // (the colour alone gates the loads: they go out together)
+        const float v = vsor<T>(t, i, 0.0f, t.e[i + 1], t.e[i + kR], t.e[i - 1], t.e[i - kR]);
+        if (t.fl[i]) t.e[i] = v;
     }
     __syncthreads();
 #pragma unroll
@@ -1935,17 +1939,14 @@ __device__ __forceinline__ bool region_any_fluid(const RegionRegs<S, NT>& g, int
 // sor_cell's update of region cell i of side S
 template <int S>
 __device__ __forceinline__ float rsor(const Region<S>& t, int i, float own, float er, float et, float el, float eb) {
-    const float id = t.id[i];
-    if (id != 0.0f) {
-        float sum = t.r[i];
-        sum = fmaf(-t.px[i], er, sum);
-        sum = fmaf(-t.py[i], et, sum);
-        sum = fmaf(-t.px[i - 1], el, sum);
-        sum = fmaf(-t.py[i - S], eb, sum);
-        const float gs = sum * id;
-        own = own + (kOmega * (gs - own));
-    }
-    return own;
+    const float id = t.id[i];  // (as vsor: operands at once, a select at the end)
+    float sum = t.r[i];
+    sum = fmaf(-t.px[i], er, sum);
+    sum = fmaf(-t.py[i], et, sum);
+    sum = fmaf(-t.px[i - 1], el, sum);
+    sum = fmaf(-t.py[i - S], eb, sum);
+    const float gs = sum * id;
+    return id != 0.0f ? own + (kOmega * (gs - own)) : own;
 }
 
 // restrict_cell's sum b - A e over the four children of a coarse cell whose
@@ -2021,8 +2022,9 @@ __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, c
         const int lx = i % kP2F, ly = i / kP2F;
         if (lx < 1 || lx >= kP2F - 1 || ly < 1 || ly >= kP2F - 1) continue;
         const int32_t x = FX0 + lx, y = FY0 + ly;
-        if (!tf.fl[i] || ((x + y) & 1) == 0) continue;
-        tf.e[i] = rsor<kP2F>(tf, i, 0.0f, tf.e[i + 1], tf.e[i + kP2F], tf.e[i - 1], tf.e[i - kP2F]);
+        if (((x + y) & 1) == 0) continue;
+        const float v = rsor<kP2F>(tf, i, 0.0f, tf.e[i + 1], tf.e[i + kP2F], tf.e[i - 1], tf.e[i - kP2F]);
+        if (tf.fl[i]) tf.e[i] = v;
     }
     __syncthreads();
     // level l + 1 out to ring kP2CR: restriction with the fused red sweep
@@ -2045,8 +2047,9 @@ __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, c
         const int lx = i % kP2C, ly = i / kP2C;
         if (lx < 1 || lx >= kP2C - 1 || ly < 1 || ly >= kP2C - 1) continue;
         const int32_t cx = CX0 + lx, cy = CY0 + ly;
-        if (!tc.fl[i] || ((cx + cy) & 1) == 0) continue;
-        tc.e[i] = rsor<kP2C>(tc, i, 0.0f, tc.e[i + 1], tc.e[i + kP2C], tc.e[i - 1], tc.e[i - kP2C]);
+        if (((cx + cy) & 1) == 0) continue;
+        const float v = rsor<kP2C>(tc, i, 0.0f, tc.e[i + 1], tc.e[i + kP2C], tc.e[i - 1], tc.e[i - kP2C]);
+        if (tc.fl[i]) tc.e[i] = v;
     }
     // level l: the tile's black cells (its red cells are already in memory)
     for (int i = threadIdx.x; i < 4 * kP2T * kP2T; i += NT) {
@@ -2168,8 +2171,9 @@ __device__ __forceinline__ void vup2(const LevelView& F, const LevelView& C, con
         const int lx = i % kU2C, ly = i / kU2C;
         if (lx < 1 || lx >= kU2C - 1 || ly < 1 || ly >= kU2C - 1) continue;
         const int32_t x = CX0 + lx, y = CY0 + ly;
-        if (!tc.fl[i] || ((x + y) & 1) == 0) continue;
-        tc.e[i] = rsor<kU2C>(tc, i, tc.dg[i], tc.dg[i + 1], tc.dg[i + kU2C], tc.dg[i - 1], tc.dg[i - kU2C]);
+        if (((x + y) & 1) == 0) continue;
+        const float v = rsor<kU2C>(tc, i, tc.dg[i], tc.dg[i + 1], tc.dg[i + kU2C], tc.dg[i - 1], tc.dg[i - kU2C]);
+        if (tc.fl[i]) tc.e[i] = v;
     }
     __syncthreads();
     // level l + 1: red cells out to ring 1, written in place (red cells read
@@ -2178,8 +2182,9 @@ __device__ __forceinline__ void vup2(const LevelView& F, const LevelView& C, con
         const int lx = i % kU2C, ly = i / kU2C;
         if (lx < 2 || lx >= kU2C - 2 || ly < 2 || ly >= kU2C - 2) continue;
         const int32_t x = CX0 + lx, y = CY0 + ly;
-        if (!tc.fl[i] || ((x + y) & 1) != 0) continue;
-        tc.e[i] = rsor<kU2C>(tc, i, tc.dg[i], tc.e[i + 1], tc.e[i + kU2C], tc.e[i - 1], tc.e[i - kU2C]);
+        if (((x + y) & 1) != 0) continue;
+        const float v = rsor<kU2C>(tc, i, tc.dg[i], tc.e[i + 1], tc.e[i + kU2C], tc.e[i - 1], tc.e[i - kU2C]);
+        if (tc.fl[i]) tc.e[i] = v;
     }
     __syncthreads();
     // level l + 1: the tile's final error
@@ -2204,8 +2209,9 @@ __device__ __forceinline__ void vup2(const LevelView& F, const LevelView& C, con
         const int lx = i % kU2F, ly = i / kU2F;
         if (lx < 1 || lx >= kU2F - 1 || ly < 1 || ly >= kU2F - 1) continue;
         const int32_t x = FX0 + lx, y = FY0 + ly;
-        if (!tf.fl[i] || ((x + y) & 1) == 0) continue;
-        tf.e[i] = rsor<kU2F>(tf, i, tf.dg[i], tf.dg[i + 1], tf.dg[i + kU2F], tf.dg[i - 1], tf.dg[i - kU2F]);
+        if (((x + y) & 1) == 0) continue;
+        const float v = rsor<kU2F>(tf, i, tf.dg[i], tf.dg[i + 1], tf.dg[i + kU2F], tf.dg[i - 1], tf.dg[i - kU2F]);
+        if (tf.fl[i]) tf.e[i] = v;
     }
     __syncthreads();
     // level l: red cells of the tile, then the tile's final error
@@ -2214,8 +2220,10 @@ __device__ __forceinline__ void vup2(const LevelView& F, const LevelView& C, con
         if (x >= w || y >= h) continue;
         const int li = (y - FY0) * kU2F + (x - FX0);
         float e = tf.e[li];
-        if (tf.fl[li] && ((x + y) & 1) == 0)
-            e = rsor<kU2F>(tf, li, tf.dg[li], tf.e[li + 1], tf.e[li + kU2F], tf.e[li - 1], tf.e[li - kU2F]);
+        if (((x + y) & 1) == 0) {
+            const float v = rsor<kU2F>(tf, li, tf.dg[li], tf.e[li + 1], tf.e[li + kU2F], tf.e[li - 1], tf.e[li - kU2F]);
+            if (tf.fl[li]) e = v;
+        }
         f_out[(size_t)(y + 1) * w + x] = e;
     }
 }
@@ -2356,8 +2364,9 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         const int lx = i % kR, ly = i / kR;
         if (lx < 1 || lx >= kR - 1 || ly < 1 || ly >= kR - 1) continue;
         const int32_t x = X0 - kVH + lx, y = Y0 - kVH + ly;
-        if (!t.fl[i] || ((x + y) & 1) == 0) continue;
-        t.e[i] = vsor<T>(t, i, t.dg[i], t.dg[i + 1], t.dg[i + kR], t.dg[i - 1], t.dg[i - kR]);
+        if (((x + y) & 1) == 0) continue;
+        const float v = vsor<T>(t, i, t.dg[i], t.dg[i + 1], t.dg[i + kR], t.dg[i - 1], t.dg[i - kR]);
+        if (t.fl[i]) t.e[i] = v;
     }
     __syncthreads();
     // second colour (red) of the tile: own prolongated (the first colour left
@@ -2371,8 +2380,10 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         if (i >= T * T || x >= w || y >= h) continue;
         const int li = (y - Y0 + kVH) * kR + (x - X0 + kVH);
         float e = t.e[li];
-        if (t.fl[li] && ((x + y) & 1) == 0)
-            e = vsor<T>(t, li, t.dg[li], t.e[li + 1], t.e[li + kR], t.e[li - 1], t.e[li - kR]);
+        if (((x + y) & 1) == 0) {
+            const float v = vsor<T>(t, li, t.dg[li], t.e[li + 1], t.e[li + kR], t.e[li - 1], t.e[li - kR]);
+            if (t.fl[li]) e = v;
+        }
         const size_t pc = (size_t)(y + 1) * w + x;
         e_out[pc] = e;
         if (r_home && t.fl[li]) r_home[pc] = t.r[li];
