@@ -2158,12 +2158,11 @@ __device__ __forceinline__ void vup2(const LevelView& F, const LevelView& C, con
     if (!__syncthreads_or(region_any_fluid<kU2C, NT, kP2T>(gc, kU2CR))) return;
     // level l + 1: e plus the parent's error over the region (the diagonal's slot)
     for (int i = threadIdx.x; i < kU2C * kU2C; i += NT) {
-        float v = tc.e[i];
-        if (tc.fl[i]) {
-            const int32_t x = CX0 + i % kU2C, y = CY0 + i / kU2C;
-            v = v + de[((y >> 1) - DY0) * kU2D + ((x >> 1) - DX0)];
-        }
-        tc.dg[i] = v;
+        // (every region cell's parent lies in de: the load needs no flag)
+        const int32_t x = CX0 + i % kU2C, y = CY0 + i / kU2C;
+        const float pe = de[((y >> 1) - DY0) * kU2D + ((x >> 1) - DX0)];
+        const float v = tc.e[i];
+        tc.dg[i] = tc.fl[i] ? v + pe : v;
     }
     __syncthreads();
     // level l + 1: black cells out to ring 2
@@ -2196,12 +2195,11 @@ __device__ __forceinline__ void vup2(const LevelView& F, const LevelView& C, con
     // level l: e plus the parent's final error (the parent of a fluid cell is
     // a cell of the grid, within ring 1 of the level l + 1 tile)
     for (int i = threadIdx.x; i < kU2F * kU2F; i += NT) {
-        float v = tf.e[i];
-        if (tf.fl[i]) {
-            const int32_t x = FX0 + i % kU2F, y = FY0 + i / kU2F;
-            v = v + tc.e[((y >> 1) - CY0) * kU2C + ((x >> 1) - CX0)];
-        }
-        tf.dg[i] = v;
+        // (every region cell's parent lies in the level l + 1 region)
+        const int32_t x = FX0 + i % kU2F, y = FY0 + i / kU2F;
+        const float pe = tc.e[((y >> 1) - CY0) * kU2C + ((x >> 1) - CX0)];
+        const float v = tf.e[i];
+        tf.dg[i] = tf.fl[i] ? v + pe : v;
     }
     __syncthreads();
     // level l: black cells of the tile and its one-cell ring
@@ -2345,17 +2343,16 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     // e plus the parent's error (prolonged) of every region cell, once, into
     // the diagonal's slot (not read on the way up); non-fluid cells keep e
     for (int i = threadIdx.x; i < G::kN; i += blockDim.x) {
+        // (every region cell's parent lies in ce: the load needs no flag)
+        const int32_t x = X0 - kVH + i % kR, y = Y0 - kVH + i / kR;
+        const float pc = ce[((y >> 1) - CY0) * kC + ((x >> 1) - CX0)];
         float v = t.e[i];
-        if (t.fl[i]) {
-            const int32_t x = X0 - kVH + i % kR, y = Y0 - kVH + i / kR;
-            const float pc = ce[((y >> 1) - CY0) * kC + ((x >> 1) - CX0)];
-            int reps = 1;
-            if (fs) {
-                const uint32_t raw = (uint32_t)(y * w + x);
-                if (last1 != 0 && raw == last1 - 1 && (cnt & 7u) != 0) reps = 1 + (int)(8u - (cnt & 7u));
-            }
-            for (int k = 0; k < reps; ++k) v = v + pc;
+        int reps = t.fl[i] ? 1 : 0;
+        if (fs && reps) {
+            const uint32_t raw = (uint32_t)(y * w + x);
+            if (last1 != 0 && raw == last1 - 1 && (cnt & 7u) != 0) reps = 1 + (int)(8u - (cnt & 7u));
         }
+        for (int k = 0; k < reps; ++k) v = v + pc;
         t.dg[i] = v;
     }
     __syncthreads();
