@@ -768,7 +768,7 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
     auto restrict_to = [&](const BLev& F, const BLev& C, int cx, int cy, bool fuse_red) {
         if (c >= C.n) return;
         const int pcc = c + C.w;
-        if (!fl[C.fl + c]) { f[C.r + pcc] = 0.0f; f[C.e + pcc] = 0.0f; return; }
+        const uint8_t cfl = fl[C.fl + c];  // (gates the stores, not the children's loads)
         float sum = 0.0f;
 #pragma unroll
         for (int dy = 0; dy < 2; ++dy)
@@ -786,10 +786,10 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
                 ax += f[F.py + pf - F.w] * f[F.e + pf - F.w];
                 sum += fl[F.fl + rf] ? b - ax : 0.0f;
             }
-        f[C.r + pcc] = sum;
         const float id = f[C.id + pcc];
         const bool red = ((cx + cy) & 1) == 0;
-        f[C.e + pcc] = (fuse_red && red && id != 0.0f) ? kOmega * (sum * id) : 0.0f;
+        f[C.r + pcc] = cfl ? sum : 0.0f;
+        f[C.e + pcc] = (cfl && fuse_red && red && id != 0.0f) ? kOmega * (sum * id) : 0.0f;
     };
     auto coords = [&](const BLev& L, int& x, int& y) {
         const bool mine = c < L.n;
@@ -1738,7 +1738,9 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         for (int i = threadIdx.x; i < G::kN; i += blockDim.x) {
             const int32_t x = X0 - kVH + i % kR, y = Y0 - kVH + i / kR;
             const bool red = ((x + y) & 1) == 0;
-            t.e[i] = (t.fl[i] && red && t.id[i] != 0.0f) ? kOmega * (t.r[i] * t.id[i]) : 0.0f;
+            const uint8_t fl = t.fl[i];  // (the three loads go out together)
+            const float id = t.id[i], r = t.r[i];
+            t.e[i] = (fl && red && id != 0.0f) ? kOmega * (r * id) : 0.0f;
         }
         __syncthreads();
     }
@@ -2031,15 +2033,14 @@ __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, c
     for (int i = threadIdx.x; i < kP2C * kP2C; i += NT) {
         const int lx = i % kP2C, ly = i / kP2C;
         const int32_t cx = CX0 + lx, cy = CY0 + ly;
-        float sum = 0.0f, e = 0.0f;
-        if (cx >= 0 && cx < cw && cy >= 0 && cy < ch && tc.fl[i]) {
-            sum = rrestrict<kP2F>(tf, (2 * ly + kP2FR - 2 * kP2CR) * kP2F + 2 * lx + kP2FR - 2 * kP2CR);
-            const float id = tc.id[i];
-            const bool red = ((cx + cy) & 1) == 0;
-            e = (red && id != 0.0f) ? kOmega * (sum * id) : 0.0f;
-        }
-        tc.r[i] = sum;
-        tc.e[i] = e;
+        // (every region cell's children lie in tf: the sum needs no flag
+        // first; a cell outside the grid or not fluid takes zeros)
+        const float rs = rrestrict<kP2F>(tf, (2 * ly + kP2FR - 2 * kP2CR) * kP2F + 2 * lx + kP2FR - 2 * kP2CR);
+        const float id = tc.id[i];
+        const bool ok = cx >= 0 && cx < cw && cy >= 0 && cy < ch && tc.fl[i];
+        const bool red = ((cx + cy) & 1) == 0;
+        tc.r[i] = ok ? rs : 0.0f;
+        tc.e[i] = (ok && red && id != 0.0f) ? kOmega * (rs * id) : 0.0f;
     }
     __syncthreads();
     // level l + 1: black cells of the tile and its one-cell ring
