@@ -210,18 +210,32 @@ struct Extrapolate {
                 if (lx <= p || lx >= kExLW - 1 - p || ly <= p || ly >= kExLH - 1 - p) continue;
                 const int32_t gx = X0 + lx, gy = Y0 + ly;
                 if (gx < 0 || gx >= nx || gy < 0 || gy >= ny) continue;
-                if (ok[src][i]) {
-                    val[dst][i] = val[src][i];
+                // the face's and its neighbours' flags and values load at
+                // once; an invalid neighbour adds +0 (the sum starts at +0
+                // and never becomes -0, so it is the sum of the valid ones)
+                const uint8_t o = ok[src][i];
+                const uint8_t ol = ok[src][i - 1], orr = ok[src][i + 1];
+                const uint8_t ob = ok[src][i - kExLW], ot = ok[src][i + kExLW];
+                const float v = val[src][i];
+                const float vl = val[src][i - 1], vr = val[src][i + 1];
+                const float vb = val[src][i - kExLW], vt = val[src][i + kExLW];
+                const bool hl = gx > 0 && ol, hr = gx < nx - 1 && orr;
+                const bool hb = gy > 0 && ob, ht = gy < ny - 1 && ot;
+                float sum = 0.0f;
+                sum += hl ? vl : 0.0f;
+                sum += hr ? vr : 0.0f;
+                sum += hb ? vb : 0.0f;
+                sum += ht ? vt : 0.0f;
+                const uint32_t cnt = (uint32_t)hl + (uint32_t)hr + (uint32_t)hb + (uint32_t)ht;
+                if (o) {
+                    val[dst][i] = v;
+                    ok[dst][i] = 1;
+                } else if (cnt > 0) {
+                    val[dst][i] = sum / (float)cnt;
                     ok[dst][i] = 1;
                 } else {
-                    float sum = 0.0f;
-                    uint32_t cnt = 0;
-                    if (gx > 0 && ok[src][i - 1]) { sum += val[src][i - 1]; ++cnt; }
-                    if (gx < nx - 1 && ok[src][i + 1]) { sum += val[src][i + 1]; ++cnt; }
-                    if (gy > 0 && ok[src][i - kExLW]) { sum += val[src][i - kExLW]; ++cnt; }
-                    if (gy < ny - 1 && ok[src][i + kExLW]) { sum += val[src][i + kExLW]; ++cnt; }
-                    if (cnt > 0) { val[dst][i] = sum / (float)cnt; ok[dst][i] = 1; }
-                    else { val[dst][i] = val[src][i]; ok[dst][i] = 0; }
+                    val[dst][i] = v;
+                    ok[dst][i] = 0;
                 }
             }
             __syncthreads();
