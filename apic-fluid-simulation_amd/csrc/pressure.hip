@@ -396,6 +396,26 @@ __device__ inline T block_sum(T v, T* scratch) {
     return s;
 }
 
+// block_sum of two values at once: each gets exactly block_sum's operations
+// (the same shuffle tree and wave order), the two chains overlapping.
+// scratch holds 2 * (blockDim.x / 64) values.
+template <typename T>
+__device__ inline void block_sum2(T& a, T& b, T* scratch) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if (lane == 0) { scratch[wave] = a; scratch[nw + wave] = b; }
+    __syncthreads();
+    T sa = scratch[0], sb = scratch[nw];
+    for (int w = 1; w < nw; ++w) { sa += scratch[w]; sb += scratch[nw + w]; }
+    a = sa;
+    b = sb;
+}
+
 __device__ inline float block_max(float v, float* scratch) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fast_max(v, __shfl_xor(v, o, 64));
@@ -1101,6 +1121,43 @@ __device__ inline T sum_partials256(const T* __restrict__ part, int n, T* scratc
     return s;
 }
 
+// sum_partials256 of two partial arrays of n values at once (each value is
+// exactly sum_partials256's); scratch holds 8 values.
+template <typename T>
+__device__ inline void sum_partials256x2(const T* __restrict__ pa, const T* __restrict__ pb, int n, T* scratch,
+                                         T& ra, T& rb) {
+    T va = 0, vb = 0;
+    if (threadIdx.x < 256) {
+        for (int i0 = (int)threadIdx.x; i0 < n; i0 += 1024) {
+            T xa[4], xb[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int i = i0 + 256 * j;
+                xa[j] = pa[i < n ? i : 0];
+                xb[j] = pb[i < n ? i : 0];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (i0 + 256 * j < n) { va += xa[j]; vb += xb[j]; }
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        va += __shfl_xor(va, o, 64);
+        vb += __shfl_xor(vb, o, 64);
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && threadIdx.x < 256) {
+        scratch[threadIdx.x >> 6] = va;
+        scratch[4 + (threadIdx.x >> 6)] = vb;
+    }
+    __syncthreads();
+    T sa = scratch[0], sb = scratch[4];
+    for (int w = 1; w < 4; ++w) { sa += scratch[w]; sb += scratch[4 + w]; }
+    ra = sa;
+    rb = sb;
+}
+
 // Iteration k: gamma from the V-cycle's partials, s = z + beta*s (s = z when
 // k = 0), As = A s and the partials of delta = s.As (apply_matrix_and_dot,
 // src/apic.rs:1823-1863; search update :1945-1954).
@@ -1130,12 +1187,14 @@ __device__ __forceinline__ void spmv_dot(const LevelView& L, const float* __rest
     fetch_flags();
     float beta;
     if (part_gamma_d) {  // relative mode: FP64 scalars, flexible beta
-        const double g = sum_partials256(part_gamma_d, ngam, dscratch);
+        // gamma and (A s_{k-1}).z_k in one pass (the second is unused when k = 0)
+        __shared__ double dscratch2[8];
+        double g, asz;
+        sum_partials256x2(part_gamma_d, part_asz_d, ngam, dscratch2, g, asz);
         if (gamma_exit_rel(st, g, k, lead)) return;
         // Polak-Ribiere: r_{k}.(z_{k} - z_{k-1}) / r_{k-1}.z_{k-1}, where
         // r_{k-1}.z_k = r_k.z_k + alpha_{k-1} (A s_{k-1}).z_k
         if (!first) {
-            const double asz = sum_partials256(part_asz_d, ngam, dscratch);
             beta = (float)(-st->alpha_d * asz / st->gamma_d[(k - 1) & 1]);
         } else {
             beta = 0.0f;
@@ -2260,7 +2319,7 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     __shared__ VTile<T> t;
     __shared__ float ce[kC * kC];
     __shared__ float scratch[NT / 64];
-    __shared__ double dscratch[NT / 64];
+    __shared__ double dscratch[2 * (NT / 64)];  // (block_sum2)
     const int32_t w = (int32_t)L.w, h = (int32_t)L.h, cw = (int32_t)C.w, chh = (int32_t)C.h;
     const int32_t X0 = (int32_t)(wg % tiles_x) * T, Y0 = (int32_t)(wg / tiles_x) * T;
     const int32_t CX0 = X0 / 2 - 1, CY0 = Y0 / 2 - 1;
@@ -2393,11 +2452,10 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         }
     }
     if (dot == 2) {
-        const double bs = block_sum(dprod, dscratch);
-        const double ba = block_sum(dasz, dscratch);
+        block_sum2(dprod, dasz, dscratch);  // (each as block_sum alone)
         if (threadIdx.x == 0) {
-            part_gamma_d[wg] = bs;
-            part_asz_d[wg] = ba;
+            part_gamma_d[wg] = dprod;
+            part_asz_d[wg] = dasz;
         }
     } else if (dot == 1) {
         const float bs = block_sum(prod, scratch);
