@@ -383,10 +383,44 @@ __device__ inline T block_sum(T v, T* scratch);
 
 // ---- block-level reductions (fixed order, deterministic)
 
+// The xor butterfly over a wave, 32, 16, 8, 4, 2, 1 in that order.  The last
+// four steps stay inside a 16-lane row and take their partner from a DPP row
+// rotation: rotating by 8 is xor 8; after the steps before it, lanes whose
+// indices differ by 8 (then 4, then 2) hold equal values, so rotating by 4,
+// 2 and 1 delivers exactly the value xor 4, 2 and 1 would.  Every lane adds
+// the same operands in the same order as with shuffles alone.
+template <int N>
+__device__ __forceinline__ int row_ror(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, 0x120 + N, 0xF, 0xF, false);
+}
+template <int N>
+__device__ __forceinline__ float row_ror(float v) {
+    return __int_as_float(row_ror<N>(__float_as_int(v)));
+}
+template <int N>
+__device__ __forceinline__ double row_ror(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = row_ror<N>((int)(b & 0xffffffffll)), hi = row_ror<N>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
+}
+template <typename T, class Op>
+__device__ __forceinline__ T wave_butterfly(T v, Op op) {
+    v = op(v, __shfl_xor(v, 32, 64));
+    v = op(v, __shfl_xor(v, 16, 64));
+    v = op(v, row_ror<8>(v));
+    v = op(v, row_ror<4>(v));
+    v = op(v, row_ror<2>(v));
+    v = op(v, row_ror<1>(v));
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+    return wave_butterfly(v, [](T a, T b) { return a + b; });
+}
+
 template <typename T>
 __device__ inline T block_sum(T v, T* scratch) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    v = wave_sum(v);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     __syncthreads();
     if (lane == 0) scratch[wave] = v;
@@ -401,11 +435,8 @@ __device__ inline T block_sum(T v, T* scratch) {
 // scratch holds 2 * (blockDim.x / 64) values.
 template <typename T>
 __device__ inline void block_sum2(T& a, T& b, T* scratch) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        a += __shfl_xor(a, o, 64);
-        b += __shfl_xor(b, o, 64);
-    }
+    a = wave_sum(a);
+    b = wave_sum(b);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     __syncthreads();
     if (lane == 0) { scratch[wave] = a; scratch[nw + wave] = b; }
@@ -417,8 +448,7 @@ __device__ inline void block_sum2(T& a, T& b, T* scratch) {
 }
 
 __device__ inline float block_max(float v, float* scratch) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fast_max(v, __shfl_xor(v, o, 64));
+    v = wave_butterfly(v, [](float a, float b) { return fast_max(a, b); });
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     __syncthreads();
     if (lane == 0) scratch[wave] = v;
@@ -612,8 +642,7 @@ __device__ void coarse_cg_wave(const LevelView& L) {
                 r = r - alpha * ap;
                 m = fabsf(r);
             }
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) m = fast_max(m, __shfl_xor(m, o, 64));
+            m = wave_butterfly(m, [](float a, float b) { return fast_max(a, b); });
             if (m < 1e-6f) break;
             const float ng = serial_sum(r * r);
             const float beta = ng / gamma;
@@ -1111,8 +1140,7 @@ __device__ inline T sum_partials256(const T* __restrict__ part, int n, T* scratc
                 if (i0 + 256 * j < n) v += x[j];
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    v = wave_sum(v);
     __syncthreads();
     if ((threadIdx.x & 63) == 0 && threadIdx.x < 256) scratch[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -1141,11 +1169,8 @@ __device__ inline void sum_partials256x2(const T* __restrict__ pa, const T* __re
                 if (i0 + 256 * j < n) { va += xa[j]; vb += xb[j]; }
         }
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        va += __shfl_xor(va, o, 64);
-        vb += __shfl_xor(vb, o, 64);
-    }
+    va = wave_sum(va);
+    vb = wave_sum(vb);
     __syncthreads();
     if ((threadIdx.x & 63) == 0 && threadIdx.x < 256) {
         scratch[threadIdx.x >> 6] = va;
