@@ -383,6 +383,22 @@ __device__ inline T block_sum(T v, T* scratch);
 
 // ---- block-level reductions (fixed order, deterministic)
 
+// Whether pred holds on any thread of the workgroup: a ballot per wave, one
+// LDS slot per wave and one barrier (which also orders the LDS stores made
+// before it, as __syncthreads does).  At most one call per kernel: a second
+// call could overwrite the slots before every thread has read them.
+template <int NT>
+__device__ __forceinline__ bool block_any(bool pred) {
+    __shared__ uint32_t slots[NT / 64];
+    const bool wave_any = __ballot(pred) != 0ull;
+    if ((threadIdx.x & 63) == 0) slots[threadIdx.x >> 6] = wave_any;
+    __syncthreads();
+    uint32_t any = 0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) any |= slots[w];
+    return any != 0;
+}
+
 // The xor butterfly over a wave, 32, 16, 8, 4, 2, 1 in that order.  The last
 // four steps stay inside a 16-lane row and take their partner from a DPP row
 // rotation: rotating by 8 is xor 8; after the steps before it, lanes whose
@@ -1761,7 +1777,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     bool any = false;
 #pragma unroll
     for (int q = 0; q < G::kCPer; ++q) any = any || (cin[q] && cfl[q] != 0);
-    if (!__syncthreads_or(any)) {
+    if (!block_any<NT>(any)) {
         if constexpr (AX) {  // no fluid cell in the tile: a zero partial, as axpy_tile's
             if (threadIdx.x == 0) ax.part_rr_out[wg] = 0.0;
         }
@@ -2102,7 +2118,7 @@ __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, c
         }
     }
     // skip: no fluid cell of level l + 1 in the tile
-    if (!__syncthreads_or(region_any_fluid<kP2C, NT, kP2T>(gc, kP2CR))) return;
+    if (!block_any<NT>(region_any_fluid<kP2C, NT, kP2T>(gc, kP2CR))) return;
     // level l: black cells out to ring kP2FR - 1 (own value zero before the sweep)
     for (int i = threadIdx.x; i < kP2F * kP2F; i += NT) {
         const int lx = i % kP2F, ly = i / kP2F;
@@ -2240,7 +2256,7 @@ __device__ __forceinline__ void vup2(const LevelView& F, const LevelView& C, con
     }
     if (dn) return;
     // skip: no fluid cell of level l + 1 in the tile
-    if (!__syncthreads_or(region_any_fluid<kU2C, NT, kP2T>(gc, kU2CR))) return;
+    if (!block_any<NT>(region_any_fluid<kU2C, NT, kP2T>(gc, kU2CR))) return;
     // level l + 1: e plus the parent's error over the region (the diagonal's slot)
     for (int i = threadIdx.x; i < kU2C * kU2C; i += NT) {
         // (every region cell's parent lies in de: the load needs no flag)
@@ -2405,7 +2421,7 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         return;
     }
     // a tile without a fluid cell (see vdown) only reports a zero partial
-    if (!__syncthreads_or(cin && cfv != 0)) {
+    if (!block_any<NT>(cin && cfv != 0)) {
         if (threadIdx.x == 0) {
             if (dot == 2) { part_gamma_d[wg] = 0.0; part_asz_d[wg] = 0.0; }
             else if (dot == 1) part_gamma[wg] = 0.0f;
