@@ -12,10 +12,12 @@ FIELDS = [
     "SMOOTHED_DENSITY", "PRESSURE", "FLUID", "OBSTACLE", "OPEN_U", "OPEN_V",
     "DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y", "RESIDUAL", "ERROR", "SEARCH",
     "MATRIX_TIMES_SEARCH", "EXT_FORCE_U", "EXT_FORCE_V", "POST_FORCE_U", "POST_FORCE_V",
+    "PRESSURE_F64", "RHS",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 U32_FIELDS = {"SORT_KEY", "SORT_VAL"}
 U8_FIELDS = {"VALID_U", "VALID_V", "FLUID", "OBSTACLE", "OPEN_U", "OPEN_V"}
+F64_FIELDS = {"PRESSURE_F64"}
 
 STATUS = {0: "APIC_OK", 1: "APIC_ERR_INVALID_ARGUMENT", 2: "APIC_ERR_UNSUPPORTED_SIZE",
           3: "APIC_ERR_OUT_OF_MEMORY", 4: "APIC_ERR_HIP", 5: "APIC_ERR_NO_DEVICE"}
@@ -35,7 +37,7 @@ SYMBOLS = [
     "apic_batch_create_sim", "apic_batch_update", "apic_device_memory",
 ]
 TIMERS = ["sort", "p2g", "pressure_build", "mg_build", "pcg", "gradient", "extrapolate", "g2p",
-          "kernel_p2g", "kernel_g2p", "kernel_spmv"]
+          "kernel_p2g", "kernel_g2p", "kernel_spmv", "kernel_pcg_iteration"]
 
 
 class WorldPropertiesC(ctypes.Structure):

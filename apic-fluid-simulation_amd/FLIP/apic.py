@@ -171,6 +171,8 @@ class Apic:
 
     @staticmethod
     def _dtype(name):
+        if name in _lib.F64_FIELDS:
+            return np.float64
         return np.uint32 if name in _lib.U32_FIELDS else (np.uint8 if name in _lib.U8_FIELDS else np.float32)
 
     def read(self, name, level=0):

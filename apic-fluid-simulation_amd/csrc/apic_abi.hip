@@ -364,6 +364,17 @@ apic_status do_sort(Sim& s) {
     return APIC_OK;
 }
 
+// One PCG iteration (src/apic.rs:1899-1955): the SpMV and dot, the update
+// and the V-cycle with the next r.z.  The kernel timer brackets iteration 1
+// of each solve (the first with every operand).
+void pcg_iteration(Sim& s, int k) {
+    const bool timed = k == 1;
+    if (timed) s.mark(APIC_T_K_PCG_ITER, 0);
+    launch_pcg_iteration_head(s, k);
+    launch_vcycle(s, 2, k);
+    if (timed) s.mark(APIC_T_K_PCG_ITER, 1);
+}
+
 // solve_pcg (src/apic.rs:1865-1956) with the reference's rules.  Iterations
 // are queued in batches; the device-side state machine turns the kernels of
 // a finished solve into no-ops, and the host reads the state only at the end
@@ -376,11 +387,7 @@ apic_status do_pcg(Sim& s, bool zero_pressure) {
     int k = 0;
     int check_at = std::max(0, s.pcg_batch);
     while (true) {
-        while (k < check_at && k < pcg_cap(s)) {
-            launch_pcg_iteration_head(s, k);
-            launch_vcycle(s, 2, k);
-            ++k;
-        }
+        while (k < check_at && k < pcg_cap(s)) pcg_iteration(s, k++);
         launch_pcg_decide(s, k);  // also writes the host mirror
         CHECK_LAUNCH();
         HIP_TRY(hipStreamSynchronize(s.stream));
@@ -427,10 +434,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     launch_pcg_reset(s);
     launch_vcycle(s, 1);
     int k = 0;
-    for (; k < std::max(0, s.pcg_batch) && k < pcg_cap(s); ++k) {
-        launch_pcg_iteration_head(s, k);
-        launch_vcycle(s, 2, k);
-    }
+    for (; k < std::max(0, s.pcg_batch) && k < pcg_cap(s); ++k) pcg_iteration(s, k);
     launch_pcg_decide(s, k);
     s.mark(APIC_T_PCG, 1);
     HIP_TRY(hipEventRecord(s.e_sync, s.stream));
@@ -471,11 +475,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
         s.drop_marks(saved.marks);
         int check_at = k + 2;
         while (true) {
-            while (k < check_at && k < pcg_cap(s)) {
-                launch_pcg_iteration_head(s, k);
-                launch_vcycle(s, 2, k);
-                ++k;
-            }
+            while (k < check_at && k < pcg_cap(s)) pcg_iteration(s, k++);
             launch_pcg_decide(s, k);
             HIP_TRY(hipEventRecord(s.e_sync, s.stream));
             CHECK_LAUNCH();
@@ -528,6 +528,8 @@ bool field_desc(Sim& s, int f, uint32_t level, FieldDesc& d) {
         case APIC_POST_FORCE_V: d.ptr = s.post_v; d.count = s.nv(); return true;
         case APIC_SEARCH: d.ptr = s.search[s.h_pcg ? (s.h_pcg->iters & 1) : 0]; d.count = s.padded(s.W, s.H); return true;
         case APIC_MATRIX_TIMES_SEARCH: d.ptr = s.As; d.count = s.padded(s.W, s.H); return true;
+        case APIC_PRESSURE_F64: d.ptr = s.pressure_d; d.count = s.cells; d.elem = 8; return true;
+        case APIC_RHS: d.ptr = s.rhs; d.count = s.padded(s.W, s.H); return true;
         default: break;
     }
     if (!L) return false;

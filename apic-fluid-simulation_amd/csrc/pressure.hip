@@ -1279,15 +1279,30 @@ __device__ __forceinline__ void spmv_dot(const LevelView& L, const float* __rest
             float sv[5];
 #pragma unroll
             for (int d = 0; d < 5; ++d) sv[d] = first ? e[j][d] : e[j][d] + (beta * so[j][d]);
-            float v = dg[j] * sv[0];
-            v = fmaf(px[j][0], sv[1], v);
-            v = fmaf(px[j][1], sv[2], v);
-            v = fmaf(py[j][0], sv[3], v);
-            v = fmaf(py[j][1], sv[4], v);
             s_new[pc] = sv[0];
-            As[pc] = v;
-            if (part_delta_d) dprod += (double)sv[0] * (double)v;
-            else prod += v * sv[0];
+            if (part_delta_d) {
+                // relative mode: the products of FP32 operands are exact in
+                // FP64; the row is summed in FP64 (the same fused chain) and
+                // rounded once, so A s carries the rounding of its own size,
+                // not that of |A||s|, and the recurrence residual keeps
+                // tracking the true residual of a smooth pressure
+                double vd = (double)dg[j] * (double)sv[0];
+                vd = fma((double)px[j][0], (double)sv[1], vd);
+                vd = fma((double)px[j][1], (double)sv[2], vd);
+                vd = fma((double)py[j][0], (double)sv[3], vd);
+                vd = fma((double)py[j][1], (double)sv[4], vd);
+                const float v = (float)vd;
+                As[pc] = v;
+                dprod += (double)sv[0] * (double)v;
+            } else {
+                float v = dg[j] * sv[0];
+                v = fmaf(px[j][0], sv[1], v);
+                v = fmaf(px[j][1], sv[2], v);
+                v = fmaf(py[j][0], sv[3], v);
+                v = fmaf(py[j][1], sv[4], v);
+                As[pc] = v;
+                prod += v * sv[0];
+            }
         }
     }
     if (part_delta_d) {

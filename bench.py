@@ -9,13 +9,17 @@ A step is one Apic::update(dt) (src/apic.rs:2429) over all particles, with
 every array resident in HBM.  W untimed warm-up updates, then K timed updates
 bracketed by device synchronisation; value = sum of particles x substeps / wall
 seconds.  HIP events recorded by the library on its own stream time the P2G
-and G2P kernels and one SpMV launch per solve (iteration 1; events around all
-of them would add their own cost to the step) inside the timed region; the one
-with the largest share of the step is the roofline kernel, its achieved
-bandwidth (algorithmic bytes per launch / average launch time) reported
-against the 8 TB/s HBM peak.  A short second pass times every stage.  The
-CPU baseline is the oracle (FP32, one thread) on the first steps of the same
-workload, timed on this host.
+and G2P kernels, and one SpMV launch and one whole PCG iteration per solve
+(iteration 1; events around all of them would add their own cost to the
+step) inside the timed region; the unit with the largest share of the step
+is the roofline unit, its achieved bandwidth (algorithmic bytes per launch /
+average launch time) reported against the 8 TB/s HBM peak.  Every timed
+solve's exit reason and residual are recorded, the FP64 true residual
+||b - A p|| / ||b|| of the last one is computed on the host from the
+library's own system and pressure, and the line says whether every solve
+met the tolerance ("valid").  A short second pass times every stage.  The
+CPU baseline is the oracle (FP32) on the first steps of the same workload,
+timed on this host.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
                        [--solver relative|reference]
@@ -37,8 +41,9 @@ METRIC = "particle-steps/s at 1024² grid, 8 ppc; served mixed-size p50/p99 step
 WORKLOAD = "cfg2_dam_break_1024"
 
 
-def algorithmic_bytes(n_p, n_alloc, n_f, n_c):
-    """Compulsory HBM bytes per launch of the two transfer kernels.
+def algorithmic_bytes(n_p, n_alloc, n_f, n_c, levels):
+    """Compulsory HBM bytes per launch of the timed kernels, and per PCG
+    iteration.
 
     G2p: per particle lane read x, y, time residual (12 B), write the 9
     particle fields (36 B) and the next sort record (8 B); per fluid cell read
@@ -50,16 +55,29 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c):
     SpmvDot: per fluid cell read z, the old search vector, diag,
     plus_x, plus_y (20 B) and write the new search vector and A s (8 B); the
     fluid flag of every cell (1 B).
+    PCG iteration (SpmvDot, the update and the V-cycle with the next r.z):
+    68 B per base fluid cell (SpMV 24, update and residual norm 28, search
+    update 16), 8 B per base cell (the dense r.z of src/apic.rs:1936), and
+    per multigrid level l, base included, 89 B per fluid cell F_l (zeroing,
+    two smoothing sweeps of 28 B, restriction 21, prolongation 12) and 4 B
+    per cell C_l (its fluid flags as words).
     """
     return {
         "kernel_g2p": 48 * n_alloc + 8 * n_p + 10 * n_f,
         "kernel_p2g": 36 * n_p + 15 * n_c,
         "kernel_spmv": 28 * n_f + n_c,
+        "kernel_pcg_iteration": 68 * n_f + 8 * n_c + sum(89 * f + 4 * c for f, c in levels),
     }
 
 
 KERNEL_NAMES = {"kernel_g2p": "one_kernel<G2p>", "kernel_p2g": "one_kernel<P2gOwner>",
-                "kernel_spmv": "one_kernel<SpmvDot<1024> >"}
+                "kernel_spmv": "one_kernel<SpmvDot<1024> >",
+                "kernel_pcg_iteration": "PCG iteration (SpmvDot<1024>, VDown<32, 512, true>, 2 x VDown2, "
+                                        "BottomLds, 2 x VUp2, VUp<32, 512>)"}
+# the kernels of one PCG iteration of the 1024^2 workload and their launches
+PCG_ITERATION_KERNELS = {"one_kernel<SpmvDot<1024> >": 1, "one_kernel<VDown<32, 512, true> >": 1,
+                         "one_kernel<VDown2>": 2, "one_kernel<BottomLds>": 1, "one_kernel<VUp2>": 2,
+                         "one_kernel<VUp<32, 512> >": 1}
 BATCH_WORKLOAD = "cfg4_batched_256"
 
 
@@ -103,10 +121,43 @@ def cpu_baseline(xy, cfg, budget_s, solver):
     }
 
 
-def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_steps):
+def true_residual(g):
+    """||b - A p||_2 / ||b||_2 of the last pressure solve, in FP64 on the host,
+    for the FP64-accumulated pressure and for the FP32 pressure the step
+    uses: b is the relative mode's copy of the right-hand side, A the
+    assembled 5-point operator (DIAG, PLUS_X, PLUS_Y) over the fluid cells."""
+    import numpy as np
+    W, H = g.width, g.height
+    b = g.read("RHS").astype(np.float64)
+    diag = g.read("DIAG").astype(np.float64)
+    px = g.read("PLUS_X").astype(np.float64)
+    py = g.read("PLUS_Y").astype(np.float64)
+    idx = np.nonzero(g.read("FLUID"))[0] + W
+    nb = np.linalg.norm(b[idx])
+    out = {}
+    for name, p in (("fp64_pressure", g.read("PRESSURE_F64")), ("fp32_pressure", g.read("PRESSURE"))):
+        pp = np.zeros(W * H + 2 * W)
+        pp[W:W + W * H] = p.astype(np.float64)
+        ap = diag[idx] * pp[idx] + px[idx] * pp[idx + 1] + px[idx - 1] * pp[idx - 1] + \
+            py[idx] * pp[idx + W] + py[idx - W] * pp[idx - W]
+        out[name] = float(np.linalg.norm(b[idx] - ap) / nb) if nb > 0 else 0.0
+    return out
+
+
+def level_counts(g):
+    """(fluid cells, cells) of every multigrid level, base grid first."""
+    out = []
+    for lvl in range(g.dims()["num_levels"] + 1):
+        fl = g.read("FLUID", lvl)
+        out.append((int(fl.sum()), int(fl.size)))
+    return out
+
+
+def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_steps, verify=True):
     """W warm-up updates, K timed updates (HIP events around the two transfer
-    kernels and one SpMV per solve only), then a short pass with events around
-    every stage."""
+    kernels, one SpMV and one whole PCG iteration per solve only), the exit
+    state of every timed solve, the true residual of the last one, then a
+    short pass with events around every stage."""
     import numpy as np
     g = Apic.with_particles(cfg["width"], cfg["height"], cfg["dx"], props, xy)
     g.set_solver_config(**SOLVERS[solver])
@@ -116,7 +167,7 @@ def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_s
         g.update(opts, dt)
     g.set_profiling(1)  # HIP events around the timed kernels only
     g.get_max_particle_velocity()  # synchronises the library's stream
-    substeps, iters, fluid, rel = 0, [], [], []
+    substeps, iters, fluid, rel, reasons = 0, [], [], [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         g.update(opts, dt)
@@ -125,9 +176,12 @@ def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_s
         iters.append(st["pcg_iterations_total"])
         fluid.append(st["fluid_cells"])
         rel.append(st["pcg_rel_residual"])
+        reasons.append(st["pcg_exit_reason"])
     g.get_max_particle_velocity()
     elapsed = time.perf_counter() - t0
     timers = g.timers()
+    check = true_residual(g) if verify and SOLVERS[solver]["mode"] == 1 else None
+    levels = level_counts(g)
     g.set_profiling(2)
     stage_substeps = 0
     for _ in range(stage_steps):
@@ -137,8 +191,9 @@ def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_s
     g.set_profiling(0)
     n_alloc = g.field_length("POS_X")
     g.close()
-    return dict(elapsed=elapsed, substeps=substeps, iters=iters, fluid=fluid, rel=rel, timers=timers,
-                stage_timers=stage_timers, stage_substeps=stage_substeps, n_alloc=n_alloc)
+    return dict(elapsed=elapsed, substeps=substeps, iters=iters, fluid=fluid, rel=rel, reasons=reasons,
+                timers=timers, stage_timers=stage_timers, stage_substeps=stage_substeps, n_alloc=n_alloc,
+                true_residual=check, levels=levels)
 
 
 def batched_run(count=256, side_min=64, side_max=512, seed=4, steps=10, warmup=3, seq_rounds=2, dt=0.002):
@@ -308,16 +363,25 @@ def copy_calibration(gib=1.0, repeats=3):
                       "gbs": 2 * 4 * n / dt / 1e9}))
 
 
-def traffic_summary():
-    """HBM bytes per launch of the roofline kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each divided by what the
-    counter reads per byte of a plain copy), or None."""
+def traffic_summary(unit):
+    """HBM bytes per launch of `unit` (a kernel name, or the PCG iteration)
+    from the committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each
+    divided by what the counter reads per byte of a plain copy), or None."""
     path = os.path.join(ROOT, "profiles", "traffic_dam_break_1024.json")
     try:
         with open(path) as f:
-            return json.load(f)
+            tr = json.load(f)
     except (OSError, ValueError):
         return None
+    per = tr.get("hbm_bytes_per_launch")
+    if not isinstance(per, dict):
+        return None
+    if unit == "kernel_pcg_iteration":
+        if not all(k in per for k in PCG_ITERATION_KERNELS):
+            return None
+        return int(sum(per[k] * n for k, n in PCG_ITERATION_KERNELS.items()))
+    v = per.get(KERNEL_NAMES[unit])
+    return int(v) if v is not None else None
 
 
 def main():
@@ -369,28 +433,33 @@ def main():
 
     value = n_p * r["substeps"] / r["elapsed"]
     n_f = float(np.mean(r["fluid"]))
-    algo = algorithmic_bytes(n_p, r["n_alloc"], n_f, cfg["width"] * cfg["height"])
+    algo = algorithmic_bytes(n_p, r["n_alloc"], n_f, cfg["width"] * cfg["height"], r["levels"])
     kernels = {}
-    # the SpMV's timer samples one launch per solve (iteration 1): its share
-    # of the step is its average times the launches per step, one per
-    # PCG iteration
-    spmv_per_step = float(np.sum(r["iters"])) / args.steps
-    for k in ("kernel_g2p", "kernel_p2g", "kernel_spmv"):
+    # the SpMV and PCG-iteration timers sample one iteration per solve
+    # (iteration 1): their share of the step is their average times the
+    # iterations per step
+    per_step_iters = float(np.sum(r["iters"])) / args.steps
+    for k in ("kernel_g2p", "kernel_p2g", "kernel_spmv", "kernel_pcg_iteration"):
         ms, cnt = r["timers"][k]
         avg_ms = ms / max(cnt, 1)
-        per_step = avg_ms * spmv_per_step if k == "kernel_spmv" else ms / args.steps
+        per_step = avg_ms * per_step_iters if k in ("kernel_spmv", "kernel_pcg_iteration") else ms / args.steps
         kernels[k] = {"avg_ms": avg_ms, "launches": cnt, "ms_per_step": per_step, "bytes": algo[k],
                       "gbs": algo[k] / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else None}
-    # the dominant kernel: the largest share of the step among the timed ones
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    # the dominant unit: the largest share of the step among the timed ones
+    # (the SpMV is part of the PCG iteration and is not a candidate)
+    dom = max(("kernel_g2p", "kernel_p2g", "kernel_pcg_iteration"), key=lambda k: kernels[k]["ms_per_step"])
     d = kernels[dom]
     stages = {k: round(v[0] / max(r["stage_substeps"], 1), 4) for k, v in r["stage_timers"].items()
               if not k.startswith("kernel_")}
     rels = [x for x in r["rel"] if x >= 0]
-    tr = traffic_summary()
-    traffic = None
-    if tr and tr.get("kernel") == KERNEL_NAMES[dom]:
-        traffic = tr["hbm_bytes_per_launch"]
+    tol = SOLVERS[args.solver].get("rel_tol")
+    # every timed solve ended by the tolerance, and the last one's FP64
+    # pressure meets it in the true residual (10% slack for the rounding of
+    # the FP32 recurrence)
+    tr64 = (r["true_residual"] or {}).get("fp64_pressure")
+    valid = all(x == 3 for x in r["reasons"]) and (tol is None or (
+        all(x <= tol for x in rels) and tr64 is not None and tr64 <= 1.1 * tol))
+    traffic = traffic_summary(dom)
 
     line = {
         "metric": METRIC,
@@ -411,11 +480,16 @@ def main():
         "roofline": {"bound": "hbm", "kernel": KERNEL_NAMES[dom], "achieved": d["gbs"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": (d["gbs"] / HBM_PEAK_GBS) if d["gbs"] else None, "traffic": traffic,
-                     "bytes_per_launch": d["bytes"], "avg_launch_ms": d["avg_ms"]},
+                     "bytes_per_launch": d["bytes"], "avg_launch_ms": d["avg_ms"],
+                     "levels_fluid_cells": [f for f, _ in r["levels"]],
+                     "levels_cells": [c for _, c in r["levels"]]},
         "kernels": kernels,
         "stage_ms_per_step": stages,
         "pcg_iterations_mean": float(np.mean(r["iters"])),
         "pcg_rel_residual_max": max(rels) if rels else None,
+        "pcg_exit_reasons": {str(k): int(sum(1 for x in r["reasons"] if x == k)) for k in sorted(set(r["reasons"]))},
+        "pcg_true_rel_residual_last": r["true_residual"],
+        "valid": valid,
         "substeps": r["substeps"],
         "other_solver": {"solver": SOLVER_NAMES[other], "steps": o_steps,
                          "value": n_p * ro["substeps"] / ro["elapsed"],

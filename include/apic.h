@@ -61,11 +61,13 @@ typedef struct apic_debug_options {
  * when ||r||_2 <= rel_tol * ||b||_2 or after max_iterations (1..100000), with
  * breakdown exits when s.As <= 0 or r.z <= 0; vectors stay FP32, the dot
  * products, ||r||, ||b|| and the step scalars are FP64 (fp64_reductions must
- * be 1), the pressure is accumulated in FP64, and the true residual b - A p
- * (FP64) replaces r every recompute_every iterations (0 = never).
- * pcg_rel_residual reports ||r||_2 / ||b||_2 of the residual the test used.
- * With FP32 SpMV the true residual levels off near eps32 * || |A| |p| || / ||b||,
- * so a rel_tol below that is met by the recurrence residual only. */
+ * be 1), each row of A s is summed in FP64 and rounded once to FP32, the
+ * pressure is accumulated in FP64 (APIC_PRESSURE_F64), and the true residual
+ * b - A p (FP64) replaces r every recompute_every iterations (0 = never).
+ * pcg_rel_residual reports ||r||_2 / ||b||_2 of the residual the test used;
+ * the FP64 rows keep it within rounding of the true residual of the FP64
+ * pressure.  The FP32 pressure (APIC_PRESSURE) cannot have a true residual
+ * much below eps32 * || |A| |p| || / ||b||. */
 typedef struct apic_solver_config {
     int32_t mode;
     float rel_tol;
@@ -90,7 +92,8 @@ typedef struct apic_step_stats {
 } apic_step_stats;
 
 /* Fields readable / writable for host mirrors and per-stage tests.
- * Element type F32 unless noted: U32 for the sort record, U8 (0/1) for flags. */
+ * Element type F32 unless noted: U32 for the sort record, U8 (0/1) for flags,
+ * F64 for APIC_PRESSURE_F64. */
 typedef enum apic_field {
     APIC_POS_X = 0, APIC_POS_Y, APIC_VEL_X, APIC_VEL_Y,
     APIC_C_U_X, APIC_C_U_Y, APIC_C_V_X, APIC_C_V_Y, APIC_TIME_RESIDUAL, /* num_chunks*8 */
@@ -112,6 +115,8 @@ typedef enum apic_field {
     APIC_SEARCH, APIC_MATRIX_TIMES_SEARCH,     /* padded, base level */
     APIC_EXT_FORCE_U, APIC_EXT_FORCE_V,        /* u faces, v faces */
     APIC_POST_FORCE_U, APIC_POST_FORCE_V,
+    APIC_PRESSURE_F64,        /* F64, W*H: the relative mode's FP64-accumulated pressure (extension) */
+    APIC_RHS,                 /* padded, base level: the relative mode's copy of b (extension) */
     APIC_FIELD_COUNT
 } apic_field;
 
@@ -187,11 +192,12 @@ typedef enum apic_timer {
     APIC_T_K_P2G,   /* the P2G kernel alone */
     APIC_T_K_G2P,   /* the G2P gather/advect kernel alone */
     APIC_T_K_SPMV,  /* the PCG's SpMV + dot kernel alone (iteration 1 of each solve) */
+    APIC_T_K_PCG_ITER, /* one whole PCG iteration: SpMV, update and V-cycle (iteration 1 of each solve) */
     APIC_TIMER_COUNT
 } apic_timer;
 
 /* enable: 0 off, 1 the kernel timers only (APIC_T_K_P2G, APIC_T_K_G2P,
- * APIC_T_K_SPMV), 2 every timer.  Resets the timers. */
+ * APIC_T_K_SPMV, APIC_T_K_PCG_ITER), 2 every timer.  Resets the timers. */
 apic_status apic_set_profiling(apic_handle* h, int enable);
 /* ms[i] and counts[i] for i < n (n <= APIC_TIMER_COUNT); either may be null. */
 apic_status apic_get_timers(apic_handle* h, double* ms, uint32_t* counts, uint32_t n);

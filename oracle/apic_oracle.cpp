@@ -729,13 +729,14 @@ template <class R> struct Sim {
     // Solver rules of the "relative" mode, an extension the reference cannot
     // express (its solve_pcg, src/apic.rs:1865-1956, hardcodes absolute exits
     // and 100 iterations): preconditioned CG with the same V-cycle and FP32
-    // vectors s, As, r, z; the pressure is accumulated in FP64 (written to the
-    // FP32 pressure as it goes); dot products, norms and step scalars in A
-    // (FP64 when fp64); stop when the residual meets ||r||_2 <= rel_tol *
-    // ||b||_2, or after max_iterations; every recompute_every iterations r is
-    // replaced by the true residual b - A p evaluated in FP64; breakdown when
-    // s.As <= 0 or r.z <= 0.  Sums run in ascending cell order.  With FP32
-    // SpMV the true residual cannot fall much below eps32 || |A| |p| || / ||b||.
+    // vectors s, As, r, z; A s summed per row in FP64 and rounded once; the
+    // pressure is accumulated in FP64 (written to the FP32 pressure as it
+    // goes); dot products, norms and step scalars in A (FP64 when fp64); stop
+    // when the residual meets ||r||_2 <= rel_tol * ||b||_2, or after
+    // max_iterations; every recompute_every iterations r is replaced by the
+    // true residual b - A p evaluated in FP64; breakdown when s.As <= 0 or
+    // r.z <= 0.  Sums run in ascending cell order.  The FP32 pressure itself
+    // cannot have a true residual much below eps32 || |A| |p| || / ||b||.
     int solver_mode = 0;
     double solver_rel_tol = 1e-6;
     int solver_max_it = 500, solver_fp64 = 1, solver_recompute = 25;
@@ -766,7 +767,18 @@ template <class R> struct Sim {
         auto pd_at = [&](int64_t raw) { return raw >= 0 && raw < (int64_t)cells ? pd[(size_t)raw] : 0.0; };
         for (int it = 0; it < solver_max_it; ++it) {
             stats.pcg_iterations = it + 1;
-            apply_matrix_and_dot(search, As);
+            // A s with FP64 row sums rounded once to R (for R = float: the
+            // products of two floats are exact in double), so that the
+            // recurrence residual tracks the true residual
+            for (uint32_t k = 0; k < fluid_cells; ++k) {
+                const size_t i = base.fluid_idx[k];
+                double v = double(base.diag[i]) * double(search[i]);
+                v = std::fma(double(base.px[i]), double(search[i + 1]), v);
+                v = std::fma(double(base.px[i - 1]), double(search[i - 1]), v);
+                v = std::fma(double(base.py[i]), double(search[i + w]), v);
+                v = std::fma(double(base.py[i - w]), double(search[i - w]), v);
+                As[i] = R(v);
+            }
             const A delta = fluid_sum([&](size_t i) { return A(search[i]) * A(As[i]); });
             if (!(delta > 0)) { stats.pcg_exit_reason = 2; return; }
             const A alpha_a = gamma / delta;

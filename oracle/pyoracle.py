@@ -19,6 +19,7 @@ FIELDS = [
     "SMOOTHED_DENSITY", "PRESSURE", "FLUID", "OBSTACLE", "OPEN_U", "OPEN_V",
     "DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y", "RESIDUAL", "ERROR", "SEARCH",
     "MATRIX_TIMES_SEARCH", "EXT_FORCE_U", "EXT_FORCE_V", "POST_FORCE_U", "POST_FORCE_V",
+    "PRESSURE_F64", "RHS",
 ]
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 U32_FIELDS = {"SORT_KEY", "SORT_VAL"}

@@ -37,7 +37,7 @@ def test_field_enum_matches_header():
     from FLIP import _lib
     text = open(os.path.join(ROOT, "include", "apic.h")).read()
     body = text[text.index("typedef enum apic_field"):text.index("} apic_field;")]
-    names = re.findall(r"APIC_([A-Z_]+)", body)
+    names = re.findall(r"APIC_([A-Z0-9_]+)", body)
     names = [n for n in names if n != "FIELD_COUNT"]
     assert names == _lib.FIELDS
 

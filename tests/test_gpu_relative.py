@@ -5,7 +5,8 @@ exits only) against the oracle's relative mode and the assembled system.
 Bars: iteration counts within one of the FP32 oracle's; pressure within 1e-5
 relative L2 of it (the FP64 sums run in another order); ||b - A p|| / ||b||
 evaluated in FP64 from the GPU's own pressure within rel_tol plus the
-rounding of an FP32 pressure (8 eps32 || |A| |p| || / ||b||).
+rounding of an FP32 pressure (8 eps32 || |A| |p| || / ||b||), and from its
+FP64-accumulated pressure within 1.1 rel_tol.
 """
 import numpy as np
 import pytest
@@ -46,6 +47,9 @@ def test_relative_solve_matches_oracle(name, every):
     assert rel_l2(g.read("PRESSURE"), o.read("PRESSURE")) <= 1e-5
     res, bound = residuals(*sys_, g.read("PRESSURE"))
     assert res <= tol + bound, (res, bound)
+    res64, _ = residuals(*sys_, g.read("PRESSURE_F64"))
+    print(f"{name} every={every}: true residual of the FP64 pressure {res64:.3e}")
+    assert res64 <= 1.1 * tol, res64
     g.close()
 
 
