@@ -34,7 +34,7 @@ SYMBOLS = [
     "apic_solve_pcg", "apic_apply_pressure_gradient", "apic_extrapolate_velocity",
     "apic_transfer_grid_to_particles_and_advect", "apic_set_profiling", "apic_get_timers",
     "apic_set_gravity_x", "apic_device_bytes", "apic_batch_create", "apic_batch_destroy",
-    "apic_batch_create_sim", "apic_batch_update", "apic_device_memory",
+    "apic_batch_create_sim", "apic_batch_update", "apic_device_memory", "apic_batch_update_noalloc",
 ]
 TIMERS = ["sort", "p2g", "pressure_build", "mg_build", "pcg", "gradient", "extrapolate", "g2p",
           "kernel_p2g", "kernel_g2p", "kernel_spmv", "kernel_pcg_iteration"]
@@ -117,6 +117,7 @@ def lib():
                                         ctypes.POINTER(vp)]
     L.apic_batch_update.argtypes = [vp, ctypes.POINTER(vp), u32, ctypes.POINTER(DebugOptionsC),
                                     ctypes.POINTER(f32)]
+    L.apic_batch_update_noalloc.argtypes = L.apic_batch_update.argtypes
     for name in SYMBOLS:
         fn = getattr(L, name)
         if name not in ("apic_destroy", "apic_last_error"):

@@ -270,17 +270,19 @@ class Batch:
         sim._batch = self  # the batch outlives its simulations
         return sim
 
-    def update(self, sims, debug_options, frame_deltatime):
+    def update(self, sims, debug_options, frame_deltatime, noalloc=False):
         """Apic.update(debug_options, dt) of every simulation in `sims`, together;
-        frame_deltatime is one value or one per simulation."""
+        frame_deltatime is one value or one per simulation.  noalloc: through
+        apic_batch_update_noalloc (launch tables in the library's fixed
+        device arena, no device allocation)."""
         n = len(sims)
         if n == 0:
             return
         dts = np.broadcast_to(np.asarray(frame_deltatime, dtype=np.float32), (n,)).copy()
         hs = (ctypes.c_void_p * n)(*[s._h.value for s in sims])
         opts = _lib.DebugOptionsC(1 if debug_options.disable_particle_sort else 0)
-        check(lib().apic_batch_update(self._h, hs, n, ctypes.byref(opts),
-                                      dts.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        fn = lib().apic_batch_update_noalloc if noalloc else lib().apic_batch_update
+        check(fn(self._h, hs, n, ctypes.byref(opts), dts.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -38,6 +39,13 @@ struct apic_batch {
     size_t pinned_cap = 0;
     uint8_t* dev = nullptr;
     size_t dev_cap = 0;
+    int device = 0;
+    hipEvent_t e_arena = nullptr;  // end of this batch's last no-allocation update
+    // Set while apic_batch_update_noalloc runs: the launch tables go to this
+    // fixed device arena (the library's static scratch) instead of `dev`.
+    uint8_t* fixed = nullptr;
+    size_t fixed_cap = 0;
+    uint8_t* arena() const { return fixed ? fixed : dev; }
 };
 
 namespace {
@@ -568,9 +576,58 @@ apic_status create(uint32_t width, uint32_t height, float cell_size, const apic_
 
 // ---- batched stepping
 
+// The arena of apic_batch_update_noalloc: the launch tables and workgroup
+// maps of one substep of a batched update, in a static device array instead
+// of memory the batch allocates.  Every GPU the library runs on holds its own
+// copy until the process ends.  Every no-allocation call on that GPU shares
+// it: the calls are serialised on the host (g_noalloc_call), and a call's
+// stream waits for the event that ended the previous call (another batch's
+// stream, perhaps) before its first copy into the arena.  Nothing in it has
+// to start at zero: a substep writes its tables before its kernels read them.
+constexpr size_t kNoAllocArenaBytes = (size_t)APIC_NOALLOC_ARENA_BYTES;
+static __device__ __align__(256) uint8_t scratch[kNoAllocArenaBytes];
+
+// Device address of `scratch` on each device, looked up once per device
+// with that device current (host code never uses the symbol itself as a
+// device pointer), and the per-device order of the calls.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;                 // guards the two arrays below
+uint8_t* g_scratch_ptr[kMaxDevices] = {};
+apic_batch* g_arena_last[kMaxDevices] = {};  // batch whose e_arena ends the last call
+std::mutex g_noalloc_call[kMaxDevices];
+
+apic_status noalloc_arena(int device, uint8_t** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(APIC_ERR_INVALID_ARGUMENT, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int prev = 0;
+        HIP_TRY(hipGetDevice(&prev));
+        if (prev != device) HIP_TRY(hipSetDevice(device));
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (prev != device) HIP_TRY(hipSetDevice(prev));
+        if (e != hipSuccess || !p)
+            return fail(APIC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<uint8_t*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return APIC_OK;
+}
+
 // Uploads host[0, used) to the batch's device arena (in stream order, before
 // the kernels that read it).
 apic_status batch_upload(apic_batch& B, size_t used) {
+    if (B.fixed) {
+        // the no-allocation path: a copy from pageable host memory (the
+        // source can be reused when the call returns) into the fixed arena,
+        // in stream order after every kernel that still reads the arena
+        if (used > B.fixed_cap)
+            return fail(APIC_ERR_INVALID_ARGUMENT,
+                        "batched launch tables need " + std::to_string(used) + " bytes, more than the " +
+                            std::to_string(B.fixed_cap) + " bytes of the no-allocation arena");
+        HIP_TRY(hipMemcpyAsync(B.fixed, B.host.data(), used, hipMemcpyHostToDevice, B.stream));
+        return APIC_OK;
+    }
     if (B.copy_pending) {
         HIP_TRY(hipEventSynchronize(B.e_copied));
         B.copy_pending = false;
@@ -696,7 +753,7 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
         }
         if ((st = batch_compile(B, used, tail)) != APIC_OK) return st;
         if ((st = batch_upload(B, used)) != APIC_OK) return st;
-        HIP_LAUNCH_TRY(issue(head, B.dev, B.stream));
+        HIP_LAUNCH_TRY(issue(head, B.arena(), B.stream));
         int batch_len = 0, cap = 0;
         std::vector<int> prev;
         for (size_t i : act) {
@@ -712,8 +769,8 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
         int k = 0, check_at = median > 0 && median < batch_len ? median : batch_len;
         size_t in_program = act.size();
         while (true) {
-            for (; k < check_at && k < cap; ++k) HIP_LAUNCH_TRY(issue(iter, B.dev, B.stream, k));
-            HIP_LAUNCH_TRY(issue(decide, B.dev, B.stream, k));
+            for (; k < check_at && k < cap; ++k) HIP_LAUNCH_TRY(issue(iter, B.arena(), B.stream, k));
+            HIP_LAUNCH_TRY(issue(decide, B.arena(), B.stream, k));
             HIP_TRY(hipEventRecord(B.e_sync, B.stream));
             HIP_TRY(hipEventSynchronize(B.e_sync));
             std::vector<size_t> live;
@@ -727,6 +784,8 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
             // return after one load.  The GPU is idle here, and a simulation's
             // launches are the same in either program, so its bits are too.
             if (live.size() * 4 <= in_program * 3) {
+                const size_t used0 = used;
+                std::vector<Recorder::Launch> iter0 = iter, decide0 = decide;
                 for (size_t r = 0; r < live.size(); ++r) {
                     Sim& s = *sims[live[r]];
                     B.rec.begin((uint32_t)r);
@@ -744,11 +803,19 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
                     s.rec = nullptr;
                 }
                 if ((st = batch_compile(B, used, decide)) != APIC_OK) return st;
-                if ((st = batch_upload(B, used)) != APIC_OK) return st;
+                if (B.fixed && used > B.fixed_cap) {
+                    // the fixed arena has no room for the shorter program:
+                    // keep the full one (its ended solves' kernels return at once)
+                    used = used0;
+                    iter.swap(iter0);
+                    decide.swap(decide0);
+                } else if ((st = batch_upload(B, used)) != APIC_OK) {
+                    return st;
+                }
                 in_program = live.size();
             }
         }
-        HIP_LAUNCH_TRY(issue(tail, B.dev, B.stream));
+        HIP_LAUNCH_TRY(issue(tail, B.arena(), B.stream));
         for (size_t i : act) {
             Sim& s = *sims[i];
             s.step_src = nullptr;
@@ -840,10 +907,13 @@ apic_status apic_batch_create(apic_batch** out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(APIC_ERR_NO_DEVICE, "no HIP device found");
     apic_batch* b = new (std::nothrow) apic_batch();
     if (!b) return fail(APIC_ERR_OUT_OF_MEMORY, "host allocation failed");
-    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    hipError_t e = hipGetDevice(&b->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->e_sync, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->e_copied, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->e_arena, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (b->e_copied) (void)hipEventDestroy(b->e_copied);
         if (b->e_sync) (void)hipEventDestroy(b->e_sync);
         if (b->stream) (void)hipStreamDestroy(b->stream);
         delete b;
@@ -857,10 +927,15 @@ apic_status apic_batch_destroy(apic_batch* b) {
     if (!b) return APIC_OK;
     if (b->members) return fail(APIC_ERR_INVALID_ARGUMENT, "the batch still has simulations");
     (void)hipStreamSynchronize(b->stream);
+    if (b->device >= 0 && b->device < kMaxDevices) {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (g_arena_last[b->device] == b) g_arena_last[b->device] = nullptr;
+    }
     if (b->pinned) (void)hipHostFree(b->pinned);
     if (b->dev) (void)hipFree(b->dev);
     (void)hipEventDestroy(b->e_sync);
     (void)hipEventDestroy(b->e_copied);
+    (void)hipEventDestroy(b->e_arena);
     (void)hipStreamDestroy(b->stream);
     delete b;
     return APIC_OK;
@@ -873,14 +948,12 @@ apic_status apic_batch_create_sim(apic_batch* b, uint32_t width, uint32_t height
     return create(width, height, cell_size, world_properties, xy != nullptr, xy, xy ? n : 0, out, b);
 }
 
-apic_status apic_batch_update(apic_batch* b, apic_handle* const* sims, uint32_t count,
-                              const apic_debug_options* opts, const float* frame_dt) {
-    if (!b) return fail(APIC_ERR_INVALID_ARGUMENT, "null batch");
-    if (count == 0) return APIC_OK;
+// The simulations of a batch update, checked.
+apic_status batch_members(apic_batch* b, apic_handle* const* sims, uint32_t count, const float* frame_dt,
+                          std::vector<Sim*>& v) {
     if (!sims || !frame_dt) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
     if (count > kMaxBatchRows) return fail(APIC_ERR_INVALID_ARGUMENT, "more than 4096 simulations in one update");
-    std::vector<Sim*> v(count);
-    std::vector<float> fdt(frame_dt, frame_dt + count);
+    v.assign(count, nullptr);
     for (uint32_t i = 0; i < count; ++i) {
         if (!sims[i] || sims[i]->batch != b)
             return fail(APIC_ERR_INVALID_ARGUMENT, "a simulation is not part of this batch");
@@ -888,7 +961,45 @@ apic_status apic_batch_update(apic_batch* b, apic_handle* const* sims, uint32_t 
         for (uint32_t j = 0; j < i; ++j)
             if (v[j] == v[i]) return fail(APIC_ERR_INVALID_ARGUMENT, "a simulation is listed twice");
     }
+    return APIC_OK;
+}
+
+apic_status apic_batch_update(apic_batch* b, apic_handle* const* sims, uint32_t count,
+                              const apic_debug_options* opts, const float* frame_dt) {
+    if (!b) return fail(APIC_ERR_INVALID_ARGUMENT, "null batch");
+    if (count == 0) return APIC_OK;
+    std::vector<Sim*> v;
+    apic_status st = batch_members(b, sims, count, frame_dt, v);
+    if (st != APIC_OK) return st;
+    std::vector<float> fdt(frame_dt, frame_dt + count);
     return batch_update(*b, v, opts && opts->disable_particle_sort, fdt);
+}
+
+apic_status apic_batch_update_noalloc(apic_batch* b, apic_handle* const* sims, uint32_t count,
+                                      const apic_debug_options* opts, const float* frame_dt) {
+    if (!b) return fail(APIC_ERR_INVALID_ARGUMENT, "null batch");
+    if (count == 0) return APIC_OK;
+    std::vector<Sim*> v;
+    apic_status st = batch_members(b, sims, count, frame_dt, v);
+    if (st != APIC_OK) return st;
+    uint8_t* arena = nullptr;
+    if ((st = noalloc_arena(b->device, &arena)) != APIC_OK) return st;
+    std::vector<float> fdt(frame_dt, frame_dt + count);
+    std::lock_guard<std::mutex> call(g_noalloc_call[b->device]);
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        apic_batch* last = g_arena_last[b->device];
+        if (last && last != b) HIP_TRY(hipStreamWaitEvent(b->stream, last->e_arena, 0));
+    }
+    b->fixed = arena;
+    b->fixed_cap = kNoAllocArenaBytes;
+    st = batch_update(*b, v, opts && opts->disable_particle_sort, fdt);
+    b->fixed = nullptr;
+    b->fixed_cap = 0;
+    HIP_TRY(hipEventRecord(b->e_arena, b->stream));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    g_arena_last[b->device] = b;
+    return st;
 }
 
 apic_status apic_update(apic_handle* h, const apic_debug_options* opts, float frame_dt) {

@@ -234,6 +234,23 @@ apic_status apic_batch_create_sim(apic_batch* b, uint32_t width, uint32_t height
  * work is queued (as apic_update). */
 apic_status apic_batch_update(apic_batch* b, apic_handle* const* sims, uint32_t count,
                               const apic_debug_options* opts, const float* frame_dt);
+/* apic_batch_update without any device allocation (extension; no counterpart
+ * in the reference).  For callers that cannot allocate device memory while
+ * they step: serving loops that must not stall on hipMalloc/hipFree, code
+ * that forbids allocation after set-up.  apic_batch_update uploads each
+ * substep's launch tables into a device arena that the batch grows with
+ * hipMalloc; this variant puts them in a static device array of
+ * APIC_NOALLOC_ARENA_BYTES bytes that the library holds on every GPU it runs
+ * on until the process ends, and computes the same bits.  An update whose
+ * tables need more fails with APIC_ERR_INVALID_ARGUMENT before it steps any
+ * simulation (tables take about 20-80 KB per simulation, by grid size).  Every
+ * call on one GPU shares that array: calls on one GPU are serialised on the
+ * host, and each call's stream waits for the previous call's kernels before
+ * it rewrites the array, so calls from several threads or batches are
+ * ordered, not overlapped. */
+#define APIC_NOALLOC_ARENA_BYTES (32u << 20)
+apic_status apic_batch_update_noalloc(apic_batch* b, apic_handle* const* sims, uint32_t count,
+                                      const apic_debug_options* opts, const float* frame_dt);
 /* Device memory (bytes) the simulation holds, for admission by free HBM. */
 apic_status apic_device_bytes(apic_handle* h, size_t* bytes);
 /* Free and total memory of the current HIP device (hipMemGetInfo). */
