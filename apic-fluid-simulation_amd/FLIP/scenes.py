@@ -117,12 +117,22 @@ def instance_mix(count, side_min, side_max, seed):
 
 
 CONFIGS = {
-    # (width, height, dx, dt, steps, ppc, seed, fill_x, fill_y)
+    # dam breaks: the block fills the left 40% x bottom 60%
     "cfg1_dam_break_128": dict(width=128, height=128, dx=1.0 / 128, dt=0.002, steps=100, ppc=4, seed=1),
     "cfg2_dam_break_1024": dict(width=1024, height=1024, dx=1.0 / 1024, dt=2.5e-4, steps=200, ppc=8, seed=2),
+    # the mixed-precision large case: the bottom half filled plus a 512 x 512
+    # block near the top (35,602,448 particles); the workload names no dt, so
+    # it is cfg2's scaled with the cell size (the same CFL number)
+    "cfg3_falling_block_4096x2048": dict(width=4096, height=2048, dx=1.0 / 4096, dt=6.25e-5, steps=50, ppc=8,
+                                         seed=3, scene="falling_block", block=512),
+    # the same construction at 1/8 scale, small enough for the oracle in tests
+    "falling_block_512x256": dict(width=512, height=256, dx=1.0 / 512, dt=5e-4, steps=50, ppc=8, seed=3,
+                                  scene="falling_block", block=64),
 }
 
 
 def config_scene(name):
     c = CONFIGS[name]
+    if c.get("scene") == "falling_block":
+        return falling_block(c["width"], c["height"], c["dx"], ppc=c["ppc"], seed=c["seed"], block=c["block"])
     return dam_break(c["width"], c["height"], c["dx"], 0.4, 0.6, c["ppc"], c["seed"])

@@ -956,6 +956,14 @@ struct BottomLds {
 
 // ---------------------------------------------------------------------- PCG
 
+// The end of a solve: every later kernel of the solve returns at once, and
+// the stages after it (guarded by done) run.
+__device__ __forceinline__ void pcg_finish(PcgState* st, int reason) {
+    st->done = 1;
+    st->halt = 1;
+    st->exit_reason = reason;
+}
+
 struct PcgReset {
     static constexpr int kThreads = 64;
     struct Args {
@@ -967,6 +975,8 @@ struct PcgReset {
         if (threadIdx.x != 0) return;
         PcgState* st = a.st;
         st->done = 0;
+        st->halt = 0;
+        st->verify_k = -1;
         st->iters = 0;
         st->exit_reason = 0;
         st->r_in_alt = 0;
@@ -1030,7 +1040,7 @@ struct PcgBnormFinalize {
         if (threadIdx.x == 0) {
             PcgState* st = a.reset.st;
             st->bnorm2 = b;
-            if (!(b > 0.0)) { st->done = 1; st->exit_reason = 1; }  // b = 0: p = 0 is the solution
+            if (!(b > 0.0)) pcg_finish(st, 1);  // b = 0: p = 0 is the solution
         }
     }
 };
@@ -1041,11 +1051,11 @@ struct PcgBnormFinalize {
 // as apply_matrix_and_dot, zero outside the grid).  part_out receives the
 // partials of ||r||^2 that the residual test reads on those iterations (on
 // the others it reads the update's per-tile partials).
-__device__ __forceinline__ void true_residual(const LevelView& L, const float* __restrict__ rhs,
-                                     const double* __restrict__ pd, const PcgState* st,
-                                     double* __restrict__ part_out, int due, uint32_t wg, uint32_t nwg) {
-    __shared__ double dscratch[kThreads / 64];
-    if (st->done || !due) return;
+template <int NT>
+__device__ __forceinline__ void true_residual_cells(const LevelView& L, const float* __restrict__ rhs,
+                                                    const double* __restrict__ pd, double* __restrict__ part_out,
+                                                    uint32_t wg, uint32_t nwg) {
+    __shared__ double dscratch[NT / 64];
     const int64_t w = L.w, n = L.cells;
     auto pa = [&](int64_t raw) { return raw >= 0 && raw < n ? pd[raw] : 0.0; };
     double acc = 0.0;
@@ -1064,6 +1074,13 @@ __device__ __forceinline__ void true_residual(const LevelView& L, const float* _
     }
     acc = block_sum(acc, dscratch);
     if (threadIdx.x == 0) part_out[wg] = acc;
+}
+
+__device__ __forceinline__ void true_residual(const LevelView& L, const float* __restrict__ rhs,
+                                              const double* __restrict__ pd, const PcgState* st,
+                                              double* __restrict__ part_out, int due, uint32_t wg, uint32_t nwg) {
+    if (st->done || !due) return;
+    true_residual_cells<kThreads>(L, rhs, pd, part_out, wg, nwg);
 }
 
 // Relative mode, after the update of iteration k (the uniform argument).
@@ -1093,11 +1110,11 @@ __global__ void zero_pressure_kernel(const uint8_t* __restrict__ fluid, float* _
 // Relative mode: breakdown when r.z <= 0 (or not finite), the iteration cap.
 __device__ inline bool gamma_exit_rel(PcgState* st, double g, int k, bool lead) {
     if (!(g > 0.0)) {
-        if (lead) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
+        if (lead) { pcg_finish(st, k == 0 ? 1 : 4); st->iters = k; }
         return true;
     }
     if (k >= st->max_iters) {
-        if (lead) { st->done = 1; st->exit_reason = 5; st->iters = k; }
+        if (lead) { pcg_finish(st, 5); st->iters = k; }
         return true;
     }
     return false;
@@ -1107,11 +1124,11 @@ __device__ inline bool gamma_exit_rel(PcgState* st, double g, int k, bool lead) 
 // the iteration cap.  Returns true when the solve ends here.
 __device__ inline bool gamma_exit(PcgState* st, float g, int k, bool lead) {
     if (g < kSafety) {
-        if (lead) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
+        if (lead) { pcg_finish(st, k == 0 ? 1 : 4); st->iters = k; }
         return true;
     }
     if (k >= kMaxPcgIterations) {
-        if (lead) { st->done = 1; st->exit_reason = 5; st->iters = k; }
+        if (lead) { pcg_finish(st, 5); st->iters = k; }
         return true;
     }
     return false;
@@ -1207,11 +1224,25 @@ __device__ __forceinline__ void spmv_dot(const LevelView& L, const float* __rest
                                 float* __restrict__ As, PcgState* st, const float* __restrict__ part_gamma,
                                 float* __restrict__ part_delta, int k, const double* __restrict__ part_gamma_d,
                                 const double* __restrict__ part_asz_d, double* __restrict__ part_delta_d, int ngam,
-                                uint32_t wg, uint32_t nwg) {
+                                const float* __restrict__ rhs, const double* __restrict__ pd,
+                                double* __restrict__ part_true, uint32_t wg, uint32_t nwg) {
     const bool lead = wg == 0 && threadIdx.x == 0;
     __shared__ float scratch[NT / 64];
     __shared__ double dscratch[NT / 64];
     if (st->done) return;
+    if (part_gamma_d && st->verify_k == k) {
+        // the relative mode's verifying iteration: the residual test of
+        // iteration k - 1 passed on the recurrence residual, so the true
+        // residual b - A p of the FP64 pressure replaces r and its partials
+        // decide, in the update that follows, whether the solve ends; gamma
+        // keeps its slot filled (the restart's beta is zero whatever it is)
+        __shared__ double dscratch2[8];
+        double g, asz;
+        sum_partials256x2(part_gamma_d, part_asz_d, ngam, dscratch2, g, asz);
+        if (lead) st->gamma_d[k & 1] = g;
+        true_residual_cells<NT>(L, rhs, pd, part_true, wg, nwg);
+        return;
+    }
     const bool first = k == 0;
     const size_t w = L.w;
     const uint32_t cells = L.cells, stride = nwg * NT;
@@ -1327,6 +1358,9 @@ struct SpmvDotArgs {
     const double* part_asz_d;
     double* part_delta_d;
     int ngam;
+    const float* rhs;        // relative mode: b, the FP64 pressure and the true residual's partials
+    const double* pd;
+    double* part_true;
     uint32_t nwg;
 };
 
@@ -1336,7 +1370,7 @@ struct SpmvDot {
     using Args = SpmvDotArgs;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         spmv_dot<NT>(a.L, a.search[k & 1], a.search[(k + 1) & 1], a.As, a.st, a.part_gamma, a.part_delta, k,
-                     a.part_gamma_d, a.part_asz_d, a.part_delta_d, a.ngam, wg, a.nwg);
+                     a.part_gamma_d, a.part_asz_d, a.part_delta_d, a.ngam, a.rhs, a.pd, a.part_true, wg, a.nwg);
     }
 };
 
@@ -1365,7 +1399,7 @@ __device__ __forceinline__ void axpy_max(const LevelView& L, const float* __rest
     fetch_flags();  // with the partials
     const float d = sum_partials256(part_delta, (int)nwg, scratch);
     if (fabsf(d) < kSafety) {
-        if (lead) { st->done = 1; st->exit_reason = 2; st->delta = d; }
+        if (lead) { pcg_finish(st, 2); st->delta = d; }
         return;
     }
     const float alpha = st->gamma[k] / d;
@@ -1576,12 +1610,46 @@ __device__ inline float vsor(const VTile<T>& t, int i, float own, float er, floa
 // relative mode ends the solve when delta <= 0); false when the solve ends.
 __device__ __forceinline__ bool pcg_alpha(PcgState* st, double d, double gam, bool lead, double& alpha_d) {
     if (!(d > 0.0)) {
-        if (lead) { st->done = 1; st->exit_reason = 2; st->delta = (float)d; }
+        if (lead) { pcg_finish(st, 2); st->delta = (float)d; }
         return false;
     }
     alpha_d = gam / d;
     if (lead) { st->delta = (float)d; st->alpha = (float)alpha_d; st->alpha_d = alpha_d; }
     return true;
+}
+
+// The verifying iteration k of the relative mode (its SpmvDot replaced r by
+// the true residual, whose squared norm is rr): the solve ends when it meets
+// the tolerance (false); otherwise iteration k restarts the search with a
+// zero step (alpha = 0 leaves p and r as they are, and the next beta is 0),
+// and the V-cycles resume (true).
+__device__ __forceinline__ bool pcg_verify(PcgState* st, double rr, int k, bool lead) {
+    if (lead) st->rnorm2 = rr;
+    if (rr <= st->tol2 * st->bnorm2) {
+        if (lead) pcg_finish(st, 3);
+        return false;
+    }
+    if (lead) {
+        st->halt = 0;
+        st->iters = k + 1;
+        st->alpha = 0.0f;
+        st->alpha_d = 0.0;
+    }
+    return true;
+}
+
+// A passing residual test of iteration k on the recurrence residual: the
+// true residual is checked in iteration k + 1 (the V-cycle of iteration k
+// halts), unless the cap leaves no iteration for it.
+__device__ __forceinline__ void pcg_candidate(PcgState* st, int k, bool lead, bool r_in_alt) {
+    if (!lead) return;
+    if (k + 1 < st->max_iters) {
+        st->halt = 1;
+        st->verify_k = k + 1;
+    } else {
+        pcg_finish(st, 3);
+        if (r_in_alt) st->r_in_alt = 1;
+    }
 }
 
 // Iteration k's update in the relative mode (src/apic.rs:1907-1928 with FP64
@@ -1600,6 +1668,7 @@ struct AxIn {
     uint32_t ndelta;
     float* r_out;               // the updated residual (the old one is read from the level's r)
     double* part_rr_out;        // one partial of ||r||^2 per tile
+    const double* part_true;    // the verifying iteration's true-residual partials (ndelta of them)
 };
 
 template <int NT>
@@ -1625,7 +1694,12 @@ __device__ __forceinline__ void axpy_tile(const LevelView& L, const AxIn& ax, Pc
     const double d = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
     if (dn) return;
     double alpha_d;
-    if (!pcg_alpha(st, d, gam, lead, alpha_d)) return;
+    if (st->verify_k == k) {
+        if (!pcg_verify(st, sum_partials256(ax.part_true, (int)ax.ndelta, dscratch), k, lead)) return;
+        alpha_d = 0.0;
+    } else if (!pcg_alpha(st, d, gam, lead, alpha_d)) {
+        return;
+    }
     const float alpha = (float)alpha_d;
     double pv[kPer];
     float sv[kPer], av[kPer], rv[kPer];
@@ -1750,13 +1824,27 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     }
     const double rr_t = (check && part_rr_t) ? sum_partials256(part_rr_t, (int)ntiles, dscratch) : 0.0;
     double dsum = 0.0;
-    if constexpr (AX) dsum = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
+    // the relative mode's verifying iteration (see pcg_verify): the update
+    // kernel decides (it ignores the halt flag, which it may clear), and the
+    // residual test of that iteration's update is not applied
+    const bool verifying = (AX || rel_test) && st->verify_k == k;
+    double rr_v = 0.0;
+    if constexpr (AX) {
+        dsum = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
+        if (verifying) rr_v = sum_partials256(ax.part_true, (int)ax.ndelta, dscratch);
+    }
     if (spec) vtile_put<T, NT>(g, t);  // read after the barrier of the tile test below
-    if (dn) return;
-    if (check && part_rr_t) {  // the update's partials, one per base tile
+    if (AX && verifying) {
+        if (!pcg_verify(st, rr_v, k, lead)) return;
+    } else if (dn) {
+        return;
+    }
+    if (verifying) {
+        // no residual test in the verifying iteration
+    } else if (check && part_rr_t) {  // the update's partials, one per base tile
         if (lead) st->rnorm2 = rr_t;
         if (rr_t <= tol2 * bn2) {
-            if (lead) { st->done = 1; st->exit_reason = 3; }
+            pcg_candidate(st, k, lead, false);
             return;
         }
     } else if (check && part_rr) {  // the true residual's
@@ -1767,7 +1855,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         rr = block_sum(rr, dscratch);
         if (lead) st->rnorm2 = rr;
         if (rr <= tol2 * bn2) {
-            if (lead) { st->done = 1; st->exit_reason = 3; }
+            if (lead) pcg_finish(st, 3);
             return;
         }
     } else if (check) {
@@ -1778,7 +1866,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         m = block_max(m, scratch);
         if (lead) st->maxerr = m;
         if (m < solve_error) {
-            if (lead) { st->done = 1; st->exit_reason = 3; }
+            if (lead) pcg_finish(st, 3);
             return;
         }
     }
@@ -1787,7 +1875,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     // times zero coefficients, so it leaves them as they are.
     double alpha_d = 0.0;
     if constexpr (AX) {
-        if (!pcg_alpha(st, dsum, gam, lead, alpha_d)) return;
+        if (!verifying && !pcg_alpha(st, dsum, gam, lead, alpha_d)) return;  // (a verifying step is 0)
     }
     bool any = false;
 #pragma unroll
@@ -2091,7 +2179,7 @@ __device__ __forceinline__ float rrestrict(const Region<S>& t, int i0) {
 
 __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, const LevelView& D, int fuse_red_d,
                                        const int32_t* __restrict__ done, uint32_t tiles_x, PcgState* st,
-                                       const double* __restrict__ part_rr_t, uint32_t ntiles, uint32_t wg) {
+                                       const double* __restrict__ part_rr_t, uint32_t ntiles, int k, uint32_t wg) {
     constexpr int NT = kP2Threads;
     __shared__ Region<kP2F> tf;
     __shared__ Region<kP2C> tc;
@@ -2122,13 +2210,13 @@ __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, c
     region_put<kP2F, NT>(gf, tf);
     region_put<kP2C, NT>(gc, tc);
     if (dn) return;
-    if (part_rr_t) {
+    if (part_rr_t && st->verify_k != k) {  // (no residual test in the verifying iteration)
         const bool lead = wg == 0 && threadIdx.x == 0;
         if (lead) st->rnorm2 = rr;
         if (rr <= tol2 * bn2) {
-            // the new residual is in the base level's second buffer: the
-            // base up-sweep copies it home
-            if (lead) { st->done = 1; st->exit_reason = 3; st->r_in_alt = 1; }
+            // the new residual is in the base level's second buffer: at the
+            // end of the solve the base up-sweep copies it home
+            pcg_candidate(st, k, lead, true);
             return;
         }
     }
@@ -2212,8 +2300,8 @@ struct VDown2Args {
 struct VDown2 {
     static constexpr int kThreads = kP2Threads;
     using Args = VDown2Args;
-    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        vdown2(a.F, a.C, a.D, a.fuse_red_d, a.done, a.tiles_x, a.st, a.part_rr_t, a.ntiles, wg);
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
+        vdown2(a.F, a.C, a.D, a.fuse_red_d, a.done, a.tiles_x, a.st, a.part_rr_t, a.ntiles, k, wg);
     }
 };
 
@@ -2641,7 +2729,7 @@ bool wide_base(const Sim& sim) { return sim.nred >= kWideMinRed; }
 // The update's operands (AxpyTile, VDownAx), r_out receiving the residual.
 AxIn update_operands(const Sim& sim, float* r_out) {
     return AxIn{{sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pressure_d, sim.part_delta_d, sim.nred,
-                r_out, sim.part_rr_t};
+                r_out, sim.part_rr_t, sim.part_rr2_d};
 }
 
 }  // namespace
@@ -2668,7 +2756,7 @@ void launch_zero_pressure(Sim& sim) {
 // mode 0: apply_preconditioner alone; 1: PCG start (gamma); 2: PCG iteration
 // (residual test in the first kernel, gamma' partials in the last).
 void launch_vcycle(Sim& sim, int mode, int k) {
-    const int32_t* done = &sim.pcg->done;
+    const int32_t* done = &sim.pcg->halt;
     const int last = (int)sim.levels.size() - 1;
     const int bottom = (int)sim.bottom_level;  // 1 <= bottom <= last
     const bool rel = sim.solver.mode == 1 && mode != 0;
@@ -2759,7 +2847,7 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     if (timed) sim.mark(APIC_T_K_SPMV, 0);
     SpmvDotArgs a{B, {sim.search[0], sim.search[1]}, sim.As, sim.pcg, sim.part_gamma, sim.part_delta,
                   rel ? sim.part_gamma_d : nullptr, sim.part_asz_d, rel ? sim.part_delta_d : nullptr,
-                  (int)gamma_partials(sim), sim.nred};
+                  (int)gamma_partials(sim), sim.rhs, sim.pressure_d, sim.part_rr2_d, sim.nred};
     const bool wide = rel && sim.nred >= kWideMinRed;
     if (wide) emit<SpmvDot<kWideThreads>>(sim, a, sim.nred, k);
     else emit<SpmvDot<kSolverThreads>>(sim, a, sim.nred, k);

@@ -70,6 +70,34 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c, levels):
     }
 
 
+def stage_bytes(n_p, n_c, n_f, levels, iters, width, height):
+    """Algorithmic bytes of each stage of one substep (compulsory traffic of
+    the stage's inputs and outputs, independent of the implementation): the
+    sort 16 N_p for the keys, 16 N_p per radix pass of 8 bits over
+    2 ceil(log2 max(W, H)) key bits, 76 N_p for the 9-field gather; P2G
+    36 N_p + 14 N_c; the pressure system (three smoothing passes, assembly)
+    92 N_f; the multigrid hierarchy 24 N_f; the PCG `iters` iterations of
+    algorithmic_bytes' PCG iteration; the gradient 34 N_c; the four
+    extrapolation passes 80 N_c; G2P 48 N_p + 8 N_c."""
+    key_bits = 2 * max(1, (max(width, height) - 1).bit_length())
+    passes = -(-key_bits // 8)
+    it = 68 * n_f + 8 * n_c + sum(89 * f + 4 * c for f, c in levels)
+    return {"sort": (16 + 16 * passes + 76) * n_p, "p2g": 36 * n_p + 14 * n_c, "pressure_build": 92 * n_f,
+            "mg_build": 24 * n_f, "pcg": iters * it, "gradient": 34 * n_c, "extrapolate": 80 * n_c,
+            "g2p": 48 * n_p + 8 * n_c}
+
+
+def stage_rates(stage_ms, n_p, n_c, n_f, levels, iters, width, height):
+    """{stage: {"ms": per substep, "gbs": algorithmic GB/s, "frac": of the HBM peak}}."""
+    sb = stage_bytes(n_p, n_c, n_f, levels, iters, width, height)
+    out = {}
+    for k, ms in stage_ms.items():
+        if k in sb and ms > 0:
+            gbs = sb[k] / (ms * 1e-3) / 1e9
+            out[k] = {"ms": ms, "bytes": sb[k], "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 KERNEL_NAMES = {"kernel_g2p": "one_kernel<G2p>", "kernel_p2g": "one_kernel<P2gOwner>",
                 "kernel_spmv": "one_kernel<SpmvDot<1024> >",
                 "kernel_pcg_iteration": "PCG iteration (SpmvDot<1024>, VDown<32, 512, true>, 2 x VDown2, "
@@ -79,6 +107,7 @@ PCG_ITERATION_KERNELS = {"one_kernel<SpmvDot<1024> >": 1, "one_kernel<VDown<32, 
                          "one_kernel<VDown2>": 2, "one_kernel<BottomLds>": 1, "one_kernel<VUp2>": 2,
                          "one_kernel<VUp<32, 512> >": 1}
 BATCH_WORKLOAD = "cfg4_batched_256"
+LARGE_WORKLOAD = "cfg3_falling_block_4096x2048"
 
 
 SOLVERS = {
@@ -344,6 +373,91 @@ def served_run(pool=128, n_requests=400, rates=(50.0, 200.0, 800.0, 3200.0), see
             "steps_per_request": [1, 20], "solver": SOLVER_NAMES["relative"], "rates": out}
 
 
+def large_run(steps=50, warmup=3, stage_steps=5, oracle_steps=2):
+    """The mixed-precision large case (a 4096 x 2048 falling block, 35.6M
+    particles): FP32 transfers and SpMV (FP64 row sums), FP64 PCG reductions,
+    residual recompute every 25 iterations, the relative 1e-6 test in FP64.
+    `steps` timed updates from the seeded scene after `warmup`, per-stage
+    algorithmic GB/s from a pass with every stage timed, and the error
+    against the FP64 oracle (pressure, grid velocity, particle positions)
+    after the first `oracle_steps` updates from the seeded scene, with the
+    FP32 oracle's own distance from the FP64 oracle beside it."""
+    import numpy as np
+    from FLIP import scenes
+    from FLIP.apic import Apic, DebugOptions, WorldProperties
+    name = LARGE_WORKLOAD
+    c = scenes.CONFIGS[name]
+    W, H, dx, dt = c["width"], c["height"], c["dx"], c["dt"]
+    t0 = time.perf_counter()
+    xy = scenes.config_scene(name)
+    n_p = xy.shape[0]
+    print(f"[bench] large workload: {n_p} particles ({time.perf_counter() - t0:.1f}s to seed)",
+          file=sys.stderr, flush=True)
+    props = WorldProperties(-9.81, 0.1, 5.0, 1e-5)
+    r = timed_run(Apic, DebugOptions, props, c, xy, "relative", steps, warmup, stage_steps)
+    print(f"[bench] large workload: {steps} steps in {r['elapsed']:.3f}s", file=sys.stderr, flush=True)
+    n_f = float(np.mean(r["fluid"]))
+    stage_ms = {k: v[0] / max(r["stage_substeps"], 1) for k, v in r["stage_timers"].items()
+                if not k.startswith("kernel_")}
+    iters = float(np.mean(r["iters"]))
+    rates = stage_rates(stage_ms, n_p, W * H, n_f, r["levels"], iters, W, H)
+    tr64 = (r["true_residual"] or {}).get("fp64_pressure")
+    out = {"workload": name, "grid": [W, H], "dx": dx, "dt": dt, "particles": n_p, "ppc": c["ppc"],
+           "seed": c["seed"], "solver": SOLVER_NAMES["relative"], "steps": steps, "warmup": warmup,
+           "value": n_p * r["substeps"] / r["elapsed"], "unit": "particle-steps/s",
+           "ms_per_step": r["elapsed"] / steps * 1e3, "pcg_iterations_mean": iters,
+           "pcg_rel_residual_max": max(x for x in r["rel"] if x >= 0),
+           "pcg_exit_reasons": {str(k): int(sum(1 for x in r["reasons"] if x == k)) for k in sorted(set(r["reasons"]))},
+           "pcg_true_rel_residual_last": r["true_residual"],
+           "valid": all(x == 3 for x in r["reasons"]) and tr64 is not None and tr64 <= 1.1e-6,
+           "stages": rates}
+    if oracle_steps > 0:
+        out["fp64_oracle_error"] = fp64_oracle_error(c, xy, props, oracle_steps)
+    return out
+
+
+def fp64_oracle_error(c, xy, props, nsteps):
+    """Relative L2 distance from the FP64 oracle after each of the first
+    `nsteps` updates of the seeded scene (the GPU, and the FP32 oracle, which
+    is the reference's own precision), under the relative solver."""
+    import numpy as np
+    from FLIP.apic import Apic, DebugOptions
+    from oracle import pyoracle as po
+    W, H, dx, dt = c["width"], c["height"], c["dx"], c["dt"]
+    g = Apic.with_particles(W, H, dx, props, xy)
+    g.set_solver_config(**SOLVERS["relative"])
+    orc = {}
+    for prec in (64, 32):
+        o = po.Oracle(W, H, dx, po.bench_props(), xy=xy, precision=prec)
+        o.set_solver(1, 1e-6, 500, True, 25)
+        orc[prec] = o
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        d = np.linalg.norm(b)
+        return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
+
+    per_step = []
+    for k in range(nsteps):
+        t0 = time.perf_counter()
+        g.update(DebugOptions(False), dt)
+        for o in orc.values():
+            o.update(dt)
+        e = {"step": k + 1}
+        for who, src in (("gpu", g), ("fp32_oracle", orc[32])):
+            e[who] = {"pressure": rel(src.read("PRESSURE"), orc[64].read("PRESSURE")),
+                      "grid_velocity_u": rel(src.read("MAC_U"), orc[64].read("MAC_U")),
+                      "grid_velocity_v": rel(src.read("MAC_V"), orc[64].read("MAC_V")),
+                      "positions": max(rel(src.read(f), orc[64].read(f)) for f in ("POS_X", "POS_Y"))}
+        e["pcg_iterations"] = {"gpu": g.stats()["pcg_iterations"], "fp64_oracle": orc[64].stats()["pcg_iterations"]}
+        per_step.append(e)
+        print(f"[bench] large workload, FP64 oracle step {k + 1}/{nsteps}: GPU error {e['gpu']} "
+              f"({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    g.close()
+    return {"steps_compared": nsteps, "of_steps": c["steps"], "per_step": per_step}
+
+
 def copy_calibration(gib=1.0, repeats=3):
     """A plain device-to-device copy of known size (the runtime's copy kernel,
     through torch), for scaling the FETCH_SIZE and WRITE_SIZE counters: run
@@ -398,6 +512,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
     ap.add_argument("--serve-only", action="store_true", help="print only the served workload's result")
     ap.add_argument("--serve-pool", type=int, default=128, help="simulations in the served pool")
+    ap.add_argument("--no-large", action="store_true", help="skip the 4096 x 2048 mixed-precision workload")
+    ap.add_argument("--large-only", action="store_true", help="print only the large workload's result")
+    ap.add_argument("--large-oracle-steps", type=int, default=2,
+                    help="updates of the large workload compared with the FP64 oracle")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 1 GiB device copy (for scaling the HBM traffic counters)")
     args = ap.parse_args()
@@ -415,6 +533,9 @@ def main():
         return
     if args.batch_only:
         print(json.dumps(batched_run(count=args.batch_count)))
+        return
+    if args.large_only:
+        print(json.dumps(large_run(oracle_steps=args.large_oracle_steps)))
         return
     if args.serve_only:
         print(json.dumps(served_run(pool=args.serve_pool)))
@@ -451,6 +572,8 @@ def main():
     d = kernels[dom]
     stages = {k: round(v[0] / max(r["stage_substeps"], 1), 4) for k, v in r["stage_timers"].items()
               if not k.startswith("kernel_")}
+    stage_gbs = stage_rates(stages, n_p, cfg["width"] * cfg["height"], n_f, r["levels"],
+                            float(np.mean(r["iters"])), cfg["width"], cfg["height"])
     rels = [x for x in r["rel"] if x >= 0]
     tol = SOLVERS[args.solver].get("rel_tol")
     # every timed solve ended by the tolerance, and the last one's FP64
@@ -485,6 +608,7 @@ def main():
                      "levels_cells": [c for _, c in r["levels"]]},
         "kernels": kernels,
         "stage_ms_per_step": stages,
+        "stage_algorithmic_gbs": stage_gbs,
         "pcg_iterations_mean": float(np.mean(r["iters"])),
         "pcg_rel_residual_max": max(rels) if rels else None,
         "pcg_exit_reasons": {str(k): int(sum(1 for x in r["reasons"] if x == k)) for k in sorted(set(r["reasons"]))},
@@ -496,6 +620,8 @@ def main():
                          "ms_per_step": ro["elapsed"] / o_steps * 1e3,
                          "pcg_iterations_mean": float(np.mean(ro["iters"]))},
     }
+    if not args.no_large:
+        line["large"] = large_run(oracle_steps=0 if args.no_cpu else args.large_oracle_steps)
     if not args.no_batch:
         line["batched"] = batched_run(count=args.batch_count)
     if not args.no_serve:

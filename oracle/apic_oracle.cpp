@@ -734,9 +734,13 @@ template <class R> struct Sim {
     // goes); dot products, norms and step scalars in A (FP64 when fp64); stop
     // when the residual meets ||r||_2 <= rel_tol * ||b||_2, or after
     // max_iterations; every recompute_every iterations r is replaced by the
-    // true residual b - A p evaluated in FP64; breakdown when s.As <= 0 or
-    // r.z <= 0.  Sums run in ascending cell order.  The FP32 pressure itself
-    // cannot have a true residual much below eps32 || |A| |p| || / ||b||.
+    // true residual b - A p evaluated in FP64; a test passed by the
+    // recurrence residual is confirmed on the true residual in the next
+    // iteration, which restarts the search (zero step, beta = 0) when it
+    // fails; breakdown when s.As <= 0 or r.z <= 0.  Sums run in ascending
+    // cell order.  The FP32 pressure itself cannot have a true residual much
+    // below eps32 || |A| |p| || / ||b||.
+    std::vector<double> pressure_d;  // the relative mode's FP64 pressure of the last solve
     int solver_mode = 0;
     double solver_rel_tol = 1e-6;
     int solver_max_it = 500, solver_fp64 = 1, solver_recompute = 25;
@@ -748,7 +752,8 @@ template <class R> struct Sim {
         stats.pcg_max_residual = 0;
         stats.pcg_rel_residual = 0;
         for (uint32_t k = 0; k < fluid_cells; ++k) pressure[base.fluid_idx[k] - W] = 0;
-        std::vector<double> pd(pressure.begin(), pressure.end());
+        std::vector<double>& pd = pressure_d;
+        pd.assign(pressure.begin(), pressure.end());
         std::vector<R> b(base.r);
         auto fluid_sum = [&](auto term) {
             A acc = 0;
@@ -765,8 +770,48 @@ template <class R> struct Sim {
         if (!(gamma > 0)) { stats.pcg_exit_reason = 1; return; }
         const A tol2 = A(solver_rel_tol) * A(solver_rel_tol);
         auto pd_at = [&](int64_t raw) { return raw >= 0 && raw < (int64_t)cells ? pd[(size_t)raw] : 0.0; };
+        // r = b - A p of the FP64 pressure (FP64, the fused chain of
+        // apply_matrix_and_dot); returns ||r||^2
+        auto true_residual = [&]() {
+            double acc = 0;
+            for (size_t raw = 0; raw < cells; ++raw) {
+                if (!base.fluid[raw]) continue;
+                size_t pi = raw + w;
+                const int64_t c = (int64_t)raw, ww = (int64_t)w;
+                double v = double(base.diag[pi]) * pd[raw];
+                v = std::fma(double(base.px[pi]), pd_at(c + 1), v);
+                v = std::fma(double(base.px[pi - 1]), pd_at(c - 1), v);
+                v = std::fma(double(base.py[pi]), pd_at(c + ww), v);
+                v = std::fma(double(base.py[pi - w]), pd_at(c - ww), v);
+                const double t = double(b[pi]) - v;
+                base.r[pi] = R(t);
+                acc += t * t;
+            }
+            return acc;
+        };
+        // A residual test that passes on the recurrence residual is checked
+        // on the true residual in the next iteration (verify_at): the solve
+        // ends if that meets the tolerance; otherwise that iteration restarts
+        // the search from the true residual with a zero step
+        int verify_at = -1;
         for (int it = 0; it < solver_max_it; ++it) {
             stats.pcg_iterations = it + 1;
+            if (it == verify_at) {
+                const A rr = A(true_residual());
+                stats.pcg_rel_residual = (double)std::sqrt((double)rr / (double)bnorm2);
+                if (rr <= tol2 * bnorm2) { stats.pcg_iterations = it; stats.pcg_exit_reason = 3; return; }
+                apply_preconditioner();
+                A ng = 0;
+                for (size_t i = start; i < end; ++i) ng += A(base.r[i]) * A(base.e[i]);
+                if (!(ng > 0)) { stats.pcg_exit_reason = 4; return; }
+                gamma = ng;
+                for (size_t raw = 0; raw < cells; ++raw) {  // beta = 0
+                    if (!base.fluid[raw]) continue;
+                    size_t i = raw + w;
+                    search[i] = base.e[i];
+                }
+                continue;
+            }
             // A s with FP64 row sums rounded once to R (for R = float: the
             // products of two floats are exact in double), so that the
             // recurrence residual tracks the true residual
@@ -792,25 +837,13 @@ template <class R> struct Sim {
             }
             A rr = fluid_sum([&](size_t i) { return A(base.r[i]) * A(base.r[i]); });
             const bool due = solver_recompute > 0 && (it + 1) % solver_recompute == 0;
-            if (due) {  // true residual, FP64
-                double acc = 0;
-                for (size_t raw = 0; raw < cells; ++raw) {
-                    if (!base.fluid[raw]) continue;
-                    size_t pi = raw + w;
-                    const int64_t c = (int64_t)raw, ww = (int64_t)w;
-                    double v = double(base.diag[pi]) * pd[raw];
-                    v = std::fma(double(base.px[pi]), pd_at(c + 1), v);
-                    v = std::fma(double(base.px[pi - 1]), pd_at(c - 1), v);
-                    v = std::fma(double(base.py[pi]), pd_at(c + ww), v);
-                    v = std::fma(double(base.py[pi - w]), pd_at(c - ww), v);
-                    const double t = double(b[pi]) - v;
-                    base.r[pi] = R(t);
-                    acc += t * t;
-                }
-                rr = A(acc);
-            }
+            if (due) rr = A(true_residual());  // true residual, FP64
             stats.pcg_rel_residual = (double)std::sqrt((double)rr / (double)bnorm2);
-            if (rr <= tol2 * bnorm2) { stats.pcg_exit_reason = 3; return; }
+            if (rr <= tol2 * bnorm2) {
+                if (due || it + 1 >= solver_max_it) { stats.pcg_exit_reason = 3; return; }
+                verify_at = it + 1;
+                continue;
+            }
             apply_preconditioner();
             // flexible (Polak-Ribiere) beta = r_{k+1}.(z_{k+1} - z_k) / r_k.z_k, with
             // r_k.z_{k+1} = r_{k+1}.z_{k+1} + alpha (A s).z_{k+1}: robust when the
@@ -1124,6 +1157,7 @@ template <class R> struct Sim {
     size_t field_length(int f, uint32_t l) {
         if (f == APIC_SORT_KEY || f == APIC_SORT_VAL) return num_particles;
         if (f == APIC_DENSITY || f == APIC_SMOOTHED_DENSITY || f == APIC_PRESSURE) return cells;
+        if (f == APIC_PRESSURE_F64) return pressure_d.size();
         if (auto* v = real_field(f, l)) return v->size();
         if (auto* v = flag_field(f, l)) return v->size();
         return (size_t)-1;
@@ -1168,6 +1202,10 @@ template <class R> int read_field_t(Sim<R>* s, int f, uint32_t l, void* dst) {
     if (f == APIC_SORT_KEY || f == APIC_SORT_VAL) {
         auto& v = f == APIC_SORT_KEY ? s->skey : s->sval;
         std::memcpy(dst, v.data(), v.size() * 4);
+        return 0;
+    }
+    if (f == APIC_PRESSURE_F64) {
+        std::memcpy(dst, s->pressure_d.data(), s->pressure_d.size() * sizeof(double));
         return 0;
     }
     if (auto* v = s->real_field(f, l)) {
