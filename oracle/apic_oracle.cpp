@@ -1260,6 +1260,8 @@ template <class R> int stage_t(Sim<R>* s, int stage, float dt, uint32_t ts) {
         case 13: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->base.restrict_to(s->levels[0]); break;
         case 14: s->base.smooth(false, 1.2f); break;
         case 15: s->base.smooth(true, 1.2f); break;
+        // the coarsest level's CG alone (solve_coarse_exact), for the dense-solve pins
+        case 16: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels.back().solve_coarse_exact(); break;
         default: return APIC_ERR_INVALID_ARGUMENT;
     }
     return 0;
