@@ -19,7 +19,8 @@ solve's exit reason and residual are recorded, the FP64 true residual
 library's own system and pressure, and the line says whether every solve
 met the tolerance ("valid").  A short second pass times every stage.  The
 CPU baseline is the oracle (FP32) on the first steps of the same workload,
-timed on this host.
+timed on this host on all the CPUs the process may use (the threaded mode:
+the reference's parallel decomposition) and on one.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--cpu-seconds S]
                        [--solver relative|reference]
@@ -120,11 +121,44 @@ SOLVER_NAMES = {"relative": "relative 1e-6 (FP64 reductions, residual recompute 
                 "reference": "reference rules (absolute 1e-5, 100-iteration cap)"}
 
 
-def cpu_baseline(xy, cfg, budget_s, solver):
-    """The oracle (FP32 restatement of the reference step, one thread) on the
-    first steps of the same workload; steps chosen to fill about budget_s."""
+def host_cpu():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
+
+
+def baseline_threads():
+    """Threads for the all-cores CPU baseline: the CPUs this process may use,
+    capped by OMP_NUM_THREADS when the environment sets it."""
+    _, _, usable = host_cpu()
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return min(usable, int(env))
+    return usable
+
+
+def cpu_baseline(xy, cfg, budget_s, solver, threads):
+    """The oracle, the FP32 restatement of the reference step, on the first
+    steps of the same workload, timed on this host.  threads > 1 runs its
+    threaded mode: the reference's rayon loops as OpenMP loops over the same
+    units (8-particle chunks with the reference's minimum lengths, CAS
+    scatters in P2G, chunked reductions); one thread is the serial
+    restatement.  Steps are chosen to fill about budget_s."""
     from oracle import pyoracle as po
     o = po.Oracle(cfg["width"], cfg["height"], cfg["dx"], po.bench_props(), xy=xy)
+    o.set_threads(threads)
     sv = SOLVERS[solver]
     if sv["mode"] == 1:
         o.set_solver(1, sv["rel_tol"], sv["max_iterations"], sv["fp64_reductions"], sv["recompute_every"])
@@ -132,21 +166,28 @@ def cpu_baseline(xy, cfg, budget_s, solver):
     t0 = time.perf_counter()
     o.update(cfg["dt"])
     first = time.perf_counter() - t0
-    steps = max(2, min(int(budget_s / max(first, 1e-3)), 200))
+    steps = max(2, min(int(budget_s / max(first, 1e-3)), cfg.get("steps", 200)))
     total, substeps = first, o.stats()["substeps"]
     for _ in range(steps - 1):
         t0 = time.perf_counter()
         o.update(cfg["dt"])
         total += time.perf_counter() - t0
         substeps += o.stats()["substeps"]
+    model, logical, usable = host_cpu()
     return {
         "value": n * substeps / total,
         "unit": "particle-steps/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": f"{WORKLOAD}: first {steps} updates of dt={cfg['dt']} from the seeded scene, solver "
-                  f"{SOLVER_NAMES[solver]}, oracle FP32 restatement, one thread, "
-                  f"{platform.processor() or platform.machine()}",
+        "ms_per_step": total / steps * 1e3,
+        "cpu_model": model,
+        "logical_cpus": logical,
+        "usable_cpus": usable,
+        "sample": f"{WORKLOAD}: first {steps} of its {cfg.get('steps', 200)} updates of dt={cfg['dt']} from the seeded "
+                  f"scene, solver {SOLVER_NAMES[solver]}; the oracle (FP32 restatement of the reference step) "
+                  + ("serial, one thread" if threads == 1 else
+                     f"threaded on {threads} OpenMP threads with the reference's parallel decomposition")
+                  + f", on {model}",
     }
 
 
@@ -430,6 +471,7 @@ def fp64_oracle_error(c, xy, props, nsteps):
     for prec in (64, 32):
         o = po.Oracle(W, H, dx, po.bench_props(), xy=xy, precision=prec)
         o.set_solver(1, 1e-6, 500, True, 25)
+        o.set_threads(baseline_threads())  # (threaded: the reference's own summation freedom)
         orc[prec] = o
 
     def rel(a, b):
@@ -627,7 +669,14 @@ def main():
     if not args.no_serve:
         line["served"] = served_run(pool=args.serve_pool)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(xy, cfg, args.cpu_seconds, args.solver)
+        threads = baseline_threads()
+        line["cpu_baseline"] = cpu_baseline(xy, cfg, args.cpu_seconds, args.solver, threads)
+        print(f"[bench] CPU baseline on {threads} threads: {line['cpu_baseline']['value']:.3e} particle-steps/s",
+              file=sys.stderr, flush=True)
+        if threads > 1:
+            line["cpu_baseline_one_core"] = cpu_baseline(xy, cfg, args.cpu_seconds, args.solver, 1)
+            print(f"[bench] CPU baseline on 1 thread: {line['cpu_baseline_one_core']['value']:.3e} particle-steps/s",
+                  file=sys.stderr, flush=True)
     print(json.dumps(line))
 
 

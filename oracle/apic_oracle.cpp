@@ -31,9 +31,38 @@
 #include <random>
 #include <vector>
 
+#include <parallel/algorithm>
+
 namespace {
 
 constexpr float kSafety = 1e-5f;  // SAFTY, src/apic.rs:8
+
+// Threaded mode (the CPU baseline): the reference's rayon loops as OpenMP
+// loops over the same units (particle chunks, fluid-list slices, grid rows),
+// parallel only above the reference's with_min_len thresholds, P2G scattering
+// through CAS adds as the reference's atomic_add_f32/add_to_mac do.  With one
+// thread (the default, the checker) every loop runs serially in the order the
+// parity tests pin; with several the sums of the scatter and of the relative
+// mode's reductions run in a thread-dependent order, as the reference's do.
+#define ORC_PRAGMA(...) _Pragma(#__VA_ARGS__)
+#define ORC_PAR(cond) ORC_PRAGMA(omp parallel for num_threads(nthreads) if(cond) schedule(static))
+#define ORC_PAR_RED(cond, op, ...) \
+    ORC_PRAGMA(omp parallel for num_threads(nthreads) if(cond) schedule(static) reduction(op : __VA_ARGS__))
+
+// dst += v by compare-and-swap on the value's bits (atomic_add_f32, src/apic.rs:271-292)
+template <class R> inline void cas_add(R& dst, R v) {
+    using U = typename std::conditional<sizeof(R) == 4, uint32_t, uint64_t>::type;
+    U* p = reinterpret_cast<U*>(&dst);
+    U old = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (true) {
+        R cur;
+        std::memcpy(&cur, &old, sizeof(R));
+        const R nv = cur + v;
+        U nb;
+        std::memcpy(&nb, &nv, sizeof(R));
+        if (__atomic_compare_exchange_n(p, &old, nb, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) return;
+    }
+}
 
 template <class R> inline R fast_max(R a, R b) { return a > b ? a : b; }
 template <class R> inline R fast_min(R a, R b) { return a < b ? a : b; }
@@ -105,6 +134,7 @@ template <class R> struct Level {
     std::vector<uint8_t> fluid;  // type_fluid bit per raw cell
     std::vector<uint32_t> fluid_idx, red, black;  // padded indices
     std::vector<R> diag, inv_diag, px, py, r, e, cg_s, cg_ap, cg_r;
+    int nthreads = 1;
 
     void init(uint32_t W, uint32_t H) {  // MultiGridLevel::new, src/apic.rs:428-453
         w = W; h = H; cells = W * H;
@@ -119,8 +149,10 @@ template <class R> struct Level {
 
     // smooth_level, src/apic.rs:471-535 (one iteration, omega = 1.2)
     void smooth_list(const std::vector<uint32_t>& list, R omega) {
-        for (uint32_t idx : list) {
-            size_t c = idx;
+        const long n = (long)list.size();
+        ORC_PAR(nthreads > 1 && n > 4096)
+        for (long li = 0; li < n; ++li) {
+            size_t c = list[(size_t)li];
             R id = inv_diag[c];
             if (id != R(0)) {
                 R sum = r[c];
@@ -142,7 +174,10 @@ template <class R> struct Level {
     // restrict, src/apic.rs:537-594
     void restrict_to(Level& coarse) const {
         std::fill(coarse.e.begin() + coarse.w, coarse.e.begin() + coarse.w + coarse.cells, R(0));
-        for (uint32_t ci : coarse.fluid_idx) {
+        const long n = (long)coarse.fluid_idx.size();
+        ORC_PAR(nthreads > 1 && n > 4096)
+        for (long li = 0; li < n; ++li) {
+            const uint32_t ci = coarse.fluid_idx[(size_t)li];
             size_t raw_c = ci - coarse.w;
             size_t cx = raw_c % coarse.w, cy = raw_c / coarse.w;
             R sum = 0;
@@ -167,7 +202,11 @@ template <class R> struct Level {
     // prolongate, src/apic.rs:596-616 (iterates the fluid list including the
     // base level's padding duplicates)
     void prolongate_from(const Level& coarse) {
-        for (uint32_t fi : fluid_idx) {
+        const long n = (long)fluid_idx.size();
+        // (the padded list's duplicates of the last cell are adjacent: one thread takes them)
+        ORC_PAR(nthreads > 1 && n > 4096)
+        for (long li = 0; li < n; ++li) {
+            const uint32_t fi = fluid_idx[(size_t)li];
             size_t raw_f = fi - w;
             size_t cx = (raw_f % w) / 2, cy = (raw_f / w) / 2;
             size_t rc = cy * coarse.w + cx + coarse.w;
@@ -243,6 +282,12 @@ template <class R> struct Sim {
     // statistics
     apic_step_stats stats{};
     uint32_t p2g_shuffle_seed = 0;  // nonzero: scatter chunks in a seeded random order
+    int nthreads = 1;               // threaded mode (the CPU baseline) when > 1
+    void set_threads(int n) {
+        nthreads = n < 1 ? 1 : n;
+        base.nthreads = nthreads;
+        for (auto& L : levels) L.nthreads = nthreads;
+    }
 
     size_t nu() const { return (size_t)cells + H; }
     size_t nv() const { return (size_t)cells + W; }
@@ -344,6 +389,7 @@ template <class R> struct Sim {
     // update_spatial_lookup, src/apic.rs:1043-1111
     void compute_keys() {
         R max_x = (R)(float)(W > 0 ? W - 1 : 0), max_y = (R)(float)(H > 0 ? H - 1 : 0);
+        ORC_PAR(nthreads > 1 && num_particles > 4096)
         for (uint32_t i = 0; i < num_particles; ++i) {
             R gx = p[PX_][i] / cell, gy = p[PY_][i] / cell;
             uint32_t ix = (uint32_t)trunc_i(fast_min(fast_max(gx, R(0)), max_x));
@@ -355,14 +401,22 @@ template <class R> struct Sim {
     void sort_records() {
         std::vector<uint32_t> order(num_particles);
         std::iota(order.begin(), order.end(), 0u);
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return skey[a] < skey[b]; });
+        auto less = [&](uint32_t a, uint32_t b) { return skey[a] < skey[b]; };
+        if (nthreads > 1)  // a parallel multiway merge sort: the same stable order
+            __gnu_parallel::stable_sort(order.begin(), order.end(), less,
+                                        __gnu_parallel::multiway_mergesort_tag(nthreads));
+        else
+            std::stable_sort(order.begin(), order.end(), less);
         std::vector<uint32_t> k(num_particles), v(num_particles);
+        ORC_PAR(nthreads > 1 && num_particles > 4096)
         for (uint32_t i = 0; i < num_particles; ++i) { k[i] = skey[order[i]]; v[i] = sval[order[i]]; }
         skey.swap(k); sval.swap(v);
     }
     void permute() {
-        for (int f = 0; f < NPF_; ++f)
+        for (int f = 0; f < NPF_; ++f) {
+            ORC_PAR(nthreads > 1 && nalloc > 8192)
             for (size_t i = 0; i < nalloc; ++i) ps[f][i] = i < num_particles ? p[f][sval[i]] : R(0);
+        }
         for (int f = 0; f < NPF_; ++f) p[f].swap(ps[f]);
     }
     void update_spatial_lookup() { compute_keys(); sort_records(); permute(); }
@@ -384,7 +438,10 @@ template <class R> struct Sim {
             std::shuffle(chunk_order.begin(), chunk_order.end(), rng);
         }
         const R half = R(0.5), one = R(1);
-        for (uint32_t chunk : chunk_order) {
+        const bool par = nthreads > 1 && num_chunks > 256;  // par_chunks, with_min_len(256) (:1237-1240)
+        ORC_PAR(par)
+        for (long ci = 0; ci < (long)num_chunks; ++ci) {
+            const uint32_t chunk = chunk_order[(size_t)ci];
             uint32_t idx_u[4][8], idx_v[4][8], idx_c[4][8], idx_f[8];
             R add_u[4][8], wt_u[4][8], add_v[4][8], wt_v[4][8], wt_c[4][8];
             for (int lane = 0; lane < 8; ++lane) {
@@ -454,16 +511,23 @@ template <class R> struct Sim {
                 idx_f[lane] = (uint32_t)(trunc_i(cly) * (int32_t)W + trunc_i(clx));
             }
             int active = (chunk == num_chunks - 1 && num_particles % 8 != 0) ? (int)(num_particles % 8) : 8;
-            for (int l = 0; l < active; ++l) base.fluid[idx_f[l]] = 1;  // add_to_fluid :371-395
-            for (int k = 0; k < 4; ++k) add_to_mac(u_mom, u_w, idx_u[k], add_u[k], wt_u[k], active);
-            for (int k = 0; k < 4; ++k) add_to_mac(v_mom, v_w, idx_v[k], add_v[k], wt_v[k], active);
-            for (int k = 0; k < 4; ++k) add_to_vec(density, idx_c[k], wt_c[k], active);
+            for (int l = 0; l < active; ++l)  // add_to_fluid :371-395 (fetch_or)
+                if (par) __atomic_store_n(&base.fluid[idx_f[l]], (uint8_t)1, __ATOMIC_RELAXED);
+                else base.fluid[idx_f[l]] = 1;
+            for (int k = 0; k < 4; ++k) add_to_mac(u_mom, u_w, idx_u[k], add_u[k], wt_u[k], active, par);
+            for (int k = 0; k < 4; ++k) add_to_mac(v_mom, v_w, idx_v[k], add_v[k], wt_v[k], active, par);
+            for (int k = 0; k < 4; ++k) add_to_vec(density, idx_c[k], wt_c[k], active, par);
         }
-        // normalisation (:1428-1459)
-        for (size_t i = 0; i < nu(); ++i)
-            if (u_w[i] > R(0)) { valid_u[i] = 1; mac_u[i] = u_mom[i] / u_w[i]; }
-        for (size_t i = 0; i < nv(); ++i)
-            if (v_w[i] > R(0)) { valid_v[i] = 1; mac_v[i] = v_mom[i] / v_w[i]; }
+        // normalisation (:1428-1459; rayon::join of u and v, with_min_len(4096))
+        {
+            const size_t nu_ = nu(), nv_ = nv();
+            ORC_PAR(nthreads > 1 && nu_ > 4096)
+            for (size_t i = 0; i < nu_; ++i)
+                if (u_w[i] > R(0)) { valid_u[i] = 1; mac_u[i] = u_mom[i] / u_w[i]; }
+            ORC_PAR(nthreads > 1 && nv_ > 4096)
+            for (size_t i = 0; i < nv_; ++i)
+                if (v_w[i] > R(0)) { valid_v[i] = 1; mac_v[i] = v_mom[i] / v_w[i]; }
+        }
         // fluid lists (:1461-1494)
         fluid_cells = 0;
         for (size_t raw = 0; raw < cells; ++raw) {
@@ -484,7 +548,7 @@ template <class R> struct Sim {
     // add_to_mac, src/apic.rs:295-344: lanes hitting the same face are summed
     // in lane order first, then added to the face.
     static void add_to_mac(std::vector<R>& mom, std::vector<R>& wsum, const uint32_t* idx, const R* val,
-                           const R* wt, int active) {
+                           const R* wt, int active, bool atomic) {
         unsigned processed = 0;
         for (int l = 0; l < active; ++l) {
             if (processed & (1u << l)) continue;
@@ -492,11 +556,13 @@ template <class R> struct Sim {
             R value = val[l], weight = wt[l];
             for (int o = l + 1; o < active; ++o)
                 if (idx[o] == t) { value += val[o]; weight += wt[o]; processed |= 1u << o; }
-            if (value != R(0) || weight != R(0)) { mom[t] = mom[t] + value; wsum[t] = wsum[t] + weight; }
+            if (value == R(0) && weight == R(0)) continue;
+            if (atomic) { cas_add(mom[t], value); cas_add(wsum[t], weight); }
+            else { mom[t] = mom[t] + value; wsum[t] = wsum[t] + weight; }
         }
     }
     // add_to_vec_f32 + atomic_add_f32, src/apic.rs:271-292, 346-369
-    static void add_to_vec(std::vector<R>& dst, const uint32_t* idx, const R* wt, int active) {
+    static void add_to_vec(std::vector<R>& dst, const uint32_t* idx, const R* wt, int active, bool atomic) {
         unsigned processed = 0;
         for (int l = 0; l < active; ++l) {
             if (processed & (1u << l)) continue;
@@ -505,7 +571,8 @@ template <class R> struct Sim {
             for (int o = l + 1; o < active; ++o)
                 if (idx[o] == t) { weight += wt[o]; processed |= 1u << o; }
             if (weight == R(0)) continue;
-            dst[t] = dst[t] + weight;
+            if (atomic) cas_add(dst[t], weight);
+            else dst[t] = dst[t] + weight;
         }
     }
 
@@ -592,8 +659,10 @@ template <class R> struct Sim {
         smoothed = density;
         old_density = density;
         const size_t w = W;
+        const bool par = nthreads > 1 && base.fluid_idx.size() > 4096;  // (:1683-1684, 1779-1781)
         for (int pass = 0; pass < 3; ++pass) {
             smoothed.swap(old_density);
+            ORC_PAR(par)
             for (uint32_t k = 0; k < fluid_cells; ++k) {
                 size_t raw = base.fluid_idx[k] - w;
                 R sum = old_density[raw] * R(4);
@@ -609,6 +678,7 @@ template <class R> struct Sim {
             }
         }
         const R target = 9, noise = target * R(1.15f), rate = R(0.1f);
+        ORC_PAR(par)
         for (uint32_t k = 0; k < fluid_cells; ++k) {
             size_t fi = base.fluid_idx[k], raw = fi - w;
             size_t gx = raw % w, gy = raw / w, ui = gy * (w + 1) + gx;
@@ -680,8 +750,12 @@ template <class R> struct Sim {
         };
         size_t n = fluid_cells;
         if (n >= 8192) {
+            const long nch = (long)((n + 8191) / 8192);
+            std::vector<R> part((size_t)nch);
+            ORC_PAR(nthreads > 1)
+            for (long c = 0; c < nch; ++c) part[(size_t)c] = chunk((size_t)c * 8192, std::min(n, (size_t)c * 8192 + 8192));
             R total = 0;
-            for (size_t b = 0; b < n; b += 8192) total += chunk(b, std::min(n, b + 8192));
+            for (long c = 0; c < nch; ++c) total += part[(size_t)c];
             return total;
         }
         return chunk(0, n);
@@ -704,12 +778,23 @@ template <class R> struct Sim {
             if (std::fabs(delta) < R(kSafety)) { stats.pcg_exit_reason = 2; return; }
             R alpha = gamma / delta;
             R maxe = 0;
-            for (uint32_t k = 0; k < fluid_cells; ++k) {
-                size_t pi = base.fluid_idx[k], raw = pi - W;
-                pressure[raw] += alpha * search[pi];
-                R nr = std::fma(-alpha, As[pi], base.r[pi]);
-                base.r[pi] = nr;
-                maxe = rmax(maxe, std::fabs(nr));
+            if (nthreads > 1) {  // par_iter().map().reduce(max) (:1910-1928): max is order-free
+                ORC_PAR_RED(true, max, maxe)
+                for (uint32_t k = 0; k < fluid_cells; ++k) {
+                    size_t pi = base.fluid_idx[k], raw = pi - W;
+                    pressure[raw] += alpha * search[pi];
+                    R nr = std::fma(-alpha, As[pi], base.r[pi]);
+                    base.r[pi] = nr;
+                    maxe = std::max(maxe, std::fabs(nr));
+                }
+            } else {
+                for (uint32_t k = 0; k < fluid_cells; ++k) {
+                    size_t pi = base.fluid_idx[k], raw = pi - W;
+                    pressure[raw] += alpha * search[pi];
+                    R nr = std::fma(-alpha, As[pi], base.r[pi]);
+                    base.r[pi] = nr;
+                    maxe = rmax(maxe, std::fabs(nr));
+                }
             }
             stats.pcg_max_residual = (float)maxe;
             if (maxe < solve_error) { stats.pcg_exit_reason = 3; return; }
@@ -718,6 +803,7 @@ template <class R> struct Sim {
             if (ng < R(kSafety)) { stats.pcg_exit_reason = 4; return; }
             R beta = ng / gamma;
             gamma = ng;
+            ORC_PAR(nthreads > 1 && fluid_cells > 4096)
             for (uint32_t k = 0; k < fluid_cells; ++k) {
                 size_t i = base.fluid_idx[k];
                 search[i] = base.e[i] + (beta * search[i]);
@@ -755,18 +841,44 @@ template <class R> struct Sim {
         std::vector<double>& pd = pressure_d;
         pd.assign(pressure.begin(), pressure.end());
         std::vector<R> b(base.r);
+        const bool par = nthreads > 1;
+        const long ncells = (long)cells;
         auto fluid_sum = [&](auto term) {
             A acc = 0;
-            for (size_t raw = 0; raw < cells; ++raw)
-                if (base.fluid[raw]) acc += term(raw + w);
+            if (par) {
+                ORC_PAR_RED(true, +, acc)
+                for (long raw = 0; raw < ncells; ++raw)
+                    if (base.fluid[(size_t)raw]) acc += term((size_t)raw + w);
+            } else {
+                for (size_t raw = 0; raw < cells; ++raw)
+                    if (base.fluid[raw]) acc += term(raw + w);
+            }
             return acc;
+        };
+        // r.z over the dense padded interior, and (A s).z beside it
+        auto dense_dots = [&](A& ng_out, A* asz) {
+            A ng = 0, az = 0;
+            if (par) {
+                ORC_PAR_RED(true, +, ng, az)
+                for (long i = (long)start; i < (long)end; ++i) {
+                    ng += A(base.r[(size_t)i]) * A(base.e[(size_t)i]);
+                    if (asz) az += A(As[(size_t)i]) * A(base.e[(size_t)i]);
+                }
+            } else {
+                for (size_t i = start; i < end; ++i) {
+                    ng += A(base.r[i]) * A(base.e[i]);
+                    if (asz) az += A(As[i]) * A(base.e[i]);
+                }
+            }
+            ng_out = ng;
+            if (asz) *asz = az;
         };
         const A bnorm2 = fluid_sum([&](size_t i) { return A(base.r[i]) * A(base.r[i]); });
         if (!(bnorm2 > 0)) { stats.pcg_exit_reason = 1; return; }
         apply_preconditioner();
         std::copy(base.e.begin() + start, base.e.begin() + end, search.begin() + start);
         A gamma = 0;
-        for (size_t i = start; i < end; ++i) gamma += A(base.r[i]) * A(base.e[i]);
+        dense_dots(gamma, nullptr);
         if (!(gamma > 0)) { stats.pcg_exit_reason = 1; return; }
         const A tol2 = A(solver_rel_tol) * A(solver_rel_tol);
         auto pd_at = [&](int64_t raw) { return raw >= 0 && raw < (int64_t)cells ? pd[(size_t)raw] : 0.0; };
@@ -774,6 +886,7 @@ template <class R> struct Sim {
         // apply_matrix_and_dot); returns ||r||^2
         auto true_residual = [&]() {
             double acc = 0;
+            ORC_PAR_RED(par, +, acc)
             for (size_t raw = 0; raw < cells; ++raw) {
                 if (!base.fluid[raw]) continue;
                 size_t pi = raw + w;
@@ -802,9 +915,10 @@ template <class R> struct Sim {
                 if (rr <= tol2 * bnorm2) { stats.pcg_iterations = it; stats.pcg_exit_reason = 3; return; }
                 apply_preconditioner();
                 A ng = 0;
-                for (size_t i = start; i < end; ++i) ng += A(base.r[i]) * A(base.e[i]);
+                dense_dots(ng, nullptr);
                 if (!(ng > 0)) { stats.pcg_exit_reason = 4; return; }
                 gamma = ng;
+                ORC_PAR(par)
                 for (size_t raw = 0; raw < cells; ++raw) {  // beta = 0
                     if (!base.fluid[raw]) continue;
                     size_t i = raw + w;
@@ -815,6 +929,7 @@ template <class R> struct Sim {
             // A s with FP64 row sums rounded once to R (for R = float: the
             // products of two floats are exact in double), so that the
             // recurrence residual tracks the true residual
+            ORC_PAR(par)
             for (uint32_t k = 0; k < fluid_cells; ++k) {
                 const size_t i = base.fluid_idx[k];
                 double v = double(base.diag[i]) * double(search[i]);
@@ -828,6 +943,7 @@ template <class R> struct Sim {
             if (!(delta > 0)) { stats.pcg_exit_reason = 2; return; }
             const A alpha_a = gamma / delta;
             const R alpha = R(alpha_a);
+            ORC_PAR(par)
             for (size_t raw = 0; raw < cells; ++raw) {
                 if (!base.fluid[raw]) continue;
                 size_t pi = raw + w;
@@ -849,13 +965,11 @@ template <class R> struct Sim {
             // r_k.z_{k+1} = r_{k+1}.z_{k+1} + alpha (A s).z_{k+1}: robust when the
             // V-cycle (a coarse CG with absolute exits) is not a fixed linear operator
             A ng = 0, asz = 0;
-            for (size_t i = start; i < end; ++i) {
-                ng += A(base.r[i]) * A(base.e[i]);
-                asz += A(As[i]) * A(base.e[i]);
-            }
+            dense_dots(ng, &asz);
             if (!(ng > 0)) { stats.pcg_exit_reason = 4; return; }
             const R beta = R(-alpha_a * asz / gamma);
             gamma = ng;
+            ORC_PAR(par)
             for (size_t raw = 0; raw < cells; ++raw) {
                 if (!base.fluid[raw]) continue;
                 size_t i = raw + w;
@@ -873,6 +987,7 @@ template <class R> struct Sim {
     // apply_pressure_gradient, src/apic.rs:1958-2055
     void apply_pressure_gradient() {
         const R target = 9, inv_cell = R(1) / cell;
+        ORC_PAR(nthreads > 1 && H > 32)
         for (int32_t y = 0; y < (int32_t)H; ++y)
             for (int32_t x = 0; x <= (int32_t)W; ++x) {
                 size_t f = u_index(x, y);
@@ -887,6 +1002,7 @@ template <class R> struct Sim {
                 else if (rf) pd = pressure[ri] / theta(smoothed[li], smoothed[ri], target);
                 mac_u[f] -= pd * inv_cell;
             }
+        ORC_PAR(nthreads > 1 && H > 32)
         for (int32_t y = 0; y < (int32_t)H + 1; ++y)
             for (int32_t x = 0; x < (int32_t)W; ++x) {
                 size_t f = v_index(x, y);
@@ -914,6 +1030,7 @@ template <class R> struct Sim {
         for (int pass = 0; pass < 4; ++pass) {
             mac_u.swap(old_u);
             valid_u.swap(next_valid_u);
+            ORC_PAR(nthreads > 1 && h > 32)
             for (size_t y = 0; y < h; ++y)
                 for (size_t x = 0; x <= w; ++x) {
                     size_t i = y * (w + 1) + x;
@@ -930,6 +1047,7 @@ template <class R> struct Sim {
         for (int pass = 0; pass < 4; ++pass) {
             mac_v.swap(old_v);
             valid_v.swap(next_valid_v);
+            ORC_PAR(nthreads > 1 && h > 32)
             for (size_t y = 0; y <= h; ++y)
                 for (size_t x = 0; x < w; ++x) {
                     size_t i = y * w + x;
@@ -951,6 +1069,7 @@ template <class R> struct Sim {
         const R min_y = R(1.001f) * cell, max_y = ((R)(float)H - R(1.001f)) * cell;
         const R h2 = cell * cell, one = R(1), eps = R(1e-6f), half = R(0.5), zero = R(0);
         const R bd = -border_damping;
+        ORC_PAR(nthreads > 1 && nalloc > 8 * 256)  // par_iter over chunks (:2079-2085)
         for (size_t i = 0; i < nalloc; ++i) {
             R x = p[PX_][i], y = p[PY_][i];
             R gx = x / cell, gy = y / cell;
@@ -1352,6 +1471,12 @@ void orc_set_gravity_x(void* hv, double gx) {
 void orc_get_stats(void* hv, apic_step_stats* out) {
     Handle* h = static_cast<Handle*>(hv);
     *out = DISPATCH(h, S<float>(h)->stats, S<double>(h)->stats);
+}
+
+// Threads of the threaded mode (the CPU baseline); 1, the default, is the checker.
+void orc_set_threads(void* hv, int threads) {
+    Handle* h = static_cast<Handle*>(hv);
+    if (h->precision == 64) S<double>(h)->set_threads(threads); else S<float>(h)->set_threads(threads);
 }
 
 void orc_set_p2g_shuffle(void* hv, uint32_t seed) {

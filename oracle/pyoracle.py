@@ -74,6 +74,7 @@ def lib():
         L.orc_set_timestamp.argtypes = [vp, ctypes.c_uint32, ctypes.c_double]
         L.orc_get_stats.argtypes = [vp, ctypes.POINTER(StepStats)]
         L.orc_set_p2g_shuffle.argtypes = [vp, ctypes.c_uint32]
+        L.orc_set_threads.argtypes = [vp, ctypes.c_int]
         L.orc_set_solver.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.orc_set_gravity_x.argtypes = [vp, ctypes.c_double]
         L.orc_max_particle_velocity.restype = ctypes.c_double
@@ -175,6 +176,11 @@ class Oracle:
         s = StepStats()
         lib().orc_get_stats(self._h, ctypes.byref(s))
         return {k: getattr(s, k) for k, _ in StepStats._fields_}
+
+    def set_threads(self, threads):
+        """Threaded mode (the CPU baseline): the reference's parallel loops on
+        `threads` OpenMP threads; 1 (the default) is the deterministic checker."""
+        lib().orc_set_threads(self._h, int(threads))
 
     def set_p2g_shuffle(self, seed):
         lib().orc_set_p2g_shuffle(self._h, seed)
