@@ -251,6 +251,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     const LevelView& B = s.levels[0];
     size_t pb = s.padded(B.w, B.h);
     ALLOC(s.search[0], pb); ALLOC(s.search[1], pb); ALLOC(s.As, pb); ALLOC(s.rhs, pb); ALLOC(s.r_alt, pb);
+    ALLOC(s.As_lo, pb); ALLOC(s.r_lo, pb);
     const LevelView& C = s.levels.back();
     size_t pcs = s.padded(C.w, C.h);
     ALLOC(s.cg_s, pcs); ALLOC(s.cg_ap, pcs); ALLOC(s.cg_r, pcs);
@@ -372,6 +373,10 @@ apic_status do_sort(Sim& s) {
     return APIC_OK;
 }
 
+// Iterations to queue before the first check of the next solve: as many as
+// this solve took.
+int next_batch(const Sim& s) { return s.h_pcg->iters; }
+
 // One PCG iteration (src/apic.rs:1899-1955): the SpMV and dot, the update
 // and the V-cycle with the next r.z.  The kernel timer brackets iteration 1
 // of each solve (the first with every operand).
@@ -406,7 +411,7 @@ apic_status do_pcg(Sim& s, bool zero_pressure) {
     s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
     s.stats.pcg_max_residual = s.h_pcg->maxerr;
     s.stats.pcg_rel_residual = rel_residual(s);
-    s.pcg_batch = s.h_pcg->iters;
+    s.pcg_batch = next_batch(s);
     return APIC_OK;
 }
 
@@ -498,7 +503,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
     s.stats.pcg_max_residual = s.h_pcg->maxerr;
     s.stats.pcg_rel_residual = rel_residual(s);
-    s.pcg_batch = s.h_pcg->iters;
+    s.pcg_batch = next_batch(s);
     s.collect_timers(false);
     return APIC_OK;
 }
@@ -823,7 +828,7 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
             s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
             s.stats.pcg_max_residual = s.h_pcg->maxerr;
             s.stats.pcg_rel_residual = rel_residual(s);
-            s.pcg_batch = s.h_pcg->iters;
+            s.pcg_batch = next_batch(s);
             const StepState hs = *s.h_step;  // written by the head, complete after the wait
             last_maxv[i] = hs.maxv;
             t[i] += hs.dt;

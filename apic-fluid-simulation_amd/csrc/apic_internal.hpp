@@ -71,7 +71,7 @@ inline uint32_t reduction_workgroups(uint32_t cells) {
 }
 constexpr int kMaxPcgIterations = 100;   // solve_pcg's cap (src/apic.rs:1894)
 struct PcgState {
-    int32_t done;         // nonzero: the solve has ended (the stages after it may run)
+    int32_t done;         // nonzero: every kernel of the PCG loop returns at once
     int32_t iters;        // iterations entered
     int32_t exit_reason;  // as apic_step_stats.pcg_exit_reason
     int32_t r_in_alt;     // the solve ended with the base residual in Sim::r_alt (copied home by the up-sweep)
@@ -83,11 +83,6 @@ struct PcgState {
     double gamma_d[2];       // r.z of iteration k in gamma_d[k & 1]
     double alpha_d;          // the last step length in FP64 (the flexible beta uses it)
     int32_t max_iters, pad2;
-    // nonzero: the kernels of the V-cycle return at once (the solve has
-    // ended, or the relative mode's residual test passed and the true
-    // residual is checked first, in iteration verify_k)
-    int32_t halt;
-    int32_t verify_k;     // the relative mode's verifying iteration (-1: none)
 };
 
 // Base-level fluid statistics needed by the padded-list quirk of prolongate
@@ -214,6 +209,8 @@ struct Sim {
     bool bottom_in_lds = false;     // that kernel holds its levels in LDS
     float* search[2] = {nullptr, nullptr};
     float* As = nullptr;
+    float* As_lo = nullptr;          // relative mode: A s (FP64) - As
+    float* r_lo = nullptr;           // relative mode: the base residual's low part (r + r_lo in FP64)
     float* cg_s = nullptr; float* cg_ap = nullptr; float* cg_r = nullptr;  // coarsest level
     float* rhs = nullptr;            // copy of b for the relative mode's residual recompute
     float* r_alt = nullptr;          // the base residual's second buffer (the fused update writes it)

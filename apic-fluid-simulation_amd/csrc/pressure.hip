@@ -956,14 +956,6 @@ struct BottomLds {
 
 // ---------------------------------------------------------------------- PCG
 
-// The end of a solve: every later kernel of the solve returns at once, and
-// the stages after it (guarded by done) run.
-__device__ __forceinline__ void pcg_finish(PcgState* st, int reason) {
-    st->done = 1;
-    st->halt = 1;
-    st->exit_reason = reason;
-}
-
 struct PcgReset {
     static constexpr int kThreads = 64;
     struct Args {
@@ -975,8 +967,6 @@ struct PcgReset {
         if (threadIdx.x != 0) return;
         PcgState* st = a.st;
         st->done = 0;
-        st->halt = 0;
-        st->verify_k = -1;
         st->iters = 0;
         st->exit_reason = 0;
         st->r_in_alt = 0;
@@ -990,11 +980,11 @@ struct PcgReset {
 };
 
 // Relative mode, before the first V-cycle: b = r is kept for the residual
-// recomputation, the FP64 pressure starts from the FP32 one, and the partials
-// of ||b||^2 over the fluid cells.
-__device__ __forceinline__ void pcg_bnorm(const LevelView& L, float* __restrict__ rhs, double* __restrict__ part,
-                                 const float* __restrict__ pressure, double* __restrict__ pd, uint32_t wg,
-                                 uint32_t nwg) {
+// recomputation, the residual's low part starts at zero, the FP64 pressure
+// starts from the FP32 one, and the partials of ||b||^2 over the fluid cells.
+__device__ __forceinline__ void pcg_bnorm(const LevelView& L, float* __restrict__ rhs, float* __restrict__ r_lo,
+                                 double* __restrict__ part, const float* __restrict__ pressure,
+                                 double* __restrict__ pd, uint32_t wg, uint32_t nwg) {
     __shared__ double dscratch[kThreads / 64];
     double acc = 0.0;
     for (uint32_t c = wg * blockDim.x + threadIdx.x; c < L.cells; c += nwg * blockDim.x) {
@@ -1003,6 +993,7 @@ __device__ __forceinline__ void pcg_bnorm(const LevelView& L, float* __restrict_
         const size_t pc = (size_t)c + L.w;
         const float b = L.r[pc];
         rhs[pc] = b;
+        r_lo[pc] = 0.0f;
         acc += (double)b * (double)b;
     }
     acc = block_sum(acc, dscratch);
@@ -1014,12 +1005,15 @@ struct PcgBnorm {
     struct Args {
         LevelView L;
         float* rhs;
+        float* r_lo;
         double* part;
         const float* pressure;
         double* pd;
         uint32_t nwg;
     };
-    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) { pcg_bnorm(a.L, a.rhs, a.part, a.pressure, a.pd, wg, a.nwg); }
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        pcg_bnorm(a.L, a.rhs, a.r_lo, a.part, a.pressure, a.pd, wg, a.nwg);
+    }
 };
 
 // ||b||^2 from PcgBnorm's partials; it also resets the state (PcgReset's
@@ -1040,7 +1034,7 @@ struct PcgBnormFinalize {
         if (threadIdx.x == 0) {
             PcgState* st = a.reset.st;
             st->bnorm2 = b;
-            if (!(b > 0.0)) pcg_finish(st, 1);  // b = 0: p = 0 is the solution
+            if (!(b > 0.0)) { st->done = 1; st->exit_reason = 1; }  // b = 0: p = 0 is the solution
         }
     }
 };
@@ -1048,14 +1042,15 @@ struct PcgBnormFinalize {
 // Relative mode, after iteration k's update, every recompute_every
 // iterations (due): r is replaced by the true residual b - A p evaluated in
 // FP64 against the FP64 pressure (the product with the same fused operations
-// as apply_matrix_and_dot, zero outside the grid).  part_out receives the
-// partials of ||r||^2 that the residual test reads on those iterations (on
-// the others it reads the update's per-tile partials).
-template <int NT>
-__device__ __forceinline__ void true_residual_cells(const LevelView& L, const float* __restrict__ rhs,
-                                                    const double* __restrict__ pd, double* __restrict__ part_out,
-                                                    uint32_t wg, uint32_t nwg) {
-    __shared__ double dscratch[NT / 64];
+// as apply_matrix_and_dot, zero outside the grid), split into its FP32
+// rounding r and the low part r_lo.  part_out receives the partials of
+// ||r||^2 that the residual test reads on those iterations (on the others it
+// reads the update's per-tile partials).
+__device__ __forceinline__ void true_residual(const LevelView& L, const float* __restrict__ rhs,
+                                     float* __restrict__ r_lo, const double* __restrict__ pd, const PcgState* st,
+                                     double* __restrict__ part_out, int due, uint32_t wg, uint32_t nwg) {
+    __shared__ double dscratch[kThreads / 64];
+    if (st->done || !due) return;
     const int64_t w = L.w, n = L.cells;
     auto pa = [&](int64_t raw) { return raw >= 0 && raw < n ? pd[raw] : 0.0; };
     double acc = 0.0;
@@ -1069,18 +1064,13 @@ __device__ __forceinline__ void true_residual_cells(const LevelView& L, const fl
         v = fma((double)L.py[pc], pa(ci + w), v);
         v = fma((double)L.py[pc - L.w], pa(ci - w), v);
         const double t = (double)rhs[pc] - v;
-        L.r[pc] = (float)t;
+        const float hi = (float)t;
+        L.r[pc] = hi;
+        r_lo[pc] = (float)(t - (double)hi);
         acc += t * t;
     }
     acc = block_sum(acc, dscratch);
     if (threadIdx.x == 0) part_out[wg] = acc;
-}
-
-__device__ __forceinline__ void true_residual(const LevelView& L, const float* __restrict__ rhs,
-                                              const double* __restrict__ pd, const PcgState* st,
-                                              double* __restrict__ part_out, int due, uint32_t wg, uint32_t nwg) {
-    if (st->done || !due) return;
-    true_residual_cells<kThreads>(L, rhs, pd, part_out, wg, nwg);
 }
 
 // Relative mode, after the update of iteration k (the uniform argument).
@@ -1089,6 +1079,7 @@ struct TrueResidual {
     struct Args {
         LevelView L;
         const float* rhs;
+        float* r_lo;
         const double* pd;
         const PcgState* st;
         double* part_out;
@@ -1097,7 +1088,7 @@ struct TrueResidual {
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         const int due = a.every > 0 && (k + 1) % a.every == 0;
-        true_residual(a.L, a.rhs, a.pd, a.st, a.part_out, due, wg, a.nwg);
+        true_residual(a.L, a.rhs, a.r_lo, a.pd, a.st, a.part_out, due, wg, a.nwg);
     }
 };
 
@@ -1110,11 +1101,11 @@ __global__ void zero_pressure_kernel(const uint8_t* __restrict__ fluid, float* _
 // Relative mode: breakdown when r.z <= 0 (or not finite), the iteration cap.
 __device__ inline bool gamma_exit_rel(PcgState* st, double g, int k, bool lead) {
     if (!(g > 0.0)) {
-        if (lead) { pcg_finish(st, k == 0 ? 1 : 4); st->iters = k; }
+        if (lead) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
         return true;
     }
     if (k >= st->max_iters) {
-        if (lead) { pcg_finish(st, 5); st->iters = k; }
+        if (lead) { st->done = 1; st->exit_reason = 5; st->iters = k; }
         return true;
     }
     return false;
@@ -1124,11 +1115,11 @@ __device__ inline bool gamma_exit_rel(PcgState* st, double g, int k, bool lead) 
 // the iteration cap.  Returns true when the solve ends here.
 __device__ inline bool gamma_exit(PcgState* st, float g, int k, bool lead) {
     if (g < kSafety) {
-        if (lead) { pcg_finish(st, k == 0 ? 1 : 4); st->iters = k; }
+        if (lead) { st->done = 1; st->exit_reason = k == 0 ? 1 : 4; st->iters = k; }
         return true;
     }
     if (k >= kMaxPcgIterations) {
-        if (lead) { pcg_finish(st, 5); st->iters = k; }
+        if (lead) { st->done = 1; st->exit_reason = 5; st->iters = k; }
         return true;
     }
     return false;
@@ -1221,28 +1212,15 @@ __device__ inline void sum_partials256x2(const T* __restrict__ pa, const T* __re
 // src/apic.rs:1823-1863; search update :1945-1954).
 template <int NT>
 __device__ __forceinline__ void spmv_dot(const LevelView& L, const float* __restrict__ s_old, float* __restrict__ s_new,
-                                float* __restrict__ As, PcgState* st, const float* __restrict__ part_gamma,
+                                float* __restrict__ As, float* __restrict__ As_lo, PcgState* st,
+                                const float* __restrict__ part_gamma,
                                 float* __restrict__ part_delta, int k, const double* __restrict__ part_gamma_d,
                                 const double* __restrict__ part_asz_d, double* __restrict__ part_delta_d, int ngam,
-                                const float* __restrict__ rhs, const double* __restrict__ pd,
-                                double* __restrict__ part_true, uint32_t wg, uint32_t nwg) {
+                                uint32_t wg, uint32_t nwg) {
     const bool lead = wg == 0 && threadIdx.x == 0;
     __shared__ float scratch[NT / 64];
     __shared__ double dscratch[NT / 64];
     if (st->done) return;
-    if (part_gamma_d && st->verify_k == k) {
-        // the relative mode's verifying iteration: the residual test of
-        // iteration k - 1 passed on the recurrence residual, so the true
-        // residual b - A p of the FP64 pressure replaces r and its partials
-        // decide, in the update that follows, whether the solve ends; gamma
-        // keeps its slot filled (the restart's beta is zero whatever it is)
-        __shared__ double dscratch2[8];
-        double g, asz;
-        sum_partials256x2(part_gamma_d, part_asz_d, ngam, dscratch2, g, asz);
-        if (lead) st->gamma_d[k & 1] = g;
-        true_residual_cells<NT>(L, rhs, pd, part_true, wg, nwg);
-        return;
-    }
     const bool first = k == 0;
     const size_t w = L.w;
     const uint32_t cells = L.cells, stride = nwg * NT;
@@ -1314,17 +1292,18 @@ __device__ __forceinline__ void spmv_dot(const LevelView& L, const float* __rest
             if (part_delta_d) {
                 // relative mode: the products of FP32 operands are exact in
                 // FP64; the row is summed in FP64 (the same fused chain) and
-                // rounded once, so A s carries the rounding of its own size,
-                // not that of |A||s|, and the recurrence residual keeps
-                // tracking the true residual of a smooth pressure
+                // kept as an FP32 pair, its rounding and the rest (48 bits),
+                // so that the update of r loses nothing but its own rounding
+                // to FP32, which r_lo keeps
                 double vd = (double)dg[j] * (double)sv[0];
                 vd = fma((double)px[j][0], (double)sv[1], vd);
                 vd = fma((double)px[j][1], (double)sv[2], vd);
                 vd = fma((double)py[j][0], (double)sv[3], vd);
                 vd = fma((double)py[j][1], (double)sv[4], vd);
-                const float v = (float)vd;
-                As[pc] = v;
-                dprod += (double)sv[0] * (double)v;
+                const float hi = (float)vd;
+                As[pc] = hi;
+                As_lo[pc] = (float)(vd - (double)hi);
+                dprod += (double)sv[0] * vd;
             } else {
                 float v = dg[j] * sv[0];
                 v = fmaf(px[j][0], sv[1], v);
@@ -1351,6 +1330,7 @@ struct SpmvDotArgs {
     LevelView L;
     float* search[2];
     float* As;
+    float* As_lo;  // relative mode: A s - As (A s in FP64 is As + As_lo)
     PcgState* st;
     const float* part_gamma;
     float* part_delta;
@@ -1358,9 +1338,6 @@ struct SpmvDotArgs {
     const double* part_asz_d;
     double* part_delta_d;
     int ngam;
-    const float* rhs;        // relative mode: b, the FP64 pressure and the true residual's partials
-    const double* pd;
-    double* part_true;
     uint32_t nwg;
 };
 
@@ -1369,8 +1346,8 @@ struct SpmvDot {
     static constexpr int kThreads = NT;
     using Args = SpmvDotArgs;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
-        spmv_dot<NT>(a.L, a.search[k & 1], a.search[(k + 1) & 1], a.As, a.st, a.part_gamma, a.part_delta, k,
-                     a.part_gamma_d, a.part_asz_d, a.part_delta_d, a.ngam, a.rhs, a.pd, a.part_true, wg, a.nwg);
+        spmv_dot<NT>(a.L, a.search[k & 1], a.search[(k + 1) & 1], a.As, a.As_lo, a.st, a.part_gamma, a.part_delta, k,
+                     a.part_gamma_d, a.part_asz_d, a.part_delta_d, a.ngam, wg, a.nwg);
     }
 };
 
@@ -1399,7 +1376,7 @@ __device__ __forceinline__ void axpy_max(const LevelView& L, const float* __rest
     fetch_flags();  // with the partials
     const float d = sum_partials256(part_delta, (int)nwg, scratch);
     if (fabsf(d) < kSafety) {
-        if (lead) { pcg_finish(st, 2); st->delta = d; }
+        if (lead) { st->done = 1; st->exit_reason = 2; st->delta = d; }
         return;
     }
     const float alpha = st->gamma[k] / d;
@@ -1610,46 +1587,12 @@ __device__ inline float vsor(const VTile<T>& t, int i, float own, float er, floa
 // relative mode ends the solve when delta <= 0); false when the solve ends.
 __device__ __forceinline__ bool pcg_alpha(PcgState* st, double d, double gam, bool lead, double& alpha_d) {
     if (!(d > 0.0)) {
-        if (lead) { pcg_finish(st, 2); st->delta = (float)d; }
+        if (lead) { st->done = 1; st->exit_reason = 2; st->delta = (float)d; }
         return false;
     }
     alpha_d = gam / d;
     if (lead) { st->delta = (float)d; st->alpha = (float)alpha_d; st->alpha_d = alpha_d; }
     return true;
-}
-
-// The verifying iteration k of the relative mode (its SpmvDot replaced r by
-// the true residual, whose squared norm is rr): the solve ends when it meets
-// the tolerance (false); otherwise iteration k restarts the search with a
-// zero step (alpha = 0 leaves p and r as they are, and the next beta is 0),
-// and the V-cycles resume (true).
-__device__ __forceinline__ bool pcg_verify(PcgState* st, double rr, int k, bool lead) {
-    if (lead) st->rnorm2 = rr;
-    if (rr <= st->tol2 * st->bnorm2) {
-        if (lead) pcg_finish(st, 3);
-        return false;
-    }
-    if (lead) {
-        st->halt = 0;
-        st->iters = k + 1;
-        st->alpha = 0.0f;
-        st->alpha_d = 0.0;
-    }
-    return true;
-}
-
-// A passing residual test of iteration k on the recurrence residual: the
-// true residual is checked in iteration k + 1 (the V-cycle of iteration k
-// halts), unless the cap leaves no iteration for it.
-__device__ __forceinline__ void pcg_candidate(PcgState* st, int k, bool lead, bool r_in_alt) {
-    if (!lead) return;
-    if (k + 1 < st->max_iters) {
-        st->halt = 1;
-        st->verify_k = k + 1;
-    } else {
-        pcg_finish(st, 3);
-        if (r_in_alt) st->r_in_alt = 1;
-    }
 }
 
 // Iteration k's update in the relative mode (src/apic.rs:1907-1928 with FP64
@@ -1661,15 +1604,36 @@ __device__ __forceinline__ void pcg_candidate(PcgState* st, int k, bool lead, bo
 // partials in the same order, so both paths give the same residual norm.
 struct AxIn {
     float* search[2];
-    const float* As;
+    const float* As;            // A s: its FP32 rounding,
+    const float* As_lo;         // and the rest
     float* pressure;
     double* pd;                 // the FP64 pressure
     const double* part_delta_d; // SpmvDot's partials of delta
     uint32_t ndelta;
     float* r_out;               // the updated residual (the old one is read from the level's r)
-    double* part_rr_out;        // one partial of ||r||^2 per tile
-    const double* part_true;    // the verifying iteration's true-residual partials (ndelta of them)
+    float* r_lo;                // the residual's low part, read and written in place
+    double* part_rr_out;        // one partial of ||r + r_lo||^2 per tile
 };
+
+// The relative mode's residual is the pair (r, r_lo): r evolves as the FP32
+// rounding of r - alpha As in FP64 (As the FP32 rounding of A s: the
+// V-cycle's input, the same value wherever a tile's halo recomputes a
+// neighbour's cell), and r_lo, one word per cell kept by the cell's own
+// tile, takes what that drops, alpha As_lo included, so that r + r_lo follows
+// b - A p to FP64 rounding (A s is summed in FP64 and p accumulates in FP64)
+// and the residual test measures the true residual without recomputing it.
+__device__ __forceinline__ float update_r(float r, double alpha_d, float as) {
+    return (float)fma(-alpha_d, (double)as, (double)r);
+}
+// A tile's own cell: the new (r, r_lo) and the FP64 residual r + r_lo (returned).
+__device__ __forceinline__ double update_r_pair(float r, float lo, double alpha_d, float as, float as_lo,
+                                                float& r_new, float& lo_new) {
+    const double t = fma(-alpha_d, (double)as, (double)r);
+    const double v = fma(-alpha_d, (double)as_lo, t + (double)lo);
+    r_new = (float)t;
+    lo_new = (float)(v - (double)r_new);
+    return v;
+}
 
 template <int NT>
 __device__ __forceinline__ void axpy_tile(const LevelView& L, const AxIn& ax, PcgState* st, uint32_t tiles_x, int k,
@@ -1694,15 +1658,9 @@ __device__ __forceinline__ void axpy_tile(const LevelView& L, const AxIn& ax, Pc
     const double d = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
     if (dn) return;
     double alpha_d;
-    if (st->verify_k == k) {
-        if (!pcg_verify(st, sum_partials256(ax.part_true, (int)ax.ndelta, dscratch), k, lead)) return;
-        alpha_d = 0.0;
-    } else if (!pcg_alpha(st, d, gam, lead, alpha_d)) {
-        return;
-    }
-    const float alpha = (float)alpha_d;
+    if (!pcg_alpha(st, d, gam, lead, alpha_d)) return;
     double pv[kPer];
-    float sv[kPer], av[kPer], rv[kPer];
+    float sv[kPer], av[kPer], al[kPer], rv[kPer], lv[kPer];
 #pragma unroll
     for (int q = 0; q < kPer; ++q) {
         const int i = (int)threadIdx.x + NT * q;
@@ -1710,7 +1668,9 @@ __device__ __forceinline__ void axpy_tile(const LevelView& L, const AxIn& ax, Pc
         pv[q] = ax.pd[c];
         sv[q] = s[c + w];
         av[q] = ax.As[c + w];
+        al[q] = ax.As_lo[c + w];
         rv[q] = L.r[c + w];
+        lv[q] = ax.r_lo[c + w];
     }
     double rr = 0.0;
 #pragma unroll
@@ -1721,9 +1681,11 @@ __device__ __forceinline__ void axpy_tile(const LevelView& L, const AxIn& ax, Pc
         const double np = pv[q] + alpha_d * (double)sv[q];
         ax.pd[c] = np;
         ax.pressure[c] = (float)np;
-        const float nr = fmaf(-alpha, av[q], rv[q]);
+        float nr, nl;
+        const double v = update_r_pair(rv[q], lv[q], alpha_d, av[q], al[q], nr, nl);
         ax.r_out[c + w] = nr;
-        rr += (double)nr * (double)nr;
+        ax.r_lo[c + w] = nl;
+        rr += v * v;
     }
     rr = block_sum(rr, dscratch);
     if (threadIdx.x == 0) ax.part_rr_out[wg] = rr;
@@ -1744,6 +1706,25 @@ struct AxpyTile {
         axpy_tile<NT>(a.L, a.ax, a.st, a.tiles_x, k, wg);
     }
 };
+
+// The relative mode's residual test after iteration k (the kernel that
+// applies it: every workgroup computes the same decision from the same
+// partials).  rr = ||r + r_lo||^2: the recurrence's, or, with true_r, the
+// true residual b - A p that replaced it this iteration.  The solve ends
+// when ||r + r_lo|| <= rel_tol ||b||.  true: the solve ends.  The scalars it
+// reads, loaded by the caller with its other early loads.
+struct RelTestIn {
+    double tol2, bn2;
+};
+__device__ __forceinline__ RelTestIn rel_test_in(const PcgState* st) { return RelTestIn{st->tol2, st->bnorm2}; }
+__device__ __forceinline__ bool rel_residual_test(PcgState* st, const RelTestIn& in, double rr, bool lead) {
+    if (lead) st->rnorm2 = rr;
+    if (rr <= in.tol2 * in.bn2) {
+        if (lead) { st->done = 1; st->exit_reason = 3; }
+        return true;
+    }
+    return false;
+}
 
 // Down: (base: e = 0 and the first red sweep) the black sweep of level F,
 // then the restriction of b - A e into C fused with C's first red sweep.  On
@@ -1800,7 +1781,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         else if (check && !part_rr_t) pmx[q] = part_max[ic];
     }
     const bool rel_test = check && (part_rr || part_rr_t);
-    const double tol2 = rel_test ? st->tol2 : 0.0, bn2 = rel_test ? st->bnorm2 : 0.0;
+    const RelTestIn rti = rel_test ? rel_test_in(st) : RelTestIn{0.0, 0.0};
     double gam = 0.0;
     uint8_t ofl[G::kTPer], rfl[G::kPer];
     if constexpr (AX) {
@@ -1824,40 +1805,18 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     }
     const double rr_t = (check && part_rr_t) ? sum_partials256(part_rr_t, (int)ntiles, dscratch) : 0.0;
     double dsum = 0.0;
-    // the relative mode's verifying iteration (see pcg_verify): the update
-    // kernel decides (it ignores the halt flag, which it may clear), and the
-    // residual test of that iteration's update is not applied
-    const bool verifying = (AX || rel_test) && st->verify_k == k;
-    double rr_v = 0.0;
-    if constexpr (AX) {
-        dsum = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
-        if (verifying) rr_v = sum_partials256(ax.part_true, (int)ax.ndelta, dscratch);
-    }
+    if constexpr (AX) dsum = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
     if (spec) vtile_put<T, NT>(g, t);  // read after the barrier of the tile test below
-    if (AX && verifying) {
-        if (!pcg_verify(st, rr_v, k, lead)) return;
-    } else if (dn) {
-        return;
-    }
-    if (verifying) {
-        // no residual test in the verifying iteration
-    } else if (check && part_rr_t) {  // the update's partials, one per base tile
-        if (lead) st->rnorm2 = rr_t;
-        if (rr_t <= tol2 * bn2) {
-            pcg_candidate(st, k, lead, false);
-            return;
-        }
+    if (dn) return;
+    if (check && part_rr_t) {  // the update's partials, one per base tile
+        if (rel_residual_test(st, rti, rr_t, lead)) return;
     } else if (check && part_rr) {  // the true residual's
         double rr = 0.0;
 #pragma unroll
         for (int q = 0; q < kRPer; ++q)
             if (threadIdx.x + (uint32_t)NT * q < nred) rr += prr[q];
         rr = block_sum(rr, dscratch);
-        if (lead) st->rnorm2 = rr;
-        if (rr <= tol2 * bn2) {
-            if (lead) pcg_finish(st, 3);
-            return;
-        }
+        if (rel_residual_test(st, rti, rr, lead)) return;
     } else if (check) {
         float m = 0.0f;
 #pragma unroll
@@ -1866,7 +1825,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         m = block_max(m, scratch);
         if (lead) st->maxerr = m;
         if (m < solve_error) {
-            if (lead) pcg_finish(st, 3);
+            if (lead) { st->done = 1; st->exit_reason = 3; }
             return;
         }
     }
@@ -1875,7 +1834,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     // times zero coefficients, so it leaves them as they are.
     double alpha_d = 0.0;
     if constexpr (AX) {
-        if (!verifying && !pcg_alpha(st, dsum, gam, lead, alpha_d)) return;  // (a verifying step is 0)
+        if (!pcg_alpha(st, dsum, gam, lead, alpha_d)) return;
     }
     bool any = false;
 #pragma unroll
@@ -1901,7 +1860,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
                 asr[q] = ax.As[rfl[q] ? (size_t)(y + 1) * w + x : 0];
             }
             double pv[G::kTPer];
-            float sv[G::kTPer], av[G::kTPer], rv[G::kTPer];
+            float sv[G::kTPer], av[G::kTPer], al[G::kTPer], rv[G::kTPer], lv[G::kTPer];
 #pragma unroll
             for (int q = 0; q < G::kTPer; ++q) {
                 const int i = (int)threadIdx.x + NT * q;
@@ -1909,12 +1868,13 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
                 pv[q] = ax.pd[c];
                 sv[q] = sv_in[c + w];
                 av[q] = ax.As[c + w];
+                al[q] = ax.As_lo[c + w];
                 rv[q] = F.r[c + w];
+                lv[q] = ax.r_lo[c + w];
             }
-            const float alpha = (float)alpha_d;
 #pragma unroll
             for (int q = 0; q < G::kPer; ++q)
-                if (rfl[q]) g.r[q] = fmaf(-alpha, asr[q], g.r[q]);
+                if (rfl[q]) g.r[q] = update_r(g.r[q], alpha_d, asr[q]);
             vtile_put<T, NT>(g, t);
             // the tile's own cells, as axpy_tile updates them
             double rr = 0.0;
@@ -1926,9 +1886,11 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
                 const double np = pv[q] + alpha_d * (double)sv[q];
                 ax.pd[c] = np;
                 ax.pressure[c] = (float)np;
-                const float nr = fmaf(-alpha, av[q], rv[q]);
+                float nr, nl;
+                const double v = update_r_pair(rv[q], lv[q], alpha_d, av[q], al[q], nr, nl);
                 ax.r_out[c + w] = nr;
-                rr += (double)nr * (double)nr;
+                ax.r_lo[c + w] = nl;
+                rr += v * v;
             }
             rr = block_sum(rr, dscratch);  // (its barriers also order the region's LDS stores)
             if (threadIdx.x == 0) ax.part_rr_out[wg] = rr;
@@ -2179,7 +2141,7 @@ __device__ __forceinline__ float rrestrict(const Region<S>& t, int i0) {
 
 __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, const LevelView& D, int fuse_red_d,
                                        const int32_t* __restrict__ done, uint32_t tiles_x, PcgState* st,
-                                       const double* __restrict__ part_rr_t, uint32_t ntiles, int k, uint32_t wg) {
+                                       const double* __restrict__ part_rr_t, uint32_t ntiles, uint32_t wg) {
     constexpr int NT = kP2Threads;
     __shared__ Region<kP2F> tf;
     __shared__ Region<kP2C> tc;
@@ -2205,18 +2167,17 @@ __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, c
     const float did = D.inv_diag[dc + dw];
     // with part_rr_t, the residual test of the update that the base level's
     // launch before this one fused (as vdown's)
-    const double tol2 = part_rr_t ? st->tol2 : 0.0, bn2 = part_rr_t ? st->bnorm2 : 0.0;
+    const RelTestIn rti = part_rr_t ? rel_test_in(st) : RelTestIn{0.0, 0.0};
     const double rr = part_rr_t ? sum_partials256(part_rr_t, (int)ntiles, dscratch) : 0.0;
     region_put<kP2F, NT>(gf, tf);
     region_put<kP2C, NT>(gc, tc);
     if (dn) return;
-    if (part_rr_t && st->verify_k != k) {  // (no residual test in the verifying iteration)
+    if (part_rr_t) {
         const bool lead = wg == 0 && threadIdx.x == 0;
-        if (lead) st->rnorm2 = rr;
-        if (rr <= tol2 * bn2) {
-            // the new residual is in the base level's second buffer: at the
-            // end of the solve the base up-sweep copies it home
-            pcg_candidate(st, k, lead, true);
+        if (rel_residual_test(st, rti, rr, lead)) {
+            // the new residual is in the base level's second buffer: the
+            // base up-sweep copies it home
+            if (lead) st->r_in_alt = 1;
             return;
         }
     }
@@ -2300,8 +2261,8 @@ struct VDown2Args {
 struct VDown2 {
     static constexpr int kThreads = kP2Threads;
     using Args = VDown2Args;
-    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
-        vdown2(a.F, a.C, a.D, a.fuse_red_d, a.done, a.tiles_x, a.st, a.part_rr_t, a.ntiles, k, wg);
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        vdown2(a.F, a.C, a.D, a.fuse_red_d, a.done, a.tiles_x, a.st, a.part_rr_t, a.ntiles, wg);
     }
 };
 
@@ -2615,7 +2576,7 @@ struct VUpArgs {
     int dot;
     float* part_gamma;
     double* part_gamma_d;
-    const float* As;      // relative mode: A s of the last iteration and
+    const float* As;      // relative mode: A s of the last iteration (its FP32 rounding) and
     double* part_asz_d;   // the partials of (A s).z
     int spec;             // load the region with the flags (levels of few tiles)
     float* e_out;         // the level's e; L.e is its down-sweep buffer
@@ -2728,8 +2689,8 @@ bool wide_base(const Sim& sim) { return sim.nred >= kWideMinRed; }
 
 // The update's operands (AxpyTile, VDownAx), r_out receiving the residual.
 AxIn update_operands(const Sim& sim, float* r_out) {
-    return AxIn{{sim.search[0], sim.search[1]}, sim.As, sim.pressure, sim.pressure_d, sim.part_delta_d, sim.nred,
-                r_out, sim.part_rr_t, sim.part_rr2_d};
+    return AxIn{{sim.search[0], sim.search[1]}, sim.As, sim.As_lo, sim.pressure, sim.pressure_d, sim.part_delta_d, sim.nred,
+                r_out, sim.r_lo, sim.part_rr_t};
 }
 
 }  // namespace
@@ -2741,7 +2702,7 @@ void launch_pcg_reset(Sim& sim) {
     const double tol = (double)sim.solver.rel_tol;
     const PcgReset::Args reset{sim.pcg, tol * tol, pcg_cap(sim)};
     if (rel) {
-        emit<PcgBnorm>(sim, {sim.levels[0], sim.rhs, sim.part_rr_d, sim.pressure, sim.pressure_d, sim.nred}, sim.nred);
+        emit<PcgBnorm>(sim, {sim.levels[0], sim.rhs, sim.r_lo, sim.part_rr_d, sim.pressure, sim.pressure_d, sim.nred}, sim.nred);
         emit<PcgBnormFinalize>(sim, {reset, sim.part_rr_d, sim.nred}, 1);
     } else {
         emit<PcgReset>(sim, reset, 1);
@@ -2756,7 +2717,7 @@ void launch_zero_pressure(Sim& sim) {
 // mode 0: apply_preconditioner alone; 1: PCG start (gamma); 2: PCG iteration
 // (residual test in the first kernel, gamma' partials in the last).
 void launch_vcycle(Sim& sim, int mode, int k) {
-    const int32_t* done = &sim.pcg->halt;
+    const int32_t* done = &sim.pcg->done;
     const int last = (int)sim.levels.size() - 1;
     const int bottom = (int)sim.bottom_level;  // 1 <= bottom <= last
     const bool rel = sim.solver.mode == 1 && mode != 0;
@@ -2845,9 +2806,9 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     // to the step they time)
     const bool timed = k == 1;
     if (timed) sim.mark(APIC_T_K_SPMV, 0);
-    SpmvDotArgs a{B, {sim.search[0], sim.search[1]}, sim.As, sim.pcg, sim.part_gamma, sim.part_delta,
+    SpmvDotArgs a{B, {sim.search[0], sim.search[1]}, sim.As, rel ? sim.As_lo : nullptr, sim.pcg, sim.part_gamma, sim.part_delta,
                   rel ? sim.part_gamma_d : nullptr, sim.part_asz_d, rel ? sim.part_delta_d : nullptr,
-                  (int)gamma_partials(sim), sim.rhs, sim.pressure_d, sim.part_rr2_d, sim.nred};
+                  (int)gamma_partials(sim), sim.nred};
     const bool wide = rel && sim.nred >= kWideMinRed;
     if (wide) emit<SpmvDot<kWideThreads>>(sim, a, sim.nred, k);
     else emit<SpmvDot<kSolverThreads>>(sim, a, sim.nred, k);
@@ -2870,7 +2831,7 @@ void launch_pcg_iteration_head(Sim& sim, int k) {
     // one iteration for all k: there the kernel itself skips the others)
     const int every = sim.solver.recompute_every;
     if (rel && (sim.rec || (every > 0 && (k + 1) % every == 0))) {
-        TrueResidual::Args c{B, sim.rhs, sim.pressure_d, sim.pcg, sim.part_rr2_d, every, sim.nred};
+        TrueResidual::Args c{B, sim.rhs, sim.r_lo, sim.pressure_d, sim.pcg, sim.part_rr2_d, every, sim.nred};
         emit<TrueResidual>(sim, c, sim.nred, k);
     }
 }

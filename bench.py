@@ -450,7 +450,7 @@ def large_run(steps=50, warmup=3, stage_steps=5, oracle_steps=2):
            "pcg_rel_residual_max": max(x for x in r["rel"] if x >= 0),
            "pcg_exit_reasons": {str(k): int(sum(1 for x in r["reasons"] if x == k)) for k in sorted(set(r["reasons"]))},
            "pcg_true_rel_residual_last": r["true_residual"],
-           "valid": all(x == 3 for x in r["reasons"]) and tr64 is not None and tr64 <= 1.1e-6,
+           "valid": all(x == 3 for x in r["reasons"]) and tr64 is not None and tr64 <= 1e-6 * (1 + 1e-6),
            "stages": rates}
     if oracle_steps > 0:
         out["fp64_oracle_error"] = fp64_oracle_error(c, xy, props, oracle_steps)
@@ -619,11 +619,11 @@ def main():
     rels = [x for x in r["rel"] if x >= 0]
     tol = SOLVERS[args.solver].get("rel_tol")
     # every timed solve ended by the tolerance, and the last one's FP64
-    # pressure meets it in the true residual (10% slack for the rounding of
-    # the FP32 recurrence)
+    # pressure meets it in the true residual evaluated on the host (up to
+    # the FP64 rounding of that evaluation)
     tr64 = (r["true_residual"] or {}).get("fp64_pressure")
     valid = all(x == 3 for x in r["reasons"]) and (tol is None or (
-        all(x <= tol for x in rels) and tr64 is not None and tr64 <= 1.1 * tol))
+        all(x <= tol for x in rels) and tr64 is not None and tr64 <= tol * (1 + 1e-6)))
     traffic = traffic_summary(dom)
 
     line = {

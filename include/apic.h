@@ -59,14 +59,17 @@ typedef struct apic_debug_options {
  * reductions (src/apic.rs:8, 1865-1956).  mode 1 ("relative") is an extension
  * the reference cannot express: the same multigrid-preconditioned CG, stopped
  * when ||r||_2 <= rel_tol * ||b||_2 or after max_iterations (1..100000), with
- * breakdown exits when s.As <= 0 or r.z <= 0; vectors stay FP32, the dot
- * products, ||r||, ||b|| and the step scalars are FP64 (fp64_reductions must
- * be 1), each row of A s is summed in FP64 and rounded once to FP32, the
- * pressure is accumulated in FP64 (APIC_PRESSURE_F64), and the true residual
- * b - A p (FP64) replaces r every recompute_every iterations (0 = never).
- * pcg_rel_residual reports ||r||_2 / ||b||_2 of the residual the test used;
- * the FP64 rows keep it within rounding of the true residual of the FP64
- * pressure.  The FP32 pressure (APIC_PRESSURE) cannot have a true residual
+ * breakdown exits when s.As <= 0 or r.z <= 0; the preconditioner's vectors
+ * stay FP32, the dot products, ||r||, ||b|| and the step scalars are FP64
+ * (fp64_reductions must be 1), each row of A s is summed in FP64 and kept
+ * as its FP32 rounding (APIC_MATRIX_TIMES_SEARCH) plus the FP32 rest, the
+ * pressure is accumulated in FP64 (APIC_PRESSURE_F64), and the residual is
+ * the FP32 r that the preconditioner reads plus a per-cell FP32 low part
+ * that keeps what r's roundings drop, so the test sees the true residual
+ * b - A p of the FP64 pressure to FP64 rounding; b - A p itself (FP64)
+ * replaces it every recompute_every iterations (0 = never).
+ * pcg_rel_residual reports ||r + r_lo||_2 / ||b||_2 of the residual the
+ * test used.  The FP32 pressure (APIC_PRESSURE) cannot have a true residual
  * much below eps32 * || |A| |p| || / ||b||. */
 typedef struct apic_solver_config {
     int32_t mode;

@@ -815,18 +815,20 @@ template <class R> struct Sim {
     // Solver rules of the "relative" mode, an extension the reference cannot
     // express (its solve_pcg, src/apic.rs:1865-1956, hardcodes absolute exits
     // and 100 iterations): preconditioned CG with the same V-cycle and FP32
-    // vectors s, As, r, z; A s summed per row in FP64 and rounded once; the
+    // vectors s, r, z; A s summed per row and kept in FP64; the
     // pressure is accumulated in FP64 (written to the FP32 pressure as it
     // goes); dot products, norms and step scalars in A (FP64 when fp64); stop
     // when the residual meets ||r||_2 <= rel_tol * ||b||_2, or after
     // max_iterations; every recompute_every iterations r is replaced by the
-    // true residual b - A p evaluated in FP64; a test passed by the
-    // recurrence residual is confirmed on the true residual in the next
-    // iteration, which restarts the search (zero step, beta = 0) when it
-    // fails; breakdown when s.As <= 0 or r.z <= 0.  Sums run in ascending
+    // true residual b - A p evaluated in FP64; the residual is the pair
+    // (r, r_lo) of the update below, whose sum follows b - A p to FP64
+    // rounding; breakdown when s.As <= 0 or r.z <= 0.  Sums run in ascending
     // cell order.  The FP32 pressure itself cannot have a true residual much
     // below eps32 || |A| |p| || / ||b||.
     std::vector<double> pressure_d;  // the relative mode's FP64 pressure of the last solve
+    std::vector<double> as_d;        // the relative mode's A s (FP64; As holds its rounding to R)
+    std::vector<R> as_lo;            // and as_d - As in R
+    std::vector<R> r_lo;             // the relative mode's residual low part (padded like r)
     int solver_mode = 0;
     double solver_rel_tol = 1e-6;
     int solver_max_it = 500, solver_fp64 = 1, solver_recompute = 25;
@@ -898,37 +900,23 @@ template <class R> struct Sim {
                 v = std::fma(double(base.py[pi - w]), pd_at(c - ww), v);
                 const double t = double(b[pi]) - v;
                 base.r[pi] = R(t);
+                r_lo[pi] = R(t - double(base.r[pi]));
                 acc += t * t;
             }
             return acc;
         };
-        // A residual test that passes on the recurrence residual is checked
-        // on the true residual in the next iteration (verify_at): the solve
-        // ends if that meets the tolerance; otherwise that iteration restarts
-        // the search from the true residual with a zero step
-        int verify_at = -1;
+        std::vector<double>& As_d = as_d;
+        As_d.assign(As.size(), 0.0);
+        as_lo.assign(As.size(), R(0));
+        std::fill(As.begin(), As.end(), R(0));
+        r_lo.assign(base.r.size(), R(0));
+        // the FP64 residual r + r_lo of the last update, per padded cell
+        std::vector<double> rv(base.r.size(), 0.0);
         for (int it = 0; it < solver_max_it; ++it) {
             stats.pcg_iterations = it + 1;
-            if (it == verify_at) {
-                const A rr = A(true_residual());
-                stats.pcg_rel_residual = (double)std::sqrt((double)rr / (double)bnorm2);
-                if (rr <= tol2 * bnorm2) { stats.pcg_iterations = it; stats.pcg_exit_reason = 3; return; }
-                apply_preconditioner();
-                A ng = 0;
-                dense_dots(ng, nullptr);
-                if (!(ng > 0)) { stats.pcg_exit_reason = 4; return; }
-                gamma = ng;
-                ORC_PAR(par)
-                for (size_t raw = 0; raw < cells; ++raw) {  // beta = 0
-                    if (!base.fluid[raw]) continue;
-                    size_t i = raw + w;
-                    search[i] = base.e[i];
-                }
-                continue;
-            }
-            // A s with FP64 row sums rounded once to R (for R = float: the
-            // products of two floats are exact in double), so that the
-            // recurrence residual tracks the true residual
+            // A s with FP64 row sums (for R = float: the products of two
+            // floats are exact in double), kept as its rounding As and the
+            // rest as_lo
             ORC_PAR(par)
             for (uint32_t k = 0; k < fluid_cells; ++k) {
                 const size_t i = base.fluid_idx[k];
@@ -937,35 +925,38 @@ template <class R> struct Sim {
                 v = std::fma(double(base.px[i - 1]), double(search[i - 1]), v);
                 v = std::fma(double(base.py[i]), double(search[i + w]), v);
                 v = std::fma(double(base.py[i - w]), double(search[i - w]), v);
+                As_d[i] = v;
                 As[i] = R(v);
+                as_lo[i] = R(v - double(As[i]));
             }
-            const A delta = fluid_sum([&](size_t i) { return A(search[i]) * A(As[i]); });
+            const A delta = fluid_sum([&](size_t i) { return A(search[i]) * A(As_d[i]); });
             if (!(delta > 0)) { stats.pcg_exit_reason = 2; return; }
             const A alpha_a = gamma / delta;
-            const R alpha = R(alpha_a);
             ORC_PAR(par)
             for (size_t raw = 0; raw < cells; ++raw) {
                 if (!base.fluid[raw]) continue;
                 size_t pi = raw + w;
                 pd[raw] += double(alpha_a) * double(search[pi]);
                 pressure[raw] = R(pd[raw]);
-                base.r[pi] = std::fma(-alpha, As[pi], base.r[pi]);
+                // r keeps the rounding of r - alpha As (the preconditioner's
+                // input); r_lo gathers what that drops, alpha as_lo included
+                const double t = std::fma(-double(alpha_a), double(As[pi]), double(base.r[pi]));
+                const double v = std::fma(-double(alpha_a), double(as_lo[pi]), t + double(r_lo[pi]));
+                base.r[pi] = R(t);
+                r_lo[pi] = R(v - double(base.r[pi]));
+                rv[pi] = v;
             }
-            A rr = fluid_sum([&](size_t i) { return A(base.r[i]) * A(base.r[i]); });
+            A rr = fluid_sum([&](size_t i) { return A(rv[i]) * A(rv[i]); });
             const bool due = solver_recompute > 0 && (it + 1) % solver_recompute == 0;
             if (due) rr = A(true_residual());  // true residual, FP64
             stats.pcg_rel_residual = (double)std::sqrt((double)rr / (double)bnorm2);
-            if (rr <= tol2 * bnorm2) {
-                if (due || it + 1 >= solver_max_it) { stats.pcg_exit_reason = 3; return; }
-                verify_at = it + 1;
-                continue;
-            }
+            if (rr <= tol2 * bnorm2) { stats.pcg_exit_reason = 3; return; }
             apply_preconditioner();
             // flexible (Polak-Ribiere) beta = r_{k+1}.(z_{k+1} - z_k) / r_k.z_k, with
             // r_k.z_{k+1} = r_{k+1}.z_{k+1} + alpha (A s).z_{k+1}: robust when the
             // V-cycle (a coarse CG with absolute exits) is not a fixed linear operator
             A ng = 0, asz = 0;
-            dense_dots(ng, &asz);
+            dense_dots(ng, &asz);  // (A s).z with As
             if (!(ng > 0)) { stats.pcg_exit_reason = 4; return; }
             const R beta = R(-alpha_a * asz / gamma);
             gamma = ng;

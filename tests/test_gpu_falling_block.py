@@ -11,7 +11,7 @@ residual recompute every 25 iterations, relative 1e-6 test in FP64.
   - At full size, properties that do not need the oracle: each solve exits by
     the tolerance with ||r|| / ||b|| <= 1e-6, and the true residual
     ||b - A p|| / ||b|| of the FP64 pressure, evaluated on the host in FP64,
-    is within 1.1e-6; the Morton sort (23-bit keys, three radix passes) equals
+    is within 1e-6 (1 + 1e-6) and within 1e-12 of the reported residual; the Morton sort (23-bit keys, three radix passes) equals
     a numpy stable argsort of keys computed independently from the positions
     (src/apic.rs:1049-1066), bit for bit; the particle-to-grid density sums to
     the sum of the particles' temporal weights (the cell stencil's weights sum
@@ -76,7 +76,8 @@ def test_falling_block_eighth_scale_against_oracle():
         print(f"  {f}: GPU vs FP32 oracle {err:.2e} (oracle spread {spread:.2e}), "
               f"vs FP64 oracle {rel_l2(g.read(f), o64.read(f)):.2e}")
         assert err <= max(1e-5, 2.0 * spread), (f, err, spread)
-    assert _host_true_residual(g) <= 1.1e-6
+    tr = _host_true_residual(g)
+    assert tr <= 1e-6 * (1 + 1e-6) and abs(tr - g.stats()["pcg_rel_residual"]) <= 1e-12, tr
     g.close()
 
 
@@ -100,7 +101,7 @@ def test_falling_block_full_size_properties():
         print(f"  update {step}: {st['pcg_iterations']} PCG iterations, ||r||/||b|| {st['pcg_rel_residual']:.3e}, "
               f"true residual of the FP64 pressure {tr:.3e}", flush=True)
         assert st["pcg_exit_reason"] == 3 and 0.0 < st["pcg_rel_residual"] <= 1e-6
-        assert tr <= 1.1e-6, tr
+        assert tr <= 1e-6 * (1 + 1e-6) and abs(tr - st["pcg_rel_residual"]) <= 1e-12, (tr, st)
     # the sort of the current positions against numpy
     x, y = g.read("POS_X")[:n], g.read("POS_Y")[:n]
     fdx = np.float32(dx)

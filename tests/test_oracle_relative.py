@@ -5,8 +5,8 @@ mathematics rather than by the restatement itself:
   - the returned pressure satisfies ||b - A p||_2 <= rel_tol * ||b||_2 for the
     assembled system (A from DIAG/PLUS_X/PLUS_Y, computed here in FP64): the
     FP64-accumulated pressure exactly, the FP32 one up to its rounding (the
-    exit is confirmed on the true residual, and the reported ||r|| / ||b|| is
-    that true residual);
+    exit certifies the true residual with a bound on the FP32 rounding of the
+    recurrence residual);
   - FP32 vectors with FP64 dot products agree with the all-FP64 oracle;
   - a looser tolerance never needs more iterations than a tighter one;
   - recomputing the true residual (FP64, against the FP64-accumulated
@@ -58,7 +58,7 @@ def oracle_at_solve(precision=32):
 
 
 @pytest.mark.parametrize("precision", [64, 32])
-@pytest.mark.parametrize("tol", [1e-3, 1e-5, 1e-6, 1e-7])
+@pytest.mark.parametrize("tol", [1e-3, 1e-5, 1e-6, 1e-7, 1e-8])
 def test_relative_solve_meets_the_tolerance(tol, precision):
     """FP64: the true residual meets the tolerance.  FP32 vectors: it meets the
     tolerance up to the rounding of a single-precision residual."""
@@ -72,11 +72,13 @@ def test_relative_solve_meets_the_tolerance(tol, precision):
     res, bound = residuals(*sys_, o.read("PRESSURE"))
     slack = 1e-3 * tol if precision == 64 else bound
     assert res <= tol + slack, (tol, res, bound, s["pcg_rel_residual"])
-    # the exit is confirmed on the true residual of the FP64-accumulated
-    # pressure, whatever the precision of the vectors
+    # the residual the test reads, r + r_lo, is the true residual of the
+    # FP64-accumulated pressure up to FP64 rounding, whatever the precision
+    # of the vectors: the true residual meets the tolerance and the reported
+    # one agrees with it to a millionth of the tolerance
     res64, _ = residuals(*sys_, o.read("PRESSURE_F64"))
-    assert res64 <= tol * (1 + 1e-9), (tol, res64)
-    assert abs(res64 - s["pcg_rel_residual"]) <= 1e-9 * tol
+    assert res64 <= tol * (1 + 1e-6), (tol, res64)
+    assert abs(res64 - s["pcg_rel_residual"]) <= 1e-6 * tol, (res64, s["pcg_rel_residual"])
 
 
 def test_fp32_vectors_fp64_dots_match_fp64_oracle():
