@@ -255,6 +255,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     const LevelView& C = s.levels.back();
     size_t pcs = s.padded(C.w, C.h);
     ALLOC(s.cg_s, pcs); ALLOC(s.cg_ap, pcs); ALLOC(s.cg_r, pcs);
+    ALLOC(s.part_cg, 3 * kWideCoarseMaxWgs); ALLOC(s.cg_bar, 1);
     ALLOC(s.pcg, 1);
     const size_t ngam = std::max<size_t>(kRedBlocks, gamma_partials(s));
     ALLOC(s.part_gamma, ngam); ALLOC(s.part_delta, kRedBlocks); ALLOC(s.part_max, kRedBlocks);
@@ -377,6 +378,23 @@ apic_status do_sort(Sim& s) {
 // this solve took.
 int next_batch(const Sim& s) { return s.h_pcg->iters; }
 
+// The solve's counters, from the host mirror.  A fault of the multi-workgroup
+// coarse CG (its workgroups were not all resident, and a barrier wait gave
+// up) fails the call and restarts the barrier's count.
+apic_status pcg_outcome(Sim& s) {
+    s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
+    s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
+    s.stats.pcg_max_residual = s.h_pcg->maxerr;
+    s.stats.pcg_rel_residual = rel_residual(s);
+    s.pcg_batch = next_batch(s);
+    if (s.h_pcg->fault) {
+        HIP_TRY(hipMemsetAsync(s.cg_bar, 0, sizeof(*s.cg_bar), s.stream));
+        HIP_TRY(hipStreamSynchronize(s.stream));
+        return fail(APIC_ERR_HIP, "pressure solve: a barrier of the multi-workgroup coarse CG gave up waiting");
+    }
+    return APIC_OK;
+}
+
 // One PCG iteration (src/apic.rs:1899-1955): the SpMV and dot, the update
 // and the V-cycle with the next r.z.  The kernel timer brackets iteration 1
 // of each solve (the first with every operand).
@@ -407,11 +425,7 @@ apic_status do_pcg(Sim& s, bool zero_pressure) {
         if (s.h_pcg->done || k >= pcg_cap(s)) break;
         check_at = k + 2;
     }
-    s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
-    s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
-    s.stats.pcg_max_residual = s.h_pcg->maxerr;
-    s.stats.pcg_rel_residual = rel_residual(s);
-    s.pcg_batch = next_batch(s);
+    if (apic_status st_ = pcg_outcome(s); st_ != APIC_OK) return st_;
     return APIC_OK;
 }
 
@@ -499,11 +513,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
         if ((st = tail()) != APIC_OK) { s.step_src = nullptr; return st; }
     }
     s.step_src = nullptr;
-    s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
-    s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
-    s.stats.pcg_max_residual = s.h_pcg->maxerr;
-    s.stats.pcg_rel_residual = rel_residual(s);
-    s.pcg_batch = next_batch(s);
+    if (apic_status st_ = pcg_outcome(s); st_ != APIC_OK) return st_;
     s.collect_timers(false);
     return APIC_OK;
 }
@@ -824,11 +834,7 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
         for (size_t i : act) {
             Sim& s = *sims[i];
             s.step_src = nullptr;
-            s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
-            s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
-            s.stats.pcg_max_residual = s.h_pcg->maxerr;
-            s.stats.pcg_rel_residual = rel_residual(s);
-            s.pcg_batch = next_batch(s);
+            if ((st = pcg_outcome(s)) != APIC_OK) return st;
             const StepState hs = *s.h_step;  // written by the head, complete after the wait
             last_maxv[i] = hs.maxv;
             t[i] += hs.dt;

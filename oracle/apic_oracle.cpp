@@ -251,6 +251,58 @@ template <class R> struct Level {
             for (uint32_t i : fluid_idx) cg_s[i] = cg_r[i] + (beta * cg_s[i]);
         }
     }
+
+    // The relative mode's coarse CG on a coarsest level of more than 1024
+    // cells (rows of at most 1024 cells, at most 64 x 4096 cells): the
+    // iteration of solve_coarse_exact above (FP32 vectors, the same SpMV
+    // chain and cap) with FP64 dot products, a search vector that is zero on
+    // non-fluid cells (their couplings are zero), and exits relative to the
+    // level's right-hand side instead of absolute ones: r.r = 0 at the start,
+    // s.As <= 0, and ||r|| <= 1e-6 ||r0||.  With absolute exits the coarse
+    // correction vanishes once the residual is small (|s.As| < 1e-5 at the
+    // first step), and on a large coarsest level (a grid side of 2 x odd)
+    // the outer CG then stalls.  The GPU splits the level over workgroups;
+    // its FP64 sums run in another order, which changes the FP32 step
+    // lengths only where an FP64 rounding tips them.
+    static bool wide_coarse(const Level& L) {
+        return L.cells > 1024 && L.w <= 1024 && (L.cells + 4095) / 4096 <= 64;
+    }
+    void solve_coarse_wide() {
+        const size_t start = w, end = start + cells;
+        std::fill(e.begin() + start, e.begin() + end, R(0));
+        std::copy(r.begin() + start, r.begin() + end, cg_r.begin() + start);
+        std::fill(cg_s.begin(), cg_s.end(), R(0));
+        double gamma = 0;
+        for (uint32_t i : fluid_idx) gamma += double(cg_r[i]) * double(cg_r[i]);
+        if (!(gamma > 0)) return;
+        const double stop = 1e-12 * gamma;
+        const size_t maxit = std::min<size_t>(fluid_idx.size(), 100);
+        R beta = 0;
+        for (size_t it = 0; it < maxit; ++it) {
+            for (uint32_t i : fluid_idx) cg_s[i] = it == 0 ? cg_r[i] : cg_r[i] + (beta * cg_s[i]);
+            double delta = 0;
+            for (uint32_t i : fluid_idx) {
+                R v = diag[i] * cg_s[i];
+                v = std::fma(px[i], cg_s[i + 1], v);
+                v = std::fma(px[i - 1], cg_s[i - 1], v);
+                v = std::fma(py[i], cg_s[i + w], v);
+                v = std::fma(py[i - w], cg_s[i - w], v);
+                cg_ap[i] = v;
+                delta += double(cg_s[i]) * double(v);
+            }
+            if (!(delta > 0)) break;
+            const R alpha = R(gamma / delta);
+            double ng = 0;
+            for (uint32_t i : fluid_idx) {
+                e[i] += alpha * cg_s[i];
+                cg_r[i] = cg_r[i] - alpha * cg_ap[i];
+                ng += double(cg_r[i]) * double(cg_r[i]);
+            }
+            if (ng <= stop) break;
+            beta = R(ng / gamma);
+            gamma = ng;
+        }
+    }
 };
 
 enum Pf { PX_ = 0, PY_, VX_, VY_, CUX_, CUY_, CVX_, CVY_, TR_, NPF_ };
@@ -721,7 +773,8 @@ template <class R> struct Sim {
                 levels[l].smooth(false, omega);
                 levels[l].restrict_to(levels[l + 1]);
             }
-            levels[last].solve_coarse_exact();
+            if (solver_mode == 1 && Level<R>::wide_coarse(levels[last])) levels[last].solve_coarse_wide();
+            else levels[last].solve_coarse_exact();
             for (size_t l = last; l-- > 0;) {
                 levels[l].prolongate_from(levels[l + 1]);
                 levels[l].smooth(true, omega);
