@@ -75,35 +75,18 @@ def dam_break_fill(width, height, dx, fill, ppc, seed):
     return dam_break(width, height, dx, fx, fy, ppc, seed)
 
 
-def solver_friendly_sides(side_min, side_max):
-    """Grid sides 2^k, 3 x 2^k and 5 x 2^k in [side_min, side_max]: the
-    reference's multigrid hierarchy halves them down to 4x4, 3x3 or 5x5 cells,
-    whose coarse CG (src/apic.rs:618-704) one wave runs in registers.  Other
-    even sides stop halving at a larger odd side (a side of 2 x 255 at 255 x
-    255 cells), and the serial sums of that level's CG, once per V-cycle,
-    dominate the step."""
-    out = set()
-    for odd in (1, 3, 5):
-        v = odd
-        while v <= side_max:
-            if v >= side_min and v % 2 == 0:
-                out.add(v)
-            v *= 2
-    return sorted(out)
-
-
 def instance_mix(count, side_min, side_max, seed):
     """Independent dam breaks of mixed sizes: square sides log-uniform in
-    [side_min, side_max], rounded (in log space) to the nearest of
-    solver_friendly_sides; fill fraction U[0.2, 0.6], 4-8 particles per cell,
-    gravity angle U[-15, 15] degrees.  Returns a list of dicts (width,
-    height, dx, ppc, fill, angle_deg, gravity, gravity_x, seed, xy)."""
+    [side_min, side_max] (both even), rounded to the nearest even side; fill
+    fraction U[0.2, 0.6], 4-8 particles per cell, gravity angle U[-15, 15]
+    degrees.  Half the even sides are 2 x odd, whose multigrid hierarchy
+    stops at (side / 2)^2 cells.  Returns a list of dicts (width, height, dx,
+    ppc, fill, angle_deg, gravity, gravity_x, seed, xy)."""
     rng = np.random.default_rng(seed)
-    allowed = np.array(solver_friendly_sides(side_min, side_max), dtype=np.float64)
     out = []
     for i in range(count):
         side = float(np.exp(rng.uniform(np.log(side_min), np.log(side_max))))
-        side = int(allowed[np.argmin(np.abs(np.log(allowed) - np.log(side)))])
+        side = int(min(max(2 * round(side / 2.0), side_min), side_max))
         fill = float(rng.uniform(0.2, 0.6))
         ppc = int(rng.integers(4, 9))
         angle = float(rng.uniform(-15.0, 15.0))

@@ -48,6 +48,9 @@ constexpr int kThreads = 256;
 constexpr int kSolverThreads = kThreads;
 constexpr int kBottomThreads = 1024;
 constexpr int kSerialCap = 4096;
+// The relative mode's coarse CG ends when r.r <= kCoarseRelStop2 r0.r0 (the
+// level's residual reduced a hundredfold).
+constexpr double kCoarseRelStop2 = 1e-4;
 
 __device__ inline float atomic_load_f(const float* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -669,13 +672,131 @@ __device__ void coarse_cg_wave(const LevelView& L) {
     if (in) L.e[pc] = e;
 }
 
+// The relative mode's coarse CG on the one-workgroup bottom: the iteration of
+// CoarseCgWide below (FP32 vectors and SpMV chain, FP64 dot products, a
+// search vector that is zero off the fluid cells, exits relative to the
+// level's right-hand side, at most min(fluid cells, 100) iterations), with
+// the FP64 sums of one wave (<= 64 cells, one lane per cell) or of the
+// workgroup.
+__device__ void coarse_cg_rel_wave(const LevelView& L) {
+    const uint32_t n = L.cells;
+    const int w = (int)L.w;
+    const int c = (int)(threadIdx.x & 63);
+    const bool in = (uint32_t)c < n;
+    const size_t pc = (size_t)c + (size_t)w;
+    const bool fl = in && L.fluid[c];
+    const float diag = fl ? L.diag[pc] : 0.0f, px = fl ? L.px[pc] : 0.0f, py = fl ? L.py[pc] : 0.0f;
+    const float pxm = fl ? L.px[pc - 1] : 0.0f, pym = fl ? L.py[pc - w] : 0.0f;
+    float r = fl ? L.r[pc] : 0.0f;
+    float sv = 0.0f, sn = 0.0f, e = 0.0f;
+    auto nb = [&](int d) {
+        const int j = c + d;
+        const float v = __shfl(sn, j & 63, 64);
+        return (j >= 0 && (uint32_t)j < n) ? v : 0.0f;
+    };
+    double gamma = wave_sum(fl ? (double)r * (double)r : 0.0);
+    const double nf = wave_sum(fl ? 1.0 : 0.0);
+    if (gamma > 0.0) {
+        const double stop = kCoarseRelStop2 * gamma;
+        const int maxit = (int)fmin(nf, 100.0);
+        float beta = 0.0f;
+        for (int it = 0; it < maxit; ++it) {
+            sn = fl ? (it == 0 ? r : r + (beta * sv)) : 0.0f;
+            const float sr = nb(1), sl = nb(-1), st = nb(w), sb = nb(-w);
+            float ap = 0.0f;
+            if (fl) {
+                ap = diag * sn;
+                ap = fmaf(px, sr, ap);
+                ap = fmaf(pxm, sl, ap);
+                ap = fmaf(py, st, ap);
+                ap = fmaf(pym, sb, ap);
+            }
+            const double delta = wave_sum(fl ? (double)sn * (double)ap : 0.0);
+            if (!(delta > 0.0)) break;
+            const float alpha = (float)(gamma / delta);
+            if (fl) {
+                e += alpha * sn;
+                r = r - alpha * ap;
+            }
+            sv = sn;
+            const double ng = wave_sum(fl ? (double)r * (double)r : 0.0);
+            if (ng <= stop) break;
+            beta = (float)(ng / gamma);
+            gamma = ng;
+        }
+    }
+    if (in) L.e[pc] = e;
+}
+
+__device__ void coarse_cg_rel_block(const LevelView& L, float* cg_s, float* cg_ap, float* cg_r) {
+    __shared__ double dscratch[2 * (kBottomThreads / 64)];
+    const uint32_t n = L.cells;
+    const size_t w = L.w;
+    // the search vector: zero on the padded range, then on the fluid cells only
+    for (size_t i = threadIdx.x; i < (size_t)n + 2 * w; i += blockDim.x) cg_s[i] = 0.0f;
+    double g = 0.0, nf = 0.0;
+    for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+        const size_t pc = c + w;
+        const bool f = L.fluid[c];
+        const float rv = f ? L.r[pc] : 0.0f;
+        L.e[pc] = 0.0f;
+        cg_r[pc] = rv;
+        if (f) {
+            g += (double)rv * (double)rv;
+            nf += 1.0;
+        }
+    }
+    block_sum2(g, nf, dscratch);  // (its barriers also order the stores above)
+    if (!(g > 0.0)) return;
+    double gamma = g;
+    const double stop = kCoarseRelStop2 * gamma;
+    const int maxit = (int)fmin(nf, 100.0);
+    float beta = 0.0f;
+    for (int it = 0; it < maxit; ++it) {
+        for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+            if (!L.fluid[c]) continue;
+            const size_t pc = c + w;
+            cg_s[pc] = it == 0 ? cg_r[pc] : cg_r[pc] + (beta * cg_s[pc]);
+        }
+        __syncthreads();
+        double dp = 0.0;
+        for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+            if (!L.fluid[c]) continue;
+            const size_t pc = c + w;
+            float v = L.diag[pc] * cg_s[pc];
+            v = fmaf(L.px[pc], cg_s[pc + 1], v);
+            v = fmaf(L.px[pc - 1], cg_s[pc - 1], v);
+            v = fmaf(L.py[pc], cg_s[pc + w], v);
+            v = fmaf(L.py[pc - w], cg_s[pc - w], v);
+            cg_ap[pc] = v;
+            dp += (double)cg_s[pc] * (double)v;
+        }
+        const double delta = block_sum(dp, dscratch);
+        if (!(delta > 0.0)) break;
+        const float alpha = (float)(gamma / delta);
+        double ng = 0.0;
+        for (uint32_t c = threadIdx.x; c < n; c += blockDim.x) {
+            if (!L.fluid[c]) continue;
+            const size_t pc = c + w;
+            L.e[pc] += alpha * cg_s[pc];
+            const float nr = cg_r[pc] - alpha * cg_ap[pc];
+            cg_r[pc] = nr;
+            ng += (double)nr * (double)nr;
+        }
+        ng = block_sum(ng, dscratch);
+        if (ng <= stop) break;
+        beta = (float)(ng / gamma);
+        gamma = ng;
+    }
+}
+
 // Levels [first, count) of the V-cycle inside one workgroup.  On entry
 // level `first` holds r (restricted) and e after its fused first red sweep
 // (or e = 0 when it is the coarsest level).  On exit level `first` holds its
 // final post-smoothed error.
 template <class Views>
 __device__ void bottom_body(const Views& V, int first, int last, float* cg_s, float* cg_ap, float* cg_r,
-                            float* sprod, float* scratch) {
+                            float* sprod, float* scratch, bool rel) {
     Prolong none{};
     // down
     for (int l = first; l < last; ++l) {
@@ -698,7 +819,12 @@ __device__ void bottom_body(const Views& V, int first, int last, float* cg_s, fl
         __syncthreads();
     }
     if (C.cells <= 64) {
-        if (threadIdx.x < 64) coarse_cg_wave(C);
+        if (threadIdx.x < 64) {
+            if (rel) coarse_cg_rel_wave(C);
+            else coarse_cg_wave(C);
+        }
+    } else if (rel) {
+        coarse_cg_rel_block(C, cg_s, cg_ap, cg_r);
     } else {
         coarse_cg_block(C, cg_s, cg_ap, cg_r, sprod, scratch);
     }
@@ -732,13 +858,14 @@ struct BottomGlobal {
         float* cg_ap;
         float* cg_r;
         const int32_t* done;
+        int rel;  // the relative mode's coarse exits
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
         __shared__ float sprod[kSerialCap];
         __shared__ float scratch[kBottomThreads / 64];
         if (*a.done) return;
         GlobalViews V{&a.ls};
-        bottom_body(V, a.first, a.ls.count - 1, a.cg_s, a.cg_ap, a.cg_r, sprod, scratch);
+        bottom_body(V, a.first, a.ls.count - 1, a.cg_s, a.cg_ap, a.cg_r, sprod, scratch, a.rel != 0);
     }
 };
 
@@ -746,12 +873,15 @@ struct BottomGlobal {
 //
 // solve_coarse_exact's iteration (src/apic.rs:618-704: the same FP32 vector
 // arithmetic and fused SpMV chain, at most min(fluid cells, 100) iterations)
-// with FP64 dot products, for a coarsest level too large for one
-// workgroup's serial sums, and with exits relative to the level's
-// right-hand side: r.r = 0 at the start, s.As <= 0, ||r|| <= 1e-6 ||r0||.
-// (With the reference's absolute exits the coarse correction vanishes once
-// the residual is small, |s.As| < 1e-5 at the first step, and on a large
-// coarsest level the outer CG then stalls.)  Workgroup j owns
+// with FP64 dot products, and with exits relative to the level's right-hand
+// side: r.r = 0 at the start, s.As <= 0, ||r|| <= 1e-2 ||r0||.  (With the
+// reference's absolute exits the coarse correction vanishes once the
+// residual is small, |s.As| < 1e-5 at the first step, and the outer CG
+// slows down, or stalls on a large coarsest level.)  This is the relative
+// mode's coarse CG whatever the coarsest level's size: on one of more than
+// 1024 cells (a grid side of 2 x odd stops the hierarchy early) it runs
+// here, over several workgroups, else in the one-workgroup bottom
+// (coarse_cg_rel_wave, coarse_cg_rel_block).  Workgroup j owns
 // the band of cells [j B, (j + 1) B) (B = kWideCoarseBand) and keeps its
 // cells' e, r, s and matrix rows in registers and its search vector, with
 // one row either side, in LDS.  A band publishes its first and last row of r
@@ -904,7 +1034,7 @@ struct CoarseCgWide {
             write_e();
             return;
         }
-        const double stop = 1e-12 * gamma;
+        const double stop = kCoarseRelStop2 * gamma;
         const int maxit = (int)fmin(nfl, 100.0);
         float beta = 0.0f;
         int cur = 0;
@@ -994,7 +1124,8 @@ struct BLev {
     uint32_t fl;        // offset of the fluid flags
 };
 
-__device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const int32_t* __restrict__ done) {
+__device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const int32_t* __restrict__ done,
+                                           bool rel) {
     __shared__ float f[kBottomLdsFloats];
     __shared__ __attribute__((aligned(16))) uint8_t fl[kBottomLdsFlags];  // copied in as words
     __shared__ float scratch[kBottomThreads / 64];
@@ -1126,7 +1257,12 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
         __syncthreads();
     }
     if (C.cells <= 64) {
-        if (threadIdx.x < 64) coarse_cg_wave(C);
+        if (threadIdx.x < 64) {
+            if (rel) coarse_cg_rel_wave(C);
+            else coarse_cg_wave(C);
+        }
+    } else if (rel) {
+        coarse_cg_rel_block(C, cg_s, cg_ap, cg_r);
     } else {
         coarse_cg_block(C, cg_s, cg_ap, cg_r, sprod, scratch);
     }
@@ -1186,8 +1322,11 @@ struct BottomLds {
         LevelSet ls;
         int first;
         const int32_t* done;
+        int rel;  // the relative mode's coarse exits
     };
-    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) { bottom_lds(a.ls, a.first, a.done); }
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
+        bottom_lds(a.ls, a.first, a.done, a.rel != 0);
+    }
 };
 
 // ---------------------------------------------------------------------- PCG
@@ -3014,13 +3153,15 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         else emit<VDown<kBaseTile>>(sim, a, nt, k);
     }
     if (sim.bottom_in_lds) {
-        emit<BottomLds>(sim, {level_set(sim), bottom, done}, 1);
+        emit<BottomLds>(sim, {level_set(sim), bottom, done, (int)(sim.solver.mode == 1)}, 1);
     } else if (const uint32_t g = wide_coarse(sim)) {
         emit<CoarseCgWide>(sim, {sim.levels[last], sim.cg_r, {sim.cg_s, sim.cg_ap}, sim.part_cg, sim.cg_bar, sim.pcg,
                                  done, g},
                            g);
     } else {
-        emit<BottomGlobal>(sim, {level_set(sim), bottom, sim.cg_s, sim.cg_ap, sim.cg_r, done}, 1);
+        emit<BottomGlobal>(sim, {level_set(sim), bottom, sim.cg_s, sim.cg_ap, sim.cg_r, done,
+                                 (int)(sim.solver.mode == 1)},
+                           1);
     }
     for (int l = bottom - 1; l >= 0; --l) {
         // levels l - 1 and l in one launch, as on the way down

@@ -252,21 +252,17 @@ template <class R> struct Level {
         }
     }
 
-    // The relative mode's coarse CG on a coarsest level of more than 1024
-    // cells (rows of at most 1024 cells, at most 64 x 4096 cells): the
-    // iteration of solve_coarse_exact above (FP32 vectors, the same SpMV
-    // chain and cap) with FP64 dot products, a search vector that is zero on
-    // non-fluid cells (their couplings are zero), and exits relative to the
-    // level's right-hand side instead of absolute ones: r.r = 0 at the start,
-    // s.As <= 0, and ||r|| <= 1e-6 ||r0||.  With absolute exits the coarse
-    // correction vanishes once the residual is small (|s.As| < 1e-5 at the
-    // first step), and on a large coarsest level (a grid side of 2 x odd)
-    // the outer CG then stalls.  The GPU splits the level over workgroups;
-    // its FP64 sums run in another order, which changes the FP32 step
+    // The relative mode's coarse CG: the iteration of solve_coarse_exact
+    // above (FP32 vectors, the same SpMV chain and cap) with FP64 dot
+    // products, a search vector that is zero on non-fluid cells (their
+    // couplings are zero), and exits relative to the level's right-hand side
+    // instead of absolute ones: r.r = 0 at the start, s.As <= 0, and
+    // ||r|| <= 1e-2 ||r0||.  With absolute exits the coarse correction
+    // vanishes once the residual is small (|s.As| < 1e-5 at the first step),
+    // and the outer CG slows down, or stalls on a large coarsest level (a
+    // grid side of 2 x odd).  The GPU sums in another order (a wave's or a
+    // workgroup's tree, or workgroups' bands), which changes the FP32 step
     // lengths only where an FP64 rounding tips them.
-    static bool wide_coarse(const Level& L) {
-        return L.cells > 1024 && L.w <= 1024 && (L.cells + 4095) / 4096 <= 64;
-    }
     void solve_coarse_wide() {
         const size_t start = w, end = start + cells;
         std::fill(e.begin() + start, e.begin() + end, R(0));
@@ -275,7 +271,7 @@ template <class R> struct Level {
         double gamma = 0;
         for (uint32_t i : fluid_idx) gamma += double(cg_r[i]) * double(cg_r[i]);
         if (!(gamma > 0)) return;
-        const double stop = 1e-12 * gamma;
+        const double stop = 1e-4 * gamma;
         const size_t maxit = std::min<size_t>(fluid_idx.size(), 100);
         R beta = 0;
         for (size_t it = 0; it < maxit; ++it) {
@@ -773,7 +769,7 @@ template <class R> struct Sim {
                 levels[l].smooth(false, omega);
                 levels[l].restrict_to(levels[l + 1]);
             }
-            if (solver_mode == 1 && Level<R>::wide_coarse(levels[last])) levels[last].solve_coarse_wide();
+            if (solver_mode == 1) levels[last].solve_coarse_wide();
             else levels[last].solve_coarse_exact();
             for (size_t l = last; l-- > 0;) {
                 levels[l].prolongate_from(levels[l + 1]);
@@ -1425,6 +1421,8 @@ template <class R> int stage_t(Sim<R>* s, int stage, float dt, uint32_t ts) {
         case 15: s->base.smooth(true, 1.2f); break;
         // the coarsest level's CG alone (solve_coarse_exact), for the dense-solve pins
         case 16: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels.back().solve_coarse_exact(); break;
+        // the relative mode's coarse CG
+        case 17: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels.back().solve_coarse_wide(); break;
         default: return APIC_ERR_INVALID_ARGUMENT;
     }
     return 0;

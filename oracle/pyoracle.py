@@ -33,6 +33,7 @@ STAGES = {
     "extrapolate_velocity": 9, "transfer_grid_to_particles_and_advect": 10, "compute_keys": 11,
     "prolongate_base": 12, "restrict_base": 13, "smooth_base_forward": 14, "smooth_base_reverse": 15,
     "solve_coarse_exact": 16,
+    "solve_coarse_relative": 17,
 }
 
 

@@ -100,6 +100,7 @@ def test_headline_preconditioner_bit_exact(warm):
     o.write("RESIDUAL", warm["b"])
     o.stage("apply_preconditioner")
     g = _gpu_with_system(warm)
+    g.set_solver_config(**RELATIVE)  # (the coarse CG's relative exits)
     g.apply_preconditioner()
     for lvl in range(warm["nl"] + 1):
         e_o = o.read("ERROR", lvl)

@@ -295,3 +295,96 @@ def test_coarse_cg_exits(precision, bval, moves):
     o.stage("solve_coarse_exact")
     e = o.read("ERROR", 1)
     assert bool(np.any(e != 0.0)) == moves, e[5:30]
+
+
+# ---- the relative mode's coarse CG ------------------------------------------
+# solve_coarse_exact's iteration with FP64 dots, a search vector that is zero
+# off the fluid cells, and exits relative to the level's right-hand side
+# (r.r = 0 at the start, s.As <= 0, ||r|| <= 1e-2 ||r0||), which the relative
+# mode runs on its coarsest level whatever its size.
+
+def square_coarse_oracle(precision, m, diag_value, b):
+    """A (2m) x (2m) grid with m odd: one multigrid level of m x m cells, the
+    coarsest, every cell fluid; operator diag_value on the diagonal, -1
+    towards neighbours (Dirichlet outside), b per cell."""
+    o = oracle(2 * m, 2 * m, 1.0, precision, xy=np.array([[float(m), float(m)]], np.float32))
+    assert o.dims()["num_levels"] == 1
+    n = o.field_length("DIAG", 1)
+    diag, px, py, r = (np.zeros(n) for _ in range(4))
+    idx = m + np.arange(m * m)
+    x = np.arange(m * m) % m
+    y = np.arange(m * m) // m
+    diag[idx] = diag_value
+    px[idx] = np.where(x + 1 < m, -1.0, 0.0)
+    py[idx] = np.where(y + 1 < m, -1.0, 0.0)
+    r[idx] = b
+    o.write("FLUID", np.ones(m * m, np.uint8), 1)
+    for name, a in (("DIAG", diag), ("INV_DIAG", np.where(diag > 0, 1.0 / np.maximum(diag, 1), 0.0)),
+                    ("PLUS_X", px), ("PLUS_Y", py), ("RESIDUAL", r)):
+        o.write(name, a, 1)
+    A = np.zeros((m * m, m * m))
+    for k in range(m * m):
+        A[k, k] = diag_value
+        for j, ok in ((k + 1, x[k] + 1 < m), (k - 1, x[k] > 0), (k + m, y[k] + 1 < m), (k - m, y[k] > 0)):
+            if ok:
+                A[k, j] = -1.0
+    return o, A
+
+
+def _rel_residual(A, b, e):
+    return np.linalg.norm(b - A @ e) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("bval", [1e-7, 1e-3])
+def test_relative_coarse_cg_moves_where_absolute_exits_stop(precision, bval):
+    """The single-cell right-hand sides on which the reference's exits return
+    e = 0 (test_coarse_cg_exits): the relative exits run the solve until the
+    residual ||b - A e|| (evaluated here in FP64 from the dense matrix) is a
+    hundredth of ||b||, for b = 1e-7 as for 1e-3."""
+    m = 5
+    b = np.zeros(m * m)
+    b[12] = bval
+    o, A = square_coarse_oracle(precision, m, 4.0, b)
+    o.stage("solve_coarse_relative")
+    e = o.read("ERROR", 1)[m:m + m * m].astype(np.float64)
+    assert np.any(e != 0.0)
+    assert _rel_residual(A, b, e) <= 1e-2 * (1 + 1e-3)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+@pytest.mark.parametrize("m", [5, 35])
+def test_relative_coarse_cg_is_scale_invariant(precision, m):
+    """With exits relative to the right-hand side, scaling b by a power of two
+    scales every operation of the solve exactly: e(2^-20 b) = 2^-20 e(b) bit
+    for bit (the absolute exits would stop at once on the small one:
+    |s.As| < 1e-5)."""
+    rng = np.random.default_rng(7)
+    b = rng.uniform(-1.0, 1.0, m * m)
+    outs = []
+    for scale in (1.0, 2.0 ** -20):
+        o, A = square_coarse_oracle(precision, m, 4.0, b * scale)
+        o.stage("solve_coarse_relative")
+        outs.append(o.read("ERROR", 1)[m:m + m * m].astype(np.float64))
+    assert np.array_equal(outs[1], outs[0] * 2.0 ** -20)
+    assert _rel_residual(A, b, outs[0]) <= 1e-2 * (1 + 1e-3)
+
+
+@pytest.mark.parametrize("precision", [64, 32])
+def test_relative_coarse_cg_on_a_well_conditioned_system(precision):
+    """On a system with condition number below 3 (diagonal 8) the hundredfold
+    residual reduction is an error below 3e-2 of the dense solution, and CG
+    reaches it in a few iterations; on 35 x 35 Dirichlet Poisson (condition
+    number about 500) it is reached within the 100-iteration cap."""
+    rng = np.random.default_rng(11)
+    m = 35
+    b = rng.uniform(-1.0, 1.0, m * m)
+    o, A = square_coarse_oracle(precision, m, 8.0, b)
+    o.stage("solve_coarse_relative")
+    e = o.read("ERROR", 1)[m:m + m * m].astype(np.float64)
+    sol = np.linalg.solve(A, b)
+    assert np.abs(e - sol).max() <= 3e-2 * np.abs(sol).max()
+    o, A = square_coarse_oracle(precision, m, 4.0, b)
+    o.stage("solve_coarse_relative")
+    e = o.read("ERROR", 1)[m:m + m * m].astype(np.float64)
+    assert _rel_residual(A, b, e) <= 1e-2 * (1 + 1e-3)
