@@ -75,13 +75,14 @@ def dam_break_fill(width, height, dx, fill, ppc, seed):
     return dam_break(width, height, dx, fx, fy, ppc, seed)
 
 
-def instance_mix(count, side_min, side_max, seed):
+def instance_mix(count, side_min, side_max, seed, progress=None):
     """Independent dam breaks of mixed sizes: square sides log-uniform in
     [side_min, side_max] (both even), rounded to the nearest even side; fill
     fraction U[0.2, 0.6], 4-8 particles per cell, gravity angle U[-15, 15]
     degrees.  Half the even sides are 2 x odd, whose multigrid hierarchy
     stops at (side / 2)^2 cells.  Returns a list of dicts (width, height, dx,
-    ppc, fill, angle_deg, gravity, gravity_x, seed, xy)."""
+    ppc, fill, angle_deg, gravity, gravity_x, seed, xy).  progress(i), when
+    given, is called after every 256th instance."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(count):
@@ -96,6 +97,8 @@ def instance_mix(count, side_min, side_max, seed):
         out.append(dict(width=side, height=side, dx=dx, ppc=ppc, fill=fill, angle_deg=angle,
                         gravity=float(-g * np.cos(np.radians(angle))), gravity_x=float(g * np.sin(np.radians(angle))),
                         seed=inst_seed, xy=dam_break_fill(side, side, dx, fill, ppc, inst_seed)))
+        if progress is not None and (i + 1) % 256 == 0:
+            progress(i + 1)
     return out
 
 

@@ -354,7 +354,7 @@ def batched_run(count=256, side_min=64, side_max=512, seed=4, steps=10, warmup=3
     }
 
 
-def served_run(pool=128, n_requests=400, rates=(50.0, 200.0, 800.0, 3200.0), seed=5, side_min=64, side_max=1024,
+def served_run(pool=2048, n_requests=400, rates=(200.0, 800.0, 3200.0), seed=5, side_min=64, side_max=1024,
                dt=0.002):
     """The served workload: requests of 1-20 updates for simulations of a
     resident pool (sides 64-1024, 4-8 ppc, relative 1e-6 solver) arriving as a
@@ -371,7 +371,8 @@ def served_run(pool=128, n_requests=400, rates=(50.0, 200.0, 800.0, 3200.0), see
     from FLIP.apic import Apic, WorldProperties
     from FLIP.serve import Server, SimSpec, poisson_trace, run_one_at_a_time, run_served
     t0 = time.perf_counter()
-    mix = scenes.instance_mix(pool, side_min, side_max, seed)
+    mix = scenes.instance_mix(pool, side_min, side_max, seed, progress=lambda i: print(
+        f"[bench] served workload: seeded {i}/{pool} ({time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True))
     specs = [SimSpec(m["width"], m["height"], m["dx"], m["xy"], gravity=m["gravity"], gravity_x=m["gravity_x"],
                      solver=SOLVERS["relative"]) for m in mix]
     n_particles = int(sum(sp.xy.shape[0] for sp in specs))
@@ -382,7 +383,10 @@ def served_run(pool=128, n_requests=400, rates=(50.0, 200.0, 800.0, 3200.0), see
         trace = poisson_trace(n_requests, rate, pool, seed * 1000 + int(rate))
         base_trace = copy.deepcopy(trace)
         server = Server(specs, frame_dt=dt)
-        server.preload(range(pool))
+        for k in range(0, pool, 256):
+            server.preload(range(k, min(pool, k + 256)))
+            print(f"[bench] served at {rate} req/s: preloaded {len(server.resident)} of {min(pool, k + 256)} "
+                  f"({server.used / 2**30:.1f} GiB)", file=sys.stderr, flush=True)
         wall = run_served(server, trace)
         lat = np.array([r.t_done - r.t_arrival for r in trace]) * 1e3
         served = {"value": server.particle_steps / wall, "unit": "particle-steps/s", "wall_s": wall,
@@ -392,13 +396,17 @@ def served_run(pool=128, n_requests=400, rates=(50.0, 200.0, 800.0, 3200.0), see
         server.close()
         print(f"[bench] served at {rate} req/s: {wall:.2f}s, p50 {served['p50_ms']:.1f} ms, "
               f"p99 {served['p99_ms']:.1f} ms", file=sys.stderr, flush=True)
+        # one at a time: the simulations the trace asks for, created beforehand
         sims = {}
-        for sid, sp in enumerate(specs):
+        for sid in sorted({r.sim for r in base_trace}):
+            sp = specs[sid]
             s = Apic.with_particles(sp.width, sp.height, sp.dx,
                                     WorldProperties(sp.gravity, sp.border_damping, sp.cfl, sp.solve_error), sp.xy)
             s.set_gravity_x(sp.gravity_x)
             s.set_solver_config(**sp.solver)
             sims[sid] = s
+        print(f"[bench] one at a time at {rate} req/s: {len(sims)} simulations created", file=sys.stderr,
+              flush=True)
         wall2, ps2 = run_one_at_a_time(sims, specs, base_trace, frame_dt=dt)
         for s in sims.values():
             s.close()
@@ -553,7 +561,9 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256, help="simulations in the batched workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
     ap.add_argument("--serve-only", action="store_true", help="print only the served workload's result")
-    ap.add_argument("--serve-pool", type=int, default=128, help="simulations in the served pool")
+    ap.add_argument("--serve-pool", type=int, default=2048, help="simulations in the served pool")
+    ap.add_argument("--serve-rates", type=lambda v: tuple(float(x) for x in v.split(",")),
+                    default=(200.0, 800.0, 3200.0), help="arrival rates of the served workload (req/s)")
     ap.add_argument("--no-large", action="store_true", help="skip the 4096 x 2048 mixed-precision workload")
     ap.add_argument("--large-only", action="store_true", help="print only the large workload's result")
     ap.add_argument("--large-oracle-steps", type=int, default=2,
@@ -580,7 +590,7 @@ def main():
         print(json.dumps(large_run(oracle_steps=args.large_oracle_steps)))
         return
     if args.serve_only:
-        print(json.dumps(served_run(pool=args.serve_pool)))
+        print(json.dumps(served_run(pool=args.serve_pool, rates=args.serve_rates)))
         return
     cfg = scenes.CONFIGS[WORKLOAD]
     xy = scenes.config_scene(WORKLOAD)
@@ -667,7 +677,7 @@ def main():
     if not args.no_batch:
         line["batched"] = batched_run(count=args.batch_count)
     if not args.no_serve:
-        line["served"] = served_run(pool=args.serve_pool)
+        line["served"] = served_run(pool=args.serve_pool, rates=args.serve_rates)
     if not args.no_cpu:
         threads = baseline_threads()
         line["cpu_baseline"] = cpu_baseline(xy, cfg, args.cpu_seconds, args.solver, threads)
