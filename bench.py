@@ -422,6 +422,43 @@ def served_run(pool=2048, n_requests=400, rates=(200.0, 800.0, 3200.0), seed=5, 
             "steps_per_request": [1, 20], "solver": SOLVER_NAMES["relative"], "rates": out}
 
 
+def harness_run(sizes=(64, 128, 256, 512, 1024), warmup=15, iters=20, dt=0.016):
+    """The reference's own benchmark (benches/fluid_benchmark.rs:10-39):
+    Apic::new(size, size, 1.0, props), the constructor's particle seeding,
+    under the reference's solver rules; `warmup` updates of dt = 0.016, then
+    `iters` timed updates (each followed by a stream synchronisation, as the
+    harness times one update call at a time).  Per size: ms per update and
+    particle-steps/s.  At cell size 1.0 the pressure solve of these updates
+    exits at its first test (r.z below 1e-5), in the oracle as on the GPU."""
+    from FLIP.apic import Apic, DebugOptions, WorldProperties
+    props = WorldProperties(-9.81, 0.1, 5.0, 1e-5)
+    opts = DebugOptions(False)
+    out = []
+    for size in sizes:
+        g = Apic.new(size, size, 1.0, props)
+        n = g.dims()["num_particles"]
+        for _ in range(warmup):
+            g.update(opts, dt)
+        g.get_max_particle_velocity()
+        t0 = time.perf_counter()
+        subs, its = 0, 0
+        for _ in range(iters):
+            g.update(opts, dt)
+            g.get_max_particle_velocity()
+            st = g.stats()
+            subs += st["substeps"]
+            its += st["pcg_iterations_total"]
+        el = time.perf_counter() - t0
+        g.close()
+        out.append({"size": size, "particles": n, "ms_per_update": el / iters * 1e3,
+                    "value": n * subs / el, "unit": "particle-steps/s", "substeps_mean": subs / iters,
+                    "pcg_iterations_mean": its / iters})
+        print(f"[bench] harness {size}^2: {n} particles, {el / iters * 1e3:.3f} ms per update",
+              file=sys.stderr, flush=True)
+    return {"workload": "reference_harness", "solver": SOLVER_NAMES["reference"], "warmup": warmup,
+            "iters": iters, "dt": dt, "sizes": out}
+
+
 def large_run(steps=50, warmup=3, stage_steps=5, oracle_steps=2):
     """The mixed-precision large case (a 4096 x 2048 falling block, 35.6M
     particles): FP32 transfers and SpMV (FP64 row sums), FP64 PCG reductions,
@@ -568,6 +605,8 @@ def main():
     ap.add_argument("--large-only", action="store_true", help="print only the large workload's result")
     ap.add_argument("--large-oracle-steps", type=int, default=2,
                     help="updates of the large workload compared with the FP64 oracle")
+    ap.add_argument("--no-harness", action="store_true", help="skip the reference harness's workload")
+    ap.add_argument("--harness-only", action="store_true", help="print only the reference harness's result")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only a 1 GiB device copy (for scaling the HBM traffic counters)")
     args = ap.parse_args()
@@ -591,6 +630,9 @@ def main():
         return
     if args.serve_only:
         print(json.dumps(served_run(pool=args.serve_pool, rates=args.serve_rates)))
+        return
+    if args.harness_only:
+        print(json.dumps(harness_run()))
         return
     cfg = scenes.CONFIGS[WORKLOAD]
     xy = scenes.config_scene(WORKLOAD)
@@ -672,6 +714,8 @@ def main():
                          "ms_per_step": ro["elapsed"] / o_steps * 1e3,
                          "pcg_iterations_mean": float(np.mean(ro["iters"]))},
     }
+    if not args.no_harness:
+        line["harness"] = harness_run()
     if not args.no_large:
         line["large"] = large_run(oracle_steps=0 if args.no_cpu else args.large_oracle_steps)
     if not args.no_batch:
