@@ -27,6 +27,7 @@ SYMBOLS = [
     "apic_create", "apic_create_with_particles", "apic_destroy", "apic_last_error", "apic_update",
     "apic_add_radial_force", "apic_get_max_particle_velocity", "apic_is_fluid",
     "apic_set_solver_config", "apic_get_stats", "apic_get_dims", "apic_get_timestamp",
+    "apic_get_tuning", "apic_set_tuning", "apic_autotune",
     "apic_set_timestamp", "apic_field_length", "apic_read_field", "apic_write_field",
     "apic_read_particles_aosoa", "apic_write_particles_aosoa",
     "apic_update_spatial_lookup", "apic_transfer_particles_to_grid", "apic_apply_external_forces",
@@ -47,6 +48,11 @@ class WorldPropertiesC(ctypes.Structure):
 
 class DebugOptionsC(ctypes.Structure):
     _fields_ = [("disable_particle_sort", ctypes.c_uint8)]
+
+
+class TuningC(ctypes.Structure):
+    _fields_ = [("radix_bits", ctypes.c_uint32), ("few_tiles", ctypes.c_uint32),
+                ("sort_ms", ctypes.c_float), ("vcycle_ms", ctypes.c_float)]
 
 
 class SolverConfigC(ctypes.Structure):
@@ -90,6 +96,9 @@ def lib():
     L.apic_get_max_particle_velocity.argtypes = [vp, ctypes.POINTER(f32)]
     L.apic_is_fluid.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_uint64)]
     L.apic_set_solver_config.argtypes = [vp, ctypes.POINTER(SolverConfigC)]
+    L.apic_get_tuning.argtypes = [vp, ctypes.POINTER(TuningC)]
+    L.apic_set_tuning.argtypes = [vp, ctypes.POINTER(TuningC)]
+    L.apic_autotune.argtypes = [vp, u32, ctypes.POINTER(TuningC)]
     L.apic_get_stats.argtypes = [vp, ctypes.POINTER(StepStatsC)]
     L.apic_get_dims.argtypes = [vp, u32p, u32p, u32p, u32p, u32p]
     L.apic_get_timestamp.argtypes = [vp, u32p, ctypes.POINTER(f32)]

@@ -95,6 +95,24 @@ class Apic:
         cfg = _lib.SolverConfigC(mode, rel_tol, max_iterations, int(fp64_reductions), recompute_every)
         check(lib().apic_set_solver_config(self._h, ctypes.byref(cfg)))
 
+    def get_tuning(self):
+        """The launch shapes (radix_bits, few_tiles) and the autotuner's timings."""
+        t = _lib.TuningC()
+        check(lib().apic_get_tuning(self._h, ctypes.byref(t)))
+        return {"radix_bits": t.radix_bits, "few_tiles": t.few_tiles, "sort_ms": t.sort_ms, "vcycle_ms": t.vcycle_ms}
+
+    def set_tuning(self, radix_bits=11, few_tiles=1024):
+        """Sets the launch shapes by hand (they never change the result)."""
+        t = _lib.TuningC(radix_bits, few_tiles, 0.0, 0.0)
+        check(lib().apic_set_tuning(self._h, ctypes.byref(t)))
+
+    def autotune(self, reps=5):
+        """Times the candidate launch shapes on the current state, keeps the
+        fastest and returns them (as get_tuning)."""
+        t = _lib.TuningC()
+        check(lib().apic_autotune(self._h, reps, ctypes.byref(t)))
+        return {"radix_bits": t.radix_bits, "few_tiles": t.few_tiles, "sort_ms": t.sort_ms, "vcycle_ms": t.vcycle_ms}
+
     def set_gravity_x(self, gx):
         """Horizontal gravity added to u faces next to fluid (extension; 0 = the reference)."""
         check(lib().apic_set_gravity_x(self._h, gx))

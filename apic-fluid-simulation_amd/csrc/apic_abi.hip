@@ -1283,6 +1283,80 @@ apic_status apic_update_multilevel_grid(apic_handle* h) {
     return APIC_OK;
 }
 
+apic_status apic_get_tuning(apic_handle* h, apic_tuning* out) {
+    if (!h || !out) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    *out = h->sim.tuning;
+    return APIC_OK;
+}
+
+apic_status apic_set_tuning(apic_handle* h, const apic_tuning* t) {
+    if (!h || !t) return fail(APIC_ERR_INVALID_ARGUMENT, "null argument");
+    if (t->radix_bits < 8 || t->radix_bits > 11) return fail(APIC_ERR_INVALID_ARGUMENT, "radix_bits must be in [8, 11]");
+    HIP_TRY(hipStreamSynchronize(h->sim.stream));
+    h->sim.tuning = apic_tuning{t->radix_bits, t->few_tiles, 0.0f, 0.0f};
+    return APIC_OK;
+}
+
+// Each candidate runs once untimed, then reps times between two events on
+// the simulation's stream; the kernel timers are off meanwhile.
+apic_status apic_autotune(apic_handle* h, uint32_t reps, apic_tuning* out) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    if (reps == 0 || reps > 1000) return fail(APIC_ERR_INVALID_ARGUMENT, "reps must be in [1, 1000]");
+    Sim& s = h->sim;
+    HIP_TRY(hipStreamSynchronize(s.stream));
+    const int prof = s.profiling;
+    s.profiling = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    apic_tuning best = s.tuning;
+    apic_status st = APIC_OK;
+    auto time_ms = [&](auto&& work, float& ms) -> apic_status {
+        work();
+        HIP_TRY(hipEventRecord(e0, s.stream));
+        for (uint32_t i = 0; i < reps; ++i) work();
+        HIP_TRY(hipEventRecord(e1, s.stream));
+        CHECK_LAUNCH();
+        HIP_TRY(hipEventSynchronize(e1));
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        ms /= (float)reps;
+        return APIC_OK;
+    };
+    // the sort of the keys of the current positions; the particles are not
+    // permuted, and the next sort makes its keys again
+    if (s.num_particles > 1) {
+        float best_ms = 0.0f;
+        for (uint32_t b = 8; b <= 11 && st == APIC_OK; ++b) {
+            s.tuning.radix_bits = b;
+            float ms = 0.0f;
+            st = time_ms([&]() { launch_keygen(s); launch_radix_sort(s); }, ms);
+            if (st == APIC_OK && (b == 8 || ms < best_ms)) { best_ms = ms; best.radix_bits = b; }
+        }
+        s.keys_valid = false;
+        best.sort_ms = best_ms;
+    }
+    // apply_preconditioner on the current system
+    if (st == APIC_OK && s.levels.size() > 1) {
+        launch_pcg_reset(s);
+        float best_ms = 0.0f;
+        const uint32_t cand[] = {0u, 64u, 256u, 1024u, 4096u, 16384u};
+        for (size_t i = 0; i < sizeof(cand) / sizeof(cand[0]) && st == APIC_OK; ++i) {
+            s.tuning.few_tiles = cand[i];
+            float ms = 0.0f;
+            st = time_ms([&]() { launch_vcycle(s, 0); }, ms);
+            if (st == APIC_OK && (i == 0 || ms < best_ms)) { best_ms = ms; best.few_tiles = cand[i]; }
+        }
+        best.vcycle_ms = best_ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s.profiling = prof;
+    s.tuning = best;
+    if (st != APIC_OK) return st;
+    if (out) *out = best;
+    return APIC_OK;
+}
+
 apic_status apic_apply_preconditioner(apic_handle* h) {
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
     launch_pcg_reset(h->sim);

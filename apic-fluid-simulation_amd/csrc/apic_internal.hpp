@@ -278,6 +278,7 @@ struct Sim {
         hipEvent_t a, b;
     };
     int profiling = 0;  // 0 off, 1 the transfer kernels only, 2 every timer
+    apic_tuning tuning{11u, 1024u, 0.0f, 0.0f};  // launch shapes (apic_set_tuning, apic_autotune)
     std::vector<hipEvent_t> ev_pool;
     std::vector<TimerMark> ev_pending;
     size_t ev_first = 0;  // ev_pending[ev_first..] are not collected yet

@@ -3027,10 +3027,8 @@ int pcg_cap(const Sim& sim) { return sim.solver.mode == 1 ? sim.solver.max_itera
 
 namespace {
 
-// 16 x 16 tiles on a level above the base with at most kFewTiles 32 x 32
-// ones (the base keeps one partial of r.z per 32 x 32 tile)
-constexpr uint32_t kFewTiles = 1024;
-
+// 16 x 16 tiles on a level above the base with at most tuning.few_tiles
+// 32 x 32 ones (the base keeps one partial of r.z per 32 x 32 tile)
 uint32_t tiles_of(const LevelView& L, uint32_t side, uint32_t& tx) {
     tx = (L.w + side - 1) / side;
     return tx * ((L.h + side - 1) / side);
@@ -3038,7 +3036,7 @@ uint32_t tiles_of(const LevelView& L, uint32_t side, uint32_t& tx) {
 
 bool few_tiles(const Sim& sim, int l) {
     uint32_t tx;
-    return l > 0 && tiles_of(sim.levels[l], kBaseTile, tx) <= kFewTiles;
+    return l > 0 && tiles_of(sim.levels[l], kBaseTile, tx) <= sim.tuning.few_tiles;
 }
 
 // Two levels of few tiles above the bottom (not the base), l and l + 1, in

@@ -414,10 +414,12 @@ void launch_keygen(Sim& sim) {
 void launch_radix_sort(Sim& sim) {
     uint32_t n = sim.num_particles;
     if (n <= 1) return;
-    // Passes of at most kRadixMaxBits bits; the first (least significant)
-    // pass takes the remainder, so the later passes get the widest digits.
+    // Passes of at most dmax <= kRadixMaxBits bits (the tuning's); the first
+    // (least significant) pass takes the remainder, so the later passes get
+    // the widest digits.
+    const int dmax = (int)std::min<uint32_t>(std::max<uint32_t>(sim.tuning.radix_bits, 1u), (uint32_t)kRadixMaxBits);
     const int total_bits = (int)key_bits(sim);
-    const int passes = (total_bits + kRadixMaxBits - 1) / kRadixMaxBits;
+    const int passes = (total_bits + dmax - 1) / dmax;
     const uint32_t ntiles = (n + kRadixTile - 1) / kRadixTile;
     const uint32_t nchunks = (ntiles + kScanChunkTiles - 1) / kScanChunkTiles;
     uint32_t* counts = sim.radix_scratch;                               // [tile][digit]
@@ -425,7 +427,7 @@ void launch_radix_sort(Sim& sim) {
     uint32_t* digit_start = chunk_sum + (size_t)kRadixMaxBuckets * nchunks;
     int src = 0, shift = 0;
     for (int p = 0; p < passes; ++p) {
-        const int bits = p == 0 ? total_bits - (passes - 1) * kRadixMaxBits : kRadixMaxBits;
+        const int bits = p == 0 ? total_bits - (passes - 1) * dmax : dmax;
         const uint32_t nb = 1u << bits;
         emit<RadixUpsweep>(sim, {sim.keys[src], n, shift, bits, counts}, ntiles);
         emit<ScanColumns>(sim, {counts, ntiles, nb, nchunks, chunk_sum}, nchunks * ((nb + 255) / 256));

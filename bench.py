@@ -235,6 +235,11 @@ def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_s
     dt = cfg["dt"]
     for _ in range(warmup):
         g.update(opts, dt)
+    # launch shapes for this grid (radix digit width, multigrid tile side and
+    # level pairing), timed on the warmed-up state; they change no result
+    tuning = g.autotune(reps=5)
+    print(f"[bench] autotuned: radix_bits {tuning['radix_bits']}, few_tiles {tuning['few_tiles']} "
+          f"(sort {tuning['sort_ms']:.3f} ms, V-cycle {tuning['vcycle_ms']:.3f} ms)", file=sys.stderr, flush=True)
     g.set_profiling(1)  # HIP events around the timed kernels only
     g.get_max_particle_velocity()  # synchronises the library's stream
     substeps, iters, fluid, rel, reasons = 0, [], [], [], []
@@ -263,7 +268,7 @@ def timed_run(Apic, DebugOptions, props, cfg, xy, solver, steps, warmup, stage_s
     g.close()
     return dict(elapsed=elapsed, substeps=substeps, iters=iters, fluid=fluid, rel=rel, reasons=reasons,
                 timers=timers, stage_timers=stage_timers, stage_substeps=stage_substeps, n_alloc=n_alloc,
-                true_residual=check, levels=levels)
+                true_residual=check, levels=levels, tuning=tuning)
 
 
 def batched_run(count=256, side_min=64, side_max=512, seed=4, steps=10, warmup=3, seq_rounds=2, dt=0.002):
@@ -496,7 +501,7 @@ def large_run(steps=50, warmup=3, stage_steps=5, oracle_steps=2):
            "pcg_exit_reasons": {str(k): int(sum(1 for x in r["reasons"] if x == k)) for k in sorted(set(r["reasons"]))},
            "pcg_true_rel_residual_last": r["true_residual"],
            "valid": all(x == 3 for x in r["reasons"]) and tr64 is not None and tr64 <= 1e-6 * (1 + 1e-6),
-           "stages": rates}
+           "stages": rates, "autotune": r["tuning"]}
     if oracle_steps > 0:
         out["fp64_oracle_error"] = fp64_oracle_error(c, xy, props, oracle_steps)
     return out
@@ -709,6 +714,7 @@ def main():
         "pcg_true_rel_residual_last": r["true_residual"],
         "valid": valid,
         "substeps": r["substeps"],
+        "autotune": r["tuning"],
         "other_solver": {"solver": SOLVER_NAMES[other], "steps": o_steps,
                          "value": n_p * ro["substeps"] / ro["elapsed"],
                          "ms_per_step": ro["elapsed"] / o_steps * 1e3,

@@ -205,6 +205,34 @@ apic_status apic_set_profiling(apic_handle* h, int enable);
 /* ms[i] and counts[i] for i < n (n <= APIC_TIMER_COUNT); either may be null. */
 apic_status apic_get_timers(apic_handle* h, double* ms, uint32_t* counts, uint32_t n);
 
+/* Launch shapes that do not change a bit of the result (extension; the
+ * reference has no counterpart: its loops run on host threads).
+ *   radix_bits   widest digit of the Morton radix sort, 8..11: a key of b
+ *                bits takes ceil(b / radix_bits) passes (the sort is
+ *                stable, so every width gives the same order);
+ *   few_tiles    a multigrid level above the base with at most this many
+ *                32 x 32 tiles runs as 16 x 16 tiles, and two such levels
+ *                above the bottom share one launch each way (the per-cell
+ *                arithmetic is the same for every tile side).
+ * sort_ms and vcycle_ms are the autotuner's timings of the chosen shapes
+ * (0 when set by hand).  Defaults: radix_bits 11, few_tiles 1024. */
+typedef struct apic_tuning {
+    uint32_t radix_bits;
+    uint32_t few_tiles;
+    float sort_ms;
+    float vcycle_ms;
+} apic_tuning;
+apic_status apic_get_tuning(apic_handle* h, apic_tuning* out);
+apic_status apic_set_tuning(apic_handle* h, const apic_tuning* t);
+/* Times the radix sort of the current particles' keys for radix_bits 8..11
+ * and apply_preconditioner on the current pressure system for few_tiles in
+ * {0, 64, 256, 1024, 4096, 16384}, reps times each after one untimed run,
+ * keeps the fastest of each and writes them (with their timings) to out if
+ * not null.  The particle state is left as it was; the preconditioner's
+ * error vectors are overwritten.  Call it after a few updates, when the
+ * particles and the pressure system are representative of the run. */
+apic_status apic_autotune(apic_handle* h, uint32_t reps, apic_tuning* out);
+
 /* Horizontal gravity (extension; the reference applies gravity along y only,
  * src/apic.rs:1604-1626).  gx is added, times dt, to every u face with a fluid
  * cell on either side, as WorldProperties.gravity is to v faces; 0 (the
