@@ -26,7 +26,7 @@ STATUS = {0: "APIC_OK", 1: "APIC_ERR_INVALID_ARGUMENT", 2: "APIC_ERR_UNSUPPORTED
 SYMBOLS = [
     "apic_create", "apic_create_with_particles", "apic_destroy", "apic_last_error", "apic_update",
     "apic_add_radial_force", "apic_get_max_particle_velocity", "apic_is_fluid",
-    "apic_set_solver_config", "apic_get_stats", "apic_get_dims", "apic_get_timestamp",
+    "apic_set_solver_config", "apic_get_stats", "apic_get_dims", "apic_get_level_counts", "apic_get_timestamp",
     "apic_get_tuning", "apic_set_tuning", "apic_autotune",
     "apic_set_timestamp", "apic_field_length", "apic_read_field", "apic_write_field",
     "apic_read_particles_aosoa", "apic_write_particles_aosoa",
@@ -101,6 +101,7 @@ def lib():
     L.apic_autotune.argtypes = [vp, u32, ctypes.POINTER(TuningC)]
     L.apic_get_stats.argtypes = [vp, ctypes.POINTER(StepStatsC)]
     L.apic_get_dims.argtypes = [vp, u32p, u32p, u32p, u32p, u32p]
+    L.apic_get_level_counts.argtypes = [vp, u32p, u32p]
     L.apic_get_timestamp.argtypes = [vp, u32p, ctypes.POINTER(f32)]
     L.apic_set_timestamp.argtypes = [vp, u32, f32]
     L.apic_field_length.argtypes = [vp, ctypes.c_int, u32, ctypes.POINTER(ctypes.c_size_t)]

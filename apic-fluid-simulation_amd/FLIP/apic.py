@@ -145,9 +145,13 @@ class Apic:
 
     # ---- pub fields --------------------------------------------------------
     def dims(self):
-        v = [ctypes.c_uint32() for _ in range(5)]
-        check(lib().apic_get_dims(self._h, *[ctypes.byref(x) for x in v]))
-        return dict(zip(("width", "height", "num_particles", "num_chunks", "num_levels"), [x.value for x in v]))
+        v = [ctypes.c_uint32() for _ in range(7)]
+        check(lib().apic_get_dims(self._h, *[ctypes.byref(x) for x in v[:5]]))
+        check(lib().apic_get_level_counts(self._h, None, ctypes.byref(v[6])))
+        # num_levels: the reference's multigrid levels; num_levels_relative:
+        # with the relative mode's deeper ones
+        return dict(zip(("width", "height", "num_particles", "num_chunks", "num_levels", "num_levels_relative"),
+                        [x.value for x in v[:5]] + [v[6].value]))
 
     @property
     def width(self):

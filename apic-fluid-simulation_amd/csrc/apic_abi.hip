@@ -168,6 +168,14 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
             return fail(APIC_ERR_UNSUPPORTED_SIZE,
                         "no multigrid level: both grid sides must be even and > 4 (the reference panics here)");
     }
+    // the relative mode's levels below: while the coarsest has more than 64
+    // cells and both sides exceed 2, sides halved and rounded up (a side of
+    // 2 x odd stops the reference's halving at up to 511^2 cells)
+    const size_t ref_n = dims.size();
+    if (ref_n > 1)
+        while ((size_t)dims.back().first * dims.back().second > 64 && dims.back().first > 2 &&
+               dims.back().second > 2)
+            dims.push_back({(dims.back().first + 1) / 2, (dims.back().second + 1) / 2});
 
     if (shared) {
         s.stream = shared;
@@ -197,26 +205,38 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
         s.levels.push_back(L);
     }
     if (s.levels.size() > 16) return fail(APIC_ERR_UNSUPPORTED_SIZE, "more than 15 multigrid levels");
-    // The one-workgroup bottom solver takes the coarse levels of at most
-    // kBottomFirstCells cells that fit in LDS (a finer level runs faster as
-    // tiles over several CUs); when even the coarsest does not fit, it works
-    // in global memory on every level with <= 4096 cells.
-    s.bottom_level = s.levels.size() > 1 ? s.levels.size() - 1 : 0;
-    s.bottom_in_lds = false;
-    for (size_t l = 1; l < s.levels.size(); ++l)
-        if (s.levels[l].cells <= kBottomFirstCells && bottom_fits_lds(s.levels, l)) {
-            s.bottom_level = l;
-            s.bottom_in_lds = true;
-            break;
-        }
-    if (!s.bottom_in_lds)
-        for (size_t l = 1; l < s.levels.size(); ++l)
-            if (s.levels[l].cells <= 4096) { s.bottom_level = l; break; }
-    // The arrays of an LDS bottom's levels lie in two blocks laid out as the
+    // Each mode's bottom: the one-workgroup bottom solver takes the coarse
+    // levels of at most kBottomFirstCells cells that fit in LDS (a finer level
+    // runs faster as tiles over several CUs); when even the coarsest does not
+    // fit, it works in global memory on every level with <= 4096 cells.
+    auto plan = [&](size_t count) {
+        Sim::Hierarchy hz;
+        hz.count = count;
+        hz.bottom = count > 1 ? count - 1 : 0;
+        for (size_t l = 1; l < count; ++l)
+            if (s.levels[l].cells <= kBottomFirstCells && bottom_fits_lds(s.levels, l, count)) {
+                hz.bottom = l;
+                hz.in_lds = true;
+                break;
+            }
+        if (!hz.in_lds)
+            for (size_t l = 1; l < count; ++l)
+                if (s.levels[l].cells <= 4096) { hz.bottom = l; break; }
+        return hz;
+    };
+    s.ref_levels = ref_n;
+    s.hier[0] = plan(ref_n);
+    s.hier[1] = plan(s.levels.size());
+    s.select_hierarchy();
+    // The arrays of the LDS bottoms' levels lie in two blocks laid out as the
     // bottom's LDS image (per level: diag, inv_diag, plus_x, plus_y, r, e over
     // the padded range; the flag bytes apart), so that it copies them in and
-    // out in one stride; every other level has its own arrays.
-    const size_t lds_first = s.bottom_in_lds ? s.bottom_level : s.levels.size();
+    // out in one stride; every other level has its own arrays.  One image
+    // from the shallower of the two modes' LDS bottoms to the last level
+    // holds both (a mode's bottom levels are a contiguous run of it).
+    size_t lds_first = s.levels.size();
+    for (const auto& hz : s.hier)
+        if (hz.in_lds) lds_first = std::min(lds_first, hz.bottom);
     for (size_t l = 0; l < lds_first; ++l) {
         LevelView& L = s.levels[l];
         const size_t pa = s.padded(L.w, L.h);
@@ -246,14 +266,17 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
             }
         }
     }
-    s.down_e.assign(s.bottom_level, nullptr);
-    for (size_t l = 0; l < s.bottom_level; ++l) ALLOC(s.down_e[l], s.padded(s.levels[l].w, s.levels[l].h));
+    const size_t ndown = std::max(s.hier[0].bottom, s.hier[1].bottom);
+    s.down_e.assign(ndown, nullptr);
+    for (size_t l = 0; l < ndown; ++l) ALLOC(s.down_e[l], s.padded(s.levels[l].w, s.levels[l].h));
     const LevelView& B = s.levels[0];
     size_t pb = s.padded(B.w, B.h);
     ALLOC(s.search[0], pb); ALLOC(s.search[1], pb); ALLOC(s.As, pb); ALLOC(s.rhs, pb); ALLOC(s.r_alt, pb);
     ALLOC(s.As_lo, pb); ALLOC(s.r_lo, pb);
-    const LevelView& C = s.levels.back();
-    size_t pcs = s.padded(C.w, C.h);
+    // the coarse CG vectors of either mode's coarsest level
+    size_t pcs = 8;
+    for (const auto& hz : s.hier)
+        if (hz.count > 0) pcs = std::max(pcs, s.padded(s.levels[hz.count - 1].w, s.levels[hz.count - 1].h));
     ALLOC(s.cg_s, pcs); ALLOC(s.cg_ap, pcs); ALLOC(s.cg_r, pcs);
     ALLOC(s.part_cg, 3 * kWideCoarseMaxWgs); ALLOC(s.cg_bar, 1);
     ALLOC(s.pcg, 1);
@@ -1095,6 +1118,7 @@ apic_status apic_set_solver_config(apic_handle* h, const apic_solver_config* cfg
     }
     HIP_TRY(hipStreamSynchronize(h->sim.stream));
     h->sim.solver = *cfg;
+    h->sim.select_hierarchy();
     return APIC_OK;
 }
 
@@ -1134,7 +1158,15 @@ apic_status apic_get_dims(apic_handle* h, uint32_t* width, uint32_t* height, uin
     if (height) *height = s.H;
     if (num_particles) *num_particles = s.num_particles;
     if (num_chunks) *num_chunks = s.num_chunks;
-    if (num_levels) *num_levels = (uint32_t)(s.levels.size() - 1);
+    if (num_levels) *num_levels = (uint32_t)(s.ref_levels > 0 ? s.ref_levels - 1 : 0);
+    return APIC_OK;
+}
+
+apic_status apic_get_level_counts(apic_handle* h, uint32_t* reference, uint32_t* relative) {
+    if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
+    const Sim& s = h->sim;
+    if (reference) *reference = (uint32_t)(s.ref_levels > 0 ? s.ref_levels - 1 : 0);
+    if (relative) *relative = (uint32_t)(s.levels.size() - 1);
     return APIC_OK;
 }
 

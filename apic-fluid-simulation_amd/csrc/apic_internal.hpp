@@ -65,13 +65,14 @@ inline uint32_t wide_coarse_wgs(const LevelView& L) {
     return g <= kWideCoarseMaxWgs ? g : 0;
 }
 
-inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first) {
+// Whether levels [first, count) fit the one-workgroup bottom's LDS.
+inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first, size_t count) {
     size_t fl = 0, fb = 0;
-    for (size_t l = first; l < lv.size(); ++l) {
+    for (size_t l = first; l < count; ++l) {
         fl += 6 * bottom_padded(lv[l]);
         fb += bottom_flags(lv[l]);
     }
-    fl += 3 * bottom_padded(lv.back()) + lv.back().cells;
+    fl += 3 * bottom_padded(lv[count - 1]) + lv[count - 1].cells;
     return fl <= kBottomLdsFloats && fb <= kBottomLdsFlags;
 }
 
@@ -216,7 +217,19 @@ struct Sim {
     bool forces_active = false;
 
     // solver
-    std::vector<LevelView> levels;  // [0] = base grid, [k] = multigrid_levels[k-1]
+    // [0] = base grid, [k] = multigrid_levels[k-1]; after the reference's
+    // levels, the relative mode's deeper ones (odd sides halved, rounded up)
+    std::vector<LevelView> levels;
+    size_t ref_levels = 0;          // levels of the reference's halving, the base included
+    // Each solver mode's V-cycle: the levels it runs through (the reference's
+    // rules: the reference's levels; the relative mode: all) and where its
+    // bottom solver starts.  select_hierarchy() copies the current mode's
+    // into level_count, bottom_level and bottom_in_lds.
+    struct Hierarchy {
+        size_t count = 0, bottom = 0;
+        bool in_lds = false;
+    } hier[2];
+    size_t level_count = 0;         // levels the current mode's V-cycle runs through
     size_t bottom_level = 0;        // first level solved by the one-workgroup kernel
     // the V-cycle's down-sweep errors of the levels above the bottom (the
     // up-sweep reads them and writes the level's e, so that no launch reads
@@ -352,6 +365,12 @@ struct Sim {
 
     size_t nu() const { return (size_t)cells + H; }
     size_t nv() const { return (size_t)cells + W; }
+    void select_hierarchy() {
+        const Hierarchy& hz = hier[solver.mode == 1 ? 1 : 0];
+        level_count = hz.count;
+        bottom_level = hz.bottom;
+        bottom_in_lds = hz.in_lds;
+    }
     size_t padded(uint32_t w, uint32_t h) const { return (((size_t)w * h + 2 * (size_t)w) + 7) / 8 * 8; }
     float* pfield(int f, int which = -1) const {
         return pbuf[which < 0 ? cur : which] + (size_t)f * nalloc;

@@ -362,6 +362,7 @@ __device__ inline void restrict_cell(const LevelView& F, const LevelView& C, uin
 #pragma unroll
         for (int dx = 0; dx < 2; ++dx) {
             uint32_t fx = cx * 2 + dx, fy = cy * 2 + dy;
+            if (fx >= F.w || fy >= F.h) continue;  // past the edge of an odd-sided level
             uint32_t rf = fy * F.w + fx;
             size_t pf = (size_t)rf + fw;
             if (F.fluid[rf]) {
@@ -1206,7 +1207,10 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
         for (int dy = 0; dy < 2; ++dy)
 #pragma unroll
             for (int dx = 0; dx < 2; ++dx) {
-                const int rf = (cy * 2 + dy) * F.w + cx * 2 + dx, pf = rf + F.w;
+                // a child past the edge of an odd-sided level adds nothing;
+                // its loads go to the first child (always inside the level)
+                const bool in = cx * 2 + dx < F.w && cy * 2 + dy < F.h;
+                const int rf = in ? (cy * 2 + dy) * F.w + cx * 2 + dx : cy * 2 * F.w + cx * 2, pf = rf + F.w;
             // every child's operands load unconditionally (they go out
             // together); a non-fluid child adds +0, which leaves the sum as
             // skipping it would: the sum starts at +0 and never becomes -0
@@ -1216,7 +1220,7 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
                 ax += f[F.py + pf] * f[F.e + pf + F.w];
                 ax += f[F.px + pf - 1] * f[F.e + pf - 1];
                 ax += f[F.py + pf - F.w] * f[F.e + pf - F.w];
-                sum += fl[F.fl + rf] ? b - ax : 0.0f;
+                sum += (in && fl[F.fl + rf]) ? b - ax : 0.0f;
             }
         const float id = f[C.id + pcc];
         const bool red = ((cx + cy) & 1) == 0;
@@ -2975,14 +2979,14 @@ unsigned blocks_for(size_t n, unsigned threads = kThreads) { return (unsigned)((
 
 LevelSet level_set(const Sim& sim) {
     LevelSet ls{};
-    ls.count = (int)sim.levels.size();
+    ls.count = (int)sim.level_count;
     for (int l = 0; l < ls.count && l < 16; ++l) ls.lv[l] = sim.levels[l];
     return ls;
 }
 
 Prolong base_prolong(const Sim& sim) {
     Prolong P{};
-    P.has = sim.levels.size() > 1;
+    P.has = sim.level_count > 1;
     if (P.has) P.C = sim.levels[1];
     P.fs = sim.fstats;
     return P;
@@ -3010,7 +3014,7 @@ void launch_fluid_stats(Sim& sim) {
 }
 
 void launch_mg_build(Sim& sim) {
-    const int n = (int)sim.levels.size();
+    const int n = (int)sim.level_count;
     int l = 1;
     for (; l < n && l < (int)sim.bottom_level; ++l)
         emit<MgBuild>(sim, {sim.levels[l - 1], sim.levels[l]}, blocks_for(sim.levels[l].cells));
@@ -3051,7 +3055,7 @@ bool level_pair(const Sim& sim, int l) {
 // Workgroups of the relative mode's multi-workgroup coarse CG (0: the
 // one-workgroup bottom solves the coarsest level).
 uint32_t wide_coarse(const Sim& sim) {
-    const size_t last = sim.levels.size() - 1;
+    const size_t last = sim.level_count - 1;
     if (sim.solver.mode != 1 || sim.bottom_in_lds || last == 0 || sim.bottom_level != last) return 0;
     return wide_coarse_wgs(sim.levels[last]);
 }
@@ -3101,7 +3105,7 @@ void launch_zero_pressure(Sim& sim) {
 // (residual test in the first kernel, gamma' partials in the last).
 void launch_vcycle(Sim& sim, int mode, int k) {
     const int32_t* done = &sim.pcg->done;
-    const int last = (int)sim.levels.size() - 1;
+    const int last = (int)sim.level_count - 1;
     const int bottom = (int)sim.bottom_level;  // 1 <= bottom <= last
     const bool rel = sim.solver.mode == 1 && mode != 0;
     auto tiles = tiles_of;

@@ -294,6 +294,14 @@ apic_status apic_device_memory(size_t* free_bytes, size_t* total_bytes);
 apic_status apic_get_dims(apic_handle* h, uint32_t* width, uint32_t* height,
                           uint32_t* num_particles, uint32_t* num_chunks,
                           uint32_t* num_levels);
+/* Multigrid levels (excluding the base grid) of each solver mode's V-cycle:
+ * `reference` is the reference's halving while both sides are even and > 4
+ * (src/apic.rs:900-908, the num_levels above); `relative` adds the relative
+ * mode's levels below it, each side halved and rounded up while the coarsest
+ * has more than 64 cells and both sides exceed 2 (a side of 2 x odd stops the
+ * reference's halving early).  Their fields are read and written by level
+ * index like the others.  No counterpart in the reference. */
+apic_status apic_get_level_counts(apic_handle* h, uint32_t* reference, uint32_t* relative);
 apic_status apic_get_timestamp(apic_handle* h, uint32_t* timestamp, float* prev_deltatime);
 apic_status apic_set_timestamp(apic_handle* h, uint32_t timestamp, float prev_deltatime);
 

@@ -171,7 +171,11 @@ template <class R> struct Level {
         else { smooth_list(red, omega); smooth_list(black, omega); }
     }
 
-    // restrict, src/apic.rs:537-594
+    // restrict, src/apic.rs:537-594.  A child past the edge of an odd-sided
+    // level (the relative mode's levels below the reference's last, whose
+    // sides are the finer ones' halves rounded up) is skipped, as
+    // update_multilevel_grid skips it (:1566); the reference's levels have
+    // even sides and never meet one.
     void restrict_to(Level& coarse) const {
         std::fill(coarse.e.begin() + coarse.w, coarse.e.begin() + coarse.w + coarse.cells, R(0));
         const long n = (long)coarse.fluid_idx.size();
@@ -184,6 +188,7 @@ template <class R> struct Level {
             for (int dy = 0; dy < 2; ++dy)
                 for (int dx = 0; dx < 2; ++dx) {
                     size_t fx = cx * 2 + dx, fy = cy * 2 + dy;
+                    if (fx >= w || fy >= h) continue;
                     size_t raw_f = fy * w + fx, pf = raw_f + w;
                     if (is_fluid(raw_f)) {
                         R b = r[pf];
@@ -323,7 +328,10 @@ template <class R> struct Sim {
     std::vector<uint32_t> skey, sval;
     uint32_t fluid_cells = 0;
     Level<R> base;
-    std::vector<Level<R>> levels;
+    std::vector<Level<R>> levels;  // the reference's levels, then the relative mode's deeper ones
+    size_t ref_levels = 0;         // how many of them the reference's halving makes
+    // the levels the current solver mode builds and cycles through
+    size_t active_levels() const { return solver_mode == 1 ? levels.size() : ref_levels; }
     uint32_t timestamp = 0;
     R prev_dt = R(1.0f / 60.0f);
 
@@ -419,6 +427,17 @@ template <class R> struct Sim {
         }
         if (levels.empty()) return APIC_ERR_UNSUPPORTED_SIZE;  // :910 panics
         levels.back().allocate_cg();
+        // The relative mode's hierarchy goes on below the reference's last
+        // level while it has more than 64 cells and both sides exceed 2, each
+        // side halved and rounded up (a grid side of 2 x odd stops the
+        // reference's halving at up to 511^2 cells)
+        ref_levels = levels.size();
+        while ((size_t)cw * ch > 64 && cw > 2 && ch > 2) {
+            cw = (cw + 1) / 2; ch = (ch + 1) / 2;
+            levels.emplace_back();
+            levels.back().init(cw, ch);
+        }
+        if (levels.size() > ref_levels) levels.back().allocate_cg();
         update_world();
         return 0;
     }
@@ -627,7 +646,7 @@ template <class R> struct Sim {
     // update_multilevel_grid, src/apic.rs:1499-1602
     void update_multilevel_grid() {
         uint32_t width = W;
-        for (size_t gi = 0; gi < levels.size(); ++gi) {
+        for (size_t gi = 0; gi < active_levels(); ++gi) {
             const Level<R>& fine = gi == 0 ? base : levels[gi - 1];
             Level<R>& coarse = levels[gi];
             coarse.fluid_idx.clear(); coarse.red.clear(); coarse.black.clear();
@@ -764,7 +783,8 @@ template <class R> struct Sim {
         base.smooth(false, omega);
         if (!levels.empty()) {
             base.restrict_to(levels[0]);
-            size_t last = levels.size() - 1;
+            // the reference's levels, or the relative mode's deeper hierarchy
+            size_t last = active_levels() - 1;
             for (size_t l = 0; l < last; ++l) {
                 levels[l].smooth(false, omega);
                 levels[l].restrict_to(levels[l + 1]);
@@ -1420,9 +1440,18 @@ template <class R> int stage_t(Sim<R>* s, int stage, float dt, uint32_t ts) {
         case 14: s->base.smooth(false, 1.2f); break;
         case 15: s->base.smooth(true, 1.2f); break;
         // the coarsest level's CG alone (solve_coarse_exact), for the dense-solve pins
-        case 16: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels.back().solve_coarse_exact(); break;
+        case 16: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels[s->ref_levels - 1].solve_coarse_exact(); break;
         // the relative mode's coarse CG
-        case 17: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels.back().solve_coarse_wide(); break;
+        case 17: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels[s->ref_levels - 1].solve_coarse_wide(); break;
+        // restrict / prolongate between level `timestamp` and the next, for the
+        // pins of the odd-sided levels
+        case 18: case 19: {
+            auto* f = s->level_at(ts);
+            auto* c = s->level_at(ts + 1);
+            if (!f || !c) return APIC_ERR_INVALID_ARGUMENT;
+            if (stage == 18) f->restrict_to(*c); else f->prolongate_from(*c);
+            break;
+        }
         default: return APIC_ERR_INVALID_ARGUMENT;
     }
     return 0;
@@ -1480,8 +1509,15 @@ int orc_write_field(void* hv, int f, uint32_t l, const void* src) {
 
 void orc_get_dims(void* hv, uint32_t* w, uint32_t* hh, uint32_t* np, uint32_t* nc, uint32_t* nl, uint32_t* fc) {
     Handle* h = static_cast<Handle*>(hv);
-    if (h->precision == 64) { auto* s = S<double>(h); *w = s->W; *hh = s->H; *np = s->num_particles; *nc = s->num_chunks; *nl = (uint32_t)s->levels.size(); *fc = s->fluid_cells; }
-    else { auto* s = S<float>(h); *w = s->W; *hh = s->H; *np = s->num_particles; *nc = s->num_chunks; *nl = (uint32_t)s->levels.size(); *fc = s->fluid_cells; }
+    if (h->precision == 64) { auto* s = S<double>(h); *w = s->W; *hh = s->H; *np = s->num_particles; *nc = s->num_chunks; *nl = (uint32_t)s->ref_levels; *fc = s->fluid_cells; }
+    else { auto* s = S<float>(h); *w = s->W; *hh = s->H; *np = s->num_particles; *nc = s->num_chunks; *nl = (uint32_t)s->ref_levels; *fc = s->fluid_cells; }
+}
+
+// levels of the reference's halving and with the relative mode's deeper ones
+void orc_get_level_counts(void* hv, uint32_t* ref, uint32_t* all) {
+    Handle* h = static_cast<Handle*>(hv);
+    if (h->precision == 64) { *ref = (uint32_t)S<double>(h)->ref_levels; *all = (uint32_t)S<double>(h)->levels.size(); }
+    else { *ref = (uint32_t)S<float>(h)->ref_levels; *all = (uint32_t)S<float>(h)->levels.size(); }
 }
 
 void orc_get_timestamp(void* hv, uint32_t* ts, double* prev_dt) {

@@ -34,6 +34,7 @@ STAGES = {
     "prolongate_base": 12, "restrict_base": 13, "smooth_base_forward": 14, "smooth_base_reverse": 15,
     "solve_coarse_exact": 16,
     "solve_coarse_relative": 17,
+    "restrict_level": 18, "prolongate_level": 19,   # level index in `timestamp`
 }
 
 
@@ -71,6 +72,7 @@ def lib():
         L.orc_write_field.argtypes = [vp, ctypes.c_int, ctypes.c_uint32, vp]
         u32p = ctypes.POINTER(ctypes.c_uint32)
         L.orc_get_dims.argtypes = [vp, u32p, u32p, u32p, u32p, u32p, u32p]
+        L.orc_get_level_counts.argtypes = [vp, u32p, u32p]
         L.orc_get_timestamp.argtypes = [vp, u32p, ctypes.POINTER(ctypes.c_double)]
         L.orc_set_timestamp.argtypes = [vp, ctypes.c_uint32, ctypes.c_double]
         L.orc_get_stats.argtypes = [vp, ctypes.POINTER(StepStats)]
@@ -126,7 +128,11 @@ class Oracle:
         v = [ctypes.c_uint32() for _ in range(6)]
         lib().orc_get_dims(self._h, *[ctypes.byref(x) for x in v])
         keys = ("width", "height", "num_particles", "num_chunks", "num_levels", "fluid_cells")
-        return dict(zip(keys, [x.value for x in v]))
+        out = dict(zip(keys, [x.value for x in v]))
+        ref, rel = ctypes.c_uint32(), ctypes.c_uint32()
+        lib().orc_get_level_counts(self._h, ctypes.byref(ref), ctypes.byref(rel))
+        out["num_levels_relative"] = rel.value  # with the relative mode's deeper levels
+        return out
 
     def update(self, frame_dt, disable_sort=False):
         lib().orc_update(self._h, int(disable_sort), frame_dt)

@@ -69,7 +69,7 @@ def warm():
     state["timestamp"] = o.timestamp()
     for st in PRE_SOLVE:
         o.stage(st, DT)
-    nl = o.dims()["num_levels"]
+    nl = o.dims()["num_levels_relative"]  # the relative mode's levels
     levels = [{f: o.read(f, l) for f in LEVEL_FIELDS} for l in range(nl + 1)]
     b = o.read("RESIDUAL")
     p0 = o.read("PRESSURE")
@@ -80,7 +80,7 @@ def warm():
 def _gpu_with_system(warm):
     from FLIP.apic import Apic
     g = Apic.with_particles(W, H, DX, props(), warm["xy"])
-    assert g.dims()["num_levels"] == warm["nl"]
+    assert g.dims()["num_levels_relative"] == warm["nl"]
     for lvl, fields in enumerate(warm["levels"]):
         for f in LEVEL_FIELDS:
             g.write(f, fields[f], lvl)

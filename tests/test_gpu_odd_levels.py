@@ -1,23 +1,22 @@
 """The relative mode on grids whose side is 2 x odd: the reference's hierarchy
-stops halving at an odd side (src/apic.rs:900-911), so the coarsest level has
-up to (side / 2)^2 cells, and the relative mode solves it with the
-multi-workgroup coarse CG (FP64 dot products, exits relative to the level's
-right-hand side).  Sides 126 (63^2 coarsest cells, one band of 4096 cells),
-254 (127^2, four bands), 510 (255^2, sixteen bands), 254 x 126 (127 x 63, a
-short last band) and 2046 x 10 (1023 x 5: the last band shorter than a row).
+stops halving at an odd side (src/apic.rs:900-911), leaving a coarsest level
+of up to (side / 2)^2 cells; the relative mode's V-cycle goes on below it,
+each side halved and rounded up, until the coarsest has at most 64 cells or
+a side of 2.  Sides 126 (63^2 -> 32^2 ... 8^2), 254 (127^2 -> 64^2 ... 8^2),
+510 (255^2 -> 128^2 ... 8^2), 254 x 126 (127 x 63 -> 64 x 32 ... 8 x 4) and
+2046 x 10 (1023 x 5 -> 512 x 3 -> 256 x 2).
 
   - apply_preconditioner on the oracle's assembled system: e on every level
-    bit-exact against the oracle (its FP64 sums run in ascending order, the
-    GPU's per band; they round to the same FP32 step lengths unless an FP64
-    rounding tips one, about one chance in 10^5 per solve);
+    bit-exact against the oracle (its coarse CG's FP64 sums run in ascending
+    order, the GPU's in a tree; they round to the same FP32 step lengths
+    unless an FP64 rounding tips one, about one chance in 10^5 per solve);
   - one relative solve (1e-6): exit by the tolerance within one iteration of
     the oracle, pressure within 1e-5 relative L2, the true residual of the
     FP64 pressure within the tolerance;
   - one update: particles, grid velocities and pressure within the mixed
     workload's bars (1e-5 positions and pressure);
   - a batch of such simulations beside others: every field and counter equal
-    to stepping each alone, bit for bit (the batched launch of this kernel
-    holds whole simulations of at most 256 workgroups).
+    to stepping each alone, bit for bit.
 """
 import numpy as np
 import pytest
@@ -32,8 +31,8 @@ PRE_SOLVE = ["update_spatial_lookup", "transfer_particles_to_grid", "apply_exter
              "build_pressure_system", "update_multilevel_grid"]
 LEVEL_FIELDS = ["FLUID", "DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y"]
 DT = 0.002
-# (width, height, bands of the coarsest level's CG)
-GRIDS = [(126, 126, 1), (254, 254, 4), (510, 510, 16), (254, 126, 2), (2046, 10, 2)]
+# (width, height, the relative mode's coarsest level)
+GRIDS = [(126, 126, (8, 8)), (254, 254, (8, 8)), (510, 510, (8, 8)), (254, 126, (8, 4)), (2046, 10, (256, 2))]
 TOL = {"POS_X": 1e-5, "POS_Y": 1e-5, "PRESSURE": 1e-5, "TIME_RESIDUAL": 2e-4, "VEL_X": 2e-5, "VEL_Y": 2e-5,
        "MAC_U": 2e-5, "MAC_V": 2e-5, "C_U_X": 2e-4, "C_U_Y": 2e-4, "C_V_X": 2e-4, "C_V_Y": 2e-4}
 
@@ -65,9 +64,10 @@ def _system(w, h):
     o.update(DT)
     for st in PRE_SOLVE:
         o.stage(st, DT)
-    nl = o.dims()["num_levels"]
+    d = o.dims()
+    nl = d["num_levels_relative"]
     g = _gpu(w, h, xy)
-    assert g.dims()["num_levels"] == nl
+    assert g.dims()["num_levels"] == d["num_levels"] and g.dims()["num_levels_relative"] == nl
     for lvl in range(nl + 1):
         for f in LEVEL_FIELDS:
             g.write(f, o.read(f, lvl), lvl)
@@ -77,18 +77,29 @@ def _system(w, h):
     return o, g, nl, b, p0
 
 
+def _sides(w, h, nl):
+    """Level sides: halved, rounded up (exact halves on the reference's levels)."""
+    out = [(w, h)]
+    for _ in range(nl):
+        out.append(((out[-1][0] + 1) // 2, (out[-1][1] + 1) // 2))
+    return out
+
+
 @pytest.mark.parametrize("grid", GRIDS)
-def test_wide_coarse_preconditioner_bit_exact(grid):
-    w, h, bands = grid
+def test_odd_levels_preconditioner_bit_exact(grid):
+    w, h, coarsest = grid
     o, g, nl, b, _ = _system(w, h)
-    cw, ch = w >> nl, h >> nl
-    assert (cw * ch + 4095) // 4096 == bands and cw * ch > 1024, (cw, ch)
-    print(f"{w} x {h}: {nl + 1} levels, coarsest {cw} x {ch} in {bands} band(s)", flush=True)
+    sides = _sides(w, h, nl)
+    assert sides[-1] == coarsest, sides
+    for lvl, (lw, lh) in enumerate(sides):
+        assert g.field_length("FLUID", lvl) == lw * lh
+    print(f"{w} x {h}: {o.dims()['num_levels'] + 1} levels of the reference, {nl + 1} in all, "
+          f"{sides[o.dims()['num_levels']]} -> {coarsest}", flush=True)
     o.write("RESIDUAL", b)
     o.stage("apply_preconditioner")
     g.apply_preconditioner()
     for lvl in range(nl + 1):
-        lw = w >> lvl
+        lw = sides[lvl][0]
         fl = o.read("FLUID", lvl).astype(bool)
         m = np.zeros(o.read("ERROR", lvl).size, dtype=bool)
         m[lw:lw + fl.size] = fl
@@ -98,7 +109,7 @@ def test_wide_coarse_preconditioner_bit_exact(grid):
 
 
 @pytest.mark.parametrize("grid", GRIDS[:4])
-def test_wide_coarse_relative_solve(grid):
+def test_odd_levels_relative_solve(grid):
     w, h, _ = grid
     o, g, _, b, p0 = _system(w, h)
     sys_ = assembled(o)
@@ -119,7 +130,7 @@ def test_wide_coarse_relative_solve(grid):
 
 
 @pytest.mark.parametrize("grid", GRIDS[:4])
-def test_wide_coarse_one_update(grid):
+def test_odd_levels_one_update(grid):
     from FLIP.apic import DebugOptions
     w, h, _ = grid
     xy = _scene(w, h)
@@ -135,14 +146,14 @@ def test_wide_coarse_one_update(grid):
     g.close()
 
 
-def test_wide_coarse_batched_equals_alone():
+def test_odd_levels_batched_equals_alone():
     from FLIP.apic import Apic, Batch, DebugOptions
     from test_gpu_batch import _assert_same
-    # (width, height, solver mode): the relative mode's wide coarse CG on
+    # (width, height, solver mode): the relative mode's deeper levels on
     # three 2 x odd grids, beside a reference-rules one (the one-workgroup
     # serial CG on 63 x 63) and a grid that halves down to 3 x 3
     specs = [(254, 254, 1), (126, 126, 1), (190, 190, 1), (126, 126, 0), (96, 96, 1)]
-    print(f"wide coarse CG, batch vs alone: {len(specs)} simulations, 3 updates", flush=True)
+    print(f"odd levels, batch vs alone: {len(specs)} simulations, 3 updates", flush=True)
     batch = Batch()
     alone, batched = [], []
     for w, h, mode in specs:

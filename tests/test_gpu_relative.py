@@ -23,7 +23,7 @@ pytestmark = pytest.mark.gpu
 def solve_pair(name, tol, every, max_it=500):
     o, g, s = make_pair(name)
     oracle_until(o, "solve_pcg", s["dt"])
-    nl = o.dims()["num_levels"]
+    nl = o.dims()["num_levels_relative"]  # the relative mode's levels
     for lvl in range(0, nl + 1):
         to_gpu(o, g, ["FLUID", "DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y"], level=lvl)
     to_gpu(o, g, ["RESIDUAL", "PRESSURE"])

@@ -303,12 +303,25 @@ def test_coarse_cg_exits(precision, bval, moves):
 # (r.r = 0 at the start, s.As <= 0, ||r|| <= 1e-2 ||r0||), which the relative
 # mode runs on its coarsest level whatever its size.
 
+def _halvings_up(m):
+    """Sides of the levels below an m x m one: halved, rounded up, while the
+    level has more than 64 cells and its side exceeds 2."""
+    out = []
+    while m * m > 64 and m > 2:
+        m = (m + 1) // 2
+        out.append(m)
+    return out
+
+
 def square_coarse_oracle(precision, m, diag_value, b):
     """A (2m) x (2m) grid with m odd: one multigrid level of m x m cells, the
-    coarsest, every cell fluid; operator diag_value on the diagonal, -1
-    towards neighbours (Dirichlet outside), b per cell."""
+    reference's coarsest, every cell fluid; operator diag_value on the
+    diagonal, -1 towards neighbours (Dirichlet outside), b per cell.  (The
+    relative mode's deeper levels, below it when m^2 > 64, are not touched by
+    the coarse-solve stages.)"""
     o = oracle(2 * m, 2 * m, 1.0, precision, xy=np.array([[float(m), float(m)]], np.float32))
     assert o.dims()["num_levels"] == 1
+    assert o.dims()["num_levels_relative"] == 1 + len(_halvings_up(m))
     n = o.field_length("DIAG", 1)
     diag, px, py, r = (np.zeros(n) for _ in range(4))
     idx = m + np.arange(m * m)
