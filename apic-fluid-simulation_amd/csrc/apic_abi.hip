@@ -278,7 +278,6 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     for (const auto& hz : s.hier)
         if (hz.count > 0) pcs = std::max(pcs, s.padded(s.levels[hz.count - 1].w, s.levels[hz.count - 1].h));
     ALLOC(s.cg_s, pcs); ALLOC(s.cg_ap, pcs); ALLOC(s.cg_r, pcs);
-    ALLOC(s.part_cg, 3 * kWideCoarseMaxWgs); ALLOC(s.cg_bar, 1);
     ALLOC(s.pcg, 1);
     const size_t ngam = std::max<size_t>(kRedBlocks, gamma_partials(s));
     ALLOC(s.part_gamma, ngam); ALLOC(s.part_delta, kRedBlocks); ALLOC(s.part_max, kRedBlocks);
@@ -401,20 +400,13 @@ apic_status do_sort(Sim& s) {
 // this solve took.
 int next_batch(const Sim& s) { return s.h_pcg->iters; }
 
-// The solve's counters, from the host mirror.  A fault of the multi-workgroup
-// coarse CG (its workgroups were not all resident, and a barrier wait gave
-// up) fails the call and restarts the barrier's count.
+// The solve's counters, from the host mirror.
 apic_status pcg_outcome(Sim& s) {
     s.stats.pcg_iterations = (uint32_t)s.h_pcg->iters;
     s.stats.pcg_exit_reason = (uint32_t)s.h_pcg->exit_reason;
     s.stats.pcg_max_residual = s.h_pcg->maxerr;
     s.stats.pcg_rel_residual = rel_residual(s);
     s.pcg_batch = next_batch(s);
-    if (s.h_pcg->fault) {
-        HIP_TRY(hipMemsetAsync(s.cg_bar, 0, sizeof(*s.cg_bar), s.stream));
-        HIP_TRY(hipStreamSynchronize(s.stream));
-        return fail(APIC_ERR_HIP, "pressure solve: a barrier of the multi-workgroup coarse CG gave up waiting");
-    }
     return APIC_OK;
 }
 

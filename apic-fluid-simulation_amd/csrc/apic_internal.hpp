@@ -50,21 +50,6 @@ __host__ __device__ inline size_t bottom_padded(const LevelView& L) {
 __host__ __device__ inline size_t bottom_flags(const LevelView& L) { return ((size_t)L.cells + 3) / 4 * 4; }
 constexpr uint32_t kBottomFirstCells = 1024;  // cells of the one-workgroup bottom's finest level
 
-// The relative mode solves a coarsest level of more than kWideCoarseMinCells
-// cells (a grid side of 2 x odd stops the coarsening early) with a CG over
-// one or more workgroups, each owning a band of kWideCoarseBand cells, when
-// the level has at most kWideCoarseMaxWgs bands and rows of at most
-// kWideCoarseMaxW cells (a band's LDS holds its cells and one row either side).
-constexpr uint32_t kWideCoarseMinCells = 1024;
-constexpr uint32_t kWideCoarseBand = 4096;
-constexpr uint32_t kWideCoarseMaxWgs = 64;
-constexpr uint32_t kWideCoarseMaxW = 1024;
-inline uint32_t wide_coarse_wgs(const LevelView& L) {
-    if (L.cells <= kWideCoarseMinCells || L.w > kWideCoarseMaxW) return 0;
-    const uint32_t g = (L.cells + kWideCoarseBand - 1) / kWideCoarseBand;
-    return g <= kWideCoarseMaxWgs ? g : 0;
-}
-
 // Whether levels [first, count) fit the one-workgroup bottom's LDS.
 inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first, size_t count) {
     size_t fl = 0, fb = 0;
@@ -99,7 +84,6 @@ struct PcgState {
     double gamma_d[2];       // r.z of iteration k in gamma_d[k & 1]
     double alpha_d;          // the last step length in FP64 (the flexible beta uses it)
     int32_t max_iters;
-    int32_t fault;           // nonzero: a workgroup of the multi-workgroup coarse CG gave up waiting
 };
 
 // Base-level fluid statistics needed by the padded-list quirk of prolongate
@@ -241,8 +225,6 @@ struct Sim {
     float* As_lo = nullptr;          // relative mode: A s (FP64) - As
     float* r_lo = nullptr;           // relative mode: the base residual's low part (r + r_lo in FP64)
     float* cg_s = nullptr; float* cg_ap = nullptr; float* cg_r = nullptr;  // coarsest level
-    double* part_cg = nullptr;               // the multi-workgroup coarse CG's partials (3 per workgroup)
-    unsigned long long* cg_bar = nullptr;    // and its barrier's arrival count
     float* rhs = nullptr;            // copy of b for the relative mode's residual recompute
     float* r_alt = nullptr;          // the base residual's second buffer (the fused update writes it)
     PcgState* pcg = nullptr;
@@ -384,7 +366,7 @@ inline void emit(Sim& s, const typename K::Args& a, uint32_t nwg, int k = 0) {
     if (nwg == 0) return;
     if (s.rec) {
         s.rec->add(reinterpret_cast<const void*>(&batch_kernel<K>), sizeof(a), &a, K::kThreads, nwg, k,
-                   !round_robin<K>::value, coresident<K>::value);
+                   !round_robin<K>::value);
         return;
     }
     hipLaunchKernelGGL(one_kernel<K>, dim3(nwg), dim3(K::kThreads), 0, s.stream, a, k);

@@ -20,7 +20,7 @@ void Recorder::begin(uint32_t sim) {
 }
 
 int Recorder::find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k,
-                          bool bands, bool cores) {
+                          bool bands) {
     for (size_t g = 0; g < groups_.size(); ++g) {
         const Group& G = groups_[g];
         if (G.fn == fn && G.occurrence == occurrence && G.k == k) return (int)g;
@@ -32,20 +32,18 @@ int Recorder::find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, 
     G.threads = threads;
     G.k = k;
     G.bands = bands;
-    G.cores = cores;
     groups_.push_back(std::move(G));
     return (int)groups_.size() - 1;
 }
 
 void Recorder::add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k,
-                   bool bands, bool cores) {
+                   bool bands) {
     if (nwg > kMapWgMask || cur_ >= kMaxBatchRows) overflow_ = true;  // kMapWgMask itself stays unused (kMapSkip)
-    if (cores && nwg > kCoresidentCap) overflow_ = true;
     std::vector<int>& chain = chains_[cur_];
     uint32_t occurrence = 0;
     for (int g : chain)
         if (groups_[g].fn == fn) ++occurrence;
-    const int g = find_or_add(fn, occurrence, arg_size, threads, k, bands, cores);
+    const int g = find_or_add(fn, occurrence, arg_size, threads, k, bands);
     Group& G = groups_[g];
     const uint8_t* p = static_cast<const uint8_t*>(args);
     G.args.insert(G.args.end(), p, p + arg_size);
@@ -85,21 +83,10 @@ std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size
     // rotated by the workgroups laid out before it so that the queues stay
     // even; the grid interleaves the eight queues (position 8 i + x runs the
     // i-th entry of XCD x), padded with entries that run nothing.
-    // A coresident group takes one map (one launch) per run of whole
-    // simulations of at most kCoresidentCap workgroups.
-    std::vector<std::vector<std::vector<uint32_t>>> maps(ng);
+    std::vector<std::vector<uint32_t>> maps(ng);
     for (int g : order) {
         const Group& G = groups_[g];
-        maps[g].resize(1);
-        if (G.cores) {
-            for (size_t row = 0; row < G.nwg.size(); ++row) {
-                if (maps[g].back().size() + G.nwg[row] > kCoresidentCap) maps[g].emplace_back();
-                for (uint32_t wg = 0; wg < G.nwg[row]; ++wg)
-                    maps[g].back().push_back(((uint32_t)row << kMapWgBits) | wg);
-            }
-            continue;
-        }
-        std::vector<uint32_t>& m = maps[g][0];
+        std::vector<uint32_t>& m = maps[g];
         if (!G.bands) {
             for (size_t row = 0; row < G.nwg.size(); ++row)
                 for (uint32_t wg = 0; wg < G.nwg[row]; ++wg) m.push_back(((uint32_t)row << kMapWgBits) | wg);
@@ -125,7 +112,7 @@ std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size
     size_t total = align(used);
     for (int g : order) {
         total = align(total) + groups_[g].args.size();
-        for (const auto& m : maps[g]) total = align(total) + m.size() * sizeof(uint32_t);
+        total = align(total) + maps[g].size() * sizeof(uint32_t);
     }
     total = align(total);
     if (host.size() < total) host.resize(total);
@@ -140,15 +127,12 @@ std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size
         L.tab_off = off;
         std::memcpy(host.data() + off, G.args.data(), G.args.size());
         off += G.args.size();
-        for (const auto& m : maps[g]) {
-            if (m.empty()) continue;
-            off = align(off);
-            L.map_off = off;
-            std::memcpy(host.data() + off, m.data(), m.size() * sizeof(uint32_t));
-            off += m.size() * sizeof(uint32_t);
-            L.grid = (uint32_t)m.size();
-            out.push_back(L);
-        }
+        off = align(off);
+        L.map_off = off;
+        std::memcpy(host.data() + off, maps[g].data(), maps[g].size() * sizeof(uint32_t));
+        off += maps[g].size() * sizeof(uint32_t);
+        L.grid = (uint32_t)maps[g].size();
+        out.push_back(L);
     }
     used = align(off);
     return out;

@@ -47,17 +47,6 @@ struct round_robin : std::false_type {};
 template <class K>
 struct round_robin<K, std::void_t<decltype(K::kRoundRobin)>> : std::bool_constant<K::kRoundRobin> {};
 
-// A kernel whose workgroups of one simulation wait for each other declares
-// kCoresident = true: a batched launch of it then holds at most
-// kCoresidentCap workgroups (one per CU of the chip), so that all of them
-// are resident at once; a group with more is issued as several launches,
-// whole simulations each.
-template <class K, class = void>
-struct coresident : std::false_type {};
-template <class K>
-struct coresident<K, std::void_t<decltype(K::kCoresident)>> : std::bool_constant<K::kCoresident> {};
-constexpr uint32_t kCoresidentCap = 256;
-
 template <class K>
 __global__ __launch_bounds__(K::kThreads) void one_kernel(typename K::Args a, int k) {
     K::run(a, round_robin<K>::value ? blockIdx.x : xcd_band(blockIdx.x, gridDim.x), k);
@@ -100,9 +89,8 @@ public:
 
     void begin(uint32_t sim);  // the following launches belong to simulation `sim`
     // bands: lay each simulation's workgroups out in XCD bands (as one_kernel
-    // does), else in dispatch order; cores: a coresident kernel (above)
-    void add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k, bool bands,
-             bool cores = false);
+    // does), else in dispatch order
+    void add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k, bool bands);
     bool empty() const { return groups_.empty(); }
     // a simulation's launch had more workgroups than a map entry can name
     bool overflow() const { return overflow_; }
@@ -122,7 +110,6 @@ private:
         uint32_t threads;
         int k;
         bool bands;
-        bool cores;
         std::vector<uint8_t> args;
         std::vector<uint32_t> nwg;  // per table row
     };
@@ -130,8 +117,7 @@ private:
     std::vector<std::vector<int>> chains_;  // per simulation: its groups in launch order
     uint32_t cur_ = 0;
     bool overflow_ = false;
-    int find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k, bool bands,
-                    bool cores);
+    int find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k, bool bands);
 };
 
 // Issues compiled launches whose tables were uploaded to `dev`.  k >= 0

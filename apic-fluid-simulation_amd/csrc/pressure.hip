@@ -673,12 +673,17 @@ __device__ void coarse_cg_wave(const LevelView& L) {
     if (in) L.e[pc] = e;
 }
 
-// The relative mode's coarse CG on the one-workgroup bottom: the iteration of
-// CoarseCgWide below (FP32 vectors and SpMV chain, FP64 dot products, a
-// search vector that is zero off the fluid cells, exits relative to the
-// level's right-hand side, at most min(fluid cells, 100) iterations), with
-// the FP64 sums of one wave (<= 64 cells, one lane per cell) or of the
-// workgroup.
+// The relative mode's coarse CG on the one-workgroup bottom:
+// solve_coarse_exact's iteration (src/apic.rs:618-704: the same FP32 vector
+// arithmetic and fused SpMV chain, at most min(fluid cells, 100) iterations)
+// with FP64 dot products, a search vector that is zero off the fluid cells
+// (its matrix couplings are zero: the reference's copies of r there are only
+// ever multiplied by them), and exits relative to the level's right-hand
+// side: r.r = 0 at the start, s.As <= 0, ||r|| <= 1e-2 ||r0||.  (With the
+// reference's absolute exits the coarse correction vanishes once the
+// residual is small, |s.As| < 1e-5 at the first step, and the outer CG slows
+// down.)  The FP64 sums are those of one wave (<= 64 cells, one lane per
+// cell) or of the workgroup.
 __device__ void coarse_cg_rel_wave(const LevelView& L) {
     const uint32_t n = L.cells;
     const int w = (int)L.w;
@@ -867,245 +872,6 @@ struct BottomGlobal {
         if (*a.done) return;
         GlobalViews V{&a.ls};
         bottom_body(V, a.first, a.ls.count - 1, a.cg_s, a.cg_ap, a.cg_r, sprod, scratch, a.rel != 0);
-    }
-};
-
-// ---- the relative mode's coarse CG over several workgroups
-//
-// solve_coarse_exact's iteration (src/apic.rs:618-704: the same FP32 vector
-// arithmetic and fused SpMV chain, at most min(fluid cells, 100) iterations)
-// with FP64 dot products, and with exits relative to the level's right-hand
-// side: r.r = 0 at the start, s.As <= 0, ||r|| <= 1e-2 ||r0||.  (With the
-// reference's absolute exits the coarse correction vanishes once the
-// residual is small, |s.As| < 1e-5 at the first step, and the outer CG
-// slows down, or stalls on a large coarsest level.)  This is the relative
-// mode's coarse CG whatever the coarsest level's size: on one of more than
-// 1024 cells (a grid side of 2 x odd stops the hierarchy early) it runs
-// here, over several workgroups, else in the one-workgroup bottom
-// (coarse_cg_rel_wave, coarse_cg_rel_block).  Workgroup j owns
-// the band of cells [j B, (j + 1) B) (B = kWideCoarseBand) and keeps its
-// cells' e, r, s and matrix rows in registers and its search vector, with
-// one row either side, in LDS.  A band publishes its first and last row of r
-// and s for its neighbours; the search vector of a non-fluid cell is zero
-// (its matrix couplings are zero: the reference's copies of r there are
-// only ever multiplied by them).  Two barriers per iteration separate the
-// partials of s.As and of r.r from their sums; every workgroup adds the same
-// partials in the same order and takes the same exits.
-
-// Values that workgroups exchange inside one launch (the XCDs' L2 caches
-// are not coherent with each other): relaxed atomics at agent scope, which
-// go past the non-coherent caches.
-__device__ __forceinline__ float xload(const float* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double xload(const double* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void xstore(float* p, float v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void xstore(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// s_waitcnt vmcnt(0): the wave's memory operations have completed
-__device__ __forceinline__ void wait_vmem() { __builtin_amdgcn_s_waitcnt(0 | (0x7 << 4) | (0xf << 8)); }
-
-constexpr uint32_t kSpinLimit = 1u << 18;  // polls of a barrier before it gives up
-
-// Every workgroup of the launch arrives.  The arrival count is never reset:
-// every workgroup passes the same barriers, so between launches it is a
-// multiple of nwg, and a barrier's target is the next multiple above the
-// count its first arrival saw.  A wait that exceeds kSpinLimit polls (the
-// launch's workgroups were not all resident) sets the state's fault flag and
-// returns false; the caller's workgroup then returns, and so do the others
-// at their next barrier, so every wave ends.
-__device__ bool wide_barrier(unsigned long long* bar, uint32_t nwg, PcgState* st) {
-    __shared__ int ok;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    wait_vmem();  // this wave's exchange stores are complete
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned long long old = __hip_atomic_fetch_add(bar, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned long long target = (old / nwg + 1) * nwg;
-        uint32_t spins = 0;
-        int good = 1;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            if (++spins > kSpinLimit) {
-                good = 0;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
-        }
-        if (!good) __hip_atomic_store(&st->fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        ok = good;
-    }
-    __syncthreads();
-    wait_vmem();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    return ok != 0;
-}
-
-// The sums over the nwg (<= 64) workgroups' partials part[j * 3 + a] and
-// part[j * 3 + b]: lane j of wave 0 loads workgroup j's, the wave adds them
-// with wave_sum's butterfly (the same order in every workgroup), and LDS
-// hands the results to every thread.
-__device__ void wide_sums(const double* part, uint32_t nwg, int a, int b, double& va, double& vb) {
-    __shared__ double res[2];
-    if (threadIdx.x < 64) {
-        const uint32_t j = threadIdx.x;
-        double x = j < nwg ? xload(&part[j * 3 + a]) : 0.0;
-        double y = j < nwg ? xload(&part[j * 3 + b]) : 0.0;
-        x = wave_sum(x);
-        y = wave_sum(y);
-        if (threadIdx.x == 0) {
-            res[0] = x;
-            res[1] = y;
-        }
-    }
-    __syncthreads();
-    va = res[0];
-    vb = res[1];
-    __syncthreads();  // (res is rewritten by the next call)
-}
-
-struct CoarseCgWide {
-    static constexpr int kThreads = 512;
-    static constexpr bool kRoundRobin = true;  // a band per workgroup, no XCD bands
-    static constexpr bool kCoresident = true;  // its workgroups wait for each other
-    static constexpr int kPer = (int)kWideCoarseBand / kThreads;
-    struct Args {
-        LevelView C;        // the coarsest level: r in, e out
-        float* rx;          // exchange copies of r and of the two search vectors (padded layout)
-        float* sx[2];
-        double* part;       // 3 partials per workgroup
-        unsigned long long* bar;
-        PcgState* st;
-        const int32_t* done;
-        uint32_t nwg;
-    };
-    static __device__ void run(const Args& a, uint32_t wg, int) {
-        __shared__ float sl[kWideCoarseBand + 2 * kWideCoarseMaxW];  // s over [lo - w, hi + w)
-        __shared__ double dscratch[2 * (kThreads / 64)];
-        if (*a.done) return;
-        const LevelView& C = a.C;
-        const int32_t n = (int32_t)C.cells, w = (int32_t)C.w;
-        const int32_t lo = (int32_t)(wg * kWideCoarseBand), hi = min(n, lo + (int32_t)kWideCoarseBand);
-        const int32_t base = lo - w;  // sl[i] holds cell base + i
-        const int32_t t = (int32_t)threadIdx.x;
-        auto boundary = [&](int32_t c) { return c < lo + w || c >= hi - w; };
-        float e[kPer], r[kPer], s[kPer], sn[kPer], ap[kPer], dg[kPer], pxr[kPer], pxl[kPer], pyu[kPer], pyd[kPer];
-        uint32_t fmask = 0u, inmask = 0u;
-        double g = 0.0, nf = 0.0;
-#pragma unroll
-        for (int q = 0; q < kPer; ++q) {
-            const int32_t c = lo + t + kThreads * q;
-            const bool in = c < hi;
-            const size_t pc = (size_t)(in ? c : 0) + w;
-            const bool f = in && C.fluid[in ? c : 0];
-            inmask |= (uint32_t)in << q;
-            fmask |= (uint32_t)f << q;
-            r[q] = f ? C.r[pc] : 0.0f;
-            dg[q] = f ? C.diag[pc] : 0.0f;
-            pxr[q] = f ? C.px[pc] : 0.0f;
-            pxl[q] = f ? C.px[pc - 1] : 0.0f;
-            pyu[q] = f ? C.py[pc] : 0.0f;
-            pyd[q] = f ? C.py[pc - w] : 0.0f;
-            e[q] = 0.0f;
-            s[q] = 0.0f;
-            if (f) {
-                g += (double)r[q] * (double)r[q];
-                nf += 1.0;
-            }
-            if (in && boundary(c)) xstore(&a.rx[pc], r[q]);
-        }
-        auto write_e = [&]() {
-#pragma unroll
-            for (int q = 0; q < kPer; ++q)
-                if ((inmask >> q) & 1u) C.e[(size_t)(lo + t + kThreads * q) + w] = e[q];
-        };
-        block_sum2(g, nf, dscratch);
-        if (threadIdx.x == 0) {  // (slots 1 and 2: slot 0 takes s.As before every workgroup has read these)
-            xstore(&a.part[wg * 3 + 1], g);
-            xstore(&a.part[wg * 3 + 2], nf);
-        }
-        if (!wide_barrier(a.bar, a.nwg, a.st)) return;
-        double gamma, nfl;
-        wide_sums(a.part, a.nwg, 1, 2, gamma, nfl);
-        if (!(gamma > 0.0)) {
-            write_e();
-            return;
-        }
-        const double stop = kCoarseRelStop2 * gamma;
-        const int maxit = (int)fmin(nfl, 100.0);
-        float beta = 0.0f;
-        int cur = 0;
-        for (int it = 0; it < maxit; ++it) {
-            const bool first = it == 0;
-            // s = r + beta s over the band (into LDS, the boundary rows also
-            // to the exchange copy) and over the rows either side of it
-#pragma unroll
-            for (int q = 0; q < kPer; ++q) {
-                const int32_t c = lo + t + kThreads * q;
-                sn[q] = ((fmask >> q) & 1u) ? (first ? r[q] : r[q] + (beta * s[q])) : 0.0f;
-                if ((inmask >> q) & 1u) {
-                    sl[c - base] = sn[q];
-                    if (boundary(c)) xstore(&a.sx[cur ^ 1][(size_t)c + w], sn[q]);
-                }
-            }
-            for (int32_t i = t; i < 2 * w; i += kThreads) {
-                const int32_t c = i < w ? lo - w + i : hi + (i - w);
-                float v = 0.0f;
-                if (c >= 0 && c < n) {
-                    const size_t pc = (size_t)c + w;
-                    const float rv = xload(&a.rx[pc]);
-                    v = first ? rv : rv + (beta * xload(&a.sx[cur][pc]));
-                }
-                sl[c - base] = v;
-            }
-            __syncthreads();
-            // A s over the band's fluid cells (the chain of coarse_cg_block)
-            double dp = 0.0;
-#pragma unroll
-            for (int q = 0; q < kPer; ++q) {
-                if (!((fmask >> q) & 1u)) continue;
-                const int32_t li = lo + t + kThreads * q - base;
-                float v = dg[q] * sl[li];
-                v = fmaf(pxr[q], sl[li + 1], v);
-                v = fmaf(pxl[q], sl[li - 1], v);
-                v = fmaf(pyu[q], sl[li + w], v);
-                v = fmaf(pyd[q], sl[li - w], v);
-                ap[q] = v;
-                dp += (double)sn[q] * (double)v;
-            }
-            dp = block_sum(dp, dscratch);
-            if (threadIdx.x == 0) xstore(&a.part[wg * 3 + 0], dp);
-            if (!wide_barrier(a.bar, a.nwg, a.st)) return;
-            double delta, unused;
-            wide_sums(a.part, a.nwg, 0, 0, delta, unused);
-            if (!(delta > 0.0)) break;
-            const float alpha = (float)(gamma / delta);
-            double ng = 0.0;
-#pragma unroll
-            for (int q = 0; q < kPer; ++q) {
-                s[q] = sn[q];
-                if (!((fmask >> q) & 1u)) continue;
-                e[q] += alpha * sn[q];
-                r[q] = r[q] - alpha * ap[q];
-                ng += (double)r[q] * (double)r[q];
-                const int32_t c = lo + t + kThreads * q;
-                if (boundary(c)) xstore(&a.rx[(size_t)c + w], r[q]);
-            }
-            ng = block_sum(ng, dscratch);
-            if (threadIdx.x == 0) xstore(&a.part[wg * 3 + 1], ng);
-            if (!wide_barrier(a.bar, a.nwg, a.st)) return;
-            double ngs, unused2;
-            wide_sums(a.part, a.nwg, 1, 1, ngs, unused2);
-            cur ^= 1;
-            if (ngs <= stop) break;
-            beta = (float)(ngs / gamma);
-            gamma = ngs;
-        }
-        write_e();
     }
 };
 
@@ -1355,7 +1121,6 @@ struct PcgReset {
         st->gamma_d[0] = st->gamma_d[1] = 0.0;
         st->alpha_d = 0.0;
         st->max_iters = a.max_iters;
-        st->fault = 0;
     }
 };
 
@@ -1847,7 +1612,6 @@ struct Decide {
             host->alpha = st->alpha;
             host->bnorm2 = st->bnorm2;
             host->rnorm2 = st->rnorm2;
-            host->fault = st->fault;
         }
     }
 };
@@ -3052,14 +2816,6 @@ bool level_pair(const Sim& sim, int l) {
     return !sim.rec && l >= 1 && l + 1 < (int)sim.bottom_level && few_tiles(sim, l);
 }
 
-// Workgroups of the relative mode's multi-workgroup coarse CG (0: the
-// one-workgroup bottom solves the coarsest level).
-uint32_t wide_coarse(const Sim& sim) {
-    const size_t last = sim.level_count - 1;
-    if (sim.solver.mode != 1 || sim.bottom_in_lds || last == 0 || sim.bottom_level != last) return 0;
-    return wide_coarse_wgs(sim.levels[last]);
-}
-
 // The relative mode's update of iteration k fused into the base level's
 // down-sweep (VDownAx, one launch fewer): a simulation stepped alone whose
 // level pair (1, 2) follows the base (it takes over the residual test), on
@@ -3156,10 +2912,6 @@ void launch_vcycle(Sim& sim, int mode, int k) {
     }
     if (sim.bottom_in_lds) {
         emit<BottomLds>(sim, {level_set(sim), bottom, done, (int)(sim.solver.mode == 1)}, 1);
-    } else if (const uint32_t g = wide_coarse(sim)) {
-        emit<CoarseCgWide>(sim, {sim.levels[last], sim.cg_r, {sim.cg_s, sim.cg_ap}, sim.part_cg, sim.cg_bar, sim.pcg,
-                                 done, g},
-                           g);
     } else {
         emit<BottomGlobal>(sim, {level_set(sim), bottom, sim.cg_s, sim.cg_ap, sim.cg_r, done,
                                  (int)(sim.solver.mode == 1)},

@@ -264,11 +264,11 @@ template <class R> struct Level {
     // instead of absolute ones: r.r = 0 at the start, s.As <= 0, and
     // ||r|| <= 1e-2 ||r0||.  With absolute exits the coarse correction
     // vanishes once the residual is small (|s.As| < 1e-5 at the first step),
-    // and the outer CG slows down, or stalls on a large coarsest level (a
-    // grid side of 2 x odd).  The GPU sums in another order (a wave's or a
-    // workgroup's tree, or workgroups' bands), which changes the FP32 step
-    // lengths only where an FP64 rounding tips them.
-    void solve_coarse_wide() {
+    // and the outer CG slows down, or stalls on a large coarsest level (the
+    // reference's coarsest on a grid side of 2 x odd).  The GPU sums in
+    // another order (a wave's or a workgroup's tree), which changes the FP32
+    // step lengths only where an FP64 rounding tips them.
+    void solve_coarse_relative() {
         const size_t start = w, end = start + cells;
         std::fill(e.begin() + start, e.begin() + end, R(0));
         std::copy(r.begin() + start, r.begin() + end, cg_r.begin() + start);
@@ -789,7 +789,7 @@ template <class R> struct Sim {
                 levels[l].smooth(false, omega);
                 levels[l].restrict_to(levels[l + 1]);
             }
-            if (solver_mode == 1) levels[last].solve_coarse_wide();
+            if (solver_mode == 1) levels[last].solve_coarse_relative();
             else levels[last].solve_coarse_exact();
             for (size_t l = last; l-- > 0;) {
                 levels[l].prolongate_from(levels[l + 1]);
@@ -1442,7 +1442,7 @@ template <class R> int stage_t(Sim<R>* s, int stage, float dt, uint32_t ts) {
         // the coarsest level's CG alone (solve_coarse_exact), for the dense-solve pins
         case 16: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels[s->ref_levels - 1].solve_coarse_exact(); break;
         // the relative mode's coarse CG
-        case 17: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels[s->ref_levels - 1].solve_coarse_wide(); break;
+        case 17: if (s->levels.empty()) return APIC_ERR_INVALID_ARGUMENT; s->levels[s->ref_levels - 1].solve_coarse_relative(); break;
         // restrict / prolongate between level `timestamp` and the next, for the
         // pins of the odd-sided levels
         case 18: case 19: {

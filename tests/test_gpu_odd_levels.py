@@ -4,7 +4,9 @@ of up to (side / 2)^2 cells; the relative mode's V-cycle goes on below it,
 each side halved and rounded up, until the coarsest has at most 64 cells or
 a side of 2.  Sides 126 (63^2 -> 32^2 ... 8^2), 254 (127^2 -> 64^2 ... 8^2),
 510 (255^2 -> 128^2 ... 8^2), 254 x 126 (127 x 63 -> 64 x 32 ... 8 x 4) and
-2046 x 10 (1023 x 5 -> 512 x 3 -> 256 x 2).
+2046 x 10 (1023 x 5 -> 512 x 3 -> 256 x 2) and 2052 x 8 (1026 x 4 -> 513 x 2:
+a coarsest level of more than 1024 cells, solved by the one-workgroup CG in
+global memory).
 
   - apply_preconditioner on the oracle's assembled system: e on every level
     bit-exact against the oracle (its coarse CG's FP64 sums run in ascending
@@ -32,7 +34,8 @@ PRE_SOLVE = ["update_spatial_lookup", "transfer_particles_to_grid", "apply_exter
 LEVEL_FIELDS = ["FLUID", "DIAG", "INV_DIAG", "PLUS_X", "PLUS_Y"]
 DT = 0.002
 # (width, height, the relative mode's coarsest level)
-GRIDS = [(126, 126, (8, 8)), (254, 254, (8, 8)), (510, 510, (8, 8)), (254, 126, (8, 4)), (2046, 10, (256, 2))]
+GRIDS = [(126, 126, (8, 8)), (254, 254, (8, 8)), (510, 510, (8, 8)), (254, 126, (8, 4)), (2046, 10, (256, 2)),
+         (2052, 8, (513, 2))]
 TOL = {"POS_X": 1e-5, "POS_Y": 1e-5, "PRESSURE": 1e-5, "TIME_RESIDUAL": 2e-4, "VEL_X": 2e-5, "VEL_Y": 2e-5,
        "MAC_U": 2e-5, "MAC_V": 2e-5, "C_U_X": 2e-4, "C_U_Y": 2e-4, "C_V_X": 2e-4, "C_V_Y": 2e-4}
 
@@ -108,7 +111,7 @@ def test_odd_levels_preconditioner_bit_exact(grid):
     g.close()
 
 
-@pytest.mark.parametrize("grid", GRIDS[:4])
+@pytest.mark.parametrize("grid", GRIDS)
 def test_odd_levels_relative_solve(grid):
     w, h, _ = grid
     o, g, _, b, p0 = _system(w, h)
@@ -129,7 +132,7 @@ def test_odd_levels_relative_solve(grid):
     g.close()
 
 
-@pytest.mark.parametrize("grid", GRIDS[:4])
+@pytest.mark.parametrize("grid", GRIDS)
 def test_odd_levels_one_update(grid):
     from FLIP.apic import DebugOptions
     w, h, _ = grid
