@@ -384,11 +384,26 @@ double rel_residual(const Sim& s) {
     return p.bnorm2 > 0.0 ? std::sqrt(p.rnorm2 / p.bnorm2) : 0.0;
 }
 
-apic_status do_sort(Sim& s) {
+// After a failed update: a deferred gather is done, so that the records are
+// in sorted order as the reference leaves them (best effort: the stream may
+// hold the error).
+void settle_particles(Sim& s) {
+    if (!s.perm_pending) return;
+    launch_permute(s);
+    s.perm_pending = false;
+    (void)hipGetLastError();
+}
+
+// The sort of update_spatial_lookup (src/apic.rs:1049-1103).  `defer`: the
+// gather of the particle records into sorted order is left to the substep's
+// P2G (which reads through the permutation) and G2P (which writes the sorted
+// order), saving a pass over every record.
+apic_status do_sort(Sim& s, bool defer) {
     if (s.num_particles == 0) return APIC_OK;
     if (!s.keys_valid) launch_keygen(s);
     launch_radix_sort(s);
-    launch_permute(s);
+    if (defer) s.perm_pending = true;
+    else launch_permute(s);
     launch_cell_ranges(s);
     s.cells_valid = true;
     s.keys_valid = false;
@@ -458,7 +473,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     launch_substep_begin(s, first, frame_dt);
     if (sort) {
         s.mark(APIC_T_SORT, 0);
-        if ((st = do_sort(s)) != APIC_OK) return st;
+        if ((st = do_sort(s, true)) != APIC_OK) return st;
         s.mark(APIC_T_SORT, 1);
     }
     s.mark(APIC_T_P2G, 0);
@@ -485,10 +500,11 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     struct Saved {
         float *mac_u, *mac_v, *old_u, *old_v;
         uint8_t *valid_u, *valid_v, *nvalid_u, *nvalid_v;
-        bool cells_valid, keys_valid, maxv_valid;
+        bool cells_valid, keys_valid, maxv_valid, perm_pending;
+        int cur;
         size_t marks;
     } saved{s.mac_u, s.mac_v, s.old_u, s.old_v, s.valid_u, s.valid_v, s.nvalid_u, s.nvalid_v,
-            s.cells_valid, s.keys_valid, s.maxv_valid, s.ev_pending.size()};
+            s.cells_valid, s.keys_valid, s.maxv_valid, s.perm_pending, s.cur, s.ev_pending.size()};
     auto tail = [&]() -> apic_status {
         s.mark(APIC_T_GRADIENT, 0);
         launch_gradient(s, true, 0.0f);
@@ -514,6 +530,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
         s.mac_u = saved.mac_u; s.mac_v = saved.mac_v; s.old_u = saved.old_u; s.old_v = saved.old_v;
         s.valid_u = saved.valid_u; s.valid_v = saved.valid_v; s.nvalid_u = saved.nvalid_u; s.nvalid_v = saved.nvalid_v;
         s.cells_valid = saved.cells_valid; s.keys_valid = saved.keys_valid; s.maxv_valid = saved.maxv_valid;
+        s.perm_pending = saved.perm_pending; s.cur = saved.cur;
         s.drop_marks(saved.marks);
         int check_at = k + 2;
         while (true) {
@@ -740,7 +757,7 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
             s.rec = &B.rec;
             s.step_src = s.step;
             launch_substep_begin(s, first, frame_dt[act[r]]);
-            if (sort) do_sort(s);
+            if (sort) do_sort(s, true);
             launch_p2g(s, true, 0.0f);
             launch_build_pressure(s, true);
             launch_mg_build(s);
@@ -1045,7 +1062,10 @@ apic_status apic_update(apic_handle* h, const apic_debug_options* opts, float fr
         s.maxv_valid = true;
     }
     while (t < frame_dt) {
-        if ((st = substep(s, !disable_sort && substeps == 0, substeps == 0, frame_dt)) != APIC_OK) return st;
+        if ((st = substep(s, !disable_sort && substeps == 0, substeps == 0, frame_dt)) != APIC_OK) {
+            settle_particles(s);
+            return st;
+        }
         const StepState hs = *s.h_step;  // complete: it was written before the solve the host waited for
         last_maxv = hs.maxv;
         t += hs.dt;
@@ -1267,7 +1287,7 @@ apic_status apic_write_particles_aosoa(apic_handle* h, apic_particle_group group
 
 apic_status apic_update_spatial_lookup(apic_handle* h) {
     if (!h) return fail(APIC_ERR_INVALID_ARGUMENT, "null handle");
-    apic_status st = do_sort(h->sim);
+    apic_status st = do_sort(h->sim, false);
     if (st != APIC_OK) return st;
     HIP_TRY(hipStreamSynchronize(h->sim.stream));
     return APIC_OK;

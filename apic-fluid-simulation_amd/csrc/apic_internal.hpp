@@ -185,6 +185,11 @@ struct Sim {
     bool keys_valid = false;  // keys[0]/vals[0] hold the current positions' keys
     uint2* cell_range = nullptr;  // [begin, end) of every cell's particles in cell order, by Morton key
     bool cells_valid = false;     // the particles are in cell order and cell_range describes it
+    // The sort's gather is deferred: the particles lie in pbuf[cur] in their
+    // order before the sort, vals[0] maps a sorted position to its record,
+    // P2G reads through it and G2P writes the sorted order into the other
+    // buffer (within one update: its first substep's G2P ends it).
+    bool perm_pending = false;
 
     // MAC grid
     float* mac_u = nullptr;   float* mac_v = nullptr;

@@ -218,7 +218,9 @@ __device__ __forceinline__ void p2g_owner(
             const uint32_t idx = base + k;
             const int lo = s_cell[k];  // the window cell holding idx
             const uint32_t q = s_src[lo] + (idx - s_off[lo]);  // position in cell order
-            const size_t src = order ? order[q] : q;
+            // (a permutation entry past the records, written through the
+            // field API, reads the last record, as the sort's gather does)
+            const size_t src = order ? min(order[q], np - 1) : q;
             s_buf[0 * kOwnCap + k] = pb[(size_t)P_X * nalloc + src];
             s_buf[1 * kOwnCap + k] = pb[(size_t)P_Y * nalloc + src];
             s_buf[2 * kOwnCap + k] = pb[(size_t)V_X * nalloc + src];
@@ -424,19 +426,33 @@ __device__ inline float masked_interp(const float* w, const uint8_t* open, const
     return vel;
 }
 
+// pin / pout: the records before and after (the same buffer, or with a
+// deferred sort gather the other one, pin read through perm: record perm[i]
+// becomes record i, and lanes past the records read zeros, as the gather
+// leaves them).
 __device__ __forceinline__ void g2p(
-    float* __restrict__ pb, size_t nalloc, uint32_t np, float dt_arg, float cell, uint32_t W, uint32_t H,
+    const float* pin, float* pout, const uint32_t* perm, size_t nalloc, uint32_t np, float dt_arg,
+    float cell, uint32_t W, uint32_t H,
     uint32_t timestamp, float border_damping, const float* __restrict__ mac_u, const float* __restrict__ mac_v,
     const uint8_t* __restrict__ open_u, const uint8_t* __restrict__ open_v,
-    int write_keys, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, float* __restrict__ part_maxv,
+    int write_keys, uint32_t* __restrict__ keys, uint32_t* vals, float* __restrict__ part_maxv,
     const StepState* __restrict__ ss, const int32_t* __restrict__ guard, uint32_t wg, uint32_t nwg) {
     __shared__ float s_max[256 / 64];
     if (guard && *guard == 0) return;
     const float dt = ss ? ss->dt : dt_arg;
     float speed_new = 0.0f;
     for (size_t i = (size_t)wg * blockDim.x + threadIdx.x; i < nalloc; i += (size_t)nwg * blockDim.x) {
-        const float x = pb[(size_t)P_X * nalloc + i], y = pb[(size_t)P_Y * nalloc + i];
-        const float res = pb[(size_t)T_RES * nalloc + i];
+        float x = 0.0f, y = 0.0f, res = 0.0f;
+        if (!perm || i < np) {
+            size_t j = i;
+            if (perm) {
+                j = perm[i];
+                if (j >= np) j = np - 1;  // (as the sort's gather)
+            }
+            x = pin[(size_t)P_X * nalloc + j];
+            y = pin[(size_t)P_Y * nalloc + j];
+            res = pin[(size_t)T_RES * nalloc + j];
+        }
         const float gx = x / cell, gy = y / cell;
         Stencil su, sv;
         make_stencil(gx, gy - 0.5f, cell, su);
@@ -496,15 +512,15 @@ __device__ __forceinline__ void g2p(
         bool rx = ovxmin || ovxmax, ry = ovymin || ovymax;
         if (rx) { c_uy = c_uy * -1.0f; c_vx = c_vx * -1.0f; }
         if (ry) { c_uy = c_uy * -1.0f; c_vx = c_vx * -1.0f; }
-        pb[(size_t)P_X * nalloc + i] = nx;
-        pb[(size_t)P_Y * nalloc + i] = ny;
-        pb[(size_t)V_X * nalloc + i] = nvx;
-        pb[(size_t)V_Y * nalloc + i] = nvy;
-        pb[(size_t)CU_X * nalloc + i] = c_ux;
-        pb[(size_t)CU_Y * nalloc + i] = c_uy;
-        pb[(size_t)CV_X * nalloc + i] = c_vx;
-        pb[(size_t)CV_Y * nalloc + i] = c_vy;
-        pb[(size_t)T_RES * nalloc + i] = new_res;
+        pout[(size_t)P_X * nalloc + i] = nx;
+        pout[(size_t)P_Y * nalloc + i] = ny;
+        pout[(size_t)V_X * nalloc + i] = nvx;
+        pout[(size_t)V_Y * nalloc + i] = nvy;
+        pout[(size_t)CU_X * nalloc + i] = c_ux;
+        pout[(size_t)CU_Y * nalloc + i] = c_uy;
+        pout[(size_t)CV_X * nalloc + i] = c_vx;
+        pout[(size_t)CV_Y * nalloc + i] = c_vy;
+        pout[(size_t)T_RES * nalloc + i] = new_res;
         speed_new = fast_max(speed_new, sqrtf(nvx * nvx + nvy * nvy));
         if (write_keys && i < np) {
             // keygen of the next update_spatial_lookup (:1049-1064)
@@ -512,7 +528,7 @@ __device__ __forceinline__ void g2p(
             uint32_t ix = (uint32_t)(int32_t)fast_min(fast_max(kx, 0.0f), (float)(W - 1));
             uint32_t iy = (uint32_t)(int32_t)fast_min(fast_max(ky, 0.0f), (float)(H - 1));
             keys[i] = morton(ix, iy);
-            vals[i] = (uint32_t)i;
+            vals[i] = (uint32_t)i;  // (perm, when given, is this array: lane i read its entry above)
         }
     }
     // max |v| over all lanes including the tail lanes of the last chunk
@@ -531,7 +547,9 @@ __device__ __forceinline__ void g2p(
 struct G2p {
     static constexpr int kThreads = 256;
     struct Args {
-        float* pb;
+        const float* pin;
+        float* pout;
+        const uint32_t* perm;
         size_t nalloc;
         uint32_t np;
         float dt;
@@ -551,7 +569,8 @@ struct G2p {
         uint32_t nwg;
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        g2p(a.pb, a.nalloc, a.np, a.dt, a.cell, a.W, a.H, a.timestamp, a.border_damping, a.mac_u, a.mac_v, a.open_u,
+        g2p(a.pin, a.pout, a.perm, a.nalloc, a.np, a.dt, a.cell, a.W, a.H, a.timestamp, a.border_damping, a.mac_u,
+            a.mac_v, a.open_u,
             a.open_v, a.write_keys, a.keys, a.vals, a.part_maxv, a.ss, a.guard, wg, a.nwg);
     }
 };
@@ -636,6 +655,8 @@ void launch_p2g(Sim& sim, bool forces, float dt) {
             order = sim.vals[0];
         }
         launch_cell_ranges(sim);
+    } else if (sim.perm_pending) {
+        order = sim.vals[0];  // the sort's gather is deferred: read through its permutation
     }
     const uint32_t tx = (sim.W + kOwnTile - 1) / kOwnTile, ty = (sim.H + kOwnTile - 1) / kOwnTile;
     const int ext = forces && sim.forces_active;
@@ -666,10 +687,19 @@ void launch_g2p(Sim& sim, float dt, bool write_keys, bool end_substep) {
     }
     const unsigned blocks = particle_blocks(sim);
     sim.mark(APIC_T_K_G2P, 0);
-    G2p::Args a{sim.pbuf[sim.cur], sim.nalloc, sim.num_particles, dt, sim.cell, sim.W, sim.H, sim.timestamp,
-                sim.props.border_damping, sim.mac_u, sim.mac_v, sim.open_u, sim.open_v, (int)write_keys,
-                sim.keys[0], sim.vals[0], sim.part_maxv, sim.step_src, sim.tail_guard, blocks};
+    // a deferred sort gather: read the records through the permutation and
+    // write the sorted order into the other buffer
+    const bool gather = sim.perm_pending;
+    float* pout = sim.pbuf[gather ? 1 - sim.cur : sim.cur];
+    G2p::Args a{sim.pbuf[sim.cur], pout, gather ? sim.vals[0] : nullptr, sim.nalloc, sim.num_particles, dt,
+                sim.cell, sim.W, sim.H, sim.timestamp, sim.props.border_damping, sim.mac_u, sim.mac_v, sim.open_u,
+                sim.open_v, (int)write_keys, sim.keys[0], sim.vals[0], sim.part_maxv, sim.step_src, sim.tail_guard,
+                blocks};
     emit<G2p>(sim, a, blocks);
+    if (gather) {
+        sim.cur = 1 - sim.cur;
+        sim.perm_pending = false;
+    }
     sim.mark(APIC_T_K_G2P, 1);
     emit<MaxvFinalize>(sim, {sim.part_maxv, blocks, sim.maxv_bits, sim.h_maxv_d, sim.tail_guard,
                              end_substep ? sim.step : nullptr},

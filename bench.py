@@ -47,9 +47,12 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c, levels):
     iteration.
 
     G2p: per particle lane read x, y, time residual (12 B), write the 9
-    particle fields (36 B) and the next sort record (8 B); per fluid cell read
-    one u and one v face value (8 B) and their open flags (2 B).
-    P2gOwner: per particle read the 9 fields (36 B); per grid cell
+    particle fields (36 B); per particle the sort's permutation entry (4 B:
+    the first substep reads its records through it, writing them in sorted
+    order) and the next sort record (8 B); per fluid cell read one u and one v
+    face value (8 B) and their open flags (2 B).
+    P2gOwner: per particle read the 9 fields (36 B) and its permutation entry
+    (4 B: the records are read through the sort's permutation); per grid cell
     write its u face and v face (4 + 4 B), their valid flags (1 + 1 B), the
     density (4 B) and the fluid flag (1 B) -- the reference rewrites every
     face and cell in this stage.
@@ -64,8 +67,8 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c, levels):
     per cell C_l (its fluid flags as words).
     """
     return {
-        "kernel_g2p": 48 * n_alloc + 8 * n_p + 10 * n_f,
-        "kernel_p2g": 36 * n_p + 15 * n_c,
+        "kernel_g2p": 48 * n_alloc + 12 * n_p + 10 * n_f,
+        "kernel_p2g": 40 * n_p + 15 * n_c,
         "kernel_spmv": 28 * n_f + n_c,
         "kernel_pcg_iteration": 68 * n_f + 8 * n_c + sum(89 * f + 4 * c for f, c in levels),
     }
@@ -73,19 +76,20 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c, levels):
 
 def stage_bytes(n_p, n_c, n_f, levels, iters, width, height):
     """Algorithmic bytes of each stage of one substep (compulsory traffic of
-    the stage's inputs and outputs, independent of the implementation): the
-    sort 16 N_p for the keys, 16 N_p per radix pass of 8 bits over
-    2 ceil(log2 max(W, H)) key bits, 76 N_p for the 9-field gather; P2G
-    36 N_p + 14 N_c; the pressure system (three smoothing passes, assembly)
+    the stage's inputs and outputs): the sort 16 N_p for the keys and 16 N_p
+    per radix pass of 8 bits over 2 ceil(log2 max(W, H)) key bits (the
+    records' gather into sorted order is not a pass of its own: P2G reads
+    them through the permutation and G2P writes them in sorted order); P2G
+    40 N_p + 14 N_c; the pressure system (three smoothing passes, assembly)
     92 N_f; the multigrid hierarchy 24 N_f; the PCG `iters` iterations of
     algorithmic_bytes' PCG iteration; the gradient 34 N_c; the four
-    extrapolation passes 80 N_c; G2P 48 N_p + 8 N_c."""
+    extrapolation passes 80 N_c; G2P 52 N_p + 8 N_c."""
     key_bits = 2 * max(1, (max(width, height) - 1).bit_length())
     passes = -(-key_bits // 8)
     it = 68 * n_f + 8 * n_c + sum(89 * f + 4 * c for f, c in levels)
-    return {"sort": (16 + 16 * passes + 76) * n_p, "p2g": 36 * n_p + 14 * n_c, "pressure_build": 92 * n_f,
+    return {"sort": (16 + 16 * passes) * n_p, "p2g": 40 * n_p + 14 * n_c, "pressure_build": 92 * n_f,
             "mg_build": 24 * n_f, "pcg": iters * it, "gradient": 34 * n_c, "extrapolate": 80 * n_c,
-            "g2p": 48 * n_p + 8 * n_c}
+            "g2p": 52 * n_p + 8 * n_c}
 
 
 def stage_rates(stage_ms, n_p, n_c, n_f, levels, iters, width, height):
