@@ -663,7 +663,11 @@ apic_status noalloc_arena(int device, uint8_t** out) {
 
 // Uploads host[0, used) to the batch's device arena (in stream order, before
 // the kernels that read it).
-apic_status batch_upload(apic_batch& B, size_t used) {
+// Uploads the launch tables host[from, used) to the same offsets of the
+// device arena, in stream order: the kernels already issued read the tables
+// before `from`, which stay as they are (a new arena receives all of them).
+apic_status batch_upload(apic_batch& B, size_t from, size_t used) {
+    if (from >= used) return APIC_OK;
     if (B.fixed) {
         // the no-allocation path: a copy from pageable host memory (the
         // source can be reused when the call returns) into the fixed arena,
@@ -672,7 +676,7 @@ apic_status batch_upload(apic_batch& B, size_t used) {
             return fail(APIC_ERR_INVALID_ARGUMENT,
                         "batched launch tables need " + std::to_string(used) + " bytes, more than the " +
                             std::to_string(B.fixed_cap) + " bytes of the no-allocation arena");
-        HIP_TRY(hipMemcpyAsync(B.fixed, B.host.data(), used, hipMemcpyHostToDevice, B.stream));
+        HIP_TRY(hipMemcpyAsync(B.fixed + from, B.host.data() + from, used - from, hipMemcpyHostToDevice, B.stream));
         return APIC_OK;
     }
     if (B.copy_pending) {
@@ -695,9 +699,10 @@ apic_status batch_upload(apic_batch& B, size_t used) {
         const size_t cap = std::max<size_t>(used + used / 2, 1 << 20);
         HIP_TRY(hipMalloc((void**)&B.dev, cap));
         B.dev_cap = cap;
+        from = 0;
     }
-    std::memcpy(B.pinned, B.host.data(), used);
-    HIP_TRY(hipMemcpyAsync(B.dev, B.pinned, used, hipMemcpyHostToDevice, B.stream));
+    std::memcpy(B.pinned + from, B.host.data() + from, used - from);
+    HIP_TRY(hipMemcpyAsync(B.dev + from, B.pinned + from, used - from, hipMemcpyHostToDevice, B.stream));
     HIP_TRY(hipEventRecord(B.e_copied, B.stream));
     B.copy_pending = true;
     return APIC_OK;
@@ -766,6 +771,10 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
             s.rec = nullptr;
         }
         if ((st = batch_compile(B, used, head)) != APIC_OK) return st;
+        // the head runs while the rest is recorded
+        if ((st = batch_upload(B, 0, used)) != APIC_OK) return st;
+        HIP_LAUNCH_TRY(issue(head, B.arena(), B.stream));
+        const size_t used_head = used;
         // one PCG iteration (issued with k = 0, 1, ...) and the check
         for (size_t r = 0; r < act.size(); ++r) {
             Sim& s = *sims[act[r]];
@@ -799,8 +808,7 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
             s.rec = nullptr;
         }
         if ((st = batch_compile(B, used, tail)) != APIC_OK) return st;
-        if ((st = batch_upload(B, used)) != APIC_OK) return st;
-        HIP_LAUNCH_TRY(issue(head, B.arena(), B.stream));
+        if ((st = batch_upload(B, used_head, used)) != APIC_OK) return st;
         int batch_len = 0, cap = 0;
         std::vector<int> prev;
         for (size_t i : act) {
@@ -828,9 +836,11 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
             // Once a quarter of the program's solves have ended, the iteration
             // and check programs are recorded again for the others alone: an
             // ended solve's workgroups would still be dispatched, each to
-            // return after one load.  The GPU is idle here, and a simulation's
-            // launches are the same in either program, so its bits are too.
+            // return after one load.  The next iteration of the program in
+            // hand keeps the GPU busy meanwhile, and a simulation's launches
+            // are the same in either program, so its bits are too.
             if (live.size() * 4 <= in_program * 3) {
+                if (k < check_at && k < cap) HIP_LAUNCH_TRY(issue(iter, B.arena(), B.stream, k++));
                 const size_t used0 = used;
                 std::vector<Recorder::Launch> iter0 = iter, decide0 = decide;
                 for (size_t r = 0; r < live.size(); ++r) {
@@ -856,7 +866,7 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
                     used = used0;
                     iter.swap(iter0);
                     decide.swap(decide0);
-                } else if ((st = batch_upload(B, used)) != APIC_OK) {
+                } else if ((st = batch_upload(B, used0, used)) != APIC_OK) {
                     return st;
                 }
                 in_program = live.size();

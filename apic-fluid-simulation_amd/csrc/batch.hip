@@ -10,17 +10,30 @@ namespace apic {
 void Recorder::clear() {
     groups_.clear();
     chains_.clear();
+    chain_fns_.clear();
+    hint_.clear();
     cur_ = 0;
     overflow_ = false;
 }
 
 void Recorder::begin(uint32_t sim) {
     cur_ = sim;
-    if (chains_.size() <= sim) chains_.resize(sim + 1);
+    if (chains_.size() <= sim) {
+        chains_.resize(sim + 1);
+        chain_fns_.resize(sim + 1);
+    }
 }
 
 int Recorder::find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k,
                           bool bands) {
+    // simulations mostly record the same launches in the same order: try the
+    // group the previous one's launch at this position joined first
+    const size_t pos = chains_[cur_].size();
+    if (pos < hint_.size()) {
+        const int h = hint_[pos];
+        const Group& G = groups_[h];
+        if (G.fn == fn && G.occurrence == occurrence && G.k == k) return h;
+    }
     for (size_t g = 0; g < groups_.size(); ++g) {
         const Group& G = groups_[g];
         if (G.fn == fn && G.occurrence == occurrence && G.k == k) return (int)g;
@@ -40,10 +53,14 @@ void Recorder::add(const void* fn, size_t arg_size, const void* args, uint32_t t
                    bool bands) {
     if (nwg > kMapWgMask || cur_ >= kMaxBatchRows) overflow_ = true;  // kMapWgMask itself stays unused (kMapSkip)
     std::vector<int>& chain = chains_[cur_];
+    std::vector<const void*>& fns = chain_fns_[cur_];
     uint32_t occurrence = 0;
-    for (int g : chain)
-        if (groups_[g].fn == fn) ++occurrence;
+    for (const void* f : fns) occurrence += f == fn;
     const int g = find_or_add(fn, occurrence, arg_size, threads, k, bands);
+    const size_t pos = chain.size();
+    if (hint_.size() <= pos) hint_.resize(pos + 1);
+    hint_[pos] = g;
+    fns.push_back(fn);
     Group& G = groups_[g];
     const uint8_t* p = static_cast<const uint8_t*>(args);
     G.args.insert(G.args.end(), p, p + arg_size);
@@ -87,12 +104,16 @@ std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size
     for (int g : order) {
         const Group& G = groups_[g];
         std::vector<uint32_t>& m = maps[g];
+        size_t total_wg = 0;
+        for (uint32_t n : G.nwg) total_wg += n;
         if (!G.bands) {
+            m.reserve(total_wg);
             for (size_t row = 0; row < G.nwg.size(); ++row)
                 for (uint32_t wg = 0; wg < G.nwg[row]; ++wg) m.push_back(((uint32_t)row << kMapWgBits) | wg);
             continue;
         }
         std::vector<uint32_t> q[kXcds];
+        for (uint32_t x = 0; x < kXcds; ++x) q[x].reserve(total_wg / kXcds + G.nwg.size());
         uint32_t rot = 0;
         for (size_t row = 0; row < G.nwg.size(); ++row) {
             const uint32_t n = G.nwg[row], qn = n / kXcds, rn = n % kXcds;

@@ -115,6 +115,8 @@ private:
     };
     std::vector<Group> groups_;
     std::vector<std::vector<int>> chains_;  // per simulation: its groups in launch order
+    std::vector<std::vector<const void*>> chain_fns_;  // and their kernels (the occurrence count scans them)
+    std::vector<int> hint_;  // per chain position: the group the last simulation's launch there joined
     uint32_t cur_ = 0;
     bool overflow_ = false;
     int find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, uint32_t threads, int k, bool bands);

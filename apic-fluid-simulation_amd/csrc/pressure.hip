@@ -389,11 +389,13 @@ __device__ inline T block_sum(T v, T* scratch);
 
 // Whether pred holds on any thread of the workgroup: a ballot per wave, one
 // LDS slot per wave and one barrier (which also orders the LDS stores made
-// before it, as __syncthreads does).  At most one call per kernel: a second
-// call could overwrite the slots before every thread has read them.
+// before it, as __syncthreads does).  The caller owns the slots (NT / 64
+// words of LDS): a wave that has returned may write them again only after a
+// barrier that every wave reaches after its read, so each call site declares
+// its own, and a call site reached more than once needs such a barrier
+// between the reads and the next call.
 template <int NT>
-__device__ __forceinline__ bool block_any(bool pred) {
-    __shared__ uint32_t slots[NT / 64];
+__device__ __forceinline__ bool block_any(bool pred, uint32_t* slots) {
     const bool wave_any = __ballot(pred) != 0ull;
     if ((threadIdx.x & 63) == 0) slots[threadIdx.x >> 6] = wave_any;
     __syncthreads();
@@ -1984,7 +1986,8 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     bool any = false;
 #pragma unroll
     for (int q = 0; q < G::kCPer; ++q) any = any || (cin[q] && cfl[q] != 0);
-    if (!block_any<NT>(any)) {
+    __shared__ uint32_t any_slots[NT / 64];  // (one call per launch)
+    if (!block_any<NT>(any, any_slots)) {
         if constexpr (AX) {  // no fluid cell in the tile: a zero partial, as axpy_tile's
             if (threadIdx.x == 0) ax.part_rr_out[wg] = 0.0;
         }
@@ -2327,7 +2330,8 @@ __device__ __forceinline__ void vdown2(const LevelView& F, const LevelView& C, c
         }
     }
     // skip: no fluid cell of level l + 1 in the tile
-    if (!block_any<NT>(region_any_fluid<kP2C, NT, kP2T>(gc, kP2CR))) return;
+    __shared__ uint32_t any_slots[NT / 64];  // (one call per launch)
+    if (!block_any<NT>(region_any_fluid<kP2C, NT, kP2T>(gc, kP2CR), any_slots)) return;
     // level l: black cells out to ring kP2FR - 1 (own value zero before the sweep)
     for (int i = threadIdx.x; i < kP2F * kP2F; i += NT) {
         const int lx = i % kP2F, ly = i / kP2F;
@@ -2465,7 +2469,8 @@ __device__ __forceinline__ void vup2(const LevelView& F, const LevelView& C, con
     }
     if (dn) return;
     // skip: no fluid cell of level l + 1 in the tile
-    if (!block_any<NT>(region_any_fluid<kU2C, NT, kP2T>(gc, kU2CR))) return;
+    __shared__ uint32_t any_slots[NT / 64];  // (one call per launch)
+    if (!block_any<NT>(region_any_fluid<kU2C, NT, kP2T>(gc, kU2CR), any_slots)) return;
     // level l + 1: e plus the parent's error over the region (the diagonal's slot)
     for (int i = threadIdx.x; i < kU2C * kU2C; i += NT) {
         // (every region cell's parent lies in de: the load needs no flag)
@@ -2630,7 +2635,8 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         return;
     }
     // a tile without a fluid cell (see vdown) only reports a zero partial
-    if (!block_any<NT>(cin && cfv != 0)) {
+    __shared__ uint32_t any_slots[NT / 64];  // (one call per launch)
+    if (!block_any<NT>(cin && cfv != 0, any_slots)) {
         if (threadIdx.x == 0) {
             if (dot == 2) { part_gamma_d[wg] = 0.0; part_asz_d[wg] = 0.0; }
             else if (dot == 1) part_gamma[wg] = 0.0f;
