@@ -214,22 +214,36 @@ __device__ __forceinline__ void p2g_owner(
             for (uint32_t idx = lo; idx < hi; ++idx) s_cell[idx - base] = (uint8_t)tid;
         }
         __syncthreads();
-        for (uint32_t k = tid; k < n; k += kOwnCells) {
-            const uint32_t idx = base + k;
-            const int lo = s_cell[k];  // the window cell holding idx
-            const uint32_t q = s_src[lo] + (idx - s_off[lo]);  // position in cell order
+        // a thread's kOwnCap / kOwnCells slots: their positions in cell order,
+        // then the permutation entries, then the fields, each stage's loads
+        // issued together
+        constexpr int kPer = kOwnCap / kOwnCells;
+        static_assert(kPer * kOwnCells == kOwnCap, "whole slots per thread");
+        uint32_t src[kPer];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t k = (uint32_t)tid + (uint32_t)(j * kOwnCells);
+            const uint32_t kk = k < n ? k : 0u;
+            const int lo = s_cell[kk];  // the window cell holding base + kk
+            src[j] = s_src[lo] + (base + kk - s_off[lo]);  // position in cell order
+        }
+        if (order) {
             // (a permutation entry past the records, written through the
             // field API, reads the last record, as the sort's gather does)
-            const size_t src = order ? min(order[q], np - 1) : q;
-            s_buf[0 * kOwnCap + k] = pb[(size_t)P_X * nalloc + src];
-            s_buf[1 * kOwnCap + k] = pb[(size_t)P_Y * nalloc + src];
-            s_buf[2 * kOwnCap + k] = pb[(size_t)V_X * nalloc + src];
-            s_buf[3 * kOwnCap + k] = pb[(size_t)V_Y * nalloc + src];
-            s_buf[4 * kOwnCap + k] = pb[(size_t)CU_X * nalloc + src];
-            s_buf[5 * kOwnCap + k] = pb[(size_t)CU_Y * nalloc + src];
-            s_buf[6 * kOwnCap + k] = pb[(size_t)CV_X * nalloc + src];
-            s_buf[7 * kOwnCap + k] = pb[(size_t)CV_Y * nalloc + src];
-            s_buf[8 * kOwnCap + k] = pb[(size_t)T_RES * nalloc + src];
+#pragma unroll
+            for (int j = 0; j < kPer; ++j) src[j] = min(order[src[j]], np - 1);
+        }
+        float v[kPer][kOwnFields];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j)
+#pragma unroll
+            for (int f = 0; f < kOwnFields; ++f) v[j][f] = pb[(size_t)f * nalloc + src[j]];
+#pragma unroll
+        for (int j = 0; j < kPer; ++j) {
+            const uint32_t k = (uint32_t)tid + (uint32_t)(j * kOwnCells);
+            if (k < n)
+#pragma unroll
+                for (int f = 0; f < kOwnFields; ++f) s_buf[f * kOwnCap + k] = v[j][f];
         }
         __syncthreads();
         const uint32_t lo = max(my0, base), hi = min(my1, base + n);
