@@ -371,7 +371,7 @@ inline void emit(Sim& s, const typename K::Args& a, uint32_t nwg, int k = 0) {
     if (nwg == 0) return;
     if (s.rec) {
         s.rec->add(reinterpret_cast<const void*>(&batch_kernel<K>), sizeof(a), &a, K::kThreads, nwg, k,
-                   !round_robin<K>::value);
+                   !round_robin<K>::value, deal_chunk<K>(a));
         return;
     }
     hipLaunchKernelGGL(one_kernel<K>, dim3(nwg), dim3(K::kThreads), 0, s.stream, a, k);

@@ -50,7 +50,7 @@ int Recorder::find_or_add(const void* fn, uint32_t occurrence, size_t arg_size, 
 }
 
 void Recorder::add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k,
-                   bool bands) {
+                   bool bands, uint32_t chunk) {
     if (nwg > kMapWgMask || cur_ >= kMaxBatchRows) overflow_ = true;  // kMapWgMask itself stays unused (kMapSkip)
     std::vector<int>& chain = chains_[cur_];
     std::vector<const void*>& fns = chain_fns_[cur_];
@@ -65,6 +65,7 @@ void Recorder::add(const void* fn, size_t arg_size, const void* args, uint32_t t
     const uint8_t* p = static_cast<const uint8_t*>(args);
     G.args.insert(G.args.end(), p, p + arg_size);
     G.nwg.push_back(nwg);
+    G.chunk.push_back(chunk);
     chain.push_back(g);
 }
 
@@ -98,8 +99,10 @@ std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size
     // XCD bands: workgroup w of a simulation with n workgroups goes to the
     // XCD whose band holds w (as xcd_band maps them for one simulation),
     // rotated by the workgroups laid out before it so that the queues stay
-    // even; the grid interleaves the eight queues (position 8 i + x runs the
-    // i-th entry of XCD x), padded with entries that run nothing.
+    // even; a simulation whose launch deals rows (chunk > 0) sends its row i
+    // to XCD (i + rotation) % kXcds instead, rotated by its rows.  The grid
+    // interleaves the eight queues (position 8 i + x runs the i-th entry of
+    // XCD x), padded with entries that run nothing.
     std::vector<std::vector<uint32_t>> maps(ng);
     for (int g : order) {
         const Group& G = groups_[g];
@@ -116,7 +119,17 @@ std::vector<Recorder::Launch> Recorder::compile(std::vector<uint8_t>& host, size
         for (uint32_t x = 0; x < kXcds; ++x) q[x].reserve(total_wg / kXcds + G.nwg.size());
         uint32_t rot = 0;
         for (size_t row = 0; row < G.nwg.size(); ++row) {
-            const uint32_t n = G.nwg[row], qn = n / kXcds, rn = n % kXcds;
+            const uint32_t n = G.nwg[row], qn = n / kXcds, rn = n % kXcds, c = G.chunk[row];
+            if (c > 0) {
+                const uint32_t rows = (n + c - 1) / c;
+                for (uint32_t i = 0; i < rows; ++i) {
+                    std::vector<uint32_t>& dst = q[(i + rot) % kXcds];
+                    for (uint32_t j = i * c; j < n && j < (i + 1) * c; ++j)
+                        dst.push_back(((uint32_t)row << kMapWgBits) | j);
+                }
+                rot = (rot + rows) % kXcds;
+                continue;
+            }
             for (uint32_t x = 0; x < kXcds; ++x) {
                 const uint32_t len = qn + (x < rn ? 1u : 0u), first = x < rn ? x * (qn + 1) : rn * (qn + 1) + (x - rn) * qn;
                 std::vector<uint32_t>& dst = q[(x + rot) % kXcds];

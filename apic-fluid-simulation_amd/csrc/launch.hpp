@@ -33,10 +33,25 @@ constexpr uint32_t kMaxBatchRows = 1u << (32 - kMapWgBits);  // simulations per 
 // rows a stencil reads mostly come from the same L2.  A bijection of
 // [0, n): every logical workgroup still runs exactly once.
 constexpr uint32_t kXcds = 8;
-__device__ __forceinline__ uint32_t xcd_band(uint32_t hw, uint32_t n) {
+__host__ __device__ __forceinline__ uint32_t xcd_band(uint32_t hw, uint32_t n) {
     const uint32_t xcd = hw % kXcds, slot = hw / kXcds;
     const uint32_t q = n / kXcds, r = n % kXcds;
     return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+// A kernel over a row-major grid of tiles declares kDealRows = true (and has
+// Args::tiles_x): its tile rows are dealt to the XCDs in turn (row i to XCD
+// i % kXcds), so that a fluid region covering only some rows of the grid (a
+// dam break's lower part) still spreads over every XCD, while a tile's left
+// and right neighbours stay in its L2.  Workgroups past the last whole round
+// of kXcds rows are laid out in bands.  A bijection of [0, n).
+__host__ __device__ __forceinline__ uint32_t xcd_deal(uint32_t hw, uint32_t n, uint32_t c) {
+    const uint32_t full = c ? n / (kXcds * c) * (kXcds * c) : 0;
+    if (hw < full) {
+        const uint32_t xcd = hw % kXcds, slot = hw / kXcds;
+        return ((slot / c) * kXcds + xcd) * c + slot % c;
+    }
+    return full + xcd_band(hw - full, n - full);
 }
 
 // A kernel whose work per workgroup follows the particles (uneven over the
@@ -47,9 +62,25 @@ struct round_robin : std::false_type {};
 template <class K>
 struct round_robin<K, std::void_t<decltype(K::kRoundRobin)>> : std::bool_constant<K::kRoundRobin> {};
 
+template <class K, class = void>
+struct deal_rows : std::false_type {};
+template <class K>
+struct deal_rows<K, std::void_t<decltype(K::kDealRows)>> : std::bool_constant<K::kDealRows> {};
+
+// The workgroups of one tile row for a kDealRows kernel, else 0 (bands).
+template <class K>
+__host__ __device__ __forceinline__ uint32_t deal_chunk(const typename K::Args& a) {
+    if constexpr (deal_rows<K>::value) return a.tiles_x;
+    else return 0;
+}
+
 template <class K>
 __global__ __launch_bounds__(K::kThreads) void one_kernel(typename K::Args a, int k) {
-    K::run(a, round_robin<K>::value ? blockIdx.x : xcd_band(blockIdx.x, gridDim.x), k);
+    uint32_t wg;
+    if constexpr (round_robin<K>::value) wg = blockIdx.x;
+    else if constexpr (deal_rows<K>::value) wg = xcd_deal(blockIdx.x, gridDim.x, a.tiles_x);
+    else wg = xcd_band(blockIdx.x, gridDim.x);
+    K::run(a, wg, k);
 }
 
 // The argument table is read through the constant address space, as kernel
@@ -88,9 +119,11 @@ public:
     };
 
     void begin(uint32_t sim);  // the following launches belong to simulation `sim`
-    // bands: lay each simulation's workgroups out in XCD bands (as one_kernel
-    // does), else in dispatch order
-    void add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k, bool bands);
+    // bands: lay each simulation's workgroups out on the XCDs as one_kernel
+    // does (in bands, or dealt in rows of `chunk` workgroups when chunk > 0),
+    // else in dispatch order
+    void add(const void* fn, size_t arg_size, const void* args, uint32_t threads, uint32_t nwg, int k, bool bands,
+             uint32_t chunk = 0);
     bool empty() const { return groups_.empty(); }
     // a simulation's launch had more workgroups than a map entry can name
     bool overflow() const { return overflow_; }
@@ -112,6 +145,7 @@ private:
         bool bands;
         std::vector<uint8_t> args;
         std::vector<uint32_t> nwg;  // per table row
+        std::vector<uint32_t> chunk;  // per table row: workgroups of a dealt row (0: bands)
     };
     std::vector<Group> groups_;
     std::vector<std::vector<int>> chains_;  // per simulation: its groups in launch order

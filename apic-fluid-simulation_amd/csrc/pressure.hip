@@ -1849,6 +1849,7 @@ template <int NT>
 struct AxpyTile {
     static constexpr int kThreads = NT;
     using Args = AxpyTileArgs;
+    static constexpr bool kDealRows = true;  // its tile rows are dealt to the XCDs in turn
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         axpy_tile<NT>(a.L, a.ax, a.st, a.tiles_x, k, wg);
     }
@@ -2136,6 +2137,7 @@ template <int T, int NT = 256, bool AX = false>
 struct VDown {
     static constexpr int kThreads = NT;
     using Args = VDownArgs;
+    static constexpr bool kDealRows = true;  // its tile rows are dealt to the XCDs in turn
     // k: the PCG iteration whose update the residual test checks
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
         const bool due = a.every > 0 && (k + 1) % a.every == 0;
@@ -2410,6 +2412,7 @@ struct VDown2Args {
 struct VDown2 {
     static constexpr int kThreads = kP2Threads;
     using Args = VDown2Args;
+    static constexpr bool kDealRows = true;  // its tile rows are dealt to the XCDs in turn
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
         vdown2(a.F, a.C, a.D, a.fuse_red_d, a.done, a.tiles_x, a.st, a.part_rr_t, a.ntiles, wg);
     }
@@ -2552,6 +2555,7 @@ struct VUp2Args {
 struct VUp2 {
     static constexpr int kThreads = kP2Threads;
     using Args = VUp2Args;
+    static constexpr bool kDealRows = true;  // its tile rows are dealt to the XCDs in turn
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
         vup2(a.F, a.C, a.D, a.f_out, a.c_out, a.done, a.tiles_x, wg);
     }
@@ -2739,6 +2743,7 @@ template <int T, int NT = 256>
 struct VUp {
     static constexpr int kThreads = NT;
     using Args = VUpArgs;
+    static constexpr bool kDealRows = true;  // its tile rows are dealt to the XCDs in turn
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
         vup<T, NT>(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, a.spec,
                a.e_out, a.r_home, a.r_in_alt, wg);
