@@ -425,10 +425,45 @@ __device__ __forceinline__ double row_ror(double v) {
     const int lo = row_ror<N>((int)(b & 0xffffffffll)), hi = row_ror<N>((int)(b >> 32));
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned int)lo);
 }
+// The partners of the first two steps (lane xor 32, lane xor 16) come from
+// gfx950's permlane swaps, VALU moves instead of two LDS permutes.
+// permlane32_swap(x, x) returns {x[l] below lane 32 and x[l - 32] above,
+// x[l + 32] below and x[l] above}; permlane16_swap(x, x) the same for the
+// rows pairs (0, 1) and (2, 3).  A lane takes the half that is not its own.
+template <int X>
+__device__ __forceinline__ uint32_t lane_xor_word(uint32_t x) {
+    static_assert(X == 32 || X == 16, "permlane swaps exchange halves or row pairs");
+    const bool upper = (__lane_id() & X) != 0;
+    if constexpr (X == 32) {
+        const auto r = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+        return upper ? r[0] : r[1];
+    } else {
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        return upper ? r[0] : r[1];
+    }
+}
+template <int X>
+__device__ __forceinline__ float lane_xor(float v) {
+    return __uint_as_float(lane_xor_word<X>(__float_as_uint(v)));
+}
+template <int X>
+__device__ __forceinline__ int lane_xor(int v) {
+    return (int)lane_xor_word<X>((uint32_t)v);
+}
+template <int X>
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v) {
+    return lane_xor_word<X>(v);
+}
+template <int X>
+__device__ __forceinline__ double lane_xor(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    const uint32_t lo = lane_xor_word<X>((uint32_t)b), hi = lane_xor_word<X>((uint32_t)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 template <typename T, class Op>
 __device__ __forceinline__ T wave_butterfly(T v, Op op) {
-    v = op(v, __shfl_xor(v, 32, 64));
-    v = op(v, __shfl_xor(v, 16, 64));
+    v = op(v, lane_xor<32>(v));
+    v = op(v, lane_xor<16>(v));
     v = op(v, row_ror<8>(v));
     v = op(v, row_ror<4>(v));
     v = op(v, row_ror<2>(v));
@@ -583,33 +618,13 @@ __device__ void coarse_cg_block(const LevelView& L, float* cg_s, float* cg_ap, f
     }
 }
 
-// Level views of the one-workgroup bottom: the levels themselves (global
-// memory), or copies of levels [first, count) in LDS, laid out level after
-// level as diag, inv_diag, px, py, r, e over the padded range
-// [0, cells + 2w), fluid flags after them, then the coarse CG vectors.
+// Level views of the one-workgroup bottom in global memory (the LDS bottom
+// copies levels [first, count) in, laid out level after level as diag,
+// inv_diag, px, py, r, e over the padded range [0, cells + 2w), fluid flags
+// after them, then the coarse CG vectors).
 struct GlobalViews {
     const LevelSet* ls;
     __device__ LevelView view(int l) const { return ls->lv[l]; }
-};
-
-struct LdsViews {
-    const LevelSet* ls;
-    int first;
-    float* f;
-    uint8_t* fl;
-    __device__ LevelView view(int l) const {
-        size_t fo = 0, bo = 0;
-        for (int k = first; k < l; ++k) {
-            fo += 6 * bottom_padded(ls->lv[k]);
-            bo += bottom_flags(ls->lv[k]);
-        }
-        LevelView L = ls->lv[l];
-        const size_t p = bottom_padded(L);
-        L.diag = f + fo; L.inv_diag = f + fo + p; L.px = f + fo + 2 * p; L.py = f + fo + 3 * p;
-        L.r = f + fo + 4 * p; L.e = f + fo + 5 * p;
-        L.fluid = fl + bo;
-        return L;
-    }
 };
 
 // solve_coarse_exact (src/apic.rs:618-704) for a level of at most 64 cells,
@@ -886,41 +901,70 @@ struct BottomGlobal {
 // copied back (the finer level's prolongation reads them).
 static_assert(kBottomFirstCells <= (uint32_t)kBottomThreads, "one cell per thread on an LDS bottom");
 
-// Offsets of a bottom level's arrays in the LDS image.
+// Offsets of a bottom level's arrays in the LDS image (the host computes them
+// from the level sizes; the bottom's levels lie in global memory in the same
+// layout, from the first bottom level's diag and flags).
 struct BLev {
-    int32_t w, h, n;    // sides and cells
+    int32_t w, h, n;                // sides and cells
     uint32_t dg, id, px, py, r, e;  // float offsets of the padded arrays
-    uint32_t fl;        // offset of the fluid flags
+    uint32_t fl;                    // offset of the fluid flags
+    uint32_t wmul;                  // ceil(2^32 / w): y = umulhi(c, wmul) for w > 1
 };
 
-__device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const int32_t* __restrict__ done,
-                                           bool rel) {
+// The row of cell c of a bottom level: c / w by a multiply-high, exact for
+// c * w < 2^32 (a bottom level has at most kBottomFirstCells cells).
+__device__ __forceinline__ int bottom_row(const BLev& L, int c) {
+    return L.w == 1 ? c : (int)__umulhi((uint32_t)c, L.wmul);
+}
+
+// Makes the compiler produce the given loaded values at this point: their
+// loads are issued before the first wait and cannot be moved into the
+// branches that use them (one LDS round trip for a phase instead of one per
+// branch).
+#define APIC_TIE4(a, b, c, d) asm volatile("" : "+v"(a), "+v"(b), "+v"(c), "+v"(d))
+#define APIC_TIE2(a, b) asm volatile("" : "+v"(a), "+v"(b))
+#define APIC_TIE1(a) asm volatile("" : "+v"(a))
+
+// sor_cell's update from the cell's coefficients, own value and neighbour
+// values: own when the cell has no inverse diagonal (the same value sor_cell
+// leaves), computed without a branch.
+__device__ __forceinline__ float bottom_sor(float own, float id, float r, float px, float py, float pxm, float pym,
+                                            float er, float et, float el, float eb) {
+    float sum = r;
+    sum = fmaf(-px, er, sum);
+    sum = fmaf(-py, et, sum);
+    sum = fmaf(-pxm, el, sum);
+    sum = fmaf(-pym, eb, sum);
+    const float gs = sum * id;
+    const float nv = own + (kOmega * (gs - own));
+    return id != 0.0f ? nv : own;
+}
+
+struct BottomLdsArgs {
+    BLev tab[16];          // by level index
+    float* img;            // the levels' arrays in global memory (the LDS image's layout)
+    const uint32_t* fimg;  // their flag bytes, as words
+    uint32_t total;        // floats of the image
+    uint32_t fwords;       // flag words of the image
+    uint32_t cg;           // float offset of the coarse CG vectors (after the image)
+    int first, last;       // the bottom's levels
+    const int32_t* done;
+    int rel;               // the relative mode's coarse exits
+};
+
+__device__ __forceinline__ void bottom_lds(const BottomLdsArgs& a) {
     __shared__ float f[kBottomLdsFloats];
-    __shared__ __attribute__((aligned(16))) uint8_t fl[kBottomLdsFlags];  // copied in as words
+    __shared__ __attribute__((aligned(16))) uint32_t fw[kBottomLdsFlags / 4];  // copied in as words
     __shared__ float scratch[kBottomThreads / 64];
-    __shared__ BLev tab[16];  // the levels' offsets, by level index
-    const int32_t dn = *done;
-    const int last = ls.count - 1;
+    const uint8_t* fl = reinterpret_cast<const uint8_t*>(fw);
+    const int32_t dn = *a.done;
+    const int first = a.first, last = a.last;
     const int c = (int)threadIdx.x;
-    // Copy-in: the levels' arrays lie in global memory in the image's layout
-    // (two blocks from the first level's diag and flags), so the image is one
-    // stride of floats and one of flag words; a thread's loads all go out
-    // before its stores.
+    // Copy-in: one stride of floats and one of flag words; a thread's loads
+    // all go out before its stores.
     {
-        uint32_t total = 0, ftotal = 0;
-        for (int l = first; l <= last; ++l) {
-            const LevelView& G = ls.lv[l];
-            if (threadIdx.x == 0) {
-                const uint32_t p = (uint32_t)bottom_padded(G);
-                tab[l] = {(int32_t)G.w, (int32_t)G.h, (int32_t)G.cells, total, total + p, total + 2 * p,
-                          total + 3 * p, total + 4 * p, total + 5 * p, ftotal};
-            }
-            total += 6 * (uint32_t)bottom_padded(G);
-            ftotal += (uint32_t)bottom_flags(G);
-        }
-        const float* __restrict__ img = ls.lv[first].diag;
-        const uint32_t* __restrict__ fimg = reinterpret_cast<const uint32_t*>(ls.lv[first].fluid);
-        uint32_t* fw = reinterpret_cast<uint32_t*>(fl);
+        const uint32_t total = a.total, fwords = a.fwords;
+        const float* __restrict__ img = a.img;
         constexpr int kPer = 12;
         for (uint32_t i0 = threadIdx.x; i0 < total; i0 += kPer * kBottomThreads) {
             float v[kPer];
@@ -930,110 +974,125 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
                 v[j] = img[i < total ? i : 0];
             }
             uint32_t fv = 0;
-            if (i0 == threadIdx.x && threadIdx.x < ftotal / 4) fv = fimg[threadIdx.x];
+            if (i0 == threadIdx.x && threadIdx.x < fwords) fv = a.fimg[threadIdx.x];
 #pragma unroll
             for (int j = 0; j < kPer; ++j) {
                 const uint32_t i = i0 + j * kBottomThreads;
                 if (i < total) f[i] = v[j];
             }
-            if (i0 == threadIdx.x && threadIdx.x < ftotal / 4) fw[threadIdx.x] = fv;
+            if (i0 == threadIdx.x && threadIdx.x < fwords) fw[threadIdx.x] = fv;
         }
-        for (uint32_t i = threadIdx.x + kBottomThreads; i < ftotal / 4; i += kBottomThreads) fw[i] = fimg[i];
+        for (uint32_t i = threadIdx.x + kBottomThreads; i < fwords; i += kBottomThreads) fw[i] = a.fimg[i];
     }
     if (dn) return;
-    LdsViews V{&ls, first, f, fl};
-    const size_t pl = bottom_padded(ls.lv[last]);
-    float* cg_s = V.view(last).e + pl;  // after the image: the coarse CG vectors
+    const BLev& CL = a.tab[last];
+    const uint32_t pl = (uint32_t)(((uint32_t)CL.n + 2u * (uint32_t)CL.w + 7u) / 8u * 8u);
+    float* cg_s = f + a.cg;  // the coarse CG vectors
     float* cg_ap = cg_s + pl;
     float* cg_r = cg_ap + pl;
     float* sprod = cg_r + pl;
     // the CG vectors' padding rows are read as neighbours: zero, as in global memory
-    for (uint32_t i = threadIdx.x; i < 3 * pl + ls.lv[last].cells; i += blockDim.x) cg_s[i] = 0.0f;
+    for (uint32_t i = threadIdx.x; i < 3 * pl + (uint32_t)CL.n; i += blockDim.x) cg_s[i] = 0.0f;
     __syncthreads();
 
-    // sor_cell's update from the cell's coefficients, own value and neighbour values
-    auto sor = [](float own, float id, float r, float px, float py, float pxm, float pym, float er, float et,
-                  float el, float eb) {
-        if (id != 0.0f) {
-            float sum = r;
-            sum = fmaf(-px, er, sum);
-            sum = fmaf(-py, et, sum);
-            sum = fmaf(-pxm, el, sum);
-            sum = fmaf(-pym, eb, sum);
-            const float gs = sum * id;
-            own = own + (kOmega * (gs - own));
+    // restrict_cell: coarse cell c of C from level F, with C's red sweep
+    // fused when asked.  Every child's operands load unconditionally (a
+    // child past the edge of an odd-sided level loads the first child's and
+    // adds nothing; a non-fluid child adds +0, which leaves the sum as
+    // skipping it would: the sum starts at +0 and never becomes -0).
+    auto restrict_to = [&](const BLev& F, const BLev& C, bool fuse_red) {
+        if ((c & ~63) >= C.n) return;  // a wave without cells of C
+        const bool mine = c < C.n;
+        const int cc = mine ? c : 0;
+        const int cy = bottom_row(C, cc), cx = cc - cy * C.w;
+        const int pcc = cc + C.w;
+        float b[4], dg[4], e0[4], pxa[4], e1[4], pya[4], ew[4], pxm[4], em[4], pym[4], es[4];
+        uint32_t cfl[4];
+        bool in[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int dx = k & 1, dy = k >> 1;
+            in[k] = cx * 2 + dx < F.w && cy * 2 + dy < F.h;
+            const int rf = in[k] ? (cy * 2 + dy) * F.w + cx * 2 + dx : cy * 2 * F.w + cx * 2, pf = rf + F.w;
+            b[k] = f[F.r + pf];
+            dg[k] = f[F.dg + pf];
+            e0[k] = f[F.e + pf];
+            pxa[k] = f[F.px + pf];
+            e1[k] = f[F.e + pf + 1];
+            pya[k] = f[F.py + pf];
+            ew[k] = f[F.e + pf + F.w];
+            pxm[k] = f[F.px + pf - 1];
+            em[k] = f[F.e + pf - 1];
+            pym[k] = f[F.py + pf - F.w];
+            es[k] = f[F.e + pf - F.w];
+            cfl[k] = fl[F.fl + rf];
         }
-        return own;
-    };
-    // restrict_cell: cell c = (cx, cy) of C from level F
-    auto restrict_to = [&](const BLev& F, const BLev& C, int cx, int cy, bool fuse_red) {
-        if (c >= C.n) return;
-        const int pcc = c + C.w;
-        const uint8_t cfl = fl[C.fl + c];  // (gates the stores, not the children's loads)
+        float id = f[C.id + pcc];
+        uint32_t own_fl = fl[C.fl + cc];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            APIC_TIE4(b[k], dg[k], e0[k], pxa[k]);
+            APIC_TIE4(e1[k], pya[k], ew[k], pxm[k]);
+            APIC_TIE4(em[k], pym[k], es[k], cfl[k]);
+        }
+        APIC_TIE2(id, own_fl);
         float sum = 0.0f;
 #pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-                // a child past the edge of an odd-sided level adds nothing;
-                // its loads go to the first child (always inside the level)
-                const bool in = cx * 2 + dx < F.w && cy * 2 + dy < F.h;
-                const int rf = in ? (cy * 2 + dy) * F.w + cx * 2 + dx : cy * 2 * F.w + cx * 2, pf = rf + F.w;
-            // every child's operands load unconditionally (they go out
-            // together); a non-fluid child adds +0, which leaves the sum as
-            // skipping it would: the sum starts at +0 and never becomes -0
-                const float b = f[F.r + pf];
-                float ax = f[F.dg + pf] * f[F.e + pf];
-                ax += f[F.px + pf] * f[F.e + pf + 1];
-                ax += f[F.py + pf] * f[F.e + pf + F.w];
-                ax += f[F.px + pf - 1] * f[F.e + pf - 1];
-                ax += f[F.py + pf - F.w] * f[F.e + pf - F.w];
-                sum += (in && fl[F.fl + rf]) ? b - ax : 0.0f;
-            }
-        const float id = f[C.id + pcc];
-        const bool red = ((cx + cy) & 1) == 0;
-        f[C.r + pcc] = cfl ? sum : 0.0f;
-        f[C.e + pcc] = (cfl && fuse_red && red && id != 0.0f) ? kOmega * (sum * id) : 0.0f;
-    };
-    auto coords = [&](const BLev& L, int& x, int& y) {
-        const bool mine = c < L.n;
-        x = mine ? c % L.w : 0;
-        y = mine ? c / L.w : 0;
-        return mine;
+        for (int k = 0; k < 4; ++k) {
+            float ax = dg[k] * e0[k];
+            ax += pxa[k] * e1[k];
+            ax += pya[k] * ew[k];
+            ax += pxm[k] * em[k];
+            ax += pym[k] * es[k];
+            sum += (in[k] && cfl[k]) ? b[k] - ax : 0.0f;
+        }
+        if (mine) {
+            const bool red = ((cx + cy) & 1) == 0;
+            f[C.r + pcc] = own_fl ? sum : 0.0f;
+            f[C.e + pcc] = (own_fl && fuse_red && red && id != 0.0f) ? kOmega * (sum * id) : 0.0f;
+        }
     };
 
     // down: restriction into level l, then its black sweep (the red one was
-    // fused into the restriction)
+    // fused into the restriction, or into the kernel above for the first)
     for (int l = first; l < last; ++l) {
-        const BLev L = tab[l];
-        int x, y;
-        const bool mine = coords(L, x, y);
+        const BLev L = a.tab[l];
         if (l > first) {
-            restrict_to(tab[l - 1], L, x, y, true);
+            restrict_to(a.tab[l - 1], L, true);
             __syncthreads();
         }
-        if (mine && ((x + y) & 1) == 1) {  // (the colour alone gates the loads)
-            const int pc = c + L.w;
-            const float v = sor(f[L.e + pc], f[L.id + pc], f[L.r + pc], f[L.px + pc], f[L.py + pc],
-                                f[L.px + pc - 1], f[L.py + pc - L.w], f[L.e + pc + 1], f[L.e + pc + L.w],
-                                f[L.e + pc - 1], f[L.e + pc - L.w]);
-            if (fl[L.fl + c]) f[L.e + pc] = v;
+        if ((c & ~63) < L.n) {  // (a wave without cells of the level skips the sweep)
+            const bool mine = c < L.n;
+            const int cc = mine ? c : 0;
+            const int y = bottom_row(L, cc), x = cc - y * L.w;
+            const int pc = cc + L.w;
+            float own = f[L.e + pc], id = f[L.id + pc], r = f[L.r + pc], px = f[L.px + pc], py = f[L.py + pc];
+            float pxm = f[L.px + pc - 1], pym = f[L.py + pc - L.w];
+            float er = f[L.e + pc + 1], et = f[L.e + pc + L.w], el = f[L.e + pc - 1], eb = f[L.e + pc - L.w];
+            uint32_t flg = fl[L.fl + cc];
+            APIC_TIE4(own, id, r, px);
+            APIC_TIE4(py, pxm, pym, er);
+            APIC_TIE4(et, el, eb, flg);
+            const float v = bottom_sor(own, id, r, px, py, pxm, pym, er, et, el, eb);
+            if (mine && ((x + y) & 1) == 1 && flg) f[L.e + pc] = v;
         }
         __syncthreads();
     }
-    const LevelView C = V.view(last);
     if (last > first) {
-        int x, y;
-        coords(tab[last], x, y);
-        restrict_to(tab[last - 1], tab[last], x, y, false);
+        restrict_to(a.tab[last - 1], CL, false);
         __syncthreads();
     }
+    LevelView C{};
+    C.w = (uint32_t)CL.w; C.h = (uint32_t)CL.h; C.cells = (uint32_t)CL.n;
+    C.diag = f + CL.dg; C.inv_diag = f + CL.id; C.px = f + CL.px; C.py = f + CL.py;
+    C.r = f + CL.r; C.e = f + CL.e;
+    C.fluid = const_cast<uint8_t*>(fl) + CL.fl;
     if (C.cells <= 64) {
         if (threadIdx.x < 64) {
-            if (rel) coarse_cg_rel_wave(C);
+            if (a.rel) coarse_cg_rel_wave(C);
             else coarse_cg_wave(C);
         }
-    } else if (rel) {
+    } else if (a.rel) {
         coarse_cg_rel_block(C, cg_s, cg_ap, cg_r);
     } else {
         coarse_cg_block(C, cg_s, cg_ap, cg_r, sprod, scratch);
@@ -1041,64 +1100,79 @@ __device__ __forceinline__ void bottom_lds(const LevelSet& ls, int first, const 
     __syncthreads();
     // up: the two colours of each level with the parent's error prolongated
     for (int l = last - 1; l >= first; --l) {
-        const BLev L = tab[l], P = tab[l + 1];
-        int x, y;
-        const bool mine = coords(L, x, y);
-        const int pc = (mine ? c : 0) + L.w;
-        const bool work = mine && fl[L.fl + c];
+        const BLev L = a.tab[l], P = a.tab[l + 1];
+        const bool act = (c & ~63) < L.n;  // a wave without cells of the level skips both colours
+        const bool mine = c < L.n;
+        const int cc = mine ? c : 0;
+        const int y = bottom_row(L, cc), x = cc - y * L.w;
+        const int pc = cc + L.w;
+        // neighbour raw indices and coordinates (rows wrap as in the flat arrays)
+        const int rr = cc + 1, rt = cc + L.w, rl = cc - 1, rb = cc - L.w;
+        const int xr = x + 1 < L.w ? x + 1 : 0, yr = x + 1 < L.w ? y : y + 1;
+        const int xl = x > 0 ? x - 1 : L.w - 1, yl = x > 0 ? y : y - 1;
         // e + the parent's error at raw index raw = (nx, ny) when it is a
-        // fluid cell of the level; e alone outside [0, n) or for other cells
-        // (rows wrap as in the flat arrays).  The loads are unconditional
-        // (clamped), so that they go out together.
-        auto nb = [&](int raw, int nx, int ny) {
-            const bool in = raw >= 0 && raw < L.n;
-            const float v = f[L.e + raw + L.w];
-            const uint8_t fv = fl[L.fl + (in ? raw : c)];
-            const float pe = f[P.e + ((in ? ny : y) / 2) * P.w + (in ? nx : x) / 2 + P.w];
-            return (in && fv) ? v + pe : v;
-        };
-        // the cell's own prolongated value and coefficients, before the first
-        // colour (it leaves the second colour's cells as they are)
-        const float own = f[L.e + pc] + f[P.e + (y / 2) * P.w + x / 2 + P.w];
-        const float id = f[L.id + pc], r = f[L.r + pc], px = f[L.px + pc], py = f[L.py + pc];
-        const float pxm = f[L.px + pc - 1], pym = f[L.py + pc - L.w];
-        const bool black = ((x + y) & 1) == 1;
-        if (mine && black) {
-            const float er = x + 1 < L.w ? nb(c + 1, x + 1, y) : nb(c + 1, 0, y + 1);
-            const float et = nb(c + L.w, x, y + 1);
-            const float el = x > 0 ? nb(c - 1, x - 1, y) : nb(c - 1, L.w - 1, y - 1);
-            const float eb = nb(c - L.w, x, y - 1);
-            const float v = sor(own, id, r, px, py, pxm, pym, er, et, el, eb);
-            if (work) f[L.e + pc] = v;
+        // fluid cell of the level; e alone outside [0, n) or for other cells.
+        // The loads are unconditional (clamped), so that they go out together.
+        const bool inr = rr < L.n, int_ = rt < L.n, inl = rl >= 0, inb = rb >= 0;
+        float own = 0.0f, id = 0.0f, r = 0.0f, px = 0.0f, py = 0.0f, pxm = 0.0f, pym = 0.0f;
+        bool work = false, black = false;
+        if (act) {
+            float vr = f[L.e + rr + L.w], vt = f[L.e + rt + L.w], vl = f[L.e + rl + L.w], vb = f[L.e + rb + L.w];
+            uint32_t fr = fl[L.fl + (inr ? rr : cc)], ft = fl[L.fl + (int_ ? rt : cc)];
+            uint32_t fl_l = fl[L.fl + (inl ? rl : cc)], fb = fl[L.fl + (inb ? rb : cc)];
+            float per = f[P.e + ((inr ? yr : y) / 2) * P.w + (inr ? xr : x) / 2 + P.w];
+            float pet = f[P.e + ((int_ ? y + 1 : y) / 2) * P.w + x / 2 + P.w];
+            float pel = f[P.e + ((inl ? yl : y) / 2) * P.w + (inl ? xl : x) / 2 + P.w];
+            float peb = f[P.e + ((inb ? y - 1 : y) / 2) * P.w + x / 2 + P.w];
+            // the cell's own prolongated value and coefficients, before the first
+            // colour (it leaves the second colour's cells as they are)
+            float e_own = f[L.e + pc], pe_own = f[P.e + (y / 2) * P.w + x / 2 + P.w];
+            id = f[L.id + pc]; r = f[L.r + pc]; px = f[L.px + pc]; py = f[L.py + pc];
+            pxm = f[L.px + pc - 1]; pym = f[L.py + pc - L.w];
+            uint32_t wfl = fl[L.fl + cc];
+            APIC_TIE4(vr, vt, vl, vb);
+            APIC_TIE4(fr, ft, fl_l, fb);
+            APIC_TIE4(per, pet, pel, peb);
+            APIC_TIE4(e_own, pe_own, id, r);
+            APIC_TIE4(px, py, pxm, pym);
+            APIC_TIE1(wfl);
+            work = mine && wfl;
+            own = e_own + pe_own;
+            black = ((x + y) & 1) == 1;
+            {
+                const float er = (inr && fr) ? vr + per : vr;
+                const float et = (int_ && ft) ? vt + pet : vt;
+                const float el = (inl && fl_l) ? vl + pel : vl;
+                const float eb = (inb && fb) ? vb + peb : vb;
+                const float v = bottom_sor(own, id, r, px, py, pxm, pym, er, et, el, eb);
+                if (work && black) f[L.e + pc] = v;
+            }
         }
         __syncthreads();
-        if (mine && !black) {
-            const float v = sor(own, id, r, px, py, pxm, pym, f[L.e + pc + 1], f[L.e + pc + L.w], f[L.e + pc - 1],
-                                f[L.e + pc - L.w]);
-            if (work) f[L.e + pc] = v;
+        if (act) {
+            float er = f[L.e + pc + 1], et = f[L.e + pc + L.w], el = f[L.e + pc - 1], eb = f[L.e + pc - L.w];
+            APIC_TIE4(er, et, el, eb);
+            const float v = bottom_sor(own, id, r, px, py, pxm, pym, er, et, el, eb);
+            if (work && !black) f[L.e + pc] = v;
         }
         __syncthreads();
     }
     // copy-back: each level's r and e are one stride of the image
     for (int l = first; l <= last; ++l) {
-        const BLev L = tab[l];
-        float* __restrict__ dst = ls.lv[l].r;  // e follows r
+        const BLev& L = a.tab[l];
+        float* __restrict__ dst = a.img + L.r;  // e follows r
         const uint32_t n2 = 2 * (L.e - L.r);
         for (uint32_t i = threadIdx.x; i < n2; i += kBottomThreads) dst[i] = f[L.r + i];
     }
 }
+#undef APIC_TIE4
+#undef APIC_TIE2
+#undef APIC_TIE1
 
 struct BottomLds {
     static constexpr int kThreads = kBottomThreads;
-    struct Args {
-        LevelSet ls;
-        int first;
-        const int32_t* done;
-        int rel;  // the relative mode's coarse exits
-    };
-    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
-        bottom_lds(a.ls, a.first, a.done, a.rel != 0);
-    }
+    using Args = BottomLdsArgs;
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) { bottom_lds(a); }
 };
 
 // ---------------------------------------------------------------------- PCG
@@ -2759,6 +2833,35 @@ LevelSet level_set(const Sim& sim) {
     return ls;
 }
 
+// The LDS bottom's level table: each level's array offsets in the image
+// (per level diag, inv_diag, plus_x, plus_y, r, e over the padded range; the
+// flag bytes apart), which is also the layout of the levels' arrays in
+// global memory from the first bottom level on.
+BottomLdsArgs bottom_lds_args(const Sim& sim, int first, const int32_t* done) {
+    BottomLdsArgs a{};
+    const int last = (int)sim.level_count - 1;
+    uint32_t total = 0, ftotal = 0;
+    for (int l = first; l <= last; ++l) {
+        const LevelView& G = sim.levels[l];
+        const uint32_t p = (uint32_t)bottom_padded(G);
+        const uint32_t wmul = G.w > 1 ? (uint32_t)(((1ull << 32) + G.w - 1) / G.w) : 0u;
+        a.tab[l] = {(int32_t)G.w, (int32_t)G.h, (int32_t)G.cells, total, total + p, total + 2 * p, total + 3 * p,
+                    total + 4 * p, total + 5 * p, ftotal, wmul};
+        total += 6 * p;
+        ftotal += (uint32_t)bottom_flags(G);
+    }
+    a.img = sim.levels[first].diag;
+    a.fimg = reinterpret_cast<const uint32_t*>(sim.levels[first].fluid);
+    a.total = total;
+    a.fwords = ftotal / 4;
+    a.cg = a.tab[last].e + (uint32_t)bottom_padded(sim.levels[last]);
+    a.first = first;
+    a.last = last;
+    a.done = done;
+    a.rel = (int)(sim.solver.mode == 1);
+    return a;
+}
+
 Prolong base_prolong(const Sim& sim) {
     Prolong P{};
     P.has = sim.level_count > 1;
@@ -2922,7 +3025,7 @@ void launch_vcycle(Sim& sim, int mode, int k) {
         else emit<VDown<kBaseTile>>(sim, a, nt, k);
     }
     if (sim.bottom_in_lds) {
-        emit<BottomLds>(sim, {level_set(sim), bottom, done, (int)(sim.solver.mode == 1)}, 1);
+        emit<BottomLds>(sim, bottom_lds_args(sim, bottom, done), 1);
     } else {
         emit<BottomGlobal>(sim, {level_set(sim), bottom, sim.cg_s, sim.cg_ap, sim.cg_r, done,
                                  (int)(sim.solver.mode == 1)},
