@@ -285,7 +285,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     ALLOC(s.part_gamma_d, ngam); ALLOC(s.part_asz_d, ngam); ALLOC(s.part_rr_t, ngam);
     ALLOC(s.part_delta_d, kRedBlocks); ALLOC(s.part_rr_d, kRedBlocks);
     ALLOC(s.part_rr2_d, kRedBlocks); ALLOC(s.pressure_d, nc);
-    s.part_maxv_cap = 1024;
+    s.part_maxv_cap = 4096;  // G2P's workgroups: 2 particles per thread and step up to 2M particles
     ALLOC(s.part_maxv, s.part_maxv_cap);
     ALLOC(s.fstats, 1);
     ALLOC(s.maxv_bits, 1);

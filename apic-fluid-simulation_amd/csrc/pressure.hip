@@ -2026,8 +2026,6 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
         }
     }
     const double rr_t = (check && part_rr_t) ? sum_partials256(part_rr_t, (int)ntiles, dscratch) : 0.0;
-    double dsum = 0.0;
-    if constexpr (AX) dsum = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
     if (spec) vtile_put<T, NT>(g, t);  // read after the barrier of the tile test below
     if (dn) return;
     if (check && part_rr_t) {  // the update's partials, one per base tile
@@ -2053,23 +2051,26 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     }
     // A tile without a fluid cell (none of its coarse cells is fluid) has
     // nothing to compute: the values of non-fluid cells are only ever read
-    // times zero coefficients, so it leaves them as they are.
+    // times zero coefficients, so it leaves them as they are.  It leaves
+    // before it loads anything more, so that its slot on the CU frees early.
     double alpha_d = 0.0;
-    if constexpr (AX) {
-        if (!pcg_alpha(st, dsum, gam, lead, alpha_d)) return;
-    }
     bool any = false;
 #pragma unroll
     for (int q = 0; q < G::kCPer; ++q) any = any || (cin[q] && cfl[q] != 0);
     __shared__ uint32_t any_slots[NT / 64];  // (one call per launch)
-    if (!block_any<NT>(any, any_slots)) {
-        if constexpr (AX) {  // no fluid cell in the tile: a zero partial, as axpy_tile's
+    const bool has_fluid = block_any<NT>(any, any_slots);
+    if constexpr (AX) {
+        // no fluid cell in the tile: a zero partial, as axpy_tile's (workgroup
+        // 0 goes on to record alpha)
+        if (!has_fluid && wg != 0) {
             if (threadIdx.x == 0) ax.part_rr_out[wg] = 0.0;
+            return;
         }
-        return;
+    } else {
+        if (!has_fluid) return;
     }
     if (!spec) {
-        vtile_fetch<T, NT>(F, X0, Y0, g, !base, true);
+        if (has_fluid) vtile_fetch<T, NT>(F, X0, Y0, g, !base, true);
         if constexpr (AX) {
             // the update's operands: A s over the region's fluid cells, and
             // the tile's fluid cells' p, s, r and A s (non-fluid slots read
@@ -2094,6 +2095,14 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
                 al[q] = ax.As_lo[c + w];
                 rv[q] = F.r[c + w];
                 lv[q] = ax.r_lo[c + w];
+            }
+            // delta's partials, their loads in flight with the tile's, then
+            // alpha (every workgroup forms the same sum)
+            const double dsum = sum_partials256(ax.part_delta_d, (int)ax.ndelta, dscratch);
+            if (!pcg_alpha(st, dsum, gam, lead, alpha_d)) return;
+            if (!has_fluid) {  // workgroup 0 of a tile without a fluid cell
+                if (threadIdx.x == 0) ax.part_rr_out[wg] = 0.0;
+                return;
             }
 #pragma unroll
             for (int q = 0; q < G::kPer; ++q)
@@ -2672,30 +2681,39 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         }
     };
     VRegs<T, NT> g;
+    // the parent's error over the region (read after a barrier)
+    float cev[G::kEPer];
+    uint32_t cin_mask = 0u;
+    auto fetch_ce = [&]() {
+#pragma unroll
+        for (int q = 0; q < G::kEPer; ++q) {
+            const int j = (int)threadIdx.x + NT * q;
+            const int32_t cx = CX0 + j % kC, cy = CY0 + j / kC;
+            const bool in = j < kC * kC && cx >= 0 && cx < cw && cy >= 0 && cy < chh;
+            cin_mask |= (uint32_t)in << q;
+            cev[q] = C.e[in ? (size_t)(cy + 1) * cw + cx : 0];
+        }
+    };
+    auto put_ce = [&]() {
+#pragma unroll
+        for (int q = 0; q < G::kEPer; ++q) {
+            const int j = (int)threadIdx.x + NT * q;
+            if (j < kC * kC) ce[j] = ((cin_mask >> q) & 1u) ? cev[q] : 0.0f;
+        }
+    };
     if (spec) {
         vtile_fetch<T, NT>(L, X0, Y0, g, true, false);
         fetch_as();
-    }
-    float cev[G::kEPer];
-    uint32_t cin_mask = 0u;
-#pragma unroll
-    for (int q = 0; q < G::kEPer; ++q) {
-        const int j = (int)threadIdx.x + NT * q;
-        const int32_t cx = CX0 + j % kC, cy = CY0 + j / kC;
-        const bool in = j < kC * kC && cx >= 0 && cx < cw && cy >= 0 && cy < chh;
-        cin_mask |= (uint32_t)in << q;
-        cev[q] = C.e[in ? (size_t)(cy + 1) * cw + cx : 0];
+        fetch_ce();
     }
     const int32_t ccx = X0 / 2 + (int32_t)(threadIdx.x % (T / 2)), ccy = Y0 / 2 + (int32_t)(threadIdx.x / (T / 2));
     const bool cin = threadIdx.x < G::kCN && ccx < cw && ccy < chh;
     const uint8_t cfv = C.fluid[cin ? (size_t)ccy * cw + ccx : 0];
     const uint32_t last1 = fs ? fs->last_raw : 0u, cnt = fs ? fs->count : 0u;
     const int32_t ria = r_home ? *r_in_alt : 0;
-    if (spec) vtile_put<T, NT>(g, t);  // read after the barrier of the tile test below
-#pragma unroll
-    for (int q = 0; q < G::kEPer; ++q) {
-        const int j = (int)threadIdx.x + NT * q;
-        if (j < kC * kC) ce[j] = ((cin_mask >> q) & 1u) ? cev[q] : 0.0f;
+    if (spec) {  // read after the barrier of the tile test below
+        vtile_put<T, NT>(g, t);
+        put_ce();
     }
     if (dn) {
         // the solve ended at the residual test of a fused update: its
@@ -2712,7 +2730,8 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
         }
         return;
     }
-    // a tile without a fluid cell (see vdown) only reports a zero partial
+    // a tile without a fluid cell (see vdown) only reports a zero partial,
+    // before it loads anything more
     __shared__ uint32_t any_slots[NT / 64];  // (one call per launch)
     if (!block_any<NT>(cin && cfv != 0, any_slots)) {
         if (threadIdx.x == 0) {
@@ -2724,6 +2743,8 @@ __device__ __forceinline__ void vup(const LevelView& L, const LevelView& C, cons
     if (!spec) {
         vtile_fetch<T, NT>(L, X0, Y0, g, true, false);
         fetch_as();
+        fetch_ce();
+        put_ce();
         if (r_home) {
             // r from the second buffer, whose non-fluid cells are never
             // written: zero there (they enter the r.z partials times e = 0)

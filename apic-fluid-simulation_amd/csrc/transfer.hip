@@ -427,10 +427,21 @@ __global__ void forces_kernel(float* __restrict__ mac_u, float* __restrict__ mac
     }
 }
 
-__device__ inline float masked_interp(const float* w, const uint8_t* open, const uint32_t* ids, const float* nodes) {
+// One particle of G2P between its loads: the record's position and time
+// residual, its two stencils and the grid values and open flags of their
+// corners.
+struct G2pLane {
+    float x, y, res;
+    Stencil su, sv;
+    uint32_t ui[4], vi[4];
+    float un[4], vn[4];
+    uint8_t ou[4], ov[4];
+};
+
+__device__ __forceinline__ float masked_interp_loaded(const float* w, const uint8_t* open, const float* nodes) {
     float mw[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) mw[k] = open[ids[k]] ? w[k] : 0.0f;
+    for (int k = 0; k < 4; ++k) mw[k] = open[k] ? w[k] : 0.0f;
     // bilinear_interpolate (src/apic.rs:138-172)
     float ws = ((mw[0] + mw[1]) + mw[2]) + mw[3];
     float inv = 1.0f / fast_max(ws, 1e-6f);
@@ -443,7 +454,12 @@ __device__ inline float masked_interp(const float* w, const uint8_t* open, const
 // pin / pout: the records before and after (the same buffer, or with a
 // deferred sort gather the other one, pin read through perm: record perm[i]
 // becomes record i, and lanes past the records read zeros, as the gather
-// leaves them).
+// leaves them).  Each thread takes U particles per step, nwg * blockDim
+// apart (each wave's lanes stay consecutive for every one of them), and
+// issues their loads stage by stage: the permutation entries, the records,
+// then the grid corners of all U, so that a step waits three times for U
+// particles instead of once per load chain of each.
+template <int U>
 __device__ __forceinline__ void g2p(
     const float* pin, float* pout, const uint32_t* perm, size_t nalloc, uint32_t np, float dt_arg,
     float cell, uint32_t W, uint32_t H,
@@ -455,94 +471,124 @@ __device__ __forceinline__ void g2p(
     if (guard && *guard == 0) return;
     const float dt = ss ? ss->dt : dt_arg;
     float speed_new = 0.0f;
-    for (size_t i = (size_t)wg * blockDim.x + threadIdx.x; i < nalloc; i += (size_t)nwg * blockDim.x) {
-        float x = 0.0f, y = 0.0f, res = 0.0f;
-        if (!perm || i < np) {
-            size_t j = i;
+    const size_t stride = (size_t)nwg * blockDim.x;
+    for (size_t i0 = (size_t)wg * blockDim.x + threadIdx.x; i0 < nalloc; i0 += (size_t)U * stride) {
+        G2pLane p[U];
+        size_t src[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = i0 + (size_t)u * stride;
+            src[u] = i < nalloc ? i : 0;
             if (perm) {
-                j = perm[i];
-                if (j >= np) j = np - 1;  // (as the sort's gather)
+                const uint32_t j = perm[i < np ? i : 0];
+                src[u] = j >= np ? np - 1 : j;  // (as the sort's gather)
             }
-            x = pin[(size_t)P_X * nalloc + j];
-            y = pin[(size_t)P_Y * nalloc + j];
-            res = pin[(size_t)T_RES * nalloc + j];
-        }
-        const float gx = x / cell, gy = y / cell;
-        Stencil su, sv;
-        make_stencil(gx, gy - 0.5f, cell, su);
-        make_stencil(gx - 0.5f, gy, cell, sv);
-        uint32_t ui[4] = {u_index(su.ix, su.iy, W, H), u_index(su.ix + 1, su.iy, W, H),
-                          u_index(su.ix, su.iy + 1, W, H), u_index(su.ix + 1, su.iy + 1, W, H)};
-        uint32_t vi[4] = {v_index(sv.ix, sv.iy, W, H), v_index(sv.ix + 1, sv.iy, W, H),
-                          v_index(sv.ix, sv.iy + 1, W, H), v_index(sv.ix + 1, sv.iy + 1, W, H)};
-        float un[4], vn[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) { un[k] = mac_u[ui[k]]; vn[k] = mac_v[vi[k]]; }
-        float nvx = masked_interp(su.w, open_u, ui, un);
-        float nvy = masked_interp(sv.w, open_v, vi, vn);
-        // jittered time step (:2173-2186)
-        float speed = sqrtf(nvx * nvx + nvy * nvy);
-        float cfl_local = (speed * dt) / cell;
-        float scale = fast_min(fast_max(cfl_local, 0.0f), 1.0f);
-        float gam = (scale * scale) * (3.0f - (2.0f * scale));
-        float jitter = (gam * random_lane((uint32_t)i, timestamp)) * dt;
-        float dt_raw = (dt + res) + jitter;
-        float dt_act = fast_min(fast_max(dt_raw, 0.0f), dt * 2.0f);
-        float new_res = (dt + res) - dt_act;
-        float nx = x + dt_act * nvx, ny = y + dt_act * nvy;
-        // wall clamp and reflection (:2188-2209)
-        const float min_x = 1.001f * cell, max_x = ((float)W - 1.001f) * cell;
-        const float min_y = 1.001f * cell, max_y = ((float)H - 1.001f) * cell;
-        const float bd = -border_damping;
-        bool oxmin = nx < min_x, oxmax = nx > max_x, oymin = ny < min_y, oymax = ny > max_y;
-        bool ovxmin = oxmin && nvx < 0.0f, ovxmax = oxmax && nvx > 0.0f;
-        bool ovymin = oymin && nvy < 0.0f, ovymax = oymax && nvy > 0.0f;
-        if (oxmin) nx = min_x;
-        if (oxmax) nx = max_x;
-        if (ovxmin) nvx = nvx * bd;
-        if (ovxmax) nvx = nvx * bd;
-        if (oymin) ny = min_y;
-        if (oymax) ny = max_y;
-        if (ovymin) nvy = nvy * bd;
-        if (ovymax) nvy = nvy * bd;
-        // affine matrix C with unmasked weights (:2211-2247)
-        const float h2 = cell * cell;
-        float udx = (h2 * su.tx) * (1.0f - su.tx), udy = (h2 * su.ty) * (1.0f - su.ty);
-        float vdx = (h2 * sv.tx) * (1.0f - sv.tx), vdy = (h2 * sv.ty) * (1.0f - sv.ty);
-        float uix = udx > 1e-6f ? 1.0f / udx : 0.0f, uiy = udy > 1e-6f ? 1.0f / udy : 0.0f;
-        float vix = vdx > 1e-6f ? 1.0f / vdx : 0.0f, viy = vdy > 1e-6f ? 1.0f / vdy : 0.0f;
-        float cux = 0.0f, cuy = 0.0f, cvx = 0.0f, cvy = 0.0f;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            cux += (su.w[k] * un[k]) * su.dx[k];
-            cuy += (su.w[k] * un[k]) * su.dy[k];
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            cvx += (sv.w[k] * vn[k]) * sv.dx[k];
-            cvy += (sv.w[k] * vn[k]) * sv.dy[k];
+        for (int u = 0; u < U; ++u) {
+            const size_t i = i0 + (size_t)u * stride;
+            const bool rec = i < nalloc && (!perm || i < np);
+            const float x = pin[(size_t)P_X * nalloc + src[u]];
+            const float y = pin[(size_t)P_Y * nalloc + src[u]];
+            const float res = pin[(size_t)T_RES * nalloc + src[u]];
+            p[u].x = rec ? x : 0.0f;
+            p[u].y = rec ? y : 0.0f;
+            p[u].res = rec ? res : 0.0f;
         }
-        float c_ux = cux * uix, c_uy = cuy * uiy, c_vx = cvx * vix, c_vy = cvy * viy;
-        bool rx = ovxmin || ovxmax, ry = ovymin || ovymax;
-        if (rx) { c_uy = c_uy * -1.0f; c_vx = c_vx * -1.0f; }
-        if (ry) { c_uy = c_uy * -1.0f; c_vx = c_vx * -1.0f; }
-        pout[(size_t)P_X * nalloc + i] = nx;
-        pout[(size_t)P_Y * nalloc + i] = ny;
-        pout[(size_t)V_X * nalloc + i] = nvx;
-        pout[(size_t)V_Y * nalloc + i] = nvy;
-        pout[(size_t)CU_X * nalloc + i] = c_ux;
-        pout[(size_t)CU_Y * nalloc + i] = c_uy;
-        pout[(size_t)CV_X * nalloc + i] = c_vx;
-        pout[(size_t)CV_Y * nalloc + i] = c_vy;
-        pout[(size_t)T_RES * nalloc + i] = new_res;
-        speed_new = fast_max(speed_new, sqrtf(nvx * nvx + nvy * nvy));
-        if (write_keys && i < np) {
-            // keygen of the next update_spatial_lookup (:1049-1064)
-            float kx = nx / cell, ky = ny / cell;
-            uint32_t ix = (uint32_t)(int32_t)fast_min(fast_max(kx, 0.0f), (float)(W - 1));
-            uint32_t iy = (uint32_t)(int32_t)fast_min(fast_max(ky, 0.0f), (float)(H - 1));
-            keys[i] = morton(ix, iy);
-            vals[i] = (uint32_t)i;  // (perm, when given, is this array: lane i read its entry above)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float gx = p[u].x / cell, gy = p[u].y / cell;
+            make_stencil(gx, gy - 0.5f, cell, p[u].su);
+            make_stencil(gx - 0.5f, gy, cell, p[u].sv);
+            const Stencil& su = p[u].su;
+            const Stencil& sv = p[u].sv;
+            p[u].ui[0] = u_index(su.ix, su.iy, W, H); p[u].ui[1] = u_index(su.ix + 1, su.iy, W, H);
+            p[u].ui[2] = u_index(su.ix, su.iy + 1, W, H); p[u].ui[3] = u_index(su.ix + 1, su.iy + 1, W, H);
+            p[u].vi[0] = v_index(sv.ix, sv.iy, W, H); p[u].vi[1] = v_index(sv.ix + 1, sv.iy, W, H);
+            p[u].vi[2] = v_index(sv.ix, sv.iy + 1, W, H); p[u].vi[3] = v_index(sv.ix + 1, sv.iy + 1, W, H);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                p[u].un[k] = mac_u[p[u].ui[k]];
+                p[u].vn[k] = mac_v[p[u].vi[k]];
+                p[u].ou[k] = open_u[p[u].ui[k]];
+                p[u].ov[k] = open_v[p[u].vi[k]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const size_t i = i0 + (size_t)u * stride;
+            if (i >= nalloc) continue;
+            const Stencil& su = p[u].su;
+            const Stencil& sv = p[u].sv;
+            const float* un = p[u].un;
+            const float* vn = p[u].vn;
+            const float x = p[u].x, y = p[u].y, res = p[u].res;
+            float nvx = masked_interp_loaded(su.w, p[u].ou, un);
+            float nvy = masked_interp_loaded(sv.w, p[u].ov, vn);
+            // jittered time step (:2173-2186)
+            float speed = sqrtf(nvx * nvx + nvy * nvy);
+            float cfl_local = (speed * dt) / cell;
+            float scale = fast_min(fast_max(cfl_local, 0.0f), 1.0f);
+            float gam = (scale * scale) * (3.0f - (2.0f * scale));
+            float jitter = (gam * random_lane((uint32_t)i, timestamp)) * dt;
+            float dt_raw = (dt + res) + jitter;
+            float dt_act = fast_min(fast_max(dt_raw, 0.0f), dt * 2.0f);
+            float new_res = (dt + res) - dt_act;
+            float nx = x + dt_act * nvx, ny = y + dt_act * nvy;
+            // wall clamp and reflection (:2188-2209)
+            const float min_x = 1.001f * cell, max_x = ((float)W - 1.001f) * cell;
+            const float min_y = 1.001f * cell, max_y = ((float)H - 1.001f) * cell;
+            const float bd = -border_damping;
+            bool oxmin = nx < min_x, oxmax = nx > max_x, oymin = ny < min_y, oymax = ny > max_y;
+            bool ovxmin = oxmin && nvx < 0.0f, ovxmax = oxmax && nvx > 0.0f;
+            bool ovymin = oymin && nvy < 0.0f, ovymax = oymax && nvy > 0.0f;
+            if (oxmin) nx = min_x;
+            if (oxmax) nx = max_x;
+            if (ovxmin) nvx = nvx * bd;
+            if (ovxmax) nvx = nvx * bd;
+            if (oymin) ny = min_y;
+            if (oymax) ny = max_y;
+            if (ovymin) nvy = nvy * bd;
+            if (ovymax) nvy = nvy * bd;
+            // affine matrix C with unmasked weights (:2211-2247)
+            const float h2 = cell * cell;
+            float udx = (h2 * su.tx) * (1.0f - su.tx), udy = (h2 * su.ty) * (1.0f - su.ty);
+            float vdx = (h2 * sv.tx) * (1.0f - sv.tx), vdy = (h2 * sv.ty) * (1.0f - sv.ty);
+            float uix = udx > 1e-6f ? 1.0f / udx : 0.0f, uiy = udy > 1e-6f ? 1.0f / udy : 0.0f;
+            float vix = vdx > 1e-6f ? 1.0f / vdx : 0.0f, viy = vdy > 1e-6f ? 1.0f / vdy : 0.0f;
+            float cux = 0.0f, cuy = 0.0f, cvx = 0.0f, cvy = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                cux += (su.w[k] * un[k]) * su.dx[k];
+                cuy += (su.w[k] * un[k]) * su.dy[k];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                cvx += (sv.w[k] * vn[k]) * sv.dx[k];
+                cvy += (sv.w[k] * vn[k]) * sv.dy[k];
+            }
+            float c_ux = cux * uix, c_uy = cuy * uiy, c_vx = cvx * vix, c_vy = cvy * viy;
+            bool rx = ovxmin || ovxmax, ry = ovymin || ovymax;
+            if (rx) { c_uy = c_uy * -1.0f; c_vx = c_vx * -1.0f; }
+            if (ry) { c_uy = c_uy * -1.0f; c_vx = c_vx * -1.0f; }
+            pout[(size_t)P_X * nalloc + i] = nx;
+            pout[(size_t)P_Y * nalloc + i] = ny;
+            pout[(size_t)V_X * nalloc + i] = nvx;
+            pout[(size_t)V_Y * nalloc + i] = nvy;
+            pout[(size_t)CU_X * nalloc + i] = c_ux;
+            pout[(size_t)CU_Y * nalloc + i] = c_uy;
+            pout[(size_t)CV_X * nalloc + i] = c_vx;
+            pout[(size_t)CV_Y * nalloc + i] = c_vy;
+            pout[(size_t)T_RES * nalloc + i] = new_res;
+            speed_new = fast_max(speed_new, sqrtf(nvx * nvx + nvy * nvy));
+            if (write_keys && i < np) {
+                // keygen of the next update_spatial_lookup (:1049-1064)
+                float kx = nx / cell, ky = ny / cell;
+                uint32_t ix = (uint32_t)(int32_t)fast_min(fast_max(kx, 0.0f), (float)(W - 1));
+                uint32_t iy = (uint32_t)(int32_t)fast_min(fast_max(ky, 0.0f), (float)(H - 1));
+                keys[i] = morton(ix, iy);
+                vals[i] = (uint32_t)i;  // (perm, when given, is this array: every lane read its entries above)
+            }
         }
     }
     // max |v| over all lanes including the tail lanes of the last chunk
@@ -558,6 +604,7 @@ __device__ __forceinline__ void g2p(
     }
 }
 
+template <int U>
 struct G2p {
     static constexpr int kThreads = 256;
     struct Args {
@@ -583,9 +630,9 @@ struct G2p {
         uint32_t nwg;
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        g2p(a.pin, a.pout, a.perm, a.nalloc, a.np, a.dt, a.cell, a.W, a.H, a.timestamp, a.border_damping, a.mac_u,
-            a.mac_v, a.open_u,
-            a.open_v, a.write_keys, a.keys, a.vals, a.part_maxv, a.ss, a.guard, wg, a.nwg);
+        g2p<U>(a.pin, a.pout, a.perm, a.nalloc, a.np, a.dt, a.cell, a.W, a.H, a.timestamp, a.border_damping,
+               a.mac_u, a.mac_v, a.open_u, a.open_v, a.write_keys, a.keys, a.vals, a.part_maxv, a.ss, a.guard, wg,
+               a.nwg);
     }
 };
 
@@ -637,6 +684,8 @@ struct MaxvFinalize {
         }
     }
 };
+
+constexpr int kG2pPer = 2;  // particles per thread and loop step of G2P
 
 unsigned particle_blocks(const Sim& sim) {
     size_t b = (sim.nalloc + 255) / 256;
@@ -705,11 +754,11 @@ void launch_g2p(Sim& sim, float dt, bool write_keys, bool end_substep) {
     // write the sorted order into the other buffer
     const bool gather = sim.perm_pending;
     float* pout = sim.pbuf[gather ? 1 - sim.cur : sim.cur];
-    G2p::Args a{sim.pbuf[sim.cur], pout, gather ? sim.vals[0] : nullptr, sim.nalloc, sim.num_particles, dt,
+    G2p<kG2pPer>::Args a{sim.pbuf[sim.cur], pout, gather ? sim.vals[0] : nullptr, sim.nalloc, sim.num_particles, dt,
                 sim.cell, sim.W, sim.H, sim.timestamp, sim.props.border_damping, sim.mac_u, sim.mac_v, sim.open_u,
                 sim.open_v, (int)write_keys, sim.keys[0], sim.vals[0], sim.part_maxv, sim.step_src, sim.tail_guard,
                 blocks};
-    emit<G2p>(sim, a, blocks);
+    emit<G2p<kG2pPer>>(sim, a, blocks);
     if (gather) {
         sim.cur = 1 - sim.cur;
         sim.perm_pending = false;
