@@ -74,18 +74,8 @@ __host__ __device__ __forceinline__ uint32_t deal_chunk(const typename K::Args& 
     else return 0;
 }
 
-// A kernel may declare kWavesPerEu: the waves per SIMD that its register
-// use must allow (the compiler keeps its VGPRs within 512 / kWavesPerEu), so
-// that enough of its workgroups are resident at once; 0 leaves the choice to
-// the compiler.
-template <class K, class = void>
-struct waves_per_eu : std::integral_constant<int, 0> {};
 template <class K>
-struct waves_per_eu<K, std::void_t<decltype(K::kWavesPerEu)>> : std::integral_constant<int, K::kWavesPerEu> {};
-
-template <class K>
-__global__ __launch_bounds__(K::kThreads) __attribute__((amdgpu_waves_per_eu(waves_per_eu<K>::value > 0 ? waves_per_eu<K>::value : 1)))
-void one_kernel(typename K::Args a, int k) {
+__global__ __launch_bounds__(K::kThreads) void one_kernel(typename K::Args a, int k) {
     uint32_t wg;
     if constexpr (round_robin<K>::value) wg = blockIdx.x;
     else if constexpr (deal_rows<K>::value) wg = xcd_deal(blockIdx.x, gridDim.x, a.tiles_x);
@@ -99,8 +89,7 @@ void one_kernel(typename K::Args a, int k) {
 constexpr uint32_t kMapSkip = 0xFFFFFFFFu;  // a map entry that runs nothing (XCD queue padding)
 
 template <class K>
-__global__ __launch_bounds__(K::kThreads) __attribute__((amdgpu_waves_per_eu(waves_per_eu<K>::value > 0 ? waves_per_eu<K>::value : 1)))
-void batch_kernel(const typename K::Args* __restrict__ tab,
+__global__ __launch_bounds__(K::kThreads) void batch_kernel(const typename K::Args* __restrict__ tab,
                                                             const uint32_t* __restrict__ map, int k) {
     using ConstArgs = const __attribute__((address_space(4))) typename K::Args;
     const uint32_t m = map[blockIdx.x];
