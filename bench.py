@@ -103,7 +103,7 @@ def stage_rates(stage_ms, n_p, n_c, n_f, levels, iters, width, height):
     return out
 
 
-KERNEL_NAMES = {"kernel_g2p": "one_kernel<G2p>", "kernel_p2g": "one_kernel<P2gOwner>",
+KERNEL_NAMES = {"kernel_g2p": "one_kernel<G2p<2> >", "kernel_p2g": "one_kernel<P2gOwner>",
                 "kernel_spmv": "one_kernel<SpmvDot<1024> >",
                 "kernel_pcg_iteration": "PCG iteration (SpmvDot<1024>, VDown<32, 512, true>, 2 x VDown2, "
                                         "BottomLds, 2 x VUp2, VUp<32, 512>)"}
