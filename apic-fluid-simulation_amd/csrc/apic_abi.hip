@@ -192,6 +192,10 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     ALLOC(s.vals[0], np); ALLOC(s.vals[1], np);
     ALLOC(s.radix_scratch, radix_scratch_words(np));
     ALLOC(s.cell_range, cell_key_count(s));
+    {
+        const size_t nblk = (size_t)p2g_blocks(s.W) * p2g_blocks(s.H);
+        ALLOC(s.p2g_halo, kP2gHaloFloats * nblk);
+    }
     const size_t nu = s.nu(), nv = s.nv(), nc = s.cells;
     ALLOC(s.mac_u, nu); ALLOC(s.mac_v, nv); ALLOC(s.old_u, nu); ALLOC(s.old_v, nv);
     ALLOC(s.valid_u, nu); ALLOC(s.valid_v, nv); ALLOC(s.nvalid_u, nu); ALLOC(s.nvalid_v, nv);

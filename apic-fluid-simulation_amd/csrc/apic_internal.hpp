@@ -61,6 +61,13 @@ inline bool bottom_fits_lds(const std::vector<LevelView>& lv, size_t first, size
     return fl <= kBottomLdsFloats && fb <= kBottomLdsFlags;
 }
 
+// P2G works on Morton-aligned blocks of 16 x 16 cells (transfer.hip); each
+// block keeps a halo record of kP2gHaloFloats partial sums for the targets
+// around its edge.
+constexpr uint32_t kP2gBlockSide = 16;
+constexpr size_t kP2gHaloFloats = 512;
+inline uint32_t p2g_blocks(uint32_t side) { return (side + kP2gBlockSide - 1) / kP2gBlockSide; }
+
 // Scalars of one pressure solve.  The reductions write per-workgroup
 // partials; the kernel that needs a reduced scalar sums the partials itself,
 // in the same fixed order in every workgroup, and applies the reference's exit
@@ -184,6 +191,7 @@ struct Sim {
     uint32_t* radix_scratch = nullptr;
     bool keys_valid = false;  // keys[0]/vals[0] hold the current positions' keys
     uint2* cell_range = nullptr;  // [begin, end) of every cell's particles in cell order, by Morton key
+    float* p2g_halo = nullptr;  // P2G: per 16 x 16 block, the partial sums of the targets around its edge
     bool cells_valid = false;     // the particles are in cell order and cell_range describes it
     // The sort's gather is deferred: the particles lie in pbuf[cur] in their
     // order before the sort, vals[0] maps a sorted position to its record,

@@ -23,31 +23,70 @@ namespace apic {
 
 namespace {
 
-// ---- cell-owner P2G -----------------------------------------------------------
+// ---- block P2G ----------------------------------------------------------------
 //
 // In cell order (the Morton-sorted particles, or the particles through a
 // sorted index `order`), the particles of one cell are the contiguous range
-// cell_range[key] = [begin, end).  A workgroup
-// takes a kOwnWin x kOwnWin window of cells: a kOwnTile x kOwnTile tile and
-// its one-cell ring.  Particles of the window are staged in LDS (coalesced
-// loads), then one thread per window cell sums the contributions of its own
-// particles into 33 register partials: the 2x3 u faces, 3x2 v faces and 3x3
-// cells that a particle of that cell can reach (src/apic.rs:1237-1426, with
-// u_index / v_index clamping, :1029-1041).  Every face and cell of the tile
-// then adds the partials of its 6 (faces) or 9 (cells) source cells in a
-// fixed order, so the result is deterministic and needs no atomics, and
-// finishes the target in the same kernel: u = momentum / weight and the valid
-// flag (:1428-1459), the fluid flag (:1461-1464) and, when asked,
-// apply_external_forces (:1604-1626).  The faces x = W and y = H belong to the
-// cells x = W-1 and y = H-1.
+// cell_range[key] = [begin, end), and the 256 cells of a Morton-aligned
+// 16 x 16 block (keys (block << 8) + 0..255) hold one contiguous run.  A
+// workgroup takes one block: it stages the run in LDS with coalesced loads,
+// then one thread per cell sums the contributions of its own particles into
+// 33 register partials: the 2x3 u faces, 3x2 v faces and 3x3 cells that a
+// particle of that cell can reach (src/apic.rs:1237-1426, with u_index /
+// v_index clamping, :1029-1041).  A face or cell whose source cells (the
+// existing cells that can reach it) all lie in the block is finished by that
+// workgroup: it adds the partials of its sources in a fixed order, then
+// u = momentum / weight and the valid flag (:1428-1459), the fluid flag
+// (:1461-1464) and, when asked, apply_external_forces (:1604-1626).  For every
+// other target it reaches (a band around its edge) the block writes its
+// partial sum into its halo record, and P2gSeam adds the records of the blocks
+// that reach such a target, in a fixed block order, and finishes it.  Each
+// particle is read once, nothing needs atomics and the result is
+// deterministic.  The faces x = W and y = H belong to the cells x = W-1 and
+// y = H-1.
 
-constexpr int kOwnWin = 16;
-constexpr int kOwnTile = kOwnWin - 2;
-constexpr int kOwnCells = kOwnWin * kOwnWin;  // one thread per window cell
-constexpr int kOwnCap = 1024;                 // particles staged per pass
+constexpr int kBlk = (int)kP2gBlockSide;        // block side in cells
+constexpr int kBlkCells = kBlk * kBlk;          // one thread per block cell
+constexpr int kOwnCap = 1024;                   // particles staged per pass
 constexpr int kOwnFields = 9;
-constexpr int kOwnSlots = 33;                 // 6 u (mom, w), 6 v (mom, w), 9 density
-static_assert(kOwnFields * kOwnCap >= kOwnSlots * kOwnCells, "partials alias the staging buffer");
+constexpr int kOwnSlots = 33;                   // 6 u (mom, w), 6 v (mom, w), 9 density
+static_assert(kOwnFields * kOwnCap >= kOwnSlots * kBlkCells, "partials alias the staging buffer");
+static_assert(kBlkCells == 256, "Morton offsets of a block are one byte");
+
+// A block's halo record: the partial sums of the targets it reaches outside
+// its interior, as five arrays (u momentum, u weight, v momentum, v weight,
+// density).  Positions are relative to the block's origin (x0, y0):
+//   u faces (x0 + i, y0 - 1 + j), i in [0,16], j in [0,17]: rows 0-1, rows
+//     16-17, then columns 0 and 16 of rows 2-15 (96);
+//   v faces (x0 - 1 + i, y0 + j), i in [0,17], j in [0,16]: the transpose
+//     of the u layout (96);
+//   cells (x0 - 1 + i, y0 - 1 + j), i, j in [0,17]: rows 0-1, rows 16-17,
+//     then columns 0, 1, 16 and 17 of rows 2-15 (128).
+constexpr int kHaloUm = 0, kHaloUw = 96, kHaloVm = 192, kHaloVw = 288, kHaloDen = 384;
+static_assert(kHaloDen + 128 == (int)kP2gHaloFloats, "halo record size");
+
+__device__ inline int halo_u_index(int i, int j) {
+    if (j < 2) return j * 17 + i;
+    if (j >= 16) return 34 + (j - 16) * 17 + i;
+    return 68 + (j - 2) * 2 + (i == 16 ? 1 : 0);
+}
+__device__ inline void halo_u_pos(int e, int& i, int& j) {
+    if (e < 34) { j = e / 17; i = e % 17; }
+    else if (e < 68) { j = 16 + (e - 34) / 17; i = (e - 34) % 17; }
+    else { j = 2 + (e - 68) / 2; i = ((e - 68) & 1) ? 16 : 0; }
+}
+__device__ inline int halo_v_index(int i, int j) { return halo_u_index(j, i); }
+__device__ inline void halo_v_pos(int e, int& i, int& j) { halo_u_pos(e, j, i); }
+__device__ inline int halo_d_index(int i, int j) {
+    if (j < 2) return j * 18 + i;
+    if (j >= 16) return 36 + (j - 16) * 18 + i;
+    return 72 + (j - 2) * 4 + (i < 2 ? i : i - 14);
+}
+__device__ inline void halo_d_pos(int e, int& i, int& j) {
+    if (e < 36) { j = e / 18; i = e % 18; }
+    else if (e < 72) { j = 16 + (e - 36) / 18; i = (e - 36) % 18; }
+    else { j = 2 + (e - 72) / 4; const int c = (e - 72) & 3; i = c < 2 ? c : c + 14; }
+}
 
 struct OwnerAcc {
     float um[6], uw[6];  // u slot kx * 3 + sy: face (cx + kx, cy - 1 + sy)
@@ -72,17 +111,25 @@ __device__ inline void shift3(bool h, float c0, float c1, float& s0, float& s1, 
     s2 += h ? c1 : z;
 }
 
+// The staged form of a particle: its record with the position in cells and
+// the temporal weight in place of the position and the time residual (the
+// divisions are done once per particle while staging, not in the per-cell
+// loop, whose lanes wait for the cell with the most particles).
+__device__ inline void owner_stage(float (&v)[kOwnFields], float prev_dt, float cell) {
+    const float tau = (-v[8]) / prev_dt;
+    v[8] = temporal_w(tau);
+    v[0] = v[0] / cell;
+    v[1] = v[1] / cell;
+}
+
 // One particle of cell (cx, cy): the same arithmetic as p2g_contrib.
-__device__ inline void owner_accumulate(const float* __restrict__ sp, int k, float prev_dt, float cell, int32_t Wi,
+__device__ inline void owner_accumulate(const float* __restrict__ sp, int k, float cell, int32_t Wi,
                                         int32_t Hi, int32_t cx, int32_t cy, OwnerAcc& a) {
-    const float x = sp[0 * kOwnCap + k], y = sp[1 * kOwnCap + k];
+    const float gx = sp[0 * kOwnCap + k], gy = sp[1 * kOwnCap + k];
     const float vx = sp[2 * kOwnCap + k], vy = sp[3 * kOwnCap + k];
     const float cux = sp[4 * kOwnCap + k], cuy = sp[5 * kOwnCap + k];
     const float cvx = sp[6 * kOwnCap + k], cvy = sp[7 * kOwnCap + k];
-    const float tres = sp[8 * kOwnCap + k];
-    const float tau = (-tres) / prev_dt;
-    const float wt = temporal_w(tau);
-    const float gx = x / cell, gy = y / cell;
+    const float wt = sp[8 * kOwnCap + k];
     const float cgx = gx - 0.5f, cgy = gy - 0.5f;
     const float cbx = floorf(cgx), cby = floorf(cgy);
     const float ctx = cgx - cbx, cty = cgy - cby;
@@ -147,234 +194,388 @@ __device__ inline void owner_accumulate(const float* __restrict__ sp, int k, flo
     }
 }
 
-__device__ __forceinline__ void p2g_owner(
-    const float* __restrict__ pb, size_t nalloc, uint32_t np, const uint32_t* __restrict__ order, float prev_dt_arg,
-    float cell, uint32_t W, uint32_t H, const uint2* __restrict__ cell_range, uint32_t tiles_x,
-    float* __restrict__ mac_u, float* __restrict__ mac_v, uint8_t* __restrict__ valid_u,
-    uint8_t* __restrict__ valid_v, float* __restrict__ density, uint8_t* __restrict__ fluid,
-    const uint8_t* __restrict__ obstacle, const float* __restrict__ ext_u, const float* __restrict__ ext_v,
-    int forces, int ext, float dt_arg, float gravity, float gravity_x, const StepState* __restrict__ ss,
-    uint32_t wg) {
-    __shared__ float s_buf[kOwnFields * kOwnCap];
-    const float prev_dt = ss ? ss->prev_dt : prev_dt_arg;
-    const float dt = ss ? ss->dt : dt_arg;
-    const float g = gravity * dt;
-    const float gx = gravity_x * dt;  // horizontal gravity (an extension; zero in the reference)
-    __shared__ uint32_t s_off[kOwnCells + 1];      // window-order prefix of the cell counts
-    __shared__ uint32_t s_src[kOwnCells];          // sorted-order begin of each window cell
-    __shared__ uint32_t s_wave[kOwnCells / 64];
-    __shared__ uint8_t s_fl[kOwnCells];
-    __shared__ uint8_t s_cell[kOwnCap];            // the window cell of each staged particle
-    static_assert(kOwnCells <= 256, "window cells are bytes");
-    const int32_t Wi = (int32_t)W, Hi = (int32_t)H;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wx = tid % kOwnWin, wy = tid / kOwnWin;
-    const int32_t cx = (int32_t)(wg % tiles_x) * kOwnTile - 1 + wx;
-    const int32_t cy = (int32_t)(wg / tiles_x) * kOwnTile - 1 + wy;
-    const bool inside = cx >= 0 && cx < Wi && cy >= 0 && cy < Hi;
-    uint32_t b = 0, cnt = 0;
-    bool fl = false;
-    if (inside) {
-        const uint2 r = cell_range[morton((uint32_t)cx, (uint32_t)cy)];
-        b = r.x;
-        cnt = (r.y > r.x && r.y <= np) ? r.y - r.x : 0u;
-        fl = cnt > 0 && !obstacle[(size_t)cy * W + cx];
-    }
-    // exclusive prefix of the counts in window order
-    uint32_t incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
-    if (lane == 63) s_wave[wave] = incl;
-    s_src[tid] = b;
-    s_fl[tid] = fl;
-    __syncthreads();
-    uint32_t wbase = 0, total = 0;
-#pragma unroll
-    for (int w = 0; w < kOwnCells / 64; ++w) {
-        if (w < wave) wbase += s_wave[w];
-        total += s_wave[w];
-    }
-    const uint32_t my0 = wbase + incl - cnt, my1 = wbase + incl;
-    s_off[tid] = my0;
-    if (tid == 0) s_off[kOwnCells] = total;
-    __syncthreads();
+struct P2gArgs {
+    const float* pb;
+    size_t nalloc;
+    uint32_t np;
+    const uint32_t* order;
+    float prev_dt;
+    float cell;
+    uint32_t W, H;
+    const uint2* cell_range;
+    uint32_t tiles_x, tiles_y;  // blocks per row and column
+    float* mac_u;
+    float* mac_v;
+    uint8_t* valid_u;
+    uint8_t* valid_v;
+    float* density;
+    uint8_t* fluid;
+    const uint8_t* obstacle;
+    const float* ext_u;
+    const float* ext_v;
+    float* halo;  // kP2gHaloFloats per block
+    int forces, ext;
+    float dt, gravity, gravity_x;
+    const StepState* ss;
+};
 
-    OwnerAcc a;
+// Whether every existing source cell of a target, columns [sx0, sx1] and rows
+// [sy0, sy1] clipped to the grid, lies in the block at (x0, y0).
+__device__ inline bool sources_in_block(int32_t sx0, int32_t sx1, int32_t sy0, int32_t sy1, int32_t Wi,
+                                        int32_t Hi, int32_t x0, int32_t y0) {
+    sx0 = max(sx0, 0); sx1 = min(sx1, Wi - 1);
+    sy0 = max(sy0, 0); sy1 = min(sy1, Hi - 1);
+    return sx0 >= x0 && sx1 < x0 + kBlk && sy0 >= y0 && sy1 < y0 + kBlk;
+}
+
+// Partial sums of one target over its source cells that lie in the block at
+// (x0, y0) and in the grid, from the block's cell partials in LDS (slot-major,
+// cells row-major), in the fixed order of the source offsets.
+__device__ inline bool block_cell(int32_t sx, int32_t sy, int32_t Wi, int32_t Hi, int32_t x0, int32_t y0, int& at) {
+    if (sx < 0 || sx >= Wi || sy < 0 || sy >= Hi) return false;
+    const int32_t lx = sx - x0, ly = sy - y0;
+    if (lx < 0 || lx >= kBlk || ly < 0 || ly >= kBlk) return false;
+    at = ly * kBlk + lx;
+    return true;
+}
+// u face (fx, fy): cells (fx - 1 + dx, fy - 1 + dy), slot (1 - dx) * 3 + (2 - dy)
+__device__ inline void part_u(const float* sb, int32_t fx, int32_t fy, int32_t Wi, int32_t Hi, int32_t x0,
+                              int32_t y0, float& m, float& w) {
+    m = 0.0f; w = 0.0f;
 #pragma unroll
-    for (int s = 0; s < 6; ++s) { a.um[s] = 0.0f; a.uw[s] = 0.0f; a.vm[s] = 0.0f; a.vw[s] = 0.0f; }
+    for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-    for (int s = 0; s < 9; ++s) a.d[s] = 0.0f;
+        for (int dx = 0; dx < 2; ++dx) {
+            int at;
+            if (!block_cell(fx - 1 + dx, fy - 1 + dy, Wi, Hi, x0, y0, at)) continue;
+            const int s = (1 - dx) * 3 + (2 - dy);
+            m += sb[(0 + s) * kBlkCells + at];
+            w += sb[(6 + s) * kBlkCells + at];
+        }
+}
+// v face (fx, fy): cells (fx - 1 + dx, fy - 1 + dy), slot (2 - dx) * 2 + (1 - dy)
+__device__ inline void part_v(const float* sb, int32_t fx, int32_t fy, int32_t Wi, int32_t Hi, int32_t x0,
+                              int32_t y0, float& m, float& w) {
+    m = 0.0f; w = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            int at;
+            if (!block_cell(fx - 1 + dx, fy - 1 + dy, Wi, Hi, x0, y0, at)) continue;
+            const int s = (2 - dx) * 2 + (1 - dy);
+            m += sb[(12 + s) * kBlkCells + at];
+            w += sb[(18 + s) * kBlkCells + at];
+        }
+}
+// cell (fx, fy): cells (fx - 1 + dx, fy - 1 + dy), slot (2 - dx) * 3 + (2 - dy)
+__device__ inline float part_d(const float* sb, int32_t fx, int32_t fy, int32_t Wi, int32_t Hi, int32_t x0,
+                               int32_t y0) {
+    float d = 0.0f;
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dx = 0; dx < 3; ++dx) {
+            int at;
+            if (!block_cell(fx - 1 + dx, fy - 1 + dy, Wi, Hi, x0, y0, at)) continue;
+            d += sb[(24 + (2 - dx) * 3 + (2 - dy)) * kBlkCells + at];
+        }
+    return d;
+}
+
+// The end of a face: the normalisation and valid flag (src/apic.rs:1428-1459)
+// and the external forces (:1604-1626; horizontal gravity is an extension,
+// zero in the reference).
+__device__ inline void finish_u(const P2gArgs& a, int32_t fx, int32_t fy, float m, float w, float dt, bool left,
+                                bool right) {
+    const size_t f = (size_t)fy * (a.W + 1) + fx;
+    const bool ok = w > 0.0f;
+    float u = ok ? m / w : 0.0f;
+    if (a.ext) u += a.ext_u[f] * dt;
+    if (a.forces && a.gravity_x != 0.0f && (left || right)) u += a.gravity_x * dt;
+    a.mac_u[f] = u;
+    a.valid_u[f] = ok;
+}
+__device__ inline void finish_v(const P2gArgs& a, int32_t fx, int32_t fy, float m, float w, float dt, bool below,
+                                bool above) {
+    const size_t f = (size_t)fy * a.W + fx;
+    const bool ok = w > 0.0f;
+    float v = ok ? m / w : 0.0f;
+    if (a.ext) v += a.ext_v[f] * dt;
+    if (a.forces && (below || above)) v += a.gravity * dt;
+    a.mac_v[f] = v;
+    a.valid_v[f] = ok;
+}
+
+// the x (even) bits of a Morton offset within a block
+__device__ inline int morton_even4(int t) { return (t & 1) | ((t >> 1) & 2) | ((t >> 2) & 4) | ((t >> 3) & 8); }
+
+__device__ __forceinline__ void p2g_block(const P2gArgs& a, uint32_t wg) {
+    __shared__ float s_buf[kOwnFields * kOwnCap];
+    __shared__ uint32_t s_red[2 * (kBlkCells / 64)];
+    __shared__ uint8_t s_fl[kBlkCells];  // fluid flags of the block's cells, row-major
+    const float prev_dt = a.ss ? a.ss->prev_dt : a.prev_dt;
+    const float dt = a.ss ? a.ss->dt : a.dt;
+    const int32_t Wi = (int32_t)a.W, Hi = (int32_t)a.H;
+    const uint32_t np = a.np;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const uint32_t bx = wg % a.tiles_x, by = wg / a.tiles_x;
+    const int32_t x0 = (int32_t)bx * kBlk, y0 = (int32_t)by * kBlk;
+    // Accumulation: thread tid takes the cell of Morton offset tid (its cell
+    // range is the tid-th of the block's 256 consecutive keys).
+    const int lx = morton_even4(tid), ly = morton_even4(tid >> 1);
+    const int32_t cx = x0 + lx, cy = y0 + ly;
+    uint32_t b = 0, e = 0;
+    bool fl = false;
+    if (cx < Wi && cy < Hi) {
+        const uint2 r = a.cell_range[(morton(bx, by) << 8) | (uint32_t)tid];
+        if (r.y > r.x && r.y <= np) { b = r.x; e = r.y; }
+        fl = e > b && !a.obstacle[(size_t)cy * a.W + cx];
+    }
+    // the block's run [run0, run1) in cell order
+    uint32_t mn = e > b ? b : 0xFFFFFFFFu, mx = e > b ? e : 0u;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor((int)mn, o, 64));
+        mx = max(mx, (uint32_t)__shfl_xor((int)mx, o, 64));
+    }
+    if (lane == 0) { s_red[wave] = mn; s_red[kBlkCells / 64 + wave] = mx; }
+    s_fl[ly * kBlk + lx] = fl;
+    __syncthreads();
+    uint32_t run0 = s_red[0], run1 = s_red[kBlkCells / 64];
+#pragma unroll
+    for (int w = 1; w < kBlkCells / 64; ++w) {
+        run0 = min(run0, s_red[w]);
+        run1 = max(run1, s_red[kBlkCells / 64 + w]);
+    }
+    const uint32_t total = run1 > run0 ? run1 - run0 : 0u;
+    const uint32_t my0 = e > b ? b - run0 : 0u, my1 = e > b ? e - run0 : 0u;
+
+    OwnerAcc acc;
+#pragma unroll
+    for (int s = 0; s < 6; ++s) { acc.um[s] = 0.0f; acc.uw[s] = 0.0f; acc.vm[s] = 0.0f; acc.vw[s] = 0.0f; }
+#pragma unroll
+    for (int s = 0; s < 9; ++s) acc.d[s] = 0.0f;
     for (uint32_t base = 0; base < total; base += kOwnCap) {
         const uint32_t n = min((uint32_t)kOwnCap, total - base);
-        {  // each cell marks its particles' slots of this pass
-            const uint32_t lo = max(my0, base), hi = min(my1, base + n);
-            for (uint32_t idx = lo; idx < hi; ++idx) s_cell[idx - base] = (uint8_t)tid;
-        }
-        __syncthreads();
-        // a thread's kOwnCap / kOwnCells slots: their positions in cell order,
+        // a thread's kOwnCap / kBlkCells slots: their positions in cell order,
         // then the permutation entries, then the fields, each stage's loads
         // issued together
-        constexpr int kPer = kOwnCap / kOwnCells;
-        static_assert(kPer * kOwnCells == kOwnCap, "whole slots per thread");
+        constexpr int kPer = kOwnCap / kBlkCells;
+        static_assert(kPer * kBlkCells == kOwnCap, "whole slots per thread");
         uint32_t src[kPer];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            const uint32_t k = (uint32_t)tid + (uint32_t)(j * kOwnCells);
-            const uint32_t kk = k < n ? k : 0u;
-            const int lo = s_cell[kk];  // the window cell holding base + kk
-            src[j] = s_src[lo] + (base + kk - s_off[lo]);  // position in cell order
+            const uint32_t k = (uint32_t)tid + (uint32_t)(j * kBlkCells);
+            src[j] = run0 + base + (k < n ? k : 0u);
         }
-        if (order) {
+        if (a.order) {
             // (a permutation entry past the records, written through the
             // field API, reads the last record, as the sort's gather does)
 #pragma unroll
-            for (int j = 0; j < kPer; ++j) src[j] = min(order[src[j]], np - 1);
+            for (int j = 0; j < kPer; ++j) src[j] = min(a.order[src[j]], np - 1);
         }
         float v[kPer][kOwnFields];
 #pragma unroll
         for (int j = 0; j < kPer; ++j)
 #pragma unroll
-            for (int f = 0; f < kOwnFields; ++f) v[j][f] = pb[(size_t)f * nalloc + src[j]];
+            for (int f = 0; f < kOwnFields; ++f) v[j][f] = a.pb[(size_t)f * a.nalloc + src[j]];
 #pragma unroll
         for (int j = 0; j < kPer; ++j) {
-            const uint32_t k = (uint32_t)tid + (uint32_t)(j * kOwnCells);
-            if (k < n)
+            const uint32_t k = (uint32_t)tid + (uint32_t)(j * kBlkCells);
+            if (k < n) {
+                owner_stage(v[j], prev_dt, a.cell);
 #pragma unroll
                 for (int f = 0; f < kOwnFields; ++f) s_buf[f * kOwnCap + k] = v[j][f];
+            }
         }
         __syncthreads();
         const uint32_t lo = max(my0, base), hi = min(my1, base + n);
         for (uint32_t idx = lo; idx < hi; ++idx)
-            owner_accumulate(s_buf, (int)(idx - base), prev_dt, cell, Wi, Hi, cx, cy, a);
+            owner_accumulate(s_buf, (int)(idx - base), a.cell, Wi, Hi, cx, cy, acc);
         __syncthreads();
     }
-    // partials of every window cell, slot-major
+    // partials of every block cell, slot-major, cells row-major
+    const int at = ly * kBlk + lx;
 #pragma unroll
     for (int s = 0; s < 6; ++s) {
-        s_buf[(0 + s) * kOwnCells + tid] = a.um[s];
-        s_buf[(6 + s) * kOwnCells + tid] = a.uw[s];
-        s_buf[(12 + s) * kOwnCells + tid] = a.vm[s];
-        s_buf[(18 + s) * kOwnCells + tid] = a.vw[s];
+        s_buf[(0 + s) * kBlkCells + at] = acc.um[s];
+        s_buf[(6 + s) * kBlkCells + at] = acc.uw[s];
+        s_buf[(12 + s) * kBlkCells + at] = acc.vm[s];
+        s_buf[(18 + s) * kBlkCells + at] = acc.vw[s];
     }
 #pragma unroll
-    for (int s = 0; s < 9; ++s) s_buf[(24 + s) * kOwnCells + tid] = a.d[s];
+    for (int s = 0; s < 9; ++s) s_buf[(24 + s) * kBlkCells + at] = acc.d[s];
     __syncthreads();
-    if (!inside || wx < 1 || wx > kOwnTile || wy < 1 || wy > kOwnTile) return;
-#define PART(slot, dwx, dwy) s_buf[(slot) * kOwnCells + tid + (dwy) * kOwnWin + (dwx)]
-    // u face (cx, cy): cells (cx - 1 + dx, cy - 1 + dy), slot (1 - dx) * 3 + (2 - dy)
-    {
-        float m = 0.0f, w = 0.0f;
-#pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 2; ++dx) {
-                const int s = (1 - dx) * 3 + (2 - dy);
-                m += PART(0 + s, dx - 1, dy - 1);
-                w += PART(6 + s, dx - 1, dy - 1);
+
+    // Targets: thread tid takes the cell (x0 + tid % 16, y0 + tid / 16), its u
+    // and v faces (and the faces x = W, y = H of the last column and row).
+    const int tx = tid % kBlk, ty = tid / kBlk;
+    const int32_t gx = x0 + tx, gy = y0 + ty;
+    auto fl_at = [&](int32_t x, int32_t y) -> bool {  // (x, y) in the block or outside the grid
+        return x >= 0 && x < Wi && y >= 0 && y < Hi && s_fl[(y - y0) * kBlk + (x - x0)];
+    };
+    if (gx < Wi && gy < Hi) {
+        a.fluid[(size_t)gy * a.W + gx] = s_fl[tid];
+        float m, w;
+        for (int32_t fx = gx; fx <= (gx == Wi - 1 ? Wi : gx); ++fx)
+            if (sources_in_block(fx - 1, fx, gy - 1, gy + 1, Wi, Hi, x0, y0)) {
+                part_u(s_buf, fx, gy, Wi, Hi, x0, y0, m, w);
+                finish_u(a, fx, gy, m, w, dt, fl_at(fx - 1, gy), fl_at(fx, gy));
             }
-        const size_t f = (size_t)cy * (W + 1) + cx;
-        const bool ok = w > 0.0f;
-        float u = ok ? m / w : 0.0f;
-        if (ext) u += ext_u[f] * dt;
-        if (forces && gravity_x != 0.0f && (s_fl[tid - 1] || s_fl[tid])) u += gx;
-        mac_u[f] = u;
-        valid_u[f] = ok;
-        if (cx == Wi - 1) {  // face x = W: only the cells x = W - 1 reach it
-            m = 0.0f; w = 0.0f;
-#pragma unroll
-            for (int dy = 0; dy < 3; ++dy) {
-                m += PART(0 + 3 + (2 - dy), 0, dy - 1);
-                w += PART(6 + 3 + (2 - dy), 0, dy - 1);
+        for (int32_t fy = gy; fy <= (gy == Hi - 1 ? Hi : gy); ++fy)
+            if (sources_in_block(gx - 1, gx + 1, fy - 1, fy, Wi, Hi, x0, y0)) {
+                part_v(s_buf, gx, fy, Wi, Hi, x0, y0, m, w);
+                finish_v(a, gx, fy, m, w, dt, fl_at(gx, fy - 1), fl_at(gx, fy));
             }
-            const bool ok2 = w > 0.0f;
-            u = ok2 ? m / w : 0.0f;
-            if (ext) u += ext_u[f + 1] * dt;
-            if (forces && gravity_x != 0.0f && s_fl[tid]) u += gx;
-            mac_u[f + 1] = u;
-            valid_u[f + 1] = ok2;
+        if (sources_in_block(gx - 1, gx + 1, gy - 1, gy + 1, Wi, Hi, x0, y0))
+            a.density[(size_t)gy * a.W + gx] = part_d(s_buf, gx, gy, Wi, Hi, x0, y0);
+    }
+    // the halo record (zeros for a block without particles)
+    float* rec = a.halo + (size_t)wg * kP2gHaloFloats;
+    if (total == 0) {
+        for (int q = tid; q < (int)kP2gHaloFloats; q += kBlkCells) rec[q] = 0.0f;
+        return;
+    }
+    for (int q = tid; q < 320; q += kBlkCells) {
+        int i, j;
+        if (q < 96) {
+            halo_u_pos(q, i, j);
+            float m, w;
+            part_u(s_buf, x0 + i, y0 - 1 + j, Wi, Hi, x0, y0, m, w);
+            rec[kHaloUm + q] = m;
+            rec[kHaloUw + q] = w;
+        } else if (q < 192) {
+            halo_v_pos(q - 96, i, j);
+            float m, w;
+            part_v(s_buf, x0 - 1 + i, y0 + j, Wi, Hi, x0, y0, m, w);
+            rec[kHaloVm + q - 96] = m;
+            rec[kHaloVw + q - 96] = w;
+        } else {
+            halo_d_pos(q - 192, i, j);
+            rec[kHaloDen + q - 192] = part_d(s_buf, x0 - 1 + i, y0 - 1 + j, Wi, Hi, x0, y0);
         }
     }
-    // v face (cx, cy): cells (cx - 1 + dx, cy - 1 + dy), slot (2 - dx) * 2 + (1 - dy)
-    {
-        float m = 0.0f, w = 0.0f;
-#pragma unroll
-        for (int dy = 0; dy < 2; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                const int s = (2 - dx) * 2 + (1 - dy);
-                m += PART(12 + s, dx - 1, dy - 1);
-                w += PART(18 + s, dx - 1, dy - 1);
-            }
-        const size_t f = (size_t)cy * W + cx;
-        const bool ok = w > 0.0f;
-        float v = ok ? m / w : 0.0f;
-        if (ext) v += ext_v[f] * dt;
-        if (forces && ((cy >= 1 && s_fl[tid - kOwnWin]) || s_fl[tid])) v += g;
-        mac_v[f] = v;
-        valid_v[f] = ok;
-        if (cy == Hi - 1) {  // face y = H: only the cells y = H - 1 reach it
-            m = 0.0f; w = 0.0f;
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) {
-                m += PART(12 + (2 - dx) * 2 + 1, dx - 1, 0);
-                w += PART(18 + (2 - dx) * 2 + 1, dx - 1, 0);
-            }
-            const bool ok2 = w > 0.0f;
-            v = ok2 ? m / w : 0.0f;
-            if (ext) v += ext_v[f + W] * dt;
-            if (forces && s_fl[tid]) v += g;
-            mac_v[f + W] = v;
-            valid_v[f + W] = ok2;
-        }
-    }
-    // cell (cx, cy): cells (cx - 1 + dx, cy - 1 + dy), slot (2 - dx) * 3 + (2 - dy)
-    {
-        float dsum = 0.0f;
-#pragma unroll
-        for (int dy = 0; dy < 3; ++dy)
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) dsum += PART(24 + (2 - dx) * 3 + (2 - dy), dx - 1, dy - 1);
-        const size_t c = (size_t)cy * W + cx;
-        density[c] = dsum;
-        fluid[c] = s_fl[tid];
-    }
-#undef PART
 }
 
-struct P2gOwner {
-    static constexpr int kThreads = kOwnCells;
-    static constexpr bool kRoundRobin = true;  // work follows the particles
-    struct Args {
-        const float* pb;
-        size_t nalloc;
-        uint32_t np;
-        const uint32_t* order;
-        float prev_dt;
-        float cell;
-        uint32_t W, H;
-        const uint2* cell_range;
-        uint32_t tiles_x;
-        float* mac_u;
-        float* mac_v;
-        uint8_t* valid_u;
-        uint8_t* valid_v;
-        float* density;
-        uint8_t* fluid;
-        const uint8_t* obstacle;
-        const float* ext_u;
-        const float* ext_v;
-        int forces, ext;
-        float dt, gravity, gravity_x;
-        const StepState* ss;
-    };
-    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        p2g_owner(a.pb, a.nalloc, a.np, a.order, a.prev_dt, a.cell, a.W, a.H, a.cell_range, a.tiles_x, a.mac_u,
-                  a.mac_v, a.valid_u, a.valid_v, a.density, a.fluid, a.obstacle, a.ext_u, a.ext_v, a.forces, a.ext,
-                  a.dt, a.gravity, a.gravity_x, a.ss, wg);
+// The targets of a block that P2gBlock left open: the band around its edge
+// whose sources reach into other blocks.  Each wave takes one kind of target:
+//   wave 0, lane q < 62: u faces: column 0, column 16 (the face x = W), rows
+//                        0 and 15 of columns 1-15;
+//   wave 1, lane q < 62: v faces: columns 0 and 15, row 0 of columns 1-14,
+//                        row 16 (the face y = H);
+//   wave 2, lane q < 60: cells: columns 0 and 15, rows 0 and 15 of columns
+//                        1-14;
+// and sums the halo records of the blocks that reach the target, in the order
+// of their block rows, then columns.
+constexpr int kSeamThreads = 192;
+template <int Kind>
+__device__ __forceinline__ void seam_target(const P2gArgs& a, uint32_t wg, int q) {
+    const int32_t Wi = (int32_t)a.W, Hi = (int32_t)a.H;
+    const int32_t bx = (int32_t)(wg % a.tiles_x), by = (int32_t)(wg / a.tiles_x);
+    const int32_t x0 = bx * kBlk, y0 = by * kBlk;
+    int32_t fx, fy, sx0, sx1, sy0, sy1;
+    bool valid;
+    if (Kind == 0) {
+        int i, t;
+        if (q < 16) { i = 0; t = q; }
+        else if (q < 32) { i = 16; t = q - 16; }
+        else if (q < 47) { i = q - 31; t = 0; }
+        else { i = q - 46; t = 15; }
+        fx = x0 + i; fy = y0 + t;
+        sx0 = fx - 1; sx1 = fx; sy0 = fy - 1; sy1 = fy + 1;
+        valid = q < 62 && fy < Hi && fx <= Wi && (i < 16 || fx == Wi);
+    } else if (Kind == 1) {
+        int t, j;
+        if (q < 16) { t = 0; j = q; }
+        else if (q < 32) { t = 15; j = q - 16; }
+        else if (q < 46) { t = q - 31; j = 0; }
+        else { t = q - 46; j = 16; }
+        fx = x0 + t; fy = y0 + j;
+        sx0 = fx - 1; sx1 = fx + 1; sy0 = fy - 1; sy1 = fy;
+        valid = q < 62 && fx < Wi && fy <= Hi && (j < 16 || fy == Hi);
+    } else {
+        int t, u;
+        if (q < 16) { t = 0; u = q; }
+        else if (q < 32) { t = 15; u = q - 16; }
+        else if (q < 46) { t = q - 31; u = 0; }
+        else { t = q - 45; u = 15; }
+        fx = x0 + t; fy = y0 + u;
+        sx0 = fx - 1; sx1 = fx + 1; sy0 = fy - 1; sy1 = fy + 1;
+        valid = q < 60 && fx < Wi && fy < Hi;
     }
+    if (!valid || sources_in_block(sx0, sx1, sy0, sy1, Wi, Hi, x0, y0)) return;  // P2gBlock finished it
+    const float dt = a.ss ? a.ss->dt : a.dt;
+    // The blocks that reach the target hold one of its source cells (the
+    // grid's or one past it): block rows r0..r1 and columns c0..c1, at most
+    // two each.  Their records and the fluid flags beside a face are loaded
+    // at once.
+    const int32_t c0 = sx0 >> 4, c1 = sx1 >> 4, r0 = sy0 >> 4, r1 = sy1 >> 4;  // (arithmetic shifts: -1 -> -1)
+    static_assert(kBlk == 16, "block index shifts");
+    constexpr int hm = Kind == 0 ? kHaloUm : Kind == 1 ? kHaloVm : kHaloDen;
+    constexpr int hw = Kind == 0 ? kHaloUw : kHaloVw;
+    bool ok[4];
+    float vm[4], vw[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int32_t ry = (k >> 1) ? r1 : r0, rx = (k & 1) ? c1 : c0;
+        ok[k] = ((k & 1) == 0 || c1 != c0) && ((k >> 1) == 0 || r1 != r0) && rx >= 0 &&
+                rx < (int32_t)a.tiles_x && ry >= 0 && ry < (int32_t)a.tiles_y;
+        const uint32_t nb = ok[k] ? (uint32_t)ry * a.tiles_x + (uint32_t)rx : wg;
+        const int32_t nx0 = rx * kBlk, ny0 = ry * kBlk;
+        int h;
+        if (Kind == 0) h = halo_u_index(fx - nx0, fy - (ny0 - 1));
+        else if (Kind == 1) h = halo_v_index(fx - (nx0 - 1), fy - ny0);
+        else h = halo_d_index(fx - (nx0 - 1), fy - (ny0 - 1));
+        const float* rec = a.halo + (size_t)nb * kP2gHaloFloats + (ok[k] ? h : 0);
+        vm[k] = rec[hm];
+        if (Kind != 2) vw[k] = rec[hw];
+    }
+    if (Kind == 2) {
+        float d = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (ok[k]) d += vm[k];
+        a.density[(size_t)fy * a.W + fx] = d;
+        return;
+    }
+    // fluid flags beside a face: (fx - 1, fy) and (fx, fy) for u, (fx, fy - 1)
+    // and (fx, fy) for v (cells outside the grid are not fluid)
+    const int32_t ax = Kind == 0 ? fx - 1 : fx, ay = Kind == 1 ? fy - 1 : fy;
+    const bool a_in = ax >= 0 && ax < Wi && ay >= 0 && ay < Hi, b_in = fx < Wi && fy < Hi;
+    const uint8_t fa = a.fluid[a_in ? (size_t)ay * a.W + ax : 0];
+    const uint8_t fb = a.fluid[b_in ? (size_t)fy * a.W + fx : 0];
+    float m = 0.0f, w = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (ok[k]) {
+            m += vm[k];
+            w += vw[k];
+        }
+    }
+    if (Kind == 0) finish_u(a, fx, fy, m, w, dt, a_in && fa, b_in && fb);
+    else finish_v(a, fx, fy, m, w, dt, a_in && fa, b_in && fb);
+}
+
+__device__ __forceinline__ void p2g_seam(const P2gArgs& a, uint32_t wg) {
+    const int wave = threadIdx.x >> 6, q = threadIdx.x & 63;
+    if (wave == 0) seam_target<0>(a, wg, q);
+    else if (wave == 1) seam_target<1>(a, wg, q);
+    else seam_target<2>(a, wg, q);
+}
+
+struct P2gBlock {
+    static constexpr int kThreads = kBlkCells;
+    static constexpr bool kRoundRobin = true;  // work follows the particles
+    using Args = P2gArgs;
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) { p2g_block(a, wg); }
+};
+
+struct P2gSeam {
+    static constexpr int kThreads = kSeamThreads;
+    static constexpr bool kDealRows = true;
+    using Args = P2gArgs;
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) { p2g_seam(a, wg); }
 };
 
 // [begin, end) of every cell's particles in cell order, indexed by key
@@ -721,14 +922,15 @@ void launch_p2g(Sim& sim, bool forces, float dt) {
     } else if (sim.perm_pending) {
         order = sim.vals[0];  // the sort's gather is deferred: read through its permutation
     }
-    const uint32_t tx = (sim.W + kOwnTile - 1) / kOwnTile, ty = (sim.H + kOwnTile - 1) / kOwnTile;
+    const uint32_t tx = p2g_blocks(sim.W), ty = p2g_blocks(sim.H);
     const int ext = forces && sim.forces_active;
     sim.mark(APIC_T_K_P2G, 0);
-    P2gOwner::Args a{sim.pbuf[sim.cur], sim.nalloc, sim.num_particles, order, sim.prev_dt, sim.cell, sim.W, sim.H,
-                     sim.cell_range, tx, sim.mac_u, sim.mac_v, sim.valid_u, sim.valid_v, sim.density,
-                     sim.levels[0].fluid, sim.obstacle, sim.ext_u, sim.ext_v, (int)forces, ext, dt,
-                     sim.props.gravity, sim.gravity_x, sim.step_src};
-    emit<P2gOwner>(sim, a, tx * ty);
+    P2gArgs a{sim.pbuf[sim.cur], sim.nalloc, sim.num_particles, order, sim.prev_dt, sim.cell, sim.W, sim.H,
+              sim.cell_range, tx, ty, sim.mac_u, sim.mac_v, sim.valid_u, sim.valid_v, sim.density,
+              sim.levels[0].fluid, sim.obstacle, sim.ext_u, sim.ext_v, sim.p2g_halo, (int)forces, ext,
+              dt, sim.props.gravity, sim.gravity_x, sim.step_src};
+    emit<P2gBlock>(sim, a, tx * ty);
+    emit<P2gSeam>(sim, a, tx * ty);
     sim.mark(APIC_T_K_P2G, 1);
 }
 

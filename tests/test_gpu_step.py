@@ -2,7 +2,8 @@
 
 Tolerances (relative L2 over all particles):
   - 1 update: 1e-5.  100 updates with the particle sort disabled: positions
-    1e-5; velocities 1e-5 or four times the oracle's self-spread.
+    1e-5; velocities 1e-5 or twice the oracle's largest self-spread over four
+    other valid scatter orders.
   - 100 updates with the sort on: the reference's step is chaotic here.  The
     jitter of each particle is hashed from its post-sort index
     (src/apic.rs:2181), so a rounding-level difference that moves one
@@ -57,32 +58,39 @@ def test_one_update(name):
 
 def test_100_updates_without_sort():
     """Without the sort no re-ordering amplifies rounding: positions must stay
-    within 1e-5; velocities within 1e-5 or four times the oracle's own
-    spread under another valid scatter order, whichever is larger."""
+    within 1e-5; velocities within 1e-5 or twice the oracle's own spread,
+    whichever is larger.  The spread is the largest deviation of the oracle
+    under four other valid orders of the reference's parallel scatter: after
+    100 updates single orders deviate by 4e-6 to 2.4e-5 in velocity (8 orders
+    sampled), so one order alone does not measure it."""
     from FLIP import scenes
     from FLIP.apic import DebugOptions
     from oracle import pyoracle as po
     g, o, dt = pair("dam128")
     o64 = pair("dam128", precision=64)[1]
     xy = scenes.config_scene("cfg1_dam_break_128")
-    o2 = po.Oracle(128, 128, 1.0 / 128, po.bench_props(), xy=xy)
-    o2.set_p2g_shuffle(12345)
+    shuffled = []
+    for seed in (12345, 1, 2, 3):
+        o2 = po.Oracle(128, 128, 1.0 / 128, po.bench_props(), xy=xy)
+        o2.set_p2g_shuffle(seed)
+        shuffled.append(o2)
     iters_g, iters_o = [], []
     for _ in range(100):
         g.update(DebugOptions(True), dt)
         o.update(dt, disable_sort=True)
-        o2.update(dt, disable_sort=True)
+        for o2 in shuffled:
+            o2.update(dt, disable_sort=True)
         o64.update(dt, disable_sort=True)
         iters_g.append(g.stats()["pcg_iterations"])
         iters_o.append(o.stats()["pcg_iterations"])
     pos = max(rel_l2(g.read(f), o.read(f)) for f in ("POS_X", "POS_Y"))
     vel = max(rel_l2(g.read(f), o.read(f)) for f in ("VEL_X", "VEL_Y"))
-    spread = max(rel_l2(o2.read(f), o.read(f)) for f in ("VEL_X", "VEL_Y"))
+    spread = max(rel_l2(o2.read(f), o.read(f)) for f in ("VEL_X", "VEL_Y") for o2 in shuffled)
     print(f"100 updates, sort off: GPU vs FP32 oracle positions {pos:.2e}, velocities {vel:.2e} "
           f"(oracle self-spread {spread:.2e}); vs FP64 oracle {particle_err(g, o64):.2e}; "
           f"PCG iterations {sum(iters_g)} (GPU) vs {sum(iters_o)} (oracle)")
     assert pos <= 1e-5
-    assert vel <= max(1e-5, 4.0 * spread)
+    assert vel <= max(1e-5, 2.0 * spread)
     assert max(abs(a - b) for a, b in zip(iters_g, iters_o)) <= 1
     assert rel_l2(g.read("PRESSURE"), o.read("PRESSURE")) <= 1e-3
 
