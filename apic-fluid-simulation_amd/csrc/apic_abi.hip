@@ -287,6 +287,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     ALLOC(s.part_gamma, ngam); ALLOC(s.part_delta, kRedBlocks); ALLOC(s.part_max, kRedBlocks);
     ALLOC(s.part_fluid, 2 * kRedBlocks);
     ALLOC(s.part_gamma_d, ngam); ALLOC(s.part_asz_d, ngam); ALLOC(s.part_rr_t, ngam);
+    ALLOC(s.tile_flag, gamma_partials(s)); ALLOC(s.tile_order, gamma_partials(s));
     ALLOC(s.part_delta_d, kRedBlocks); ALLOC(s.part_rr_d, kRedBlocks);
     ALLOC(s.part_rr2_d, kRedBlocks); ALLOC(s.pressure_d, nc);
     s.part_maxv_cap = 4096;  // G2P's workgroups: 2 particles per thread and step up to 2M particles
@@ -1243,6 +1244,7 @@ apic_status apic_write_field(apic_handle* h, apic_field field, uint32_t level, c
     if (count != d.count) return fail(APIC_ERR_INVALID_ARGUMENT, "count does not match the field length");
     HIP_TRY(hipStreamSynchronize(s.stream));
     if (count) HIP_TRY(hipMemcpy(d.ptr, src, count * d.elem, hipMemcpyHostToDevice));
+    s.tile_order_valid = false;  // the solver's flags may have changed: base tiles in their own order
     if (field >= APIC_POS_X && field <= APIC_TIME_RESIDUAL) s.cells_valid = false;
     if (field == APIC_POS_X || field == APIC_POS_Y) s.keys_valid = false;
     if (field == APIC_VEL_X || field == APIC_VEL_Y) s.maxv_valid = false;

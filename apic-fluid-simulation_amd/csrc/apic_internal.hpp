@@ -233,6 +233,12 @@ struct Sim {
     // a tile ring that another workgroup of the same launch writes)
     std::vector<float*> down_e;
     bool bottom_in_lds = false;     // that kernel holds its levels in LDS
+    // The base tiles' launch order (TileOrder, pressure.hip): per base tile a
+    // mark that it holds fluid (set by MgBuild), and the order itself, valid
+    // from the hierarchy's build until a field is written
+    uint8_t* tile_flag = nullptr;
+    uint32_t* tile_order = nullptr;
+    bool tile_order_valid = false;
     float* search[2] = {nullptr, nullptr};
     float* As = nullptr;
     float* As_lo = nullptr;          // relative mode: A s (FP64) - As

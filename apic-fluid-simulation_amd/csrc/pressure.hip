@@ -216,7 +216,7 @@ struct FstatsFinalize {
     }
 };
 
-__device__ inline void mg_build_cell(const LevelView& F, const LevelView& C, uint32_t c) {
+__device__ inline bool mg_build_cell(const LevelView& F, const LevelView& C, uint32_t c) {
     uint32_t cx = c % C.w, cy = c / C.w;
     size_t pcc = (size_t)c + C.w;
     bool any = false;
@@ -248,16 +248,24 @@ __device__ inline void mg_build_cell(const LevelView& F, const LevelView& C, uin
     } else {
         C.diag[pcc] = 0.0f; C.inv_diag[pcc] = 0.0f; C.px[pcc] = 0.0f; C.py[pcc] = 0.0f;
     }
+    return any;
 }
 
+// tile_flag (level 1 only, else null): a fluid coarse cell marks the base
+// tile above it (a kBaseTile x kBaseTile tile holds kBaseTile / 2 coarse
+// cells a side); TileOrder reads the marks and clears them.
 struct MgBuild {
     static constexpr int kThreads = kSolverThreads;
     struct Args {
         LevelView F, C;
+        uint8_t* tile_flag;
+        uint32_t tiles_x;
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
         const uint32_t c = wg * kThreads + threadIdx.x;
-        if (c < a.C.cells) mg_build_cell(a.F, a.C, c);
+        if (c >= a.C.cells) return;
+        const bool fl = mg_build_cell(a.F, a.C, c);
+        if (fl && a.tile_flag) a.tile_flag[(c / a.C.w) / 16 * a.tiles_x + (c % a.C.w) / 16] = 1;
     }
 };
 
@@ -266,11 +274,105 @@ struct LevelSet {
     int count;  // number of levels in lv (lv[0] = base)
 };
 
+// The base tiles' launch order for a simulation stepped alone.  A base-level
+// launch has a workgroup per tile, and most tiles of a scene hold no fluid:
+// such a tile only takes part in the solve's decisions and writes zero
+// partials.  In launch order those tiles would hold their CU slots while
+// fluid tiles behind them wait.  TileOrder keeps each tile on the workgroups
+// of its XCD (launch index i runs on XCD (i + c) mod 8, and one_kernel deals
+// tile rows to the XCDs) but gives the fluid tiles of each XCD its first
+// workgroups, in their order; it zeroes the partials of the tiles without
+// fluid (they stay zero for the whole solve), and order[l] = the tile that
+// launch-logical workgroup l runs, bit 31 set for a tile without fluid.
+// One wave per XCD residue class, up to kOrderMaxPer tiles a class.  The
+// marks come from MgBuild (level 1) and are cleared here.
+// Threads 0 .. 64 kXcds - 1 of a workgroup run it: MgBuildBottom's after
+// the bottom levels, or TileOrder's own launch.
+constexpr int kOrderMaxPer = 32;  // tiles per lane and class
+constexpr uint32_t kOrderMaxTiles = kXcds * 64 * kOrderMaxPer;
+constexpr int kOrderThreads = 64 * kXcds;
+struct TileOrderArgs {
+    uint8_t* flag;
+    uint32_t* order;
+    uint32_t n, tiles_x;
+    uint32_t cmul;  // ceil(2^32 / tiles_x) for tiles_x > 1 (a quotient by tiles_x as a multiply), else 0
+    double* part_rr_t;
+    double* part_gamma_d;
+    double* part_asz_d;
+    float* part_gamma;
+};
+__device__ __forceinline__ void tile_order(const TileOrderArgs& a) {
+    if (threadIdx.x >= (unsigned)kOrderThreads) return;
+    const uint32_t x = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const uint32_t n = a.n, c = a.tiles_x;
+    const uint32_t per = x < n ? (n - x + kXcds - 1) / kXcds : 0u;  // launch indices x, x + 8, ... below n
+    const uint32_t full = n / (kXcds * c) * (kXcds * c);
+    // xcd_deal(hw, n, c), its quotient by c from the multiplier (exact:
+    // the slots and c are below 2^16)
+    auto deal = [&](uint32_t hw) -> uint32_t {
+        if (hw < full) {
+            const uint32_t xcd = hw % kXcds, slot = hw / kXcds;
+            const uint32_t q = a.cmul ? __umulhi(slot, a.cmul) : slot;
+            return (q * kXcds + xcd) * c + (slot - q * c);
+        }
+        return full + xcd_band(hw - full, n - full);
+    };
+    uint32_t tl[kOrderMaxPer];
+    bool fl[kOrderMaxPer];
+#pragma unroll
+    for (int m = 0; m < kOrderMaxPer; ++m) {
+        tl[m] = 0u;
+        fl[m] = false;
+        const uint32_t s = lane + 64u * m;
+        if (64u * m < per && s < per) {
+            tl[m] = deal(x + kXcds * s);
+            fl[m] = a.flag[tl[m]] != 0;
+        }
+    }
+    uint32_t nw = 0;
+#pragma unroll
+    for (int m = 0; m < kOrderMaxPer; ++m)
+        if (64u * m < per) nw += (uint32_t)__popcll(__ballot(fl[m]));
+    const uint64_t below = (1ull << lane) - 1ull;
+    uint32_t rw = 0, rn = 0;
+#pragma unroll
+    for (int m = 0; m < kOrderMaxPer; ++m) {
+        if (64u * m >= per) break;  // (uniform in the wave)
+        const uint32_t s = lane + 64u * m;
+        const uint64_t bw = __ballot(fl[m]), bv = __ballot(s < per);
+        const uint64_t bn = bv & ~bw;
+        if (s < per) {
+            const uint32_t pos = fl[m] ? rw + (uint32_t)__popcll(bw & below)
+                                       : nw + rn + (uint32_t)__popcll(bn & below);
+            a.order[deal(x + kXcds * pos)] = tl[m] | (fl[m] ? 0u : 0x80000000u);
+            a.flag[tl[m]] = 0;
+            if (!fl[m]) {
+                a.part_rr_t[tl[m]] = 0.0;
+                a.part_gamma_d[tl[m]] = 0.0;
+                a.part_asz_d[tl[m]] = 0.0;
+                a.part_gamma[tl[m]] = 0.0f;
+            }
+        }
+        rw += (uint32_t)__popcll(bw);
+        rn += (uint32_t)__popcll(bn);
+    }
+}
+
+struct TileOrder {
+    static constexpr int kThreads = kOrderThreads;
+    using Args = TileOrderArgs;
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) { tile_order(a); }
+};
+
+// with_order: TileOrder's work after the levels (one launch fewer)
 struct MgBuildBottom {
     static constexpr int kThreads = kBottomThreads;
+    static_assert(kBottomThreads >= kOrderThreads, "the tile order's waves");
     struct Args {
         LevelSet ls;
         int first;
+        int with_order;
+        TileOrderArgs order;
     };
     static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
         for (int l = a.first; l < a.ls.count; ++l) {
@@ -279,6 +381,7 @@ struct MgBuildBottom {
             for (uint32_t c = threadIdx.x; c < C.cells; c += blockDim.x) mg_build_cell(F, C, c);
             __syncthreads();
         }
+        if (a.with_order) tile_order(a.order);
     }
 };
 
@@ -1708,6 +1811,7 @@ struct Decide {
 
 constexpr int kVH = 2;         // ring: the first colour of a sweep needs two
 constexpr int kBaseTile = 32;  // tile side on the base level (one partial of r.z per tile)
+static_assert(kBaseTile / 2 == 16, "MgBuild marks base tiles of 16 x 16 coarse cells");
 constexpr int kWideTileThreads = 512;
 
 template <int T, int NT = 256>
@@ -1958,13 +2062,15 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
                              const int32_t* __restrict__ done, uint32_t tiles_x, PcgState* st, int check,
                              float solve_error, const float* __restrict__ part_max, const double* __restrict__ part_rr,
                              const double* __restrict__ part_rr_t, uint32_t ntiles, uint32_t nred, const AxIn& ax,
-                             int k, uint32_t wg) {
+                             int k, uint32_t wg, uint32_t lwg) {
+    // wg: the tile; lwg: the workgroup's index in the launch (workgroup 0
+    // records the solve's decisions)
     using G = VGeom<T, NT>;
     constexpr int kR = G::kR;
     __shared__ VTile<T> t;
     __shared__ float scratch[NT / 64];
     __shared__ double dscratch[NT / 64];
-    const bool lead = wg == 0 && threadIdx.x == 0;
+    const bool lead = lwg == 0 && threadIdx.x == 0;
     const int32_t X0 = (int32_t)(wg % tiles_x) * T, Y0 = (int32_t)(wg / tiles_x) * T;
     const int32_t w = (int32_t)F.w, h = (int32_t)F.h;
     const int32_t cw = (int32_t)C.w, ch = (int32_t)C.h;
@@ -2062,7 +2168,7 @@ __device__ __forceinline__ void vdown(const LevelView& F, const LevelView& C, in
     if constexpr (AX) {
         // no fluid cell in the tile: a zero partial, as axpy_tile's (workgroup
         // 0 goes on to record alpha)
-        if (!has_fluid && wg != 0) {
+        if (!has_fluid && lwg != 0) {
             if (threadIdx.x == 0) ax.part_rr_out[wg] = 0.0;
             return;
         }
@@ -2214,7 +2320,19 @@ struct VDownArgs {
     int every;                // the true residual is recomputed after iteration k when (k + 1) % every == 0
     uint32_t nred;
     AxIn ax;                  // VDownAx: the iteration's update
+    const uint32_t* order;    // base level of a simulation stepped alone (else null): TileOrder's tile order
 };
+
+// The tile a base-level workgroup runs: its own, or with TileOrder's order
+// the one that order gives it, and whether it may leave at once (a tile
+// without fluid whose partials TileOrder zeroed; workgroup 0 always runs).
+__device__ __forceinline__ bool ordered_tile(const uint32_t* order, uint32_t wg, uint32_t& tile) {
+    tile = wg;
+    if (!order) return true;
+    const uint32_t o = order[wg];
+    tile = o & 0x7FFFFFFFu;
+    return (o >> 31) == 0u || wg == 0;
+}
 
 template <int T, int NT = 256, bool AX = false>
 struct VDown {
@@ -2223,10 +2341,12 @@ struct VDown {
     static constexpr bool kDealRows = true;  // its tile rows are dealt to the XCDs in turn
     // k: the PCG iteration whose update the residual test checks
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int k) {
+        uint32_t tile;
+        if (!ordered_tile(a.order, wg, tile)) return;
         const bool due = a.every > 0 && (k + 1) % a.every == 0;
         vdown<T, NT, AX>(a.F, a.C, a.base, a.fuse_red, a.spec, a.done, a.tiles_x, a.st, a.check, a.solve_error,
                          a.part_max, due ? a.part_rr2 : nullptr, due ? nullptr : a.part_rr_t, a.ntiles, a.nred, a.ax,
-                         k, wg);
+                         k, tile, wg);
     }
 };
 // The base level's down-sweep with iteration k's update (axpy_tile) applied
@@ -2832,6 +2952,7 @@ struct VUpArgs {
     float* e_out;         // the level's e; L.e is its down-sweep buffer
     float* r_home;        // base level after VDownAx (L.r its second buffer): the residual's own buffer
     const int32_t* r_in_alt;
+    const uint32_t* order;  // base level of a simulation stepped alone (else null): TileOrder's tile order
 };
 
 template <int T, int NT = 256>
@@ -2840,10 +2961,14 @@ struct VUp {
     using Args = VUpArgs;
     static constexpr bool kDealRows = true;  // its tile rows are dealt to the XCDs in turn
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        uint32_t tile;
+        if (!ordered_tile(a.order, wg, tile)) return;
         vup<T, NT>(a.L, a.C, a.fs, a.done, a.tiles_x, a.dot, a.part_gamma, a.part_gamma_d, a.As, a.part_asz_d, a.spec,
-               a.e_out, a.r_home, a.r_in_alt, wg);
+               a.e_out, a.r_home, a.r_in_alt, tile);
     }
 };
+
+
 
 unsigned blocks_for(size_t n, unsigned threads = kThreads) { return (unsigned)((n + threads - 1) / threads); }
 
@@ -2912,13 +3037,7 @@ void launch_fluid_stats(Sim& sim) {
     emit<FstatsFinalize>(sim, {sim.part_fluid, sim.nred, sim.fstats, sim.h_fstats_d}, 1);
 }
 
-void launch_mg_build(Sim& sim) {
-    const int n = (int)sim.level_count;
-    int l = 1;
-    for (; l < n && l < (int)sim.bottom_level; ++l)
-        emit<MgBuild>(sim, {sim.levels[l - 1], sim.levels[l]}, blocks_for(sim.levels[l].cells));
-    if (l < n) emit<MgBuildBottom>(sim, {level_set(sim), l}, 1);
-}
+
 
 // The base level's up-sweep writes one partial of r.z per tile.
 size_t gamma_partials(const Sim& sim) {
@@ -2965,6 +3084,13 @@ bool fused_update(const Sim& sim, int k) {
 // waves.  AxpyTile takes the same count, so that its partials are VDownAx's.
 bool wide_base(const Sim& sim) { return sim.nred >= kWideMinRed; }
 
+// Whether the base-level V-cycle kernels of this simulation run their tiles
+// in TileOrder's order: a simulation stepped alone whose base level takes the
+// wide kernels and whose level 1 MgBuild builds.
+bool ordered_base(const Sim& sim) {
+    return !sim.rec && wide_base(sim) && 1 < (int)sim.bottom_level && (uint32_t)gamma_partials(sim) <= kOrderMaxTiles;
+}
+
 // The update's operands (AxpyTile, VDownAx), r_out receiving the residual.
 AxIn update_operands(const Sim& sim, float* r_out) {
     return AxIn{{sim.search[0], sim.search[1]}, sim.As, sim.As_lo, sim.pressure, sim.pressure_d, sim.part_delta_d, sim.nred,
@@ -2975,6 +3101,23 @@ AxIn update_operands(const Sim& sim, float* r_out) {
 
 // Before the solve: the state, and in the relative mode b = r (kept for the
 // residual recomputation) and ||b||^2.
+void launch_mg_build(Sim& sim) {
+    const int n = (int)sim.level_count;
+    const bool ordered = ordered_base(sim);
+    uint32_t btx = 0;
+    const uint32_t nt = tiles_of(sim.levels[0], kBaseTile, btx);
+    int l = 1;
+    for (; l < n && l < (int)sim.bottom_level; ++l)
+        emit<MgBuild>(sim, {sim.levels[l - 1], sim.levels[l], (ordered && l == 1) ? sim.tile_flag : nullptr, btx},
+                      blocks_for(sim.levels[l].cells));
+    const uint32_t cmul = btx > 1 ? (uint32_t)(((1ull << 32) + btx - 1) / btx) : 0u;
+    const TileOrderArgs order{sim.tile_flag, sim.tile_order, nt, btx, cmul, sim.part_rr_t, sim.part_gamma_d,
+                              sim.part_asz_d, sim.part_gamma};
+    if (l < n) emit<MgBuildBottom>(sim, {level_set(sim), l, (int)ordered, order}, 1);
+    else if (ordered) emit<TileOrder>(sim, order, 1);
+    sim.tile_order_valid = ordered;
+}
+
 void launch_pcg_reset(Sim& sim) {
     const bool rel = sim.solver.mode == 1;
     const double tol = (double)sim.solver.rel_tol;
@@ -3008,6 +3151,9 @@ void launch_vcycle(Sim& sim, int mode, int k) {
     const bool fused = mode == 2 && fused_update(sim, k);
     const uint32_t ntiles = (uint32_t)gamma_partials(sim);
     const AxIn ax = update_operands(sim, fused ? sim.r_alt : sim.levels[0].r);
+    // TileOrder's order for the wide base kernels (built with the hierarchy
+    // of this substep; not after fields were written since)
+    const uint32_t* base_order = (sim.tile_order_valid && ordered_base(sim)) ? sim.tile_order : nullptr;
     // A level above the bottom with its down-sweep buffer as e: the way down
     // reads and writes it, the way up reads it and writes the level's e
     // (the bottom's levels keep theirs: one workgroup runs them)
@@ -3039,6 +3185,7 @@ void launch_vcycle(Sim& sim, int mode, int k) {
                           (int)(l == 0 && mode == 2 && !fused), sim.props.solve_error, sim.part_max,
                           rel_check ? sim.part_rr_t : nullptr, ntiles, rel_check ? sim.part_rr2_d : nullptr,
                           rel_check ? sim.solver.recompute_every : 0, sim.nred, ax};
+        if (l == 0 && wide) a.order = base_order;
         if (ax_here && wide) emit<VDownAx<kWideTileThreads>>(sim, a, nt, k);
         else if (ax_here) emit<VDownAx<kSolverThreads>>(sim, a, nt, k);
         else if (sm) emit<VDown<16>>(sim, a, nt, k);
@@ -3074,7 +3221,10 @@ void launch_vcycle(Sim& sim, int mode, int k) {
                         sim.part_gamma, sim.part_gamma_d, sim.As, sim.part_asz_d, (int)(l > 0), sim.levels[l].e,
                         home ? sim.levels[0].r : nullptr, &sim.pcg->r_in_alt};
         if (sm) emit<VUp<16>>(sim, a, nt);
-        else if (l == 0 && wide && dot != 1) emit<VUp<kBaseTile, kWideTileThreads>>(sim, a, nt);
+        else if (l == 0 && wide && dot != 1) {
+            a.order = base_order;
+            emit<VUp<kBaseTile, kWideTileThreads>>(sim, a, nt);
+        }
         else emit<VUp<kBaseTile>>(sim, a, nt);
     }
 }

@@ -51,11 +51,12 @@ def algorithmic_bytes(n_p, n_alloc, n_f, n_c, levels):
     the first substep reads its records through it, writing them in sorted
     order) and the next sort record (8 B); per fluid cell read one u and one v
     face value (8 B) and their open flags (2 B).
-    P2gOwner: per particle read the 9 fields (36 B) and its permutation entry
-    (4 B: the records are read through the sort's permutation); per grid cell
-    write its u face and v face (4 + 4 B), their valid flags (1 + 1 B), the
-    density (4 B) and the fluid flag (1 B) -- the reference rewrites every
-    face and cell in this stage.
+    P2G (P2gBlock and P2gSeam): per particle read the 9 fields (36 B) and its
+    permutation entry (4 B: the records are read through the sort's
+    permutation); per grid cell write its u face and v face (4 + 4 B), their
+    valid flags (1 + 1 B), the density (4 B) and the fluid flag (1 B) -- the
+    reference rewrites every face and cell in this stage.  (The blocks' halo
+    records, 2 KB per 16 x 16 block written and read back, are overhead.)
     SpmvDot: per fluid cell read z, the old search vector, diag,
     plus_x, plus_y (20 B) and write the new search vector and A s (8 B); the
     fluid flag of every cell (1 B).
@@ -103,7 +104,7 @@ def stage_rates(stage_ms, n_p, n_c, n_f, levels, iters, width, height):
     return out
 
 
-KERNEL_NAMES = {"kernel_g2p": "one_kernel<G2p<2> >", "kernel_p2g": "one_kernel<P2gOwner>",
+KERNEL_NAMES = {"kernel_g2p": "one_kernel<G2p<2> >", "kernel_p2g": "P2G (P2gBlock, P2gSeam)",
                 "kernel_spmv": "one_kernel<SpmvDot<1024> >",
                 "kernel_pcg_iteration": "PCG iteration (SpmvDot<1024>, VDown<32, 512, true>, 2 x VDown2, "
                                         "BottomLds, 2 x VUp2, VUp<32, 512>)"}
@@ -111,6 +112,9 @@ KERNEL_NAMES = {"kernel_g2p": "one_kernel<G2p<2> >", "kernel_p2g": "one_kernel<P
 PCG_ITERATION_KERNELS = {"one_kernel<SpmvDot<1024> >": 1, "one_kernel<VDown<32, 512, true> >": 1,
                          "one_kernel<VDown2>": 2, "one_kernel<BottomLds>": 1, "one_kernel<VUp2>": 2,
                          "one_kernel<VUp<32, 512> >": 1}
+# the launches of a unit timed as a whole
+UNIT_KERNELS = {"kernel_p2g": {"one_kernel<P2gBlock>": 1, "one_kernel<P2gSeam>": 1},
+                "kernel_pcg_iteration": PCG_ITERATION_KERNELS}
 BATCH_WORKLOAD = "cfg4_batched_256"
 LARGE_WORKLOAD = "cfg3_falling_block_4096x2048"
 
@@ -586,10 +590,11 @@ def traffic_summary(unit):
     per = tr.get("hbm_bytes_per_launch")
     if not isinstance(per, dict):
         return None
-    if unit == "kernel_pcg_iteration":
-        if not all(k in per for k in PCG_ITERATION_KERNELS):
+    if unit in UNIT_KERNELS:
+        ks = UNIT_KERNELS[unit]
+        if not all(k in per for k in ks):
             return None
-        return int(sum(per[k] * n for k, n in PCG_ITERATION_KERNELS.items()))
+        return int(sum(per[k] * n for k, n in ks.items()))
     v = per.get(KERNEL_NAMES[unit])
     return int(v) if v is not None else None
 
