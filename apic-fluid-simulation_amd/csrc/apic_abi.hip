@@ -199,7 +199,7 @@ apic_status build(Sim& s, uint32_t W, uint32_t H, float cell, const apic_world_p
     const size_t nu = s.nu(), nv = s.nv(), nc = s.cells;
     ALLOC(s.mac_u, nu); ALLOC(s.mac_v, nv); ALLOC(s.old_u, nu); ALLOC(s.old_v, nv);
     ALLOC(s.valid_u, nu); ALLOC(s.valid_v, nv); ALLOC(s.nvalid_u, nu); ALLOC(s.nvalid_v, nv);
-    ALLOC(s.density, nc); ALLOC(s.smooth_a, nc); ALLOC(s.smooth_b, nc); ALLOC(s.pressure, nc);
+    ALLOC(s.density, nc); ALLOC(s.smooth_a, nc); ALLOC(s.pressure, nc);
     ALLOC(s.obstacle, nc); ALLOC(s.open_u, nu); ALLOC(s.open_v, nv);
     ALLOC(s.ext_u, nu); ALLOC(s.ext_v, nv); ALLOC(s.post_u, nu); ALLOC(s.post_v, nv);
     s.levels.clear();

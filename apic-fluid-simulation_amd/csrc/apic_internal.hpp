@@ -205,7 +205,7 @@ struct Sim {
     uint8_t* valid_u = nullptr; uint8_t* valid_v = nullptr;
     uint8_t* nvalid_u = nullptr; uint8_t* nvalid_v = nullptr;
     float* density = nullptr;
-    float* smooth_a = nullptr; float* smooth_b = nullptr;  // smoothed density ends in smooth_a
+    float* smooth_a = nullptr;  // smoothed density
     float* pressure = nullptr;
     uint8_t* obstacle = nullptr;
     uint8_t* open_u = nullptr; uint8_t* open_v = nullptr;

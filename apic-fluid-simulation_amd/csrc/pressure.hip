@@ -58,33 +58,6 @@ __device__ inline float atomic_load_f(const float* p) {
 
 // ---------------------------------------------------------------- assembly
 
-struct SmoothDensity {
-    static constexpr int kThreads = kSolverThreads;
-    struct Args {
-        const float* in;
-        float* out;
-        const uint8_t* fluid;
-        uint32_t W, cells;
-    };
-    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        const uint32_t c = wg * kThreads + threadIdx.x;
-        if (c >= a.cells) return;
-        const float* __restrict__ in = a.in;
-        const uint8_t* __restrict__ fluid = a.fluid;
-        const uint32_t W = a.W;
-        if (!fluid[c]) { a.out[c] = in[c]; return; }
-        float sum = in[c] * 4.0f;
-        float weight = 4.0f;
-        float fl = (float)fluid[c - 1], fr = (float)fluid[c + 1], fb = (float)fluid[c - W], ft = (float)fluid[c + W];
-        sum += in[c - 1] * fl;
-        sum += in[c + 1] * fr;
-        sum += in[c - W] * fb;
-        sum += in[c + W] * ft;
-        weight += ((fl + fr) + fb) + ft;
-        a.out[c] = sum / weight;
-    }
-};
-
 // Per-workgroup (count, last raw index + 1) of the base fluid cells, reduced
 // by fstats_finalize_kernel: the padded-list quirk of prolongate needs both.
 __device__ inline void fluid_partial(bool fl, uint32_t c, uint32_t& cnt, uint32_t& last) {
@@ -507,6 +480,92 @@ __device__ __forceinline__ bool block_any(bool pred, uint32_t* slots) {
     for (int w = 0; w < NT / 64; ++w) any |= slots[w];
     return any != 0;
 }
+
+// The three smoothing passes of build_pressure_system (src/apic.rs:1653-1685)
+// in one launch.  A workgroup loads a kSmT x kSmT tile of the density and the
+// fluid flags with a kSmR-cell ring into LDS, runs the first pass over the
+// tile and a two-cell ring, the second over the tile and a one-cell ring, the
+// third over the tile, and writes the tile: each pass reads only values the
+// pass before computed inside the workgroup.  Per cell it is the reference's
+// arithmetic: a fluid cell becomes (4 rho + the fluid neighbours' rho) / (4 +
+// their count), every other cell keeps its value.  Cells past the grid enter
+// as non-fluid zeros (no fluid cell borders the grid's edge: the edge cells
+// are walls).
+constexpr int kSmT = 32;
+constexpr int kSmR = 3;
+constexpr int kSmN = kSmT + 2 * kSmR;
+struct Smooth3 {
+    static constexpr int kThreads = 256;
+    static constexpr bool kDealRows = true;  // its tile rows are dealt to the XCDs in turn
+    struct Args {
+        const float* in;   // density
+        float* out;        // smoothed density
+        const uint8_t* fluid;
+        uint32_t W, H, tiles_x;
+    };
+    static __device__ __forceinline__ float cell(const float* src, const uint8_t* f, int i) {
+        const float own = src[i];
+        if (!f[i]) return own;
+        float sum = own * 4.0f;
+        float weight = 4.0f;
+        const float fl = (float)f[i - 1], fr = (float)f[i + 1], fb = (float)f[i - kSmN], ft = (float)f[i + kSmN];
+        sum += src[i - 1] * fl;
+        sum += src[i + 1] * fr;
+        sum += src[i - kSmN] * fb;
+        sum += src[i + kSmN] * ft;
+        weight += ((fl + fr) + fb) + ft;
+        return sum / weight;
+    }
+    static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
+        __shared__ float s_a[kSmN * kSmN], s_b[kSmN * kSmN];
+        __shared__ uint8_t s_f[kSmN * kSmN];
+        __shared__ uint32_t any_slots[kThreads / 64];
+        const int32_t W = (int32_t)a.W, H = (int32_t)a.H;
+        const int32_t X0 = (int32_t)(wg % a.tiles_x) * kSmT - kSmR, Y0 = (int32_t)(wg / a.tiles_x) * kSmT - kSmR;
+        constexpr int kPer = (kSmN * kSmN + kThreads - 1) / kThreads;
+        float v[kPer];
+        uint8_t fv[kPer];
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = (int)threadIdx.x + kThreads * q;
+            const int32_t x = X0 + i % kSmN, y = Y0 + i / kSmN;
+            const bool in = i < kSmN * kSmN && x >= 0 && x < W && y >= 0 && y < H;
+            const size_t c = in ? (size_t)y * W + x : 0;
+            v[q] = in ? a.in[c] : 0.0f;
+            fv[q] = in ? a.fluid[c] : 0;
+        }
+        bool any = false;
+#pragma unroll
+        for (int q = 0; q < kPer; ++q) {
+            const int i = (int)threadIdx.x + kThreads * q;
+            if (i < kSmN * kSmN) {
+                s_a[i] = v[q];
+                s_f[i] = fv[q];
+                any = any || fv[q] != 0;
+            }
+        }
+        const bool work = block_any<kThreads>(any, any_slots);  // (its barrier orders the stores)
+        if (work) {
+            for (int i = threadIdx.x; i < kSmN * kSmN; i += kThreads) {
+                const int lx = i % kSmN, ly = i / kSmN;
+                if (lx >= 1 && lx < kSmN - 1 && ly >= 1 && ly < kSmN - 1) s_b[i] = cell(s_a, s_f, i);
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < kSmN * kSmN; i += kThreads) {
+                const int lx = i % kSmN, ly = i / kSmN;
+                if (lx >= 2 && lx < kSmN - 2 && ly >= 2 && ly < kSmN - 2) s_a[i] = cell(s_b, s_f, i);
+            }
+            __syncthreads();
+        }
+        for (int i = threadIdx.x; i < kSmT * kSmT; i += kThreads) {
+            const int tx = i % kSmT, ty = i / kSmT;
+            const int32_t x = X0 + kSmR + tx, y = Y0 + kSmR + ty;
+            if (x >= W || y >= H) continue;
+            const int li = (ty + kSmR) * kSmN + tx + kSmR;
+            a.out[(size_t)y * W + x] = work ? cell(s_a, s_f, li) : s_a[li];
+        }
+    }
+};
 
 // The xor butterfly over a wave, 32, 16, 8, 4, 2, 1 in that order.  The last
 // four steps stay inside a 16-lane row and take their partner from a DPP row
@@ -3021,10 +3080,9 @@ Prolong base_prolong(const Sim& sim) {
 void launch_build_pressure(Sim& sim, bool zero_pressure) {
     const uint32_t cells = sim.cells;
     const uint8_t* fluid = sim.levels[0].fluid;
-    // three smoothing passes: density -> a -> b -> a, so the result ends in smooth_a
-    emit<SmoothDensity>(sim, {sim.density, sim.smooth_a, fluid, sim.W, cells}, blocks_for(cells));
-    emit<SmoothDensity>(sim, {sim.smooth_a, sim.smooth_b, fluid, sim.W, cells}, blocks_for(cells));
-    emit<SmoothDensity>(sim, {sim.smooth_b, sim.smooth_a, fluid, sim.W, cells}, blocks_for(cells));
+    // the three smoothing passes, density -> smooth_a, in one launch
+    const uint32_t stx = (sim.W + kSmT - 1) / kSmT, sty = (sim.H + kSmT - 1) / kSmT;
+    emit<Smooth3>(sim, {sim.density, sim.smooth_a, fluid, sim.W, sim.H, stx}, stx * sty);
     Assemble::Args a{sim.mac_u, sim.mac_v, sim.smooth_a, fluid, sim.obstacle, sim.W, cells, sim.cell, sim.levels[0],
                      sim.pressure, (int)zero_pressure, sim.part_fluid, sim.nred};
     emit<Assemble>(sim, a, sim.nred);
