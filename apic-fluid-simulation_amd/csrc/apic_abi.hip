@@ -485,7 +485,7 @@ apic_status substep(Sim& s, bool sort, bool first, float frame_dt) {
     launch_p2g(s, true, 0.0f);
     s.mark(APIC_T_P2G, 1);
     s.mark(APIC_T_PRESSURE_BUILD, 0);
-    launch_build_pressure(s, true);
+    launch_build_pressure(s, true, true);
     s.mark(APIC_T_PRESSURE_BUILD, 1);
     s.mark(APIC_T_MG_BUILD, 0);
     launch_mg_build(s);
@@ -769,7 +769,7 @@ apic_status batch_update(apic_batch& B, const std::vector<Sim*>& sims, bool disa
             launch_substep_begin(s, first, frame_dt[act[r]]);
             if (sort) do_sort(s, true);
             launch_p2g(s, true, 0.0f);
-            launch_build_pressure(s, true);
+            launch_build_pressure(s, true, true);
             launch_mg_build(s);
             launch_pcg_reset(s);
             launch_vcycle(s, 1);

@@ -145,7 +145,9 @@ void launch_radial_force(Sim& sim, float wx, float wy, float radius, float stren
 void launch_open_faces(Sim& sim);
 
 // pressure.hip
-void launch_build_pressure(Sim& sim, bool zero_pressure);
+// defer_fstats: the fluid statistics are finished by the launch_mg_build
+// that follows at once (its bottom build), not by a launch of their own
+void launch_build_pressure(Sim& sim, bool zero_pressure, bool defer_fstats = false);
 void launch_mg_build(Sim& sim);
 // mode 0: plain, 1: PCG start (gamma), 2: after PCG iteration k (residual test, gamma')
 void launch_vcycle(Sim& sim, int mode, int k = 0);
@@ -239,6 +241,7 @@ struct Sim {
     uint8_t* tile_flag = nullptr;
     uint32_t* tile_order = nullptr;
     bool tile_order_valid = false;
+    bool fstats_pending = false;    // launch_build_pressure left the fluid statistics to launch_mg_build
     float* search[2] = {nullptr, nullptr};
     float* As = nullptr;
     float* As_lo = nullptr;          // relative mode: A s (FP64) - As

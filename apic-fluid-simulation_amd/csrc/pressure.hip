@@ -161,32 +161,41 @@ __global__ __launch_bounds__(kThreads) void fluid_stats_kernel(const uint8_t* __
     write_fluid_partial(cnt, last, part_fluid, blockIdx.x);
 }
 
+struct FstatsArgs {
+    const uint32_t* part_fluid;
+    uint32_t n;
+    FluidStats* fstats;
+    FluidStats* host;
+};
+// The fluid statistics from the assembly's per-workgroup (count, last + 1):
+// every thread of the workgroup takes part (integer sums and maxima, so the
+// order does not matter).
+__device__ __forceinline__ void fstats_finalize(const FstatsArgs& a) {
+    __shared__ uint32_t s_cnt[1024 / 64], s_last[1024 / 64];
+    uint32_t cnt = 0, last = 0;
+    for (uint32_t b = threadIdx.x; b < a.n; b += blockDim.x) {
+        cnt += a.part_fluid[2 * b];
+        last = max(last, a.part_fluid[2 * b + 1]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        cnt += __shfl_xor(cnt, o, 64);
+        last = max(last, (uint32_t)__shfl_xor(last, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) { s_cnt[threadIdx.x >> 6] = cnt; s_last[threadIdx.x >> 6] = last; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0, l = 0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) { c += s_cnt[w]; l = max(l, s_last[w]); }
+        a.fstats->count = c;
+        a.fstats->last_raw = l;
+        if (a.host) { a.host->count = c; a.host->last_raw = l; }
+    }
+}
 struct FstatsFinalize {
     static constexpr int kThreads = kSolverThreads;
-    struct Args {
-        const uint32_t* part_fluid;
-        uint32_t n;
-        FluidStats* fstats;
-        FluidStats* host;
-    };
-    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) {
-        uint32_t cnt = 0, last = 0;
-        for (uint32_t b = threadIdx.x; b < a.n; b += blockDim.x) {
-            cnt += a.part_fluid[2 * b];
-            last = max(last, a.part_fluid[2 * b + 1]);
-        }
-        __shared__ uint32_t s_cnt[kThreads], s_last[kThreads];
-        s_cnt[threadIdx.x] = cnt;
-        s_last[threadIdx.x] = last;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t c = 0, l = 0;
-            for (int t = 0; t < (int)blockDim.x; ++t) { c += s_cnt[t]; l = max(l, s_last[t]); }
-            a.fstats->count = c;
-            a.fstats->last_raw = l;
-            if (a.host) { a.host->count = c; a.host->last_raw = l; }
-        }
-    }
+    using Args = FstatsArgs;
+    static __device__ __forceinline__ void run(const Args& a, uint32_t, int) { fstats_finalize(a); }
 };
 
 __device__ inline bool mg_build_cell(const LevelView& F, const LevelView& C, uint32_t c) {
@@ -337,13 +346,16 @@ struct TileOrder {
     static __device__ __forceinline__ void run(const Args& a, uint32_t, int) { tile_order(a); }
 };
 
-// with_order: TileOrder's work after the levels (one launch fewer)
+// with_fstats: FstatsFinalize's work (the assembly's partials are complete);
+// with_order: TileOrder's; both after the levels, launches fewer
 struct MgBuildBottom {
     static constexpr int kThreads = kBottomThreads;
-    static_assert(kBottomThreads >= kOrderThreads, "the tile order's waves");
+    static_assert(kBottomThreads >= kOrderThreads && kBottomThreads <= 1024, "the tail's waves");
     struct Args {
         LevelSet ls;
         int first;
+        int with_fstats;
+        FstatsArgs fstats;
         int with_order;
         TileOrderArgs order;
     };
@@ -354,6 +366,7 @@ struct MgBuildBottom {
             for (uint32_t c = threadIdx.x; c < C.cells; c += blockDim.x) mg_build_cell(F, C, c);
             __syncthreads();
         }
+        if (a.with_fstats) fstats_finalize(a.fstats);  // (every thread reaches its barrier)
         if (a.with_order) tile_order(a.order);
     }
 };
@@ -3077,7 +3090,7 @@ Prolong base_prolong(const Sim& sim) {
 
 }  // namespace
 
-void launch_build_pressure(Sim& sim, bool zero_pressure) {
+void launch_build_pressure(Sim& sim, bool zero_pressure, bool defer_fstats) {
     const uint32_t cells = sim.cells;
     const uint8_t* fluid = sim.levels[0].fluid;
     // the three smoothing passes, density -> smooth_a, in one launch
@@ -3086,7 +3099,9 @@ void launch_build_pressure(Sim& sim, bool zero_pressure) {
     Assemble::Args a{sim.mac_u, sim.mac_v, sim.smooth_a, fluid, sim.obstacle, sim.W, cells, sim.cell, sim.levels[0],
                      sim.pressure, (int)zero_pressure, sim.part_fluid, sim.nred};
     emit<Assemble>(sim, a, sim.nred);
-    emit<FstatsFinalize>(sim, {sim.part_fluid, sim.nred, sim.fstats, sim.h_fstats_d}, 1);
+    // (the hierarchy's bottom build finishes the statistics when it follows)
+    if (defer_fstats) sim.fstats_pending = true;
+    else emit<FstatsFinalize>(sim, {sim.part_fluid, sim.nred, sim.fstats, sim.h_fstats_d}, 1);
 }
 
 void launch_fluid_stats(Sim& sim) {
@@ -3171,8 +3186,15 @@ void launch_mg_build(Sim& sim) {
     const uint32_t cmul = btx > 1 ? (uint32_t)(((1ull << 32) + btx - 1) / btx) : 0u;
     const TileOrderArgs order{sim.tile_flag, sim.tile_order, nt, btx, cmul, sim.part_rr_t, sim.part_gamma_d,
                               sim.part_asz_d, sim.part_gamma};
-    if (l < n) emit<MgBuildBottom>(sim, {level_set(sim), l, (int)ordered, order}, 1);
-    else if (ordered) emit<TileOrder>(sim, order, 1);
+    const FstatsArgs fst{sim.part_fluid, sim.nred, sim.fstats, sim.h_fstats_d};
+    const bool fstats = sim.fstats_pending;
+    sim.fstats_pending = false;
+    if (l < n) {
+        emit<MgBuildBottom>(sim, {level_set(sim), l, (int)fstats, fst, (int)ordered, order}, 1);
+    } else {
+        if (fstats) emit<FstatsFinalize>(sim, fst, 1);
+        if (ordered) emit<TileOrder>(sim, order, 1);
+    }
     sim.tile_order_valid = ordered;
 }
 
