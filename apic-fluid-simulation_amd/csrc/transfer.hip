@@ -586,16 +586,22 @@ struct CellRanges {
         uint32_t np;
         uint2* range;
         uint32_t nkeys;
+        uint32_t nwg;
     };
+    // (a grid-stride loop over at most kMaxWorkgroups workgroups: one
+    // workgroup per 256 keys would be dispatch-bound at tens of millions)
+    static constexpr uint32_t kMaxWorkgroups = 8192;
     static __device__ __forceinline__ void run(const Args& a, uint32_t wg, int) {
-        const uint32_t i = wg * kThreads + threadIdx.x;
         const uint32_t np = a.np;
-        if (i >= np) return;
         const uint32_t* __restrict__ keys = a.keys;
-        const uint32_t k = keys[i];
-        if (k >= a.nkeys) return;  // keys come from clamped cell coordinates; kept as a guard
-        if (i == 0 || keys[i - 1] != k) a.range[k].x = i;
-        if (i + 1 == np || keys[i + 1] != k) a.range[k].y = i + 1;
+        const uint32_t stride = a.nwg * kThreads;
+        for (uint32_t i = wg * kThreads + threadIdx.x; i < np; i += stride) {
+            const uint32_t k = keys[i];
+            const uint32_t kp = i > 0 ? keys[i - 1] : ~0u, kn = i + 1 < np ? keys[i + 1] : ~0u;
+            if (k >= a.nkeys) continue;  // keys come from clamped cell coordinates; kept as a guard
+            if (i == 0 || kp != k) a.range[k].x = i;
+            if (i + 1 == np || kn != k) a.range[k].y = i + 1;
+        }
     }
 };
 
@@ -903,8 +909,8 @@ void launch_cell_ranges(Sim& sim) {
     const size_t nk = cell_key_count(sim);
     launch_fill_u32(sim, reinterpret_cast<uint32_t*>(sim.cell_range), 0u, 2 * nk);
     if (sim.num_particles == 0) return;
-    emit<CellRanges>(sim, {sim.keys[0], sim.num_particles, sim.cell_range, (uint32_t)nk},
-                     (sim.num_particles + 255) / 256);
+    const uint32_t nwg = std::min<uint32_t>((sim.num_particles + 255) / 256, CellRanges::kMaxWorkgroups);
+    emit<CellRanges>(sim, {sim.keys[0], sim.num_particles, sim.cell_range, (uint32_t)nk, nwg}, nwg);
 }
 
 void launch_p2g(Sim& sim, bool forces, float dt) {
