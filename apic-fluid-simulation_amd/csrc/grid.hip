@@ -106,8 +106,9 @@ struct Gradient {
         const float* __restrict__ pressure = a.pressure;
         const float* __restrict__ smoothed = a.smoothed;
         const float dt = a.ss ? a.ss->dt : a.dt_arg;
-        const size_t nu = (size_t)W * H + H, nv = (size_t)W * H + W;
-        size_t f = (size_t)wg * kThreads + threadIdx.x;
+        // (face indices fit 32 bits: 32-bit quotients, far cheaper than 64-bit ones)
+        const uint32_t nu = W * H + H, nv = W * H + W;
+        const uint32_t f = wg * kThreads + threadIdx.x;
         if (f < nu) {
             int32_t y = (int32_t)(f / (W + 1)), x = (int32_t)(f % (W + 1));
             float u;

@@ -70,7 +70,7 @@ __device__ inline void write_fluid_partial(uint32_t cnt, uint32_t last, uint32_t
         cnt += __shfl_xor(cnt, o, 64);
         last = max(last, (uint32_t)__shfl_xor(last, o, 64));
     }
-    __shared__ uint32_t s_cnt[kThreads / 64], s_last[kThreads / 64];
+    __shared__ uint32_t s_cnt[1024 / 64], s_last[1024 / 64];  // (up to 1024 threads)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     if (lane == 0) { s_cnt[wave] = cnt; s_last[wave] = last; }
     __syncthreads();
@@ -131,8 +131,11 @@ __device__ __forceinline__ void assemble(
     write_fluid_partial(cnt, last, part_fluid, wg);
 }
 
+// 1024 threads a workgroup: a cell's loads depend on its fluid flag, so the
+// loop needs many waves in flight (the partials of the fluid statistics are
+// sums and maxima, the same for any partition of the cells)
 struct Assemble {
-    static constexpr int kThreads = kSolverThreads;
+    static constexpr int kThreads = 1024;
     struct Args {
         const float* mac_u;
         const float* mac_v;
