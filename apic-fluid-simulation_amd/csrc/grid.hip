@@ -109,42 +109,55 @@ struct Gradient {
         // (face indices fit 32 bits: 32-bit quotients, far cheaper than 64-bit ones)
         const uint32_t nu = W * H + H, nv = W * H + W;
         const uint32_t f = wg * kThreads + threadIdx.x;
+        // Every operand of a face is loaded before the first test (cells past
+        // the grid read cell 0 and are not used): one wait instead of a chain
+        // of obstacle, fluid and pressure loads.
         if (f < nu) {
-            int32_t y = (int32_t)(f / (W + 1)), x = (int32_t)(f % (W + 1));
+            const int32_t y = (int32_t)(f / (W + 1)), x = (int32_t)(f % (W + 1));
+            const bool in_l = x >= 1, in_r = x < (int32_t)W;
+            const size_t li = in_l ? (size_t)y * W + (x - 1) : 0, ri = in_r ? (size_t)y * W + x : 0;
+            const uint8_t ol = obstacle[li], orr = obstacle[ri], fl = fluid[li], fr = fluid[ri];
+            const float pl = pressure[li], pr = pressure[ri], sl = smoothed[li], sr = smoothed[ri];
+            const float mu = a.mac_u[f];
+            const float pu = a.post ? a.post_u[f] : 0.0f;
             float u;
-            bool lo = obst(obstacle, x - 1, y, W, H), ro = obst(obstacle, x, y, W, H);
+            const bool lo = !in_l || ol != 0, ro = !in_r || orr != 0;
             if (lo != ro) {
                 u = 0.0f;
                 a.valid_u[f] = 1;
             } else {
-                bool lf = fluid_at(fluid, x - 1, y, W, H), rf = fluid_at(fluid, x, y, W, H);
+                const bool lf = in_l && fl != 0, rf = in_r && fr != 0;
                 float pd = 0.0f;
-                size_t li = (size_t)y * W + (x - 1), ri = (size_t)y * W + x;
-                if (lf && rf) pd = pressure[ri] - pressure[li];
-                else if (lf) pd = (-pressure[li]) / theta(smoothed[ri], smoothed[li]);
-                else if (rf) pd = pressure[ri] / theta(smoothed[li], smoothed[ri]);
-                u = a.mac_u[f] - pd * a.inv_cell;
+                if (lf && rf) pd = pr - pl;
+                else if (lf) pd = (-pl) / theta(sr, sl);
+                else if (rf) pd = pr / theta(sl, sr);
+                u = mu - pd * a.inv_cell;
             }
-            if (a.post) u += a.post_u[f] * dt;
+            if (a.post) u += pu * dt;
             a.mac_u[f] = u;
         }
         if (f < nv) {
-            int32_t y = (int32_t)(f / W), x = (int32_t)(f % W);
+            const int32_t y = (int32_t)(f / W), x = (int32_t)(f % W);
+            const bool in_b = y >= 1, in_t = y < (int32_t)H;
+            const size_t bi = in_b ? (size_t)(y - 1) * W + x : 0, ti = in_t ? (size_t)y * W + x : 0;
+            const uint8_t ob = obstacle[bi], ot = obstacle[ti], fb = fluid[bi], ft = fluid[ti];
+            const float pb = pressure[bi], pt = pressure[ti], sb = smoothed[bi], st = smoothed[ti];
+            const float mv = a.mac_v[f];
+            const float pv = a.post ? a.post_v[f] : 0.0f;
             float v;
-            bool bo = obst(obstacle, x, y - 1, W, H), to = obst(obstacle, x, y, W, H);
+            const bool bo = !in_b || ob != 0, to = !in_t || ot != 0;
             if (bo != to) {
                 v = 0.0f;
                 a.valid_v[f] = 1;
             } else {
-                bool bf = fluid_at(fluid, x, y - 1, W, H), tf = fluid_at(fluid, x, y, W, H);
+                const bool bf = in_b && fb != 0, tf = in_t && ft != 0;
                 float pd = 0.0f;
-                size_t bi = (size_t)(y - 1) * W + x, ti = (size_t)y * W + x;
-                if (bf && tf) pd = pressure[ti] - pressure[bi];
-                else if (bf) pd = (-pressure[bi]) / theta(smoothed[ti], smoothed[bi]);
-                else if (tf) pd = pressure[ti] / theta(smoothed[bi], smoothed[ti]);
-                v = a.mac_v[f] - pd * a.inv_cell;
+                if (bf && tf) pd = pt - pb;
+                else if (bf) pd = (-pb) / theta(st, sb);
+                else if (tf) pd = pt / theta(sb, st);
+                v = mv - pd * a.inv_cell;
             }
-            if (a.post) v += a.post_v[f] * dt;
+            if (a.post) v += pv * dt;
             a.mac_v[f] = v;
         }
     }
