@@ -204,22 +204,42 @@ struct ScanChunks {
         const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;  // 1 or 2
         const uint32_t d0 = threadIdx.x * per;
         uint32_t run[2] = {0u, 0u};
-        for (uint32_t c0 = 0; c0 < nchunks; c0 += 16) {
-            uint32_t v[2][16];
+        // Each step loads a batch of chunk totals, then rewrites them: one
+        // wait per batch (32 chunks with one digit a thread when there are
+        // more than 16, else 16; larger batches spill)
+        if (per == 1 && nchunks > 16) {
+            for (uint32_t c0 = 0; c0 < nchunks; c0 += 32) {
+                uint32_t v[32];
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    const bool ok = (uint32_t)q < per && d0 + q < nb && c0 + c < nchunks;
-                    v[q][c] = ok ? chunk_sum[(size_t)(c0 + c) * nb + d0 + q] : 0u;
+                for (int c = 0; c < 32; ++c) {
+                    const bool ok = d0 < nb && c0 + c < nchunks;
+                    v[c] = ok ? chunk_sum[(size_t)(c0 + c) * nb + d0] : 0u;
                 }
 #pragma unroll
-            for (int q = 0; q < 2; ++q)
-#pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    if ((uint32_t)q < per && d0 + q < nb && c0 + c < nchunks) chunk_sum[(size_t)(c0 + c) * nb + d0 + q] = run[q];
-                    run[q] += v[q][c];
+                for (int c = 0; c < 32; ++c) {
+                    if (d0 < nb && c0 + c < nchunks) chunk_sum[(size_t)(c0 + c) * nb + d0] = run[0];
+                    run[0] += v[c];
                 }
+            }
+        } else {
+            for (uint32_t c0 = 0; c0 < nchunks; c0 += 16) {
+                uint32_t v[2][16];
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) {
+                        const bool ok = (uint32_t)q < per && d0 + q < nb && c0 + c < nchunks;
+                        v[q][c] = ok ? chunk_sum[(size_t)(c0 + c) * nb + d0 + q] : 0u;
+                    }
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+#pragma unroll
+                    for (int c = 0; c < 16; ++c) {
+                        if ((uint32_t)q < per && d0 + q < nb && c0 + c < nchunks)
+                            chunk_sum[(size_t)(c0 + c) * nb + d0 + q] = run[q];
+                        run[q] += v[q][c];
+                    }
+            }
         }
         uint32_t total;
         uint32_t start = block_exclusive_scan(run[0] + run[1], wave_sums, total);
